@@ -1,0 +1,5 @@
+2
+X 0
+X 1
+def2 TEST samples/arbitrary2qubitmatrixset/matrix20.gate
+TEST 0 1

@@ -1,0 +1,5 @@
+2
+X 0
+X 1
+def2 TEST samples/arbitrary2qubitmatrixset/matrix80.gate
+TEST 0 1
