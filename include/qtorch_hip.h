@@ -1,0 +1,112 @@
+/*
+ * qtorch_hip.h — C ABI of libqtorch_hip.so, the MI355X (gfx950) contraction engine
+ * behind qTorch's pairwise contraction.
+ *
+ * Plain C types only (no C++ or torch types cross this boundary); every entry
+ * point returns a qt_status and never throws.  Tensors are dense, device
+ * resident, interleaved complex (re, im) with the reference's flat layout:
+ * index = sum_i d_i * D^i over the node's wire order, D = 4 for qTorch's
+ * Liouville wires (reference src/Node.h:112-113, 168-186) and D = 2 for
+ * amplitude networks (an extension of this engine).
+ *
+ * Threading: a context owns one HIP stream and is internally serialised; every
+ * call on a context is stream ordered.  Use one context per host thread for
+ * concurrency.  Calls that return host data (downloads, batch results)
+ * synchronise the context's stream.
+ */
+#ifndef QTORCH_HIP_H
+#define QTORCH_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct qt_ctx qt_ctx;
+typedef struct qt_tensor qt_tensor;
+typedef struct qt_program qt_program;
+
+typedef enum {
+    QT_OK = 0,
+    QT_INVALID_INPUT = 1, /* maps to qtorch::InvalidFunctionInput (src/Exceptions.h:44) */
+    QT_OOM = 2,
+    QT_DEVICE = 3,        /* HIP runtime or kernel failure */
+    QT_TIMEOUT = 4,       /* maps to qtorch::ContractionFailure (src/Exceptions.h:36) */
+    QT_UNSUPPORTED = 5
+} qt_status;
+
+/* Element types of device tensors. */
+typedef enum {
+    QT_DTYPE_C128 = 0, /* std::complex<double>, the reference's Node storage (src/Node.h:160) */
+    QT_DTYPE_C64 = 1   /* std::complex<float> */
+} qt_dtype;
+
+/* Arithmetic forms of a contraction. */
+typedef enum {
+    QT_PLAIN_C128 = 0, /* complex128 operands and accumulation: the drop-in form */
+    QT_C64_4M = 1,     /* complex64, 4 real products per complex MAC, FP32 accumulation */
+    QT_C64_3M = 2,     /* complex64, 3 real products per complex MAC (Gauss), FP32 accumulation */
+    QT_C64_BF16X3 = 3  /* complex64 operands split hi+lo into bf16; hi*hi + hi*lo + lo*hi on
+                          MFMA with FP32 accumulation */
+} qt_precision;
+
+/* Bit 30 of a free-axis code marks an axis of A (else of B). */
+#define QT_FROM_A (1 << 30)
+
+/* ---- context ----------------------------------------------------------------- */
+
+/* Creates a context on HIP device `device` (stream, caching allocator). */
+qt_status qt_ctx_create(int device, qt_ctx** out);
+qt_status qt_ctx_destroy(qt_ctx* ctx);
+/* Blocks until every call issued on the context has finished. */
+qt_status qt_sync(qt_ctx* ctx);
+/* Message of the last error on the context ("" if none).  Owned by the context. */
+const char* qt_last_error(const qt_ctx* ctx);
+/* Device bytes held by the context's allocator (in use + cached). */
+int64_t qt_mem_reserved(const qt_ctx* ctx);
+
+/* ---- tensors (replace Node::mVals, src/Node.h:112-113,137,151,160) ---------------- */
+
+/* Allocates a zero-filled tensor of D^rank elements, D = 1 << dim_log2 (Node ctor,
+ * src/Node.h:112-113). */
+qt_status qt_tensor_alloc(qt_ctx* ctx, int rank, int dim_log2, int dtype, qt_tensor** out);
+/* Host -> device copy of interleaved (re, im) values; nbytes must equal the tensor size. */
+qt_status qt_tensor_upload(qt_ctx* ctx, qt_tensor* t, const void* host, int64_t nbytes);
+/* Device -> host copy (synchronises); replaces reading Node::GetTensorVals (src/Node.h:151). */
+qt_status qt_tensor_download(qt_ctx* ctx, const qt_tensor* t, void* host, int64_t nbytes);
+/* Creates n tensors at once from host values (leaf gate/state/measure tensors of a
+ * network, src/Node.h:197-898): one pinned staging copy and one synchronisation
+ * instead of one per tensor.  host[i] holds D^ranks[i] interleaved values. */
+qt_status qt_tensor_create_many(qt_ctx* ctx, int n, const int* ranks, int dim_log2, int dtype,
+                                const void* const* host, qt_tensor** out);
+/* Returns the storage to the context's cache (Node::ClearNodeData, src/Node.h:137). */
+qt_status qt_tensor_free(qt_ctx* ctx, qt_tensor* t);
+int64_t qt_tensor_numel(const qt_tensor* t);
+int qt_tensor_rank(const qt_tensor* t);
+
+/* ---- the pairwise contraction (Network::ContractIndices, src/Network.h:876-971) ----
+ *
+ * C[c] = sum_s A[a(c,s)] * B[b(c,s)] over the k shared axes:
+ *   pair_a[i], pair_b[i]  : toSumOn[i] (axis of A, axis of B), i < k      (src/Network.h:793-800)
+ *   free_src[i], i < rank_c: toNotSumOn[i]: axis | QT_FROM_A for an axis of A, plain axis of B;
+ *                           digit i of C's flat index                      (src/Network.h:902-907)
+ * C must be a distinct, already allocated tensor of rank_c axes with the same
+ * dtype and dim as A and B.  `precision` must be QT_PLAIN_C128 for C128 tensors
+ * and one of the C64 forms for C64 tensors.  On return *cmacs (if non-NULL) holds
+ * D^(rank_c + k), the reference's complex multiply-add counter (src/Network.h:884-885).
+ * The call is asynchronous on the context's stream. */
+qt_status qt_contract_pair(qt_ctx* ctx, const qt_tensor* A, int rank_a, const qt_tensor* B,
+                           int rank_b, int k, const int32_t* pair_a, const int32_t* pair_b,
+                           int rank_c, const int32_t* free_src, qt_tensor* C, int precision,
+                           int64_t* cmacs);
+
+/* Reads the rank-0 result of a finished contraction (the mFinalVal read of
+ * src/Network.h:961-969 and src/LineGraph.h:388); synchronises. */
+qt_status qt_tensor_read_scalar(qt_ctx* ctx, const qt_tensor* t, double* re, double* im);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QTORCH_HIP_H */

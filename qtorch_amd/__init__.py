@@ -1,0 +1,14 @@
+"""qtorch_amd — qTorch's tensor-network contraction path on AMD Instinct MI355X (gfx950).
+
+The product is native code: libqtorch_hip.so (HIP kernels + C ABI, include/qtorch_hip.h)
+and libqtorch_host.so (qTorch's C++ host surface + C ABI, include/qtorch_host.h).
+This package only binds them for Python callers (tests, benchmarks, services).
+"""
+from ._native import (  # noqa: F401
+    QT_C64_3M, QT_C64_4M, QT_C64_BF16X3, QT_DTYPE_C64, QT_DTYPE_C128, QT_FROM_A, QT_PLAIN_C128,
+    ContractionFailure, Context, DeviceFailure, EngineError, InvalidContractionMethod, InvalidFile,
+    InvalidFileFormat, InvalidFunctionInput, InvalidUserContractionSequence, Network, QbbFailure, Tensor,
+    contract_pair, declared_symbols, qaoa_edge_text, qaoa_energy_plain, qaoa_num_edges,
+)
+
+__all__ = [n for n in dir() if not n.startswith("_")]

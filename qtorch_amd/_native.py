@@ -1,0 +1,358 @@
+"""ctypes bindings of libqtorch_hip.so and libqtorch_host.so (in-tree builds).
+
+There is no CPU fallback: if a library is missing the import fails, and creating
+a device context without a GPU raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(_PKG, "lib")
+INCLUDE_DIR = os.path.join(os.path.dirname(_PKG), "include")
+HIP_LIB_PATH = os.path.join(LIB_DIR, "libqtorch_hip.so")
+HOST_LIB_PATH = os.path.join(LIB_DIR, "libqtorch_host.so")
+
+QT_OK, QT_INVALID_INPUT, QT_OOM, QT_DEVICE, QT_TIMEOUT, QT_UNSUPPORTED = range(6)
+QT_DTYPE_C128, QT_DTYPE_C64 = 0, 1
+QT_PLAIN_C128, QT_C64_4M, QT_C64_3M, QT_C64_BF16X3 = 0, 1, 2, 3
+QT_FROM_A = 1 << 30
+
+_c = ctypes
+_vp = _c.c_void_p
+_i32p = _c.POINTER(_c.c_int32)
+_i64p = _c.POINTER(_c.c_int64)
+_dp = _c.POINTER(_c.c_double)
+
+
+def _load(path: str) -> ctypes.CDLL:
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is not built: run `python -m qtorch_amd.build`")
+    return ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL)
+
+
+hip = _load(HIP_LIB_PATH)
+host = _load(HOST_LIB_PATH)
+
+_HIP_SIGS = {
+    "qt_ctx_create": (_c.c_int, [_c.c_int, _c.POINTER(_vp)]),
+    "qt_ctx_destroy": (_c.c_int, [_vp]),
+    "qt_sync": (_c.c_int, [_vp]),
+    "qt_last_error": (_c.c_char_p, [_vp]),
+    "qt_mem_reserved": (_c.c_int64, [_vp]),
+    "qt_tensor_alloc": (_c.c_int, [_vp, _c.c_int, _c.c_int, _c.c_int, _c.POINTER(_vp)]),
+    "qt_tensor_upload": (_c.c_int, [_vp, _vp, _vp, _c.c_int64]),
+    "qt_tensor_download": (_c.c_int, [_vp, _vp, _vp, _c.c_int64]),
+    "qt_tensor_create_many": (_c.c_int, [_vp, _c.c_int, _i32p, _c.c_int, _c.c_int, _c.POINTER(_vp),
+                                         _c.POINTER(_vp)]),
+    "qt_tensor_free": (_c.c_int, [_vp, _vp]),
+    "qt_tensor_numel": (_c.c_int64, [_vp]),
+    "qt_tensor_rank": (_c.c_int, [_vp]),
+    "qt_contract_pair": (_c.c_int, [_vp, _vp, _c.c_int, _vp, _c.c_int, _c.c_int, _i32p, _i32p, _c.c_int,
+                                    _i32p, _vp, _c.c_int, _i64p]),
+    "qt_tensor_read_scalar": (_c.c_int, [_vp, _vp, _dp, _dp]),
+}
+_HOST_SIGS = {
+    "qth_last_error": (_c.c_char_p, []),
+    "qth_network_create": (_c.c_int, [_vp, _c.c_char_p, _c.c_char_p, _c.c_char_p, _c.POINTER(_vp)]),
+    "qth_network_destroy": (_c.c_int, [_vp]),
+    "qth_network_reduce": (_c.c_int, [_vp]),
+    "qth_contract_nodes": (_c.c_int, [_vp, _c.c_int, _c.c_int, _c.c_int, _c.POINTER(_c.c_int)]),
+    "qth_contract_stochastic": (_c.c_int, [_vp, _c.c_uint64, _dp, _dp]),
+    "qth_contract_given": (_c.c_int, [_vp, _i32p, _c.c_int, _dp, _dp]),
+    "qth_contract_user": (_c.c_int, [_vp, _i32p, _c.c_int, _dp, _dp]),
+    "qth_linegraph_cnf": (_c.c_int, [_vp, _c.c_char_p, _c.c_int64, _i64p]),
+    "qth_linegraph_minfill": (_c.c_int, [_vp, _i32p, _c.c_int, _c.POINTER(_c.c_int)]),
+    "qth_linegraph_contract": (_c.c_int, [_vp, _i32p, _c.c_int, _dp, _dp]),
+    "qth_record": (_c.c_int, [_vp, _i32p, _c.c_int, _c.POINTER(_c.c_int)]),
+    "qth_flops": (_c.c_int64, [_vp]),
+    "qth_counts": (_c.c_int, [_vp] + [_c.POINTER(_c.c_int)] * 4),
+    "qth_plan": (_c.c_int, [_vp, _i32p, _c.c_int, _c.POINTER(_c.c_int)]),
+    "qth_qaoa_num_edges": (_c.c_int, [_c.c_char_p, _c.POINTER(_c.c_int), _c.POINTER(_c.c_int)]),
+    "qth_qaoa_edge_text": (_c.c_int, [_c.c_char_p, _c.c_int, _c.c_int, _c.c_int, _dp, _c.c_char_p, _c.c_int64,
+                                      _c.c_char_p, _c.c_int64]),
+    "qth_qaoa_energy_plain": (_c.c_int, [_vp, _c.c_char_p, _c.c_int, _c.c_int, _dp, _c.c_uint64, _dp, _dp]),
+}
+
+
+def _bind(lib, sigs):
+    for name, (res, args) in sigs.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+
+
+_bind(hip, _HIP_SIGS)
+_bind(host, _HOST_SIGS)
+
+
+def declared_symbols(header: str) -> List[str]:
+    """Function names declared in include/<header>."""
+    with open(os.path.join(INCLUDE_DIR, header)) as f:
+        text = f.read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(qth?_[a-z0-9_]+)\s*\(", text)))
+
+
+class EngineError(RuntimeError):
+    pass
+
+
+# exception classes matching the reference's (src/Exceptions.h)
+class InvalidFunctionInput(EngineError):
+    pass
+
+
+class InvalidFile(EngineError):
+    pass
+
+
+class InvalidFileFormat(EngineError):
+    pass
+
+
+class ContractionFailure(EngineError):
+    pass
+
+
+class InvalidUserContractionSequence(EngineError):
+    pass
+
+
+class InvalidContractionMethod(EngineError):
+    pass
+
+
+class QbbFailure(EngineError):
+    pass
+
+
+class DeviceFailure(EngineError):
+    pass
+
+
+_HOST_ERRORS = {1: InvalidFunctionInput, 2: InvalidFile, 3: InvalidFileFormat, 4: ContractionFailure,
+                5: InvalidUserContractionSequence, 6: InvalidContractionMethod, 7: QbbFailure, 8: DeviceFailure,
+                9: EngineError}
+
+
+def _host_check(rc: int) -> None:
+    if rc != 0:
+        msg = host.qth_last_error().decode(errors="replace")
+        raise _HOST_ERRORS.get(rc, EngineError)(msg)
+
+
+class Context:
+    """A device context: one HIP stream and a caching allocator (qt_ctx)."""
+
+    def __init__(self, device: int = 0):
+        h = _vp()
+        rc = hip.qt_ctx_create(device, ctypes.byref(h))
+        if rc != QT_OK:
+            raise DeviceFailure(f"qt_ctx_create(device={device}) failed with status {rc}: no usable HIP device")
+        self.handle = h
+
+    def check(self, rc: int, what: str = "") -> None:
+        if rc == QT_OK:
+            return
+        msg = hip.qt_last_error(self.handle).decode(errors="replace")
+        if rc == QT_INVALID_INPUT:
+            raise InvalidFunctionInput(f"{what}: {msg}")
+        raise DeviceFailure(f"{what}: status {rc}: {msg}")
+
+    def sync(self) -> None:
+        self.check(hip.qt_sync(self.handle), "qt_sync")
+
+    def mem_reserved(self) -> int:
+        return int(hip.qt_mem_reserved(self.handle))
+
+    def close(self) -> None:
+        if self.handle:
+            hip.qt_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Tensor:
+    """A device tensor (qt_tensor): D^rank interleaved complex values."""
+
+    def __init__(self, ctx: Context, rank: int, dim_log2: int = 2, dtype: int = QT_DTYPE_C128):
+        self.ctx = ctx
+        self.rank = rank
+        self.dim_log2 = dim_log2
+        self.dtype = dtype
+        h = _vp()
+        ctx.check(hip.qt_tensor_alloc(ctx.handle, rank, dim_log2, dtype, ctypes.byref(h)), "qt_tensor_alloc")
+        self.handle = h
+
+    @property
+    def np_dtype(self):
+        return np.complex128 if self.dtype == QT_DTYPE_C128 else np.complex64
+
+    @property
+    def numel(self) -> int:
+        return 1 << (self.rank * self.dim_log2)
+
+    def upload(self, values: np.ndarray) -> "Tensor":
+        v = np.ascontiguousarray(values, dtype=self.np_dtype).reshape(-1)
+        self.ctx.check(hip.qt_tensor_upload(self.ctx.handle, self.handle, v.ctypes.data, v.nbytes), "upload")
+        return self
+
+    def download(self) -> np.ndarray:
+        v = np.empty(self.numel, dtype=self.np_dtype)
+        self.ctx.check(hip.qt_tensor_download(self.ctx.handle, self.handle, v.ctypes.data, v.nbytes), "download")
+        return v
+
+    def free(self) -> None:
+        if self.handle:
+            hip.qt_tensor_free(self.ctx.handle, self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            if self.handle and self.ctx.handle:
+                self.free()
+        except Exception:
+            pass
+
+
+def _i32(vals: Sequence[int]):
+    arr = (ctypes.c_int32 * max(len(vals), 1))(*vals)
+    return arr
+
+
+def contract_pair(ctx: Context, a: Tensor, b: Tensor, pairs: Sequence[Tuple[int, int]],
+                  free: Sequence[Tuple[bool, int]], precision: int = QT_PLAIN_C128) -> Tuple[Tensor, int]:
+    """qt_contract_pair: C = contraction of A and B (Network::ContractIndices semantics)."""
+    c = Tensor(ctx, len(free), a.dim_log2, a.dtype)
+    pa = _i32([p[0] for p in pairs])
+    pb = _i32([p[1] for p in pairs])
+    fs = _i32([(ax | QT_FROM_A) if from_a else ax for from_a, ax in free])
+    macs = ctypes.c_int64(0)
+    ctx.check(hip.qt_contract_pair(ctx.handle, a.handle, a.rank, b.handle, b.rank, len(pairs), pa, pb,
+                                   len(free), fs, c.handle, precision, ctypes.byref(macs)), "qt_contract_pair")
+    return c, macs.value
+
+
+class Network:
+    """qTorch Network built by the C++ host library (libqtorch_host.so).
+
+    ``ctx=None`` builds a plan-only network (ordering logic and step records, no
+    values)."""
+
+    def __init__(self, qasm: str, measure: Optional[str], ctx: Optional[Context] = None, base_dir: str = ""):
+        h = _vp()
+        _host_check(host.qth_network_create(ctx.handle if ctx else None, qasm.encode(),
+                                            measure.encode() if measure is not None else None,
+                                            base_dir.encode(), ctypes.byref(h)))
+        self.handle = h
+        self.ctx = ctx
+
+    def close(self):
+        if self.handle:
+            host.qth_network_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def reduce(self) -> None:
+        _host_check(host.qth_network_reduce(self.handle))
+
+    def contract_nodes(self, a: int, b: int, threshold: int) -> int:
+        r = ctypes.c_int(-1)
+        _host_check(host.qth_contract_nodes(self.handle, a, b, threshold, ctypes.byref(r)))
+        return r.value
+
+    def _val(self, fn, *args) -> complex:
+        re, im = ctypes.c_double(), ctypes.c_double()
+        _host_check(fn(self.handle, *args, ctypes.byref(re), ctypes.byref(im)))
+        return complex(re.value, im.value)
+
+    def contract_stochastic(self, seed: int) -> complex:
+        return self._val(host.qth_contract_stochastic, ctypes.c_uint64(seed))
+
+    def contract_given(self, seq: Sequence[Tuple[int, int]]) -> complex:
+        flat = [x for p in seq for x in p]
+        return self._val(host.qth_contract_given, _i32(flat), len(seq))
+
+    def contract_user(self, seq: Sequence[Tuple[int, int]]) -> complex:
+        flat = [x for p in seq for x in p]
+        return self._val(host.qth_contract_user, _i32(flat), len(seq))
+
+    def linegraph_cnf(self) -> str:
+        n = ctypes.c_int64(0)
+        _host_check(host.qth_linegraph_cnf(self.handle, None, 0, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value + 1)
+        _host_check(host.qth_linegraph_cnf(self.handle, buf, n.value + 1, ctypes.byref(n)))
+        return buf.value.decode()
+
+    def linegraph_minfill(self) -> List[int]:
+        n = ctypes.c_int(0)
+        _host_check(host.qth_linegraph_minfill(self.handle, None, 0, ctypes.byref(n)))
+        out = (ctypes.c_int32 * max(n.value, 1))()
+        _host_check(host.qth_linegraph_minfill(self.handle, out, n.value, ctypes.byref(n)))
+        return list(out[:n.value])
+
+    def linegraph_contract(self, order: Sequence[int]) -> complex:
+        return self._val(host.qth_linegraph_contract, _i32(list(order)), len(order))
+
+    def record(self) -> List[Tuple[int, int]]:
+        n = ctypes.c_int(0)
+        _host_check(host.qth_record(self.handle, None, 0, ctypes.byref(n)))
+        out = (ctypes.c_int32 * max(2 * n.value, 1))()
+        _host_check(host.qth_record(self.handle, out, n.value, ctypes.byref(n)))
+        return [(out[2 * i], out[2 * i + 1]) for i in range(n.value)]
+
+    def flops(self) -> int:
+        return int(host.qth_flops(self.handle))
+
+    def counts(self):
+        v = [ctypes.c_int() for _ in range(4)]
+        _host_check(host.qth_counts(self.handle, *[ctypes.byref(x) for x in v]))
+        return {"all": v[0].value, "uncontracted": v[1].value, "original": v[2].value, "done": bool(v[3].value)}
+
+    def plan(self) -> np.ndarray:
+        n = ctypes.c_int(0)
+        _host_check(host.qth_plan(self.handle, None, 0, ctypes.byref(n)))
+        out = np.zeros((max(n.value, 1), 8), dtype=np.int32)
+        _host_check(host.qth_plan(self.handle, out.ctypes.data_as(_i32p), n.value, ctypes.byref(n)))
+        return out[:n.value]
+
+
+def qaoa_edge_text(dgf: str, p: int, ref_map: bool, e: int, betas_gammas: Sequence[float]) -> Tuple[str, str]:
+    bg = (ctypes.c_double * len(betas_gammas))(*betas_gammas)
+    q = ctypes.create_string_buffer(1 << 20)
+    m = ctypes.create_string_buffer(1 << 16)
+    _host_check(host.qth_qaoa_edge_text(dgf.encode(), p, 1 if ref_map else 0, e, bg, q, 1 << 20, m, 1 << 16))
+    return q.value.decode(), m.value.decode()
+
+
+def qaoa_num_edges(dgf: str) -> Tuple[int, int]:
+    ne, nv = ctypes.c_int(), ctypes.c_int()
+    _host_check(host.qth_qaoa_num_edges(dgf.encode(), ctypes.byref(ne), ctypes.byref(nv)))
+    return ne.value, nv.value
+
+
+def qaoa_energy_plain(ctx: Context, dgf: str, p: int, ref_map: bool, betas_gammas: Sequence[float],
+                      seed: int = 0) -> Tuple[float, np.ndarray]:
+    ne, _ = qaoa_num_edges(dgf)
+    bg = (ctypes.c_double * len(betas_gammas))(*betas_gammas)
+    zz = np.zeros(ne, dtype=np.float64)
+    energy = ctypes.c_double()
+    _host_check(host.qth_qaoa_energy_plain(ctx.handle, dgf.encode(), p, 1 if ref_map else 0, bg,
+                                           ctypes.c_uint64(seed), zz.ctypes.data_as(_dp), ctypes.byref(energy)))
+    return energy.value, zz

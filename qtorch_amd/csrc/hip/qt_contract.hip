@@ -1,0 +1,578 @@
+// Pairwise contraction kernels of libqtorch_hip.so (gfx950).
+//
+// C[c] = sum_s A[a(c,s)] * B[b(c,s)] (Network::ContractIndices, reference
+// src/Network.h:876-971), on the bit-level description qt::BitMap.  Three regimes:
+//
+//   k_direct        : one lane per output element, the summed index in the inner
+//                     loop in the reference's order (last pair fastest).  In the
+//                     plain complex128 form floating-point contraction is off, so
+//                     results are bit-identical to the reference's loop.  Covers
+//                     every shape; chosen for small summed extents (memory-bound).
+//   k_reduce_*      : small outputs with a large summed extent (e.g. a scalar from
+//                     two big tensors): the summed index is split over workgroups,
+//                     wavefront/LDS reductions, then an ordered second pass.
+//   k_permute +     : large GEMM-shaped steps: operands are rewritten by the
+//   k_gemm_c128       LDS-staged bit-permute kernel into A'[s][m], B'[s][n], then a
+//                     complex FP64 GEMM on MFMA (v_mfma_f64_16x16x4f64) writes
+//                     C[m + M n], which is already the reference's C layout
+//                     (A's free axes low, B's high: src/Network.h:739-769).
+#include <algorithm>
+#include <cstring>
+
+#include "qt_internal.h"
+
+namespace qt {
+
+namespace {
+
+constexpr int kThreads = 256;
+
+enum { MODE_PLAIN = 0, MODE_4M = 1, MODE_3M = 2, MODE_BF16X3 = 3 };
+
+template <class V>
+struct Real;
+template <>
+struct Real<double2> {
+    using T = double;
+};
+template <>
+struct Real<float2> {
+    using T = float;
+};
+
+// Accumulator of one complex output.  The 3M form keeps the three real sums.
+template <class V, int MODE>
+struct Acc {
+    using T = typename Real<V>::T;
+    T re = 0, im = 0, p3 = 0;
+};
+
+__device__ __forceinline__ void mac_plain(Acc<double2, MODE_PLAIN>& acc, double2 a, double2 b) {
+#pragma clang fp contract(off)
+    // std::complex<double> product then +=, exactly as the reference evaluates it
+    const double pr = a.x * b.x - a.y * b.y;
+    const double pi = a.x * b.y + a.y * b.x;
+    acc.re = acc.re + pr;
+    acc.im = acc.im + pi;
+}
+
+__device__ __forceinline__ float bf16_round(float x) {
+    // round-to-nearest-even to bfloat16, returned widened to float
+    uint32_t u = __float_as_uint(x);
+    const uint32_t lsb = (u >> 16) & 1u;
+    u += 0x7fffu + lsb;
+    u &= 0xffff0000u;
+    return __uint_as_float(u);
+}
+
+template <class V, int MODE>
+__device__ __forceinline__ void mac(Acc<V, MODE>& acc, V a, V b) {
+    if constexpr (MODE == MODE_PLAIN) {
+        mac_plain(acc, a, b);
+    } else if constexpr (MODE == MODE_4M) {
+        acc.re = fmaf(a.x, b.x, acc.re);
+        acc.re = fmaf(-a.y, b.y, acc.re);
+        acc.im = fmaf(a.x, b.y, acc.im);
+        acc.im = fmaf(a.y, b.x, acc.im);
+    } else if constexpr (MODE == MODE_3M) {
+        acc.re = fmaf(a.x, b.x, acc.re);          // P1 = sum ar br
+        acc.im = fmaf(a.y, b.y, acc.im);          // P2 = sum ai bi
+        acc.p3 = fmaf(a.x + a.y, b.x + b.y, acc.p3);  // P3 = sum (ar+ai)(br+bi)
+    } else {
+        // bf16x3: x = hi + lo with hi, lo bfloat16; a*b ~ ah*bh + ah*bl + al*bh
+        const float arh = bf16_round(a.x), arl = bf16_round(a.x - arh);
+        const float aih = bf16_round(a.y), ail = bf16_round(a.y - aih);
+        const float brh = bf16_round(b.x), brl = bf16_round(b.x - brh);
+        const float bih = bf16_round(b.y), bil = bf16_round(b.y - bih);
+        float re = acc.re, im = acc.im;
+        re = fmaf(arh, brh, re); re = fmaf(arh, brl, re); re = fmaf(arl, brh, re);
+        re = fmaf(-aih, bih, re); re = fmaf(-aih, bil, re); re = fmaf(-ail, bih, re);
+        im = fmaf(arh, bih, im); im = fmaf(arh, bil, im); im = fmaf(arl, bih, im);
+        im = fmaf(aih, brh, im); im = fmaf(aih, brl, im); im = fmaf(ail, brh, im);
+        acc.re = re;
+        acc.im = im;
+    }
+}
+
+template <class V, int MODE>
+__device__ __forceinline__ V finish(const Acc<V, MODE>& acc) {
+    V r;
+    if constexpr (MODE == MODE_3M) {
+        r.x = acc.re - acc.im;
+        r.y = acc.p3 - acc.re - acc.im;
+    } else {
+        r.x = acc.re;
+        r.y = acc.im;
+    }
+    return r;
+}
+
+// offsets of the summed bits of s in A and B
+__device__ __forceinline__ void s_offsets(const BitMap& m, int64_t s, int64_t& ga, int64_t& gb) {
+    ga = 0;
+    gb = 0;
+    for (int j = 0; j < m.nbits_s; ++j)
+        if ((s >> j) & 1) {
+            ga += int64_t(1) << m.s_a[j];
+            gb += int64_t(1) << m.s_b[j];
+        }
+}
+
+__device__ __forceinline__ void c_offsets(const BitMap& m, int64_t c, int lo, int hi, int64_t& oa,
+                                          int64_t& ob) {
+    oa = 0;
+    ob = 0;
+    for (int i = lo; i < hi; ++i)
+        if ((c >> i) & 1) {
+            const int src = m.c_src[i];
+            if (src < 64) oa += int64_t(1) << src;
+            else ob += int64_t(1) << (src - 64);
+        }
+}
+
+template <class V, int MODE>
+__global__ __launch_bounds__(kThreads) void k_direct(const V* __restrict__ A, const V* __restrict__ B,
+                                                     V* __restrict__ C, BitMap m, int64_t nC) {
+    __shared__ int64_t low_a[kThreads], low_b[kThreads];
+    __shared__ int64_t ga[kThreads], gb[kThreads];
+    const int tid = threadIdx.x;
+    const int lo = m.nbits_c < 8 ? m.nbits_c : 8;
+    {
+        int64_t oa, ob;
+        c_offsets(m, tid, 0, lo, oa, ob);
+        low_a[tid] = oa;
+        low_b[tid] = ob;
+    }
+    const int64_t nS = int64_t(1) << m.nbits_s;
+    const bool one_chunk = nS <= kThreads;
+    if (one_chunk) {
+        if (tid < nS) {
+            int64_t a, b;
+            s_offsets(m, tid, a, b);
+            ga[tid] = a;
+            gb[tid] = b;
+        }
+    }
+    __syncthreads();
+    const int64_t stride = int64_t(gridDim.x) * kThreads;
+    for (int64_t base = int64_t(blockIdx.x) * kThreads; base < nC; base += stride) {
+        int64_t ha, hb;
+        c_offsets(m, base, lo, m.nbits_c, ha, hb);  // uniform across the workgroup
+        const int64_t c = base + tid;
+        const int64_t oa = ha + low_a[tid], ob = hb + low_b[tid];
+        Acc<V, MODE> acc;
+        if (one_chunk) {
+            if (c < nC)
+                for (int j = 0; j < int(nS); ++j) mac<V, MODE>(acc, A[oa + ga[j]], B[ob + gb[j]]);
+        } else {
+            for (int64_t s0 = 0; s0 < nS; s0 += kThreads) {
+                __syncthreads();
+                {
+                    int64_t a, b;
+                    s_offsets(m, s0 + tid, a, b);
+                    ga[tid] = a;
+                    gb[tid] = b;
+                }
+                __syncthreads();
+                if (c < nC)
+                    for (int j = 0; j < kThreads; ++j) mac<V, MODE>(acc, A[oa + ga[j]], B[ob + gb[j]]);
+            }
+        }
+        if (c < nC) C[c] = finish<V, MODE>(acc);
+    }
+}
+
+// Split reduction for outputs of at most kReduceMaxOut elements.  Block b sums the
+// summed-index chunk [b*chunk, (b+1)*chunk) for every output and writes
+// partials[b * nC + c]; k_reduce_final adds the partials in block order.
+constexpr int kReduceMaxOut = 64;
+
+template <class V>
+__device__ __forceinline__ V shfl_down_v(V v, int d) {
+    V r;
+    r.x = __shfl_down(v.x, d, 64);
+    r.y = __shfl_down(v.y, d, 64);
+    return r;
+}
+
+template <class V, int MODE>
+__global__ __launch_bounds__(kThreads) void k_reduce_partial(const V* __restrict__ A,
+                                                             const V* __restrict__ B,
+                                                             V* __restrict__ partials, BitMap m,
+                                                             int64_t nC, int64_t chunk) {
+    __shared__ V wsum[kThreads / 64];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int64_t nS = int64_t(1) << m.nbits_s;
+    const int64_t s_begin = int64_t(blockIdx.x) * chunk;
+    const int64_t s_end = s_begin + chunk < nS ? s_begin + chunk : nS;
+    const int lo = m.nbits_s < 8 ? m.nbits_s : 8;
+    int64_t la = 0, lb = 0;  // offsets of the low summed bits owned by this lane
+    for (int j = 0; j < lo; ++j)
+        if ((tid >> j) & 1) {
+            la += int64_t(1) << m.s_a[j];
+            lb += int64_t(1) << m.s_b[j];
+        }
+    for (int64_t c = 0; c < nC; ++c) {
+        int64_t oa, ob;
+        c_offsets(m, c, 0, m.nbits_c, oa, ob);
+        Acc<V, MODE> acc;
+        for (int64_t s = s_begin; s < s_end; s += kThreads) {
+            int64_t ha = 0, hb = 0;  // uniform high part of s
+            for (int j = lo; j < m.nbits_s; ++j)
+                if ((s >> j) & 1) {
+                    ha += int64_t(1) << m.s_a[j];
+                    hb += int64_t(1) << m.s_b[j];
+                }
+            if (s + tid < s_end) mac<V, MODE>(acc, A[oa + ha + la], B[ob + hb + lb]);
+        }
+        V v = finish<V, MODE>(acc);
+        for (int d = 32; d >= 1; d >>= 1) {
+            V o = shfl_down_v(v, d);
+            v.x += o.x;
+            v.y += o.y;
+        }
+        if (lane == 0) wsum[wave] = v;
+        __syncthreads();
+        if (tid == 0) {
+            V t = wsum[0];
+            for (int w = 1; w < kThreads / 64; ++w) {
+                t.x += wsum[w].x;
+                t.y += wsum[w].y;
+            }
+            partials[int64_t(blockIdx.x) * nC + c] = t;
+        }
+        __syncthreads();
+    }
+}
+
+template <class V>
+__global__ void k_reduce_final(const V* __restrict__ partials, V* __restrict__ C, int64_t nC,
+                               int64_t nblocks) {
+    const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= nC) return;
+    V t = partials[c];
+    for (int64_t b = 1; b < nblocks; ++b) {
+        t.x += partials[b * nC + c].x;
+        t.y += partials[b * nC + c].y;
+    }
+    C[c] = t;
+}
+
+// LDS-staged bit permutation (see PermPlan).  Reads walk the input-low bits
+// contiguously, writes walk the output-low bits contiguously.
+template <class V>
+__global__ __launch_bounds__(kThreads) void k_permute(const V* __restrict__ in, V* __restrict__ out,
+                                                      PermPlan p) {
+    extern __shared__ unsigned char smem_raw[];
+    V* tile = reinterpret_cast<V*>(smem_raw);
+    const int64_t rb = int64_t(blockIdx.x) + int64_t(blockIdx.y) * gridDim.x;
+    int64_t base_in = 0, base_out = 0;
+    for (int i = 0; i < p.n_rest; ++i)
+        if ((rb >> i) & 1) {
+            base_in += int64_t(1) << p.rest_in[i];
+            base_out += int64_t(1) << p.rest_out[i];
+        }
+    const int tbits = p.tl + p.n_extra_in;
+    const int n = 1 << tbits;
+    const int low_mask = (1 << p.tl) - 1;
+    for (int t = threadIdx.x; t < n; t += kThreads) {
+        int64_t off = base_in + (t & low_mask);
+        for (int j = 0; j < p.n_extra_in; ++j)
+            if ((t >> (p.tl + j)) & 1) off += int64_t(1) << p.extra_in_pos[j];
+        tile[t] = in[off];
+    }
+    __syncthreads();
+    for (int u = threadIdx.x; u < n; u += kThreads) {
+        int t = 0;
+        int64_t off = base_out + (u & low_mask);
+        for (int j = 0; j < p.tl; ++j)
+            if ((u >> j) & 1) t |= 1 << p.lowout_tile[j];
+        for (int j = 0; j < p.n_extra_out; ++j)
+            if ((u >> (p.tl + j)) & 1) {
+                t |= 1 << p.extra_out_tile[j];
+                off += int64_t(1) << p.extra_out_pos[j];
+            }
+        out[off] = tile[t];
+    }
+}
+
+// Complex FP64 GEMM on MFMA: C[m + M n] = sum_s A'[s][m] * B'[s][n].
+// 64x64 output tile per workgroup, four waves of 32x32; each 16x16 block of a wave
+// takes four real v_mfma_f64_16x16x4f64 per k-step of 4 (re*re, -im*im, re*im, im*re).
+// The MFMA row operand carries B' (rows = n) and the column operand A' (cols = m), so
+// the lanes of one accumulator row hold 16 consecutive m and store contiguously.
+constexpr int GM = 64, GN = 64, GK = 16;
+typedef double d4_t __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(kThreads) void k_gemm_c128(const double2* __restrict__ Ap,
+                                                        const double2* __restrict__ Bp,
+                                                        double2* __restrict__ C, int64_t M,
+                                                        int64_t N, int64_t K) {
+    __shared__ double as_re[GK][GM + 1], as_im[GK][GM + 1];
+    __shared__ double bs_re[GK][GN + 1], bs_im[GK][GN + 1];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int64_t tiles_m = M / GM;
+    const int64_t tile = int64_t(blockIdx.x) + int64_t(blockIdx.y) * gridDim.x;
+    const int64_t m0 = (tile % tiles_m) * GM, n0 = (tile / tiles_m) * GN;
+    d4_t acc_re[2][2], acc_im[2][2];
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) {
+            acc_re[i][j] = d4_t{0, 0, 0, 0};
+            acc_im[i][j] = d4_t{0, 0, 0, 0};
+        }
+    const int li = lane & 15, lk = lane >> 4;
+    for (int64_t k0 = 0; k0 < K; k0 += GK) {
+        for (int e = tid; e < GK * GM; e += kThreads) {
+            const int kk = e / GM, mm = e % GM;
+            const double2 v = Ap[(k0 + kk) * M + m0 + mm];
+            as_re[kk][mm] = v.x;
+            as_im[kk][mm] = v.y;
+        }
+        for (int e = tid; e < GK * GN; e += kThreads) {
+            const int kk = e / GN, nn = e % GN;
+            const double2 v = Bp[(k0 + kk) * N + n0 + nn];
+            bs_re[kk][nn] = v.x;
+            bs_im[kk][nn] = v.y;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < GK; kk += 4) {
+            const int kl = kk + lk;
+            double are[2], aim[2];
+#pragma unroll
+            for (int bm = 0; bm < 2; ++bm) {
+                const int mm = wm * 32 + bm * 16 + li;
+                are[bm] = as_re[kl][mm];
+                aim[bm] = as_im[kl][mm];
+            }
+#pragma unroll
+            for (int bn = 0; bn < 2; ++bn) {
+                const int nn = wn * 32 + bn * 16 + li;
+                const double bre = bs_re[kl][nn], bim = bs_im[kl][nn];
+#pragma unroll
+                for (int bm = 0; bm < 2; ++bm) {
+                    acc_re[bn][bm] = __builtin_amdgcn_mfma_f64_16x16x4f64(bre, are[bm], acc_re[bn][bm], 0, 0, 0);
+                    acc_re[bn][bm] = __builtin_amdgcn_mfma_f64_16x16x4f64(-bim, aim[bm], acc_re[bn][bm], 0, 0, 0);
+                    acc_im[bn][bm] = __builtin_amdgcn_mfma_f64_16x16x4f64(bim, are[bm], acc_im[bn][bm], 0, 0, 0);
+                    acc_im[bn][bm] = __builtin_amdgcn_mfma_f64_16x16x4f64(bre, aim[bm], acc_im[bn][bm], 0, 0, 0);
+                }
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int bn = 0; bn < 2; ++bn)
+#pragma unroll
+        for (int bm = 0; bm < 2; ++bm)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t n = n0 + wn * 32 + bn * 16 + 4 * lk + r;
+                const int64_t mm = m0 + wm * 32 + bm * 16 + li;
+                C[mm + M * n] = double2{acc_re[bn][bm][r], acc_im[bn][bm][r]};
+            }
+}
+
+__global__ void k_fill_zero(uint4* p, int64_t n16) {
+    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n16; i += stride)
+        p[i] = uint4{0, 0, 0, 0};
+}
+
+inline unsigned grid_for(int64_t items, int64_t per_block, unsigned cap) {
+    int64_t g = (items + per_block - 1) / per_block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return unsigned(g);
+}
+
+template <class V, int MODE>
+qt_status run_direct(Ctx* c, const V* A, const V* B, V* C, const BitMap& m) {
+    const int64_t nC = int64_t(1) << m.nbits_c;
+    const unsigned grid = grid_for(nC, kThreads, 16384);
+    hipLaunchKernelGGL((k_direct<V, MODE>), dim3(grid), dim3(kThreads), 0, c->stream, A, B, C, m, nC);
+    return check_hip(c, hipGetLastError(), "k_direct");
+}
+
+template <class V, int MODE>
+qt_status run_reduce(Ctx* c, const V* A, const V* B, V* C, const BitMap& m) {
+    const int64_t nC = int64_t(1) << m.nbits_c;
+    const int64_t nS = int64_t(1) << m.nbits_s;
+    int64_t chunk = kThreads * 16;
+    while (nS / chunk > 2048) chunk <<= 1;
+    const int64_t nblocks = (nS + chunk - 1) / chunk;
+    const size_t need = size_t(nblocks * nC) * sizeof(V);
+    if (c->partials_bytes < need) {
+        if (c->partials) {
+            (void)hipStreamSynchronize(c->stream);
+            (void)hipFree(c->partials);
+            c->partials = nullptr;
+            c->partials_bytes = 0;
+        }
+        qt_status s = check_hip(c, hipMalloc(&c->partials, need), "partials");
+        if (s != QT_OK) return s;
+        c->partials_bytes = need;
+    }
+    V* part = static_cast<V*>(c->partials);
+    hipLaunchKernelGGL((k_reduce_partial<V, MODE>), dim3(unsigned(nblocks)), dim3(kThreads), 0,
+                       c->stream, A, B, part, m, nC, chunk);
+    qt_status s = check_hip(c, hipGetLastError(), "k_reduce_partial");
+    if (s != QT_OK) return s;
+    hipLaunchKernelGGL((k_reduce_final<V>), dim3(grid_for(nC, 64, 1024)), dim3(64), 0, c->stream,
+                       part, C, nC, nblocks);
+    return check_hip(c, hipGetLastError(), "k_reduce_final");
+}
+
+template <class V>
+qt_status run_permute(Ctx* c, const V* in, V* out, const PermPlan& p) {
+    const int64_t blocks = int64_t(1) << p.n_rest;
+    const unsigned gx = unsigned(blocks < 65536 ? blocks : 65536);
+    const unsigned gy = unsigned((blocks + gx - 1) / gx);
+    const size_t lds = (size_t(1) << (p.tl + p.n_extra_in)) * sizeof(V);
+    hipLaunchKernelGGL((k_permute<V>), dim3(gx, gy), dim3(kThreads), lds, c->stream, in, out, p);
+    return check_hip(c, hipGetLastError(), "k_permute");
+}
+
+bool is_identity(const int* src, int n) {
+    for (int i = 0; i < n; ++i)
+        if (src[i] != i) return false;
+    return true;
+}
+
+}  // namespace
+
+PermPlan make_perm_plan(int nbits, const int* src) {
+    PermPlan p;
+    std::memset(&p, 0, sizeof p);
+    p.nbits = nbits;
+    const int tl = nbits < 5 ? nbits : 5;
+    p.tl = tl;
+    int inv[64];
+    for (int j = 0; j < nbits; ++j) inv[src[j]] = j;
+    int tile_of_input[64];
+    for (int i = 0; i < 64; ++i) tile_of_input[i] = -1;
+    for (int i = 0; i < tl; ++i) tile_of_input[i] = i;
+    for (int j = 0; j < tl; ++j) {
+        const int s = src[j];
+        if (s >= tl) {
+            p.extra_in_pos[p.n_extra_in] = uint8_t(s);
+            p.extra_in_out[p.n_extra_in] = uint8_t(j);
+            tile_of_input[s] = tl + p.n_extra_in;
+            ++p.n_extra_in;
+        }
+    }
+    for (int i = 0; i < tl; ++i) p.lowin_out[i] = uint8_t(inv[i]);
+    for (int j = 0; j < tl; ++j) p.lowout_tile[j] = uint8_t(tile_of_input[src[j]]);
+    for (int i = 0; i < tl; ++i)
+        if (inv[i] >= tl) {
+            p.extra_out_pos[p.n_extra_out] = uint8_t(inv[i]);
+            p.extra_out_tile[p.n_extra_out] = uint8_t(i);
+            ++p.n_extra_out;
+        }
+    for (int i = 0; i < nbits; ++i)
+        if (tile_of_input[i] < 0) {
+            p.rest_in[p.n_rest] = uint8_t(i);
+            p.rest_out[p.n_rest] = uint8_t(inv[i]);
+            ++p.n_rest;
+        }
+    return p;
+}
+
+qt_status permute(Ctx* c, const void* in, void* out, int dtype, const PermPlan& p) {
+    if (dtype == QT_DTYPE_C128)
+        return run_permute<double2>(c, static_cast<const double2*>(in), static_cast<double2*>(out), p);
+    return run_permute<float2>(c, static_cast<const float2*>(in), static_cast<float2*>(out), p);
+}
+
+qt_status launch_fill_zero(Ctx* c, void* p, size_t bytes) {
+    const int64_t n16 = int64_t((bytes + 15) / 16);  // pool blocks are >= 256 B, 16 B aligned
+    hipLaunchKernelGGL(k_fill_zero, dim3(grid_for(n16, 256, 8192)), dim3(256), 0, c->stream,
+                       static_cast<uint4*>(p), n16);
+    return check_hip(c, hipGetLastError(), "k_fill_zero");
+}
+
+namespace {
+
+// Large complex128 steps: permute into GEMM layout (skipped when already there), MFMA GEMM.
+qt_status run_gemm_c128(Ctx* c, const Tensor* A, const Tensor* B, Tensor* C, const BitMap& m,
+                        int mbits) {
+    const int nbits_n = m.nbits_c - mbits;
+    const int sb = m.nbits_s;
+    int src_a[64], src_b[64];
+    for (int j = 0; j < mbits; ++j) src_a[j] = m.c_src[j];
+    for (int j = 0; j < sb; ++j) src_a[mbits + j] = m.s_a[j];
+    for (int j = 0; j < nbits_n; ++j) src_b[j] = m.c_src[mbits + j] - 64;
+    for (int j = 0; j < sb; ++j) src_b[nbits_n + j] = m.s_b[j];
+    const double2* ap = static_cast<const double2*>(A->data);
+    const double2* bp = static_cast<const double2*>(B->data);
+    void* abuf = nullptr;
+    void* bbuf = nullptr;
+    qt_status s = QT_OK;
+    if (!is_identity(src_a, mbits + sb)) {
+        hipError_t e;
+        abuf = c->pool.alloc(A->bytes, &e);
+        if (!abuf) return check_hip(c, e, "gemm scratch A");
+        s = permute(c, A->data, abuf, QT_DTYPE_C128, make_perm_plan(mbits + sb, src_a));
+        ap = static_cast<const double2*>(abuf);
+    }
+    if (s == QT_OK && !is_identity(src_b, nbits_n + sb)) {
+        hipError_t e;
+        bbuf = c->pool.alloc(B->bytes, &e);
+        if (!bbuf) {
+            c->pool.release(abuf);
+            return check_hip(c, e, "gemm scratch B");
+        }
+        s = permute(c, B->data, bbuf, QT_DTYPE_C128, make_perm_plan(nbits_n + sb, src_b));
+        bp = static_cast<const double2*>(bbuf);
+    }
+    if (s == QT_OK) {
+        const int64_t M = int64_t(1) << mbits, N = int64_t(1) << nbits_n, K = int64_t(1) << sb;
+        const int64_t tiles = (M / GM) * (N / GN);
+        const unsigned gx = unsigned(tiles < 65536 ? tiles : 65536);
+        const unsigned gy = unsigned((tiles + gx - 1) / gx);
+        hipLaunchKernelGGL(k_gemm_c128, dim3(gx, gy), dim3(kThreads), 0, c->stream, ap, bp,
+                           static_cast<double2*>(C->data), M, N, K);
+        s = check_hip(c, hipGetLastError(), "k_gemm_c128");
+    }
+    // stream-ordered reuse: later work on this stream may take these blocks
+    c->pool.release(abuf);
+    c->pool.release(bbuf);
+    return s;
+}
+
+template <class V, int MODE>
+qt_status dispatch_small(Ctx* c, const Tensor* A, const Tensor* B, Tensor* C, const BitMap& m) {
+    const V* a = static_cast<const V*>(A->data);
+    const V* b = static_cast<const V*>(B->data);
+    V* cc = static_cast<V*>(C->data);
+    const int64_t nC = int64_t(1) << m.nbits_c;
+    const int64_t nS = int64_t(1) << m.nbits_s;
+    if (nC <= kReduceMaxOut && nS >= 8192) return run_reduce<V, MODE>(c, a, b, cc, m);
+    return run_direct<V, MODE>(c, a, b, cc, m);
+}
+
+}  // namespace
+
+qt_status contract_pair(Ctx* c, const Tensor* A, const Tensor* B, Tensor* C, const BitMap& m,
+                        int precision) {
+    if (A->dtype == QT_DTYPE_C128) {
+        // GEMM shape: C's low bits all from A, high bits all from B
+        int mbits = 0;
+        while (mbits < m.nbits_c && m.c_src[mbits] < 64) ++mbits;
+        bool split = true;
+        for (int i = mbits; i < m.nbits_c; ++i)
+            if (m.c_src[i] < 64) split = false;
+        const int nbits_n = m.nbits_c - mbits;
+        if (split && mbits >= 6 && nbits_n >= 6 && m.nbits_s >= 4)
+            return run_gemm_c128(c, A, B, C, m, mbits);
+        return dispatch_small<double2, MODE_PLAIN>(c, A, B, C, m);
+    }
+    switch (precision) {
+        case QT_C64_4M: return dispatch_small<float2, MODE_4M>(c, A, B, C, m);
+        case QT_C64_3M: return dispatch_small<float2, MODE_3M>(c, A, B, C, m);
+        default: return dispatch_small<float2, MODE_BF16X3>(c, A, B, C, m);
+    }
+}
+
+}  // namespace qt

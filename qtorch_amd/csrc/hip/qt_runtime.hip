@@ -1,0 +1,345 @@
+// Context, caching allocator and tensor entry points of libqtorch_hip.so.
+#include <cstring>
+#include <new>
+
+#include "qt_internal.h"
+
+namespace qt {
+
+size_t Pool::round(size_t bytes) {
+    if (bytes <= 256) return 256;
+    if (bytes <= (size_t(1) << 24)) {  // power-of-two classes up to 16 MiB
+        size_t r = 256;
+        while (r < bytes) r <<= 1;
+        return r;
+    }
+    const size_t g = size_t(1) << 21;  // 2 MiB granules above
+    return (bytes + g - 1) / g * g;
+}
+
+void* Pool::alloc(size_t bytes, hipError_t* err) {
+    const size_t r = round(bytes);
+    auto it = free_.find(r);
+    if (it != free_.end() && !it->second.empty()) {
+        void* p = it->second.back();
+        it->second.pop_back();
+        *err = hipSuccess;
+        return p;
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, r);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        trim();
+        e = hipMalloc(&p, r);
+    }
+    *err = e;
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    sizes_[p] = r;
+    reserved_ += int64_t(r);
+    return p;
+}
+
+void Pool::release(void* p) {
+    if (!p) return;
+    auto it = sizes_.find(p);
+    if (it == sizes_.end()) return;
+    free_[it->second].push_back(p);
+}
+
+void Pool::trim() {
+    for (auto& kv : free_) {
+        for (void* p : kv.second) {
+            (void)hipFree(p);
+            reserved_ -= int64_t(kv.first);
+            sizes_.erase(p);
+        }
+        kv.second.clear();
+    }
+}
+
+qt_status set_error(Ctx* c, qt_status s, const std::string& msg) {
+    if (c) c->err = msg;
+    return s;
+}
+
+qt_status check_hip(Ctx* c, hipError_t e, const char* what) {
+    if (e == hipSuccess) return QT_OK;
+    (void)hipGetLastError();
+    return set_error(c, e == hipErrorOutOfMemory ? QT_OOM : QT_DEVICE,
+                     std::string(what) + ": " + hipGetErrorString(e));
+}
+
+}  // namespace qt
+
+using qt::check_hip;
+using qt::set_error;
+
+extern "C" {
+
+qt_status qt_ctx_create(int device, qt_ctx** out) {
+    if (!out) return QT_INVALID_INPUT;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+        (void)hipGetLastError();
+        return QT_DEVICE;
+    }
+    if (device < 0 || device >= n) return QT_INVALID_INPUT;
+    qt_ctx* c = new (std::nothrow) qt_ctx();
+    if (!c) return QT_OOM;
+    c->impl.device = device;
+    if (hipSetDevice(device) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->impl.stream, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipGetLastError();
+        delete c;
+        return QT_DEVICE;
+    }
+    *out = c;
+    return QT_OK;
+}
+
+qt_status qt_ctx_destroy(qt_ctx* c) {
+    if (!c) return QT_INVALID_INPUT;
+    {
+        std::lock_guard<std::recursive_mutex> g(c->impl.mu);
+        (void)hipSetDevice(c->impl.device);
+        (void)hipStreamSynchronize(c->impl.stream);
+        c->impl.pool.trim();
+        if (c->impl.partials) (void)hipFree(c->impl.partials);
+        if (c->impl.staging) (void)hipHostFree(c->impl.staging);
+        (void)hipStreamDestroy(c->impl.stream);
+    }
+    delete c;
+    return QT_OK;
+}
+
+qt_status qt_sync(qt_ctx* c) {
+    if (!c) return QT_INVALID_INPUT;
+    std::lock_guard<std::recursive_mutex> g(c->impl.mu);
+    (void)hipSetDevice(c->impl.device);
+    return check_hip(&c->impl, hipStreamSynchronize(c->impl.stream), "hipStreamSynchronize");
+}
+
+const char* qt_last_error(const qt_ctx* c) { return c ? c->impl.err.c_str() : "null context"; }
+
+int64_t qt_mem_reserved(const qt_ctx* c) { return c ? c->impl.pool.reserved() : 0; }
+
+qt_status qt_tensor_alloc(qt_ctx* c, int rank, int dim_log2, int dtype, qt_tensor** out) {
+    if (!c || !out) return QT_INVALID_INPUT;
+    *out = nullptr;
+    auto* ctx = &c->impl;
+    if (rank < 0 || (dim_log2 != 1 && dim_log2 != 2) ||
+        (dtype != QT_DTYPE_C128 && dtype != QT_DTYPE_C64))
+        return set_error(ctx, QT_INVALID_INPUT, "qt_tensor_alloc: bad rank/dim/dtype");
+    if (rank * dim_log2 > 40)
+        return set_error(ctx, QT_OOM, "qt_tensor_alloc: tensor larger than 2^40 elements");
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    qt_tensor* t = new (std::nothrow) qt_tensor();
+    if (!t) return QT_OOM;
+    t->impl.rank = rank;
+    t->impl.dim_log2 = dim_log2;
+    t->impl.dtype = dtype;
+    t->impl.numel = int64_t(1) << (rank * dim_log2);
+    t->impl.bytes = size_t(t->impl.numel) * qt::elem_bytes(dtype);
+    hipError_t e;
+    t->impl.data = ctx->pool.alloc(t->impl.bytes, &e);
+    if (!t->impl.data) {
+        delete t;
+        return check_hip(ctx, e, "qt_tensor_alloc");
+    }
+    qt_status s = qt::launch_fill_zero(ctx, t->impl.data, t->impl.bytes);
+    if (s != QT_OK) {
+        ctx->pool.release(t->impl.data);
+        delete t;
+        return s;
+    }
+    *out = t;
+    return QT_OK;
+}
+
+qt_status qt_tensor_create_many(qt_ctx* c, int n, const int* ranks, int dim_log2, int dtype,
+                                const void* const* host, qt_tensor** out) {
+    if (!c || n < 0 || (n > 0 && (!ranks || !host || !out))) return QT_INVALID_INPUT;
+    auto* ctx = &c->impl;
+    if ((dim_log2 != 1 && dim_log2 != 2) || (dtype != QT_DTYPE_C128 && dtype != QT_DTYPE_C64))
+        return set_error(ctx, QT_INVALID_INPUT, "qt_tensor_create_many: bad dim/dtype");
+    size_t total = 0;
+    for (int i = 0; i < n; ++i) {
+        if (ranks[i] < 0 || ranks[i] * dim_log2 > 30 || !host[i])
+            return set_error(ctx, QT_INVALID_INPUT, "qt_tensor_create_many: bad rank or host pointer");
+        total += (size_t(1) << (ranks[i] * dim_log2)) * qt::elem_bytes(dtype);
+    }
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    if (ctx->staging_bytes < total) {
+        if (ctx->staging) (void)hipHostFree(ctx->staging);
+        ctx->staging = nullptr;
+        ctx->staging_bytes = 0;
+        qt_status s = check_hip(ctx, hipHostMalloc(&ctx->staging, total), "staging");
+        if (s != QT_OK) return s;
+        ctx->staging_bytes = total;
+    }
+    for (int i = 0; i < n; ++i) out[i] = nullptr;
+    size_t off = 0;
+    qt_status s = QT_OK;
+    for (int i = 0; i < n && s == QT_OK; ++i) {
+        qt_tensor* t = new (std::nothrow) qt_tensor();
+        if (!t) {
+            s = QT_OOM;
+            break;
+        }
+        t->impl.rank = ranks[i];
+        t->impl.dim_log2 = dim_log2;
+        t->impl.dtype = dtype;
+        t->impl.numel = int64_t(1) << (ranks[i] * dim_log2);
+        t->impl.bytes = size_t(t->impl.numel) * qt::elem_bytes(dtype);
+        hipError_t e;
+        t->impl.data = ctx->pool.alloc(t->impl.bytes, &e);
+        if (!t->impl.data) {
+            delete t;
+            s = check_hip(ctx, e, "qt_tensor_create_many");
+            break;
+        }
+        out[i] = t;
+        char* stage = static_cast<char*>(ctx->staging) + off;
+        std::memcpy(stage, host[i], t->impl.bytes);
+        s = check_hip(ctx, hipMemcpyAsync(t->impl.data, stage, t->impl.bytes, hipMemcpyHostToDevice,
+                                          ctx->stream), "qt_tensor_create_many copy");
+        off += t->impl.bytes;
+    }
+    qt_status ss = check_hip(ctx, hipStreamSynchronize(ctx->stream), "qt_tensor_create_many sync");
+    if (s == QT_OK) s = ss;
+    if (s != QT_OK) {
+        for (int i = 0; i < n; ++i)
+            if (out[i]) {
+                ctx->pool.release(out[i]->impl.data);
+                delete out[i];
+                out[i] = nullptr;
+            }
+    }
+    return s;
+}
+
+qt_status qt_tensor_upload(qt_ctx* c, qt_tensor* t, const void* host, int64_t nbytes) {
+    if (!c || !t || !host) return QT_INVALID_INPUT;
+    auto* ctx = &c->impl;
+    if (nbytes != int64_t(t->impl.bytes))
+        return set_error(ctx, QT_INVALID_INPUT, "qt_tensor_upload: size mismatch");
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    // host memory is pageable: the copy is synchronous with respect to the host buffer
+    qt_status s = check_hip(ctx, hipMemcpyAsync(t->impl.data, host, size_t(nbytes),
+                                                hipMemcpyHostToDevice, ctx->stream),
+                            "qt_tensor_upload");
+    if (s != QT_OK) return s;
+    return check_hip(ctx, hipStreamSynchronize(ctx->stream), "qt_tensor_upload sync");
+}
+
+qt_status qt_tensor_download(qt_ctx* c, const qt_tensor* t, void* host, int64_t nbytes) {
+    if (!c || !t || !host) return QT_INVALID_INPUT;
+    auto* ctx = &c->impl;
+    if (nbytes != int64_t(t->impl.bytes))
+        return set_error(ctx, QT_INVALID_INPUT, "qt_tensor_download: size mismatch");
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    qt_status s = check_hip(ctx, hipMemcpyAsync(host, t->impl.data, size_t(nbytes),
+                                                hipMemcpyDeviceToHost, ctx->stream),
+                            "qt_tensor_download");
+    if (s != QT_OK) return s;
+    return check_hip(ctx, hipStreamSynchronize(ctx->stream), "qt_tensor_download sync");
+}
+
+qt_status qt_tensor_read_scalar(qt_ctx* c, const qt_tensor* t, double* re, double* im) {
+    if (!c || !t || !re || !im) return QT_INVALID_INPUT;
+    if (t->impl.numel != 1)
+        return set_error(&c->impl, QT_INVALID_INPUT, "qt_tensor_read_scalar: tensor is not rank 0");
+    if (t->impl.dtype == QT_DTYPE_C128) {
+        double v[2];
+        qt_status s = qt_tensor_download(c, t, v, sizeof v);
+        *re = v[0];
+        *im = v[1];
+        return s;
+    }
+    float v[2];
+    qt_status s = qt_tensor_download(c, t, v, sizeof v);
+    *re = v[0];
+    *im = v[1];
+    return s;
+}
+
+qt_status qt_tensor_free(qt_ctx* c, qt_tensor* t) {
+    if (!c || !t) return QT_INVALID_INPUT;
+    std::lock_guard<std::recursive_mutex> g(c->impl.mu);
+    c->impl.pool.release(t->impl.data);
+    delete t;
+    return QT_OK;
+}
+
+int64_t qt_tensor_numel(const qt_tensor* t) { return t ? t->impl.numel : 0; }
+int qt_tensor_rank(const qt_tensor* t) { return t ? t->impl.rank : -1; }
+
+qt_status qt_contract_pair(qt_ctx* c, const qt_tensor* A, int rank_a, const qt_tensor* B,
+                           int rank_b, int k, const int32_t* pair_a, const int32_t* pair_b,
+                           int rank_c, const int32_t* free_src, qt_tensor* C, int precision,
+                           int64_t* cmacs) {
+    if (!c) return QT_INVALID_INPUT;
+    auto* ctx = &c->impl;
+    if (!A || !B || !C || (k > 0 && (!pair_a || !pair_b)) || (rank_c > 0 && !free_src))
+        return set_error(ctx, QT_INVALID_INPUT, "qt_contract_pair: null argument");
+    const qt::Tensor& a = A->impl;
+    const qt::Tensor& b = B->impl;
+    qt::Tensor& cc = C->impl;
+    if (a.rank != rank_a || b.rank != rank_b || cc.rank != rank_c)
+        return set_error(ctx, QT_INVALID_INPUT, "qt_contract_pair: rank does not match tensor");
+    if (a.dtype != b.dtype || a.dtype != cc.dtype || a.dim_log2 != b.dim_log2 ||
+        a.dim_log2 != cc.dim_log2)
+        return set_error(ctx, QT_INVALID_INPUT, "qt_contract_pair: dtype/dim mismatch");
+    if (&a == &cc || &b == &cc)
+        return set_error(ctx, QT_INVALID_INPUT, "qt_contract_pair: C aliases an operand");
+    if (k < 0 || k > rank_a || k > rank_b || rank_c != rank_a + rank_b - 2 * k)
+        return set_error(ctx, QT_INVALID_INPUT, "qt_contract_pair: inconsistent ranks");
+    const bool c128 = a.dtype == QT_DTYPE_C128;
+    if ((c128 && precision != QT_PLAIN_C128) ||
+        (!c128 && precision != QT_C64_4M && precision != QT_C64_3M && precision != QT_C64_BF16X3))
+        return set_error(ctx, QT_INVALID_INPUT, "qt_contract_pair: precision does not fit dtype");
+    const int dl = a.dim_log2;
+    if ((rank_c + k) * dl > 62)
+        return set_error(ctx, QT_INVALID_INPUT, "qt_contract_pair: index space too large");
+    // every axis of A and of B is used exactly once (summed or free)
+    std::vector<int> used_a(rank_a, 0), used_b(rank_b, 0);
+    qt::BitMap m;
+    std::memset(&m, 0, sizeof m);
+    m.nbits_c = rank_c * dl;
+    m.nbits_s = k * dl;
+    for (int i = 0; i < k; ++i) {
+        const int pa = pair_a[i], pb = pair_b[i];
+        if (pa < 0 || pa >= rank_a || pb < 0 || pb >= rank_b || used_a[pa]++ || used_b[pb]++)
+            return set_error(ctx, QT_INVALID_INPUT, "qt_contract_pair: bad pair");
+        // digit j of s belongs to pair k-1-j (src/Network.h:912-916)
+        const int j = k - 1 - i;
+        for (int t = 0; t < dl; ++t) {
+            m.s_a[j * dl + t] = uint8_t(pa * dl + t);
+            m.s_b[j * dl + t] = uint8_t(pb * dl + t);
+        }
+    }
+    for (int i = 0; i < rank_c; ++i) {
+        const bool from_a = (free_src[i] & QT_FROM_A) != 0;
+        const int ax = free_src[i] & ~QT_FROM_A;
+        if (from_a ? (ax < 0 || ax >= rank_a || used_a[ax]++) : (ax < 0 || ax >= rank_b || used_b[ax]++))
+            return set_error(ctx, QT_INVALID_INPUT, "qt_contract_pair: bad free axis");
+        for (int t = 0; t < dl; ++t)
+            m.c_src[i * dl + t] = uint8_t(from_a ? ax * dl + t : 64 + ax * dl + t);
+    }
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    qt_status s = qt::contract_pair(ctx, &a, &b, &cc, m, precision);
+    if (s == QT_OK && cmacs) *cmacs = int64_t(1) << ((rank_c + k) * dl);
+    return s;
+}
+
+}  // extern "C"

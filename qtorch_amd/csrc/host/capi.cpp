@@ -1,0 +1,255 @@
+// C ABI of the host library (include/qtorch_host.h).  Exceptions never cross it.
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <stdexcept>
+
+#include "qtorch.hpp"
+#include "qtorch_host.h"
+
+using namespace qtorch;
+
+struct qth_network {
+    std::shared_ptr<Network> net;
+    std::unique_ptr<LineGraph> lg;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+template <class F>
+int Guard(F&& f) {
+    try {
+        f();
+        g_err.clear();
+        return QTH_OK;
+    } catch (const InvalidFunctionInput& e) {
+        g_err = e.what();
+        return QTH_INVALID_FUNCTION_INPUT;
+    } catch (const InvalidFile& e) {
+        g_err = e.what();
+        return QTH_INVALID_FILE;
+    } catch (const InvalidFileFormat& e) {
+        g_err = e.what();
+        return QTH_INVALID_FILE_FORMAT;
+    } catch (const ContractionFailure& e) {
+        g_err = e.what();
+        return QTH_CONTRACTION_FAILURE;
+    } catch (const InvalidUserContractionSequence& e) {
+        g_err = e.what();
+        return QTH_INVALID_USER_SEQUENCE;
+    } catch (const InvalidContractionMethod& e) {
+        g_err = e.what();
+        return QTH_INVALID_METHOD;
+    } catch (const QbbFailure& e) {
+        g_err = e.what();
+        return QTH_QBB_FAILURE;
+    } catch (const DeviceFailure& e) {
+        g_err = e.what();
+        return QTH_DEVICE;
+    } catch (const std::invalid_argument& e) {  // std::stoi / std::stof on a bad token
+        g_err = std::string("invalid number in input: ") + e.what();
+        return QTH_INVALID_FILE_FORMAT;
+    } catch (const std::out_of_range& e) {
+        g_err = std::string("number out of range in input: ") + e.what();
+        return QTH_INVALID_FILE_FORMAT;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return QTH_OTHER;
+    }
+}
+
+void Out(const std::complex<double>& v, double* re, double* im) {
+    if (re) *re = v.real();
+    if (im) *im = v.imag();
+}
+
+std::vector<std::pair<int, int>> Pairs(const int32_t* p, int n) {
+    std::vector<std::pair<int, int>> v;
+    for (int i = 0; i < n; ++i) v.push_back({p[2 * i], p[2 * i + 1]});
+    return v;
+}
+
+LineGraph& Lg(qth_network* h) {
+    if (!h->lg) h->lg = std::make_unique<LineGraph>(h->net);
+    return *h->lg;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* qth_last_error(void) { return g_err.c_str(); }
+
+int qth_network_create(qt_ctx* ctx, const char* qasm, const char* measure, const char* base_dir,
+                       qth_network** out) {
+    if (!qasm || !out) return QTH_INVALID_FUNCTION_INPUT;
+    *out = nullptr;
+    return Guard([&] {
+        NetworkOptions opt;
+        opt.ctx = ctx;
+        opt.base_dir = base_dir ? base_dir : "";
+        std::string m = measure ? measure : "";
+        auto h = std::make_unique<qth_network>();
+        h->net = std::make_shared<Network>(std::string(qasm), measure ? &m : nullptr, opt);
+        *out = h.release();
+    });
+}
+
+int qth_network_destroy(qth_network* h) {
+    delete h;
+    return QTH_OK;
+}
+
+int qth_network_reduce(qth_network* h) {
+    if (!h) return QTH_INVALID_FUNCTION_INPUT;
+    return Guard([&] { h->net->ReduceCircuit(); });
+}
+
+int qth_contract_nodes(qth_network* h, int a, int b, int threshold, int* result_id) {
+    if (!h) return QTH_INVALID_FUNCTION_INPUT;
+    return Guard([&] {
+        const auto& all = h->net->GetAllNodes();
+        if (a < 0 || b < 0 || size_t(a) >= all.size() || size_t(b) >= all.size()) throw InvalidFunctionInput();
+        auto r = h->net->ContractNodes(all[size_t(a)], all[size_t(b)], threshold);
+        if (result_id) *result_id = r ? r->mID : -1;
+    });
+}
+
+int qth_contract_stochastic(qth_network* h, uint64_t seed, double* re, double* im) {
+    if (!h) return QTH_INVALID_FUNCTION_INPUT;
+    return Guard([&] {
+        ContractionTools ct(h->net);
+        ct.SetSeed(seed);
+        ct.Contract(Stochastic);
+        Out(ct.GetFinalVal(), re, im);
+    });
+}
+
+int qth_contract_given(qth_network* h, const int32_t* pairs, int n, double* re, double* im) {
+    if (!h || (n > 0 && !pairs)) return QTH_INVALID_FUNCTION_INPUT;
+    return Guard([&] {
+        ContractionTools ct(h->net);
+        ct.ContractGivenSequence(Pairs(pairs, n));
+        Out(ct.GetFinalVal(), re, im);
+    });
+}
+
+int qth_contract_user(qth_network* h, const int32_t* pairs, int n, double* re, double* im) {
+    if (!h || (n > 0 && !pairs)) return QTH_INVALID_FUNCTION_INPUT;
+    return Guard([&] {
+        ContractionTools ct(h->net);
+        ct.ContractUserDefinedSequence(Pairs(pairs, n));
+        Out(ct.GetFinalVal(), re, im);
+    });
+}
+
+int qth_linegraph_cnf(qth_network* h, char* buf, int64_t cap, int64_t* len) {
+    if (!h) return QTH_INVALID_FUNCTION_INPUT;
+    return Guard([&] {
+        const std::string s = Lg(h).CnfText();
+        if (len) *len = int64_t(s.size());
+        if (buf && cap > 0) {
+            const size_t n = std::min(size_t(cap - 1), s.size());
+            std::memcpy(buf, s.data(), n);
+            buf[n] = 0;
+        }
+    });
+}
+
+int qth_linegraph_minfill(qth_network* h, int32_t* order, int cap, int* n) {
+    if (!h) return QTH_INVALID_FUNCTION_INPUT;
+    return Guard([&] {
+        const auto o = Lg(h).MinFillOrder();
+        if (n) *n = int(o.size());
+        for (int i = 0; i < int(o.size()) && i < cap; ++i) order[i] = o[size_t(i)];
+    });
+}
+
+int qth_linegraph_contract(qth_network* h, const int32_t* order, int n, double* re, double* im) {
+    if (!h || (n > 0 && !order)) return QTH_INVALID_FUNCTION_INPUT;
+    return Guard([&] {
+        std::vector<int> o(order, order + n);
+        Out(Lg(h).ContractInOrder(o), re, im);
+    });
+}
+
+int qth_record(qth_network* h, int32_t* pairs, int cap, int* n) {
+    if (!h) return QTH_INVALID_FUNCTION_INPUT;
+    return Guard([&] {
+        const auto r = RecordSequence(*h->net);
+        if (n) *n = int(r.size());
+        for (int i = 0; i < int(r.size()) && i < cap; ++i) {
+            pairs[2 * i] = r[size_t(i)].first;
+            pairs[2 * i + 1] = r[size_t(i)].second;
+        }
+    });
+}
+
+int64_t qth_flops(qth_network* h) { return h ? h->net->getNumFloatOps() : -1; }
+
+int qth_counts(qth_network* h, int* all_nodes, int* uncontracted, int* original, int* done) {
+    if (!h) return QTH_INVALID_FUNCTION_INPUT;
+    if (all_nodes) *all_nodes = int(h->net->GetAllNodes().size());
+    if (uncontracted) *uncontracted = int(h->net->GetUncontractedNodes().size());
+    if (original) *original = h->net->GetNumOriginalNodes();
+    if (done) *done = h->net->IsDone() ? 1 : 0;
+    return QTH_OK;
+}
+
+int qth_plan(qth_network* h, int32_t* steps, int cap, int* n) {
+    if (!h) return QTH_INVALID_FUNCTION_INPUT;
+    const auto& plan = h->net->GetPlan();
+    if (n) *n = int(plan.size());
+    for (int i = 0; i < int(plan.size()) && i < cap; ++i) {
+        const auto& s = plan[size_t(i)];
+        int32_t* o = steps + 8 * i;
+        o[0] = s.a_id;
+        o[1] = s.b_id;
+        o[2] = s.c_id;
+        o[3] = s.rank_a;
+        o[4] = s.rank_b;
+        o[5] = s.k;
+        o[6] = s.rank_c;
+        o[7] = 0;
+    }
+    return QTH_OK;
+}
+
+int qth_qaoa_num_edges(const char* dgf, int* n_edges, int* n_vertices) {
+    if (!dgf) return QTH_INVALID_FUNCTION_INPUT;
+    return Guard([&] {
+        auto d = ExtraData::FromText(1, dgf, true);
+        if (n_edges) *n_edges = int(d.pairs.size());
+        if (n_vertices) *n_vertices = d.numQubits;
+    });
+}
+
+int qth_qaoa_edge_text(const char* dgf, int p, int ref_map, int e, const double* bg, char* qasm,
+                       int64_t qasm_cap, char* measure, int64_t measure_cap) {
+    if (!dgf || !bg || p < 1) return QTH_INVALID_FUNCTION_INPUT;
+    return Guard([&] {
+        auto d = ExtraData::FromText(p, dgf, ref_map != 0);
+        if (e < 0 || size_t(e) >= d.pairs.size()) throw InvalidFunctionInput();
+        std::vector<double> ang(bg, bg + 2 * p);
+        const std::string q = QaoaEdgeQasm(d, e, ang), m = QaoaEdgeMeasure(d, e);
+        if (int64_t(q.size()) >= qasm_cap || int64_t(m.size()) >= measure_cap) throw InvalidFunctionInput();
+        std::memcpy(qasm, q.c_str(), q.size() + 1);
+        std::memcpy(measure, m.c_str(), m.size() + 1);
+    });
+}
+
+int qth_qaoa_energy_plain(qt_ctx* ctx, const char* dgf, int p, int ref_map, const double* bg, uint64_t seed,
+                          double* zz, double* energy) {
+    if (!ctx || !dgf || !bg || p < 1) return QTH_INVALID_FUNCTION_INPUT;
+    return Guard([&] {
+        auto d = ExtraData::FromText(p, dgf, ref_map != 0);
+        std::vector<double> ang(bg, bg + 2 * p), v;
+        const double f = QaoaEnergyPlain(ctx, d, ang, seed, &v);
+        if (zz) std::memcpy(zz, v.data(), v.size() * sizeof(double));
+        if (energy) *energy = f;
+    });
+}
+
+}  // extern "C"
