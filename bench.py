@@ -1,10 +1,19 @@
 #!/usr/bin/env python3
 """QAOA edge-evaluation throughput on one MI355X.
 
-Workload (the metric's configuration): QAOA p=1 MaxCut on a seeded random
-3-regular graph with 30 vertices (45 edges; tests/golden/graphs/3reg_n30_s2024.dgf).
-Each <Z_i Z_j> is its own light-cone network (qTorch's F_p, src/maxcut.cpp:162-204).
-One step = one energy evaluation of the whole graph (45 edge networks).
+Workload: QAOA p=1 MaxCut on a seeded random 3-regular graph with 30 vertices
+(45 edges; tests/golden/graphs/3reg_n30_s2024.dgf).  Each <Z_i Z_j> is its own
+light-cone network in qTorch's Liouville form, exactly the networks F_p builds
+(src/maxcut.cpp:162-204, reference qubit map).  One step = one energy evaluation
+of the graph for each of B angle sets (gamma ~ U[0, pi], beta ~ U[0, pi/2], seeded)
+= 45 * B edge networks, inputs resident on the device.
+
+value: edge-evals/s of the batched complex64 form (persistent kernel, greedy
+contraction order, HIP-graph replay, autotuned executor shape), with per-edge and
+per-energy error against complex128 (the FP64 oracle's values for the reference's
+angle set come from tests/golden/cases.json).  Beside it: the drop-in complex128
+path (reference stochastic order, one pairwise contraction per launch) on the same
+GPU, and the reference itself on this host's CPU cores.
 
 Prints one JSON line.  --gpus N with N > 1 is not supported (one GPU).
 """
@@ -14,7 +23,6 @@ import argparse
 import json
 import math
 import os
-import random
 import subprocess
 import sys
 import tempfile
@@ -24,20 +32,22 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
 GRAPH = os.path.join(REPO, "tests", "golden", "graphs", "3reg_n30_s2024.dgf")
+FP32_PEAK_TFLOPS = 157.3  # MI355X dense FP32 (vector and matrix)
 
 
-def angles(seed: int = 77):
-    """gamma ~ U[0, pi], beta ~ U[0, pi/2]; exp(-i gamma ZZ) and RX(2 beta) in the
-    reference's angle vector [beta, gamma_ref] with gamma_ref = -2 gamma."""
-    rng = random.Random(seed)
-    gamma = rng.uniform(0, math.pi)
-    beta = rng.uniform(0, math.pi / 2)
-    return [beta, -2.0 * gamma]
+def angle_sets(n: int, seed: int = 2024):
+    """Reference angle vectors [beta, gamma_ref], gamma_ref = -2 gamma, so that the
+    emitted CNOT-Rz-CNOT is exp(-i gamma ZZ) and the mixer RX(2 beta)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    gamma = rng.uniform(0, math.pi, n)
+    beta = rng.uniform(0, math.pi / 2, n)
+    return np.stack([beta, -2.0 * gamma], axis=1)
 
 
 def cpu_baseline(dgf: str, bg, target_s: float = 12.0):
-    """The reference itself (oracle/_ref/ref_driver_O2, built from the qTorch sources)
-    contracting every edge network stochastically, as F_p does, on this host."""
+    """The reference (oracle/_ref/ref_driver_O2, built from the qTorch sources) runs
+    F_p's per-edge ContractionTools::Contract(Stochastic) on this host."""
     import qtorch_amd as qa
     drv = os.path.join(REPO, "oracle", "_ref", "ref_driver_O2")
     if not os.path.exists(drv):
@@ -47,10 +57,12 @@ def cpu_baseline(dgf: str, bg, target_s: float = 12.0):
     lst = os.path.join(tmp, "jobs.txt")
     with open(lst, "w") as f:
         for e in range(ne):
-            q, m = qa.qaoa_edge_text(dgf, 1, True, e, bg)
+            q, m = qa.qaoa_edge_text(dgf, 1, True, e, list(bg))
             qp, mp = os.path.join(tmp, f"e{e}.qasm"), os.path.join(tmp, f"e{e}.meas")
-            open(qp, "w").write(q)
-            open(mp, "w").write(m)
+            with open(qp, "w") as g:
+                g.write(q)
+            with open(mp, "w") as g:
+                g.write(m)
             f.write(f"{qp} {mp}\n")
     out = subprocess.run([drv, "time", lst, "1"], capture_output=True, text=True, timeout=600).stdout
     t1 = float(out.split("TIME")[1].split()[0])
@@ -59,17 +71,29 @@ def cpu_baseline(dgf: str, bg, target_s: float = 12.0):
     t = float(out.split("TIME")[1].split()[0])
     jobs = int(out.split("JOBS")[1].split()[0])
     return {"value": jobs / t, "unit": "edge-evals/s", "cores": 8, "kind": "reference",
-            "sample": f"{jobs} edge networks ({reps} x 45 edges) through the reference's "
-                      f"ContractionTools::Contract(Stochastic) built -O2 from its sources "
-                      f"(up to 8 std::threads: 2 partition threads, 8 for output rank >= 8), {t:.1f} s"}
+            "sample": f"{jobs} edge networks ({reps} energy evaluations of the 45-edge graph) through the "
+                      f"reference's ContractionTools::Contract(Stochastic), built -O2 from its sources; it "
+                      f"spawns up to 8 std::threads (2 partition threads, 8 for output rank >= 8); {t:.1f} s"}
+
+
+def golden_reference(golden_path: str):
+    """complex128 values the reference printed for the golden angle set."""
+    with open(golden_path) as f:
+        cases = json.load(f)["cases"]
+    ref = sorted((c for c in cases if c["name"].startswith("qaoa3_ref_")), key=lambda c: c["edge"])
+    import numpy as np
+    return np.array([c["value"][0] for c in ref]), ref[0]["betas_gammas"]
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--sets", type=int, default=8192, help="angle sets per step")
+    ap.add_argument("--form", default="c64_3m", choices=["c64_4m", "c64_3m", "c64_bf16x3", "c128"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine measures one GPU; --gpus must be 1"}))
@@ -78,33 +102,90 @@ def main():
 
     import qtorch_amd as qa
 
+    forms = {"c64_4m": qa.QT_C64_4M, "c64_3m": qa.QT_C64_3M, "c64_bf16x3": qa.QT_C64_BF16X3,
+             "c128": qa.QT_PLAIN_C128}
     dgf = open(GRAPH).read()
-    bg = angles()
     ctx = qa.Context(0)
-    ne, _ = qa.qaoa_num_edges(dgf)
+    zz_ref, bg_ref = golden_reference(os.path.join(REPO, "tests", "golden", "cases.json"))
+    sets = angle_sets(args.sets)
+    sets[0] = bg_ref  # set 0 is the angle set whose reference values are known
+
+    # ---- headline: batched form ----
+    b = qa.QaoaBatch(ctx, dgf, 1, ref_map=True, precision=forms[args.form], order=qa.ORDER_GREEDY)
+    b.set_angles(sets)
+    lanes, wgs, _ = b.autotune()
+    b.capture()
     for _ in range(args.warmup):
-        qa.qaoa_energy_plain(ctx, dgf, 1, True, bg, seed=0)
+        b.launch()
     ctx.sync()
+    ne = b.n_edges
+    # per-step latency (launch + completion), then the throughput run
     lat = []
+    for _ in range(min(args.steps, 20)):
+        t0 = time.perf_counter()
+        b.launch()
+        ctx.sync()
+        lat.append(time.perf_counter() - t0)
+    ctx.sync()
     t0 = time.perf_counter()
-    for s in range(args.steps):
-        ts = time.perf_counter()
-        qa.qaoa_energy_plain(ctx, dgf, 1, True, bg, seed=s)
-        lat.append(time.perf_counter() - ts)
+    for _ in range(args.steps):
+        b.launch()
     ctx.sync()
     wall = time.perf_counter() - t0
-    value = ne * args.steps / wall
+    kernel_ms = b.time(args.steps)  # HIP events on the engine's stream around the same launches
+    energies, zz = b.results()
+    info = b.info()
+    value = ne * args.sets * args.steps / wall
+
+    # ---- accuracy against complex128 on the same plan, and against the reference ----
+    b64 = qa.QaoaBatch(ctx, dgf, 1, ref_map=True, precision=qa.QT_PLAIN_C128, order=qa.ORDER_GREEDY)
+    b64.set_angles(sets)
+    b64.launch()
+    e64, zz64 = b64.results()
+    edge_err = float(np.max(np.abs(zz - zz64) / np.maximum(np.abs(zz64), 1e-3)))
+    energy_err = float(np.max(np.abs(energies - e64) / np.abs(e64)))
+    ref_err = float(np.max(np.abs(zz[0] - zz_ref) / np.maximum(np.abs(zz_ref), 1e-3)))
+    c128_vs_ref = float(np.max(np.abs(zz64[0] - zz_ref) / np.maximum(np.abs(zz_ref), 1e-15)))
+    b64.close()
+
+    # ---- drop-in complex128 path, one energy evaluation per step ----
+    plain = None
+    if not args.no_plain:
+        n_plain = 5
+        qa.qaoa_energy_plain(ctx, dgf, 1, True, list(bg_ref), seed=0)
+        ctx.sync()
+        t0 = time.perf_counter()
+        for s in range(n_plain):
+            qa.qaoa_energy_plain(ctx, dgf, 1, True, list(sets[s]), seed=s)
+        ctx.sync()
+        tp = time.perf_counter() - t0
+        plain = {"value": ne * n_plain / tp, "unit": "edge-evals/s", "ms_per_energy": 1e3 * tp / n_plain,
+                 "order": "reference stochastic (seeded), one pairwise contraction per launch"}
+
+    flops_per_launch = 8.0 * info["cmacs"]
+    achieved = flops_per_launch / (kernel_ms * 1e-3) / 1e12
     res = {
-        "metric": "QAOA edge-evals/s (p99 latency per energy evaluation), complex128 plain path",
+        "metric": "QAOA edge-evals/s (p99 step latency), complex64 batched vs complex128 plain, rel err <= 1e-5",
         "value": value, "unit": "edge-evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": 1e3 * wall / args.steps, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "c128", "data": "synthetic",
-        "config": {"workload": "qaoa_p1_maxcut_3regular_n30", "edges": ne, "form": "plain_c128",
-                   "order": "reference stochastic (seeded)"},
-        "p50_ms": 1e3 * float(np.percentile(lat, 50)), "p99_ms": 1e3 * float(np.percentile(lat, 99)),
+        "vs_baseline": None, "dtype": "c64" if args.form != "c128" else "c128", "data": "synthetic",
+        "config": {"workload": "qaoa_p1_maxcut_3regular_n30_edges45", "angle_sets_per_step": args.sets,
+                   "edge_networks_per_step": ne * args.sets, "form": args.form,
+                   "order": "greedy", "executor_lanes": lanes, "wgs_per_cu": wgs,
+                   "global_scratch": info["global_scratch"]},
+        "p50_step_ms": 1e3 * float(np.percentile(lat, 50)), "p99_step_ms": 1e3 * float(np.percentile(lat, 99)),
+        "accuracy": {"max_edge_rel_err_vs_c128": edge_err, "max_energy_rel_err_vs_c128": energy_err,
+                     "max_edge_rel_err_vs_reference": ref_err, "c128_batched_vs_reference": c128_vs_ref,
+                     "floor": "|d| / max(|x|, 1e-3)"},
+        "plain_c128_gpu": plain,
+        "speedup_vs_plain_c128": (value / plain["value"]) if plain else None,
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "k_batch (persistent fused contraction)", "kernel_ms_per_launch": kernel_ms,
+                     "algorithmic_flops_per_launch": flops_per_launch},
     }
     if not args.no_cpu:
-        res["cpu_baseline"] = cpu_baseline(dgf, bg)
+        res["cpu_baseline"] = cpu_baseline(dgf, bg_ref)
     print(json.dumps(res))
 
 

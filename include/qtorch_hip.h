@@ -105,6 +105,57 @@ qt_status qt_contract_pair(qt_ctx* ctx, const qt_tensor* A, int rank_a, const qt
  * src/Network.h:961-969 and src/LineGraph.h:388); synchronises. */
 qt_status qt_tensor_read_scalar(qt_ctx* ctx, const qt_tensor* t, double* re, double* im);
 
+/* ---- programs and batches (many networks per launch) ---------------------------------
+ * A program is a network's whole sequence of ContractNodes steps (the calls a
+ * qTorch ordering makes: LGContract src/LineGraph.h:357-376, ParallelContract
+ * src/ContractionTools.h:219-388, ContractGivenSequence :392-411) compiled for the
+ * device.  Operand ids: leaves 0..n_leaves-1, then the result of step i is id
+ * n_leaves + i; every intermediate is consumed exactly once and the last step
+ * yields the network's scalar.
+ *   step_desc : 6 ints per step: a, b, rank_a, rank_b, k, rank_c
+ *   step_axes : per step pair_a[k], pair_b[k], free_src[rank_c] (the same conventions
+ *               as qt_contract_pair), concatenated
+ *   leaf_refs : element offset of each leaf in the batch's leaf pool; with bit 62 set
+ *               the offset is relative to the instance's parameter base
+ * Tensors of a program are limited to 2^16 elements (D = 4: rank 8; D = 2: rank 16).
+ * Compilation runs on the host only; ctx may be NULL (errors are then only returned). */
+#define QT_LEAF_PARAM (1LL << 62)
+qt_status qt_program_create(qt_ctx* ctx, int dim_log2, int n_leaves, const int32_t* leaf_ranks,
+                            const int64_t* leaf_refs, int n_steps, const int32_t* step_desc,
+                            const int32_t* step_axes, qt_program** out);
+/* complex MACs (src/Network.h:884-885 counter), scratch elements and step count */
+qt_status qt_program_stats(const qt_program* p, int64_t* cmacs, int64_t* scratch_elems, int* n_steps);
+qt_status qt_program_destroy(qt_program* p);
+
+typedef struct qt_batch qt_batch;
+/* A batch holds programs on the device in one arithmetic form (qt_precision). */
+qt_status qt_batch_create(qt_ctx* ctx, int n_programs, qt_program* const* progs, int precision,
+                          qt_batch** out);
+/* Leaf pool: interleaved complex FP64 values (converted on the device for C64 forms). */
+qt_status qt_batch_set_leaves(qt_batch* b, const double* pool, int64_t n_elems);
+/* Instances: program index and parameter base of each; optional FP64 group sums
+ * sum_{i in g} (c0 + c1 * Re value_i) (the per-graph energy 0.5 * (1 - <ZZ>) of
+ * src/maxcut.cpp:196 with c0 = 0.5, c1 = -0.5). */
+qt_status qt_batch_set_instances(qt_batch* b, int n, const int32_t* prog, const int64_t* param_base,
+                                 int n_groups, const int32_t* group, double c0, double c1);
+/* Executor shape: lanes per network (64 = one wavefront, 256 = one workgroup) and
+ * workgroups per compute unit. */
+qt_status qt_batch_config(qt_batch* b, int lanes, int wgs_per_cu);
+/* Times every executor shape on the current instances and keeps the fastest. */
+qt_status qt_batch_autotune(qt_batch* b, int* lanes, int* wgs_per_cu, float* ms);
+/* Asynchronous run of every instance (replays the captured graph if there is one). */
+qt_status qt_batch_launch(qt_batch* b);
+/* Captures the launch sequence (counter reset, persistent kernel, group sums) into a
+ * HIP graph; later launches replay it. */
+qt_status qt_batch_capture(qt_batch* b);
+/* Synchronises and copies per-instance values (2 doubles each) and group sums. */
+qt_status qt_batch_results(qt_batch* b, double* values, double* group_sums);
+/* Mean time of `reps` launches, by HIP events on the context's stream. */
+qt_status qt_batch_time(qt_batch* b, int reps, float* ms_per_launch);
+qt_status qt_batch_info(const qt_batch* b, int64_t* cmacs, int* lanes, int* wgs_per_cu,
+                        int* global_scratch, int* scratch_elems);
+qt_status qt_batch_destroy(qt_batch* b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -77,6 +77,35 @@ int qth_qaoa_edge_text(const char* dgf, int p, int ref_map, int e, const double*
 int qth_qaoa_energy_plain(qt_ctx* ctx, const char* dgf, int p, int ref_map, const double* betas_gammas,
                           uint64_t seed, double* zz, double* energy);
 
+/* ---- batched evaluation (device programs, include/qtorch_hip.h qt_batch_*) -------------
+ * order: 0 = given sequences (ContractGivenSequence), 1 = greedy cost order,
+ *        2 = the reference's stochastic order (seeded), 3 = line-graph min-fill order. */
+enum { QTH_ORDER_GIVEN = 0, QTH_ORDER_GREEDY = 1, QTH_ORDER_STOCHASTIC = 2, QTH_ORDER_LINEGRAPH = 3 };
+
+/* F_p (src/maxcut.cpp:162-204) for many angle sets per launch: one program per edge
+ * light cone; instances = angle sets x edges; FP64 energy per angle set. */
+typedef struct qth_qaoa_batch qth_qaoa_batch;
+int qth_qaoa_batch_create(qt_ctx* ctx, const char* dgf, int p, int ref_map, int precision, int order,
+                          uint64_t seed, qth_qaoa_batch** out);
+/* bg: n_sets x 2p doubles, each set [betas..., gammas...] */
+int qth_qaoa_batch_set_angles(qth_qaoa_batch* b, int n_sets, const double* bg);
+/* energies: n_sets; zz: n_sets x n_edges values of <Z Z> (either may be NULL); synchronises */
+int qth_qaoa_batch_results(qth_qaoa_batch* b, double* energies, double* zz);
+qt_batch* qth_qaoa_batch_device(qth_qaoa_batch* b);
+int64_t qth_qaoa_batch_cmacs(qth_qaoa_batch* b); /* complex MACs of one energy evaluation */
+int qth_qaoa_batch_destroy(qth_qaoa_batch* b);
+
+/* Any list of networks (QASM + measurement text) in one batch.  given / given_len:
+ * concatenated node-id pairs per network for QTH_ORDER_GIVEN (else NULL). */
+typedef struct qth_netbatch qth_netbatch;
+int qth_netbatch_create(qt_ctx* ctx, int n, const char* const* qasm, const char* const* measure,
+                        const char* base_dir, int order, uint64_t seed, const int32_t* given,
+                        const int32_t* given_len, int precision, qth_netbatch** out);
+qt_batch* qth_netbatch_device(qth_netbatch* b);
+int qth_netbatch_results(qth_netbatch* b, double* values); /* 2 doubles per network; synchronises */
+int qth_netbatch_cmacs(qth_netbatch* b, int64_t* cmacs);   /* complex MACs per network */
+int qth_netbatch_destroy(qth_netbatch* b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -56,6 +56,23 @@ _HIP_SIGS = {
     "qt_contract_pair": (_c.c_int, [_vp, _vp, _c.c_int, _vp, _c.c_int, _c.c_int, _i32p, _i32p, _c.c_int,
                                     _i32p, _vp, _c.c_int, _i64p]),
     "qt_tensor_read_scalar": (_c.c_int, [_vp, _vp, _dp, _dp]),
+    "qt_program_create": (_c.c_int, [_vp, _c.c_int, _c.c_int, _i32p, _i64p, _c.c_int, _i32p, _i32p,
+                                     _c.POINTER(_vp)]),
+    "qt_program_stats": (_c.c_int, [_vp, _i64p, _i64p, _c.POINTER(_c.c_int)]),
+    "qt_program_destroy": (_c.c_int, [_vp]),
+    "qt_batch_create": (_c.c_int, [_vp, _c.c_int, _c.POINTER(_vp), _c.c_int, _c.POINTER(_vp)]),
+    "qt_batch_set_leaves": (_c.c_int, [_vp, _dp, _c.c_int64]),
+    "qt_batch_set_instances": (_c.c_int, [_vp, _c.c_int, _i32p, _i64p, _c.c_int, _i32p, _c.c_double,
+                                          _c.c_double]),
+    "qt_batch_config": (_c.c_int, [_vp, _c.c_int, _c.c_int]),
+    "qt_batch_autotune": (_c.c_int, [_vp, _c.POINTER(_c.c_int), _c.POINTER(_c.c_int), _c.POINTER(_c.c_float)]),
+    "qt_batch_launch": (_c.c_int, [_vp]),
+    "qt_batch_capture": (_c.c_int, [_vp]),
+    "qt_batch_results": (_c.c_int, [_vp, _dp, _dp]),
+    "qt_batch_time": (_c.c_int, [_vp, _c.c_int, _c.POINTER(_c.c_float)]),
+    "qt_batch_info": (_c.c_int, [_vp, _i64p, _c.POINTER(_c.c_int), _c.POINTER(_c.c_int), _c.POINTER(_c.c_int),
+                                 _c.POINTER(_c.c_int)]),
+    "qt_batch_destroy": (_c.c_int, [_vp]),
 }
 _HOST_SIGS = {
     "qth_last_error": (_c.c_char_p, []),
@@ -77,7 +94,23 @@ _HOST_SIGS = {
     "qth_qaoa_edge_text": (_c.c_int, [_c.c_char_p, _c.c_int, _c.c_int, _c.c_int, _dp, _c.c_char_p, _c.c_int64,
                                       _c.c_char_p, _c.c_int64]),
     "qth_qaoa_energy_plain": (_c.c_int, [_vp, _c.c_char_p, _c.c_int, _c.c_int, _dp, _c.c_uint64, _dp, _dp]),
+    "qth_qaoa_batch_create": (_c.c_int, [_vp, _c.c_char_p, _c.c_int, _c.c_int, _c.c_int, _c.c_int, _c.c_uint64,
+                                         _c.POINTER(_vp)]),
+    "qth_qaoa_batch_set_angles": (_c.c_int, [_vp, _c.c_int, _dp]),
+    "qth_qaoa_batch_results": (_c.c_int, [_vp, _dp, _dp]),
+    "qth_qaoa_batch_device": (_vp, [_vp]),
+    "qth_qaoa_batch_cmacs": (_c.c_int64, [_vp]),
+    "qth_qaoa_batch_destroy": (_c.c_int, [_vp]),
+    "qth_netbatch_create": (_c.c_int, [_vp, _c.c_int, _c.POINTER(_c.c_char_p), _c.POINTER(_c.c_char_p),
+                                       _c.c_char_p, _c.c_int, _c.c_uint64, _i32p, _i32p, _c.c_int,
+                                       _c.POINTER(_vp)]),
+    "qth_netbatch_device": (_vp, [_vp]),
+    "qth_netbatch_results": (_c.c_int, [_vp, _dp]),
+    "qth_netbatch_cmacs": (_c.c_int, [_vp, _i64p]),
+    "qth_netbatch_destroy": (_c.c_int, [_vp]),
 }
+
+ORDER_GIVEN, ORDER_GREEDY, ORDER_STOCHASTIC, ORDER_LINEGRAPH = 0, 1, 2, 3
 
 
 def _bind(lib, sigs):
@@ -356,3 +389,123 @@ def qaoa_energy_plain(ctx: Context, dgf: str, p: int, ref_map: bool, betas_gamma
     _host_check(host.qth_qaoa_energy_plain(ctx.handle, dgf.encode(), p, 1 if ref_map else 0, bg,
                                            ctypes.c_uint64(seed), zz.ctypes.data_as(_dp), ctypes.byref(energy)))
     return energy.value, zz
+
+
+class _DeviceBatch:
+    """Launch / capture / timing of a qt_batch owned by a host-side batch object."""
+
+    ctx: Context
+    dev: ctypes.c_void_p
+
+    def launch(self) -> None:
+        self.ctx.check(hip.qt_batch_launch(self.dev), "qt_batch_launch")
+
+    def capture(self) -> None:
+        self.ctx.check(hip.qt_batch_capture(self.dev), "qt_batch_capture")
+
+    def config(self, lanes: int, wgs_per_cu: int) -> None:
+        self.ctx.check(hip.qt_batch_config(self.dev, lanes, wgs_per_cu), "qt_batch_config")
+
+    def autotune(self):
+        l, w, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_float()
+        self.ctx.check(hip.qt_batch_autotune(self.dev, ctypes.byref(l), ctypes.byref(w), ctypes.byref(ms)),
+                       "qt_batch_autotune")
+        return l.value, w.value, ms.value
+
+    def time(self, reps: int) -> float:
+        ms = ctypes.c_float()
+        self.ctx.check(hip.qt_batch_time(self.dev, reps, ctypes.byref(ms)), "qt_batch_time")
+        return ms.value
+
+    def info(self):
+        cm = ctypes.c_int64()
+        l, w, g, se = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self.ctx.check(hip.qt_batch_info(self.dev, ctypes.byref(cm), ctypes.byref(l), ctypes.byref(w),
+                                         ctypes.byref(g), ctypes.byref(se)), "qt_batch_info")
+        return {"cmacs": cm.value, "lanes": l.value, "wgs_per_cu": w.value, "global_scratch": bool(g.value),
+                "scratch_elems": se.value}
+
+
+class QaoaBatch(_DeviceBatch):
+    """F_p for many angle sets per launch (qth_qaoa_batch_*)."""
+
+    def __init__(self, ctx: Context, dgf: str, p: int, ref_map: bool = True, precision: int = QT_C64_4M,
+                 order: int = ORDER_GREEDY, seed: int = 0):
+        h = _vp()
+        _host_check(host.qth_qaoa_batch_create(ctx.handle, dgf.encode(), p, 1 if ref_map else 0, precision, order,
+                                               ctypes.c_uint64(seed), ctypes.byref(h)))
+        self.handle = h
+        self.ctx = ctx
+        self.p = p
+        self.dev = host.qth_qaoa_batch_device(h)
+        self.n_edges, _ = qaoa_num_edges(dgf)
+        self.n_sets = 0
+
+    def set_angles(self, bg: np.ndarray) -> None:
+        bg = np.ascontiguousarray(bg, dtype=np.float64).reshape(-1, 2 * self.p)
+        _host_check(host.qth_qaoa_batch_set_angles(self.handle, bg.shape[0], bg.ctypes.data_as(_dp)))
+        self.n_sets = bg.shape[0]
+
+    def results(self):
+        e = np.zeros(self.n_sets)
+        zz = np.zeros(self.n_sets * self.n_edges)
+        _host_check(host.qth_qaoa_batch_results(self.handle, e.ctypes.data_as(_dp), zz.ctypes.data_as(_dp)))
+        return e, zz.reshape(self.n_sets, self.n_edges)
+
+    def cmacs_per_energy(self) -> int:
+        return int(host.qth_qaoa_batch_cmacs(self.handle))
+
+    def close(self):
+        if self.handle:
+            host.qth_qaoa_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class NetBatch(_DeviceBatch):
+    """Many networks (QASM + measurement) in one batch (qth_netbatch_*)."""
+
+    def __init__(self, ctx: Context, qasms: Sequence[str], measures: Sequence[Optional[str]], base_dir: str = "",
+                 order: int = ORDER_GREEDY, seed: int = 0, given: Optional[Sequence[Sequence[Tuple[int, int]]]] = None,
+                 precision: int = QT_C64_4M):
+        n = len(qasms)
+        q = (ctypes.c_char_p * n)(*[x.encode() for x in qasms])
+        m = (ctypes.c_char_p * n)(*[(x.encode() if x is not None else None) for x in measures])
+        g = gl = None
+        if order == ORDER_GIVEN:
+            flat = [v for seq in given for pr in seq for v in pr]
+            g = _i32(flat)
+            gl = _i32([len(seq) for seq in given])
+        h = _vp()
+        _host_check(host.qth_netbatch_create(ctx.handle, n, q, m, base_dir.encode(), order, ctypes.c_uint64(seed),
+                                             g, gl, precision, ctypes.byref(h)))
+        self.handle = h
+        self.ctx = ctx
+        self.n = n
+        self.dev = host.qth_netbatch_device(h)
+
+    def results(self) -> np.ndarray:
+        v = np.zeros(2 * self.n)
+        _host_check(host.qth_netbatch_results(self.handle, v.ctypes.data_as(_dp)))
+        return v[0::2] + 1j * v[1::2]
+
+    def cmacs(self) -> np.ndarray:
+        c = np.zeros(self.n, dtype=np.int64)
+        _host_check(host.qth_netbatch_cmacs(self.handle, c.ctypes.data_as(_i64p)))
+        return c
+
+    def close(self):
+        if self.handle:
+            host.qth_netbatch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

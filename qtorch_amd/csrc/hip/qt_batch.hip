@@ -1,0 +1,794 @@
+// Batched execution of many small contraction programs (libqtorch_hip.so).
+//
+// A program is the step list of one network (e.g. one QAOA edge light cone) in a
+// fixed order.  qt_program_create compiles it once on the host: every step's index
+// maps become two tables of packed 16-bit offsets (one entry per output element,
+// one per summed index), intermediates get slots in a per-executor scratch area by
+// interval allocation, and leaves become references into a leaf pool (a reference
+// may be relative to the instance's parameter block, so instances of one program
+// can carry different angles).
+//
+// qt_batch_launch runs a persistent kernel: each executor (one wavefront, or one
+// 256-thread workgroup for larger programs) takes instances from an atomic work
+// counter and contracts a whole network, step after step, with its intermediates in
+// LDS (or, when they do not fit, in a global scratch area), then writes the final
+// scalar.  A second kernel sums per-group results in FP64 (QAOA energies).  The
+// launch sequence can be captured into a HIP graph and replayed.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "qt_internal.h"
+
+namespace qt {
+
+struct ProgHdr {
+    int32_t step_begin, n_steps;
+    int32_t leaf_begin, n_leaves;
+    int32_t scratch_elems;
+    int32_t flags;
+};
+
+struct StepDesc {
+    int32_t a, b;       // >= 0: scratch element offset; < 0: leaf -(1 + index)
+    int32_t c;          // scratch element offset of the result (final step: unused)
+    int32_t tab;        // offset of the step's tables in the table array (uint32 units)
+    int32_t nc_bits;    // output elements = 1 << nc_bits
+    int32_t k_bits;     // summed elements = 1 << k_bits
+    int32_t flags;      // bit 0: final step
+    int32_t pad;
+};
+
+struct Instance {
+    int32_t prog;
+    int32_t pad;
+    int64_t param_base;
+};
+
+}  // namespace qt
+
+struct qt_program {
+    std::vector<qt::StepDesc> steps;  // offsets relative to this program's table block
+    std::vector<uint32_t> tables;
+    std::vector<int64_t> leaf_refs;
+    std::vector<int32_t> leaf_ranks;
+    int dim_log2 = 2;
+    int64_t cmacs = 0;
+    int32_t scratch_elems = 0;
+    int32_t max_tensor_elems = 0;
+};
+
+struct qt_batch {
+    qt::Ctx* ctx = nullptr;
+    int precision = QT_C64_4M;
+    int n_programs = 0;
+    int32_t scratch_elems = 0;  // max over programs
+    int64_t cmacs_total = 0;    // per-instance sum, for instances set
+    std::vector<int64_t> prog_cmacs;
+    // device arrays
+    qt::ProgHdr* d_progs = nullptr;
+    qt::StepDesc* d_steps = nullptr;
+    uint32_t* d_tabs = nullptr;
+    int64_t* d_leaf_refs = nullptr;
+    void* d_pool = nullptr;  // leaf pool in the batch precision
+    int64_t pool_elems = 0;
+    int64_t max_leaf_ref_end = 0;  // absolute part
+    int64_t max_param_ref_end = 0; // relative part
+    qt::Instance* d_inst = nullptr;
+    int n_inst = 0;
+    double* d_out = nullptr;       // 2 * n_inst
+    int32_t* d_counter = nullptr;
+    // group sums
+    int32_t* d_seg = nullptr;
+    int32_t* d_members = nullptr;
+    double* d_group = nullptr;
+    int n_groups = 0;
+    double c0 = 0.0, c1 = 1.0;
+    void* d_global_scratch = nullptr;
+    size_t global_scratch_bytes = 0;
+    // launch configuration
+    int lanes = 64;
+    int wgs_per_cu = 4;
+    int n_cus = 0;
+    bool use_global_scratch = false;
+    bool rescale = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace qt {
+namespace {
+
+constexpr int kWg = 256;
+
+template <class V>
+struct Real;
+template <>
+struct Real<double2> {
+    using T = double;
+};
+template <>
+struct Real<float2> {
+    using T = float;
+};
+
+__device__ __forceinline__ float bf16r(float x) {
+    uint32_t u = __float_as_uint(x);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return __uint_as_float(u & 0xffff0000u);
+}
+
+// complex multiply-accumulate in the batch's arithmetic form
+template <class V, int MODE>
+struct Acc {
+    using T = typename Real<V>::T;
+    T re = 0, im = 0, p3 = 0;
+    __device__ __forceinline__ void mac(V a, V b) {
+        if constexpr (MODE == QT_C64_3M) {
+            re = fmaf(a.x, b.x, re);
+            im = fmaf(a.y, b.y, im);
+            p3 = fmaf(a.x + a.y, b.x + b.y, p3);
+        } else if constexpr (MODE == QT_C64_BF16X3) {
+            const float arh = bf16r(a.x), arl = bf16r(a.x - arh), aih = bf16r(a.y), ail = bf16r(a.y - aih);
+            const float brh = bf16r(b.x), brl = bf16r(b.x - brh), bih = bf16r(b.y), bil = bf16r(b.y - bih);
+            re = fmaf(arh, brh, re); re = fmaf(arh, brl, re); re = fmaf(arl, brh, re);
+            re = fmaf(-aih, bih, re); re = fmaf(-aih, bil, re); re = fmaf(-ail, bih, re);
+            im = fmaf(arh, bih, im); im = fmaf(arh, bil, im); im = fmaf(arl, bih, im);
+            im = fmaf(aih, brh, im); im = fmaf(aih, brl, im); im = fmaf(ail, brh, im);
+        } else {
+            re = fma(a.x, b.x, re);
+            re = fma(-a.y, b.y, re);
+            im = fma(a.x, b.y, im);
+            im = fma(a.y, b.x, im);
+        }
+    }
+    __device__ __forceinline__ V get() const {
+        V r;
+        if constexpr (MODE == QT_C64_3M) {
+            r.x = re - im;
+            r.y = p3 - re - im;
+        } else {
+            r.x = re;
+            r.y = im;
+        }
+        return r;
+    }
+};
+
+template <int LANES, bool GLOBAL_SCRATCH>
+__device__ __forceinline__ void exec_sync() {
+    if constexpr (LANES == kWg) {
+        __syncthreads();
+    } else if constexpr (GLOBAL_SCRATCH) {
+        // intermediates in global memory: wait for this wave's stores before any
+        // lane reads another lane's results
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+    } else {
+        // one wavefront: LDS operations of a wave complete in order; keep the
+        // compiler from moving memory operations across the step boundary
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+}
+
+template <class V, int MODE, int LANES, class PA, class PB>
+__device__ __forceinline__ void run_step(PA A, PB B, V* C, const uint32_t* __restrict__ fc,
+                                         const uint32_t* __restrict__ gs, int nC, int K, int lane,
+                                         double2* out_final) {
+    for (int c = lane; c < nC; c += LANES) {
+        const uint32_t f = fc[c];
+        const int oa = int(f & 0xffffu), ob = int(f >> 16);
+        Acc<V, MODE> acc;
+        for (int s = 0; s < K; ++s) {
+            const uint32_t g = gs[s];
+            acc.mac(A[oa + int(g & 0xffffu)], B[ob + int(g >> 16)]);
+        }
+        const V v = acc.get();
+        if (out_final) *out_final = double2{double(v.x), double(v.y)};
+        else C[c] = v;
+    }
+}
+
+template <class V, int MODE, int LANES, bool GLOBAL_SCRATCH>
+__global__ __launch_bounds__(kWg) void k_batch(const ProgHdr* __restrict__ progs, const StepDesc* __restrict__ steps,
+                                               const uint32_t* __restrict__ tabs,
+                                               const int64_t* __restrict__ leaf_refs, const V* __restrict__ pool,
+                                               const Instance* __restrict__ inst, int n_inst,
+                                               int32_t* __restrict__ counter, double2* __restrict__ out,
+                                               V* __restrict__ gscratch, int scratch_elems) {
+    extern __shared__ unsigned char lds_raw[];
+    __shared__ int wg_next;
+    constexpr int kExec = kWg / LANES;
+    const int ex = threadIdx.x / LANES;
+    const int lane = threadIdx.x % LANES;
+    V* mem;
+    if constexpr (GLOBAL_SCRATCH)
+        mem = gscratch + (int64_t(blockIdx.x) * kExec + ex) * scratch_elems;
+    else
+        mem = reinterpret_cast<V*>(lds_raw) + ex * scratch_elems;
+    while (true) {
+        int i;
+        if constexpr (LANES == kWg) {
+            if (threadIdx.x == 0) wg_next = atomicAdd(counter, 1);
+            __syncthreads();
+            i = wg_next;
+            __syncthreads();
+        } else {
+            int t = 0;
+            if (lane == 0) t = atomicAdd(counter, 1);
+            i = __shfl(t, 0, 64);
+        }
+        i = __builtin_amdgcn_readfirstlane(i);
+        if (i >= n_inst) break;
+        const Instance in = inst[i];
+        const ProgHdr P = progs[in.prog];
+        const int64_t* refs = leaf_refs + P.leaf_begin;
+        for (int si = 0; si < P.n_steps; ++si) {
+            const StepDesc st = steps[P.step_begin + si];
+            const uint32_t* fc = tabs + st.tab;
+            const int nC = 1 << st.nc_bits, K = 1 << st.k_bits;
+            const uint32_t* gs = fc + nC;
+            double2* fin = (st.flags & 1) ? out + i : nullptr;
+            V* C = mem + st.c;
+            auto leaf = [&](int code) -> const V* {
+                const int64_t r = refs[-1 - code];
+                const int64_t off = (r & (int64_t(1) << 62)) ? in.param_base + (r & ~(int64_t(1) << 62)) : r;
+                return pool + off;
+            };
+            if (st.a < 0 && st.b < 0)
+                run_step<V, MODE, LANES>(leaf(st.a), leaf(st.b), C, fc, gs, nC, K, lane, fin);
+            else if (st.a < 0)
+                run_step<V, MODE, LANES>(leaf(st.a), (const V*)(mem + st.b), C, fc, gs, nC, K, lane, fin);
+            else if (st.b < 0)
+                run_step<V, MODE, LANES>((const V*)(mem + st.a), leaf(st.b), C, fc, gs, nC, K, lane, fin);
+            else
+                run_step<V, MODE, LANES>((const V*)(mem + st.a), (const V*)(mem + st.b), C, fc, gs, nC, K, lane,
+                                         fin);
+            exec_sync<LANES, GLOBAL_SCRATCH>();
+        }
+    }
+}
+
+__global__ void k_reset_counter(int32_t* c) { *c = 0; }
+
+// FP64 group sums: group g = sum over its members of c0 + c1 * Re(value).
+__global__ void k_group_sum(const double2* __restrict__ out, const int32_t* __restrict__ seg,
+                            const int32_t* __restrict__ members, double* __restrict__ sums, int n_groups, double c0,
+                            double c1) {
+    const int g = blockIdx.x;
+    if (g >= n_groups) return;
+    double acc = 0.0;
+    for (int j = seg[g] + int(threadIdx.x); j < seg[g + 1]; j += 64) acc += c0 + c1 * out[members[j]].x;
+    for (int d = 32; d >= 1; d >>= 1) acc += __shfl_down(acc, d, 64);
+    if (threadIdx.x == 0) sums[g] = acc;
+}
+
+__global__ void k_to_c64(const double2* __restrict__ in, float2* __restrict__ out, int64_t n) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = float2{float(in[i].x), float(in[i].y)};
+}
+
+template <class T>
+qt_status dev_upload(Ctx* c, T** dst, const T* src, size_t n) {
+    if (*dst) {
+        (void)hipFree(*dst);
+        *dst = nullptr;
+    }
+    if (n == 0) return QT_OK;
+    qt_status s = check_hip(c, hipMalloc(reinterpret_cast<void**>(dst), n * sizeof(T)), "batch alloc");
+    if (s != QT_OK) return s;
+    return check_hip(c, hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice), "batch upload");
+}
+
+template <class V, int MODE>
+qt_status launch_batch_t(qt_batch* b) {
+    Ctx* c = b->ctx;
+    const int kExec = kWg / b->lanes;
+    const unsigned grid = unsigned(b->n_cus * b->wgs_per_cu);
+    const size_t lds = b->use_global_scratch ? 0 : size_t(kExec) * size_t(b->scratch_elems) * sizeof(V);
+    V* gs = static_cast<V*>(b->d_global_scratch);
+    const V* pool = static_cast<const V*>(b->d_pool);
+    auto d_out = reinterpret_cast<double2*>(b->d_out);
+#define QT_LAUNCH(L, G)                                                                                         \
+    hipLaunchKernelGGL((k_batch<V, MODE, L, G>), dim3(grid), dim3(kWg), lds, c->stream, b->d_progs, b->d_steps, \
+                       b->d_tabs, b->d_leaf_refs, pool, b->d_inst, b->n_inst, b->d_counter, d_out, gs,         \
+                       b->scratch_elems)
+    if (b->use_global_scratch) {
+        if (b->lanes == 64) QT_LAUNCH(64, true);
+        else QT_LAUNCH(256, true);
+    } else {
+        if (b->lanes == 64) QT_LAUNCH(64, false);
+        else QT_LAUNCH(256, false);
+    }
+#undef QT_LAUNCH
+    return check_hip(c, hipGetLastError(), "k_batch");
+}
+
+qt_status launch_batch(qt_batch* b) {
+    Ctx* c = b->ctx;
+    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, c->stream, b->d_counter);
+    qt_status s = check_hip(c, hipGetLastError(), "k_reset_counter");
+    if (s != QT_OK) return s;
+    switch (b->precision) {
+        case QT_PLAIN_C128: s = launch_batch_t<double2, QT_PLAIN_C128>(b); break;
+        case QT_C64_3M: s = launch_batch_t<float2, QT_C64_3M>(b); break;
+        case QT_C64_BF16X3: s = launch_batch_t<float2, QT_C64_BF16X3>(b); break;
+        default: s = launch_batch_t<float2, QT_C64_4M>(b); break;
+    }
+    if (s != QT_OK || b->n_groups == 0) return s;
+    hipLaunchKernelGGL(k_group_sum, dim3(unsigned(b->n_groups)), dim3(64), 0, c->stream,
+                       reinterpret_cast<const double2*>(b->d_out), b->d_seg, b->d_members, b->d_group, b->n_groups,
+                       b->c0, b->c1);
+    return check_hip(c, hipGetLastError(), "k_group_sum");
+}
+
+size_t elem_size(int precision) { return precision == QT_PLAIN_C128 ? 16 : 8; }
+
+qt_status configure_scratch(qt_batch* b) {
+    Ctx* c = b->ctx;
+    const size_t es = elem_size(b->precision);
+    const int kExec = kWg / b->lanes;
+    const size_t lds_need = size_t(kExec) * size_t(b->scratch_elems) * es;
+    const size_t lds_cap = 160 * 1024;
+    b->use_global_scratch = lds_need * size_t(b->wgs_per_cu) > lds_cap || lds_need > 64 * 1024;
+    if (b->use_global_scratch) {
+        const size_t need = size_t(b->n_cus) * size_t(b->wgs_per_cu) * size_t(kExec) * size_t(b->scratch_elems) * es;
+        if (need > (size_t(4) << 30))
+            return set_error(c, QT_OOM, "batch: scratch of this executor shape exceeds 4 GiB");
+        if (need > b->global_scratch_bytes) {
+            if (b->d_global_scratch) (void)hipFree(b->d_global_scratch);
+            b->d_global_scratch = nullptr;
+            b->global_scratch_bytes = 0;
+            qt_status s = check_hip(c, hipMalloc(&b->d_global_scratch, need), "batch global scratch");
+            if (s != QT_OK) return s;
+            b->global_scratch_bytes = need;
+        }
+    }
+    if (b->graph_exec) {
+        (void)hipGraphExecDestroy(b->graph_exec);
+        b->graph_exec = nullptr;
+    }
+    if (b->graph) {
+        (void)hipGraphDestroy(b->graph);
+        b->graph = nullptr;
+    }
+    return QT_OK;
+}
+
+}  // namespace
+}  // namespace qt
+
+using qt::check_hip;
+using qt::set_error;
+
+extern "C" {
+
+qt_status qt_program_create(qt_ctx* cx, int dim_log2, int n_leaves, const int32_t* leaf_ranks,
+                            const int64_t* leaf_refs, int n_steps, const int32_t* step_desc,
+                            const int32_t* step_axes, qt_program** out) {
+    if (!out || n_leaves < 1 || n_steps < 1 || !leaf_ranks || !leaf_refs || !step_desc || !step_axes)
+        return QT_INVALID_INPUT;
+    *out = nullptr;
+    qt::Ctx* ctx = cx ? &cx->impl : nullptr;  // compilation touches no device memory
+    if (dim_log2 != 1 && dim_log2 != 2) return set_error(ctx, QT_INVALID_INPUT, "qt_program_create: bad dim");
+    const int dl = dim_log2;
+    auto p = std::make_unique<qt_program>();
+    p->dim_log2 = dl;
+    p->leaf_ranks.assign(leaf_ranks, leaf_ranks + n_leaves);
+    p->leaf_refs.assign(leaf_refs, leaf_refs + n_leaves);
+    for (int i = 0; i < n_leaves; ++i)
+        if (leaf_ranks[i] < 0 || leaf_ranks[i] * dl > 16 || leaf_refs[i] < 0)
+            return set_error(ctx, QT_INVALID_INPUT, "qt_program_create: leaf too large or bad reference");
+    // ranks of every operand id; uses of every intermediate (an id is consumed once)
+    std::vector<int> rank_of(size_t(n_leaves + n_steps), -1), last_use(size_t(n_steps), -1);
+    for (int i = 0; i < n_leaves; ++i) rank_of[size_t(i)] = leaf_ranks[i];
+    size_t axis_pos = 0;
+    std::vector<size_t> axis_begin(static_cast<size_t>(n_steps));
+    for (int s = 0; s < n_steps; ++s) {
+        const int32_t* d = step_desc + 6 * s;
+        const int a = d[0], b = d[1], ra = d[2], rb = d[3], k = d[4], rc = d[5];
+        const int lim = n_leaves + s;
+        if (a < 0 || b < 0 || a >= lim || b >= lim || a == b || rank_of[size_t(a)] != ra || rank_of[size_t(b)] != rb ||
+            k < 0 || k > ra || k > rb || rc != ra + rb - 2 * k || rc * dl > 16 || (rc + k) * dl > 30)
+            return set_error(ctx, QT_INVALID_INPUT, "qt_program_create: inconsistent step " + std::to_string(s));
+        for (int op : {a, b})
+            if (op >= n_leaves) {
+                const int j = op - n_leaves;
+                if (last_use[size_t(j)] >= 0)
+                    return set_error(ctx, QT_INVALID_INPUT, "qt_program_create: intermediate used twice");
+                last_use[size_t(j)] = s;
+            }
+        rank_of[size_t(n_leaves + s)] = rc;
+        axis_begin[size_t(s)] = axis_pos;
+        axis_pos += size_t(2 * k + rc);
+    }
+    for (int s = 0; s + 1 < n_steps; ++s)
+        if (last_use[size_t(s)] < 0)
+            return set_error(ctx, QT_INVALID_INPUT, "qt_program_create: an intermediate is never consumed");
+    if (rank_of[size_t(n_leaves + n_steps - 1)] != 0)
+        return set_error(ctx, QT_INVALID_INPUT, "qt_program_create: the last step must produce a scalar");
+
+    // scratch slots: first fit over live intervals (an intermediate lives from its
+    // creating step until the step that consumes it)
+    struct Block {
+        int32_t off, len;
+        int free_after;  // step index after which it can be reused
+    };
+    std::vector<Block> live;
+    std::vector<int32_t> slot(size_t(n_steps), -1);
+    int32_t peak = 0;
+    for (int s = 0; s + 1 < n_steps; ++s) {
+        const int32_t need = int32_t(1) << (rank_of[size_t(n_leaves + s)] * dl);
+        // blocks whose consumer ran before step s are free (the result of step s is
+        // written while its own operands are still read, so they stay reserved)
+        std::vector<std::pair<int32_t, int32_t>> busy;
+        for (const auto& bl : live)
+            if (bl.free_after >= s) busy.push_back({bl.off, bl.off + bl.len});
+        std::sort(busy.begin(), busy.end());
+        int32_t off = 0;
+        for (const auto& iv : busy) {
+            if (off + need <= iv.first) break;
+            off = std::max(off, iv.second);
+        }
+        slot[size_t(s)] = off;
+        live.push_back({off, need, last_use[size_t(s)]});
+        peak = std::max(peak, off + need);
+    }
+    p->scratch_elems = std::max(peak, 1);
+
+    // tables and step descriptors
+    for (int s = 0; s < n_steps; ++s) {
+        const int32_t* d = step_desc + 6 * s;
+        const int a = d[0], b = d[1], ra = d[2], rb = d[3], k = d[4], rc = d[5];
+        const int32_t* ax = step_axes + axis_begin[size_t(s)];
+        std::vector<int> used_a(size_t(ra), 0), used_b(size_t(rb), 0);
+        std::vector<int> c_src(size_t(rc * dl)), sa(size_t(k * dl)), sb(size_t(k * dl));
+        for (int i = 0; i < k; ++i) {
+            const int pa = ax[i], pb = ax[k + i];
+            if (pa < 0 || pa >= ra || pb < 0 || pb >= rb || used_a[size_t(pa)]++ || used_b[size_t(pb)]++)
+                return set_error(ctx, QT_INVALID_INPUT, "qt_program_create: bad pair");
+            const int j = k - 1 - i;  // the last pair varies fastest (src/Network.h:912-916)
+            for (int t = 0; t < dl; ++t) {
+                sa[size_t(j * dl + t)] = pa * dl + t;
+                sb[size_t(j * dl + t)] = pb * dl + t;
+            }
+        }
+        for (int i = 0; i < rc; ++i) {
+            const int32_t f = ax[2 * k + i];
+            const bool from_a = (f & QT_FROM_A) != 0;
+            const int axis = f & ~QT_FROM_A;
+            if (from_a ? (axis < 0 || axis >= ra || used_a[size_t(axis)]++)
+                       : (axis < 0 || axis >= rb || used_b[size_t(axis)]++))
+                return set_error(ctx, QT_INVALID_INPUT, "qt_program_create: bad free axis");
+            for (int t = 0; t < dl; ++t) c_src[size_t(i * dl + t)] = from_a ? axis * dl + t : 64 + axis * dl + t;
+        }
+        qt::StepDesc st{};
+        st.nc_bits = rc * dl;
+        st.k_bits = k * dl;
+        st.tab = int32_t(p->tables.size());
+        const int64_t nC = int64_t(1) << st.nc_bits, K = int64_t(1) << st.k_bits;
+        for (int64_t cc = 0; cc < nC; ++cc) {
+            uint32_t oa = 0, ob = 0;
+            for (int i = 0; i < st.nc_bits; ++i)
+                if ((cc >> i) & 1) {
+                    if (c_src[size_t(i)] < 64) oa += 1u << c_src[size_t(i)];
+                    else ob += 1u << (c_src[size_t(i)] - 64);
+                }
+            p->tables.push_back(oa | (ob << 16));
+        }
+        for (int64_t ss = 0; ss < K; ++ss) {
+            uint32_t ga = 0, gb = 0;
+            for (int j = 0; j < st.k_bits; ++j)
+                if ((ss >> j) & 1) {
+                    ga += 1u << sa[size_t(j)];
+                    gb += 1u << sb[size_t(j)];
+                }
+            p->tables.push_back(ga | (gb << 16));
+        }
+        st.a = a < n_leaves ? -(1 + a) : slot[size_t(a - n_leaves)];
+        st.b = b < n_leaves ? -(1 + b) : slot[size_t(b - n_leaves)];
+        st.c = s + 1 < n_steps ? slot[size_t(s)] : 0;
+        st.flags = s + 1 == n_steps ? 1 : 0;
+        p->steps.push_back(st);
+        p->cmacs += int64_t(1) << ((rc + k) * dl);
+        p->max_tensor_elems = std::max(p->max_tensor_elems, int32_t(1) << (rc * dl));
+    }
+    *out = p.release();
+    return QT_OK;
+}
+
+qt_status qt_program_stats(const qt_program* p, int64_t* cmacs, int64_t* scratch_elems, int* n_steps) {
+    if (!p) return QT_INVALID_INPUT;
+    if (cmacs) *cmacs = p->cmacs;
+    if (scratch_elems) *scratch_elems = p->scratch_elems;
+    if (n_steps) *n_steps = int(p->steps.size());
+    return QT_OK;
+}
+
+qt_status qt_program_destroy(qt_program* p) {
+    delete p;
+    return QT_OK;
+}
+
+qt_status qt_batch_create(qt_ctx* cx, int n_programs, qt_program* const* progs, int precision, qt_batch** out) {
+    if (!cx || !out || n_programs < 1 || !progs) return QT_INVALID_INPUT;
+    *out = nullptr;
+    auto* ctx = &cx->impl;
+    if (precision != QT_PLAIN_C128 && precision != QT_C64_4M && precision != QT_C64_3M &&
+        precision != QT_C64_BF16X3)
+        return set_error(ctx, QT_INVALID_INPUT, "qt_batch_create: bad precision");
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    auto b = std::make_unique<qt_batch>();
+    b->ctx = ctx;
+    b->precision = precision;
+    b->n_programs = n_programs;
+    std::vector<qt::ProgHdr> hdr;
+    std::vector<qt::StepDesc> steps;
+    std::vector<uint32_t> tabs;
+    std::vector<int64_t> refs;
+    for (int i = 0; i < n_programs; ++i) {
+        const qt_program* p = progs[i];
+        if (!p) return set_error(ctx, QT_INVALID_INPUT, "qt_batch_create: null program");
+        qt::ProgHdr h{};
+        h.step_begin = int32_t(steps.size());
+        h.n_steps = int32_t(p->steps.size());
+        h.leaf_begin = int32_t(refs.size());
+        h.n_leaves = int32_t(p->leaf_refs.size());
+        h.scratch_elems = p->scratch_elems;
+        const int32_t tab0 = int32_t(tabs.size());
+        for (auto st : p->steps) {
+            st.tab += tab0;
+            steps.push_back(st);
+        }
+        tabs.insert(tabs.end(), p->tables.begin(), p->tables.end());
+        for (size_t l = 0; l < p->leaf_refs.size(); ++l) {
+            const int64_t r = p->leaf_refs[l];
+            const int64_t end = (r & ~(int64_t(1) << 62)) + (int64_t(1) << (p->leaf_ranks[l] * p->dim_log2));
+            if (r & (int64_t(1) << 62)) b->max_param_ref_end = std::max(b->max_param_ref_end, end);
+            else b->max_leaf_ref_end = std::max(b->max_leaf_ref_end, end);
+            refs.push_back(r);
+        }
+        b->scratch_elems = std::max(b->scratch_elems, p->scratch_elems);
+        b->prog_cmacs.push_back(p->cmacs);
+        hdr.push_back(h);
+    }
+    b->scratch_elems = (b->scratch_elems + 15) / 16 * 16;
+    // default executor shape from the scratch footprint (the autotuner may refine it)
+    const size_t per_exec = size_t(b->scratch_elems) * qt::elem_size(precision);
+    if (per_exec * 4 <= 16 * 1024) {
+        b->lanes = 64;
+        b->wgs_per_cu = 4;
+    } else if (per_exec <= 32 * 1024) {
+        b->lanes = 256;
+        b->wgs_per_cu = 2;
+    } else {
+        b->lanes = 256;
+        b->wgs_per_cu = 1;
+    }
+    qt_status s = qt::dev_upload(ctx, &b->d_progs, hdr.data(), hdr.size());
+    if (s == QT_OK) s = qt::dev_upload(ctx, &b->d_steps, steps.data(), steps.size());
+    if (s == QT_OK) s = qt::dev_upload(ctx, &b->d_tabs, tabs.data(), tabs.size());
+    if (s == QT_OK) s = qt::dev_upload(ctx, &b->d_leaf_refs, refs.data(), refs.size());
+    if (s == QT_OK) s = check_hip(ctx, hipMalloc(reinterpret_cast<void**>(&b->d_counter), sizeof(int32_t)), "counter");
+    if (s == QT_OK) s = check_hip(ctx, hipDeviceGetAttribute(&b->n_cus, hipDeviceAttributeMultiprocessorCount,
+                                                              ctx->device), "CU count");
+    if (s == QT_OK) s = check_hip(ctx, hipEventCreate(&b->ev0), "event");
+    if (s == QT_OK) s = check_hip(ctx, hipEventCreate(&b->ev1), "event");
+    if (s == QT_OK) s = qt::configure_scratch(b.get());
+    if (s != QT_OK) {
+        qt_batch_destroy(b.release());
+        return s;
+    }
+    *out = b.release();
+    return QT_OK;
+}
+
+qt_status qt_batch_set_leaves(qt_batch* b, const double* pool, int64_t n_elems) {
+    if (!b || !pool || n_elems < 1) return QT_INVALID_INPUT;
+    auto* ctx = b->ctx;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    if (b->d_pool) (void)hipFree(b->d_pool);
+    b->d_pool = nullptr;
+    b->pool_elems = 0;
+    const size_t es = qt::elem_size(b->precision);
+    qt_status s = check_hip(ctx, hipMalloc(&b->d_pool, size_t(n_elems) * es), "leaf pool");
+    if (s != QT_OK) return s;
+    if (b->precision == QT_PLAIN_C128) {
+        s = check_hip(ctx, hipMemcpy(b->d_pool, pool, size_t(n_elems) * 16, hipMemcpyHostToDevice), "leaf pool copy");
+    } else {
+        void* tmp = nullptr;
+        s = check_hip(ctx, hipMalloc(&tmp, size_t(n_elems) * 16), "leaf pool staging");
+        if (s == QT_OK) s = check_hip(ctx, hipMemcpy(tmp, pool, size_t(n_elems) * 16, hipMemcpyHostToDevice), "copy");
+        if (s == QT_OK) {
+            hipLaunchKernelGGL(qt::k_to_c64, dim3(unsigned((n_elems + 255) / 256)), dim3(256), 0, ctx->stream,
+                               static_cast<const double2*>(tmp), static_cast<float2*>(b->d_pool), n_elems);
+            s = check_hip(ctx, hipGetLastError(), "k_to_c64");
+            if (s == QT_OK) s = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync");
+        }
+        if (tmp) (void)hipFree(tmp);
+    }
+    if (s == QT_OK) b->pool_elems = n_elems;
+    if (s == QT_OK && b->max_leaf_ref_end > n_elems)
+        return set_error(ctx, QT_INVALID_INPUT, "qt_batch_set_leaves: pool smaller than the leaf references");
+    return s;
+}
+
+qt_status qt_batch_set_instances(qt_batch* b, int n, const int32_t* prog, const int64_t* param_base, int n_groups,
+                                 const int32_t* group, double c0, double c1) {
+    if (!b || n < 0 || (n > 0 && (!prog || !param_base)) || n_groups < 0 || (n_groups > 0 && !group))
+        return QT_INVALID_INPUT;
+    auto* ctx = b->ctx;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    if (b->pool_elems == 0) return set_error(ctx, QT_INVALID_INPUT, "qt_batch_set_instances: set the leaves first");
+    std::vector<qt::Instance> inst(static_cast<size_t>(n));
+    b->cmacs_total = 0;
+    for (int i = 0; i < n; ++i) {
+        if (prog[i] < 0 || prog[i] >= b->n_programs || param_base[i] < 0 ||
+            param_base[i] + b->max_param_ref_end > b->pool_elems)
+            return set_error(ctx, QT_INVALID_INPUT, "qt_batch_set_instances: bad program or parameter block");
+        inst[size_t(i)] = qt::Instance{prog[i], 0, param_base[i]};
+        b->cmacs_total += b->prog_cmacs[size_t(prog[i])];
+    }
+    qt_status s = qt::dev_upload(ctx, &b->d_inst, inst.data(), inst.size());
+    if (s != QT_OK) return s;
+    if (b->d_out) (void)hipFree(b->d_out);
+    b->d_out = nullptr;
+    s = check_hip(ctx, hipMalloc(reinterpret_cast<void**>(&b->d_out), size_t(std::max(n, 1)) * 16), "batch out");
+    if (s != QT_OK) return s;
+    b->n_inst = n;
+    b->n_groups = n_groups;
+    b->c0 = c0;
+    b->c1 = c1;
+    if (n_groups > 0) {
+        std::vector<int32_t> seg(size_t(n_groups) + 1, 0), members(static_cast<size_t>(n));
+        for (int i = 0; i < n; ++i) {
+            if (group[i] < 0 || group[i] >= n_groups)
+                return set_error(ctx, QT_INVALID_INPUT, "qt_batch_set_instances: bad group");
+            ++seg[size_t(group[i]) + 1];
+        }
+        for (int k = 0; k < n_groups; ++k) seg[size_t(k) + 1] += seg[size_t(k)];
+        std::vector<int32_t> fill(seg.begin(), seg.end() - 1);
+        for (int i = 0; i < n; ++i) members[size_t(fill[size_t(group[i])]++)] = i;
+        s = qt::dev_upload(ctx, &b->d_seg, seg.data(), seg.size());
+        if (s == QT_OK) s = qt::dev_upload(ctx, &b->d_members, members.data(), members.size());
+        if (b->d_group) (void)hipFree(b->d_group);
+        b->d_group = nullptr;
+        if (s == QT_OK)
+            s = check_hip(ctx, hipMalloc(reinterpret_cast<void**>(&b->d_group), size_t(n_groups) * 8), "group sums");
+    }
+    if (s == QT_OK) s = qt::configure_scratch(b);  // drops a captured graph
+    return s;
+}
+
+qt_status qt_batch_config(qt_batch* b, int lanes, int wgs_per_cu) {
+    if (!b || (lanes != 64 && lanes != 256) || wgs_per_cu < 1 || wgs_per_cu > 16) return QT_INVALID_INPUT;
+    std::lock_guard<std::recursive_mutex> g(b->ctx->mu);
+    b->lanes = lanes;
+    b->wgs_per_cu = wgs_per_cu;
+    return qt::configure_scratch(b);
+}
+
+qt_status qt_batch_launch(qt_batch* b) {
+    if (!b) return QT_INVALID_INPUT;
+    auto* ctx = b->ctx;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    if (b->n_inst == 0) return QT_OK;
+    if (b->graph_exec) return check_hip(ctx, hipGraphLaunch(b->graph_exec, ctx->stream), "hipGraphLaunch");
+    return qt::launch_batch(b);
+}
+
+qt_status qt_batch_capture(qt_batch* b) {
+    if (!b) return QT_INVALID_INPUT;
+    auto* ctx = b->ctx;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    if (b->graph_exec) return QT_OK;
+    qt_status s = check_hip(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "begin capture");
+    if (s != QT_OK) return s;
+    qt_status ls = qt::launch_batch(b);
+    hipGraph_t graph = nullptr;
+    s = check_hip(ctx, hipStreamEndCapture(ctx->stream, &graph), "end capture");
+    if (ls != QT_OK) s = ls;
+    if (s != QT_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return s;
+    }
+    b->graph = graph;
+    return check_hip(ctx, hipGraphInstantiate(&b->graph_exec, graph, nullptr, nullptr, 0), "graph instantiate");
+}
+
+qt_status qt_batch_results(qt_batch* b, double* values, double* group_sums) {
+    if (!b) return QT_INVALID_INPUT;
+    auto* ctx = b->ctx;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    qt_status s = check_hip(ctx, hipStreamSynchronize(ctx->stream), "batch sync");
+    if (s == QT_OK && values && b->n_inst > 0)
+        s = check_hip(ctx, hipMemcpy(values, b->d_out, size_t(b->n_inst) * 16, hipMemcpyDeviceToHost), "results");
+    if (s == QT_OK && group_sums && b->n_groups > 0)
+        s = check_hip(ctx, hipMemcpy(group_sums, b->d_group, size_t(b->n_groups) * 8, hipMemcpyDeviceToHost),
+                      "group sums");
+    return s;
+}
+
+qt_status qt_batch_time(qt_batch* b, int reps, float* ms_per_launch) {
+    if (!b || reps < 1 || !ms_per_launch) return QT_INVALID_INPUT;
+    auto* ctx = b->ctx;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    qt_status s = check_hip(ctx, hipEventRecord(b->ev0, ctx->stream), "event record");
+    for (int r = 0; r < reps && s == QT_OK; ++r) s = qt_batch_launch(b);
+    if (s == QT_OK) s = check_hip(ctx, hipEventRecord(b->ev1, ctx->stream), "event record");
+    if (s == QT_OK) s = check_hip(ctx, hipEventSynchronize(b->ev1), "event sync");
+    float ms = 0;
+    if (s == QT_OK) s = check_hip(ctx, hipEventElapsedTime(&ms, b->ev0, b->ev1), "elapsed");
+    *ms_per_launch = ms / float(reps);
+    return s;
+}
+
+qt_status qt_batch_autotune(qt_batch* b, int* lanes, int* wgs_per_cu, float* ms) {
+    if (!b) return QT_INVALID_INPUT;
+    std::lock_guard<std::recursive_mutex> g(b->ctx->mu);
+    const int lane_opts[2] = {64, 256};
+    const int wg_opts[5] = {1, 2, 4, 8, 16};
+    int best_l = b->lanes, best_w = b->wgs_per_cu;
+    float best = 1e30f;
+    for (int l : lane_opts)
+        for (int w : wg_opts) {
+            if (qt_batch_config(b, l, w) != QT_OK) continue;
+            if (qt_batch_launch(b) != QT_OK) continue;  // warm-up
+            float t = 0;
+            if (qt_batch_time(b, 3, &t) != QT_OK) continue;
+            if (t < best) {
+                best = t;
+                best_l = l;
+                best_w = w;
+            }
+        }
+    qt_status s = qt_batch_config(b, best_l, best_w);
+    if (lanes) *lanes = best_l;
+    if (wgs_per_cu) *wgs_per_cu = best_w;
+    if (ms) *ms = best;
+    return s;
+}
+
+qt_status qt_batch_info(const qt_batch* b, int64_t* cmacs, int* lanes, int* wgs_per_cu, int* global_scratch,
+                        int* scratch_elems) {
+    if (!b) return QT_INVALID_INPUT;
+    if (cmacs) *cmacs = b->cmacs_total;
+    if (lanes) *lanes = b->lanes;
+    if (wgs_per_cu) *wgs_per_cu = b->wgs_per_cu;
+    if (global_scratch) *global_scratch = b->use_global_scratch ? 1 : 0;
+    if (scratch_elems) *scratch_elems = b->scratch_elems;
+    return QT_OK;
+}
+
+qt_status qt_batch_destroy(qt_batch* b) {
+    if (!b) return QT_INVALID_INPUT;
+    {
+        std::lock_guard<std::recursive_mutex> g(b->ctx->mu);
+        (void)hipSetDevice(b->ctx->device);
+        (void)hipStreamSynchronize(b->ctx->stream);
+        if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+        if (b->graph) (void)hipGraphDestroy(b->graph);
+        if (b->ev0) (void)hipEventDestroy(b->ev0);
+        if (b->ev1) (void)hipEventDestroy(b->ev1);
+        for (void* p : {(void*)b->d_progs, (void*)b->d_steps, (void*)b->d_tabs, (void*)b->d_leaf_refs, b->d_pool,
+                        (void*)b->d_inst, (void*)b->d_out, (void*)b->d_counter, (void*)b->d_seg, (void*)b->d_members,
+                        (void*)b->d_group, b->d_global_scratch})
+            if (p) (void)hipFree(p);
+    }
+    delete b;
+    return QT_OK;
+}
+
+}  // extern "C"

@@ -368,7 +368,8 @@ __global__ __launch_bounds__(kThreads) void k_gemm_c128(const double2* __restric
         for (int bm = 0; bm < 2; ++bm)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int64_t n = n0 + wn * 32 + bn * 16 + 4 * lk + r;
+                // f64 16x16x4 accumulator: item r of lane l is row (l / 16) + 4 r, column l % 16
+                const int64_t n = n0 + wn * 32 + bn * 16 + lk + 4 * r;
                 const int64_t mm = m0 + wm * 32 + bm * 16 + li;
                 C[mm + M * n] = double2{acc_re[bn][bm][r], acc_im[bn][bm][r]};
             }

@@ -1,0 +1,278 @@
+// Batched evaluation on the device: networks are ordered on the host (the
+// reference's orderings, or a greedy cost order), compiled into device programs
+// and run many at a time by the persistent batch kernel (include/qtorch_hip.h).
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <sstream>
+
+#include "batch.hpp"
+
+namespace qtorch {
+
+namespace {
+
+void CheckQt(qt_ctx* ctx, qt_status s, const char* what) {
+    if (s == QT_OK) return;
+    if (s == QT_INVALID_INPUT) throw DeviceFailure(std::string(what) + ": invalid input: " + qt_last_error(ctx));
+    throw DeviceFailure(std::string(what) + ": " + qt_last_error(ctx));
+}
+
+int SharedWires(const std::shared_ptr<Node>& a, const std::shared_ptr<Node>& b) {
+    int n = 0;
+    for (const auto& w : a->GetWires())
+        if (w->GetNodeA().lock() == b || w->GetNodeB().lock() == b) ++n;
+    return n;
+}
+
+}  // namespace
+
+// Greedy order: repeatedly contract the connected pair whose result grows the
+// stored size least (D^rC - D^rA - D^rB), then the one with fewer MACs; ties go to
+// the lower node ids.  Scalars of disconnected components are multiplied last.
+void GreedyContract(Network& net) {
+    while (!net.IsDone()) {
+        const auto un = net.GetUncontractedNodes();
+        std::shared_ptr<Node> best_a, best_b;
+        long double best_cost = 0, best_flops = 0;
+        for (const auto& a : un) {
+            for (const auto& w : a->GetWires()) {
+                if (w->IsContracted()) continue;
+                auto b = w->GetNodeA().lock() == a ? w->GetNodeB().lock() : w->GetNodeA().lock();
+                if (!b || b == a || b->mContracted) continue;
+                const int k = SharedWires(a, b);
+                const int rc = a->mRank + b->mRank - 2 * k;
+                const long double cost = std::ldexp(1.0L, 2 * rc) - std::ldexp(1.0L, 2 * a->mRank) -
+                                         std::ldexp(1.0L, 2 * b->mRank);
+                const long double flops = std::ldexp(1.0L, 2 * (rc + k));
+                const bool better = !best_a || cost < best_cost || (cost == best_cost && flops < best_flops) ||
+                                    (cost == best_cost && flops == best_flops &&
+                                     std::make_pair(a->mID, b->mID) < std::make_pair(best_a->mID, best_b->mID));
+                if (better) {
+                    best_a = a;
+                    best_b = b;
+                    best_cost = cost;
+                    best_flops = flops;
+                }
+            }
+        }
+        if (!best_a) {  // only scalars of separate components are left
+            std::vector<std::shared_ptr<Node>> sc;
+            for (const auto& n : un)
+                if (n->mRank == 0) sc.push_back(n);
+            if (sc.size() < 2) throw ContractionFailure();
+            best_a = sc[0];
+            best_b = sc[1];
+        }
+        net.ContractNodes(best_a, best_b, 1 << 20);
+    }
+}
+
+int64_t LeafPool::Add(const std::vector<std::complex<double>>& v) {
+    std::string key(reinterpret_cast<const char*>(v.data()), v.size() * sizeof(std::complex<double>));
+    auto it = index_.find(key);
+    if (it != index_.end()) return it->second;
+    const int64_t off = int64_t(data_.size());
+    data_.insert(data_.end(), v.begin(), v.end());
+    index_.emplace(std::move(key), off);
+    return off;
+}
+
+qt_program* CompileProgram(qt_ctx* ctx, const Network& net, const std::vector<int64_t>& leaf_refs) {
+    const int n_leaves = net.GetNumOriginalNodes();
+    const auto& all = net.GetAllNodes();
+    const auto& plan = net.GetPlan();
+    if (plan.empty() || int(leaf_refs.size()) != n_leaves) throw InvalidFunctionInput();
+    std::vector<int32_t> ranks(static_cast<size_t>(n_leaves));
+    for (int i = 0; i < n_leaves; ++i) ranks[size_t(i)] = all[size_t(i)]->mRank;
+    std::vector<int32_t> desc, axes;
+    for (const auto& st : plan) {
+        desc.insert(desc.end(), {st.a_id, st.b_id, st.rank_a, st.rank_b, st.k, st.rank_c});
+        axes.insert(axes.end(), st.pair_a.begin(), st.pair_a.end());
+        axes.insert(axes.end(), st.pair_b.begin(), st.pair_b.end());
+        axes.insert(axes.end(), st.free_src.begin(), st.free_src.end());
+    }
+    qt_program* p = nullptr;
+    CheckQt(ctx, qt_program_create(ctx, 2, n_leaves, ranks.data(), leaf_refs.data(), int(plan.size()), desc.data(),
+                                   axes.data(), &p),
+            "qt_program_create");
+    return p;
+}
+
+void OrderNetwork(Network& net, int order, uint64_t seed, const std::vector<std::pair<int, int>>* given) {
+    if (order == kOrderGreedy) {
+        GreedyContract(net);
+    } else if (order == kOrderStochastic) {
+        std::shared_ptr<Network> alias(&net, [](Network*) {});
+        ContractionTools ct(alias);
+        ct.SetSeed(seed);
+        ct.Contract(Stochastic);
+    } else if (order == kOrderGiven) {
+        if (!given) throw InvalidFunctionInput();
+        for (const auto& p : *given) {
+            const auto& all = net.GetAllNodes();
+            if (p.first < 0 || p.second < 0 || size_t(p.first) >= all.size() || size_t(p.second) >= all.size())
+                throw InvalidFunctionInput();
+            net.ContractNodes(all[size_t(p.first)], all[size_t(p.second)], 100);
+        }
+        if (!net.IsDone()) throw ContractionFailure();
+    } else if (order == kOrderLineGraph) {
+        std::shared_ptr<Network> alias(&net, [](Network*) {});
+        LineGraph lg(alias);
+        lg.ContractInOrder(lg.MinFillOrder());
+    } else {
+        throw InvalidContractionMethod();
+    }
+}
+
+// ---- QAOA batch --------------------------------------------------------------------
+
+namespace {
+
+std::vector<std::complex<double>> GateValues(const std::string& line) {
+    NetworkOptions opt;  // plan-only: leaves keep their host values
+    const std::string meas;
+    Network n("1\n" + line + "\n", &meas, opt);
+    return n.GetAllNodes()[1]->mHostVals;
+}
+
+std::string Fmt(double x) {
+    std::ostringstream os;
+    os << x;  // default ostream precision, as the reference writes angles (src/maxcut.h:211,217)
+    return os.str();
+}
+
+}  // namespace
+
+QaoaBatch::QaoaBatch(qt_ctx* ctx, const std::string& dgf, int p, bool ref_map, int precision, int order,
+                     uint64_t seed)
+    : ctx_(ctx), data_(ExtraData::FromText(p, dgf, ref_map)), p_(p) {
+    if (!ctx) throw DeviceFailure("QaoaBatch needs a device context");
+    std::vector<double> zero(size_t(2 * p), 0.5);
+    std::vector<qt_program*> progs;
+    for (size_t e = 0; e < data_.pairs.size(); ++e) {
+        const std::string q = QaoaEdgeQasm(data_, int(e), zero), m = QaoaEdgeMeasure(data_, int(e));
+        NetworkOptions opt;
+        Network net(q, &m, opt);
+        OrderNetwork(net, order, seed + e, nullptr);
+        // leaves: angle-dependent Rz / Rx gates read the instance's parameter block
+        std::vector<int64_t> refs;
+        int n_rz = 0, n_rx = 0;
+        const int n_iter = int(data_.realIterations[e].size()), nq = data_.qubitsNeeded[e];
+        for (int i = 0; i < net.GetNumOriginalNodes(); ++i) {
+            const auto& node = net.GetAllNodes()[size_t(i)];
+            if (node->GetTypeOfNode() == GateType::RZ) {
+                const int layer = n_iter ? n_rz++ / n_iter : 0;
+                refs.push_back(QT_LEAF_PARAM | int64_t(32 * layer));
+            } else if (node->GetTypeOfNode() == GateType::RX) {
+                const int layer = n_rx++ / nq;
+                refs.push_back(QT_LEAF_PARAM | int64_t(32 * layer + 16));
+            } else {
+                refs.push_back(pool_.Add(node->mHostVals));
+            }
+        }
+        edge_cmacs_.push_back(net.getNumFloatOps());
+        progs.push_back(CompileProgram(ctx, net, refs));
+    }
+    const_elems_ = int64_t(pool_.Data().size());
+    qt_status s = qt_batch_create(ctx, int(progs.size()), progs.data(), precision, &batch_);
+    for (auto* pr : progs) qt_program_destroy(pr);
+    CheckQt(ctx, s, "qt_batch_create");
+}
+
+QaoaBatch::~QaoaBatch() {
+    if (batch_) qt_batch_destroy(batch_);
+}
+
+void QaoaBatch::SetAngles(int n_sets, const double* bg) {
+    if (n_sets < 1 || !bg) throw InvalidFunctionInput();
+    n_sets_ = n_sets;
+    std::vector<std::complex<double>> pool = pool_.Data();
+    const int64_t block = 32 * int64_t(p_);
+    pool.resize(size_t(const_elems_ + block * n_sets));
+    for (int a = 0; a < n_sets; ++a) {
+        const double* ang = bg + size_t(2 * p_) * size_t(a);
+        for (int i = 0; i < p_; ++i) {
+            const double beta = ang[i], gamma = ang[i + p_];
+            const auto rz = GateValues("Rz " + Fmt(-gamma) + " 0");  // src/maxcut.h:211
+            const auto rx = GateValues("Rx " + Fmt(beta * 2.0) + " 0");  // src/maxcut.h:217
+            std::copy(rz.begin(), rz.end(), pool.begin() + long(const_elems_ + block * a + 32 * i));
+            std::copy(rx.begin(), rx.end(), pool.begin() + long(const_elems_ + block * a + 32 * i + 16));
+        }
+    }
+    CheckQt(ctx_, qt_batch_set_leaves(batch_, reinterpret_cast<const double*>(pool.data()), int64_t(pool.size())),
+            "qt_batch_set_leaves");
+    const int ne = int(data_.pairs.size());
+    std::vector<int32_t> prog, group;
+    std::vector<int64_t> base;
+    for (int a = 0; a < n_sets; ++a)
+        for (int e = 0; e < ne; ++e) {
+            prog.push_back(e);
+            base.push_back(const_elems_ + block * a);
+            group.push_back(a);
+        }
+    CheckQt(ctx_, qt_batch_set_instances(batch_, int(prog.size()), prog.data(), base.data(), n_sets, group.data(),
+                                         0.5, -0.5),
+            "qt_batch_set_instances");
+}
+
+void QaoaBatch::Results(double* energies, double* zz) {
+    std::vector<double> vals(size_t(2 * n_sets_) * data_.pairs.size());
+    CheckQt(ctx_, qt_batch_results(batch_, vals.data(), energies), "qt_batch_results");
+    if (zz)
+        for (size_t i = 0; i < vals.size() / 2; ++i) zz[i] = vals[2 * i];
+}
+
+int64_t QaoaBatch::CmacsPerEnergy() const {
+    int64_t s = 0;
+    for (auto c : edge_cmacs_) s += c;
+    return s;
+}
+
+// ---- generic batch of networks -----------------------------------------------------
+
+NetBatch::NetBatch(qt_ctx* ctx, const std::vector<std::string>& qasm, const std::vector<const std::string*>& meas,
+                   const std::string& base_dir, int order, uint64_t seed,
+                   const std::vector<std::vector<std::pair<int, int>>>* given, int precision)
+    : ctx_(ctx) {
+    if (!ctx || qasm.size() != meas.size() || qasm.empty()) throw InvalidFunctionInput();
+    std::vector<qt_program*> progs;
+    try {
+        for (size_t i = 0; i < qasm.size(); ++i) {
+            NetworkOptions opt;
+            opt.base_dir = base_dir;
+            Network net(qasm[i], meas[i], opt);
+            OrderNetwork(net, order, seed + i, given ? &(*given)[i] : nullptr);
+            std::vector<int64_t> refs;
+            for (int l = 0; l < net.GetNumOriginalNodes(); ++l) refs.push_back(pool_.Add(net.GetAllNodes()[size_t(l)]->mHostVals));
+            cmacs_.push_back(net.getNumFloatOps());
+            progs.push_back(CompileProgram(ctx, net, refs));
+        }
+    } catch (...) {
+        for (auto* p : progs) qt_program_destroy(p);
+        throw;
+    }
+    qt_status s = qt_batch_create(ctx, int(progs.size()), progs.data(), precision, &batch_);
+    for (auto* p : progs) qt_program_destroy(p);
+    CheckQt(ctx, s, "qt_batch_create");
+    CheckQt(ctx, qt_batch_set_leaves(batch_, reinterpret_cast<const double*>(pool_.Data().data()),
+                                     int64_t(pool_.Data().size())),
+            "qt_batch_set_leaves");
+    const int n = int(qasm.size());
+    std::vector<int32_t> prog(static_cast<size_t>(n)), group(static_cast<size_t>(n), 0);
+    std::vector<int64_t> base(size_t(n), 0);
+    for (int i = 0; i < n; ++i) prog[size_t(i)] = i;
+    CheckQt(ctx, qt_batch_set_instances(batch_, n, prog.data(), base.data(), 0, nullptr, 0.0, 1.0),
+            "qt_batch_set_instances");
+    n_ = n;
+}
+
+NetBatch::~NetBatch() {
+    if (batch_) qt_batch_destroy(batch_);
+}
+
+void NetBatch::Results(std::complex<double>* values) {
+    CheckQt(ctx_, qt_batch_results(batch_, reinterpret_cast<double*>(values), nullptr), "qt_batch_results");
+}
+
+}  // namespace qtorch

@@ -4,6 +4,7 @@
 #include <limits>
 #include <stdexcept>
 
+#include "batch.hpp"
 #include "qtorch.hpp"
 #include "qtorch_host.h"
 
@@ -12,6 +13,12 @@ using namespace qtorch;
 struct qth_network {
     std::shared_ptr<Network> net;
     std::unique_ptr<LineGraph> lg;
+};
+struct qth_qaoa_batch {
+    std::unique_ptr<QaoaBatch> b;
+};
+struct qth_netbatch {
+    std::unique_ptr<NetBatch> b;
 };
 
 namespace {
@@ -250,6 +257,88 @@ int qth_qaoa_energy_plain(qt_ctx* ctx, const char* dgf, int p, int ref_map, cons
         if (zz) std::memcpy(zz, v.data(), v.size() * sizeof(double));
         if (energy) *energy = f;
     });
+}
+
+int qth_qaoa_batch_create(qt_ctx* ctx, const char* dgf, int p, int ref_map, int precision, int order,
+                          uint64_t seed, qth_qaoa_batch** out) {
+    if (!ctx || !dgf || !out || p < 1) return QTH_INVALID_FUNCTION_INPUT;
+    *out = nullptr;
+    return Guard([&] {
+        auto h = std::make_unique<qth_qaoa_batch>();
+        h->b = std::make_unique<QaoaBatch>(ctx, dgf, p, ref_map != 0, precision, order, seed);
+        *out = h.release();
+    });
+}
+
+int qth_qaoa_batch_set_angles(qth_qaoa_batch* h, int n_sets, const double* bg) {
+    if (!h) return QTH_INVALID_FUNCTION_INPUT;
+    return Guard([&] { h->b->SetAngles(n_sets, bg); });
+}
+
+int qth_qaoa_batch_results(qth_qaoa_batch* h, double* energies, double* zz) {
+    if (!h) return QTH_INVALID_FUNCTION_INPUT;
+    return Guard([&] { h->b->Results(energies, zz); });
+}
+
+qt_batch* qth_qaoa_batch_device(qth_qaoa_batch* h) { return h ? h->b->Batch() : nullptr; }
+
+int64_t qth_qaoa_batch_cmacs(qth_qaoa_batch* h) { return h ? h->b->CmacsPerEnergy() : -1; }
+
+int qth_qaoa_batch_destroy(qth_qaoa_batch* h) {
+    delete h;
+    return QTH_OK;
+}
+
+int qth_netbatch_create(qt_ctx* ctx, int n, const char* const* qasm, const char* const* measure,
+                        const char* base_dir, int order, uint64_t seed, const int32_t* given,
+                        const int32_t* given_len, int precision, qth_netbatch** out) {
+    if (!ctx || n < 1 || !qasm || !out) return QTH_INVALID_FUNCTION_INPUT;
+    *out = nullptr;
+    return Guard([&] {
+        std::vector<std::string> q, mstore(static_cast<size_t>(n));
+        std::vector<const std::string*> m;
+        for (int i = 0; i < n; ++i) {
+            q.emplace_back(qasm[i]);
+            if (measure && measure[i]) {
+                mstore[size_t(i)] = measure[i];
+                m.push_back(&mstore[size_t(i)]);
+            } else {
+                m.push_back(nullptr);
+            }
+        }
+        std::vector<std::vector<std::pair<int, int>>> seqs;
+        if (order == QTH_ORDER_GIVEN) {
+            if (!given || !given_len) throw InvalidFunctionInput();
+            size_t pos = 0;
+            for (int i = 0; i < n; ++i) {
+                seqs.push_back(Pairs(given + pos, given_len[i]));
+                pos += size_t(2 * given_len[i]);
+            }
+        }
+        auto h = std::make_unique<qth_netbatch>();
+        h->b = std::make_unique<NetBatch>(ctx, q, m, base_dir ? base_dir : "", order, seed,
+                                          order == QTH_ORDER_GIVEN ? &seqs : nullptr, precision);
+        *out = h.release();
+    });
+}
+
+qt_batch* qth_netbatch_device(qth_netbatch* h) { return h ? h->b->Batch() : nullptr; }
+
+int qth_netbatch_results(qth_netbatch* h, double* values) {
+    if (!h || !values) return QTH_INVALID_FUNCTION_INPUT;
+    return Guard([&] { h->b->Results(reinterpret_cast<std::complex<double>*>(values)); });
+}
+
+int qth_netbatch_cmacs(qth_netbatch* h, int64_t* cmacs) {
+    if (!h || !cmacs) return QTH_INVALID_FUNCTION_INPUT;
+    const auto& c = h->b->Cmacs();
+    for (size_t i = 0; i < c.size(); ++i) cmacs[i] = c[i];
+    return QTH_OK;
+}
+
+int qth_netbatch_destroy(qth_netbatch* h) {
+    delete h;
+    return QTH_OK;
 }
 
 }  // extern "C"
