@@ -71,13 +71,16 @@ int64_t qt_mem_reserved(const qt_ctx* ctx);
 /* Allocates a zero-filled tensor of D^rank elements, D = 1 << dim_log2 (Node ctor,
  * src/Node.h:112-113). */
 qt_status qt_tensor_alloc(qt_ctx* ctx, int rank, int dim_log2, int dtype, qt_tensor** out);
+/* Same without the zero fill: for contraction outputs, which are written in full. */
+qt_status qt_tensor_alloc_empty(qt_ctx* ctx, int rank, int dim_log2, int dtype, qt_tensor** out);
 /* Host -> device copy of interleaved (re, im) values; nbytes must equal the tensor size. */
 qt_status qt_tensor_upload(qt_ctx* ctx, qt_tensor* t, const void* host, int64_t nbytes);
 /* Device -> host copy (synchronises); replaces reading Node::GetTensorVals (src/Node.h:151). */
 qt_status qt_tensor_download(qt_ctx* ctx, const qt_tensor* t, void* host, int64_t nbytes);
 /* Creates n tensors at once from host values (leaf gate/state/measure tensors of a
- * network, src/Node.h:197-898): one pinned staging copy and one synchronisation
- * instead of one per tensor.  host[i] holds D^ranks[i] interleaved values. */
+ * network, src/Node.h:197-898): one device allocation and one asynchronous copy
+ * from pinned staging instead of one of each per tensor.  host[i] holds
+ * D^ranks[i] interleaved values and may be reused when the call returns. */
 qt_status qt_tensor_create_many(qt_ctx* ctx, int n, const int* ranks, int dim_log2, int dtype,
                                 const void* const* host, qt_tensor** out);
 /* Returns the storage to the context's cache (Node::ClearNodeData, src/Node.h:137). */

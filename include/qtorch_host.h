@@ -42,6 +42,11 @@ const char* qth_last_error(void);
  * measure may be NULL (every qubit traced); base_dir resolves def1/def2 paths. */
 int qth_network_create(qt_ctx* ctx, const char* qasm, const char* measure, const char* base_dir,
                        qth_network** out);
+/* Same, with the wire dimension: dim_log2 = 2 is qTorch's Liouville picture, 1 the
+ * amplitude picture (gates as unitaries on kets, the measurement string is the
+ * output bitstring x and the value is <x|U|0...0>). */
+int qth_network_create_ex(qt_ctx* ctx, const char* qasm, const char* measure, const char* base_dir,
+                          int dim_log2, qth_network** out);
 int qth_network_destroy(qth_network* net);
 /* Network::ReduceCircuit (src/Network.h:1015-1118) */
 int qth_network_reduce(qth_network* net);
@@ -59,6 +64,9 @@ int qth_contract_user(qth_network* net, const int32_t* pairs, int n, double* re,
 int qth_linegraph_cnf(qth_network* net, char* buf, int64_t cap, int64_t* len);
 int qth_linegraph_minfill(qth_network* net, int32_t* order, int cap, int* n);
 int qth_linegraph_contract(qth_network* net, const int32_t* order, int n, double* re, double* im);
+/* Contracts the network in one of the QTH_ORDER_* orders (below); on a plan-only
+ * network this only records the plan. */
+int qth_network_order(qth_network* net, int order, uint64_t seed);
 /* mCreatedFrom record (src/preprocess.h:41-46): 2 ints per contraction */
 int qth_record(qth_network* net, int32_t* pairs, int cap, int* n);
 /* Node::mNumFloatOps counter (src/Network.h:884-885): complex MACs */
@@ -99,7 +107,7 @@ int qth_qaoa_batch_destroy(qth_qaoa_batch* b);
  * concatenated node-id pairs per network for QTH_ORDER_GIVEN (else NULL). */
 typedef struct qth_netbatch qth_netbatch;
 int qth_netbatch_create(qt_ctx* ctx, int n, const char* const* qasm, const char* const* measure,
-                        const char* base_dir, int order, uint64_t seed, const int32_t* given,
+                        const char* base_dir, int dim_log2, int order, uint64_t seed, const int32_t* given,
                         const int32_t* given_len, int precision, qth_netbatch** out);
 qt_batch* qth_netbatch_device(qth_netbatch* b);
 int qth_netbatch_results(qth_netbatch* b, double* values); /* 2 doubles per network; synchronises */

@@ -32,7 +32,7 @@ _dp = _c.POINTER(_c.c_double)
 
 def _load(path: str) -> ctypes.CDLL:
     if not os.path.exists(path):
-        raise ImportError(f"{path} is not built: run `python -m qtorch_amd.build`")
+        raise ImportError(f"{path} is not built: run `python qtorch_amd/build.py`")
     return ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL)
 
 
@@ -46,6 +46,7 @@ _HIP_SIGS = {
     "qt_last_error": (_c.c_char_p, [_vp]),
     "qt_mem_reserved": (_c.c_int64, [_vp]),
     "qt_tensor_alloc": (_c.c_int, [_vp, _c.c_int, _c.c_int, _c.c_int, _c.POINTER(_vp)]),
+    "qt_tensor_alloc_empty": (_c.c_int, [_vp, _c.c_int, _c.c_int, _c.c_int, _c.POINTER(_vp)]),
     "qt_tensor_upload": (_c.c_int, [_vp, _vp, _vp, _c.c_int64]),
     "qt_tensor_download": (_c.c_int, [_vp, _vp, _vp, _c.c_int64]),
     "qt_tensor_create_many": (_c.c_int, [_vp, _c.c_int, _i32p, _c.c_int, _c.c_int, _c.POINTER(_vp),
@@ -77,6 +78,7 @@ _HIP_SIGS = {
 _HOST_SIGS = {
     "qth_last_error": (_c.c_char_p, []),
     "qth_network_create": (_c.c_int, [_vp, _c.c_char_p, _c.c_char_p, _c.c_char_p, _c.POINTER(_vp)]),
+    "qth_network_create_ex": (_c.c_int, [_vp, _c.c_char_p, _c.c_char_p, _c.c_char_p, _c.c_int, _c.POINTER(_vp)]),
     "qth_network_destroy": (_c.c_int, [_vp]),
     "qth_network_reduce": (_c.c_int, [_vp]),
     "qth_contract_nodes": (_c.c_int, [_vp, _c.c_int, _c.c_int, _c.c_int, _c.POINTER(_c.c_int)]),
@@ -86,6 +88,7 @@ _HOST_SIGS = {
     "qth_linegraph_cnf": (_c.c_int, [_vp, _c.c_char_p, _c.c_int64, _i64p]),
     "qth_linegraph_minfill": (_c.c_int, [_vp, _i32p, _c.c_int, _c.POINTER(_c.c_int)]),
     "qth_linegraph_contract": (_c.c_int, [_vp, _i32p, _c.c_int, _dp, _dp]),
+    "qth_network_order": (_c.c_int, [_vp, _c.c_int, _c.c_uint64]),
     "qth_record": (_c.c_int, [_vp, _i32p, _c.c_int, _c.POINTER(_c.c_int)]),
     "qth_flops": (_c.c_int64, [_vp]),
     "qth_counts": (_c.c_int, [_vp] + [_c.POINTER(_c.c_int)] * 4),
@@ -102,7 +105,7 @@ _HOST_SIGS = {
     "qth_qaoa_batch_cmacs": (_c.c_int64, [_vp]),
     "qth_qaoa_batch_destroy": (_c.c_int, [_vp]),
     "qth_netbatch_create": (_c.c_int, [_vp, _c.c_int, _c.POINTER(_c.c_char_p), _c.POINTER(_c.c_char_p),
-                                       _c.c_char_p, _c.c_int, _c.c_uint64, _i32p, _i32p, _c.c_int,
+                                       _c.c_char_p, _c.c_int, _c.c_int, _c.c_uint64, _i32p, _i32p, _c.c_int,
                                        _c.POINTER(_vp)]),
     "qth_netbatch_device": (_vp, [_vp]),
     "qth_netbatch_results": (_c.c_int, [_vp, _dp]),
@@ -113,9 +116,19 @@ _HOST_SIGS = {
 ORDER_GIVEN, ORDER_GREEDY, ORDER_STOCHASTIC, ORDER_LINEGRAPH = 0, 1, 2, 3
 
 
+def _missing(name: str):
+    def call(*_args, **_kw):
+        raise ImportError(f"{name} is not exported by the loaded library: rebuild with `python qtorch_amd/build.py`")
+    return call
+
+
 def _bind(lib, sigs):
     for name, (res, args) in sigs.items():
-        f = getattr(lib, name)
+        try:
+            f = getattr(lib, name)
+        except AttributeError:
+            setattr(lib, name, _missing(name))
+            continue
         f.restype = res
         f.argtypes = args
 
@@ -283,11 +296,12 @@ class Network:
     ``ctx=None`` builds a plan-only network (ordering logic and step records, no
     values)."""
 
-    def __init__(self, qasm: str, measure: Optional[str], ctx: Optional[Context] = None, base_dir: str = ""):
+    def __init__(self, qasm: str, measure: Optional[str], ctx: Optional[Context] = None, base_dir: str = "",
+                 dim_log2: int = 2):
         h = _vp()
-        _host_check(host.qth_network_create(ctx.handle if ctx else None, qasm.encode(),
-                                            measure.encode() if measure is not None else None,
-                                            base_dir.encode(), ctypes.byref(h)))
+        _host_check(host.qth_network_create_ex(ctx.handle if ctx else None, qasm.encode(),
+                                               measure.encode() if measure is not None else None,
+                                               base_dir.encode(), dim_log2, ctypes.byref(h)))
         self.handle = h
         self.ctx = ctx
 
@@ -325,6 +339,9 @@ class Network:
     def contract_user(self, seq: Sequence[Tuple[int, int]]) -> complex:
         flat = [x for p in seq for x in p]
         return self._val(host.qth_contract_user, _i32(flat), len(seq))
+
+    def order(self, order: int, seed: int = 0) -> None:
+        _host_check(host.qth_network_order(self.handle, order, ctypes.c_uint64(seed)))
 
     def linegraph_cnf(self) -> str:
         n = ctypes.c_int64(0)
@@ -472,7 +489,7 @@ class NetBatch(_DeviceBatch):
 
     def __init__(self, ctx: Context, qasms: Sequence[str], measures: Sequence[Optional[str]], base_dir: str = "",
                  order: int = ORDER_GREEDY, seed: int = 0, given: Optional[Sequence[Sequence[Tuple[int, int]]]] = None,
-                 precision: int = QT_C64_4M):
+                 precision: int = QT_C64_4M, dim_log2: int = 2):
         n = len(qasms)
         q = (ctypes.c_char_p * n)(*[x.encode() for x in qasms])
         m = (ctypes.c_char_p * n)(*[(x.encode() if x is not None else None) for x in measures])
@@ -482,7 +499,8 @@ class NetBatch(_DeviceBatch):
             g = _i32(flat)
             gl = _i32([len(seq) for seq in given])
         h = _vp()
-        _host_check(host.qth_netbatch_create(ctx.handle, n, q, m, base_dir.encode(), order, ctypes.c_uint64(seed),
+        _host_check(host.qth_netbatch_create(ctx.handle, n, q, m, base_dir.encode(), dim_log2, order,
+                                             ctypes.c_uint64(seed),
                                              g, gl, precision, ctypes.byref(h)))
         self.handle = h
         self.ctx = ctx

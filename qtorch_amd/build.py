@@ -4,7 +4,7 @@
   qtorch_amd/lib/libqtorch_host.so  qTorch's C++ host surface + its C ABI (include/qtorch_host.h)
   qtorch_amd/bin/qtorch             the measure-file CLI (src/main.cpp counterpart)
 
-Run ``python -m qtorch_amd.build`` (hipcc cross-compiles; no GPU needed).
+Run ``python qtorch_amd/build.py`` (hipcc cross-compiles; no GPU needed).
 """
 from __future__ import annotations
 

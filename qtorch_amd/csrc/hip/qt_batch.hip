@@ -594,9 +594,12 @@ qt_status qt_batch_set_leaves(qt_batch* b, const double* pool, int64_t n_elems) 
     auto* ctx = b->ctx;
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
     (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
     if (b->d_pool) (void)hipFree(b->d_pool);
     b->d_pool = nullptr;
     b->pool_elems = 0;
+    b->n_inst = 0;  // parameter bases were checked against the old pool: set instances again
+    (void)qt::configure_scratch(b);
     const size_t es = qt::elem_size(b->precision);
     qt_status s = check_hip(ctx, hipMalloc(&b->d_pool, size_t(n_elems) * es), "leaf pool");
     if (s != QT_OK) return s;

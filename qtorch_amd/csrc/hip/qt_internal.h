@@ -47,6 +47,9 @@ struct Ctx {
 
 struct Tensor {
     void* data = nullptr;
+    // tensors created together share one allocation; the last one freed returns it
+    void* block = nullptr;
+    int* block_refs = nullptr;
     int64_t numel = 0;
     int rank = 0;
     int dim_log2 = 2;

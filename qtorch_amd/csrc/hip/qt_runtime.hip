@@ -168,59 +168,68 @@ qt_status qt_tensor_create_many(qt_ctx* c, int n, const int* ranks, int dim_log2
     auto* ctx = &c->impl;
     if ((dim_log2 != 1 && dim_log2 != 2) || (dtype != QT_DTYPE_C128 && dtype != QT_DTYPE_C64))
         return set_error(ctx, QT_INVALID_INPUT, "qt_tensor_create_many: bad dim/dtype");
+    if (n == 0) return QT_OK;
+    // one device block (256-byte aligned slots) and one pinned staging copy for all tensors
+    std::vector<size_t> off(static_cast<size_t>(n));
     size_t total = 0;
     for (int i = 0; i < n; ++i) {
         if (ranks[i] < 0 || ranks[i] * dim_log2 > 30 || !host[i])
             return set_error(ctx, QT_INVALID_INPUT, "qt_tensor_create_many: bad rank or host pointer");
-        total += (size_t(1) << (ranks[i] * dim_log2)) * qt::elem_bytes(dtype);
+        off[size_t(i)] = total;
+        const size_t bytes = (size_t(1) << (ranks[i] * dim_log2)) * qt::elem_bytes(dtype);
+        total += (bytes + 255) / 256 * 256;
     }
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
     (void)hipSetDevice(ctx->device);
     if (ctx->staging_bytes < total) {
+        (void)hipStreamSynchronize(ctx->stream);  // the previous staging copy may still be in flight
         if (ctx->staging) (void)hipHostFree(ctx->staging);
         ctx->staging = nullptr;
         ctx->staging_bytes = 0;
         qt_status s = check_hip(ctx, hipHostMalloc(&ctx->staging, total), "staging");
         if (s != QT_OK) return s;
         ctx->staging_bytes = total;
+    } else {
+        qt_status s = check_hip(ctx, hipStreamSynchronize(ctx->stream), "staging reuse");
+        if (s != QT_OK) return s;
     }
-    for (int i = 0; i < n; ++i) out[i] = nullptr;
-    size_t off = 0;
-    qt_status s = QT_OK;
-    for (int i = 0; i < n && s == QT_OK; ++i) {
+    hipError_t e;
+    void* block = ctx->pool.alloc(total, &e);
+    if (!block) return check_hip(ctx, e, "qt_tensor_create_many");
+    int* refs = new (std::nothrow) int(n);
+    if (!refs) {
+        ctx->pool.release(block);
+        return QT_OOM;
+    }
+    for (int i = 0; i < n; ++i) {
         qt_tensor* t = new (std::nothrow) qt_tensor();
         if (!t) {
-            s = QT_OOM;
-            break;
+            for (int j = 0; j < i; ++j) delete out[j];
+            delete refs;
+            ctx->pool.release(block);
+            return QT_OOM;
         }
         t->impl.rank = ranks[i];
         t->impl.dim_log2 = dim_log2;
         t->impl.dtype = dtype;
         t->impl.numel = int64_t(1) << (ranks[i] * dim_log2);
         t->impl.bytes = size_t(t->impl.numel) * qt::elem_bytes(dtype);
-        hipError_t e;
-        t->impl.data = ctx->pool.alloc(t->impl.bytes, &e);
-        if (!t->impl.data) {
-            delete t;
-            s = check_hip(ctx, e, "qt_tensor_create_many");
-            break;
-        }
+        t->impl.data = static_cast<char*>(block) + off[size_t(i)];
+        t->impl.block = block;
+        t->impl.block_refs = refs;
+        std::memcpy(static_cast<char*>(ctx->staging) + off[size_t(i)], host[i], t->impl.bytes);
         out[i] = t;
-        char* stage = static_cast<char*>(ctx->staging) + off;
-        std::memcpy(stage, host[i], t->impl.bytes);
-        s = check_hip(ctx, hipMemcpyAsync(t->impl.data, stage, t->impl.bytes, hipMemcpyHostToDevice,
-                                          ctx->stream), "qt_tensor_create_many copy");
-        off += t->impl.bytes;
     }
-    qt_status ss = check_hip(ctx, hipStreamSynchronize(ctx->stream), "qt_tensor_create_many sync");
-    if (s == QT_OK) s = ss;
+    // asynchronous: the staging buffer is reused only after a synchronisation (above)
+    qt_status s = check_hip(ctx, hipMemcpyAsync(block, ctx->staging, total, hipMemcpyHostToDevice, ctx->stream),
+                            "qt_tensor_create_many copy");
     if (s != QT_OK) {
-        for (int i = 0; i < n; ++i)
-            if (out[i]) {
-                ctx->pool.release(out[i]->impl.data);
-                delete out[i];
-                out[i] = nullptr;
-            }
+        for (int i = 0; i < n; ++i) {
+            delete out[i];
+            out[i] = nullptr;
+        }
+        delete refs;
+        ctx->pool.release(block);
     }
     return s;
 }
@@ -275,8 +284,42 @@ qt_status qt_tensor_read_scalar(qt_ctx* c, const qt_tensor* t, double* re, doubl
 qt_status qt_tensor_free(qt_ctx* c, qt_tensor* t) {
     if (!c || !t) return QT_INVALID_INPUT;
     std::lock_guard<std::recursive_mutex> g(c->impl.mu);
-    c->impl.pool.release(t->impl.data);
+    if (t->impl.block_refs) {
+        if (--*t->impl.block_refs == 0) {
+            c->impl.pool.release(t->impl.block);
+            delete t->impl.block_refs;
+        }
+    } else {
+        c->impl.pool.release(t->impl.data);
+    }
     delete t;
+    return QT_OK;
+}
+
+qt_status qt_tensor_alloc_empty(qt_ctx* c, int rank, int dim_log2, int dtype, qt_tensor** out) {
+    if (!c || !out) return QT_INVALID_INPUT;
+    *out = nullptr;
+    auto* ctx = &c->impl;
+    if (rank < 0 || (dim_log2 != 1 && dim_log2 != 2) || (dtype != QT_DTYPE_C128 && dtype != QT_DTYPE_C64))
+        return set_error(ctx, QT_INVALID_INPUT, "qt_tensor_alloc_empty: bad rank/dim/dtype");
+    if (rank * dim_log2 > 40)
+        return set_error(ctx, QT_OOM, "qt_tensor_alloc_empty: tensor larger than 2^40 elements");
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    qt_tensor* t = new (std::nothrow) qt_tensor();
+    if (!t) return QT_OOM;
+    t->impl.rank = rank;
+    t->impl.dim_log2 = dim_log2;
+    t->impl.dtype = dtype;
+    t->impl.numel = int64_t(1) << (rank * dim_log2);
+    t->impl.bytes = size_t(t->impl.numel) * qt::elem_bytes(dtype);
+    hipError_t e;
+    t->impl.data = ctx->pool.alloc(t->impl.bytes, &e);
+    if (!t->impl.data) {
+        delete t;
+        return check_hip(ctx, e, "qt_tensor_alloc_empty");
+    }
+    *out = t;
     return QT_OK;
 }
 

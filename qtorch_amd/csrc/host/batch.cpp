@@ -93,7 +93,7 @@ qt_program* CompileProgram(qt_ctx* ctx, const Network& net, const std::vector<in
         axes.insert(axes.end(), st.free_src.begin(), st.free_src.end());
     }
     qt_program* p = nullptr;
-    CheckQt(ctx, qt_program_create(ctx, 2, n_leaves, ranks.data(), leaf_refs.data(), int(plan.size()), desc.data(),
+    CheckQt(ctx, qt_program_create(ctx, net.GetDimLog2(), n_leaves, ranks.data(), leaf_refs.data(), int(plan.size()), desc.data(),
                                    axes.data(), &p),
             "qt_program_create");
     return p;
@@ -232,7 +232,7 @@ int64_t QaoaBatch::CmacsPerEnergy() const {
 // ---- generic batch of networks -----------------------------------------------------
 
 NetBatch::NetBatch(qt_ctx* ctx, const std::vector<std::string>& qasm, const std::vector<const std::string*>& meas,
-                   const std::string& base_dir, int order, uint64_t seed,
+                   const std::string& base_dir, int dim_log2, int order, uint64_t seed,
                    const std::vector<std::vector<std::pair<int, int>>>* given, int precision)
     : ctx_(ctx) {
     if (!ctx || qasm.size() != meas.size() || qasm.empty()) throw InvalidFunctionInput();
@@ -241,6 +241,7 @@ NetBatch::NetBatch(qt_ctx* ctx, const std::vector<std::string>& qasm, const std:
         for (size_t i = 0; i < qasm.size(); ++i) {
             NetworkOptions opt;
             opt.base_dir = base_dir;
+            opt.dim_log2 = dim_log2;
             Network net(qasm[i], meas[i], opt);
             OrderNetwork(net, order, seed + i, given ? &(*given)[i] : nullptr);
             std::vector<int64_t> refs;

@@ -60,7 +60,7 @@ private:
 class NetBatch {
 public:
     NetBatch(qt_ctx* ctx, const std::vector<std::string>& qasm, const std::vector<const std::string*>& meas,
-             const std::string& base_dir, int order, uint64_t seed,
+             const std::string& base_dir, int dim_log2, int order, uint64_t seed,
              const std::vector<std::vector<std::pair<int, int>>>* given, int precision);
     ~NetBatch();
     NetBatch(const NetBatch&) = delete;

@@ -104,6 +104,22 @@ int qth_network_create(qt_ctx* ctx, const char* qasm, const char* measure, const
     });
 }
 
+int qth_network_create_ex(qt_ctx* ctx, const char* qasm, const char* measure, const char* base_dir,
+                          int dim_log2, qth_network** out) {
+    if (!qasm || !out || (dim_log2 != 1 && dim_log2 != 2)) return QTH_INVALID_FUNCTION_INPUT;
+    *out = nullptr;
+    return Guard([&] {
+        NetworkOptions opt;
+        opt.ctx = ctx;
+        opt.base_dir = base_dir ? base_dir : "";
+        opt.dim_log2 = dim_log2;
+        std::string m = measure ? measure : "";
+        auto h = std::make_unique<qth_network>();
+        h->net = std::make_shared<Network>(std::string(qasm), measure ? &m : nullptr, opt);
+        *out = h.release();
+    });
+}
+
 int qth_network_destroy(qth_network* h) {
     delete h;
     return QTH_OK;
@@ -150,6 +166,11 @@ int qth_contract_user(qth_network* h, const int32_t* pairs, int n, double* re, d
         ct.ContractUserDefinedSequence(Pairs(pairs, n));
         Out(ct.GetFinalVal(), re, im);
     });
+}
+
+int qth_network_order(qth_network* h, int order, uint64_t seed) {
+    if (!h || order == QTH_ORDER_GIVEN) return QTH_INVALID_FUNCTION_INPUT;
+    return Guard([&] { OrderNetwork(*h->net, order, seed, nullptr); });
 }
 
 int qth_linegraph_cnf(qth_network* h, char* buf, int64_t cap, int64_t* len) {
@@ -290,9 +311,9 @@ int qth_qaoa_batch_destroy(qth_qaoa_batch* h) {
 }
 
 int qth_netbatch_create(qt_ctx* ctx, int n, const char* const* qasm, const char* const* measure,
-                        const char* base_dir, int order, uint64_t seed, const int32_t* given,
+                        const char* base_dir, int dim_log2, int order, uint64_t seed, const int32_t* given,
                         const int32_t* given_len, int precision, qth_netbatch** out) {
-    if (!ctx || n < 1 || !qasm || !out) return QTH_INVALID_FUNCTION_INPUT;
+    if (!ctx || n < 1 || !qasm || !out || (dim_log2 != 1 && dim_log2 != 2)) return QTH_INVALID_FUNCTION_INPUT;
     *out = nullptr;
     return Guard([&] {
         std::vector<std::string> q, mstore(static_cast<size_t>(n));
@@ -316,7 +337,7 @@ int qth_netbatch_create(qt_ctx* ctx, int n, const char* const* qasm, const char*
             }
         }
         auto h = std::make_unique<qth_netbatch>();
-        h->b = std::make_unique<NetBatch>(ctx, q, m, base_dir ? base_dir : "", order, seed,
+        h->b = std::make_unique<NetBatch>(ctx, q, m, base_dir ? base_dir : "", dim_log2, order, seed,
                                           order == QTH_ORDER_GIVEN ? &seqs : nullptr, precision);
         *out = h.release();
     });

@@ -1,6 +1,7 @@
 // Network, Node and gate tensors over the device engine.
 // Semantics follow qTorch's src/Network.h and src/Node.h (cited per function).
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <fstream>
 #include <iostream>
@@ -16,12 +17,79 @@ namespace {
 constexpr double kPi = 3.14159265358979323846;  // src/Node.h:45
 using cd = std::complex<double>;
 
-std::shared_ptr<Node> Leaf(int rank, GateType type, const std::string& name) {
+std::shared_ptr<Node> Leaf(int rank, GateType type, const std::string& name, int dim_log2 = 2) {
     auto n = std::make_shared<Node>(rank);
+    n->mDimLog2 = dim_log2;
     n->SetTypeOfNode(type);
     n->SetTypeOfNodeString(name);
-    n->mHostVals.assign(size_t(1) << (2 * rank), cd(0.0, 0.0));
+    n->mHostVals.assign(size_t(1) << (dim_log2 * rank), cd(0.0, 0.0));
     return n;
+}
+
+// ---- amplitude picture (dimension-2 wires) -------------------------------------------
+// The same gates as unitaries U acting on kets, in the conventions the superoperator
+// tables encode: Rx(t) = exp(-i t X / 2), Ry(t) = [[c, -s], [s, c]], Rz(t) = PHASE(t) =
+// diag(e^{-it/2}, e^{it/2}), CPHASE(t) = diag(1, 1, 1, e^{it}), CRk c t = CPHASE(2 pi /
+// 2^(c+1)); two-qubit matrices act on the basis 2 q1 + q2.  A one-qubit tensor over
+// wires [in, out] holds U[out][in] at in + 2 out; a two-qubit one over
+// [in1, in2, out1, out2] holds U[2 o1 + o2][2 i1 + i2] at i1 + 2 i2 + 4 o1 + 8 o2.
+std::shared_ptr<Node> AmpOne(GateType type, const std::string& name, const std::array<cd, 4>& u) {
+    auto n = Leaf(2, type, name, 1);
+    for (int i = 0; i < 2; ++i)
+        for (int o = 0; o < 2; ++o) n->mHostVals[size_t(i + 2 * o)] = u[size_t(2 * o + i)];
+    return n;
+}
+
+std::shared_ptr<Node> AmpTwo(GateType type, const std::string& name, const std::array<cd, 16>& u) {
+    auto n = Leaf(4, type, name, 1);
+    for (int i1 = 0; i1 < 2; ++i1)
+        for (int i2 = 0; i2 < 2; ++i2)
+            for (int o1 = 0; o1 < 2; ++o1)
+                for (int o2 = 0; o2 < 2; ++o2)
+                    n->mHostVals[size_t(i1 + 2 * i2 + 4 * o1 + 8 * o2)] = u[size_t(4 * (2 * o1 + o2) + 2 * i1 + i2)];
+    return n;
+}
+
+std::shared_ptr<Node> AmpRotation(const std::string& g, double t) {
+    const double c = std::cos(t / 2.0), s = std::sin(t / 2.0);
+    if (g == "Rx" || g == "RX") return AmpOne(GateType::RX, "Rx", {cd(c, 0), cd(0, -s), cd(0, -s), cd(c, 0)});
+    if (g == "Ry" || g == "RY") return AmpOne(GateType::RY, "Ry", {cd(c, 0), cd(-s, 0), cd(s, 0), cd(c, 0)});
+    const GateType ty = g == "PHASE" ? GateType::PHASE : GateType::RZ;
+    return AmpOne(ty, g == "PHASE" ? "Phase" : "Rz", {cd(c, -s), cd(0), cd(0), cd(c, s)});
+}
+
+std::shared_ptr<Node> AmpFixed(const std::string& g) {
+    const double h = 1.0 / std::sqrt(2.0);
+    if (g == "H") return AmpOne(GateType::HADAMARD, "H", {cd(h), cd(h), cd(h), cd(-h)});
+    if (g == "X") return AmpOne(GateType::X, "X", {cd(0), cd(1), cd(1), cd(0)});
+    if (g == "Y") return AmpOne(GateType::Y, "Y", {cd(0), cd(0, -1), cd(0, 1), cd(0)});
+    return AmpOne(GateType::Z, "Z", {cd(1), cd(0), cd(0), cd(-1)});
+}
+
+std::array<cd, 16> Diag4(cd d3) {
+    std::array<cd, 16> u{};
+    u[0] = u[5] = u[10] = 1.0;
+    u[15] = d3;
+    return u;
+}
+
+std::shared_ptr<Node> AmpTwoQubitGate(const std::string& g, double angle, int control) {
+    if (g == "CNOT") {
+        std::array<cd, 16> u{};
+        u[0] = u[5] = u[4 * 2 + 3] = u[4 * 3 + 2] = 1.0;
+        return AmpTwo(GateType::CNOT, "CNOT", u);
+    }
+    if (g == "SWAP") {
+        std::array<cd, 16> u{};
+        u[0] = u[4 * 1 + 2] = u[4 * 2 + 1] = u[15] = 1.0;
+        return AmpTwo(GateType::SWAP, "SWAP", u);
+    }
+    if (g == "CZ") return AmpTwo(GateType::CZ, "CZ", Diag4(cd(-1.0)));
+    if (g == "CRk") {
+        const double th = 2.0 * kPi / std::pow(2, control + 1.0);
+        return AmpTwo(GateType::CRK, "CRk", Diag4(cd(std::cos(th), std::sin(th))));
+    }
+    return AmpTwo(GateType::CPHASE, "CPhase", Diag4(cd(std::cos(angle), std::sin(angle))));
 }
 
 // ---- superoperator tables (src/Node.h:197-898) ----------------------------------
@@ -167,6 +235,16 @@ std::shared_ptr<Node> ArbitraryTwoQubit(const std::string& path, const std::stri
     return n;
 }
 
+// def1 / def2 gates in the amplitude picture: U itself (row-major in the file)
+std::shared_ptr<Node> AmpArbitrary(const std::string& path, const std::string& name, int qubits) {
+    const auto u = ReadComplexes(path, qubits == 1 ? 4 : 16);
+    if (qubits == 1)
+        return AmpOne(GateType::ARBITRARYONEQUBITUNITARY, name, {u[0], u[1], u[2], u[3]});
+    std::array<cd, 16> m;
+    for (size_t i = 0; i < 16; ++i) m[i] = u[i];
+    return AmpTwo(GateType::ARBITRARYTWOQUBITUNITARY, name, m);
+}
+
 // Network::ParseTokens (src/Network.h:976-992): '#' only at column 0; split on single
 // spaces; an empty final piece is dropped.
 std::vector<std::string> Tokens(const std::string& line) {
@@ -201,7 +279,7 @@ std::complex<double>& Node::Index(const std::vector<int>& digits) {
     size_t idx = 0, mult = 1;
     for (int d : digits) {
         idx += size_t(d) * mult;
-        mult *= 4;
+        mult <<= mDimLog2;
     }
     return mHostVals[idx];
 }
@@ -302,7 +380,13 @@ void Network::ParseNetwork(const std::string& qasm, const std::string* measure) 
     if (mNumberOfQubits < 0) throw InvalidFileFormat();
     mNodesByWire.assign(size_t(mNumberOfQubits), {});
     for (int i = 0; i < mNumberOfQubits; ++i) {
-        auto n = Vec4(GateType::INITSTATE, "|0><0|", 1.0, 0.0, 0.0, 0.0);
+        std::shared_ptr<Node> n;
+        if (mOpt.dim_log2 == 2) {
+            n = Vec4(GateType::INITSTATE, "|0><0|", 1.0, 0.0, 0.0, 0.0);
+        } else {
+            n = Leaf(1, GateType::INITSTATE, "|0>", 1);
+            n->mHostVals = {1.0, 0.0};
+        }
         if (mOpt.verbose) std::cout << "Creating qubit " << i << " in the initial state: |0><0|" << std::endl;
         auto w = std::make_shared<Wire>(n, nullptr, i);
         n->GetWires().push_back(w);
@@ -323,7 +407,16 @@ void Network::ParseNetwork(const std::string& qasm, const std::string* measure) 
         }
     }
     for (int i = 0; i < mNumberOfQubits; ++i) {
-        auto n = MeasureLeaf(meas[size_t(i)]);
+        std::shared_ptr<Node> n;
+        if (mOpt.dim_log2 == 2) {
+            n = MeasureLeaf(meas[size_t(i)]);
+        } else {
+            // amplitude picture: the measurement string is the output bitstring
+            const char ch = meas[size_t(i)];
+            if (ch != '0' && ch != '1') throw InvalidFileFormat();
+            n = Leaf(1, GateType::MEASURETRACE, ch == '0' ? "<0|" : "<1|", 1);
+            n->mHostVals = {ch == '0' ? 1.0 : 0.0, ch == '1' ? 1.0 : 0.0};
+        }
         mHanging[size_t(i)]->SetNodeB(n);
         n->GetWires().push_back(mHanging[size_t(i)]);
         mAllNodes.push_back(n);
@@ -380,20 +473,26 @@ void Network::ParseNode(const std::string& line) {
     };
     if (g == "Rx" || g == "RX" || g == "Ry" || g == "RY" || g == "Rz" || g == "RZ" || g == "PHASE") {
         const float angle = std::stof(arg(1));  // FP32 parse, src/Network.h:342,383,402,422
-        if (g == "Rx" || g == "RX") n = OneQubitTable(GateType::RX, "Rx", kRx, angle);
+        if (mOpt.dim_log2 == 1) n = AmpRotation(g, angle);
+        else if (g == "Rx" || g == "RX") n = OneQubitTable(GateType::RX, "Rx", kRx, angle);
         else if (g == "Ry" || g == "RY") n = OneQubitTable(GateType::RY, "Ry", kRy, angle);
         else if (g == "Rz" || g == "RZ") n = OneQubitTable(GateType::RZ, "Rz", kPhase, angle);
         else n = OneQubitTable(GateType::PHASE, "Phase", kPhase, angle);
         AttachOneQubit(n, std::stoi(arg(2)));
     } else if (g == "H" || g == "X" || g == "Y" || g == "Z") {
-        if (g == "H") n = OneQubitTable(GateType::HADAMARD, "H", kH, 0.0);
+        if (mOpt.dim_log2 == 1) n = AmpFixed(g);
+        else if (g == "H") n = OneQubitTable(GateType::HADAMARD, "H", kH, 0.0);
         else if (g == "X") n = OneQubitTable(GateType::X, "X", kX, 0.0);
         else if (g == "Y") n = OneQubitTable(GateType::Y, "Y", kY, 0.0);
         else n = OneQubitTable(GateType::Z, "Z", kZ, 0.0);
         AttachOneQubit(n, std::stoi(arg(1)));
     } else if (g == "CNOT" || g == "SWAP" || g == "CZ" || g == "CRk") {
         const int q1 = std::stoi(arg(1)), q2 = std::stoi(arg(2));
-        if (g == "CNOT") n = CnotLeaf();
+        if (mOpt.dim_log2 == 1) {
+            if (g == "CRk" && (q1 > mNumberOfQubits - 1 || q2 > mNumberOfQubits - 1 || q1 == q2))
+                throw InvalidFileFormat();
+            n = AmpTwoQubitGate(g, 0.0, q1);
+        } else if (g == "CNOT") n = CnotLeaf();
         else if (g == "SWAP") n = SwapLeaf();
         else if (g == "CZ") n = CzLeaf();
         else {
@@ -403,7 +502,7 @@ void Network::ParseNode(const std::string& line) {
         AttachTwoQubit(n, q1, q2);
     } else if (g == "CPHASE") {
         const double angle = std::stod(arg(1));  // src/Network.h:615
-        n = CPhaseLeaf(angle);
+        n = mOpt.dim_log2 == 1 ? AmpTwoQubitGate("CPHASE", angle, 0) : CPhaseLeaf(angle);
         AttachTwoQubit(n, std::stoi(arg(2)), std::stoi(arg(3)));
     } else if (g == "def1") {
         mOneQubitGates.insert({arg(1), arg(2)});
@@ -414,12 +513,12 @@ void Network::ParseNode(const std::string& line) {
     } else if (mOneQubitGates.count(g)) {
         std::string path = mOneQubitGates[g];
         if (!mOpt.base_dir.empty() && !path.empty() && path[0] != '/') path = mOpt.base_dir + "/" + path;
-        n = ArbitraryOneQubit(path, g);
+        n = mOpt.dim_log2 == 1 ? AmpArbitrary(path, g, 1) : ArbitraryOneQubit(path, g);
         AttachOneQubit(n, std::stoi(arg(1)));
     } else if (mTwoQubitGates.count(g)) {
         std::string path = mTwoQubitGates[g];
         if (!mOpt.base_dir.empty() && !path.empty() && path[0] != '/') path = mOpt.base_dir + "/" + path;
-        n = ArbitraryTwoQubit(path, g);
+        n = mOpt.dim_log2 == 1 ? AmpArbitrary(path, g, 2) : ArbitraryTwoQubit(path, g);
         AttachTwoQubit(n, std::stoi(arg(1)), std::stoi(arg(2)));
     } else {
         throw InvalidFileFormat();  // src/Network.h:694-702
@@ -440,7 +539,7 @@ void Network::UploadLeaves() {
     }
     if (nodes.empty()) return;
     std::vector<qt_tensor*> out(nodes.size(), nullptr);
-    Check(mCtx, qt_tensor_create_many(mCtx, int(nodes.size()), ranks.data(), 2, QT_DTYPE_C128, host.data(),
+    Check(mCtx, qt_tensor_create_many(mCtx, int(nodes.size()), ranks.data(), mOpt.dim_log2, QT_DTYPE_C128, host.data(),
                                       out.data()),
           "leaf upload");
     for (size_t i = 0; i < nodes.size(); ++i) {
@@ -454,7 +553,7 @@ void Network::EnsureOnDevice(const std::shared_ptr<Node>& n) {
     if (n->mDev || !mCtx) return;
     if (n->mHostVals.empty()) throw InvalidFunctionInput();
     qt_tensor* t = nullptr;
-    Check(mCtx, qt_tensor_alloc(mCtx, n->mRank, 2, QT_DTYPE_C128, &t), "alloc");
+    Check(mCtx, qt_tensor_alloc(mCtx, n->mRank, mOpt.dim_log2, QT_DTYPE_C128, &t), "alloc");
     n->mDev = t;
     n->mCtx = mCtx;
     Check(mCtx, qt_tensor_upload(mCtx, t, n->mHostVals.data(), int64_t(n->mHostVals.size() * sizeof(cd))),
@@ -549,7 +648,7 @@ std::shared_ptr<Node> Network::ContractNodes(std::shared_ptr<Node> A, std::share
 void Network::ContractIndices(const std::vector<std::pair<bool, int>>& free,
                               const std::vector<std::pair<int, int>>& sum, const std::shared_ptr<Node>& A,
                               const std::shared_ptr<Node>& B, const std::shared_ptr<Node>& C) {
-    mNumFloatOps += 1LL << (2 * (free.size() + sum.size()));
+    mNumFloatOps += 1LL << (mOpt.dim_log2 * (free.size() + sum.size()));
     if (!A->HasData() || !B->HasData()) throw InvalidFunctionInput();
     PlanStep st;
     st.a_id = A->mID;
@@ -569,7 +668,7 @@ void Network::ContractIndices(const std::vector<std::pair<bool, int>>& free,
         EnsureOnDevice(A);
         EnsureOnDevice(B);
         qt_tensor* t = nullptr;
-        Check(mCtx, qt_tensor_alloc(mCtx, C->mRank, 2, QT_DTYPE_C128, &t), "alloc C");
+        Check(mCtx, qt_tensor_alloc_empty(mCtx, C->mRank, mOpt.dim_log2, QT_DTYPE_C128, &t), "alloc C");
         C->mDev = t;
         C->mCtx = mCtx;
         Check(mCtx,

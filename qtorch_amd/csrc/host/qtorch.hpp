@@ -105,6 +105,7 @@ public:
     Node& operator=(const Node&) = delete;
 
     int mRank;
+    int mDimLog2 = 2;  // wire dimension 2^mDimLog2 (4: Liouville, 2: amplitudes)
     int mID = -1;
     int mIndexOfPreviousNode = -1;
     bool mContracted = false;
@@ -152,6 +153,9 @@ struct PlanStep {
 
 struct NetworkOptions {
     qt_ctx* ctx = nullptr;     // nullptr: plan-only (structure, no values)
+    // 2: qTorch's Liouville picture (dimension-4 wires, values Tr(M rho));
+    // 1: amplitude picture (dimension-2 wires, value <x|U|0...0> for the measured bitstring x)
+    int dim_log2 = 2;
     std::string base_dir;      // resolves relative def1/def2 gate-file paths ("" = cwd)
     bool verbose = false;      // the reference's std::cout progress messages
 };
@@ -189,6 +193,7 @@ public:
     qt_ctx* GetContext() const noexcept { return mCtx; }
     const std::vector<std::vector<std::shared_ptr<Node>>>& GetNodesByWire() const { return mNodesByWire; }
     int GetNumOriginalNodes() const noexcept { return mNumOriginal; }
+    int GetDimLog2() const noexcept { return mOpt.dim_log2; }
 
 private:
     void ParseNetwork(const std::string& qasm_text, const std::string* measure_text);
