@@ -103,6 +103,16 @@ qt_batch* qth_qaoa_batch_device(qth_qaoa_batch* b);
 int64_t qth_qaoa_batch_cmacs(qth_qaoa_batch* b); /* complex MACs of one energy evaluation */
 int qth_qaoa_batch_destroy(qth_qaoa_batch* b);
 
+/* The QAOA outer loop (src/maxcut.cpp:145-222): maximises F_p over 2p angles with
+ * n_random seeded random starts evaluated in one batch, then a batched Nelder-Mead
+ * simplex (the reference uses nlopt COBYLA).  angles receives 2p doubles. */
+int qth_qaoa_optimize(qt_ctx* ctx, const char* dgf, int p, int precision, int n_random, int max_evals,
+                      uint64_t seed, double* angles, double* energy, int* evaluations);
+/* maxcutGetFinalString (src/maxcut.cpp:29-140): greedy bit-by-bit cut from full-circuit
+ * projector probabilities; bits receives one int per vertex. */
+int qth_maxcut_final_string(qt_ctx* ctx, const char* dgf, int p, const double* betas_gammas, uint64_t seed,
+                            int* bits, int* cut_edges);
+
 /* Any list of networks (QASM + measurement text) in one batch.  given / given_len:
  * concatenated node-id pairs per network for QTH_ORDER_GIVEN (else NULL). */
 typedef struct qth_netbatch qth_netbatch;

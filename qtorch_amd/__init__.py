@@ -9,7 +9,7 @@ from ._native import (  # noqa: F401
     ORDER_GIVEN, ORDER_GREEDY, ORDER_LINEGRAPH, ORDER_STOCHASTIC, ContractionFailure, Context, NetBatch,
     QaoaBatch, DeviceFailure, EngineError, InvalidContractionMethod, InvalidFile,
     InvalidFileFormat, InvalidFunctionInput, InvalidUserContractionSequence, Network, QbbFailure, Tensor,
-    contract_pair, declared_symbols, qaoa_edge_text, qaoa_energy_plain, qaoa_num_edges,
+    contract_pair, declared_symbols, maxcut_final_string, qaoa_optimize, qaoa_edge_text, qaoa_energy_plain, qaoa_num_edges,
 )
 
 __all__ = [n for n in dir() if not n.startswith("_")]

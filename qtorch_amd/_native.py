@@ -107,6 +107,10 @@ _HOST_SIGS = {
     "qth_netbatch_create": (_c.c_int, [_vp, _c.c_int, _c.POINTER(_c.c_char_p), _c.POINTER(_c.c_char_p),
                                        _c.c_char_p, _c.c_int, _c.c_int, _c.c_uint64, _i32p, _i32p, _c.c_int,
                                        _c.POINTER(_vp)]),
+    "qth_qaoa_optimize": (_c.c_int, [_vp, _c.c_char_p, _c.c_int, _c.c_int, _c.c_int, _c.c_int, _c.c_uint64, _dp,
+                                     _dp, _c.POINTER(_c.c_int)]),
+    "qth_maxcut_final_string": (_c.c_int, [_vp, _c.c_char_p, _c.c_int, _dp, _c.c_uint64, _c.POINTER(_c.c_int),
+                                           _c.POINTER(_c.c_int)]),
     "qth_netbatch_device": (_vp, [_vp]),
     "qth_netbatch_results": (_c.c_int, [_vp, _dp]),
     "qth_netbatch_cmacs": (_c.c_int, [_vp, _i64p]),
@@ -527,3 +531,26 @@ class NetBatch(_DeviceBatch):
             self.close()
         except Exception:
             pass
+
+
+def qaoa_optimize(ctx: Context, dgf: str, p: int, precision: int = QT_C64_4M, n_random: int = 256,
+                  max_evals: int = 2000, seed: int = 0):
+    """Maximise F_p (src/maxcut.cpp:145-222) on the device; returns (angles, energy, evaluations)."""
+    ang = np.zeros(2 * p)
+    e = ctypes.c_double()
+    n = ctypes.c_int()
+    _host_check(host.qth_qaoa_optimize(ctx.handle, dgf.encode(), p, precision, n_random, max_evals,
+                                       ctypes.c_uint64(seed), ang.ctypes.data_as(_dp), ctypes.byref(e),
+                                       ctypes.byref(n)))
+    return ang, e.value, n.value
+
+
+def maxcut_final_string(ctx: Context, dgf: str, p: int, betas_gammas: Sequence[float], seed: int = 0):
+    """Greedy final cut (src/maxcut.cpp:29-140); returns (bits, cut edges)."""
+    _, nv = qaoa_num_edges(dgf)
+    bg = (ctypes.c_double * len(betas_gammas))(*betas_gammas)
+    bits = (ctypes.c_int * nv)()
+    cut = ctypes.c_int()
+    _host_check(host.qth_maxcut_final_string(ctx.handle, dgf.encode(), p, bg, ctypes.c_uint64(seed), bits,
+                                             ctypes.byref(cut)))
+    return list(bits), cut.value

@@ -5,6 +5,7 @@
 #include <stdexcept>
 
 #include "batch.hpp"
+#include "maxcut_driver.hpp"
 #include "qtorch.hpp"
 #include "qtorch_host.h"
 
@@ -308,6 +309,26 @@ int64_t qth_qaoa_batch_cmacs(qth_qaoa_batch* h) { return h ? h->b->CmacsPerEnerg
 int qth_qaoa_batch_destroy(qth_qaoa_batch* h) {
     delete h;
     return QTH_OK;
+}
+
+int qth_qaoa_optimize(qt_ctx* ctx, const char* dgf, int p, int precision, int n_random, int max_evals,
+                      uint64_t seed, double* angles, double* energy, int* evaluations) {
+    if (!ctx || !dgf || !angles) return QTH_INVALID_FUNCTION_INPUT;
+    return Guard([&] {
+        auto r = MaxcutOptimalAngles(ctx, dgf, p, precision, n_random, max_evals, seed);
+        std::memcpy(angles, r.angles.data(), r.angles.size() * sizeof(double));
+        if (energy) *energy = r.energy;
+        if (evaluations) *evaluations = r.evaluations;
+    });
+}
+
+int qth_maxcut_final_string(qt_ctx* ctx, const char* dgf, int p, const double* bg, uint64_t seed, int* bits,
+                            int* cut_edges) {
+    if (!ctx || !dgf || !bg || !bits || p < 1) return QTH_INVALID_FUNCTION_INPUT;
+    return Guard([&] {
+        const auto b = MaxcutFinalString(ctx, dgf, p, std::vector<double>(bg, bg + 2 * p), seed, cut_edges);
+        std::memcpy(bits, b.data(), b.size() * sizeof(int));
+    });
 }
 
 int qth_netbatch_create(qt_ctx* ctx, int n, const char* const* qasm, const char* const* measure,
