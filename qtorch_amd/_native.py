@@ -222,8 +222,9 @@ class Context:
         return int(hip.qt_mem_reserved(self.handle))
 
     def close(self) -> None:
-        if self.handle:
-            hip.qt_ctx_destroy(self.handle)
+        """Destroys the context unless tensors or batches are still alive on it (then
+        it stays until they are gone and the object itself is collected)."""
+        if self.handle and hip.qt_ctx_destroy(self.handle) == QT_OK:
             self.handle = None
 
     def __del__(self):

@@ -581,6 +581,7 @@ qt_status qt_batch_create(qt_ctx* cx, int n_programs, qt_program* const* progs, 
     if (s == QT_OK) s = check_hip(ctx, hipEventCreate(&b->ev0), "event");
     if (s == QT_OK) s = check_hip(ctx, hipEventCreate(&b->ev1), "event");
     if (s == QT_OK) s = qt::configure_scratch(b.get());
+    ++ctx->live;
     if (s != QT_OK) {
         qt_batch_destroy(b.release());
         return s;
@@ -789,6 +790,7 @@ qt_status qt_batch_destroy(qt_batch* b) {
                         (void*)b->d_inst, (void*)b->d_out, (void*)b->d_counter, (void*)b->d_seg, (void*)b->d_members,
                         (void*)b->d_group, b->d_global_scratch})
             if (p) (void)hipFree(p);
+        --b->ctx->live;
     }
     delete b;
     return QT_OK;

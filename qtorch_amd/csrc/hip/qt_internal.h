@@ -43,6 +43,8 @@ struct Ctx {
     // pinned host staging for batched uploads
     void* staging = nullptr;
     size_t staging_bytes = 0;
+    // tensors and batches alive on this context: it is not destroyed under them
+    int64_t live = 0;
 };
 
 struct Tensor {

@@ -106,6 +106,9 @@ qt_status qt_ctx_destroy(qt_ctx* c) {
     if (!c) return QT_INVALID_INPUT;
     {
         std::lock_guard<std::recursive_mutex> g(c->impl.mu);
+        if (c->impl.live > 0)
+            return set_error(&c->impl, QT_INVALID_INPUT,
+                             "qt_ctx_destroy: " + std::to_string(c->impl.live) + " tensors or batches still alive");
         (void)hipSetDevice(c->impl.device);
         (void)hipStreamSynchronize(c->impl.stream);
         c->impl.pool.trim();
@@ -158,6 +161,7 @@ qt_status qt_tensor_alloc(qt_ctx* c, int rank, int dim_log2, int dtype, qt_tenso
         delete t;
         return s;
     }
+    ++ctx->live;
     *out = t;
     return QT_OK;
 }
@@ -230,6 +234,8 @@ qt_status qt_tensor_create_many(qt_ctx* c, int n, const int* ranks, int dim_log2
         }
         delete refs;
         ctx->pool.release(block);
+    } else {
+        ctx->live += n;
     }
     return s;
 }
@@ -284,6 +290,7 @@ qt_status qt_tensor_read_scalar(qt_ctx* c, const qt_tensor* t, double* re, doubl
 qt_status qt_tensor_free(qt_ctx* c, qt_tensor* t) {
     if (!c || !t) return QT_INVALID_INPUT;
     std::lock_guard<std::recursive_mutex> g(c->impl.mu);
+    --c->impl.live;
     if (t->impl.block_refs) {
         if (--*t->impl.block_refs == 0) {
             c->impl.pool.release(t->impl.block);
@@ -319,6 +326,7 @@ qt_status qt_tensor_alloc_empty(qt_ctx* c, int rank, int dim_log2, int dtype, qt
         delete t;
         return check_hip(ctx, e, "qt_tensor_alloc_empty");
     }
+    ++ctx->live;
     *out = t;
     return QT_OK;
 }

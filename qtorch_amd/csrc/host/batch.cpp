@@ -69,6 +69,7 @@ void GreedyContract(Network& net) {
 }
 
 int64_t LeafPool::Add(const std::vector<std::complex<double>>& v) {
+    if (v.empty()) throw InvalidFunctionInput();  // a leaf without values (already contracted)
     std::string key(reinterpret_cast<const char*>(v.data()), v.size() * sizeof(std::complex<double>));
     auto it = index_.find(key);
     if (it != index_.end()) return it->second;
@@ -154,8 +155,8 @@ QaoaBatch::QaoaBatch(qt_ctx* ctx, const std::string& dgf, int p, bool ref_map, i
         const std::string q = QaoaEdgeQasm(data_, int(e), zero), m = QaoaEdgeMeasure(data_, int(e));
         NetworkOptions opt;
         Network net(q, &m, opt);
-        OrderNetwork(net, order, seed + e, nullptr);
-        // leaves: angle-dependent Rz / Rx gates read the instance's parameter block
+        // leaves (before ordering: contraction clears a node's data): angle-dependent
+        // Rz / Rx gates read the instance's parameter block, the rest the shared pool
         std::vector<int64_t> refs;
         int n_rz = 0, n_rx = 0;
         const int n_iter = int(data_.realIterations[e].size()), nq = data_.qubitsNeeded[e];
@@ -171,6 +172,7 @@ QaoaBatch::QaoaBatch(qt_ctx* ctx, const std::string& dgf, int p, bool ref_map, i
                 refs.push_back(pool_.Add(node->mHostVals));
             }
         }
+        OrderNetwork(net, order, seed + e, nullptr);
         edge_cmacs_.push_back(net.getNumFloatOps());
         progs.push_back(CompileProgram(ctx, net, refs));
     }
@@ -243,9 +245,9 @@ NetBatch::NetBatch(qt_ctx* ctx, const std::vector<std::string>& qasm, const std:
             opt.base_dir = base_dir;
             opt.dim_log2 = dim_log2;
             Network net(qasm[i], meas[i], opt);
-            OrderNetwork(net, order, seed + i, given ? &(*given)[i] : nullptr);
-            std::vector<int64_t> refs;
+            std::vector<int64_t> refs;  // before ordering: contraction clears a node's data
             for (int l = 0; l < net.GetNumOriginalNodes(); ++l) refs.push_back(pool_.Add(net.GetAllNodes()[size_t(l)]->mHostVals));
+            OrderNetwork(net, order, seed + i, given ? &(*given)[i] : nullptr);
             cmacs_.push_back(net.getNumFloatOps());
             progs.push_back(CompileProgram(ctx, net, refs));
         }
