@@ -94,10 +94,23 @@ def main():
     ap.add_argument("--form", default="c64_3m", choices=["c64_4m", "c64_3m", "c64_bf16x3", "c128"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-forms", action="store_true", help="skip the other forms' accuracy and speed")
+    ap.add_argument("--exec", default="", help="executor shape 'lanes,wgs_per_cu' instead of the autotuner's")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="first run plain device copies of this many MiB (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine measures one GPU; --gpus must be 1"}))
         sys.exit(2)
+    if args.copy_mib:
+        import torch
+        src = torch.ones(args.copy_mib << 20, dtype=torch.uint8, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": args.copy_mib << 20, "copies": 3}), flush=True)
+        del src, dst
     import numpy as np
 
     import qtorch_amd as qa
@@ -113,7 +126,11 @@ def main():
     # ---- headline: batched form ----
     b = qa.QaoaBatch(ctx, dgf, 1, ref_map=True, precision=forms[args.form], order=qa.ORDER_GREEDY)
     b.set_angles(sets)
-    lanes, wgs, _ = b.autotune()
+    if args.exec:
+        lanes, wgs = (int(x) for x in args.exec.split(","))
+        b.config(lanes, wgs)
+    else:
+        lanes, wgs, _ = b.autotune()
     b.capture()
     for _ in range(args.warmup):
         b.launch()
@@ -146,6 +163,28 @@ def main():
     energy_err = float(np.max(np.abs(energies - e64) / np.abs(e64)))
     ref_err = float(np.max(np.abs(zz[0] - zz_ref) / np.maximum(np.abs(zz_ref), 1e-3)))
     c128_vs_ref = float(np.max(np.abs(zz64[0] - zz_ref) / np.maximum(np.abs(zz_ref), 1e-15)))
+
+    # ---- every batched form on the same sets: error against complex128 and kernel time ----
+    by_form = {}
+    if not args.no_forms:
+        for name, prec in forms.items():
+            if name == args.form:
+                continue
+            if prec == qa.QT_PLAIN_C128:
+                bf, e_f, zz_f = b64, e64, zz64
+            else:
+                bf = qa.QaoaBatch(ctx, dgf, 1, ref_map=True, precision=prec, order=qa.ORDER_GREEDY)
+                bf.set_angles(sets)
+                bf.launch()
+                e_f, zz_f = bf.results()
+            l_f, w_f, _ = bf.autotune()
+            ms_f = bf.time(5)
+            by_form[name] = {
+                "edge_evals_per_s": ne * args.sets / (ms_f * 1e-3), "kernel_ms": ms_f, "executor": [l_f, w_f],
+                "max_edge_rel_err_vs_c128": float(np.max(np.abs(zz_f - zz64) / np.maximum(np.abs(zz64), 1e-3))),
+                "max_energy_rel_err_vs_c128": float(np.max(np.abs(e_f - e64) / np.abs(e64)))}
+            if bf is not b64:
+                bf.close()
     b64.close()
 
     # ---- drop-in complex128 path, one energy evaluation per step ----
@@ -177,6 +216,7 @@ def main():
         "accuracy": {"max_edge_rel_err_vs_c128": edge_err, "max_energy_rel_err_vs_c128": energy_err,
                      "max_edge_rel_err_vs_reference": ref_err, "c128_batched_vs_reference": c128_vs_ref,
                      "floor": "|d| / max(|x|, 1e-3)"},
+        "other_forms": by_form,
         "plain_c128_gpu": plain,
         "speedup_vs_plain_c128": (value / plain["value"]) if plain else None,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",

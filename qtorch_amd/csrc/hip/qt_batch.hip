@@ -89,8 +89,12 @@ struct qt_batch {
     void* d_global_scratch = nullptr;
     size_t global_scratch_bytes = 0;
     // launch configuration
-    int lanes = 64;
+    int lanes = 64;  // 256: a workgroup per instance; 64: a wavefront; < 64: lanes per instance in a wave
     int wgs_per_cu = 4;
+    int32_t slot_stride = 0;      // elements between the scratch areas of two executors
+    std::vector<int32_t> h_prog;  // program of each instance (host copy, for chunking)
+    int2* d_chunks = nullptr;
+    int n_chunks = 0;
     int n_cus = 0;
     bool use_global_scratch = false;
     bool rescale = false;
@@ -184,9 +188,24 @@ __device__ __forceinline__ void run_step(PA A, PB B, V* C, const uint32_t* __res
         const uint32_t f = fc[c];
         const int oa = int(f & 0xffffu), ob = int(f >> 16);
         Acc<V, MODE> acc;
-        for (int s = 0; s < K; ++s) {
-            const uint32_t g = gs[s];
-            acc.mac(A[oa + int(g & 0xffffu)], B[ob + int(g >> 16)]);
+        if ((K & 3) == 0) {
+            // summed extents are powers of D: four independent operand loads in flight
+            for (int s = 0; s < K; s += 4) {
+                const uint32_t g0 = gs[s], g1 = gs[s + 1], g2 = gs[s + 2], g3 = gs[s + 3];
+                const V a0 = A[oa + int(g0 & 0xffffu)], b0 = B[ob + int(g0 >> 16)];
+                const V a1 = A[oa + int(g1 & 0xffffu)], b1 = B[ob + int(g1 >> 16)];
+                const V a2 = A[oa + int(g2 & 0xffffu)], b2 = B[ob + int(g2 >> 16)];
+                const V a3 = A[oa + int(g3 & 0xffffu)], b3 = B[ob + int(g3 >> 16)];
+                acc.mac(a0, b0);
+                acc.mac(a1, b1);
+                acc.mac(a2, b2);
+                acc.mac(a3, b3);
+            }
+        } else {
+            for (int s = 0; s < K; ++s) {
+                const uint32_t g = gs[s];
+                acc.mac(A[oa + int(g & 0xffffu)], B[ob + int(g >> 16)]);
+            }
         }
         const V v = acc.get();
         if (out_final) *out_final = double2{double(v.x), double(v.y)};
@@ -254,6 +273,64 @@ __global__ __launch_bounds__(kWg) void k_batch(const ProgHdr* __restrict__ progs
     }
 }
 
+// Several instances of one program per wavefront: lane l serves instance slot l / G
+// with G lanes; the wave takes a chunk of up to 64 / G consecutive instances of the
+// same program (built on the host) and walks the program once for all of them.
+template <class V, int MODE, int G, bool GLOBAL_SCRATCH>
+__global__ __launch_bounds__(64) void k_batch_grouped(const ProgHdr* __restrict__ progs,
+                                                      const StepDesc* __restrict__ steps,
+                                                      const uint32_t* __restrict__ tabs,
+                                                      const int64_t* __restrict__ leaf_refs,
+                                                      const V* __restrict__ pool, const Instance* __restrict__ inst,
+                                                      const int2* __restrict__ chunks, int n_chunks,
+                                                      int32_t* __restrict__ counter, double2* __restrict__ out,
+                                                      V* __restrict__ gscratch, int slot_stride) {
+    extern __shared__ unsigned char lds_raw[];
+    constexpr int W = 64 / G;
+    const int lane = threadIdx.x, slot = lane / G, gl = lane % G;
+    V* mem;
+    if constexpr (GLOBAL_SCRATCH)
+        mem = gscratch + (int64_t(blockIdx.x) * W + slot) * slot_stride;
+    else
+        mem = reinterpret_cast<V*>(lds_raw) + slot * slot_stride;
+    while (true) {
+        int t = 0;
+        if (lane == 0) t = atomicAdd(counter, 1);
+        const int ci = __builtin_amdgcn_readfirstlane(__shfl(t, 0, 64));
+        if (ci >= n_chunks) break;
+        const int2 ch = chunks[ci];
+        const bool active = slot < ch.y;
+        const int i = ch.x + (active ? slot : 0);
+        const int64_t pbase = inst[i].param_base;
+        const ProgHdr P = progs[__builtin_amdgcn_readfirstlane(inst[ch.x].prog)];
+        const int64_t* refs = leaf_refs + P.leaf_begin;
+        for (int si = 0; si < P.n_steps; ++si) {
+            const StepDesc st = steps[P.step_begin + si];
+            const uint32_t* fc = tabs + st.tab;
+            const int nC = 1 << st.nc_bits, K = 1 << st.k_bits;
+            const uint32_t* gs = fc + nC;
+            double2* fin = (st.flags & 1) ? out + i : nullptr;
+            V* C = mem + st.c;
+            auto leaf = [&](int code) -> const V* {
+                const int64_t r = refs[-1 - code];
+                const int64_t off = (r & (int64_t(1) << 62)) ? pbase + (r & ~(int64_t(1) << 62)) : r;
+                return pool + off;
+            };
+            if (active) {
+                if (st.a < 0 && st.b < 0)
+                    run_step<V, MODE, G>(leaf(st.a), leaf(st.b), C, fc, gs, nC, K, gl, fin);
+                else if (st.a < 0)
+                    run_step<V, MODE, G>(leaf(st.a), (const V*)(mem + st.b), C, fc, gs, nC, K, gl, fin);
+                else if (st.b < 0)
+                    run_step<V, MODE, G>((const V*)(mem + st.a), leaf(st.b), C, fc, gs, nC, K, gl, fin);
+                else
+                    run_step<V, MODE, G>((const V*)(mem + st.a), (const V*)(mem + st.b), C, fc, gs, nC, K, gl, fin);
+            }
+            exec_sync<64, GLOBAL_SCRATCH>();
+        }
+    }
+}
+
 __global__ void k_reset_counter(int32_t* c) { *c = 0; }
 
 // FP64 group sums: group g = sum over its members of c0 + c1 * Re(value).
@@ -285,8 +362,43 @@ qt_status dev_upload(Ctx* c, T** dst, const T* src, size_t n) {
     return check_hip(c, hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice), "batch upload");
 }
 
+// executors per workgroup and workgroup size of an executor shape
+inline int execs_per_wg(int lanes) { return lanes >= 64 ? kWg / lanes : 64 / lanes; }
+inline int grid_wgs(const qt_batch* b) { return b->n_cus * b->wgs_per_cu * (b->lanes >= 64 ? 1 : 4); }
+
+template <class V, int MODE>
+qt_status launch_grouped_t(qt_batch* b) {
+    Ctx* c = b->ctx;
+    const unsigned grid = unsigned(grid_wgs(b));
+    const size_t lds =
+        b->use_global_scratch ? 0 : size_t(execs_per_wg(b->lanes)) * size_t(b->slot_stride) * sizeof(V);
+    V* gs = static_cast<V*>(b->d_global_scratch);
+    const V* pool = static_cast<const V*>(b->d_pool);
+    auto d_out = reinterpret_cast<double2*>(b->d_out);
+#define QT_LAUNCH_G(GL, GS)                                                                                        \
+    hipLaunchKernelGGL((k_batch_grouped<V, MODE, GL, GS>), dim3(grid), dim3(64), lds, c->stream, b->d_progs,       \
+                       b->d_steps, b->d_tabs, b->d_leaf_refs, pool, b->d_inst, b->d_chunks, b->n_chunks,           \
+                       b->d_counter, d_out, gs, b->slot_stride)
+#define QT_LAUNCH_GS(GS)                      \
+    switch (b->lanes) {                       \
+        case 4: QT_LAUNCH_G(4, GS); break;    \
+        case 8: QT_LAUNCH_G(8, GS); break;    \
+        case 16: QT_LAUNCH_G(16, GS); break;  \
+        default: QT_LAUNCH_G(32, GS); break;  \
+    }
+    if (b->use_global_scratch) {
+        QT_LAUNCH_GS(true);
+    } else {
+        QT_LAUNCH_GS(false);
+    }
+#undef QT_LAUNCH_GS
+#undef QT_LAUNCH_G
+    return check_hip(c, hipGetLastError(), "k_batch_grouped");
+}
+
 template <class V, int MODE>
 qt_status launch_batch_t(qt_batch* b) {
+    if (b->lanes < 64) return launch_grouped_t<V, MODE>(b);
     Ctx* c = b->ctx;
     const int kExec = kWg / b->lanes;
     const unsigned grid = unsigned(b->n_cus * b->wgs_per_cu);
@@ -332,12 +444,30 @@ size_t elem_size(int precision) { return precision == QT_PLAIN_C128 ? 16 : 8; }
 qt_status configure_scratch(qt_batch* b) {
     Ctx* c = b->ctx;
     const size_t es = elem_size(b->precision);
-    const int kExec = kWg / b->lanes;
-    const size_t lds_need = size_t(kExec) * size_t(b->scratch_elems) * es;
+    const int kExec = execs_per_wg(b->lanes);
+    // instance slots of a lane-grouped wave sit an odd number of elements apart, so
+    // the slots' same-offset accesses fall into different LDS banks
+    b->slot_stride = b->lanes < 64 ? b->scratch_elems + 1 : b->scratch_elems;
+    const size_t lds_need = size_t(kExec) * size_t(b->slot_stride) * es;
     const size_t lds_cap = 160 * 1024;
-    b->use_global_scratch = lds_need * size_t(b->wgs_per_cu) > lds_cap || lds_need > 64 * 1024;
+    const size_t wgs_on_cu = size_t(b->wgs_per_cu) * (b->lanes >= 64 ? 1 : 4);
+    b->use_global_scratch = lds_need * wgs_on_cu > lds_cap || lds_need > 64 * 1024;
+    // chunks of consecutive instances of one program, at most one per instance slot
+    if (b->lanes < 64) {
+        const int W = 64 / b->lanes;
+        std::vector<int2> ch;
+        for (int i = 0; i < b->n_inst;) {
+            int j = i + 1;
+            while (j < b->n_inst && j - i < W && b->h_prog[size_t(j)] == b->h_prog[size_t(i)]) ++j;
+            ch.push_back(int2{i, j - i});
+            i = j;
+        }
+        qt_status s = dev_upload(c, &b->d_chunks, ch.data(), ch.size());
+        if (s != QT_OK) return s;
+        b->n_chunks = int(ch.size());
+    }
     if (b->use_global_scratch) {
-        const size_t need = size_t(b->n_cus) * size_t(b->wgs_per_cu) * size_t(kExec) * size_t(b->scratch_elems) * es;
+        const size_t need = size_t(grid_wgs(b)) * size_t(kExec) * size_t(b->slot_stride) * es;
         if (need > (size_t(4) << 30))
             return set_error(c, QT_OOM, "batch: scratch of this executor shape exceeds 4 GiB");
         if (need > b->global_scratch_bytes) {
@@ -633,14 +763,17 @@ qt_status qt_batch_set_instances(qt_batch* b, int n, const int32_t* prog, const 
     (void)hipSetDevice(ctx->device);
     if (b->pool_elems == 0) return set_error(ctx, QT_INVALID_INPUT, "qt_batch_set_instances: set the leaves first");
     std::vector<qt::Instance> inst(static_cast<size_t>(n));
-    b->cmacs_total = 0;
+    int64_t cmacs = 0;
     for (int i = 0; i < n; ++i) {
         if (prog[i] < 0 || prog[i] >= b->n_programs || param_base[i] < 0 ||
             param_base[i] + b->max_param_ref_end > b->pool_elems)
             return set_error(ctx, QT_INVALID_INPUT, "qt_batch_set_instances: bad program or parameter block");
         inst[size_t(i)] = qt::Instance{prog[i], 0, param_base[i]};
-        b->cmacs_total += b->prog_cmacs[size_t(prog[i])];
+        cmacs += b->prog_cmacs[size_t(prog[i])];
     }
+    b->n_inst = 0;
+    b->cmacs_total = cmacs;
+    b->h_prog.assign(prog, prog + n);
     qt_status s = qt::dev_upload(ctx, &b->d_inst, inst.data(), inst.size());
     if (s != QT_OK) return s;
     if (b->d_out) (void)hipFree(b->d_out);
@@ -673,7 +806,9 @@ qt_status qt_batch_set_instances(qt_batch* b, int n, const int32_t* prog, const 
 }
 
 qt_status qt_batch_config(qt_batch* b, int lanes, int wgs_per_cu) {
-    if (!b || (lanes != 64 && lanes != 256) || wgs_per_cu < 1 || wgs_per_cu > 16) return QT_INVALID_INPUT;
+    if (!b || (lanes != 4 && lanes != 8 && lanes != 16 && lanes != 32 && lanes != 64 && lanes != 256) ||
+        wgs_per_cu < 1 || wgs_per_cu > 16)
+        return QT_INVALID_INPUT;
     std::lock_guard<std::recursive_mutex> g(b->ctx->mu);
     b->lanes = lanes;
     b->wgs_per_cu = wgs_per_cu;
@@ -742,7 +877,7 @@ qt_status qt_batch_time(qt_batch* b, int reps, float* ms_per_launch) {
 qt_status qt_batch_autotune(qt_batch* b, int* lanes, int* wgs_per_cu, float* ms) {
     if (!b) return QT_INVALID_INPUT;
     std::lock_guard<std::recursive_mutex> g(b->ctx->mu);
-    const int lane_opts[2] = {64, 256};
+    const int lane_opts[6] = {4, 8, 16, 32, 64, 256};
     const int wg_opts[5] = {1, 2, 4, 8, 16};
     int best_l = b->lanes, best_w = b->wgs_per_cu;
     float best = 1e30f;
@@ -788,7 +923,7 @@ qt_status qt_batch_destroy(qt_batch* b) {
         if (b->ev1) (void)hipEventDestroy(b->ev1);
         for (void* p : {(void*)b->d_progs, (void*)b->d_steps, (void*)b->d_tabs, (void*)b->d_leaf_refs, b->d_pool,
                         (void*)b->d_inst, (void*)b->d_out, (void*)b->d_counter, (void*)b->d_seg, (void*)b->d_members,
-                        (void*)b->d_group, b->d_global_scratch})
+                        (void*)b->d_group, b->d_global_scratch, (void*)b->d_chunks})
             if (p) (void)hipFree(p);
         --b->ctx->live;
     }

@@ -207,8 +207,10 @@ void QaoaBatch::SetAngles(int n_sets, const double* bg) {
     const int ne = int(data_.pairs.size());
     std::vector<int32_t> prog, group;
     std::vector<int64_t> base;
-    for (int a = 0; a < n_sets; ++a)
-        for (int e = 0; e < ne; ++e) {
+    // edge-major: consecutive instances share a program, so a wavefront can run
+    // several angle sets of one edge network side by side
+    for (int e = 0; e < ne; ++e)
+        for (int a = 0; a < n_sets; ++a) {
             prog.push_back(e);
             base.push_back(const_elems_ + block * a);
             group.push_back(a);
@@ -221,8 +223,10 @@ void QaoaBatch::SetAngles(int n_sets, const double* bg) {
 void QaoaBatch::Results(double* energies, double* zz) {
     std::vector<double> vals(size_t(2 * n_sets_) * data_.pairs.size());
     CheckQt(ctx_, qt_batch_results(batch_, vals.data(), energies), "qt_batch_results");
+    const size_t ne = data_.pairs.size(), ns = size_t(n_sets_);
     if (zz)
-        for (size_t i = 0; i < vals.size() / 2; ++i) zz[i] = vals[2 * i];
+        for (size_t e = 0; e < ne; ++e)
+            for (size_t a = 0; a < ns; ++a) zz[a * ne + e] = vals[2 * (e * ns + a)];
 }
 
 int64_t QaoaBatch::CmacsPerEnergy() const {

@@ -141,10 +141,32 @@ def test_autotune_keeps_results(ctx):
     b.launch()
     e0, _ = b.results()
     lanes, wgs, ms = b.autotune()
-    assert lanes in (64, 256) and wgs >= 1 and ms > 0
+    assert lanes in (4, 8, 16, 32, 64, 256) and wgs >= 1 and ms > 0
     b.launch()
     e1, _ = b.results()
     np.testing.assert_allclose(e1, e0, rtol=1e-6)
+
+
+@pytest.mark.parametrize("precision", [qa.QT_C64_3M, qa.QT_PLAIN_C128])
+def test_executor_shapes_agree_bitwise(ctx, precision):
+    """Every executor shape (4..32 lanes per instance inside a wave, a wave, a
+    workgroup) computes each output with one lane in the same summation order, so the
+    results are identical bit for bit; the odd set count leaves partial lane groups."""
+    rng = np.random.default_rng(5)
+    sets = np.stack([rng.uniform(0, math.pi / 2, 37), -2 * rng.uniform(0, math.pi, 37)], axis=1)
+    b = qa.QaoaBatch(ctx, _graph(), 1, ref_map=False, precision=precision)
+    b.set_angles(sets)
+    ref = None
+    for lanes in (64, 4, 8, 16, 32, 256):
+        for wgs in (1, 3):
+            b.config(lanes, wgs)
+            b.launch()
+            e, zz = b.results()
+            if ref is None:
+                ref = (e.copy(), zz.copy())
+            else:
+                assert np.array_equal(zz, ref[1]), (lanes, wgs)
+                assert np.array_equal(e, ref[0]), (lanes, wgs)
 
 
 def test_p2_light_cones_batched(ctx):
@@ -158,3 +180,8 @@ def test_p2_light_cones_batched(ctx):
     e, zz = b.results()
     e_plain, zz_plain = qa.qaoa_energy_plain(ctx, dgf, 2, False, bg, seed=1)
     np.testing.assert_allclose(zz[0], zz_plain, rtol=1e-11, atol=1e-13)
+    b.config(16, 2)  # lane groups with intermediates in global scratch
+    b.launch()
+    e16, zz16 = b.results()
+    assert b.info()["global_scratch"]
+    assert np.array_equal(zz16, zz) and np.array_equal(e16, e)
