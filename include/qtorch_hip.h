@@ -51,6 +51,12 @@ typedef enum {
                           MFMA with FP32 accumulation */
 } qt_precision;
 
+/* Or-ed into a complex64 precision of qt_contract_pair: after the step, C is scaled
+ * by a power of two so that its largest |re|, |im| lies in [0.5, 1), and the
+ * exponent is kept on the device beside C (value = data * 2^exponent, exponents of A
+ * and B included), so that long chains of small amplitudes cannot underflow FP32. */
+#define QT_RESCALE (1 << 8)
+
 /* Bit 30 of a free-axis code marks an axis of A (else of B). */
 #define QT_FROM_A (1 << 30)
 
@@ -96,7 +102,9 @@ int qt_tensor_rank(const qt_tensor* t);
  *                           digit i of C's flat index                      (src/Network.h:902-907)
  * C must be a distinct, already allocated tensor of rank_c axes with the same
  * dtype and dim as A and B.  `precision` must be QT_PLAIN_C128 for C128 tensors
- * and one of the C64 forms for C64 tensors.  On return *cmacs (if non-NULL) holds
+ * and one of the C64 forms (optionally | QT_RESCALE) for C64 tensors.  Large
+ * GEMM-shaped steps run on MFMA: FP64 for complex128, FP32 for 4M / 3M, bf16 for
+ * bf16x3.  On return *cmacs (if non-NULL) holds
  * D^(rank_c + k), the reference's complex multiply-add counter (src/Network.h:884-885).
  * The call is asynchronous on the context's stream. */
 qt_status qt_contract_pair(qt_ctx* ctx, const qt_tensor* A, int rank_a, const qt_tensor* B,
@@ -105,8 +113,15 @@ qt_status qt_contract_pair(qt_ctx* ctx, const qt_tensor* A, int rank_a, const qt
                            int64_t* cmacs);
 
 /* Reads the rank-0 result of a finished contraction (the mFinalVal read of
- * src/Network.h:961-969 and src/LineGraph.h:388); synchronises. */
+ * src/Network.h:961-969 and src/LineGraph.h:388), with a rescaled tensor's exponent
+ * applied in FP64; synchronises. */
 qt_status qt_tensor_read_scalar(qt_ctx* ctx, const qt_tensor* t, double* re, double* im);
+/* acc += t for rank-0 tensors, in FP64 on the stream (acc complex128; t's exponent
+ * applied): sums of sliced partial amplitudes stay on the device. */
+qt_status qt_tensor_accumulate(qt_ctx* ctx, const qt_tensor* t, qt_tensor* acc);
+/* Power-of-two exponent of a rescaled complex64 tensor (0 if it never was):
+ * its values are the downloaded data times 2^exponent.  Synchronises. */
+qt_status qt_tensor_exponent(qt_ctx* ctx, const qt_tensor* t, int32_t* exponent);
 
 /* ---- programs and batches (many networks per launch) ---------------------------------
  * A program is a network's whole sequence of ContractNodes steps (the calls a
@@ -131,23 +146,44 @@ qt_status qt_program_stats(const qt_program* p, int64_t* cmacs, int64_t* scratch
 qt_status qt_program_destroy(qt_program* p);
 
 typedef struct qt_batch qt_batch;
-/* A batch holds programs on the device in one arithmetic form (qt_precision). */
+/* A batch holds programs on the device in one arithmetic form (qt_precision);
+ * n_programs may be 0 (programs added later).  Constant leaf references of these
+ * programs are absolute pool offsets. */
 qt_status qt_batch_create(qt_ctx* ctx, int n_programs, qt_program* const* progs, int precision,
                           qt_batch** out);
+/* Appends programs (program ids first_id, first_id + 1, ...); the constant leaf
+ * references of program i are offsets from const_base[i] in the pool.  Waits for
+ * the batch's work in flight. */
+qt_status qt_batch_add_programs(qt_batch* b, int n, qt_program* const* progs, const int64_t* const_base,
+                                int32_t* first_id);
 /* Leaf pool: interleaved complex FP64 values (converted on the device for C64 forms). */
 qt_status qt_batch_set_leaves(qt_batch* b, const double* pool, int64_t n_elems);
+/* Writes n_elems values at pool offset `offset`, growing the pool (contents kept) as
+ * needed; waits for the batch's work in flight. */
+qt_status qt_batch_write_leaves(qt_batch* b, int64_t offset, const double* values, int64_t n_elems);
 /* Instances: program index and parameter base of each; optional FP64 group sums
  * sum_{i in g} (c0 + c1 * Re value_i) (the per-graph energy 0.5 * (1 - <ZZ>) of
- * src/maxcut.cpp:196 with c0 = 0.5, c1 = -0.5). */
+ * src/maxcut.cpp:196 with c0 = 0.5, c1 = -0.5); group -1 puts an instance in no group.
+ * Waits for the batch's work in flight. */
 qt_status qt_batch_set_instances(qt_batch* b, int n, const int32_t* prog, const int64_t* param_base,
                                  int n_groups, const int32_t* group, double c0, double c1);
-/* Executor shape: lanes per network (64 = one wavefront, 256 = one workgroup) and
- * workgroups per compute unit. */
+/* Executor shape: lanes per network (4, 8, 16 or 32 = that many lanes of a
+ * wavefront per network, several networks of one program side by side; 64 = one
+ * wavefront; 256 = one workgroup) and workgroups per compute unit (workgroups are
+ * 64 threads for fewer than 64 lanes, else 256).  Intermediates live in LDS when
+ * they fit, else in a device scratch area. */
 qt_status qt_batch_config(qt_batch* b, int lanes, int wgs_per_cu);
-/* Times every executor shape on the current instances and keeps the fastest. */
+enum { QT_SCRATCH_AUTO = 0, QT_SCRATCH_LDS = 1, QT_SCRATCH_GLOBAL = 2 };
+/* The same with an explicit place for the intermediates (QT_INVALID_INPUT, and the
+ * previous shape kept, when LDS is asked for and a workgroup's share exceeds 64 KiB). */
+qt_status qt_batch_config_ex(qt_batch* b, int lanes, int wgs_per_cu, int scratch);
+/* Times every executor shape and scratch place on the current instances and keeps
+ * the fastest. */
 qt_status qt_batch_autotune(qt_batch* b, int* lanes, int* wgs_per_cu, float* ms);
 /* Asynchronous run of every instance (replays the captured graph if there is one). */
 qt_status qt_batch_launch(qt_batch* b);
+/* *done = 1 once the last launch has finished (0 while it runs); never blocks. */
+qt_status qt_batch_poll(qt_batch* b, int* done);
 /* Captures the launch sequence (counter reset, persistent kernel, group sums) into a
  * HIP graph; later launches replay it. */
 qt_status qt_batch_capture(qt_batch* b);

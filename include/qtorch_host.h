@@ -74,6 +74,10 @@ int64_t qth_flops(qth_network* net);
 int qth_counts(qth_network* net, int* all_nodes, int* uncontracted, int* original, int* done);
 /* Recorded plan steps: 8 ints each (a_id, b_id, c_id, rank_a, rank_b, k, rank_c, 0) */
 int qth_plan(qth_network* net, int32_t* steps, int cap, int* n);
+/* Compiles the recorded plan into a device program (no device needed) and reports
+ * its complex-MAC count, the scratch elements one executor needs (steps scheduled
+ * to keep the fewest live) and the step count. */
+int qth_program_stats(qth_network* net, int64_t* cmacs, int64_t* scratch_elems, int* n_steps);
 
 /* QAOA MaxCut (src/maxcut.h:31-233): per-edge light-cone texts of F_p for a .dgf
  * graph.  ref_map != 0 keeps the reference's light-cone qubit map. */
@@ -123,6 +127,55 @@ qt_batch* qth_netbatch_device(qth_netbatch* b);
 int qth_netbatch_results(qth_netbatch* b, double* values); /* 2 doubles per network; synchronises */
 int qth_netbatch_cmacs(qth_netbatch* b, int64_t* cmacs);   /* complex MACs per network */
 int qth_netbatch_destroy(qth_netbatch* b);
+
+/* ---- serving: continuous batching of QAOA-energy and amplitude requests ----------
+ * Requests are submitted from any thread and finish asynchronously.  Each request's
+ * network is planned once per structure and then runs as a program instance in the
+ * next launch of an idle stream (or, when its tensors exceed a program's limit, on
+ * the drop-in complex128 path), admitted while the planned device bytes of the
+ * requests in flight fit hbm_budget.  Results do not depend on what else shares a
+ * launch. */
+typedef struct qth_server qth_server;
+enum { QTH_SERVE_QAOA_ENERGY = 0, QTH_SERVE_AMPLITUDE = 1 };
+enum { QTH_ROUTE_SMALL = 0, QTH_ROUTE_MEDIUM = 1, QTH_ROUTE_EAGER = 2 };
+int qth_server_create(int device, int slots, int precision, int64_t hbm_budget, int max_requests_per_launch,
+                      qth_server** out);
+int qth_server_destroy(qth_server* s); /* finishes the submitted requests first */
+/* QAOA energy F_p of a .dgf graph for one angle set [betas..., gammas...] (2p doubles). */
+int qth_server_submit_qaoa(qth_server* s, const char* dgf, int p, int ref_map, const double* betas_gammas,
+                           int64_t* id);
+/* <bits|C|0...0> of a QASM circuit (amplitude picture), bits = one '0'/'1' per qubit. */
+int qth_server_submit_amplitude(qth_server* s, const char* qasm, const char* bits, int64_t* id);
+/* Non-blocking: *done = 1 and the result once request `id` has finished (it is then
+ * forgotten).  status: the request's QTH_* code; value: energy (re) or amplitude. */
+int qth_server_poll(qth_server* s, int64_t id, int* done, int* status, double* re, double* im, double* latency_ms,
+                    int* route);
+/* Blocking variant; returns the request's QTH_* code. */
+int qth_server_wait(qth_server* s, int64_t id, double* re, double* im, double* latency_ms, int* route);
+int qth_server_drain(qth_server* s);
+/* counts: submitted, completed, rejected, launches, QAOA edge networks evaluated;
+ * ms: p50, p99, mean latency, time with two or more launches in flight, time with
+ * any launch in flight; peak: largest planned device bytes admitted at once. */
+int qth_server_stats(qth_server* s, int64_t* counts, double* ms, int64_t* peak);
+/* Route and planned device bytes of a request (planning only; no device needed).
+ * kind QTH_SERVE_QAOA_ENERGY: text = .dgf, p, ref_map; QTH_SERVE_AMPLITUDE: text =
+ * QASM, bits = the output bitstring. */
+int qth_serve_plan(int kind, const char* text, int p, int ref_map, const char* bits, int precision,
+                   int* route, int64_t* planned_bytes, int64_t* cmacs);
+
+/* ---- one large network, sliced --------------------------------------------------
+ * Planned once in `order` (QTH_ORDER_*), then wires are sliced until every tensor
+ * has at most max_bits index bits; the D^s slices run step by step (MFMA GEMMs for
+ * large steps) in the given form and their scalars are summed in FP64 on the device.
+ * dim_log2 = 1: amplitude picture (measure = output bitstring); 2: qTorch's picture. */
+typedef struct qth_sliced qth_sliced;
+int qth_sliced_create(const char* qasm, const char* measure, int dim_log2, int order, uint64_t seed, int max_bits,
+                      qth_sliced** out);
+/* plan: sliced wires, peak bits, slices, complex MACs per slice, of those in
+ * GEMM-shaped steps, peak bytes of a slice (complex128) */
+int qth_sliced_plan(qth_sliced* s, int64_t* values6);
+int qth_sliced_run(qth_sliced* s, qt_ctx* ctx, int precision, double* re, double* im, double* ms);
+int qth_sliced_destroy(qth_sliced* s);
 
 #ifdef __cplusplus
 }

@@ -6,10 +6,12 @@ This package only binds them for Python callers (tests, benchmarks, services).
 """
 from ._native import (  # noqa: F401
     QT_C64_3M, QT_C64_4M, QT_C64_BF16X3, QT_DTYPE_C64, QT_DTYPE_C128, QT_FROM_A, QT_PLAIN_C128,
+    QT_SCRATCH_AUTO, QT_SCRATCH_GLOBAL, QT_SCRATCH_LDS, QT_RESCALE, SlicedNetwork,
     ORDER_GIVEN, ORDER_GREEDY, ORDER_LINEGRAPH, ORDER_STOCHASTIC, ContractionFailure, Context, NetBatch,
     QaoaBatch, DeviceFailure, EngineError, InvalidContractionMethod, InvalidFile,
     InvalidFileFormat, InvalidFunctionInput, InvalidUserContractionSequence, Network, QbbFailure, Tensor,
     contract_pair, declared_symbols, maxcut_final_string, qaoa_optimize, qaoa_edge_text, qaoa_energy_plain, qaoa_num_edges,
+    ROUTE_EAGER, ROUTE_MEDIUM, ROUTE_SMALL, SERVE_AMPLITUDE, SERVE_QAOA_ENERGY, Server, serve_plan,
 )
 
 __all__ = [n for n in dir() if not n.startswith("_")]

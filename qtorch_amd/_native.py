@@ -21,6 +21,8 @@ HOST_LIB_PATH = os.path.join(LIB_DIR, "libqtorch_host.so")
 QT_OK, QT_INVALID_INPUT, QT_OOM, QT_DEVICE, QT_TIMEOUT, QT_UNSUPPORTED = range(6)
 QT_DTYPE_C128, QT_DTYPE_C64 = 0, 1
 QT_PLAIN_C128, QT_C64_4M, QT_C64_3M, QT_C64_BF16X3 = 0, 1, 2, 3
+QT_SCRATCH_AUTO, QT_SCRATCH_LDS, QT_SCRATCH_GLOBAL = 0, 1, 2
+QT_RESCALE = 1 << 8  # or-ed into a complex64 form: per-step power-of-two rescaling
 QT_FROM_A = 1 << 30
 
 _c = ctypes
@@ -57,6 +59,11 @@ _HIP_SIGS = {
     "qt_contract_pair": (_c.c_int, [_vp, _vp, _c.c_int, _vp, _c.c_int, _c.c_int, _i32p, _i32p, _c.c_int,
                                     _i32p, _vp, _c.c_int, _i64p]),
     "qt_tensor_read_scalar": (_c.c_int, [_vp, _vp, _dp, _dp]),
+    "qt_tensor_accumulate": (_c.c_int, [_vp, _vp, _vp]),
+    "qt_tensor_exponent": (_c.c_int, [_vp, _vp, _c.POINTER(_c.c_int32)]),
+    "qt_batch_add_programs": (_c.c_int, [_vp, _c.c_int, _c.POINTER(_vp), _i64p, _i32p]),
+    "qt_batch_write_leaves": (_c.c_int, [_vp, _c.c_int64, _dp, _c.c_int64]),
+    "qt_batch_poll": (_c.c_int, [_vp, _c.POINTER(_c.c_int)]),
     "qt_program_create": (_c.c_int, [_vp, _c.c_int, _c.c_int, _i32p, _i64p, _c.c_int, _i32p, _i32p,
                                      _c.POINTER(_vp)]),
     "qt_program_stats": (_c.c_int, [_vp, _i64p, _i64p, _c.POINTER(_c.c_int)]),
@@ -66,6 +73,7 @@ _HIP_SIGS = {
     "qt_batch_set_instances": (_c.c_int, [_vp, _c.c_int, _i32p, _i64p, _c.c_int, _i32p, _c.c_double,
                                           _c.c_double]),
     "qt_batch_config": (_c.c_int, [_vp, _c.c_int, _c.c_int]),
+    "qt_batch_config_ex": (_c.c_int, [_vp, _c.c_int, _c.c_int, _c.c_int]),
     "qt_batch_autotune": (_c.c_int, [_vp, _c.POINTER(_c.c_int), _c.POINTER(_c.c_int), _c.POINTER(_c.c_float)]),
     "qt_batch_launch": (_c.c_int, [_vp]),
     "qt_batch_capture": (_c.c_int, [_vp]),
@@ -93,6 +101,7 @@ _HOST_SIGS = {
     "qth_flops": (_c.c_int64, [_vp]),
     "qth_counts": (_c.c_int, [_vp] + [_c.POINTER(_c.c_int)] * 4),
     "qth_plan": (_c.c_int, [_vp, _i32p, _c.c_int, _c.POINTER(_c.c_int)]),
+    "qth_program_stats": (_c.c_int, [_vp, _c.POINTER(_c.c_int64), _c.POINTER(_c.c_int64), _c.POINTER(_c.c_int)]),
     "qth_qaoa_num_edges": (_c.c_int, [_c.c_char_p, _c.POINTER(_c.c_int), _c.POINTER(_c.c_int)]),
     "qth_qaoa_edge_text": (_c.c_int, [_c.c_char_p, _c.c_int, _c.c_int, _c.c_int, _dp, _c.c_char_p, _c.c_int64,
                                       _c.c_char_p, _c.c_int64]),
@@ -115,6 +124,26 @@ _HOST_SIGS = {
     "qth_netbatch_results": (_c.c_int, [_vp, _dp]),
     "qth_netbatch_cmacs": (_c.c_int, [_vp, _i64p]),
     "qth_netbatch_destroy": (_c.c_int, [_vp]),
+    "qth_server_create": (_c.c_int, [_c.c_int, _c.c_int, _c.c_int, _c.c_int64, _c.c_int, _c.POINTER(_vp)]),
+    "qth_server_destroy": (_c.c_int, [_vp]),
+    "qth_server_submit_qaoa": (_c.c_int, [_vp, _c.c_char_p, _c.c_int, _c.c_int, _c.POINTER(_c.c_double),
+                                          _c.POINTER(_c.c_int64)]),
+    "qth_server_submit_amplitude": (_c.c_int, [_vp, _c.c_char_p, _c.c_char_p, _c.POINTER(_c.c_int64)]),
+    "qth_server_poll": (_c.c_int, [_vp, _c.c_int64, _c.POINTER(_c.c_int), _c.POINTER(_c.c_int),
+                                   _c.POINTER(_c.c_double), _c.POINTER(_c.c_double), _c.POINTER(_c.c_double),
+                                   _c.POINTER(_c.c_int)]),
+    "qth_server_wait": (_c.c_int, [_vp, _c.c_int64, _c.POINTER(_c.c_double), _c.POINTER(_c.c_double),
+                                   _c.POINTER(_c.c_double), _c.POINTER(_c.c_int)]),
+    "qth_server_drain": (_c.c_int, [_vp]),
+    "qth_server_stats": (_c.c_int, [_vp, _c.POINTER(_c.c_int64), _c.POINTER(_c.c_double), _c.POINTER(_c.c_int64)]),
+    "qth_sliced_create": (_c.c_int, [_c.c_char_p, _c.c_char_p, _c.c_int, _c.c_int, _c.c_uint64, _c.c_int,
+                                     _c.POINTER(_vp)]),
+    "qth_sliced_plan": (_c.c_int, [_vp, _c.POINTER(_c.c_int64)]),
+    "qth_sliced_run": (_c.c_int, [_vp, _vp, _c.c_int, _c.POINTER(_c.c_double), _c.POINTER(_c.c_double),
+                                  _c.POINTER(_c.c_double)]),
+    "qth_sliced_destroy": (_c.c_int, [_vp]),
+    "qth_serve_plan": (_c.c_int, [_c.c_int, _c.c_char_p, _c.c_int, _c.c_int, _c.c_char_p, _c.c_int,
+                                  _c.POINTER(_c.c_int), _c.POINTER(_c.c_int64), _c.POINTER(_c.c_int64)]),
 }
 
 ORDER_GIVEN, ORDER_GREEDY, ORDER_STOCHASTIC, ORDER_LINEGRAPH = 0, 1, 2, 3
@@ -264,6 +293,23 @@ class Tensor:
         self.ctx.check(hip.qt_tensor_download(self.ctx.handle, self.handle, v.ctypes.data, v.nbytes), "download")
         return v
 
+    def exponent(self) -> int:
+        """Power-of-two exponent of a rescaled complex64 tensor (values = data * 2^e)."""
+        e = ctypes.c_int32()
+        self.ctx.check(hip.qt_tensor_exponent(self.ctx.handle, self.handle, ctypes.byref(e)), "qt_tensor_exponent")
+        return e.value
+
+    def scalar(self) -> complex:
+        """Value of a rank-0 tensor (exponent applied in FP64)."""
+        re, im = ctypes.c_double(), ctypes.c_double()
+        self.ctx.check(hip.qt_tensor_read_scalar(self.ctx.handle, self.handle, ctypes.byref(re), ctypes.byref(im)),
+                       "qt_tensor_read_scalar")
+        return complex(re.value, im.value)
+
+    def accumulate_into(self, acc: "Tensor") -> None:
+        """acc += self (rank-0 tensors, acc complex128), in FP64 on the device."""
+        self.ctx.check(hip.qt_tensor_accumulate(self.ctx.handle, self.handle, acc.handle), "qt_tensor_accumulate")
+
     def free(self) -> None:
         if self.handle:
             hip.qt_tensor_free(self.ctx.handle, self.handle)
@@ -380,6 +426,13 @@ class Network:
         _host_check(host.qth_counts(self.handle, *[ctypes.byref(x) for x in v]))
         return {"all": v[0].value, "uncontracted": v[1].value, "original": v[2].value, "done": bool(v[3].value)}
 
+    def program_stats(self):
+        """complex MACs, scratch elements per executor and steps of the device program
+        compiled from the recorded plan (no device needed)."""
+        cm, se, ns = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int()
+        _host_check(host.qth_program_stats(self.handle, ctypes.byref(cm), ctypes.byref(se), ctypes.byref(ns)))
+        return {"cmacs": cm.value, "scratch_elems": se.value, "n_steps": ns.value}
+
     def plan(self) -> np.ndarray:
         n = ctypes.c_int(0)
         _host_check(host.qth_plan(self.handle, None, 0, ctypes.byref(n)))
@@ -425,8 +478,9 @@ class _DeviceBatch:
     def capture(self) -> None:
         self.ctx.check(hip.qt_batch_capture(self.dev), "qt_batch_capture")
 
-    def config(self, lanes: int, wgs_per_cu: int) -> None:
-        self.ctx.check(hip.qt_batch_config(self.dev, lanes, wgs_per_cu), "qt_batch_config")
+    def config(self, lanes: int, wgs_per_cu: int, scratch: int = 0) -> None:
+        """Executor shape; scratch: 0 automatic, 1 LDS, 2 device memory."""
+        self.ctx.check(hip.qt_batch_config_ex(self.dev, lanes, wgs_per_cu, scratch), "qt_batch_config_ex")
 
     def autotune(self):
         l, w, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_float()
@@ -555,3 +609,124 @@ def maxcut_final_string(ctx: Context, dgf: str, p: int, betas_gammas: Sequence[f
     _host_check(host.qth_maxcut_final_string(ctx.handle, dgf.encode(), p, bg, ctypes.c_uint64(seed), bits,
                                              ctypes.byref(cut)))
     return list(bits), cut.value
+
+
+# ---- serving -----------------------------------------------------------------------
+
+SERVE_QAOA_ENERGY, SERVE_AMPLITUDE = 0, 1
+ROUTE_SMALL, ROUTE_MEDIUM, ROUTE_EAGER = 0, 1, 2
+_ROUTES = {ROUTE_SMALL: "small", ROUTE_MEDIUM: "medium", ROUTE_EAGER: "eager"}
+
+
+def serve_plan(kind: int, text: str, p: int = 1, ref_map: bool = False, bits: Optional[str] = None,
+               precision: int = QT_C64_4M):
+    """Route, planned device bytes and complex MACs of a serving request (host only)."""
+    route, planned, cm = ctypes.c_int(), ctypes.c_int64(), ctypes.c_int64()
+    _host_check(host.qth_serve_plan(kind, text.encode(), p, 1 if ref_map else 0,
+                                    bits.encode() if bits is not None else None, precision, ctypes.byref(route),
+                                    ctypes.byref(planned), ctypes.byref(cm)))
+    return {"route": _ROUTES[route.value], "planned_bytes": planned.value, "cmacs": cm.value}
+
+
+class Server:
+    """Continuous-batching request loop (include/qtorch_host.h, qth_server_*): QAOA
+    energies and single amplitudes submitted from any thread, results polled or
+    awaited by id."""
+
+    def __init__(self, device: int = 0, slots: int = 2, precision: int = QT_C64_4M, hbm_budget: int = 16 << 30,
+                 max_requests_per_launch: int = 512):
+        h = _vp()
+        _host_check(host.qth_server_create(device, slots, precision, hbm_budget, max_requests_per_launch,
+                                           ctypes.byref(h)))
+        self.handle = h
+
+    def submit_qaoa(self, dgf: str, p: int, betas_gammas: Sequence[float], ref_map: bool = False) -> int:
+        bg = (ctypes.c_double * len(betas_gammas))(*betas_gammas)
+        i = ctypes.c_int64()
+        _host_check(host.qth_server_submit_qaoa(self.handle, dgf.encode(), p, 1 if ref_map else 0, bg,
+                                                ctypes.byref(i)))
+        return i.value
+
+    def submit_amplitude(self, qasm: str, bits: str) -> int:
+        i = ctypes.c_int64()
+        _host_check(host.qth_server_submit_amplitude(self.handle, qasm.encode(), bits.encode(), ctypes.byref(i)))
+        return i.value
+
+    @staticmethod
+    def _result(status, re, im, lat, route):
+        return {"status": status, "value": complex(re, im), "latency_ms": lat, "route": _ROUTES.get(route, None)}
+
+    def poll(self, rid: int):
+        done, st, route = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        re, im, lat = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        _host_check(host.qth_server_poll(self.handle, rid, ctypes.byref(done), ctypes.byref(st), ctypes.byref(re),
+                                         ctypes.byref(im), ctypes.byref(lat), ctypes.byref(route)))
+        return self._result(st.value, re.value, im.value, lat.value, route.value) if done.value else None
+
+    def wait(self, rid: int):
+        route = ctypes.c_int()
+        re, im, lat = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        st = host.qth_server_wait(self.handle, rid, ctypes.byref(re), ctypes.byref(im), ctypes.byref(lat),
+                                  ctypes.byref(route))
+        return self._result(st, re.value, im.value, lat.value, route.value)
+
+    def drain(self) -> None:
+        _host_check(host.qth_server_drain(self.handle))
+
+    def stats(self):
+        c = (ctypes.c_int64 * 5)()
+        m = (ctypes.c_double * 5)()
+        peak = ctypes.c_int64()
+        _host_check(host.qth_server_stats(self.handle, c, m, ctypes.byref(peak)))
+        return {"submitted": c[0], "completed": c[1], "rejected": c[2], "launches": c[3], "edge_evals": c[4],
+                "p50_ms": m[0], "p99_ms": m[1], "mean_ms": m[2], "overlap_ms": m[3], "busy_ms": m[4],
+                "peak_planned_bytes": peak.value}
+
+    def close(self) -> None:
+        if self.handle:
+            host.qth_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---- one large network, sliced -----------------------------------------------------
+
+class SlicedNetwork:
+    """A network planned once (QTH_ORDER_* order), wires sliced until every tensor
+    has at most max_bits index bits; run() contracts every slice in the given form and
+    sums the slices in FP64 on the device."""
+
+    def __init__(self, qasm: str, measure: str, dim_log2: int = 1, order: int = ORDER_LINEGRAPH, seed: int = 0,
+                 max_bits: int = 28):
+        h = _vp()
+        _host_check(host.qth_sliced_create(qasm.encode(), measure.encode(), dim_log2, order, ctypes.c_uint64(seed),
+                                           max_bits, ctypes.byref(h)))
+        self.handle = h
+
+    def plan(self):
+        v = (ctypes.c_int64 * 6)()
+        _host_check(host.qth_sliced_plan(self.handle, v))
+        return {"sliced_wires": v[0], "peak_bits": v[1], "slices": v[2], "cmacs_per_slice": v[3],
+                "gemm_cmacs_per_slice": v[4], "peak_bytes": v[5]}
+
+    def run(self, ctx: Context, precision: int = QT_PLAIN_C128):
+        re, im, ms = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        _host_check(host.qth_sliced_run(self.handle, ctx.handle, precision, ctypes.byref(re), ctypes.byref(im),
+                                        ctypes.byref(ms)))
+        return complex(re.value, im.value), ms.value
+
+    def close(self) -> None:
+        if self.handle:
+            host.qth_sliced_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -29,6 +29,7 @@ struct ProgHdr {
     int32_t leaf_begin, n_leaves;
     int32_t scratch_elems;
     int32_t flags;
+    int64_t const_base;  // pool offset added to the program's constant leaf references
 };
 
 struct StepDesc {
@@ -79,6 +80,7 @@ struct qt_batch {
     qt::Instance* d_inst = nullptr;
     int n_inst = 0;
     double* d_out = nullptr;       // 2 * n_inst
+    size_t cap_inst = 0, cap_out = 0, cap_seg = 0, cap_members = 0, cap_group = 0, cap_chunks = 0;
     int32_t* d_counter = nullptr;
     // group sums
     int32_t* d_seg = nullptr;
@@ -91,7 +93,9 @@ struct qt_batch {
     // launch configuration
     int lanes = 64;  // 256: a workgroup per instance; 64: a wavefront; < 64: lanes per instance in a wave
     int wgs_per_cu = 4;
+    int scratch_place = QT_SCRATCH_AUTO;
     int32_t slot_stride = 0;      // elements between the scratch areas of two executors
+    int32_t pstage = 0, cstage = 0;  // largest parameter / constant leaf (elements), LDS stages
     std::vector<int32_t> h_prog;  // program of each instance (host copy, for chunking)
     int2* d_chunks = nullptr;
     int n_chunks = 0;
@@ -100,7 +104,17 @@ struct qt_batch {
     bool rescale = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_done = nullptr;
+    // host copies of the device program arrays (programs can be appended)
+    std::vector<qt::ProgHdr> h_hdr;
+    std::vector<qt::StepDesc> h_steps;
+    std::vector<uint32_t> h_tabs;
+    std::vector<int64_t> h_refs;
+    std::vector<int64_t> prog_const_end, prog_param_end;  // pool extents each program reads
+    size_t cap_progs = 0, cap_steps = 0, cap_tabs = 0, cap_refs = 0;
+    int64_t pool_cap = 0;           // elements allocated for the pool
+    void* d_pool_stage = nullptr;   // FP64 staging of pool writes for complex64 batches
+    int64_t pool_stage_cap = 0;
 };
 
 namespace qt {
@@ -256,7 +270,8 @@ __global__ __launch_bounds__(kWg) void k_batch(const ProgHdr* __restrict__ progs
             V* C = mem + st.c;
             auto leaf = [&](int code) -> const V* {
                 const int64_t r = refs[-1 - code];
-                const int64_t off = (r & (int64_t(1) << 62)) ? in.param_base + (r & ~(int64_t(1) << 62)) : r;
+                const int64_t off =
+                    (r & (int64_t(1) << 62)) ? in.param_base + (r & ~(int64_t(1) << 62)) : P.const_base + r;
                 return pool + off;
             };
             if (st.a < 0 && st.b < 0)
@@ -276,6 +291,9 @@ __global__ __launch_bounds__(kWg) void k_batch(const ProgHdr* __restrict__ progs
 // Several instances of one program per wavefront: lane l serves instance slot l / G
 // with G lanes; the wave takes a chunk of up to 64 / G consecutive instances of the
 // same program (built on the host) and walks the program once for all of them.
+// Leaf operands are staged in LDS once per step (constant leaves once per wave,
+// parameter leaves once per slot), so every multiply-add reads LDS, not the pool.
+// LDS per wave: [W slot scratch areas (LDS mode)] [W x 2 parameter stages] [2 constant stages].
 template <class V, int MODE, int G, bool GLOBAL_SCRATCH>
 __global__ __launch_bounds__(64) void k_batch_grouped(const ProgHdr* __restrict__ progs,
                                                       const StepDesc* __restrict__ steps,
@@ -284,15 +302,22 @@ __global__ __launch_bounds__(64) void k_batch_grouped(const ProgHdr* __restrict_
                                                       const V* __restrict__ pool, const Instance* __restrict__ inst,
                                                       const int2* __restrict__ chunks, int n_chunks,
                                                       int32_t* __restrict__ counter, double2* __restrict__ out,
-                                                      V* __restrict__ gscratch, int slot_stride) {
+                                                      V* __restrict__ gscratch, int slot_stride, int pstage,
+                                                      int cstage) {
     extern __shared__ unsigned char lds_raw[];
     constexpr int W = 64 / G;
+    constexpr int64_t kParam = int64_t(1) << 62;
     const int lane = threadIdx.x, slot = lane / G, gl = lane % G;
+    V* lds = reinterpret_cast<V*>(lds_raw);
     V* mem;
-    if constexpr (GLOBAL_SCRATCH)
+    if constexpr (GLOBAL_SCRATCH) {
         mem = gscratch + (int64_t(blockIdx.x) * W + slot) * slot_stride;
-    else
-        mem = reinterpret_cast<V*>(lds_raw) + slot * slot_stride;
+    } else {
+        mem = lds + slot * slot_stride;
+        lds += W * slot_stride;
+    }
+    V* const pst = lds + slot * 2 * pstage;  // this slot's parameter-leaf stages
+    V* const cst = lds + W * 2 * pstage;     // the wave's constant-leaf stages
     while (true) {
         int t = 0;
         if (lane == 0) t = atomicAdd(counter, 1);
@@ -311,20 +336,38 @@ __global__ __launch_bounds__(64) void k_batch_grouped(const ProgHdr* __restrict_
             const uint32_t* gs = fc + nC;
             double2* fin = (st.flags & 1) ? out + i : nullptr;
             V* C = mem + st.c;
-            auto leaf = [&](int code) -> const V* {
+            // copies leaf `code` (1 << bits elements) into stage `which`
+            auto stage = [&](int code, int bits, int which) -> V* {
                 const int64_t r = refs[-1 - code];
-                const int64_t off = (r & (int64_t(1) << 62)) ? pbase + (r & ~(int64_t(1) << 62)) : r;
-                return pool + off;
+                const int n = 1 << bits;
+                if (r & kParam) {
+                    V* dst = pst + which * pstage;
+                    const V* src = pool + pbase + (r & ~kParam);
+                    if (active)
+                        for (int e = gl; e < n; e += G) dst[e] = src[e];
+                    return dst;
+                }
+                V* dst = cst + which * cstage;
+                const V* src = pool + P.const_base + r;
+                for (int e = lane; e < n; e += 64) dst[e] = src[e];
+                return dst;
             };
-            if (active) {
-                if (st.a < 0 && st.b < 0)
-                    run_step<V, MODE, G>(leaf(st.a), leaf(st.b), C, fc, gs, nC, K, gl, fin);
-                else if (st.a < 0)
-                    run_step<V, MODE, G>(leaf(st.a), (const V*)(mem + st.b), C, fc, gs, nC, K, gl, fin);
-                else if (st.b < 0)
-                    run_step<V, MODE, G>((const V*)(mem + st.a), leaf(st.b), C, fc, gs, nC, K, gl, fin);
-                else
-                    run_step<V, MODE, G>((const V*)(mem + st.a), (const V*)(mem + st.b), C, fc, gs, nC, K, gl, fin);
+            const int a_bits = st.pad & 0xff, b_bits = (st.pad >> 8) & 0xff;
+            if (st.a < 0 && st.b < 0) {
+                V* sa = stage(st.a, a_bits, 0);
+                V* sb = stage(st.b, b_bits, 1);
+                exec_sync<64, GLOBAL_SCRATCH>();
+                if (active) run_step<V, MODE, G>((const V*)sa, (const V*)sb, C, fc, gs, nC, K, gl, fin);
+            } else if (st.a < 0) {
+                V* sa = stage(st.a, a_bits, 0);
+                exec_sync<64, GLOBAL_SCRATCH>();
+                if (active) run_step<V, MODE, G>((const V*)sa, (const V*)(mem + st.b), C, fc, gs, nC, K, gl, fin);
+            } else if (st.b < 0) {
+                V* sb = stage(st.b, b_bits, 1);
+                exec_sync<64, GLOBAL_SCRATCH>();
+                if (active) run_step<V, MODE, G>((const V*)(mem + st.a), (const V*)sb, C, fc, gs, nC, K, gl, fin);
+            } else if (active) {
+                run_step<V, MODE, G>((const V*)(mem + st.a), (const V*)(mem + st.b), C, fc, gs, nC, K, gl, fin);
             }
             exec_sync<64, GLOBAL_SCRATCH>();
         }
@@ -350,6 +393,22 @@ __global__ void k_to_c64(const double2* __restrict__ in, float2* __restrict__ ou
     if (i < n) out[i] = float2{float(in[i].x), float(in[i].y)};
 }
 
+// Device array of at least n elements, reused while it is large enough (contents
+// not kept), then filled from src if src is given.
+template <class T>
+qt_status dev_put(Ctx* c, T** dst, size_t* cap, const T* src, size_t n) {
+    if (n > *cap) {
+        if (*dst) (void)hipFree(*dst);
+        *dst = nullptr;
+        *cap = 0;
+        qt_status s = check_hip(c, hipMalloc(reinterpret_cast<void**>(dst), n * sizeof(T)), "batch alloc");
+        if (s != QT_OK) return s;
+        *cap = n;
+    }
+    if (!src || n == 0) return QT_OK;
+    return check_hip(c, hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice), "batch upload");
+}
+
 template <class T>
 qt_status dev_upload(Ctx* c, T** dst, const T* src, size_t n) {
     if (*dst) {
@@ -364,21 +423,26 @@ qt_status dev_upload(Ctx* c, T** dst, const T* src, size_t n) {
 
 // executors per workgroup and workgroup size of an executor shape
 inline int execs_per_wg(int lanes) { return lanes >= 64 ? kWg / lanes : 64 / lanes; }
+// LDS elements of a lane-grouped wave: slot scratch (LDS mode) and leaf stages
+inline size_t grouped_lds_elems(const qt_batch* b) {
+    const size_t w = size_t(64 / b->lanes);
+    return (b->use_global_scratch ? 0 : w * size_t(b->slot_stride)) + 2 * w * size_t(b->pstage) +
+           2 * size_t(b->cstage);
+}
 inline int grid_wgs(const qt_batch* b) { return b->n_cus * b->wgs_per_cu * (b->lanes >= 64 ? 1 : 4); }
 
 template <class V, int MODE>
 qt_status launch_grouped_t(qt_batch* b) {
     Ctx* c = b->ctx;
     const unsigned grid = unsigned(grid_wgs(b));
-    const size_t lds =
-        b->use_global_scratch ? 0 : size_t(execs_per_wg(b->lanes)) * size_t(b->slot_stride) * sizeof(V);
+    const size_t lds = grouped_lds_elems(b) * sizeof(V);
     V* gs = static_cast<V*>(b->d_global_scratch);
     const V* pool = static_cast<const V*>(b->d_pool);
     auto d_out = reinterpret_cast<double2*>(b->d_out);
 #define QT_LAUNCH_G(GL, GS)                                                                                        \
     hipLaunchKernelGGL((k_batch_grouped<V, MODE, GL, GS>), dim3(grid), dim3(64), lds, c->stream, b->d_progs,       \
                        b->d_steps, b->d_tabs, b->d_leaf_refs, pool, b->d_inst, b->d_chunks, b->n_chunks,           \
-                       b->d_counter, d_out, gs, b->slot_stride)
+                       b->d_counter, d_out, gs, b->slot_stride, b->pstage, b->cstage)
 #define QT_LAUNCH_GS(GS)                      \
     switch (b->lanes) {                       \
         case 4: QT_LAUNCH_G(4, GS); break;    \
@@ -448,10 +512,20 @@ qt_status configure_scratch(qt_batch* b) {
     // instance slots of a lane-grouped wave sit an odd number of elements apart, so
     // the slots' same-offset accesses fall into different LDS banks
     b->slot_stride = b->lanes < 64 ? b->scratch_elems + 1 : b->scratch_elems;
-    const size_t lds_need = size_t(kExec) * size_t(b->slot_stride) * es;
+    // leaf stages of lane-grouped waves (LDS in either placement)
+    const size_t stage =
+        b->lanes < 64 ? (2 * size_t(kExec) * size_t(b->pstage) + 2 * size_t(b->cstage)) * es : 0;
+    const size_t lds_need = size_t(kExec) * size_t(b->slot_stride) * es + stage;
     const size_t lds_cap = 160 * 1024;
     const size_t wgs_on_cu = size_t(b->wgs_per_cu) * (b->lanes >= 64 ? 1 : 4);
-    b->use_global_scratch = lds_need * wgs_on_cu > lds_cap || lds_need > 64 * 1024;
+    if (stage > 64 * 1024)
+        return set_error(c, QT_INVALID_INPUT, "batch: the leaf stages do not fit a workgroup's LDS");
+    if (b->scratch_place == QT_SCRATCH_LDS && lds_need > 64 * 1024)
+        return set_error(c, QT_INVALID_INPUT, "batch: the executors' scratch does not fit a workgroup's LDS");
+    // LDS unless it does not fit or would hold fewer workgroups per CU than asked for
+    // (then the intermediates go through L1/L2); an explicit placement wins
+    b->use_global_scratch = b->scratch_place == QT_SCRATCH_GLOBAL ||
+                            (b->scratch_place == QT_SCRATCH_AUTO && (lds_need * wgs_on_cu > lds_cap || lds_need > 64 * 1024));
     // chunks of consecutive instances of one program, at most one per instance slot
     if (b->lanes < 64) {
         const int W = 64 / b->lanes;
@@ -462,7 +536,7 @@ qt_status configure_scratch(qt_batch* b) {
             ch.push_back(int2{i, j - i});
             i = j;
         }
-        qt_status s = dev_upload(c, &b->d_chunks, ch.data(), ch.size());
+        qt_status s = dev_put(c, &b->d_chunks, &b->cap_chunks, ch.data(), ch.size());
         if (s != QT_OK) return s;
         b->n_chunks = int(ch.size());
     }
@@ -490,6 +564,188 @@ qt_status configure_scratch(qt_batch* b) {
     return QT_OK;
 }
 
+// Appends programs (host copies, then the device arrays again); the constant leaf
+// references of program i are offsets from const_base[i] in the pool.
+qt_status append_programs(qt_batch* b, int n, qt_program* const* progs, const int64_t* const_base,
+                          bool configure = true) {
+    Ctx* c = b->ctx;
+    for (int i = 0; i < n; ++i)
+        if (!progs[i] || const_base[i] < 0) return set_error(c, QT_INVALID_INPUT, "batch: null program or base");
+    if (int64_t(b->h_tabs.size()) + [&] {
+            int64_t t = 0;
+            for (int i = 0; i < n; ++i) t += int64_t(progs[i]->tables.size());
+            return t;
+        }() > int64_t(INT32_MAX))
+        return set_error(c, QT_INVALID_INPUT, "batch: program tables exceed 2^31 entries");
+    for (int i = 0; i < n; ++i) {
+        const qt_program* p = progs[i];
+        ProgHdr h{};
+        h.step_begin = int32_t(b->h_steps.size());
+        h.n_steps = int32_t(p->steps.size());
+        h.leaf_begin = int32_t(b->h_refs.size());
+        h.n_leaves = int32_t(p->leaf_refs.size());
+        h.scratch_elems = p->scratch_elems;
+        h.const_base = const_base[i];
+        const int32_t tab0 = int32_t(b->h_tabs.size());
+        for (auto st : p->steps) {
+            st.tab += tab0;
+            b->h_steps.push_back(st);
+        }
+        b->h_tabs.insert(b->h_tabs.end(), p->tables.begin(), p->tables.end());
+        int64_t cend = 0, pend = 0;
+        for (size_t l = 0; l < p->leaf_refs.size(); ++l) {
+            const int64_t r = p->leaf_refs[l];
+            const int32_t elems = int32_t(1) << (p->leaf_ranks[l] * p->dim_log2);
+            const int64_t end = (r & ~QT_LEAF_PARAM) + elems;
+            if (r & QT_LEAF_PARAM) {
+                pend = std::max(pend, end);
+                b->pstage = std::max(b->pstage, elems);
+            } else {
+                cend = std::max(cend, const_base[i] + end);
+                b->cstage = std::max(b->cstage, elems);
+            }
+            b->h_refs.push_back(r);
+        }
+        b->prog_const_end.push_back(cend);
+        b->prog_param_end.push_back(pend);
+        b->max_leaf_ref_end = std::max(b->max_leaf_ref_end, cend);
+        b->max_param_ref_end = std::max(b->max_param_ref_end, pend);
+        b->scratch_elems = std::max(b->scratch_elems, (p->scratch_elems + 15) / 16 * 16);
+        b->prog_cmacs.push_back(p->cmacs);
+        b->h_hdr.push_back(h);
+    }
+    b->n_programs += n;
+    (void)hipStreamSynchronize(c->stream);  // a launch in flight reads the arrays
+    qt_status s = dev_put(c, &b->d_progs, &b->cap_progs, b->h_hdr.data(), b->h_hdr.size());
+    if (s == QT_OK) s = dev_put(c, &b->d_steps, &b->cap_steps, b->h_steps.data(), b->h_steps.size());
+    if (s == QT_OK) s = dev_put(c, &b->d_tabs, &b->cap_tabs, b->h_tabs.data(), b->h_tabs.size());
+    if (s == QT_OK) s = dev_put(c, &b->d_leaf_refs, &b->cap_refs, b->h_refs.data(), b->h_refs.size());
+    if (s == QT_OK && configure) s = configure_scratch(b);
+    return s;
+}
+
+// Writes n FP64 complex values at pool offset `offset` (converted for complex64
+// batches), growing the pool (contents kept) when needed.  The stream is idle.
+qt_status write_pool(qt_batch* b, int64_t offset, const double* values, int64_t n) {
+    Ctx* c = b->ctx;
+    const size_t es = elem_size(b->precision);
+    const int64_t need = offset + n;
+    if (need > b->pool_cap) {
+        const int64_t cap = std::max(need, b->pool_cap + b->pool_cap / 2);
+        void* fresh = nullptr;
+        qt_status s = check_hip(c, hipMalloc(&fresh, size_t(cap) * es), "leaf pool");
+        if (s != QT_OK) return s;
+        if (b->d_pool && b->pool_elems > 0)
+            s = check_hip(c, hipMemcpy(fresh, b->d_pool, size_t(b->pool_elems) * es, hipMemcpyDeviceToDevice),
+                          "leaf pool grow");
+        if (s != QT_OK) {
+            (void)hipFree(fresh);
+            return s;
+        }
+        if (b->d_pool) (void)hipFree(b->d_pool);
+        b->d_pool = fresh;
+        b->pool_cap = cap;
+        if (b->graph_exec) (void)configure_scratch(b);  // the captured graph holds the old pool
+    }
+    char* dst = static_cast<char*>(b->d_pool) + size_t(offset) * es;
+    qt_status s;
+    if (b->precision == QT_PLAIN_C128) {
+        s = check_hip(c, hipMemcpy(dst, values, size_t(n) * 16, hipMemcpyHostToDevice), "leaf pool copy");
+    } else {
+        if (n > b->pool_stage_cap) {
+            if (b->d_pool_stage) (void)hipFree(b->d_pool_stage);
+            b->d_pool_stage = nullptr;
+            b->pool_stage_cap = 0;
+            s = check_hip(c, hipMalloc(&b->d_pool_stage, size_t(n) * 16), "leaf pool staging");
+            if (s != QT_OK) return s;
+            b->pool_stage_cap = n;
+        }
+        s = check_hip(c, hipMemcpy(b->d_pool_stage, values, size_t(n) * 16, hipMemcpyHostToDevice), "copy");
+        if (s == QT_OK) {
+            hipLaunchKernelGGL(k_to_c64, dim3(unsigned((n + 255) / 256)), dim3(256), 0, c->stream,
+                               static_cast<const double2*>(b->d_pool_stage), reinterpret_cast<float2*>(dst), n);
+            s = check_hip(c, hipGetLastError(), "k_to_c64");
+            if (s == QT_OK) s = check_hip(c, hipStreamSynchronize(c->stream), "sync");
+        }
+    }
+    if (s == QT_OK) b->pool_elems = std::max(b->pool_elems, need);
+    return s;
+}
+
+// A program's steps form a tree: every intermediate is consumed exactly once and
+// the last step is the root.  Any post-order of that tree computes the same values
+// (every step's own arithmetic is unchanged), but the scratch an executor needs
+// depends on the order.  Sethi-Ullman: for each step, evaluating operand X first
+// costs max(need(X), size(X) + need(Y), size(X) + size(Y) + size(result)), where
+// need() is a subtree's peak and size() its result (leaves live in the pool and
+// cost nothing); take the cheaper of the two operand orders.  Returns false (and
+// leaves the order as given) for input that is not such a tree: the validation
+// that follows reports it.
+bool reorder_steps(int dl, int n_leaves, int n_steps, const int32_t* desc, const int32_t* axes,
+                   std::vector<int32_t>& desc_out, std::vector<int32_t>& axes_out) {
+    if (dl != 1 && dl != 2) return false;
+    std::vector<int64_t> need(size_t(n_steps), 0), size(size_t(n_steps), 0);
+    std::vector<int> first(size_t(n_steps), 0), used(size_t(n_steps), 0);
+    std::vector<size_t> axis_begin(size_t(n_steps) + 1, 0);
+    for (int s = 0; s < n_steps; ++s) {
+        const int32_t* d = desc + 6 * s;
+        const int a = d[0], b = d[1], k = d[4], rc = d[5];
+        if (a < 0 || b < 0 || a >= n_leaves + s || b >= n_leaves + s || a == b || k < 0 || rc < 0 || rc * dl > 30)
+            return false;
+        axis_begin[size_t(s) + 1] = axis_begin[size_t(s)] + size_t(2 * k + rc);
+        int64_t n_op[2] = {0, 0}, s_op[2] = {0, 0};
+        const int ops[2] = {a, b};
+        for (int t = 0; t < 2; ++t)
+            if (ops[t] >= n_leaves) {
+                const int j = ops[t] - n_leaves;
+                if (used[size_t(j)]++) return false;
+                n_op[t] = need[size_t(j)];
+                s_op[t] = size[size_t(j)];
+            }
+        size[size_t(s)] = int64_t(1) << (rc * dl);
+        const int64_t both = s_op[0] + s_op[1] + size[size_t(s)];
+        const int64_t a_first = std::max({n_op[0], s_op[0] + n_op[1], both});
+        const int64_t b_first = std::max({n_op[1], s_op[1] + n_op[0], both});
+        first[size_t(s)] = b_first < a_first ? 1 : 0;
+        need[size_t(s)] = std::min(a_first, b_first);
+    }
+    for (int s = 0; s + 1 < n_steps; ++s)
+        if (!used[size_t(s)]) return false;
+    // iterative post-order from the root
+    std::vector<int> order;
+    order.reserve(size_t(n_steps));
+    std::vector<std::pair<int, int>> stack{{n_steps - 1, 0}};
+    while (!stack.empty()) {
+        auto& top = stack.back();
+        const int s = top.first;
+        if (top.second == 2) {
+            order.push_back(s);
+            stack.pop_back();
+            continue;
+        }
+        const int which = top.second == 0 ? first[size_t(s)] : 1 - first[size_t(s)];
+        ++top.second;
+        const int op = desc[6 * s + which];
+        if (op >= n_leaves) stack.push_back({op - n_leaves, 0});
+    }
+    if (int(order.size()) != n_steps) return false;
+    std::vector<int> new_pos(static_cast<size_t>(n_steps));
+    for (int i = 0; i < n_steps; ++i) new_pos[size_t(order[size_t(i)])] = i;
+    desc_out.resize(size_t(6 * n_steps));
+    axes_out.clear();
+    axes_out.reserve(axis_begin[size_t(n_steps)]);
+    for (int i = 0; i < n_steps; ++i) {
+        const int s = order[size_t(i)];
+        for (int t = 0; t < 6; ++t) desc_out[size_t(6 * i + t)] = desc[6 * s + t];
+        for (int t = 0; t < 2; ++t) {
+            const int op = desc[6 * s + t];
+            if (op >= n_leaves) desc_out[size_t(6 * i + t)] = n_leaves + new_pos[size_t(op - n_leaves)];
+        }
+        axes_out.insert(axes_out.end(), axes + axis_begin[size_t(s)], axes + axis_begin[size_t(s) + 1]);
+    }
+    return true;
+}
+
 }  // namespace
 }  // namespace qt
 
@@ -504,6 +760,12 @@ qt_status qt_program_create(qt_ctx* cx, int dim_log2, int n_leaves, const int32_
     if (!out || n_leaves < 1 || n_steps < 1 || !leaf_ranks || !leaf_refs || !step_desc || !step_axes)
         return QT_INVALID_INPUT;
     *out = nullptr;
+    // run the same contraction tree in the order that keeps the least scratch live
+    std::vector<int32_t> desc2, axes2;
+    if (qt::reorder_steps(dim_log2, n_leaves, n_steps, step_desc, step_axes, desc2, axes2)) {
+        step_desc = desc2.data();
+        step_axes = axes2.data();
+    }
     qt::Ctx* ctx = cx ? &cx->impl : nullptr;  // compilation touches no device memory
     if (dim_log2 != 1 && dim_log2 != 2) return set_error(ctx, QT_INVALID_INPUT, "qt_program_create: bad dim");
     const int dl = dim_log2;
@@ -624,6 +886,7 @@ qt_status qt_program_create(qt_ctx* cx, int dim_log2, int n_leaves, const int32_
         st.b = b < n_leaves ? -(1 + b) : slot[size_t(b - n_leaves)];
         st.c = s + 1 < n_steps ? slot[size_t(s)] : 0;
         st.flags = s + 1 == n_steps ? 1 : 0;
+        st.pad = (ra * dl) | ((rb * dl) << 8);  // element bits of the operands (leaf staging)
         p->steps.push_back(st);
         p->cmacs += int64_t(1) << ((rc + k) * dl);
         p->max_tensor_elems = std::max(p->max_tensor_elems, int32_t(1) << (rc * dl));
@@ -646,7 +909,7 @@ qt_status qt_program_destroy(qt_program* p) {
 }
 
 qt_status qt_batch_create(qt_ctx* cx, int n_programs, qt_program* const* progs, int precision, qt_batch** out) {
-    if (!cx || !out || n_programs < 1 || !progs) return QT_INVALID_INPUT;
+    if (!cx || !out || n_programs < 0 || (n_programs > 0 && !progs)) return QT_INVALID_INPUT;
     *out = nullptr;
     auto* ctx = &cx->impl;
     if (precision != QT_PLAIN_C128 && precision != QT_C64_4M && precision != QT_C64_3M &&
@@ -657,61 +920,24 @@ qt_status qt_batch_create(qt_ctx* cx, int n_programs, qt_program* const* progs, 
     auto b = std::make_unique<qt_batch>();
     b->ctx = ctx;
     b->precision = precision;
-    b->n_programs = n_programs;
-    std::vector<qt::ProgHdr> hdr;
-    std::vector<qt::StepDesc> steps;
-    std::vector<uint32_t> tabs;
-    std::vector<int64_t> refs;
-    for (int i = 0; i < n_programs; ++i) {
-        const qt_program* p = progs[i];
-        if (!p) return set_error(ctx, QT_INVALID_INPUT, "qt_batch_create: null program");
-        qt::ProgHdr h{};
-        h.step_begin = int32_t(steps.size());
-        h.n_steps = int32_t(p->steps.size());
-        h.leaf_begin = int32_t(refs.size());
-        h.n_leaves = int32_t(p->leaf_refs.size());
-        h.scratch_elems = p->scratch_elems;
-        const int32_t tab0 = int32_t(tabs.size());
-        for (auto st : p->steps) {
-            st.tab += tab0;
-            steps.push_back(st);
-        }
-        tabs.insert(tabs.end(), p->tables.begin(), p->tables.end());
-        for (size_t l = 0; l < p->leaf_refs.size(); ++l) {
-            const int64_t r = p->leaf_refs[l];
-            const int64_t end = (r & ~(int64_t(1) << 62)) + (int64_t(1) << (p->leaf_ranks[l] * p->dim_log2));
-            if (r & (int64_t(1) << 62)) b->max_param_ref_end = std::max(b->max_param_ref_end, end);
-            else b->max_leaf_ref_end = std::max(b->max_leaf_ref_end, end);
-            refs.push_back(r);
-        }
-        b->scratch_elems = std::max(b->scratch_elems, p->scratch_elems);
-        b->prog_cmacs.push_back(p->cmacs);
-        hdr.push_back(h);
-    }
-    b->scratch_elems = (b->scratch_elems + 15) / 16 * 16;
-    // default executor shape from the scratch footprint (the autotuner may refine it)
-    const size_t per_exec = size_t(b->scratch_elems) * qt::elem_size(precision);
-    if (per_exec * 4 <= 16 * 1024) {
-        b->lanes = 64;
-        b->wgs_per_cu = 4;
-    } else if (per_exec <= 32 * 1024) {
-        b->lanes = 256;
-        b->wgs_per_cu = 2;
-    } else {
-        b->lanes = 256;
-        b->wgs_per_cu = 1;
-    }
-    qt_status s = qt::dev_upload(ctx, &b->d_progs, hdr.data(), hdr.size());
-    if (s == QT_OK) s = qt::dev_upload(ctx, &b->d_steps, steps.data(), steps.size());
-    if (s == QT_OK) s = qt::dev_upload(ctx, &b->d_tabs, tabs.data(), tabs.size());
-    if (s == QT_OK) s = qt::dev_upload(ctx, &b->d_leaf_refs, refs.data(), refs.size());
-    if (s == QT_OK) s = check_hip(ctx, hipMalloc(reinterpret_cast<void**>(&b->d_counter), sizeof(int32_t)), "counter");
+    qt_status s = check_hip(ctx, hipMalloc(reinterpret_cast<void**>(&b->d_counter), sizeof(int32_t)), "counter");
     if (s == QT_OK) s = check_hip(ctx, hipDeviceGetAttribute(&b->n_cus, hipDeviceAttributeMultiprocessorCount,
                                                               ctx->device), "CU count");
     if (s == QT_OK) s = check_hip(ctx, hipEventCreate(&b->ev0), "event");
     if (s == QT_OK) s = check_hip(ctx, hipEventCreate(&b->ev1), "event");
-    if (s == QT_OK) s = qt::configure_scratch(b.get());
+    if (s == QT_OK) s = check_hip(ctx, hipEventCreateWithFlags(&b->ev_done, hipEventDisableTiming), "event");
     ++ctx->live;
+    if (s == QT_OK && n_programs > 0) {
+        std::vector<int64_t> zero(static_cast<size_t>(n_programs), 0);
+        s = qt::append_programs(b.get(), n_programs, progs, zero.data(), false);  // shape chosen below
+    }
+    if (s == QT_OK) {
+        // default executor shape from the scratch footprint (the autotuner may refine it)
+        const size_t per_exec = size_t(b->scratch_elems) * qt::elem_size(precision);
+        b->lanes = per_exec * 4 <= 16 * 1024 ? 64 : 256;
+        b->wgs_per_cu = per_exec * 4 <= 16 * 1024 ? 4 : per_exec <= 32 * 1024 ? 2 : 1;
+        s = qt::configure_scratch(b.get());
+    }
     if (s != QT_OK) {
         qt_batch_destroy(b.release());
         return s;
@@ -720,38 +946,35 @@ qt_status qt_batch_create(qt_ctx* cx, int n_programs, qt_program* const* progs, 
     return QT_OK;
 }
 
+qt_status qt_batch_add_programs(qt_batch* b, int n, qt_program* const* progs, const int64_t* const_base,
+                                int32_t* first_id) {
+    if (!b || n < 1 || !progs || !const_base) return QT_INVALID_INPUT;
+    std::lock_guard<std::recursive_mutex> g(b->ctx->mu);
+    (void)hipSetDevice(b->ctx->device);
+    if (first_id) *first_id = b->n_programs;
+    return qt::append_programs(b, n, progs, const_base);
+}
+
 qt_status qt_batch_set_leaves(qt_batch* b, const double* pool, int64_t n_elems) {
     if (!b || !pool || n_elems < 1) return QT_INVALID_INPUT;
     auto* ctx = b->ctx;
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
-    if (b->d_pool) (void)hipFree(b->d_pool);
-    b->d_pool = nullptr;
+    if (b->max_leaf_ref_end > n_elems)
+        return set_error(ctx, QT_INVALID_INPUT, "qt_batch_set_leaves: pool smaller than the leaf references");
     b->pool_elems = 0;
     b->n_inst = 0;  // parameter bases were checked against the old pool: set instances again
     (void)qt::configure_scratch(b);
-    const size_t es = qt::elem_size(b->precision);
-    qt_status s = check_hip(ctx, hipMalloc(&b->d_pool, size_t(n_elems) * es), "leaf pool");
-    if (s != QT_OK) return s;
-    if (b->precision == QT_PLAIN_C128) {
-        s = check_hip(ctx, hipMemcpy(b->d_pool, pool, size_t(n_elems) * 16, hipMemcpyHostToDevice), "leaf pool copy");
-    } else {
-        void* tmp = nullptr;
-        s = check_hip(ctx, hipMalloc(&tmp, size_t(n_elems) * 16), "leaf pool staging");
-        if (s == QT_OK) s = check_hip(ctx, hipMemcpy(tmp, pool, size_t(n_elems) * 16, hipMemcpyHostToDevice), "copy");
-        if (s == QT_OK) {
-            hipLaunchKernelGGL(qt::k_to_c64, dim3(unsigned((n_elems + 255) / 256)), dim3(256), 0, ctx->stream,
-                               static_cast<const double2*>(tmp), static_cast<float2*>(b->d_pool), n_elems);
-            s = check_hip(ctx, hipGetLastError(), "k_to_c64");
-            if (s == QT_OK) s = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync");
-        }
-        if (tmp) (void)hipFree(tmp);
-    }
-    if (s == QT_OK) b->pool_elems = n_elems;
-    if (s == QT_OK && b->max_leaf_ref_end > n_elems)
-        return set_error(ctx, QT_INVALID_INPUT, "qt_batch_set_leaves: pool smaller than the leaf references");
-    return s;
+    return qt::write_pool(b, 0, pool, n_elems);
+}
+
+qt_status qt_batch_write_leaves(qt_batch* b, int64_t offset, const double* values, int64_t n_elems) {
+    if (!b || offset < 0 || n_elems < 1 || !values) return QT_INVALID_INPUT;
+    std::lock_guard<std::recursive_mutex> g(b->ctx->mu);
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->stream);  // a launch in flight may read the pool
+    return qt::write_pool(b, offset, values, n_elems);
 }
 
 qt_status qt_batch_set_instances(qt_batch* b, int n, const int32_t* prog, const int64_t* param_base, int n_groups,
@@ -761,58 +984,73 @@ qt_status qt_batch_set_instances(qt_batch* b, int n, const int32_t* prog, const 
     auto* ctx = b->ctx;
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
     (void)hipSetDevice(ctx->device);
+    // a launch still in flight reads the instance arrays that are replaced below
+    (void)hipStreamSynchronize(ctx->stream);
     if (b->pool_elems == 0) return set_error(ctx, QT_INVALID_INPUT, "qt_batch_set_instances: set the leaves first");
+    // validate everything before any state changes
     std::vector<qt::Instance> inst(static_cast<size_t>(n));
     int64_t cmacs = 0;
     for (int i = 0; i < n; ++i) {
         if (prog[i] < 0 || prog[i] >= b->n_programs || param_base[i] < 0 ||
-            param_base[i] + b->max_param_ref_end > b->pool_elems)
+            param_base[i] + b->prog_param_end[size_t(prog[i])] > b->pool_elems ||
+            b->prog_const_end[size_t(prog[i])] > b->pool_elems)
             return set_error(ctx, QT_INVALID_INPUT, "qt_batch_set_instances: bad program or parameter block");
+        if (n_groups > 0 && (group[i] < -1 || group[i] >= n_groups))
+            return set_error(ctx, QT_INVALID_INPUT, "qt_batch_set_instances: bad group");
         inst[size_t(i)] = qt::Instance{prog[i], 0, param_base[i]};
         cmacs += b->prog_cmacs[size_t(prog[i])];
     }
+    // group g's members in instance order (group -1: no group)
+    std::vector<int32_t> seg(size_t(n_groups) + 1, 0), members;
+    if (n_groups > 0) {
+        for (int i = 0; i < n; ++i)
+            if (group[i] >= 0) ++seg[size_t(group[i]) + 1];
+        for (int k = 0; k < n_groups; ++k) seg[size_t(k) + 1] += seg[size_t(k)];
+        members.resize(size_t(seg[size_t(n_groups)]));
+        std::vector<int32_t> fill(seg.begin(), seg.end() - 1);
+        for (int i = 0; i < n; ++i)
+            if (group[i] >= 0) members[size_t(fill[size_t(group[i])]++)] = i;
+    }
     b->n_inst = 0;
-    b->cmacs_total = cmacs;
+    b->n_groups = 0;
     b->h_prog.assign(prog, prog + n);
-    qt_status s = qt::dev_upload(ctx, &b->d_inst, inst.data(), inst.size());
-    if (s != QT_OK) return s;
-    if (b->d_out) (void)hipFree(b->d_out);
-    b->d_out = nullptr;
-    s = check_hip(ctx, hipMalloc(reinterpret_cast<void**>(&b->d_out), size_t(std::max(n, 1)) * 16), "batch out");
+    qt_status s = qt::dev_put(ctx, &b->d_inst, &b->cap_inst, inst.data(), inst.size());
+    if (s == QT_OK) s = qt::dev_put<double>(ctx, &b->d_out, &b->cap_out, nullptr, size_t(std::max(n, 1)) * 2);
+    if (s == QT_OK && n_groups > 0) {
+        s = qt::dev_put(ctx, &b->d_seg, &b->cap_seg, seg.data(), seg.size());
+        if (s == QT_OK) s = qt::dev_put(ctx, &b->d_members, &b->cap_members, members.data(), members.size());
+        if (s == QT_OK) s = qt::dev_put<double>(ctx, &b->d_group, &b->cap_group, nullptr, size_t(n_groups));
+    }
     if (s != QT_OK) return s;
     b->n_inst = n;
     b->n_groups = n_groups;
+    b->cmacs_total = cmacs;
     b->c0 = c0;
     b->c1 = c1;
-    if (n_groups > 0) {
-        std::vector<int32_t> seg(size_t(n_groups) + 1, 0), members(static_cast<size_t>(n));
-        for (int i = 0; i < n; ++i) {
-            if (group[i] < 0 || group[i] >= n_groups)
-                return set_error(ctx, QT_INVALID_INPUT, "qt_batch_set_instances: bad group");
-            ++seg[size_t(group[i]) + 1];
-        }
-        for (int k = 0; k < n_groups; ++k) seg[size_t(k) + 1] += seg[size_t(k)];
-        std::vector<int32_t> fill(seg.begin(), seg.end() - 1);
-        for (int i = 0; i < n; ++i) members[size_t(fill[size_t(group[i])]++)] = i;
-        s = qt::dev_upload(ctx, &b->d_seg, seg.data(), seg.size());
-        if (s == QT_OK) s = qt::dev_upload(ctx, &b->d_members, members.data(), members.size());
-        if (b->d_group) (void)hipFree(b->d_group);
-        b->d_group = nullptr;
-        if (s == QT_OK)
-            s = check_hip(ctx, hipMalloc(reinterpret_cast<void**>(&b->d_group), size_t(n_groups) * 8), "group sums");
-    }
-    if (s == QT_OK) s = qt::configure_scratch(b);  // drops a captured graph
-    return s;
+    return qt::configure_scratch(b);  // rebuilds the chunk list, drops a captured graph
 }
 
 qt_status qt_batch_config(qt_batch* b, int lanes, int wgs_per_cu) {
+    return qt_batch_config_ex(b, lanes, wgs_per_cu, QT_SCRATCH_AUTO);
+}
+
+qt_status qt_batch_config_ex(qt_batch* b, int lanes, int wgs_per_cu, int scratch) {
     if (!b || (lanes != 4 && lanes != 8 && lanes != 16 && lanes != 32 && lanes != 64 && lanes != 256) ||
-        wgs_per_cu < 1 || wgs_per_cu > 16)
+        wgs_per_cu < 1 || wgs_per_cu > 16 || scratch < QT_SCRATCH_AUTO || scratch > QT_SCRATCH_GLOBAL)
         return QT_INVALID_INPUT;
     std::lock_guard<std::recursive_mutex> g(b->ctx->mu);
+    const int l0 = b->lanes, w0 = b->wgs_per_cu, s0 = b->scratch_place;
     b->lanes = lanes;
     b->wgs_per_cu = wgs_per_cu;
-    return qt::configure_scratch(b);
+    b->scratch_place = scratch;
+    const qt_status s = qt::configure_scratch(b);
+    if (s == QT_INVALID_INPUT) {  // keep the previous, working shape
+        b->lanes = l0;
+        b->wgs_per_cu = w0;
+        b->scratch_place = s0;
+        (void)qt::configure_scratch(b);
+    }
+    return s;
 }
 
 qt_status qt_batch_launch(qt_batch* b) {
@@ -821,8 +1059,22 @@ qt_status qt_batch_launch(qt_batch* b) {
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
     (void)hipSetDevice(ctx->device);
     if (b->n_inst == 0) return QT_OK;
-    if (b->graph_exec) return check_hip(ctx, hipGraphLaunch(b->graph_exec, ctx->stream), "hipGraphLaunch");
-    return qt::launch_batch(b);
+    const qt_status s = b->graph_exec ? check_hip(ctx, hipGraphLaunch(b->graph_exec, ctx->stream), "hipGraphLaunch")
+                                      : qt::launch_batch(b);
+    if (s != QT_OK) return s;
+    return check_hip(ctx, hipEventRecord(b->ev_done, ctx->stream), "event record");
+}
+
+qt_status qt_batch_poll(qt_batch* b, int* done) {
+    if (!b || !done) return QT_INVALID_INPUT;
+    std::lock_guard<std::recursive_mutex> g(b->ctx->mu);
+    const hipError_t e = hipEventQuery(b->ev_done);
+    if (e == hipErrorNotReady) {
+        *done = 0;
+        return QT_OK;
+    }
+    *done = 1;
+    return check_hip(b->ctx, e, "batch poll");
 }
 
 qt_status qt_batch_capture(qt_batch* b) {
@@ -879,21 +1131,23 @@ qt_status qt_batch_autotune(qt_batch* b, int* lanes, int* wgs_per_cu, float* ms)
     std::lock_guard<std::recursive_mutex> g(b->ctx->mu);
     const int lane_opts[6] = {4, 8, 16, 32, 64, 256};
     const int wg_opts[5] = {1, 2, 4, 8, 16};
-    int best_l = b->lanes, best_w = b->wgs_per_cu;
+    int best_l = b->lanes, best_w = b->wgs_per_cu, best_p = b->scratch_place;
     float best = 1e30f;
     for (int l : lane_opts)
-        for (int w : wg_opts) {
-            if (qt_batch_config(b, l, w) != QT_OK) continue;
-            if (qt_batch_launch(b) != QT_OK) continue;  // warm-up
-            float t = 0;
-            if (qt_batch_time(b, 3, &t) != QT_OK) continue;
-            if (t < best) {
-                best = t;
-                best_l = l;
-                best_w = w;
+        for (int w : wg_opts)
+            for (int place : {QT_SCRATCH_LDS, QT_SCRATCH_GLOBAL}) {
+                if (qt_batch_config_ex(b, l, w, place) != QT_OK) continue;
+                if (qt_batch_launch(b) != QT_OK) continue;  // warm-up
+                float t = 0;
+                if (qt_batch_time(b, 3, &t) != QT_OK) continue;
+                if (t < best) {
+                    best = t;
+                    best_l = l;
+                    best_w = w;
+                    best_p = place;
+                }
             }
-        }
-    qt_status s = qt_batch_config(b, best_l, best_w);
+    qt_status s = qt_batch_config_ex(b, best_l, best_w, best_p);
     if (lanes) *lanes = best_l;
     if (wgs_per_cu) *wgs_per_cu = best_w;
     if (ms) *ms = best;
@@ -921,9 +1175,10 @@ qt_status qt_batch_destroy(qt_batch* b) {
         if (b->graph) (void)hipGraphDestroy(b->graph);
         if (b->ev0) (void)hipEventDestroy(b->ev0);
         if (b->ev1) (void)hipEventDestroy(b->ev1);
+        if (b->ev_done) (void)hipEventDestroy(b->ev_done);
         for (void* p : {(void*)b->d_progs, (void*)b->d_steps, (void*)b->d_tabs, (void*)b->d_leaf_refs, b->d_pool,
                         (void*)b->d_inst, (void*)b->d_out, (void*)b->d_counter, (void*)b->d_seg, (void*)b->d_members,
-                        (void*)b->d_group, b->d_global_scratch, (void*)b->d_chunks})
+                        (void*)b->d_group, b->d_global_scratch, (void*)b->d_chunks, b->d_pool_stage})
             if (p) (void)hipFree(p);
         --b->ctx->live;
     }

@@ -315,6 +315,7 @@ __global__ __launch_bounds__(kThreads) void k_gemm_c128(const double2* __restric
     const int wm = wave & 1, wn = wave >> 1;
     const int64_t tiles_m = M / GM;
     const int64_t tile = int64_t(blockIdx.x) + int64_t(blockIdx.y) * gridDim.x;
+    if (tile >= tiles_m * (N / GN)) return;
     const int64_t m0 = (tile % tiles_m) * GM, n0 = (tile / tiles_m) * GN;
     d4_t acc_re[2][2], acc_im[2][2];
     for (int i = 0; i < 2; ++i)
@@ -372,6 +373,168 @@ __global__ __launch_bounds__(kThreads) void k_gemm_c128(const double2* __restric
                 const int64_t n = n0 + wn * 32 + bn * 16 + lk + 4 * r;
                 const int64_t mm = m0 + wm * 32 + bm * 16 + li;
                 C[mm + M * n] = double2{acc_re[bn][bm][r], acc_im[bn][bm][r]};
+            }
+}
+
+// Complex64 GEMM on MFMA, C[m + M n] = sum_s A'[s][m] * B'[s][n], in the three
+// complex64 forms.  64x64 output tile per workgroup, four waves of 32x32, k-tiles
+// of 32.  4M and 3M use v_mfma_f32_16x16x4f32 (four / three real products per
+// complex product); bf16x3 splits every FP32 operand into bf16 hi + lo when staging
+// it in LDS and takes hi*hi + hi*lo + lo*hi on v_mfma_f32_16x16x32_bf16, all with
+// FP32 accumulation.  f32 16x16 accumulator layout: item r of lane l is row
+// 4 (l / 16) + r, column l % 16; rows are n (the B' operand), columns m (A').
+constexpr int HK = 32;
+typedef float f4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ uint16_t bf16_bits(float x) {  // round to nearest even
+    return uint16_t(__float_as_uint(bf16_round(x)) >> 16);
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kThreads) void k_gemm_c64(const float2* __restrict__ Ap, const float2* __restrict__ Bp,
+                                                       float2* __restrict__ C, int64_t M, int64_t N, int64_t K) {
+    // f32 forms: [k][row] planes of re / im; bf16x3: [row][k] planes of re_hi, re_lo,
+    // im_hi, im_lo (k contiguous: one 16-byte load is a lane's 8 operands)
+    constexpr int kPlaneF = HK * (GM + 1);
+    constexpr int kRowB = HK + 8;
+    constexpr int kPlaneB = GM * kRowB;
+    __shared__ __align__(16) unsigned char smem[MODE == MODE_BF16X3 ? 2 * 4 * kPlaneB * 2 : 2 * 2 * kPlaneF * 4];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int li = lane & 15, lk = lane >> 4;
+    const int64_t tiles_m = M / GM;
+    const int64_t tile = int64_t(blockIdx.x) + int64_t(blockIdx.y) * gridDim.x;
+    if (tile >= tiles_m * (N / GN)) return;
+    const int64_t m0 = (tile % tiles_m) * GM, n0 = (tile / tiles_m) * GN;
+    f4_t acc0[2][2], acc1[2][2], acc2[2][2];  // 4M / bf16x3: re, im; 3M: P1, P2, P3
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) acc0[i][j] = acc1[i][j] = acc2[i][j] = f4_t{0, 0, 0, 0};
+    for (int64_t k0 = 0; k0 < K; k0 += HK) {
+        if constexpr (MODE == MODE_BF16X3) {
+            uint16_t* sa = reinterpret_cast<uint16_t*>(smem);  // 4 planes of A', then 4 of B'
+            uint16_t* sb = sa + 4 * kPlaneB;
+            for (int e = tid; e < HK * GM; e += kThreads) {
+                const int kk = e / GM, rr = e % GM;
+                const float2 va = Ap[(k0 + kk) * M + m0 + rr];
+                const float2 vb = Bp[(k0 + kk) * N + n0 + rr];
+                const int o = rr * kRowB + kk;
+                const float arh = bf16_round(va.x), aih = bf16_round(va.y);
+                const float brh = bf16_round(vb.x), bih = bf16_round(vb.y);
+                sa[o] = bf16_bits(arh);
+                sa[kPlaneB + o] = bf16_bits(va.x - arh);
+                sa[2 * kPlaneB + o] = bf16_bits(aih);
+                sa[3 * kPlaneB + o] = bf16_bits(va.y - aih);
+                sb[o] = bf16_bits(brh);
+                sb[kPlaneB + o] = bf16_bits(vb.x - brh);
+                sb[2 * kPlaneB + o] = bf16_bits(bih);
+                sb[3 * kPlaneB + o] = bf16_bits(vb.y - bih);
+            }
+            __syncthreads();
+            auto ld = [&](const uint16_t* plane, int row) {
+                return *reinterpret_cast<const bf8_t*>(plane + row * kRowB + 8 * lk);
+            };
+            auto neg = [](bf8_t v) {
+                uint4 u = *reinterpret_cast<uint4*>(&v);
+                u.x ^= 0x80008000u;
+                u.y ^= 0x80008000u;
+                u.z ^= 0x80008000u;
+                u.w ^= 0x80008000u;
+                return *reinterpret_cast<bf8_t*>(&u);
+            };
+            bf8_t a[2][4];
+#pragma unroll
+            for (int bm = 0; bm < 2; ++bm)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) a[bm][q] = ld(sa + q * kPlaneB, wm * 32 + bm * 16 + li);
+#pragma unroll
+            for (int bn = 0; bn < 2; ++bn) {
+                bf8_t b[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) b[q] = ld(sb + q * kPlaneB, wn * 32 + bn * 16 + li);
+                const bf8_t nbih = neg(b[2]), nbil = neg(b[3]);
+#pragma unroll
+                for (int bm = 0; bm < 2; ++bm) {
+                    f4_t re = acc0[bn][bm], im = acc1[bn][bm];
+                    // re += br ar - bi ai; im += bi ar + br ai (each product hi*hi + hi*lo + lo*hi)
+                    re = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[0], a[bm][0], re, 0, 0, 0);
+                    re = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[0], a[bm][1], re, 0, 0, 0);
+                    re = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[1], a[bm][0], re, 0, 0, 0);
+                    re = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nbih, a[bm][2], re, 0, 0, 0);
+                    re = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nbih, a[bm][3], re, 0, 0, 0);
+                    re = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nbil, a[bm][2], re, 0, 0, 0);
+                    im = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[2], a[bm][0], im, 0, 0, 0);
+                    im = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[2], a[bm][1], im, 0, 0, 0);
+                    im = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[3], a[bm][0], im, 0, 0, 0);
+                    im = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[0], a[bm][2], im, 0, 0, 0);
+                    im = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[0], a[bm][3], im, 0, 0, 0);
+                    im = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[1], a[bm][2], im, 0, 0, 0);
+                    acc0[bn][bm] = re;
+                    acc1[bn][bm] = im;
+                }
+            }
+        } else {
+            float* sa = reinterpret_cast<float*>(smem);  // re plane, im plane of A', then of B'
+            float* sb = sa + 2 * kPlaneF;
+            for (int e = tid; e < HK * GM; e += kThreads) {
+                const int kk = e / GM, rr = e % GM;
+                const float2 va = Ap[(k0 + kk) * M + m0 + rr];
+                const float2 vb = Bp[(k0 + kk) * N + n0 + rr];
+                const int o = kk * (GM + 1) + rr;
+                sa[o] = va.x;
+                sa[kPlaneF + o] = va.y;
+                sb[o] = vb.x;
+                sb[kPlaneF + o] = vb.y;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int kk = 0; kk < HK; kk += 4) {
+                const int o = (kk + lk) * (GM + 1);
+                float are[2], aim[2];
+#pragma unroll
+                for (int bm = 0; bm < 2; ++bm) {
+                    are[bm] = sa[o + wm * 32 + bm * 16 + li];
+                    aim[bm] = sa[kPlaneF + o + wm * 32 + bm * 16 + li];
+                }
+#pragma unroll
+                for (int bn = 0; bn < 2; ++bn) {
+                    const float bre = sb[o + wn * 32 + bn * 16 + li], bim = sb[kPlaneF + o + wn * 32 + bn * 16 + li];
+#pragma unroll
+                    for (int bm = 0; bm < 2; ++bm) {
+                        if constexpr (MODE == MODE_3M) {
+                            acc0[bn][bm] = __builtin_amdgcn_mfma_f32_16x16x4f32(bre, are[bm], acc0[bn][bm], 0, 0, 0);
+                            acc1[bn][bm] = __builtin_amdgcn_mfma_f32_16x16x4f32(bim, aim[bm], acc1[bn][bm], 0, 0, 0);
+                            acc2[bn][bm] = __builtin_amdgcn_mfma_f32_16x16x4f32(bre + bim, are[bm] + aim[bm],
+                                                                               acc2[bn][bm], 0, 0, 0);
+                        } else {
+                            acc0[bn][bm] = __builtin_amdgcn_mfma_f32_16x16x4f32(bre, are[bm], acc0[bn][bm], 0, 0, 0);
+                            acc0[bn][bm] = __builtin_amdgcn_mfma_f32_16x16x4f32(-bim, aim[bm], acc0[bn][bm], 0, 0, 0);
+                            acc1[bn][bm] = __builtin_amdgcn_mfma_f32_16x16x4f32(bim, are[bm], acc1[bn][bm], 0, 0, 0);
+                            acc1[bn][bm] = __builtin_amdgcn_mfma_f32_16x16x4f32(bre, aim[bm], acc1[bn][bm], 0, 0, 0);
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int bn = 0; bn < 2; ++bn)
+#pragma unroll
+        for (int bm = 0; bm < 2; ++bm)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t n = n0 + wn * 32 + bn * 16 + 4 * lk + r;
+                const int64_t mm = m0 + wm * 32 + bm * 16 + li;
+                float2 v;
+                if constexpr (MODE == MODE_3M) {
+                    v.x = acc0[bn][bm][r] - acc1[bn][bm][r];
+                    v.y = acc2[bn][bm][r] - acc0[bn][bm][r] - acc1[bn][bm][r];
+                } else {
+                    v.x = acc0[bn][bm][r];
+                    v.y = acc1[bn][bm][r];
+                }
+                C[mm + M * n] = v;
             }
 }
 
@@ -486,6 +649,90 @@ qt_status permute(Ctx* c, const void* in, void* out, int dtype, const PermPlan& 
     return run_permute<float2>(c, static_cast<const float2*>(in), static_cast<float2*>(out), p);
 }
 
+namespace {
+
+__global__ void k_absmax_c64(const float2* __restrict__ x, int64_t n, uint32_t* __restrict__ out) {
+    float m = 0.0f;
+    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+        m = fmaxf(m, fmaxf(fabsf(x[i].x), fabsf(x[i].y)));
+    for (int d = 32; d >= 1; d >>= 1) m = fmaxf(m, __shfl_down(m, d, 64));
+    // non-negative floats order like their bit patterns
+    if ((threadIdx.x & 63) == 0 && m > 0.0f) atomicMax(out, __float_as_uint(m));
+}
+
+__global__ void k_rescale_c64(float2* __restrict__ x, int64_t n, const uint32_t* __restrict__ absmax,
+                              const int32_t* __restrict__ ea, const int32_t* __restrict__ eb,
+                              int32_t* __restrict__ ec) {
+    const float m = __uint_as_float(*absmax);
+    int e = 0;
+    if (m > 0.0f && isfinite(m)) (void)frexpf(m, &e);  // m = f 2^e, f in [0.5, 1)
+    const float s = ldexpf(1.0f, -e);                 // exact power of two
+    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+        x[i] = float2{x[i].x * s, x[i].y * s};
+    if (blockIdx.x == 0 && threadIdx.x == 0) *ec = (ea ? *ea : 0) + (eb ? *eb : 0) + e;
+}
+
+__global__ void k_set_u32(uint32_t* p, uint32_t v) { *p = v; }
+
+template <class V>
+__global__ void k_accumulate(const V* __restrict__ t, const int32_t* __restrict__ e, double2* __restrict__ acc) {
+    const int x = e ? *e : 0;
+    const double2 a = *acc;
+    *acc = double2{a.x + ldexp(double(t->x), x), a.y + ldexp(double(t->y), x)};
+}
+
+}  // namespace
+
+qt_status accumulate_scalar(Ctx* c, const Tensor* t, Tensor* acc) {
+    const int32_t* e = t->exp_slot >= 0 ? c->d_exp + t->exp_slot : nullptr;
+    if (t->dtype == QT_DTYPE_C128)
+        hipLaunchKernelGGL(k_accumulate<double2>, dim3(1), dim3(1), 0, c->stream,
+                           static_cast<const double2*>(t->data), e, static_cast<double2*>(acc->data));
+    else
+        hipLaunchKernelGGL(k_accumulate<float2>, dim3(1), dim3(1), 0, c->stream, static_cast<const float2*>(t->data),
+                           e, static_cast<double2*>(acc->data));
+    return check_hip(c, hipGetLastError(), "k_accumulate");
+}
+
+qt_status rescale_c64(Ctx* c, const Tensor* A, const Tensor* B, Tensor* C) {
+    if (!c->d_exp) {
+        c->exp_cap = 1 << 20;
+        qt_status s = check_hip(c, hipMalloc(reinterpret_cast<void**>(&c->d_exp), size_t(c->exp_cap) * 4), "exponents");
+        if (s == QT_OK) s = check_hip(c, hipMalloc(reinterpret_cast<void**>(&c->d_absmax), 4), "absmax");
+        if (s != QT_OK) return s;
+        for (int32_t i = c->exp_cap - 1; i >= 0; --i) c->exp_free.push_back(i);
+    }
+    if (C->exp_slot < 0) {
+        if (c->exp_free.empty()) return set_error(c, QT_OOM, "rescale: exponent slots exhausted");
+        C->exp_slot = c->exp_free.back();
+        c->exp_free.pop_back();
+    }
+    const int64_t n = C->numel;
+    float2* x = static_cast<float2*>(C->data);
+    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, c->stream, c->d_absmax, 0u);
+    const unsigned grid = grid_for(n, kThreads, 4096);
+    hipLaunchKernelGGL(k_absmax_c64, dim3(grid), dim3(kThreads), 0, c->stream, x, n, c->d_absmax);
+    hipLaunchKernelGGL(k_rescale_c64, dim3(grid), dim3(kThreads), 0, c->stream, x, n, c->d_absmax,
+                       A->exp_slot >= 0 ? c->d_exp + A->exp_slot : nullptr,
+                       B->exp_slot >= 0 ? c->d_exp + B->exp_slot : nullptr, c->d_exp + C->exp_slot);
+    return check_hip(c, hipGetLastError(), "k_rescale_c64");
+}
+
+qt_status read_exponent(Ctx* c, const Tensor* t, int32_t* e) {
+    *e = 0;
+    if (t->exp_slot < 0) return QT_OK;
+    qt_status s = check_hip(c, hipStreamSynchronize(c->stream), "sync");
+    if (s != QT_OK) return s;
+    return check_hip(c, hipMemcpy(e, c->d_exp + t->exp_slot, 4, hipMemcpyDeviceToHost), "exponent");
+}
+
+void release_exponent(Ctx* c, Tensor* t) {
+    if (t->exp_slot >= 0) c->exp_free.push_back(t->exp_slot);
+    t->exp_slot = -1;
+}
+
 qt_status launch_fill_zero(Ctx* c, void* p, size_t bytes) {
     const int64_t n16 = int64_t((bytes + 15) / 16);  // pool blocks are >= 256 B, 16 B aligned
     hipLaunchKernelGGL(k_fill_zero, dim3(grid_for(n16, 256, 8192)), dim3(256), 0, c->stream,
@@ -495,9 +742,9 @@ qt_status launch_fill_zero(Ctx* c, void* p, size_t bytes) {
 
 namespace {
 
-// Large complex128 steps: permute into GEMM layout (skipped when already there), MFMA GEMM.
-qt_status run_gemm_c128(Ctx* c, const Tensor* A, const Tensor* B, Tensor* C, const BitMap& m,
-                        int mbits) {
+// Large steps: permute into GEMM layout (skipped when already there), MFMA GEMM.
+template <class V, int MODE>
+qt_status run_gemm(Ctx* c, const Tensor* A, const Tensor* B, Tensor* C, const BitMap& m, int mbits) {
     const int nbits_n = m.nbits_c - mbits;
     const int sb = m.nbits_s;
     int src_a[64], src_b[64];
@@ -505,8 +752,8 @@ qt_status run_gemm_c128(Ctx* c, const Tensor* A, const Tensor* B, Tensor* C, con
     for (int j = 0; j < sb; ++j) src_a[mbits + j] = m.s_a[j];
     for (int j = 0; j < nbits_n; ++j) src_b[j] = m.c_src[mbits + j] - 64;
     for (int j = 0; j < sb; ++j) src_b[nbits_n + j] = m.s_b[j];
-    const double2* ap = static_cast<const double2*>(A->data);
-    const double2* bp = static_cast<const double2*>(B->data);
+    const V* ap = static_cast<const V*>(A->data);
+    const V* bp = static_cast<const V*>(B->data);
     void* abuf = nullptr;
     void* bbuf = nullptr;
     qt_status s = QT_OK;
@@ -514,8 +761,8 @@ qt_status run_gemm_c128(Ctx* c, const Tensor* A, const Tensor* B, Tensor* C, con
         hipError_t e;
         abuf = c->pool.alloc(A->bytes, &e);
         if (!abuf) return check_hip(c, e, "gemm scratch A");
-        s = permute(c, A->data, abuf, QT_DTYPE_C128, make_perm_plan(mbits + sb, src_a));
-        ap = static_cast<const double2*>(abuf);
+        s = permute(c, A->data, abuf, A->dtype, make_perm_plan(mbits + sb, src_a));
+        ap = static_cast<const V*>(abuf);
     }
     if (s == QT_OK && !is_identity(src_b, nbits_n + sb)) {
         hipError_t e;
@@ -524,17 +771,23 @@ qt_status run_gemm_c128(Ctx* c, const Tensor* A, const Tensor* B, Tensor* C, con
             c->pool.release(abuf);
             return check_hip(c, e, "gemm scratch B");
         }
-        s = permute(c, B->data, bbuf, QT_DTYPE_C128, make_perm_plan(nbits_n + sb, src_b));
-        bp = static_cast<const double2*>(bbuf);
+        s = permute(c, B->data, bbuf, B->dtype, make_perm_plan(nbits_n + sb, src_b));
+        bp = static_cast<const V*>(bbuf);
     }
     if (s == QT_OK) {
         const int64_t M = int64_t(1) << mbits, N = int64_t(1) << nbits_n, K = int64_t(1) << sb;
         const int64_t tiles = (M / GM) * (N / GN);
         const unsigned gx = unsigned(tiles < 65536 ? tiles : 65536);
         const unsigned gy = unsigned((tiles + gx - 1) / gx);
-        hipLaunchKernelGGL(k_gemm_c128, dim3(gx, gy), dim3(kThreads), 0, c->stream, ap, bp,
-                           static_cast<double2*>(C->data), M, N, K);
-        s = check_hip(c, hipGetLastError(), "k_gemm_c128");
+        if constexpr (MODE == MODE_PLAIN) {
+            hipLaunchKernelGGL(k_gemm_c128, dim3(gx, gy), dim3(kThreads), 0, c->stream, ap, bp,
+                               static_cast<double2*>(C->data), M, N, K);
+            s = check_hip(c, hipGetLastError(), "k_gemm_c128");
+        } else {
+            hipLaunchKernelGGL((k_gemm_c64<MODE>), dim3(gx, gy), dim3(kThreads), 0, c->stream, ap, bp,
+                               static_cast<float2*>(C->data), M, N, K);
+            s = check_hip(c, hipGetLastError(), "k_gemm_c64");
+        }
     }
     // stream-ordered reuse: later work on this stream may take these blocks
     c->pool.release(abuf);
@@ -557,22 +810,27 @@ qt_status dispatch_small(Ctx* c, const Tensor* A, const Tensor* B, Tensor* C, co
 
 qt_status contract_pair(Ctx* c, const Tensor* A, const Tensor* B, Tensor* C, const BitMap& m,
                         int precision) {
+    // GEMM shape: C's low bits all from A, high bits all from B
+    int mbits = 0;
+    while (mbits < m.nbits_c && m.c_src[mbits] < 64) ++mbits;
+    bool split = true;
+    for (int i = mbits; i < m.nbits_c; ++i)
+        if (m.c_src[i] < 64) split = false;
+    const int nbits_n = m.nbits_c - mbits;
+    const bool gemm = split && mbits >= 6 && nbits_n >= 6;
     if (A->dtype == QT_DTYPE_C128) {
-        // GEMM shape: C's low bits all from A, high bits all from B
-        int mbits = 0;
-        while (mbits < m.nbits_c && m.c_src[mbits] < 64) ++mbits;
-        bool split = true;
-        for (int i = mbits; i < m.nbits_c; ++i)
-            if (m.c_src[i] < 64) split = false;
-        const int nbits_n = m.nbits_c - mbits;
-        if (split && mbits >= 6 && nbits_n >= 6 && m.nbits_s >= 4)
-            return run_gemm_c128(c, A, B, C, m, mbits);
+        if (gemm && m.nbits_s >= 4) return run_gemm<double2, MODE_PLAIN>(c, A, B, C, m, mbits);
         return dispatch_small<double2, MODE_PLAIN>(c, A, B, C, m);
     }
+    const bool gemm64 = gemm && m.nbits_s >= 5;  // k-tiles of 32
     switch (precision) {
-        case QT_C64_4M: return dispatch_small<float2, MODE_4M>(c, A, B, C, m);
-        case QT_C64_3M: return dispatch_small<float2, MODE_3M>(c, A, B, C, m);
-        default: return dispatch_small<float2, MODE_BF16X3>(c, A, B, C, m);
+        case QT_C64_4M:
+            return gemm64 ? run_gemm<float2, MODE_4M>(c, A, B, C, m, mbits) : dispatch_small<float2, MODE_4M>(c, A, B, C, m);
+        case QT_C64_3M:
+            return gemm64 ? run_gemm<float2, MODE_3M>(c, A, B, C, m, mbits) : dispatch_small<float2, MODE_3M>(c, A, B, C, m);
+        default:
+            return gemm64 ? run_gemm<float2, MODE_BF16X3>(c, A, B, C, m, mbits)
+                          : dispatch_small<float2, MODE_BF16X3>(c, A, B, C, m);
     }
 }
 

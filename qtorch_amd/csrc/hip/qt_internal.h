@@ -45,6 +45,11 @@ struct Ctx {
     size_t staging_bytes = 0;
     // tensors and batches alive on this context: it is not destroyed under them
     int64_t live = 0;
+    // power-of-two exponents of rescaled complex64 tensors (value = data * 2^exp)
+    int32_t* d_exp = nullptr;
+    std::vector<int32_t> exp_free;
+    int32_t exp_cap = 0;
+    uint32_t* d_absmax = nullptr;  // scratch of one rescale
 };
 
 struct Tensor {
@@ -57,6 +62,7 @@ struct Tensor {
     int dim_log2 = 2;
     int dtype = QT_DTYPE_C128;
     size_t bytes = 0;
+    int32_t exp_slot = -1;  // slot in Ctx::d_exp once rescaled (-1: exponent 0)
 };
 
 inline size_t elem_bytes(int dtype) { return dtype == QT_DTYPE_C128 ? 16 : 8; }
@@ -98,6 +104,14 @@ qt_status launch_fill_zero(Ctx* c, void* p, size_t bytes);
 qt_status contract_pair(Ctx* c, const Tensor* A, const Tensor* B, Tensor* C, const BitMap& m,
                         int precision);
 qt_status permute(Ctx* c, const void* in, void* out, int dtype, const PermPlan& p);
+// Scales complex64 C by 2^-e so that its largest |re|, |im| lies in [0.5, 1) and
+// records exp(C) = exp(A) + exp(B) + e on the device (no host synchronisation).
+qt_status rescale_c64(Ctx* c, const Tensor* A, const Tensor* B, Tensor* C);
+// Reads a tensor's exponent (0 if never rescaled); synchronises.
+qt_status read_exponent(Ctx* c, const Tensor* t, int32_t* e);
+void release_exponent(Ctx* c, Tensor* t);
+// acc (complex128 scalar) += t (scalar, exponent applied), in FP64 on the stream.
+qt_status accumulate_scalar(Ctx* c, const Tensor* t, Tensor* acc);
 PermPlan make_perm_plan(int nbits, const int* src);
 
 }  // namespace qt

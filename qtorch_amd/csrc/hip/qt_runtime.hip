@@ -1,4 +1,5 @@
 // Context, caching allocator and tensor entry points of libqtorch_hip.so.
+#include <cmath>
 #include <cstring>
 #include <new>
 
@@ -113,6 +114,8 @@ qt_status qt_ctx_destroy(qt_ctx* c) {
         (void)hipStreamSynchronize(c->impl.stream);
         c->impl.pool.trim();
         if (c->impl.partials) (void)hipFree(c->impl.partials);
+        if (c->impl.d_exp) (void)hipFree(c->impl.d_exp);
+        if (c->impl.d_absmax) (void)hipFree(c->impl.d_absmax);
         if (c->impl.staging) (void)hipHostFree(c->impl.staging);
         (void)hipStreamDestroy(c->impl.stream);
     }
@@ -282,15 +285,37 @@ qt_status qt_tensor_read_scalar(qt_ctx* c, const qt_tensor* t, double* re, doubl
     }
     float v[2];
     qt_status s = qt_tensor_download(c, t, v, sizeof v);
-    *re = v[0];
-    *im = v[1];
+    int32_t e = 0;
+    if (s == QT_OK) {
+        std::lock_guard<std::recursive_mutex> g(c->impl.mu);
+        s = qt::read_exponent(&c->impl, &t->impl, &e);
+    }
+    *re = std::ldexp(double(v[0]), e);
+    *im = std::ldexp(double(v[1]), e);
     return s;
+}
+
+qt_status qt_tensor_accumulate(qt_ctx* c, const qt_tensor* t, qt_tensor* acc) {
+    if (!c || !t || !acc) return QT_INVALID_INPUT;
+    if (t->impl.numel != 1 || acc->impl.numel != 1 || acc->impl.dtype != QT_DTYPE_C128 || t == acc)
+        return set_error(&c->impl, QT_INVALID_INPUT, "qt_tensor_accumulate: needs two distinct scalars, acc complex128");
+    std::lock_guard<std::recursive_mutex> g(c->impl.mu);
+    (void)hipSetDevice(c->impl.device);
+    return qt::accumulate_scalar(&c->impl, &t->impl, &acc->impl);
+}
+
+qt_status qt_tensor_exponent(qt_ctx* c, const qt_tensor* t, int32_t* e) {
+    if (!c || !t || !e) return QT_INVALID_INPUT;
+    std::lock_guard<std::recursive_mutex> g(c->impl.mu);
+    (void)hipSetDevice(c->impl.device);
+    return qt::read_exponent(&c->impl, &t->impl, e);
 }
 
 qt_status qt_tensor_free(qt_ctx* c, qt_tensor* t) {
     if (!c || !t) return QT_INVALID_INPUT;
     std::lock_guard<std::recursive_mutex> g(c->impl.mu);
     --c->impl.live;
+    qt::release_exponent(&c->impl, &t->impl);
     if (t->impl.block_refs) {
         if (--*t->impl.block_refs == 0) {
             c->impl.pool.release(t->impl.block);
@@ -355,7 +380,9 @@ qt_status qt_contract_pair(qt_ctx* c, const qt_tensor* A, int rank_a, const qt_t
     if (k < 0 || k > rank_a || k > rank_b || rank_c != rank_a + rank_b - 2 * k)
         return set_error(ctx, QT_INVALID_INPUT, "qt_contract_pair: inconsistent ranks");
     const bool c128 = a.dtype == QT_DTYPE_C128;
-    if ((c128 && precision != QT_PLAIN_C128) ||
+    const bool rescale = (precision & QT_RESCALE) != 0;
+    precision &= ~QT_RESCALE;
+    if ((c128 && (precision != QT_PLAIN_C128 || rescale)) ||
         (!c128 && precision != QT_C64_4M && precision != QT_C64_3M && precision != QT_C64_BF16X3))
         return set_error(ctx, QT_INVALID_INPUT, "qt_contract_pair: precision does not fit dtype");
     const int dl = a.dim_log2;
@@ -389,6 +416,8 @@ qt_status qt_contract_pair(qt_ctx* c, const qt_tensor* A, int rank_a, const qt_t
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
     (void)hipSetDevice(ctx->device);
     qt_status s = qt::contract_pair(ctx, &a, &b, &cc, m, precision);
+    if (s == QT_OK && rescale) s = qt::rescale_c64(ctx, &a, &b, &cc);
+    else if (s == QT_OK) qt::release_exponent(ctx, &cc);  // C holds plain values
     if (s == QT_OK && cmacs) *cmacs = int64_t(1) << ((rank_c + k) * dl);
     return s;
 }

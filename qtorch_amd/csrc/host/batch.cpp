@@ -10,13 +10,13 @@
 
 namespace qtorch {
 
-namespace {
-
 void CheckQt(qt_ctx* ctx, qt_status s, const char* what) {
     if (s == QT_OK) return;
     if (s == QT_INVALID_INPUT) throw DeviceFailure(std::string(what) + ": invalid input: " + qt_last_error(ctx));
     throw DeviceFailure(std::string(what) + ": " + qt_last_error(ctx));
 }
+
+namespace {
 
 int SharedWires(const std::shared_ptr<Node>& a, const std::shared_ptr<Node>& b) {
     int n = 0;
@@ -31,6 +31,7 @@ int SharedWires(const std::shared_ptr<Node>& a, const std::shared_ptr<Node>& b) 
 // stored size least (D^rC - D^rA - D^rB), then the one with fewer MACs; ties go to
 // the lower node ids.  Scalars of disconnected components are multiplied last.
 void GreedyContract(Network& net) {
+    const int dl = net.GetDimLog2();
     while (!net.IsDone()) {
         const auto un = net.GetUncontractedNodes();
         std::shared_ptr<Node> best_a, best_b;
@@ -42,9 +43,9 @@ void GreedyContract(Network& net) {
                 if (!b || b == a || b->mContracted) continue;
                 const int k = SharedWires(a, b);
                 const int rc = a->mRank + b->mRank - 2 * k;
-                const long double cost = std::ldexp(1.0L, 2 * rc) - std::ldexp(1.0L, 2 * a->mRank) -
-                                         std::ldexp(1.0L, 2 * b->mRank);
-                const long double flops = std::ldexp(1.0L, 2 * (rc + k));
+                const long double cost = std::ldexp(1.0L, dl * rc) - std::ldexp(1.0L, dl * a->mRank) -
+                                         std::ldexp(1.0L, dl * b->mRank);
+                const long double flops = std::ldexp(1.0L, dl * (rc + k));
                 const bool better = !best_a || cost < best_cost || (cost == best_cost && flops < best_flops) ||
                                     (cost == best_cost && flops == best_flops &&
                                      std::make_pair(a->mID, b->mID) < std::make_pair(best_a->mID, best_b->mID));
@@ -145,21 +146,22 @@ std::string Fmt(double x) {
 
 }  // namespace
 
-QaoaBatch::QaoaBatch(qt_ctx* ctx, const std::string& dgf, int p, bool ref_map, int precision, int order,
-                     uint64_t seed)
-    : ctx_(ctx), data_(ExtraData::FromText(p, dgf, ref_map)), p_(p) {
-    if (!ctx) throw DeviceFailure("QaoaBatch needs a device context");
-    std::vector<double> zero(size_t(2 * p), 0.5);
-    std::vector<qt_program*> progs;
-    for (size_t e = 0; e < data_.pairs.size(); ++e) {
-        const std::string q = QaoaEdgeQasm(data_, int(e), zero), m = QaoaEdgeMeasure(data_, int(e));
+CompiledQaoa::~CompiledQaoa() {
+    for (auto* p : progs) qt_program_destroy(p);
+}
+
+CompiledQaoa CompileQaoa(const ExtraData& data, int p, int order, uint64_t seed, LeafPool& pool) {
+    CompiledQaoa out;
+    const std::vector<double> placeholder(size_t(2 * p), 0.5);  // structure only: angles are parameters
+    for (size_t e = 0; e < data.pairs.size(); ++e) {
+        const std::string q = QaoaEdgeQasm(data, int(e), placeholder), m = QaoaEdgeMeasure(data, int(e));
         NetworkOptions opt;
         Network net(q, &m, opt);
         // leaves (before ordering: contraction clears a node's data): angle-dependent
         // Rz / Rx gates read the instance's parameter block, the rest the shared pool
         std::vector<int64_t> refs;
         int n_rz = 0, n_rx = 0;
-        const int n_iter = int(data_.realIterations[e].size()), nq = data_.qubitsNeeded[e];
+        const int n_iter = int(data.realIterations[e].size()), nq = data.qubitsNeeded[e];
         for (int i = 0; i < net.GetNumOriginalNodes(); ++i) {
             const auto& node = net.GetAllNodes()[size_t(i)];
             if (node->GetTypeOfNode() == GateType::RZ) {
@@ -169,17 +171,40 @@ QaoaBatch::QaoaBatch(qt_ctx* ctx, const std::string& dgf, int p, bool ref_map, i
                 const int layer = n_rx++ / nq;
                 refs.push_back(QT_LEAF_PARAM | int64_t(32 * layer + 16));
             } else {
-                refs.push_back(pool_.Add(node->mHostVals));
+                refs.push_back(pool.Add(node->mHostVals));
             }
         }
         OrderNetwork(net, order, seed + e, nullptr);
-        edge_cmacs_.push_back(net.getNumFloatOps());
-        progs.push_back(CompileProgram(ctx, net, refs));
+        out.edge_cmacs.push_back(net.getNumFloatOps());
+        out.cmacs += net.getNumFloatOps();
+        out.progs.push_back(CompileProgram(nullptr, net, refs));
+        int64_t scratch = 0;
+        (void)qt_program_stats(out.progs.back(), nullptr, &scratch, nullptr);
+        out.max_scratch = std::max(out.max_scratch, scratch);
     }
+    return out;
+}
+
+std::vector<std::complex<double>> QaoaParamBlock(int p, const double* bg) {
+    std::vector<std::complex<double>> block(size_t(32 * p));
+    for (int i = 0; i < p; ++i) {
+        const double beta = bg[i], gamma = bg[i + p];
+        const auto rz = GateValues("Rz " + Fmt(-gamma) + " 0");     // src/maxcut.h:211
+        const auto rx = GateValues("Rx " + Fmt(beta * 2.0) + " 0");  // src/maxcut.h:217
+        std::copy(rz.begin(), rz.end(), block.begin() + 32 * i);
+        std::copy(rx.begin(), rx.end(), block.begin() + 32 * i + 16);
+    }
+    return block;
+}
+
+QaoaBatch::QaoaBatch(qt_ctx* ctx, const std::string& dgf, int p, bool ref_map, int precision, int order,
+                     uint64_t seed)
+    : ctx_(ctx), data_(ExtraData::FromText(p, dgf, ref_map)), p_(p) {
+    if (!ctx) throw DeviceFailure("QaoaBatch needs a device context");
+    CompiledQaoa c = CompileQaoa(data_, p, order, seed, pool_);
+    edge_cmacs_ = c.edge_cmacs;
     const_elems_ = int64_t(pool_.Data().size());
-    qt_status s = qt_batch_create(ctx, int(progs.size()), progs.data(), precision, &batch_);
-    for (auto* pr : progs) qt_program_destroy(pr);
-    CheckQt(ctx, s, "qt_batch_create");
+    CheckQt(ctx, qt_batch_create(ctx, int(c.progs.size()), c.progs.data(), precision, &batch_), "qt_batch_create");
 }
 
 QaoaBatch::~QaoaBatch() {
@@ -191,16 +216,10 @@ void QaoaBatch::SetAngles(int n_sets, const double* bg) {
     n_sets_ = n_sets;
     std::vector<std::complex<double>> pool = pool_.Data();
     const int64_t block = 32 * int64_t(p_);
-    pool.resize(size_t(const_elems_ + block * n_sets));
+    pool.reserve(size_t(const_elems_ + block * n_sets));
     for (int a = 0; a < n_sets; ++a) {
-        const double* ang = bg + size_t(2 * p_) * size_t(a);
-        for (int i = 0; i < p_; ++i) {
-            const double beta = ang[i], gamma = ang[i + p_];
-            const auto rz = GateValues("Rz " + Fmt(-gamma) + " 0");  // src/maxcut.h:211
-            const auto rx = GateValues("Rx " + Fmt(beta * 2.0) + " 0");  // src/maxcut.h:217
-            std::copy(rz.begin(), rz.end(), pool.begin() + long(const_elems_ + block * a + 32 * i));
-            std::copy(rx.begin(), rx.end(), pool.begin() + long(const_elems_ + block * a + 32 * i + 16));
-        }
+        const auto b = QaoaParamBlock(p_, bg + size_t(2 * p_) * size_t(a));
+        pool.insert(pool.end(), b.begin(), b.end());
     }
     CheckQt(ctx_, qt_batch_set_leaves(batch_, reinterpret_cast<const double*>(pool.data()), int64_t(pool.size())),
             "qt_batch_set_leaves");

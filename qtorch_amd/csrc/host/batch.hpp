@@ -27,8 +27,35 @@ private:
     std::unordered_map<std::string, int64_t> index_;
 };
 
+// Engine status -> qtorch exception (DeviceFailure); ctx may be NULL.
+void CheckQt(qt_ctx* ctx, qt_status s, const char* what);
+
 // Device program of a contracted (recorded) network; leaf i is original node i.
+// ctx may be NULL (compilation is host work).
 qt_program* CompileProgram(qt_ctx* ctx, const Network& net, const std::vector<int64_t>& leaf_refs);
+
+// F_p's edge light cones (src/maxcut.cpp:171-198) compiled once: one program per
+// edge, constant leaves added to `pool` (references from 0), the Rz / Rx gates of
+// layer i read parameter elements 32 i and 32 i + 16.  Throws DeviceFailure when a
+// light cone's tensors exceed a program's limit.
+struct CompiledQaoa {
+    CompiledQaoa() = default;
+    CompiledQaoa(CompiledQaoa&& o) noexcept
+        : progs(std::move(o.progs)), edge_cmacs(std::move(o.edge_cmacs)), cmacs(o.cmacs), max_scratch(o.max_scratch) {
+        o.progs.clear();
+    }
+    CompiledQaoa(const CompiledQaoa&) = delete;
+    CompiledQaoa& operator=(const CompiledQaoa&) = delete;
+    ~CompiledQaoa();
+    std::vector<qt_program*> progs;
+    std::vector<int64_t> edge_cmacs;
+    int64_t cmacs = 0;
+    int64_t max_scratch = 0;
+};
+CompiledQaoa CompileQaoa(const ExtraData& data, int p, int order, uint64_t seed, LeafPool& pool);
+// The 32 p parameter elements of one angle set [betas..., gammas...]: Rz(-gamma_i)
+// at 32 i and Rx(2 beta_i) at 32 i + 16, angles printed as the reference prints them.
+std::vector<std::complex<double>> QaoaParamBlock(int p, const double* betas_gammas);
 
 // F_p for many angle sets at once: one program per edge light cone, the Rz / Rx
 // tensors of each angle set in a parameter block, FP64 energy sums on the device.

@@ -6,6 +6,8 @@
 
 #include "batch.hpp"
 #include "maxcut_driver.hpp"
+#include "serve.hpp"
+#include "sliced.hpp"
 #include "qtorch.hpp"
 #include "qtorch_host.h"
 
@@ -246,6 +248,17 @@ int qth_plan(qth_network* h, int32_t* steps, int cap, int* n) {
     return QTH_OK;
 }
 
+int qth_program_stats(qth_network* h, int64_t* cmacs, int64_t* scratch_elems, int* n_steps) {
+    if (!h) return QTH_INVALID_FUNCTION_INPUT;
+    return Guard([&] {
+        const std::vector<int64_t> refs(static_cast<size_t>(h->net->GetNumOriginalNodes()), 0);
+        qt_program* p = CompileProgram(nullptr, *h->net, refs);
+        const qt_status s = qt_program_stats(p, cmacs, scratch_elems, n_steps);
+        qt_program_destroy(p);
+        if (s != QT_OK) throw InvalidFunctionInput();
+    });
+}
+
 int qth_qaoa_num_edges(const char* dgf, int* n_edges, int* n_vertices) {
     if (!dgf) return QTH_INVALID_FUNCTION_INPUT;
     return Guard([&] {
@@ -380,6 +393,177 @@ int qth_netbatch_cmacs(qth_netbatch* h, int64_t* cmacs) {
 
 int qth_netbatch_destroy(qth_netbatch* h) {
     delete h;
+    return QTH_OK;
+}
+
+// ---- serving ---------------------------------------------------------------------
+
+struct qth_server {
+    std::unique_ptr<Server> s;
+};
+
+int qth_server_create(int device, int slots, int precision, int64_t hbm_budget, int max_requests_per_launch,
+                      qth_server** out) {
+    if (!out) return QTH_INVALID_FUNCTION_INPUT;
+    *out = nullptr;
+    return Guard([&] {
+        ServeOptions o;
+        o.device = device;
+        o.slots = slots;
+        o.precision = precision;
+        o.hbm_budget = hbm_budget;
+        o.max_requests_per_launch = max_requests_per_launch;
+        auto h = std::make_unique<qth_server>();
+        h->s = std::make_unique<Server>(o);
+        *out = h.release();
+    });
+}
+
+int qth_server_destroy(qth_server* s) {
+    if (!s) return QTH_INVALID_FUNCTION_INPUT;
+    return Guard([&] { delete s; });
+}
+
+int qth_server_submit_qaoa(qth_server* s, const char* dgf, int p, int ref_map, const double* bg, int64_t* id) {
+    if (!s || !dgf || !bg || !id || p < 1) return QTH_INVALID_FUNCTION_INPUT;
+    return Guard([&] {
+        ServeRequest r;
+        r.kind = kServeQaoaEnergy;
+        r.text = dgf;
+        r.p = p;
+        r.ref_map = ref_map != 0;
+        r.angles.assign(bg, bg + 2 * p);
+        *id = s->s->Submit(std::move(r));
+    });
+}
+
+int qth_server_submit_amplitude(qth_server* s, const char* qasm, const char* bits, int64_t* id) {
+    if (!s || !qasm || !bits || !id) return QTH_INVALID_FUNCTION_INPUT;
+    return Guard([&] {
+        ServeRequest r;
+        r.kind = kServeAmplitude;
+        r.text = qasm;
+        r.bits = bits;
+        *id = s->s->Submit(std::move(r));
+    });
+}
+
+int qth_server_poll(qth_server* s, int64_t id, int* done, int* status, double* re, double* im, double* latency_ms,
+                    int* route) {
+    if (!s || !done) return QTH_INVALID_FUNCTION_INPUT;
+    return Guard([&] {
+        ServeResult r;
+        *done = s->s->TryGet(id, &r) ? 1 : 0;
+        if (!*done) return;
+        if (status) *status = r.status;
+        if (re) *re = r.re;
+        if (im) *im = r.im;
+        if (latency_ms) *latency_ms = r.latency_ms;
+        if (route) *route = r.route;
+    });
+}
+
+int qth_server_wait(qth_server* s, int64_t id, double* re, double* im, double* latency_ms, int* route) {
+    if (!s) return QTH_INVALID_FUNCTION_INPUT;
+    ServeResult r;
+    const int rc = Guard([&] { r = s->s->Wait(id); });
+    if (rc != QTH_OK) return rc;
+    if (re) *re = r.re;
+    if (im) *im = r.im;
+    if (latency_ms) *latency_ms = r.latency_ms;
+    if (route) *route = r.route;
+    return r.status;
+}
+
+int qth_server_drain(qth_server* s) {
+    if (!s) return QTH_INVALID_FUNCTION_INPUT;
+    return Guard([&] { s->s->Drain(); });
+}
+
+int qth_server_stats(qth_server* s, int64_t* counts, double* ms, int64_t* peak) {
+    if (!s) return QTH_INVALID_FUNCTION_INPUT;
+    return Guard([&] {
+        const ServeStats st = s->s->Stats();
+        if (counts) {
+            counts[0] = st.submitted;
+            counts[1] = st.completed;
+            counts[2] = st.rejected;
+            counts[3] = st.launches;
+            counts[4] = st.edge_evals;
+        }
+        if (ms) {
+            ms[0] = st.p50_ms;
+            ms[1] = st.p99_ms;
+            ms[2] = st.mean_ms;
+            ms[3] = st.overlap_ms;
+            ms[4] = st.busy_ms;
+        }
+        if (peak) *peak = st.peak_planned_bytes;
+    });
+}
+
+int qth_serve_plan(int kind, const char* text, int p, int ref_map, const char* bits, int precision, int* route,
+                   int64_t* planned_bytes, int64_t* cmacs) {
+    if (!text) return QTH_INVALID_FUNCTION_INPUT;
+    return Guard([&] {
+        ServeRequest r;
+        r.kind = kind;
+        r.text = text;
+        r.p = p;
+        r.ref_map = ref_map != 0;
+        if (bits) r.bits = bits;
+        ServeOptions o;
+        o.precision = precision;
+        const ServePlan pl = PlanRequest(r, o);
+        if (route) *route = pl.route;
+        if (planned_bytes) *planned_bytes = pl.planned_bytes;
+        if (cmacs) *cmacs = pl.cmacs;
+    });
+}
+
+// ---- sliced networks ------------------------------------------------------------
+
+struct qth_sliced {
+    std::unique_ptr<SlicedNetwork> n;
+};
+
+int qth_sliced_create(const char* qasm, const char* measure, int dim_log2, int order, uint64_t seed, int max_bits,
+                      qth_sliced** out) {
+    if (!qasm || !measure || !out) return QTH_INVALID_FUNCTION_INPUT;
+    *out = nullptr;
+    return Guard([&] {
+        auto h = std::make_unique<qth_sliced>();
+        h->n = std::make_unique<SlicedNetwork>(qasm, measure, dim_log2, order, seed, max_bits);
+        *out = h.release();
+    });
+}
+
+int qth_sliced_plan(qth_sliced* s, int64_t* v) {
+    if (!s || !v) return QTH_INVALID_FUNCTION_INPUT;
+    const SliceStats& p = s->n->Plan();
+    v[0] = p.sliced_wires;
+    v[1] = p.peak_bits;
+    v[2] = p.slices;
+    v[3] = p.cmacs_per_slice;
+    v[4] = p.gemm_cmacs_per_slice;
+    v[5] = p.peak_bytes;
+    return QTH_OK;
+}
+
+int qth_sliced_run(qth_sliced* s, qt_ctx* ctx, int precision, double* re, double* im, double* ms) {
+    if (!s || !ctx) return QTH_INVALID_FUNCTION_INPUT;
+    return Guard([&] {
+        SliceStats st;
+        const auto v = s->n->Run(ctx, precision, &st);
+        if (re) *re = v.real();
+        if (im) *im = v.imag();
+        if (ms) *ms = st.ms;
+    });
+}
+
+int qth_sliced_destroy(qth_sliced* s) {
+    if (!s) return QTH_INVALID_FUNCTION_INPUT;
+    delete s;
     return QTH_OK;
 }
 

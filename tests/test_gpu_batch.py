@@ -158,15 +158,20 @@ def test_executor_shapes_agree_bitwise(ctx, precision):
     b.set_angles(sets)
     ref = None
     for lanes in (64, 4, 8, 16, 32, 256):
-        for wgs in (1, 3):
-            b.config(lanes, wgs)
+        for wgs, place in ((1, qa.QT_SCRATCH_LDS), (3, qa.QT_SCRATCH_LDS), (2, qa.QT_SCRATCH_GLOBAL)):
+            try:
+                b.config(lanes, wgs, place)
+            except qa.InvalidFunctionInput:  # this shape's intermediates do not fit in LDS
+                assert place == qa.QT_SCRATCH_LDS
+                continue
+            assert b.info()["global_scratch"] == (place == qa.QT_SCRATCH_GLOBAL)
             b.launch()
             e, zz = b.results()
             if ref is None:
                 ref = (e.copy(), zz.copy())
             else:
-                assert np.array_equal(zz, ref[1]), (lanes, wgs)
-                assert np.array_equal(e, ref[0]), (lanes, wgs)
+                assert np.array_equal(zz, ref[1]), (lanes, wgs, place)
+                assert np.array_equal(e, ref[0]), (lanes, wgs, place)
 
 
 def test_p2_light_cones_batched(ctx):

@@ -86,9 +86,12 @@ def test_contract_pair_plain_vs_oracle(ctx, ra, rb, k, d):
 
 
 @pytest.mark.parametrize("prec,tol", [(qa.QT_C64_4M, 2e-6), (qa.QT_C64_3M, 4e-6), (qa.QT_C64_BF16X3, 1e-4)])
-@pytest.mark.parametrize("ra,rb,k,d", [(4, 4, 2, 2), (7, 7, 7, 2), (6, 6, 3, 2), (10, 10, 5, 1)])
+@pytest.mark.parametrize("ra,rb,k,d", [(4, 4, 2, 2), (7, 7, 7, 2), (6, 6, 3, 2), (10, 10, 5, 1),
+                                      (12, 12, 6, 1), (14, 13, 7, 1), (8, 8, 3, 2), (9, 9, 4, 2)])
 def test_contract_pair_c64_forms(ctx, prec, tol, ra, rb, k, d):
-    """complex64 forms against the FP64 oracle: error relative to the largest |C|."""
+    """complex64 forms against the FP64 oracle: error relative to the largest |C|.
+    Shapes with 64+ free elements on each side and 32+ summed run on the MFMA GEMM
+    (FP32 MFMA for 4M / 3M, bf16 MFMA for bf16x3), the rest on the vector kernels."""
     rng = np.random.default_rng(7 + ra + rb)
     pairs, free = _case(rng, ra, rb, k)
     a = _rand(rng, 1 << (d * ra), np.complex64)
@@ -100,6 +103,52 @@ def test_contract_pair_c64_forms(ctx, prec, tol, ra, rb, k, d):
     ref = qo.contract_indices(a.astype(np.complex128), ra, b.astype(np.complex128), rb, free, pairs, d)
     scale = np.abs(ref).max() * np.sqrt(1 << (d * k))
     assert np.abs(got - ref).max() <= tol * scale
+
+
+def test_rescaled_chain_does_not_underflow(ctx):
+    """A chain of contractions whose values shrink by 2^-30 per factor: plain complex64
+    underflows to zero, while per-step power-of-two rescaling keeps the FP64 value to
+    FP32 accuracy (exponents carried on the device, applied when the scalar is read)."""
+    rng = np.random.default_rng(3)
+    scale = 2.0 ** -30
+    v0 = (_rand(rng, 2) * scale).astype(np.complex64)
+    mats = [(_rand(rng, 4) * scale).astype(np.complex64) for _ in range(6)]
+    u = (_rand(rng, 2) * scale).astype(np.complex64)
+    ref = v0.astype(np.complex128)
+    for m in mats:  # t'[j] = sum_i t[i] M[i + 2 j]
+        ref = ref @ m.astype(np.complex128).reshape(2, 2, order="F")
+    ref = complex(ref @ u.astype(np.complex128))
+    assert abs(ref) < 2.0 ** -200
+    for rescale in (True, False):
+        prec = qa.QT_C64_4M | (qa.QT_RESCALE if rescale else 0)
+        t = qa.Tensor(ctx, 1, 1, qa.QT_DTYPE_C64).upload(v0)
+        for m in mats:
+            tm = qa.Tensor(ctx, 2, 1, qa.QT_DTYPE_C64).upload(m)
+            t, _ = qa.contract_pair(ctx, t, tm, [(0, 0)], [(False, 1)], prec)
+        tu = qa.Tensor(ctx, 1, 1, qa.QT_DTYPE_C64).upload(u)
+        s_, _ = qa.contract_pair(ctx, t, tu, [(0, 0)], [], prec)
+        got = s_.scalar()
+        if rescale:
+            assert abs(got - ref) <= 1e-5 * abs(ref), (got, ref)
+            assert s_.exponent() < -200
+        else:
+            assert got == 0
+
+
+def test_device_fp64_accumulation(ctx):
+    """Rank-0 results of any form are summed in FP64 on the device."""
+    rng = np.random.default_rng(4)
+    acc = qa.Tensor(ctx, 0, 2, qa.QT_DTYPE_C128)
+    total = 0j
+    for i in range(5):
+        a = _rand(rng, 16, np.complex64)
+        b = _rand(rng, 16, np.complex64)
+        ta = qa.Tensor(ctx, 2, 2, qa.QT_DTYPE_C64).upload(a)
+        tb = qa.Tensor(ctx, 2, 2, qa.QT_DTYPE_C64).upload(b)
+        ts, _ = qa.contract_pair(ctx, ta, tb, [(0, 1), (1, 0)], [], qa.QT_C64_4M | qa.QT_RESCALE)
+        ts.accumulate_into(acc)
+        total += ts.scalar()
+    assert abs(acc.scalar() - total) <= 1e-12 * abs(total)
 
 
 def test_invalid_arguments_are_rejected(ctx):

@@ -1,0 +1,173 @@
+"""GPU tests of the serving loop (qth_server_*): continuous batching of QAOA-energy
+and single-amplitude requests.  Results must not depend on what else shares a
+launch: each request's value in a mixed stream is compared bit for bit with the
+same request served alone, and against independent references (the drop-in
+complex128 path, a state-vector simulation)."""
+import math
+import os
+import random
+import threading
+import time
+
+import numpy as np
+import pytest
+
+import qtorch_amd as qa
+from qtorch_amd import workloads as wl
+from tests import statevector as sv
+
+pytestmark = pytest.mark.gpu
+GOLDEN_DIR = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _graph():
+    with open(os.path.join(GOLDEN_DIR, "graphs", "3reg_n30_s2024.dgf")) as f:
+        return f.read()
+
+
+def _amp_index(bits):
+    return sum(int(b) << q for q, b in enumerate(bits))
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = qa.Context(0)
+    yield c
+    c.close()
+
+
+def _mix(seed):
+    """A seeded mix: QAOA energies (3- and 4-regular graphs, p = 1, 2) and amplitudes
+    of small grid circuits and QFTs; every request appears in arrival order."""
+    rng = random.Random(seed)
+    reqs = []
+    graphs = [(wl.dgf_text(wl.regular_graph(n, d, seed=100 + n + d)), d) for n, d in ((20, 3), (24, 4), (30, 3))]
+    circuits = [(wl.grid_random_circuit(3, 4, 8, seed=5), 12), (wl.qft_qasm(8), 8),
+                (wl.grid_random_circuit(4, 4, 8, seed=6), 16)]
+    for i in range(48):
+        if rng.random() < 0.7:
+            g, d = rng.choice(graphs)
+            p = rng.choice([1, 1, 2]) if d == 3 else 1
+            ang = wl.qaoa_angles(p, rng.randrange(1 << 20))
+            reqs.append(("qaoa", g, p, ang))
+        else:
+            q, n = rng.choice(circuits)
+            reqs.append(("amp", q, wl.bitstring(n, rng.randrange(1 << 20))))
+    return reqs
+
+
+def _submit(srv, r):
+    if r[0] == "qaoa":
+        return srv.submit_qaoa(r[1], r[2], r[3])
+    return srv.submit_amplitude(r[1], r[2])
+
+
+def test_request_alone_equals_request_in_mix():
+    """Every request of a mixed stream (submitted from another thread while launches
+    are in flight) returns exactly the value it returns when served alone."""
+    reqs = _mix(7)
+    alone = []
+    srv = qa.Server(slots=2, precision=qa.QT_C64_4M)
+    try:
+        for r in reqs:
+            res = srv.wait(_submit(srv, r))
+            assert res["status"] == 0, res
+            alone.append(res["value"])
+    finally:
+        srv.close()
+    srv = qa.Server(slots=2, precision=qa.QT_C64_4M)
+    try:
+        ids = [None] * len(reqs)
+        order = list(range(len(reqs)))
+        random.Random(3).shuffle(order)
+
+        def producer():
+            for k in order:
+                ids[k] = _submit(srv, reqs[k])
+                time.sleep(0.0005)
+
+        t = threading.Thread(target=producer)
+        t.start()
+        t.join()
+        got = [srv.wait(i) for i in ids]
+        st = srv.stats()
+    finally:
+        srv.close()
+    for k, (a, g) in enumerate(zip(alone, got)):
+        assert g["status"] == 0, g
+        assert g["value"] == a, (k, reqs[k][0], a, g)
+    assert st["completed"] == len(reqs) and st["launches"] >= 2
+    assert st["p99_ms"] >= st["p50_ms"] > 0
+
+
+def test_served_values_match_references(ctx):
+    """QAOA energies against the drop-in complex128 path (1e-5 relative); amplitudes
+    against a state-vector simulation (1e-5 of the state's scale)."""
+    dgf = _graph()
+    srv = qa.Server(slots=2, precision=qa.QT_C64_4M)
+    try:
+        checks = []
+        for s in range(4):
+            ang = wl.qaoa_angles(1, 40 + s)
+            checks.append(("qaoa", srv.submit_qaoa(dgf, 1, ang), ang))
+        for s, (r, c) in enumerate([(3, 4), (4, 4)]):
+            q = wl.grid_random_circuit(r, c, 8, seed=20 + s)
+            bits = wl.bitstring(r * c, 9 + s)
+            checks.append(("amp", srv.submit_amplitude(q, bits), (q, bits)))
+        for kind, rid, arg in checks:
+            res = srv.wait(rid)
+            assert res["status"] == 0, res
+            if kind == "qaoa":
+                e, _ = qa.qaoa_energy_plain(ctx, dgf, 1, False, list(arg), seed=1)
+                assert abs(res["value"].real - e) <= 1e-5 * abs(e)
+            else:
+                q, bits = arg
+                psi, n = sv.simulate(q)
+                ref = psi[_amp_index(bits)]
+                assert abs(res["value"] - ref) <= 1e-5 * max(abs(ref), 2.0 ** (-n / 2)), (res, ref)
+    finally:
+        srv.close()
+
+
+def test_large_network_takes_the_drop_in_path():
+    """A 20-qubit depth-14 grid amplitude has tensors beyond a program's limit: it runs
+    on the complex128 path (matching a state-vector simulation to 1e-10) while small
+    requests keep flowing, and its planned bytes are counted at admission."""
+    q = wl.grid_random_circuit(4, 5, 14, seed=1)
+    bits = wl.bitstring(20, 3)
+    plan = qa.serve_plan(qa.SERVE_AMPLITUDE, q, bits=bits)
+    assert plan["route"] == "eager"
+    srv = qa.Server(slots=2)
+    try:
+        big = srv.submit_amplitude(q, bits)
+        small = [srv.submit_qaoa(_graph(), 1, wl.qaoa_angles(1, s)) for s in range(8)]
+        res = srv.wait(big)
+        for i in small:
+            assert srv.wait(i)["status"] == 0
+        st = srv.stats()
+    finally:
+        srv.close()
+    assert res["status"] == 0 and res["route"] == "eager"
+    psi, n = sv.simulate(q)
+    ref = psi[_amp_index(bits)]
+    assert abs(res["value"] - ref) <= 1e-10 * max(abs(ref), 2.0 ** (-n / 2))
+    assert st["peak_planned_bytes"] >= plan["planned_bytes"]
+
+
+def test_admission_rejects_what_cannot_fit():
+    """A request whose planned device bytes exceed the whole budget is refused
+    (QTH_OTHER) without touching the device; others still run."""
+    q = wl.grid_random_circuit(4, 5, 14, seed=1)
+    bits = wl.bitstring(20, 3)
+    plan = qa.serve_plan(qa.SERVE_AMPLITUDE, q, bits=bits)
+    srv = qa.Server(slots=1, hbm_budget=plan["planned_bytes"] // 2)
+    try:
+        r1 = srv.wait(srv.submit_amplitude(q, bits))
+        r2 = srv.wait(srv.submit_qaoa(_graph(), 1, wl.qaoa_angles(1, 1)))
+        bad = srv.wait(srv.submit_amplitude(q, bits[:-1] + "2"))
+        st = srv.stats()
+    finally:
+        srv.close()
+    assert r1["status"] == 9 and r2["status"] == 0
+    assert bad["status"] != 0
+    assert st["rejected"] == 2

@@ -184,3 +184,73 @@ def test_disconnected_network_final_value_rule():
     rec = net.record()
     onet = qo.Network("2\nH 0\nH 1\n", "0 0")
     assert abs(onet.contract_given_sequence(rec) - 0.25) < 1e-15
+
+
+def test_program_schedule_keeps_scratch_small():
+    """Compiling a plan into a device program schedules its steps (same tree, same
+    per-step arithmetic) to keep few intermediates live: the executor's scratch is
+    at least the live set of the worst step under any order and, for the p = 1
+    light cones, at most a few of the largest intermediates (peak rank 3 = 64
+    elements); the MAC count is the plan's."""
+    with open(os.path.join(GOLDEN_DIR, "graphs", "3reg_n30_s2024.dgf")) as f:
+        dgf = f.read()
+    ne, _ = qa.qaoa_num_edges(dgf)
+    for e in range(ne):
+        q, m = qa.qaoa_edge_text(dgf, 1, True, e, [0.3, 0.7])
+        net = qa.Network(q, m, None)
+        n_orig = net.counts()["original"]
+        net.order(qa.ORDER_GREEDY, 0)
+        plan = net.plan()
+        st = net.program_stats()
+        assert st["cmacs"] == net.flops() and st["n_steps"] == len(plan)
+        lower = 0
+        for i, (a, b, c, ra, rb, k, rc, _) in enumerate(plan):
+            live = (4 ** rc if i + 1 < len(plan) else 0) + sum(4 ** r for op, r in ((a, ra), (b, rb)) if op >= n_orig)
+            lower = max(lower, live)
+        assert lower <= st["scratch_elems"] <= 256, (e, lower, st)
+        net.close()
+
+
+def test_serving_plan_routes_and_bytes():
+    """Serving plans each request on the host: small light cones run in lane-grouped
+    waves, larger programs in workgroup executors, networks with tensors beyond a
+    program's 2^16 elements on the drop-in path, whose planned bytes cover at least
+    its largest intermediate (complex128)."""
+    from qtorch_amd import workloads as wl
+    with open(os.path.join(GOLDEN_DIR, "graphs", "3reg_n30_s2024.dgf")) as f:
+        dgf = f.read()
+    p1 = qa.serve_plan(qa.SERVE_QAOA_ENERGY, dgf, 1)
+    assert p1["route"] == "small" and p1["cmacs"] == 45 * 3840
+    assert qa.serve_plan(qa.SERVE_QAOA_ENERGY, dgf, 2)["route"] == "medium"
+    assert qa.serve_plan(qa.SERVE_AMPLITUDE, wl.qft_qasm(8), bits="01100101")["route"] == "small"
+    q = wl.grid_random_circuit(4, 5, 14, seed=1)
+    bits = wl.bitstring(20, 3)
+    big = qa.serve_plan(qa.SERVE_AMPLITUDE, q, bits=bits)
+    net = qa.Network(q, bits, None, dim_log2=1)
+    net.order(qa.ORDER_GREEDY, 0)
+    peak_rank = int(net.plan()[:, 6].max())
+    assert peak_rank > 16 and big["route"] == "eager"
+    assert big["planned_bytes"] >= 16 * 2 ** peak_rank and big["cmacs"] == net.flops()
+    with pytest.raises(qa.InvalidFunctionInput):
+        qa.serve_plan(7, dgf, 1)
+    with pytest.raises(qa.InvalidFunctionInput):
+        qa.serve_plan(qa.SERVE_AMPLITUDE, wl.qft_qasm(4), bits="01x1")
+
+
+def test_sliced_plan_bounds():
+    """Slicing keeps every tensor of a slice within max_bits index bits; D^s slices of
+    one shape; without slicing the per-slice MAC count is the plan's own."""
+    from qtorch_amd import workloads as wl
+    q = wl.grid_random_circuit(4, 5, 8, seed=2)
+    bits = wl.bitstring(20, 8)
+    whole = qa.SlicedNetwork(q, bits, dim_log2=1, order=qa.ORDER_LINEGRAPH, max_bits=40).plan()
+    net = qa.Network(q, bits, None, dim_log2=1)
+    net.order(qa.ORDER_LINEGRAPH, 0)
+    assert whole["sliced_wires"] == 0 and whole["cmacs_per_slice"] == net.flops()
+    assert whole["peak_bits"] == int(net.plan()[:, 6].max()) or whole["peak_bits"] >= int(net.plan()[:, 6].max())
+    for mb in (whole["peak_bits"] - 2, 10, 8):
+        p = qa.SlicedNetwork(q, bits, dim_log2=1, order=qa.ORDER_LINEGRAPH, max_bits=mb).plan()
+        assert p["peak_bits"] <= mb and p["slices"] == 2 ** p["sliced_wires"] and p["sliced_wires"] >= 1
+        assert p["cmacs_per_slice"] * p["slices"] >= whole["cmacs_per_slice"] // 4
+    with pytest.raises(qa.InvalidFunctionInput):
+        qa.SlicedNetwork(q, bits, dim_log2=3, max_bits=8)
