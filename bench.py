@@ -23,6 +23,7 @@ import argparse
 import json
 import math
 import os
+import random
 import subprocess
 import sys
 import tempfile
@@ -85,23 +86,259 @@ def golden_reference(golden_path: str):
     return np.array([c["value"][0] for c in ref]), ref[0]["betas_gammas"]
 
 
+# ---- the other configurations (--workload) ----------------------------------------------
+
+def _form_name(qa, prec):
+    return {qa.QT_PLAIN_C128: "c128", qa.QT_C64_4M: "c64_4m", qa.QT_C64_3M: "c64_3m", qa.QT_C64_BF16X3: "c64_bf16x3"}[prec]
+
+
+def run_qft8(args, qa, np):
+    """QFT on |0^8>: one amplitude's probability in the line-graph (min-fill) order on the
+    drop-in complex128 path, and the full 256-entry probability vector as one batch of
+    256 measured networks; every entry is 1/256."""
+    from qtorch_amd import workloads as wl
+    ctx = qa.Context(0)
+    q = wl.qft_qasm(8)
+    bits = wl.bitstring(8, args.seed)
+    net = qa.Network(q, wl.measure_text(bits), ctx)
+    order = net.linegraph_minfill()
+    t0 = time.perf_counter()
+    p_one = net.linegraph_contract(order)
+    t_one = time.perf_counter() - t0
+    strings = [format(i, "08b")[::-1] for i in range(256)]
+    nb = qa.NetBatch(ctx, [q] * 256, [wl.measure_text(b) for b in strings], precision=qa.QT_PLAIN_C128)
+    nb.autotune()
+    nb.launch()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        nb.launch()
+    ctx.sync()
+    t_vec = (time.perf_counter() - t0) / args.steps
+    probs = nb.results().real
+    return {"metric": "QFT8 probability vector (256 entries) per second", "value": 1.0 / t_vec,
+            "unit": "vectors/s", "n_gpus": 1, "steps": args.steps, "warmup": 1, "ms_per_step": 1e3 * t_vec,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "c128", "data": "synthetic",
+            "config": {"workload": "qft8_probability_vector", "networks_per_step": 256, "measure_bits": bits},
+            "one_amplitude": {"order": "line-graph min-fill", "probability": p_one.real,
+                              "abs_err_vs_1_256": abs(p_one - 1 / 256), "ms": 1e3 * t_one},
+            "vector": {"max_abs_err_vs_1_256": float(np.max(np.abs(probs - 1 / 256))), "sum": float(probs.sum())}}
+
+
+def run_grid(args, qa, np):
+    """Precision stress: one amplitude of seeded grid random circuits (RX/RY/RZ layers,
+    CZ on alternating grid-edge patterns) in the amplitude picture, line-graph order,
+    sliced to at most --max-bits index bits per tensor; every form against complex128."""
+    from qtorch_amd import workloads as wl
+    ctx = qa.Context(0)
+    cases = [(6, 8, 8), (4, 6, 14)]
+    out = []
+    for r, c, d in cases:
+        q = wl.grid_random_circuit(r, c, d, seed=2024)
+        bits = wl.bitstring(r * c, args.seed)
+        t0 = time.perf_counter()
+        sn = qa.SlicedNetwork(q, bits, dim_log2=1, order=qa.ORDER_LINEGRAPH, max_bits=args.max_bits)
+        plan_s = time.perf_counter() - t0
+        plan = sn.plan()
+        res = {}
+        ref = None
+        for prec in (qa.QT_PLAIN_C128, qa.QT_C64_4M, qa.QT_C64_3M, qa.QT_C64_BF16X3, qa.QT_C64_4M | qa.QT_RESCALE):
+            sn.run(ctx, prec)  # warm the allocator
+            v, ms = sn.run(ctx, prec)
+            if prec == qa.QT_PLAIN_C128:
+                ref = v
+            flops = 8.0 * plan["cmacs_per_slice"] * plan["slices"]
+            name = _form_name(qa, prec & ~qa.QT_RESCALE) + ("_rescaled" if prec & qa.QT_RESCALE else "")
+            res[name] = {"amplitude": [v.real, v.imag], "ms": ms, "tflops": flops / (ms * 1e-3) / 1e12,
+                         "rel_err_vs_c128": abs(v - ref) / abs(ref)}
+        out.append({"circuit": f"grid_{r}x{c}_depth{d}", "qubits": r * c, "bits": bits, "plan": plan,
+                    "plan_s": plan_s, "forms": res})
+    best = max(out[0]["forms"].items(), key=lambda kv: kv[1]["tflops"])
+    return {"metric": "grid amplitude contraction TFLOP/s by form (error vs complex128)",
+            "value": best[1]["tflops"], "unit": "TFLOP/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+            "ms_per_step": best[1]["ms"], "higher_is_better": True, "scaling": "strong", "vs_baseline": None,
+            "dtype": best[0], "data": "synthetic",
+            "config": {"workload": "grid_random_circuit_amplitudes", "max_bits": args.max_bits,
+                       "order": "line-graph min-fill", "cases": [o["circuit"] for o in out]},
+            "cases": out}
+
+
+def run_uneven(args, qa, np):
+    """512 QAOA p=2 edge light-cone networks from seeded 4-regular graphs of 20-120
+    vertices (log-uniform): networks whose tensors fit a device program (<= 2^16
+    elements) run in one persistent, graph-captured, autotuned launch; the larger ones
+    step by step with MFMA GEMMs; all in complex64 4M against the same networks
+    launched one pairwise contraction at a time in complex128."""
+    from qtorch_amd import workloads as wl
+    rng = random.Random(args.seed)
+    ctx = qa.Context(0)
+    nets = []
+    g = 0
+    while len(nets) < args.networks:
+        n = int(round(math.exp(rng.uniform(math.log(20), math.log(120)))))
+        n += n % 2
+        dgf = wl.dgf_text(wl.regular_graph(n, 4, seed=7000 + g))
+        g += 1
+        ne, _ = qa.qaoa_num_edges(dgf)
+        ang = wl.qaoa_angles(2, rng.randrange(1 << 30))
+        for e in rng.sample(range(ne), min(ne, 8)):
+            q, m = qa.qaoa_edge_text(dgf, 2, False, e, ang)
+            nets.append((n, q, m))
+            if len(nets) == args.networks:
+                break
+    t0 = time.perf_counter()
+    small, big = [], []
+    for i, (n, q, m) in enumerate(nets):
+        net = qa.Network(q, m, None)
+        net.order(qa.ORDER_GREEDY, 0)
+        (small if int(net.plan()[:, 6].max()) <= 8 else big).append(i)
+        net.close()
+    sliced = {i: qa.SlicedNetwork(nets[i][1], nets[i][2], dim_log2=2, order=qa.ORDER_LINEGRAPH, max_bits=28)
+              for i in big}
+    plan_s = time.perf_counter() - t0
+    nb = qa.NetBatch(ctx, [nets[i][1] for i in small], [nets[i][2] for i in small], precision=qa.QT_C64_4M)
+    lanes, wgs, _ = nb.autotune()
+    nb.capture()
+    nb.launch()
+    ctx.sync()
+    t0 = time.perf_counter()
+    nb.launch()
+    ctx.sync()
+    t_small = time.perf_counter() - t0
+    v_small = nb.results()
+    t0 = time.perf_counter()
+    v_big = {i: sliced[i].run(ctx, qa.QT_C64_4M)[0] for i in big}
+    t_big = time.perf_counter() - t0
+    # one pairwise contraction per launch, complex128, the same greedy order
+    t0 = time.perf_counter()
+    ref = []
+    for n, q, m in nets:
+        net = qa.Network(q, m, ctx)
+        net.order(qa.ORDER_GREEDY, 0)
+        ref.append(net.value())
+        net.close()
+    t_one = time.perf_counter() - t0
+    got = {i: v for i, v in zip(small, v_small)}
+    got.update(v_big)
+    err = max(abs(got[i] - ref[i]) / max(abs(ref[i]), 1e-3) for i in range(len(nets)))
+    total = t_small + t_big
+    return {"metric": "QAOA p=2 uneven light-cone networks per second (batched + GEMM path vs one at a time)",
+            "value": len(nets) / total, "unit": "networks/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+            "ms_per_step": 1e3 * total, "higher_is_better": True, "scaling": "strong", "vs_baseline": None,
+            "dtype": "c64", "data": "synthetic",
+            "config": {"workload": "qaoa_p2_4regular_n20_120_edge_networks", "networks": len(nets),
+                       "batched": len(small), "gemm_path": len(big), "executor": [lanes, wgs]},
+            "batched_ms": 1e3 * t_small, "gemm_path_ms": 1e3 * t_big,
+            "one_at_a_time_c128_ms": 1e3 * t_one, "speedup_vs_one_at_a_time": t_one / total,
+            "max_rel_err_vs_c128": err, "floor": "|d| / max(|x|, 1e-3)", "host_planning_s": plan_s}
+
+
+def run_serving(args, qa, np):
+    """Poisson arrivals at a fixed rate: 70% QAOA energies (3-regular graphs with
+    p = 1 or 2, 4-regular with p = 1; 20-200 vertices) and 30% single amplitudes of
+    grid random circuits (16-24 qubits, depth 8), from a pool of structures planned
+    before the timed region; requests run in the continuous-batching server (complex64
+    4M).  Latency = submission to result."""
+    from qtorch_amd import workloads as wl
+    rng = random.Random(args.seed)
+    graphs = []
+    for i in range(12):
+        d = 3 if i % 3 else 4
+        n = int(round(math.exp(rng.uniform(math.log(20), math.log(200)))))
+        n += (n * d) % 2
+        graphs.append((wl.dgf_text(wl.regular_graph(n, d, seed=500 + i)), d))
+    circuits = [(wl.grid_random_circuit(r, c, 8, seed=600 + i), r * c) for i, (r, c) in
+                enumerate([(4, 4), (4, 5), (4, 6), (3, 6)])]
+    reqs = []
+    n_req = int(args.qps * args.duration)
+    for _ in range(n_req):
+        if rng.random() < 0.7:
+            dgf, d = rng.choice(graphs)
+            p = rng.choice([1, 2]) if d == 3 else 1
+            reqs.append(("qaoa", dgf, p, wl.qaoa_angles(p, rng.randrange(1 << 30))))
+        else:
+            q, n = rng.choice(circuits)
+            reqs.append(("amp", q, wl.bitstring(n, rng.randrange(1 << 30))))
+    srv = qa.Server(slots=2, precision=qa.QT_C64_4M)
+
+    def submit(r):
+        return srv.submit_qaoa(r[1], r[2], r[3]) if r[0] == "qaoa" else srv.submit_amplitude(r[1], r[2])
+
+    # plan every structure once (outside the timed region)
+    seen = set()
+    for r in reqs:
+        key = (r[0], r[1], r[2] if r[0] == "qaoa" else 0)
+        if key not in seen:
+            seen.add(key)
+            srv.wait(submit(r))
+    base = srv.stats()
+    arrivals = np.cumsum(np.random.default_rng(args.seed).exponential(1.0 / args.qps, n_req))
+    ids = [0] * n_req
+    t0 = time.perf_counter()
+    for k, t in enumerate(arrivals):
+        delay = t0 + t - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        ids[k] = submit(reqs[k])
+    res = [srv.wait(i) for i in ids]
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    lat = np.array([r["latency_ms"] for r in res])
+    ok = sum(1 for r in res if r["status"] == 0)
+    routes = {k: sum(1 for r in res if r["route"] == k) for k in ("small", "medium", "eager")}
+    # the FP64 plain path on a sample of the same requests
+    ctx = qa.Context(0)
+    sample = [k for k in range(n_req) if reqs[k][0] == "qaoa"][:10]
+    t1 = time.perf_counter()
+    errs = []
+    for k in sample:
+        e, _ = qa.qaoa_energy_plain(ctx, reqs[k][1], reqs[k][2], False, list(reqs[k][3]), seed=k)
+        errs.append(abs(res[k]["value"].real - e) / abs(e))
+    plain_ms = 1e3 * (time.perf_counter() - t1) / max(len(sample), 1)
+    edge_evals = st["edge_evals"] - base["edge_evals"]
+    return {"metric": "serving requests/s (p50/p99 latency) under Poisson arrivals", "value": ok / wall,
+            "unit": "requests/s", "n_gpus": 1, "steps": n_req, "warmup": len(seen), "ms_per_step": 1e3 * wall / n_req,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "c64", "data": "synthetic",
+            "config": {"workload": "serving_mix_qaoa70_amplitude30", "qps": args.qps, "duration_s": args.duration,
+                       "slots": 2, "structures": len(seen)},
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "completed": ok, "routes": routes, "edge_evals_per_s": edge_evals / wall,
+            "launches": st["launches"] - base["launches"], "overlap_ms": st["overlap_ms"] - base["overlap_ms"],
+            "busy_ms": st["busy_ms"] - base["busy_ms"], "peak_planned_bytes": st["peak_planned_bytes"],
+            "plain_c128_ms_per_qaoa_request": plain_ms, "max_energy_rel_err_vs_plain_c128": max(errs) if errs else None}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--sets", type=int, default=8192, help="angle sets per step")
-    ap.add_argument("--form", default="c64_3m", choices=["c64_4m", "c64_3m", "c64_bf16x3", "c128"])
+    ap.add_argument("--form", default="c64_4m", choices=["c64_4m", "c64_3m", "c64_bf16x3", "c128"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-forms", action="store_true", help="skip the other forms' accuracy and speed")
     ap.add_argument("--exec", default="", help="executor shape 'lanes,wgs_per_cu' instead of the autotuner's")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run plain device copies of this many MiB (calibrates HBM byte counters)")
+    ap.add_argument("--workload", default="qaoa_p1", choices=["qaoa_p1", "qft8", "grid", "uneven", "serving"])
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--max-bits", type=int, default=22, help="grid: index bits per tensor after slicing")
+    ap.add_argument("--networks", type=int, default=512, help="uneven: edge networks")
+    ap.add_argument("--qps", type=float, default=400.0, help="serving: arrival rate")
+    ap.add_argument("--duration", type=float, default=10.0, help="serving: seconds of arrivals")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine measures one GPU; --gpus must be 1"}))
         sys.exit(2)
+    if args.workload != "qaoa_p1":
+        import numpy as np
+
+        import qtorch_amd as qa
+        run = {"qft8": run_qft8, "grid": run_grid, "uneven": run_uneven, "serving": run_serving}[args.workload]
+        print(json.dumps(run(args, qa, np)))
+        return
     if args.copy_mib:
         import torch
         src = torch.ones(args.copy_mib << 20, dtype=torch.uint8, device="cuda:0")
@@ -211,7 +448,7 @@ def main():
         "config": {"workload": "qaoa_p1_maxcut_3regular_n30_edges45", "angle_sets_per_step": args.sets,
                    "edge_networks_per_step": ne * args.sets, "form": args.form,
                    "order": "greedy", "executor_lanes": lanes, "wgs_per_cu": wgs,
-                   "global_scratch": info["global_scratch"]},
+                   "global_scratch": info["global_scratch"], "staged_leaves": info["staged_leaves"]},
         "p50_step_ms": 1e3 * float(np.percentile(lat, 50)), "p99_step_ms": 1e3 * float(np.percentile(lat, 99)),
         "accuracy": {"max_edge_rel_err_vs_c128": edge_err, "max_energy_rel_err_vs_c128": energy_err,
                      "max_edge_rel_err_vs_reference": ref_err, "c128_batched_vs_reference": c128_vs_ref,

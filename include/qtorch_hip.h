@@ -173,12 +173,14 @@ qt_status qt_batch_set_instances(qt_batch* b, int n, const int32_t* prog, const 
  * 64 threads for fewer than 64 lanes, else 256).  Intermediates live in LDS when
  * they fit, else in a device scratch area. */
 qt_status qt_batch_config(qt_batch* b, int lanes, int wgs_per_cu);
-enum { QT_SCRATCH_AUTO = 0, QT_SCRATCH_LDS = 1, QT_SCRATCH_GLOBAL = 2 };
+enum { QT_SCRATCH_AUTO = 0, QT_SCRATCH_LDS = 1, QT_SCRATCH_GLOBAL = 2, QT_SCRATCH_STAGE_LEAVES = 4 };
 /* The same with an explicit place for the intermediates (QT_INVALID_INPUT, and the
- * previous shape kept, when LDS is asked for and a workgroup's share exceeds 64 KiB). */
+ * previous shape kept, when LDS is asked for and a workgroup's share exceeds 64 KiB);
+ * QT_SCRATCH_STAGE_LEAVES or-ed in makes lane-grouped waves copy each step's leaf
+ * operands to LDS first (else they are read from the pool through the caches). */
 qt_status qt_batch_config_ex(qt_batch* b, int lanes, int wgs_per_cu, int scratch);
-/* Times every executor shape and scratch place on the current instances and keeps
- * the fastest. */
+/* Times every executor shape, scratch place and leaf staging choice on the current
+ * instances and keeps the fastest. */
 qt_status qt_batch_autotune(qt_batch* b, int* lanes, int* wgs_per_cu, float* ms);
 /* Asynchronous run of every instance (replays the captured graph if there is one). */
 qt_status qt_batch_launch(qt_batch* b);
@@ -191,6 +193,8 @@ qt_status qt_batch_capture(qt_batch* b);
 qt_status qt_batch_results(qt_batch* b, double* values, double* group_sums);
 /* Mean time of `reps` launches, by HIP events on the context's stream. */
 qt_status qt_batch_time(qt_batch* b, int reps, float* ms_per_launch);
+/* complex MACs of one launch, executor shape, flags (1: intermediates in device
+ * memory, 2: leaves staged in LDS) and scratch elements per executor */
 qt_status qt_batch_info(const qt_batch* b, int64_t* cmacs, int* lanes, int* wgs_per_cu,
                         int* global_scratch, int* scratch_elems);
 qt_status qt_batch_destroy(qt_batch* b);

@@ -71,6 +71,8 @@ int qth_network_order(qth_network* net, int order, uint64_t seed);
 int qth_record(qth_network* net, int32_t* pairs, int cap, int* n);
 /* Node::mNumFloatOps counter (src/Network.h:884-885): complex MACs */
 int64_t qth_flops(qth_network* net);
+/* mFinalVal (src/Network.h:61) of a contracted network */
+int qth_network_value(qth_network* net, double* re, double* im);
 int qth_counts(qth_network* net, int* all_nodes, int* uncontracted, int* original, int* done);
 /* Recorded plan steps: 8 ints each (a_id, b_id, c_id, rank_a, rank_b, k, rank_c, 0) */
 int qth_plan(qth_network* net, int32_t* steps, int cap, int* n);
@@ -166,7 +168,8 @@ int qth_serve_plan(int kind, const char* text, int p, int ref_map, const char* b
 /* ---- one large network, sliced --------------------------------------------------
  * Planned once in `order` (QTH_ORDER_*), then wires are sliced until every tensor
  * has at most max_bits index bits; the D^s slices run step by step (MFMA GEMMs for
- * large steps) in the given form and their scalars are summed in FP64 on the device.
+ * large steps) in the given form (a complex64 form may be or-ed with QT_RESCALE for
+ * per-step power-of-two rescaling) and their scalars are summed in FP64 on the device.
  * dim_log2 = 1: amplitude picture (measure = output bitstring); 2: qTorch's picture. */
 typedef struct qth_sliced qth_sliced;
 int qth_sliced_create(const char* qasm, const char* measure, int dim_log2, int order, uint64_t seed, int max_bits,

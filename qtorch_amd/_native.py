@@ -21,7 +21,7 @@ HOST_LIB_PATH = os.path.join(LIB_DIR, "libqtorch_host.so")
 QT_OK, QT_INVALID_INPUT, QT_OOM, QT_DEVICE, QT_TIMEOUT, QT_UNSUPPORTED = range(6)
 QT_DTYPE_C128, QT_DTYPE_C64 = 0, 1
 QT_PLAIN_C128, QT_C64_4M, QT_C64_3M, QT_C64_BF16X3 = 0, 1, 2, 3
-QT_SCRATCH_AUTO, QT_SCRATCH_LDS, QT_SCRATCH_GLOBAL = 0, 1, 2
+QT_SCRATCH_AUTO, QT_SCRATCH_LDS, QT_SCRATCH_GLOBAL, QT_SCRATCH_STAGE_LEAVES = 0, 1, 2, 4
 QT_RESCALE = 1 << 8  # or-ed into a complex64 form: per-step power-of-two rescaling
 QT_FROM_A = 1 << 30
 
@@ -99,6 +99,7 @@ _HOST_SIGS = {
     "qth_network_order": (_c.c_int, [_vp, _c.c_int, _c.c_uint64]),
     "qth_record": (_c.c_int, [_vp, _i32p, _c.c_int, _c.POINTER(_c.c_int)]),
     "qth_flops": (_c.c_int64, [_vp]),
+    "qth_network_value": (_c.c_int, [_vp, _dp, _dp]),
     "qth_counts": (_c.c_int, [_vp] + [_c.POINTER(_c.c_int)] * 4),
     "qth_plan": (_c.c_int, [_vp, _i32p, _c.c_int, _c.POINTER(_c.c_int)]),
     "qth_program_stats": (_c.c_int, [_vp, _c.POINTER(_c.c_int64), _c.POINTER(_c.c_int64), _c.POINTER(_c.c_int)]),
@@ -418,6 +419,10 @@ class Network:
         _host_check(host.qth_record(self.handle, out, n.value, ctypes.byref(n)))
         return [(out[2 * i], out[2 * i + 1]) for i in range(n.value)]
 
+    def value(self) -> complex:
+        """Final value of a contracted network (mFinalVal)."""
+        return self._val(host.qth_network_value)
+
     def flops(self) -> int:
         return int(host.qth_flops(self.handle))
 
@@ -498,8 +503,8 @@ class _DeviceBatch:
         l, w, g, se = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         self.ctx.check(hip.qt_batch_info(self.dev, ctypes.byref(cm), ctypes.byref(l), ctypes.byref(w),
                                          ctypes.byref(g), ctypes.byref(se)), "qt_batch_info")
-        return {"cmacs": cm.value, "lanes": l.value, "wgs_per_cu": w.value, "global_scratch": bool(g.value),
-                "scratch_elems": se.value}
+        return {"cmacs": cm.value, "lanes": l.value, "wgs_per_cu": w.value, "global_scratch": bool(g.value & 1),
+                "staged_leaves": bool(g.value & 2), "scratch_elems": se.value}
 
 
 class QaoaBatch(_DeviceBatch):

@@ -93,7 +93,9 @@ struct qt_batch {
     // launch configuration
     int lanes = 64;  // 256: a workgroup per instance; 64: a wavefront; < 64: lanes per instance in a wave
     int wgs_per_cu = 4;
-    int scratch_place = QT_SCRATCH_AUTO;
+    int scratch_place = QT_SCRATCH_AUTO;  // with QT_SCRATCH_STAGE_LEAVES or-ed in
+    bool stage_leaves = false;
+    bool shape_ok = false;  // device arrays fit the executor shape
     int32_t slot_stride = 0;      // elements between the scratch areas of two executors
     int32_t pstage = 0, cstage = 0;  // largest parameter / constant leaf (elements), LDS stages
     std::vector<int32_t> h_prog;  // program of each instance (host copy, for chunking)
@@ -198,32 +200,57 @@ template <class V, int MODE, int LANES, class PA, class PB>
 __device__ __forceinline__ void run_step(PA A, PB B, V* C, const uint32_t* __restrict__ fc,
                                          const uint32_t* __restrict__ gs, int nC, int K, int lane,
                                          double2* out_final) {
+    auto put = [&](int c, const Acc<V, MODE>& acc) {
+        const V v = acc.get();
+        if (out_final) *out_final = double2{double(v.x), double(v.y)};
+        else C[c] = v;
+    };
+    if (K == 4) {
+        // one summed D = 4 axis (the most common step): the four summed offsets are
+        // wave-uniform and stay in scalar registers for every output of the lane
+        const uint32_t g0 = gs[0], g1 = gs[1], g2 = gs[2], g3 = gs[3];
+        const int a0 = int(g0 & 0xffffu), a1 = int(g1 & 0xffffu), a2 = int(g2 & 0xffffu), a3 = int(g3 & 0xffffu);
+        const int b0 = int(g0 >> 16), b1 = int(g1 >> 16), b2 = int(g2 >> 16), b3 = int(g3 >> 16);
+        for (int c = lane; c < nC; c += LANES) {
+            const uint32_t f = fc[c];
+            const auto pa = A + int(f & 0xffffu);
+            const auto pb = B + int(f >> 16);
+            const V x0 = pa[a0], y0 = pb[b0], x1 = pa[a1], y1 = pb[b1];
+            const V x2 = pa[a2], y2 = pb[b2], x3 = pa[a3], y3 = pb[b3];
+            Acc<V, MODE> acc;
+            acc.mac(x0, y0);
+            acc.mac(x1, y1);
+            acc.mac(x2, y2);
+            acc.mac(x3, y3);
+            put(c, acc);
+        }
+        return;
+    }
     for (int c = lane; c < nC; c += LANES) {
         const uint32_t f = fc[c];
-        const int oa = int(f & 0xffffu), ob = int(f >> 16);
+        const auto pa = A + int(f & 0xffffu);
+        const auto pb = B + int(f >> 16);
         Acc<V, MODE> acc;
         if ((K & 3) == 0) {
             // summed extents are powers of D: four independent operand loads in flight
             for (int s = 0; s < K; s += 4) {
                 const uint32_t g0 = gs[s], g1 = gs[s + 1], g2 = gs[s + 2], g3 = gs[s + 3];
-                const V a0 = A[oa + int(g0 & 0xffffu)], b0 = B[ob + int(g0 >> 16)];
-                const V a1 = A[oa + int(g1 & 0xffffu)], b1 = B[ob + int(g1 >> 16)];
-                const V a2 = A[oa + int(g2 & 0xffffu)], b2 = B[ob + int(g2 >> 16)];
-                const V a3 = A[oa + int(g3 & 0xffffu)], b3 = B[ob + int(g3 >> 16)];
-                acc.mac(a0, b0);
-                acc.mac(a1, b1);
-                acc.mac(a2, b2);
-                acc.mac(a3, b3);
+                const V x0 = pa[int(g0 & 0xffffu)], y0 = pb[int(g0 >> 16)];
+                const V x1 = pa[int(g1 & 0xffffu)], y1 = pb[int(g1 >> 16)];
+                const V x2 = pa[int(g2 & 0xffffu)], y2 = pb[int(g2 >> 16)];
+                const V x3 = pa[int(g3 & 0xffffu)], y3 = pb[int(g3 >> 16)];
+                acc.mac(x0, y0);
+                acc.mac(x1, y1);
+                acc.mac(x2, y2);
+                acc.mac(x3, y3);
             }
         } else {
             for (int s = 0; s < K; ++s) {
                 const uint32_t g = gs[s];
-                acc.mac(A[oa + int(g & 0xffffu)], B[ob + int(g >> 16)]);
+                acc.mac(pa[int(g & 0xffffu)], pb[int(g >> 16)]);
             }
         }
-        const V v = acc.get();
-        if (out_final) *out_final = double2{double(v.x), double(v.y)};
-        else C[c] = v;
+        put(c, acc);
     }
 }
 
@@ -294,7 +321,7 @@ __global__ __launch_bounds__(kWg) void k_batch(const ProgHdr* __restrict__ progs
 // Leaf operands are staged in LDS once per step (constant leaves once per wave,
 // parameter leaves once per slot), so every multiply-add reads LDS, not the pool.
 // LDS per wave: [W slot scratch areas (LDS mode)] [W x 2 parameter stages] [2 constant stages].
-template <class V, int MODE, int G, bool GLOBAL_SCRATCH>
+template <class V, int MODE, int G, bool GLOBAL_SCRATCH, bool STAGE>
 __global__ __launch_bounds__(64) void k_batch_grouped(const ProgHdr* __restrict__ progs,
                                                       const StepDesc* __restrict__ steps,
                                                       const uint32_t* __restrict__ tabs,
@@ -336,10 +363,16 @@ __global__ __launch_bounds__(64) void k_batch_grouped(const ProgHdr* __restrict_
             const uint32_t* gs = fc + nC;
             double2* fin = (st.flags & 1) ? out + i : nullptr;
             V* C = mem + st.c;
-            // copies leaf `code` (1 << bits elements) into stage `which`
-            auto stage = [&](int code, int bits, int which) -> V* {
+            // copies leaf `code` (1 << bits elements) into stage `which`; without
+            // staging, the leaf in the pool
+            auto stage = [&](int code, int bits, int which) -> const V* {
                 const int64_t r = refs[-1 - code];
                 const int n = 1 << bits;
+                if constexpr (!STAGE) {
+                    (void)n;
+                    (void)which;
+                    return pool + ((r & kParam) ? pbase + (r & ~kParam) : P.const_base + r);
+                }
                 if (r & kParam) {
                     V* dst = pst + which * pstage;
                     const V* src = pool + pbase + (r & ~kParam);
@@ -354,18 +387,18 @@ __global__ __launch_bounds__(64) void k_batch_grouped(const ProgHdr* __restrict_
             };
             const int a_bits = st.pad & 0xff, b_bits = (st.pad >> 8) & 0xff;
             if (st.a < 0 && st.b < 0) {
-                V* sa = stage(st.a, a_bits, 0);
-                V* sb = stage(st.b, b_bits, 1);
-                exec_sync<64, GLOBAL_SCRATCH>();
-                if (active) run_step<V, MODE, G>((const V*)sa, (const V*)sb, C, fc, gs, nC, K, gl, fin);
+                const V* sa = stage(st.a, a_bits, 0);
+                const V* sb = stage(st.b, b_bits, 1);
+                if constexpr (STAGE) exec_sync<64, GLOBAL_SCRATCH>();
+                if (active) run_step<V, MODE, G>(sa, sb, C, fc, gs, nC, K, gl, fin);
             } else if (st.a < 0) {
-                V* sa = stage(st.a, a_bits, 0);
-                exec_sync<64, GLOBAL_SCRATCH>();
-                if (active) run_step<V, MODE, G>((const V*)sa, (const V*)(mem + st.b), C, fc, gs, nC, K, gl, fin);
+                const V* sa = stage(st.a, a_bits, 0);
+                if constexpr (STAGE) exec_sync<64, GLOBAL_SCRATCH>();
+                if (active) run_step<V, MODE, G>(sa, (const V*)(mem + st.b), C, fc, gs, nC, K, gl, fin);
             } else if (st.b < 0) {
-                V* sb = stage(st.b, b_bits, 1);
-                exec_sync<64, GLOBAL_SCRATCH>();
-                if (active) run_step<V, MODE, G>((const V*)(mem + st.a), (const V*)sb, C, fc, gs, nC, K, gl, fin);
+                const V* sb = stage(st.b, b_bits, 1);
+                if constexpr (STAGE) exec_sync<64, GLOBAL_SCRATCH>();
+                if (active) run_step<V, MODE, G>((const V*)(mem + st.a), sb, C, fc, gs, nC, K, gl, fin);
             } else if (active) {
                 run_step<V, MODE, G>((const V*)(mem + st.a), (const V*)(mem + st.b), C, fc, gs, nC, K, gl, fin);
             }
@@ -426,8 +459,8 @@ inline int execs_per_wg(int lanes) { return lanes >= 64 ? kWg / lanes : 64 / lan
 // LDS elements of a lane-grouped wave: slot scratch (LDS mode) and leaf stages
 inline size_t grouped_lds_elems(const qt_batch* b) {
     const size_t w = size_t(64 / b->lanes);
-    return (b->use_global_scratch ? 0 : w * size_t(b->slot_stride)) + 2 * w * size_t(b->pstage) +
-           2 * size_t(b->cstage);
+    return (b->use_global_scratch ? 0 : w * size_t(b->slot_stride)) +
+           (b->stage_leaves ? 2 * w * size_t(b->pstage) + 2 * size_t(b->cstage) : 0);
 }
 inline int grid_wgs(const qt_batch* b) { return b->n_cus * b->wgs_per_cu * (b->lanes >= 64 ? 1 : 4); }
 
@@ -440,7 +473,13 @@ qt_status launch_grouped_t(qt_batch* b) {
     const V* pool = static_cast<const V*>(b->d_pool);
     auto d_out = reinterpret_cast<double2*>(b->d_out);
 #define QT_LAUNCH_G(GL, GS)                                                                                        \
-    hipLaunchKernelGGL((k_batch_grouped<V, MODE, GL, GS>), dim3(grid), dim3(64), lds, c->stream, b->d_progs,       \
+    if (b->stage_leaves) {                                                                                          \
+        QT_LAUNCH_GST(GL, GS, true);                                                                               \
+    } else {                                                                                                       \
+        QT_LAUNCH_GST(GL, GS, false);                                                                              \
+    }
+#define QT_LAUNCH_GST(GL, GS, ST)                                                                                  \
+    hipLaunchKernelGGL((k_batch_grouped<V, MODE, GL, GS, ST>), dim3(grid), dim3(64), lds, c->stream, b->d_progs,   \
                        b->d_steps, b->d_tabs, b->d_leaf_refs, pool, b->d_inst, b->d_chunks, b->n_chunks,           \
                        b->d_counter, d_out, gs, b->slot_stride, b->pstage, b->cstage)
 #define QT_LAUNCH_GS(GS)                      \
@@ -457,6 +496,7 @@ qt_status launch_grouped_t(qt_batch* b) {
     }
 #undef QT_LAUNCH_GS
 #undef QT_LAUNCH_G
+#undef QT_LAUNCH_GST
     return check_hip(c, hipGetLastError(), "k_batch_grouped");
 }
 
@@ -487,6 +527,8 @@ qt_status launch_batch_t(qt_batch* b) {
 
 qt_status launch_batch(qt_batch* b) {
     Ctx* c = b->ctx;
+    // a failed reconfiguration leaves scratch or chunk arrays that do not fit the shape
+    if (!b->shape_ok) return set_error(c, QT_INVALID_INPUT, "batch: executor shape not configured (last change failed)");
     hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, c->stream, b->d_counter);
     qt_status s = check_hip(c, hipGetLastError(), "k_reset_counter");
     if (s != QT_OK) return s;
@@ -507,25 +549,27 @@ size_t elem_size(int precision) { return precision == QT_PLAIN_C128 ? 16 : 8; }
 
 qt_status configure_scratch(qt_batch* b) {
     Ctx* c = b->ctx;
+    b->shape_ok = false;  // set once the arrays below match the executor shape
     const size_t es = elem_size(b->precision);
     const int kExec = execs_per_wg(b->lanes);
     // instance slots of a lane-grouped wave sit an odd number of elements apart, so
     // the slots' same-offset accesses fall into different LDS banks
     b->slot_stride = b->lanes < 64 ? b->scratch_elems + 1 : b->scratch_elems;
     // leaf stages of lane-grouped waves (LDS in either placement)
-    const size_t stage =
-        b->lanes < 64 ? (2 * size_t(kExec) * size_t(b->pstage) + 2 * size_t(b->cstage)) * es : 0;
+    b->stage_leaves = b->lanes < 64 && (b->scratch_place & QT_SCRATCH_STAGE_LEAVES);
+    const size_t stage = b->stage_leaves ? (2 * size_t(kExec) * size_t(b->pstage) + 2 * size_t(b->cstage)) * es : 0;
     const size_t lds_need = size_t(kExec) * size_t(b->slot_stride) * es + stage;
     const size_t lds_cap = 160 * 1024;
     const size_t wgs_on_cu = size_t(b->wgs_per_cu) * (b->lanes >= 64 ? 1 : 4);
     if (stage > 64 * 1024)
         return set_error(c, QT_INVALID_INPUT, "batch: the leaf stages do not fit a workgroup's LDS");
-    if (b->scratch_place == QT_SCRATCH_LDS && lds_need > 64 * 1024)
+    const int place = b->scratch_place & 3;
+    if (place == QT_SCRATCH_LDS && lds_need > 64 * 1024)
         return set_error(c, QT_INVALID_INPUT, "batch: the executors' scratch does not fit a workgroup's LDS");
     // LDS unless it does not fit or would hold fewer workgroups per CU than asked for
     // (then the intermediates go through L1/L2); an explicit placement wins
-    b->use_global_scratch = b->scratch_place == QT_SCRATCH_GLOBAL ||
-                            (b->scratch_place == QT_SCRATCH_AUTO && (lds_need * wgs_on_cu > lds_cap || lds_need > 64 * 1024));
+    b->use_global_scratch = place == QT_SCRATCH_GLOBAL ||
+                            (place == QT_SCRATCH_AUTO && (lds_need * wgs_on_cu > lds_cap || lds_need > 64 * 1024));
     // chunks of consecutive instances of one program, at most one per instance slot
     if (b->lanes < 64) {
         const int W = 64 / b->lanes;
@@ -561,6 +605,7 @@ qt_status configure_scratch(qt_batch* b) {
         (void)hipGraphDestroy(b->graph);
         b->graph = nullptr;
     }
+    b->shape_ok = true;
     return QT_OK;
 }
 
@@ -1036,7 +1081,7 @@ qt_status qt_batch_config(qt_batch* b, int lanes, int wgs_per_cu) {
 
 qt_status qt_batch_config_ex(qt_batch* b, int lanes, int wgs_per_cu, int scratch) {
     if (!b || (lanes != 4 && lanes != 8 && lanes != 16 && lanes != 32 && lanes != 64 && lanes != 256) ||
-        wgs_per_cu < 1 || wgs_per_cu > 16 || scratch < QT_SCRATCH_AUTO || scratch > QT_SCRATCH_GLOBAL)
+        wgs_per_cu < 1 || wgs_per_cu > 16 || (scratch & ~(3 | QT_SCRATCH_STAGE_LEAVES)) != 0 || (scratch & 3) == 3)
         return QT_INVALID_INPUT;
     std::lock_guard<std::recursive_mutex> g(b->ctx->mu);
     const int l0 = b->lanes, w0 = b->wgs_per_cu, s0 = b->scratch_place;
@@ -1044,7 +1089,7 @@ qt_status qt_batch_config_ex(qt_batch* b, int lanes, int wgs_per_cu, int scratch
     b->wgs_per_cu = wgs_per_cu;
     b->scratch_place = scratch;
     const qt_status s = qt::configure_scratch(b);
-    if (s == QT_INVALID_INPUT) {  // keep the previous, working shape
+    if (s != QT_OK) {  // keep the previous, working shape
         b->lanes = l0;
         b->wgs_per_cu = w0;
         b->scratch_place = s0;
@@ -1135,7 +1180,9 @@ qt_status qt_batch_autotune(qt_batch* b, int* lanes, int* wgs_per_cu, float* ms)
     float best = 1e30f;
     for (int l : lane_opts)
         for (int w : wg_opts)
-            for (int place : {QT_SCRATCH_LDS, QT_SCRATCH_GLOBAL}) {
+            for (int place : {int(QT_SCRATCH_LDS), int(QT_SCRATCH_GLOBAL), QT_SCRATCH_LDS | QT_SCRATCH_STAGE_LEAVES,
+                              QT_SCRATCH_GLOBAL | QT_SCRATCH_STAGE_LEAVES}) {
+                if ((place & QT_SCRATCH_STAGE_LEAVES) && l >= 64) continue;
                 if (qt_batch_config_ex(b, l, w, place) != QT_OK) continue;
                 if (qt_batch_launch(b) != QT_OK) continue;  // warm-up
                 float t = 0;
@@ -1160,7 +1207,7 @@ qt_status qt_batch_info(const qt_batch* b, int64_t* cmacs, int* lanes, int* wgs_
     if (cmacs) *cmacs = b->cmacs_total;
     if (lanes) *lanes = b->lanes;
     if (wgs_per_cu) *wgs_per_cu = b->wgs_per_cu;
-    if (global_scratch) *global_scratch = b->use_global_scratch ? 1 : 0;
+    if (global_scratch) *global_scratch = (b->use_global_scratch ? 1 : 0) | (b->stage_leaves ? 2 : 0);
     if (scratch_elems) *scratch_elems = b->scratch_elems;
     return QT_OK;
 }

@@ -674,6 +674,28 @@ __global__ void k_rescale_c64(float2* __restrict__ x, int64_t n, const uint32_t*
     if (blockIdx.x == 0 && threadIdx.x == 0) *ec = (ea ? *ea : 0) + (eb ? *eb : 0) + e;
 }
 
+// tensors of up to kRescaleSmall elements: the maximum, the scale and the exponent in
+// one single-workgroup launch
+constexpr int64_t kRescaleSmall = 1 << 14;
+__global__ __launch_bounds__(kThreads) void k_rescale_small_c64(float2* __restrict__ x, int64_t n,
+                                                                const int32_t* __restrict__ ea,
+                                                                const int32_t* __restrict__ eb,
+                                                                int32_t* __restrict__ ec) {
+    __shared__ float part[kThreads / 64];
+    float m = 0.0f;
+    for (int64_t i = threadIdx.x; i < n; i += kThreads) m = fmaxf(m, fmaxf(fabsf(x[i].x), fabsf(x[i].y)));
+    for (int d = 32; d >= 1; d >>= 1) m = fmaxf(m, __shfl_down(m, d, 64));
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
+    __syncthreads();
+    m = part[0];
+    for (int w = 1; w < kThreads / 64; ++w) m = fmaxf(m, part[w]);
+    int e = 0;
+    if (m > 0.0f && isfinite(m)) (void)frexpf(m, &e);
+    const float s = ldexpf(1.0f, -e);
+    for (int64_t i = threadIdx.x; i < n; i += kThreads) x[i] = float2{x[i].x * s, x[i].y * s};
+    if (threadIdx.x == 0) *ec = (ea ? *ea : 0) + (eb ? *eb : 0) + e;
+}
+
 __global__ void k_set_u32(uint32_t* p, uint32_t v) { *p = v; }
 
 template <class V>
@@ -711,12 +733,18 @@ qt_status rescale_c64(Ctx* c, const Tensor* A, const Tensor* B, Tensor* C) {
     }
     const int64_t n = C->numel;
     float2* x = static_cast<float2*>(C->data);
+    const int32_t* ea = A->exp_slot >= 0 ? c->d_exp + A->exp_slot : nullptr;
+    const int32_t* eb = B->exp_slot >= 0 ? c->d_exp + B->exp_slot : nullptr;
+    if (n <= kRescaleSmall) {
+        hipLaunchKernelGGL(k_rescale_small_c64, dim3(1), dim3(kThreads), 0, c->stream, x, n, ea, eb,
+                           c->d_exp + C->exp_slot);
+        return check_hip(c, hipGetLastError(), "k_rescale_small_c64");
+    }
     hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, c->stream, c->d_absmax, 0u);
     const unsigned grid = grid_for(n, kThreads, 4096);
     hipLaunchKernelGGL(k_absmax_c64, dim3(grid), dim3(kThreads), 0, c->stream, x, n, c->d_absmax);
-    hipLaunchKernelGGL(k_rescale_c64, dim3(grid), dim3(kThreads), 0, c->stream, x, n, c->d_absmax,
-                       A->exp_slot >= 0 ? c->d_exp + A->exp_slot : nullptr,
-                       B->exp_slot >= 0 ? c->d_exp + B->exp_slot : nullptr, c->d_exp + C->exp_slot);
+    hipLaunchKernelGGL(k_rescale_c64, dim3(grid), dim3(kThreads), 0, c->stream, x, n, c->d_absmax, ea, eb,
+                       c->d_exp + C->exp_slot);
     return check_hip(c, hipGetLastError(), "k_rescale_c64");
 }
 

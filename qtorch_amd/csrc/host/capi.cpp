@@ -220,6 +220,14 @@ int qth_record(qth_network* h, int32_t* pairs, int cap, int* n) {
 
 int64_t qth_flops(qth_network* h) { return h ? h->net->getNumFloatOps() : -1; }
 
+int qth_network_value(qth_network* h, double* re, double* im) {
+    if (!h || !re || !im) return QTH_INVALID_FUNCTION_INPUT;
+    const auto v = h->net->GetFinalValue();
+    *re = v.real();
+    *im = v.imag();
+    return QTH_OK;
+}
+
 int qth_counts(qth_network* h, int* all_nodes, int* uncontracted, int* original, int* done) {
     if (!h) return QTH_INVALID_FUNCTION_INPUT;
     if (all_nodes) *all_nodes = int(h->net->GetAllNodes().size());

@@ -152,11 +152,13 @@ SlicedNetwork::SlicedNetwork(const std::string& qasm, const std::string& measure
 
 std::complex<double> SlicedNetwork::Run(qt_ctx* ctx, int precision, SliceStats* stats) const {
     if (!ctx) throw DeviceFailure("SlicedNetwork::Run needs a device context");
-    const bool c128 = precision == QT_PLAIN_C128;
-    if (!c128 && precision != QT_C64_4M && precision != QT_C64_3M && precision != QT_C64_BF16X3)
+    const int base = precision & ~QT_RESCALE;
+    const bool c128 = base == QT_PLAIN_C128;
+    if ((c128 && base != precision) ||
+        (!c128 && base != QT_C64_4M && base != QT_C64_3M && base != QT_C64_BF16X3))
         throw InvalidFunctionInput();
     const int dtype = c128 ? QT_DTYPE_C128 : QT_DTYPE_C64;
-    const int form = c128 ? precision : (precision | QT_RESCALE);
+    const int form = precision;
     const int D = 1 << dl_;
     const auto t0 = std::chrono::steady_clock::now();
     qt_tensor* acc = nullptr;

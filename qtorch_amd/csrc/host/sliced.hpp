@@ -27,8 +27,9 @@ class SlicedNetwork {
 public:
     SlicedNetwork(const std::string& qasm, const std::string& measure, int dim_log2, int order, uint64_t seed,
                   int max_bits);
-    // precision: QT_PLAIN_C128 (complex128), or a complex64 form (steps rescaled by
-    // powers of two).  Contracts every slice on ctx's stream.
+    // precision: QT_PLAIN_C128 (complex128), or a complex64 form, optionally or-ed
+    // with QT_RESCALE (every step rescaled by a power of two: no FP32 underflow for
+    // any depth, at two extra passes per step).  Contracts every slice on ctx's stream.
     std::complex<double> Run(qt_ctx* ctx, int precision, SliceStats* stats) const;
     const SliceStats& Plan() const { return plan_; }
 

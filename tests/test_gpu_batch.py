@@ -158,13 +158,17 @@ def test_executor_shapes_agree_bitwise(ctx, precision):
     b.set_angles(sets)
     ref = None
     for lanes in (64, 4, 8, 16, 32, 256):
-        for wgs, place in ((1, qa.QT_SCRATCH_LDS), (3, qa.QT_SCRATCH_LDS), (2, qa.QT_SCRATCH_GLOBAL)):
+        for wgs, place in ((1, qa.QT_SCRATCH_LDS), (3, qa.QT_SCRATCH_LDS), (2, qa.QT_SCRATCH_GLOBAL),
+                           (2, qa.QT_SCRATCH_LDS | qa.QT_SCRATCH_STAGE_LEAVES),
+                           (1, qa.QT_SCRATCH_GLOBAL | qa.QT_SCRATCH_STAGE_LEAVES)):
             try:
                 b.config(lanes, wgs, place)
             except qa.InvalidFunctionInput:  # this shape's intermediates do not fit in LDS
-                assert place == qa.QT_SCRATCH_LDS
+                assert (place & 3) == qa.QT_SCRATCH_LDS
                 continue
-            assert b.info()["global_scratch"] == (place == qa.QT_SCRATCH_GLOBAL)
+            info = b.info()
+            assert info["global_scratch"] == ((place & 3) == qa.QT_SCRATCH_GLOBAL)
+            assert info["staged_leaves"] == (lanes < 64 and bool(place & qa.QT_SCRATCH_STAGE_LEAVES))
             b.launch()
             e, zz = b.results()
             if ref is None:

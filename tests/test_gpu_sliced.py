@@ -23,7 +23,7 @@ def _amp_index(bits):
 
 
 @pytest.mark.parametrize("prec,tol", [(qa.QT_PLAIN_C128, 1e-10), (qa.QT_C64_4M, 2e-5), (qa.QT_C64_3M, 4e-5),
-                                      (qa.QT_C64_BF16X3, 2e-3)])
+                                      (qa.QT_C64_BF16X3, 2e-3), (qa.QT_C64_4M | qa.QT_RESCALE, 2e-5)])
 def test_sliced_amplitude_matches_statevector(ctx, prec, tol):
     q = wl.grid_random_circuit(4, 4, 8, seed=11)
     bits = wl.bitstring(16, 5)
