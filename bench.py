@@ -34,6 +34,11 @@ sys.path.insert(0, REPO)
 
 GRAPH = os.path.join(REPO, "tests", "golden", "graphs", "3reg_n30_s2024.dgf")
 FP32_PEAK_TFLOPS = 157.3  # MI355X dense FP32 (vector and matrix)
+# HBM bytes per edge network of the lane-grouped kernel with intermediates in LDS,
+# from rocprofv3 --pmc FETCH_SIZE (scaled x2.0) and WRITE_SIZE (x1.0) runs of this
+# benchmark; the factors are what the counters read for a 1 GiB device copy in the
+# same process (profiles/k_batch_qaoa_p1_3reg_n30.md)
+PMC_BYTES_PER_NETWORK = (2.0 * 16927.1 + 7328.0) * 1024 / 368640
 
 
 def angle_sets(n: int, seed: int = 2024):
@@ -396,7 +401,9 @@ def main():
     b64.set_angles(sets)
     b64.launch()
     e64, zz64 = b64.results()
-    edge_err = float(np.max(np.abs(zz - zz64) / np.maximum(np.abs(zz64), 1e-3)))
+    rel = np.abs(zz - zz64) / np.maximum(np.abs(zz64), 1e-3)
+    edge_err = float(np.max(rel))
+    within = float(np.mean(rel <= 1e-5))
     energy_err = float(np.max(np.abs(energies - e64) / np.abs(e64)))
     ref_err = float(np.max(np.abs(zz[0] - zz_ref) / np.maximum(np.abs(zz_ref), 1e-3)))
     c128_vs_ref = float(np.max(np.abs(zz64[0] - zz_ref) / np.maximum(np.abs(zz_ref), 1e-15)))
@@ -416,9 +423,10 @@ def main():
                 e_f, zz_f = bf.results()
             l_f, w_f, _ = bf.autotune()
             ms_f = bf.time(5)
+            rel_f = np.abs(zz_f - zz64) / np.maximum(np.abs(zz64), 1e-3)
             by_form[name] = {
                 "edge_evals_per_s": ne * args.sets / (ms_f * 1e-3), "kernel_ms": ms_f, "executor": [l_f, w_f],
-                "max_edge_rel_err_vs_c128": float(np.max(np.abs(zz_f - zz64) / np.maximum(np.abs(zz64), 1e-3))),
+                "max_edge_rel_err_vs_c128": float(np.max(rel_f)), "edges_within_1e-5": float(np.mean(rel_f <= 1e-5)),
                 "max_energy_rel_err_vs_c128": float(np.max(np.abs(e_f - e64) / np.abs(e64)))}
             if bf is not b64:
                 bf.close()
@@ -440,6 +448,7 @@ def main():
 
     flops_per_launch = 8.0 * info["cmacs"]
     achieved = flops_per_launch / (kernel_ms * 1e-3) / 1e12
+    traffic = PMC_BYTES_PER_NETWORK * ne * args.sets if lanes < 64 and not info["global_scratch"] else None
     res = {
         "metric": "QAOA edge-evals/s (p99 step latency), complex64 batched vs complex128 plain, rel err <= 1e-5",
         "value": value, "unit": "edge-evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -450,14 +459,15 @@ def main():
                    "order": "greedy", "executor_lanes": lanes, "wgs_per_cu": wgs,
                    "global_scratch": info["global_scratch"], "staged_leaves": info["staged_leaves"]},
         "p50_step_ms": 1e3 * float(np.percentile(lat, 50)), "p99_step_ms": 1e3 * float(np.percentile(lat, 99)),
-        "accuracy": {"max_edge_rel_err_vs_c128": edge_err, "max_energy_rel_err_vs_c128": energy_err,
+        "accuracy": {"max_edge_rel_err_vs_c128": edge_err, "edges_within_1e-5": within,
+                     "max_energy_rel_err_vs_c128": energy_err,
                      "max_edge_rel_err_vs_reference": ref_err, "c128_batched_vs_reference": c128_vs_ref,
                      "floor": "|d| / max(|x|, 1e-3)"},
         "other_forms": by_form,
         "plain_c128_gpu": plain,
         "speedup_vs_plain_c128": (value / plain["value"]) if plain else None,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": traffic,
                      "kernel": "k_batch (persistent fused contraction)", "kernel_ms_per_launch": kernel_ms,
                      "algorithmic_flops_per_launch": flops_per_launch},
     }

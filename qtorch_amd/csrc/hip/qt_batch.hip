@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <numeric>
 #include <vector>
 
@@ -30,6 +31,16 @@ struct ProgHdr {
     int32_t scratch_elems;
     int32_t flags;
     int64_t const_base;  // pool offset added to the program's constant leaf references
+    int32_t cst_begin, n_cst;  // the program's distinct constant leaves (CstEntry list)
+    int32_t param_elems;       // extent of an instance's parameter block that it reads
+    int32_t pad;
+};
+
+// A distinct constant leaf of a program and its place in a wave's LDS leaf area.
+struct CstEntry {
+    int64_t src;  // pool offset relative to the program's const_base
+    int32_t n;    // elements
+    int32_t dst;  // element offset in the wave's constant area
 };
 
 struct StepDesc {
@@ -97,7 +108,7 @@ struct qt_batch {
     bool stage_leaves = false;
     bool shape_ok = false;  // device arrays fit the executor shape
     int32_t slot_stride = 0;      // elements between the scratch areas of two executors
-    int32_t pstage = 0, cstage = 0;  // largest parameter / constant leaf (elements), LDS stages
+    int32_t pstage = 0, cstage = 0;  // LDS leaf areas: parameter block per slot, constant leaves per wave
     std::vector<int32_t> h_prog;  // program of each instance (host copy, for chunking)
     int2* d_chunks = nullptr;
     int n_chunks = 0;
@@ -112,6 +123,11 @@ struct qt_batch {
     std::vector<qt::StepDesc> h_steps;
     std::vector<uint32_t> h_tabs;
     std::vector<int64_t> h_refs;
+    std::vector<int32_t> h_leaf_lds;     // per leaf: offset in the constant area, or (1 << 31) | parameter offset
+    std::vector<qt::CstEntry> h_csts;
+    int32_t* d_leaf_lds = nullptr;
+    qt::CstEntry* d_csts = nullptr;
+    size_t cap_leaf_lds = 0, cap_csts = 0;
     std::vector<int64_t> prog_const_end, prog_param_end;  // pool extents each program reads
     size_t cap_progs = 0, cap_steps = 0, cap_tabs = 0, cap_refs = 0;
     int64_t pool_cap = 0;           // elements allocated for the pool
@@ -211,8 +227,11 @@ __device__ __forceinline__ void run_step(PA A, PB B, V* C, const uint32_t* __res
         const uint32_t g0 = gs[0], g1 = gs[1], g2 = gs[2], g3 = gs[3];
         const int a0 = int(g0 & 0xffffu), a1 = int(g1 & 0xffffu), a2 = int(g2 & 0xffffu), a3 = int(g3 & 0xffffu);
         const int b0 = int(g0 >> 16), b1 = int(g1 >> 16), b2 = int(g2 >> 16), b3 = int(g3 >> 16);
+        // the next output's table entry is loaded while this one is computed
+        uint32_t f_next = lane < nC ? fc[lane] : 0u;
         for (int c = lane; c < nC; c += LANES) {
-            const uint32_t f = fc[c];
+            const uint32_t f = f_next;
+            if (c + LANES < nC) f_next = fc[c + LANES];
             const auto pa = A + int(f & 0xffffu);
             const auto pb = B + int(f >> 16);
             const V x0 = pa[a0], y0 = pb[b0], x1 = pa[a1], y1 = pb[b1];
@@ -226,8 +245,10 @@ __device__ __forceinline__ void run_step(PA A, PB B, V* C, const uint32_t* __res
         }
         return;
     }
+    uint32_t f_next = lane < nC ? fc[lane] : 0u;
     for (int c = lane; c < nC; c += LANES) {
-        const uint32_t f = fc[c];
+        const uint32_t f = f_next;
+        if (c + LANES < nC) f_next = fc[c + LANES];
         const auto pa = A + int(f & 0xffffu);
         const auto pb = B + int(f >> 16);
         Acc<V, MODE> acc;
@@ -318,19 +339,21 @@ __global__ __launch_bounds__(kWg) void k_batch(const ProgHdr* __restrict__ progs
 // Several instances of one program per wavefront: lane l serves instance slot l / G
 // with G lanes; the wave takes a chunk of up to 64 / G consecutive instances of the
 // same program (built on the host) and walks the program once for all of them.
-// Leaf operands are staged in LDS once per step (constant leaves once per wave,
-// parameter leaves once per slot), so every multiply-add reads LDS, not the pool.
-// LDS per wave: [W slot scratch areas (LDS mode)] [W x 2 parameter stages] [2 constant stages].
+// With STAGE, a wave copies its program's distinct constant leaves to LDS once per
+// chunk, and every slot its instance's parameter block, so the steps read leaves
+// from LDS; without it, leaves are read from the pool through the caches.
+// LDS per wave: [W slot scratch areas (LDS mode)] [W parameter blocks] [constant leaves].
 template <class V, int MODE, int G, bool GLOBAL_SCRATCH, bool STAGE>
 __global__ __launch_bounds__(64) void k_batch_grouped(const ProgHdr* __restrict__ progs,
                                                       const StepDesc* __restrict__ steps,
                                                       const uint32_t* __restrict__ tabs,
                                                       const int64_t* __restrict__ leaf_refs,
+                                                      const int32_t* __restrict__ leaf_lds,
+                                                      const CstEntry* __restrict__ csts,
                                                       const V* __restrict__ pool, const Instance* __restrict__ inst,
                                                       const int2* __restrict__ chunks, int n_chunks,
                                                       int32_t* __restrict__ counter, double2* __restrict__ out,
-                                                      V* __restrict__ gscratch, int slot_stride, int pstage,
-                                                      int cstage) {
+                                                      V* __restrict__ gscratch, int slot_stride, int pstage) {
     extern __shared__ unsigned char lds_raw[];
     constexpr int W = 64 / G;
     constexpr int64_t kParam = int64_t(1) << 62;
@@ -343,8 +366,8 @@ __global__ __launch_bounds__(64) void k_batch_grouped(const ProgHdr* __restrict_
         mem = lds + slot * slot_stride;
         lds += W * slot_stride;
     }
-    V* const pst = lds + slot * 2 * pstage;  // this slot's parameter-leaf stages
-    V* const cst = lds + W * 2 * pstage;     // the wave's constant-leaf stages
+    V* const prm = lds + slot * pstage;  // this slot's parameter block
+    V* const cst = lds + W * pstage;     // the program's constant leaves
     while (true) {
         int t = 0;
         if (lane == 0) t = atomicAdd(counter, 1);
@@ -356,6 +379,18 @@ __global__ __launch_bounds__(64) void k_batch_grouped(const ProgHdr* __restrict_
         const int64_t pbase = inst[i].param_base;
         const ProgHdr P = progs[__builtin_amdgcn_readfirstlane(inst[ch.x].prog)];
         const int64_t* refs = leaf_refs + P.leaf_begin;
+        const int32_t* lofs = leaf_lds + P.leaf_begin;
+        if constexpr (STAGE) {
+            exec_sync<64, GLOBAL_SCRATCH>();  // the previous chunk's steps are done with the leaves
+            for (int j = 0; j < P.n_cst; ++j) {
+                const CstEntry e = csts[P.cst_begin + j];
+                const V* src = pool + P.const_base + e.src;
+                for (int x = lane; x < e.n; x += 64) cst[e.dst + x] = src[x];
+            }
+            if (active)
+                for (int x = gl; x < P.param_elems; x += G) prm[x] = pool[pbase + x];
+            exec_sync<64, GLOBAL_SCRATCH>();
+        }
         for (int si = 0; si < P.n_steps; ++si) {
             const StepDesc st = steps[P.step_begin + si];
             const uint32_t* fc = tabs + st.tab;
@@ -363,44 +398,24 @@ __global__ __launch_bounds__(64) void k_batch_grouped(const ProgHdr* __restrict_
             const uint32_t* gs = fc + nC;
             double2* fin = (st.flags & 1) ? out + i : nullptr;
             V* C = mem + st.c;
-            // copies leaf `code` (1 << bits elements) into stage `which`; without
-            // staging, the leaf in the pool
-            auto stage = [&](int code, int bits, int which) -> const V* {
-                const int64_t r = refs[-1 - code];
-                const int n = 1 << bits;
-                if constexpr (!STAGE) {
-                    (void)n;
-                    (void)which;
+            auto leaf = [&](int code) -> const V* {
+                if constexpr (STAGE) {
+                    const int32_t o = lofs[-1 - code];
+                    return o < 0 ? prm + (o & 0x7fffffff) : cst + o;
+                } else {
+                    const int64_t r = refs[-1 - code];
                     return pool + ((r & kParam) ? pbase + (r & ~kParam) : P.const_base + r);
                 }
-                if (r & kParam) {
-                    V* dst = pst + which * pstage;
-                    const V* src = pool + pbase + (r & ~kParam);
-                    if (active)
-                        for (int e = gl; e < n; e += G) dst[e] = src[e];
-                    return dst;
-                }
-                V* dst = cst + which * cstage;
-                const V* src = pool + P.const_base + r;
-                for (int e = lane; e < n; e += 64) dst[e] = src[e];
-                return dst;
             };
-            const int a_bits = st.pad & 0xff, b_bits = (st.pad >> 8) & 0xff;
-            if (st.a < 0 && st.b < 0) {
-                const V* sa = stage(st.a, a_bits, 0);
-                const V* sb = stage(st.b, b_bits, 1);
-                if constexpr (STAGE) exec_sync<64, GLOBAL_SCRATCH>();
-                if (active) run_step<V, MODE, G>(sa, sb, C, fc, gs, nC, K, gl, fin);
-            } else if (st.a < 0) {
-                const V* sa = stage(st.a, a_bits, 0);
-                if constexpr (STAGE) exec_sync<64, GLOBAL_SCRATCH>();
-                if (active) run_step<V, MODE, G>(sa, (const V*)(mem + st.b), C, fc, gs, nC, K, gl, fin);
-            } else if (st.b < 0) {
-                const V* sb = stage(st.b, b_bits, 1);
-                if constexpr (STAGE) exec_sync<64, GLOBAL_SCRATCH>();
-                if (active) run_step<V, MODE, G>((const V*)(mem + st.a), sb, C, fc, gs, nC, K, gl, fin);
-            } else if (active) {
-                run_step<V, MODE, G>((const V*)(mem + st.a), (const V*)(mem + st.b), C, fc, gs, nC, K, gl, fin);
+            if (active) {
+                if (st.a < 0 && st.b < 0)
+                    run_step<V, MODE, G>(leaf(st.a), leaf(st.b), C, fc, gs, nC, K, gl, fin);
+                else if (st.a < 0)
+                    run_step<V, MODE, G>(leaf(st.a), (const V*)(mem + st.b), C, fc, gs, nC, K, gl, fin);
+                else if (st.b < 0)
+                    run_step<V, MODE, G>((const V*)(mem + st.a), leaf(st.b), C, fc, gs, nC, K, gl, fin);
+                else
+                    run_step<V, MODE, G>((const V*)(mem + st.a), (const V*)(mem + st.b), C, fc, gs, nC, K, gl, fin);
             }
             exec_sync<64, GLOBAL_SCRATCH>();
         }
@@ -460,7 +475,7 @@ inline int execs_per_wg(int lanes) { return lanes >= 64 ? kWg / lanes : 64 / lan
 inline size_t grouped_lds_elems(const qt_batch* b) {
     const size_t w = size_t(64 / b->lanes);
     return (b->use_global_scratch ? 0 : w * size_t(b->slot_stride)) +
-           (b->stage_leaves ? 2 * w * size_t(b->pstage) + 2 * size_t(b->cstage) : 0);
+           (b->stage_leaves ? w * size_t(b->pstage) + size_t(b->cstage) : 0);
 }
 inline int grid_wgs(const qt_batch* b) { return b->n_cus * b->wgs_per_cu * (b->lanes >= 64 ? 1 : 4); }
 
@@ -480,8 +495,8 @@ qt_status launch_grouped_t(qt_batch* b) {
     }
 #define QT_LAUNCH_GST(GL, GS, ST)                                                                                  \
     hipLaunchKernelGGL((k_batch_grouped<V, MODE, GL, GS, ST>), dim3(grid), dim3(64), lds, c->stream, b->d_progs,   \
-                       b->d_steps, b->d_tabs, b->d_leaf_refs, pool, b->d_inst, b->d_chunks, b->n_chunks,           \
-                       b->d_counter, d_out, gs, b->slot_stride, b->pstage, b->cstage)
+                       b->d_steps, b->d_tabs, b->d_leaf_refs, b->d_leaf_lds, b->d_csts, pool, b->d_inst,           \
+                       b->d_chunks, b->n_chunks, b->d_counter, d_out, gs, b->slot_stride, b->pstage)
 #define QT_LAUNCH_GS(GS)                      \
     switch (b->lanes) {                       \
         case 4: QT_LAUNCH_G(4, GS); break;    \
@@ -557,7 +572,7 @@ qt_status configure_scratch(qt_batch* b) {
     b->slot_stride = b->lanes < 64 ? b->scratch_elems + 1 : b->scratch_elems;
     // leaf stages of lane-grouped waves (LDS in either placement)
     b->stage_leaves = b->lanes < 64 && (b->scratch_place & QT_SCRATCH_STAGE_LEAVES);
-    const size_t stage = b->stage_leaves ? (2 * size_t(kExec) * size_t(b->pstage) + 2 * size_t(b->cstage)) * es : 0;
+    const size_t stage = b->stage_leaves ? (size_t(kExec) * size_t(b->pstage) + size_t(b->cstage)) * es : 0;
     const size_t lds_need = size_t(kExec) * size_t(b->slot_stride) * es + stage;
     const size_t lds_cap = 160 * 1024;
     const size_t wgs_on_cu = size_t(b->wgs_per_cu) * (b->lanes >= 64 ? 1 : 4);
@@ -638,19 +653,32 @@ qt_status append_programs(qt_batch* b, int n, qt_program* const* progs, const in
         }
         b->h_tabs.insert(b->h_tabs.end(), p->tables.begin(), p->tables.end());
         int64_t cend = 0, pend = 0;
+        std::map<int64_t, int32_t> cst_at;  // distinct constant leaves -> offset in the LDS area
+        int32_t cst_elems = 0;
+        h.cst_begin = int32_t(b->h_csts.size());
         for (size_t l = 0; l < p->leaf_refs.size(); ++l) {
             const int64_t r = p->leaf_refs[l];
             const int32_t elems = int32_t(1) << (p->leaf_ranks[l] * p->dim_log2);
             const int64_t end = (r & ~QT_LEAF_PARAM) + elems;
             if (r & QT_LEAF_PARAM) {
                 pend = std::max(pend, end);
-                b->pstage = std::max(b->pstage, elems);
+                b->h_leaf_lds.push_back(int32_t(uint32_t(1) << 31 | uint32_t(r & ~QT_LEAF_PARAM)));
             } else {
                 cend = std::max(cend, const_base[i] + end);
-                b->cstage = std::max(b->cstage, elems);
+                auto it = cst_at.find(r);
+                if (it == cst_at.end()) {
+                    it = cst_at.emplace(r, cst_elems).first;
+                    b->h_csts.push_back(CstEntry{r, elems, cst_elems});
+                    cst_elems += elems;
+                }
+                b->h_leaf_lds.push_back(it->second);
             }
             b->h_refs.push_back(r);
         }
+        h.n_cst = int32_t(b->h_csts.size()) - h.cst_begin;
+        h.param_elems = int32_t(pend);
+        b->pstage = std::max(b->pstage, int32_t(pend));
+        b->cstage = std::max(b->cstage, cst_elems);
         b->prog_const_end.push_back(cend);
         b->prog_param_end.push_back(pend);
         b->max_leaf_ref_end = std::max(b->max_leaf_ref_end, cend);
@@ -665,6 +693,8 @@ qt_status append_programs(qt_batch* b, int n, qt_program* const* progs, const in
     if (s == QT_OK) s = dev_put(c, &b->d_steps, &b->cap_steps, b->h_steps.data(), b->h_steps.size());
     if (s == QT_OK) s = dev_put(c, &b->d_tabs, &b->cap_tabs, b->h_tabs.data(), b->h_tabs.size());
     if (s == QT_OK) s = dev_put(c, &b->d_leaf_refs, &b->cap_refs, b->h_refs.data(), b->h_refs.size());
+    if (s == QT_OK) s = dev_put(c, &b->d_leaf_lds, &b->cap_leaf_lds, b->h_leaf_lds.data(), b->h_leaf_lds.size());
+    if (s == QT_OK) s = dev_put(c, &b->d_csts, &b->cap_csts, b->h_csts.data(), b->h_csts.size());
     if (s == QT_OK && configure) s = configure_scratch(b);
     return s;
 }
@@ -1225,7 +1255,8 @@ qt_status qt_batch_destroy(qt_batch* b) {
         if (b->ev_done) (void)hipEventDestroy(b->ev_done);
         for (void* p : {(void*)b->d_progs, (void*)b->d_steps, (void*)b->d_tabs, (void*)b->d_leaf_refs, b->d_pool,
                         (void*)b->d_inst, (void*)b->d_out, (void*)b->d_counter, (void*)b->d_seg, (void*)b->d_members,
-                        (void*)b->d_group, b->d_global_scratch, (void*)b->d_chunks, b->d_pool_stage})
+                        (void*)b->d_group, b->d_global_scratch, (void*)b->d_chunks, b->d_pool_stage,
+                        (void*)b->d_leaf_lds, (void*)b->d_csts})
             if (p) (void)hipFree(p);
         --b->ctx->live;
     }
