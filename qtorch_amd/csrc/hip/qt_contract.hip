@@ -247,16 +247,29 @@ __global__ __launch_bounds__(kThreads) void k_reduce_partial(const V* __restrict
 }
 
 template <class V>
-__global__ void k_reduce_final(const V* __restrict__ partials, V* __restrict__ C, int64_t nC,
-                               int64_t nblocks) {
-    const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (c >= nC) return;
-    V t = partials[c];
-    for (int64_t b = 1; b < nblocks; ++b) {
+__global__ __launch_bounds__(kThreads) void k_reduce_final(const V* __restrict__ partials, V* __restrict__ C,
+                                                           int64_t nC, int64_t nblocks) {
+    // one workgroup per output: strided partial sums, then a fixed-shape tree (the
+    // order does not depend on timing, so the result is reproducible)
+    __shared__ V part[kThreads];
+    const int64_t c = blockIdx.x;
+    V t;
+    t.x = 0;
+    t.y = 0;
+    for (int64_t b = threadIdx.x; b < nblocks; b += kThreads) {
         t.x += partials[b * nC + c].x;
         t.y += partials[b * nC + c].y;
     }
-    C[c] = t;
+    part[threadIdx.x] = t;
+    __syncthreads();
+    for (int d = kThreads / 2; d >= 1; d >>= 1) {
+        if (int(threadIdx.x) < d) {
+            part[threadIdx.x].x += part[threadIdx.x + d].x;
+            part[threadIdx.x].y += part[threadIdx.x + d].y;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) C[c] = part[0];
 }
 
 // LDS-staged bit permutation (see PermPlan).  Reads walk the input-low bits
@@ -583,8 +596,7 @@ qt_status run_reduce(Ctx* c, const V* A, const V* B, V* C, const BitMap& m) {
                        c->stream, A, B, part, m, nC, chunk);
     qt_status s = check_hip(c, hipGetLastError(), "k_reduce_partial");
     if (s != QT_OK) return s;
-    hipLaunchKernelGGL((k_reduce_final<V>), dim3(grid_for(nC, 64, 1024)), dim3(64), 0, c->stream,
-                       part, C, nC, nblocks);
+    hipLaunchKernelGGL((k_reduce_final<V>), dim3(unsigned(nC)), dim3(kThreads), 0, c->stream, part, C, nC, nblocks);
     return check_hip(c, hipGetLastError(), "k_reduce_final");
 }
 

@@ -5,6 +5,7 @@
 #include <cstring>
 #include <map>
 #include <sstream>
+#include <unordered_map>
 
 #include "batch.hpp"
 
@@ -101,6 +102,131 @@ qt_program* CompileProgram(qt_ctx* ctx, const Network& net, const std::vector<in
     return p;
 }
 
+namespace {
+
+// One pairwise contraction on the host in FP64 (the qt_contract_pair conventions:
+// digit i of C is free axis free_src[i], digit j of the summed index is pair k-1-j).
+std::vector<std::complex<double>> HostContract(const std::vector<std::complex<double>>& A,
+                                               const std::vector<std::complex<double>>& B, const PlanStep& st,
+                                               int dl) {
+    const int k = st.k, rc = st.rank_c;
+    std::vector<int> c_src, s_a, s_b;  // bit positions (B's bits + 64)
+    for (int i = 0; i < rc; ++i) {
+        const int32_t f = st.free_src[size_t(i)];
+        const int ax = f & ~QT_FROM_A;
+        for (int t = 0; t < dl; ++t) c_src.push_back((f & QT_FROM_A) ? ax * dl + t : 64 + ax * dl + t);
+    }
+    s_a.resize(size_t(k * dl));
+    s_b.resize(size_t(k * dl));
+    for (int i = 0; i < k; ++i)
+        for (int t = 0; t < dl; ++t) {
+            s_a[size_t((k - 1 - i) * dl + t)] = st.pair_a[size_t(i)] * dl + t;
+            s_b[size_t((k - 1 - i) * dl + t)] = st.pair_b[size_t(i)] * dl + t;
+        }
+    const int64_t nC = int64_t(1) << (rc * dl), nS = int64_t(1) << (k * dl);
+    std::vector<int64_t> ga(static_cast<size_t>(nS)), gb(static_cast<size_t>(nS));
+    for (int64_t s = 0; s < nS; ++s) {
+        int64_t a = 0, b = 0;
+        for (int j = 0; j < k * dl; ++j)
+            if ((s >> j) & 1) {
+                a += int64_t(1) << s_a[size_t(j)];
+                b += int64_t(1) << s_b[size_t(j)];
+            }
+        ga[size_t(s)] = a;
+        gb[size_t(s)] = b;
+    }
+    std::vector<std::complex<double>> C(static_cast<size_t>(nC));
+    for (int64_t c = 0; c < nC; ++c) {
+        int64_t a = 0, b = 0;
+        for (int j = 0; j < rc * dl; ++j)
+            if ((c >> j) & 1) {
+                if (c_src[size_t(j)] < 64) a += int64_t(1) << c_src[size_t(j)];
+                else b += int64_t(1) << (c_src[size_t(j)] - 64);
+            }
+        std::complex<double> acc = 0.0;
+        for (int64_t s = 0; s < nS; ++s) acc += A[size_t(a + ga[size_t(s)])] * B[size_t(b + gb[size_t(s)])];
+        C[size_t(c)] = acc;
+    }
+    return C;
+}
+
+}  // namespace
+
+qt_program* CompileProgramFolded(const Network& net, const std::vector<int64_t>& param_refs,
+                                 const std::vector<std::vector<std::complex<double>>>& values, LeafPool& pool,
+                                 int64_t max_fold_elems, int64_t* folded_cmacs) {
+    const int n_leaves = net.GetNumOriginalNodes();
+    const auto& all = net.GetAllNodes();
+    const auto& plan = net.GetPlan();
+    const int dl = net.GetDimLog2();
+    if (plan.empty() || int(param_refs.size()) != n_leaves || int(values.size()) != n_leaves)
+        throw InvalidFunctionInput();
+    const int n_ops = n_leaves + int(plan.size());
+    std::vector<char> cst(static_cast<size_t>(n_ops), 0);
+    std::vector<std::vector<std::complex<double>>> val(static_cast<size_t>(n_ops));
+    std::vector<int> rank(static_cast<size_t>(n_ops), 0);
+    for (int i = 0; i < n_leaves; ++i) {
+        cst[size_t(i)] = param_refs[size_t(i)] < 0;
+        if (cst[size_t(i)]) val[size_t(i)] = values[size_t(i)];
+        rank[size_t(i)] = all[size_t(i)]->mRank;
+    }
+    std::unordered_map<int, int> op_of_node;
+    for (int i = 0; i < n_leaves; ++i) op_of_node[i] = i;
+    std::vector<std::pair<int, int>> ops(plan.size());
+    std::vector<char> kept(plan.size(), 1);
+    int64_t folded = 0;
+    for (size_t s = 0; s < plan.size(); ++s) {
+        const auto& st = plan[s];
+        const int a = op_of_node.at(st.a_id), b = op_of_node.at(st.b_id), c = n_leaves + int(s);
+        ops[s] = {a, b};
+        if (st.c_id >= 0) op_of_node[st.c_id] = c;
+        rank[size_t(c)] = st.rank_c;
+        const int64_t elems = int64_t(1) << (st.rank_c * dl);
+        // constant folding: both operands known here, result small enough for a leaf
+        if (s + 1 < plan.size() && cst[size_t(a)] && cst[size_t(b)] && elems <= max_fold_elems) {
+            val[size_t(c)] = HostContract(val[size_t(a)], val[size_t(b)], st, dl);
+            cst[size_t(c)] = 1;
+            kept[s] = 0;
+            folded += int64_t(1) << ((st.rank_c + st.k) * dl);
+            val[size_t(a)].clear();
+            val[size_t(b)].clear();
+        }
+    }
+    // device leaves: parameter leaves and constants consumed by device steps
+    std::vector<int> dev_id(static_cast<size_t>(n_ops), -1);
+    std::vector<int32_t> ranks;
+    std::vector<int64_t> refs;
+    for (size_t s = 0; s < plan.size(); ++s) {
+        if (!kept[s]) continue;
+        for (int o : {ops[s].first, ops[s].second}) {
+            const bool leaf_like = o < n_leaves || cst[size_t(o)];
+            if (!leaf_like || dev_id[size_t(o)] >= 0) continue;
+            dev_id[size_t(o)] = int(refs.size());
+            ranks.push_back(rank[size_t(o)]);
+            refs.push_back(cst[size_t(o)] ? pool.Add(val[size_t(o)]) : param_refs[size_t(o)]);
+        }
+    }
+    const int n_dev_leaves = int(refs.size());
+    std::vector<int32_t> desc, axes;
+    int n_steps = 0;
+    for (size_t s = 0; s < plan.size(); ++s) {
+        if (!kept[s]) continue;
+        dev_id[size_t(n_leaves) + s] = n_dev_leaves + n_steps++;
+        const auto& st = plan[s];
+        desc.insert(desc.end(), {dev_id[size_t(ops[s].first)], dev_id[size_t(ops[s].second)], st.rank_a, st.rank_b,
+                                 st.k, st.rank_c});
+        axes.insert(axes.end(), st.pair_a.begin(), st.pair_a.end());
+        axes.insert(axes.end(), st.pair_b.begin(), st.pair_b.end());
+        axes.insert(axes.end(), st.free_src.begin(), st.free_src.end());
+    }
+    if (folded_cmacs) *folded_cmacs = folded;
+    qt_program* p = nullptr;
+    CheckQt(nullptr,
+            qt_program_create(nullptr, dl, n_dev_leaves, ranks.data(), refs.data(), n_steps, desc.data(), axes.data(), &p),
+            "qt_program_create");
+    return p;
+}
+
 void OrderNetwork(Network& net, int order, uint64_t seed, const std::vector<std::pair<int, int>>* given) {
     if (order == kOrderGreedy) {
         GreedyContract(net);
@@ -160,10 +286,12 @@ CompiledQaoa CompileQaoa(const ExtraData& data, int p, int order, uint64_t seed,
         // leaves (before ordering: contraction clears a node's data): angle-dependent
         // Rz / Rx gates read the instance's parameter block, the rest the shared pool
         std::vector<int64_t> refs;
+        std::vector<std::vector<std::complex<double>>> vals;
         int n_rz = 0, n_rx = 0;
         const int n_iter = int(data.realIterations[e].size()), nq = data.qubitsNeeded[e];
         for (int i = 0; i < net.GetNumOriginalNodes(); ++i) {
             const auto& node = net.GetAllNodes()[size_t(i)];
+            vals.push_back(node->mHostVals);
             if (node->GetTypeOfNode() == GateType::RZ) {
                 const int layer = n_iter ? n_rz++ / n_iter : 0;
                 refs.push_back(QT_LEAF_PARAM | int64_t(32 * layer));
@@ -171,13 +299,15 @@ CompiledQaoa CompileQaoa(const ExtraData& data, int p, int order, uint64_t seed,
                 const int layer = n_rx++ / nq;
                 refs.push_back(QT_LEAF_PARAM | int64_t(32 * layer + 16));
             } else {
-                refs.push_back(pool.Add(node->mHostVals));
+                refs.push_back(-1);  // angle-free: folded with its neighbours where possible
             }
         }
         OrderNetwork(net, order, seed + e, nullptr);
         out.edge_cmacs.push_back(net.getNumFloatOps());
         out.cmacs += net.getNumFloatOps();
-        out.progs.push_back(CompileProgram(nullptr, net, refs));
+        int64_t folded = 0;
+        out.progs.push_back(CompileProgramFolded(net, refs, vals, pool, int64_t(1) << 12, &folded));
+        out.folded_cmacs += folded;
         int64_t scratch = 0;
         (void)qt_program_stats(out.progs.back(), nullptr, &scratch, nullptr);
         out.max_scratch = std::max(out.max_scratch, scratch);

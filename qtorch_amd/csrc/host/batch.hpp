@@ -1,6 +1,7 @@
 // Batched evaluation of many networks on the device (see batch.cpp).
 #pragma once
 
+#include <complex>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -34,22 +35,34 @@ void CheckQt(qt_ctx* ctx, qt_status s, const char* what);
 // ctx may be NULL (compilation is host work).
 qt_program* CompileProgram(qt_ctx* ctx, const Network& net, const std::vector<int64_t>& leaf_refs);
 
+// The same with constant folding: leaf i is a parameter when param_refs[i] >= 0
+// (that reference, QT_LEAF_PARAM | offset), else a constant with values[i].  Every
+// step whose two operands are constants (and whose result has at most
+// max_fold_elems elements) is evaluated here in FP64 and its result becomes a
+// constant leaf (added to pool); the last step always runs on the device.
+qt_program* CompileProgramFolded(const Network& net, const std::vector<int64_t>& param_refs,
+                                 const std::vector<std::vector<std::complex<double>>>& values, LeafPool& pool,
+                                 int64_t max_fold_elems, int64_t* folded_cmacs);
+
 // F_p's edge light cones (src/maxcut.cpp:171-198) compiled once: one program per
 // edge, constant leaves added to `pool` (references from 0), the Rz / Rx gates of
-// layer i read parameter elements 32 i and 32 i + 16.  Throws DeviceFailure when a
-// light cone's tensors exceed a program's limit.
+// layer i read parameter elements 32 i and 32 i + 16; the angle-free parts of each
+// light cone are contracted here once (constant folding).  Throws DeviceFailure
+// when a light cone's tensors exceed a program's limit.
 struct CompiledQaoa {
     CompiledQaoa() = default;
     CompiledQaoa(CompiledQaoa&& o) noexcept
-        : progs(std::move(o.progs)), edge_cmacs(std::move(o.edge_cmacs)), cmacs(o.cmacs), max_scratch(o.max_scratch) {
+        : progs(std::move(o.progs)), edge_cmacs(std::move(o.edge_cmacs)), cmacs(o.cmacs),
+          folded_cmacs(o.folded_cmacs), max_scratch(o.max_scratch) {
         o.progs.clear();
     }
     CompiledQaoa(const CompiledQaoa&) = delete;
     CompiledQaoa& operator=(const CompiledQaoa&) = delete;
     ~CompiledQaoa();
     std::vector<qt_program*> progs;
-    std::vector<int64_t> edge_cmacs;
+    std::vector<int64_t> edge_cmacs;  // the reference's counter per edge network
     int64_t cmacs = 0;
+    int64_t folded_cmacs = 0;         // of those, evaluated once here (angle-free subtrees)
     int64_t max_scratch = 0;
 };
 CompiledQaoa CompileQaoa(const ExtraData& data, int p, int order, uint64_t seed, LeafPool& pool);

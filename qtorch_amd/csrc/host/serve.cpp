@@ -127,17 +127,19 @@ std::shared_ptr<Server::Structure> BuildStructure(const ServeRequest& r, const S
             // the output bitstring's <0| / <1| leaves are the request's parameters
             const int n_orig = net.GetNumOriginalNodes();
             std::vector<int64_t> refs;
+            std::vector<std::vector<std::complex<double>>> vals;
             LeafPool pool;
             for (int i = 0; i < n_orig; ++i) {
                 const int m = i - (n_orig - st->n_qubits);
-                if (m >= 0) refs.push_back(QT_LEAF_PARAM | int64_t(2 * m));
-                else refs.push_back(pool.Add(net.GetAllNodes()[size_t(i)]->mHostVals));
+                refs.push_back(m >= 0 ? (QT_LEAF_PARAM | int64_t(2 * m)) : -1);
+                vals.push_back(net.GetAllNodes()[size_t(i)]->mHostVals);
             }
             st->param_elems = 2 * int64_t(st->n_qubits);
             GreedyContract(net);
             st->cmacs = net.getNumFloatOps();
             try {
-                qt_program* p = CompileProgram(nullptr, net, refs);
+                // the circuit's bitstring-free parts are contracted once, here
+                qt_program* p = CompileProgramFolded(net, refs, vals, pool, int64_t(1) << 12, nullptr);
                 st->progs.push_back(p);
                 int64_t scratch = 0;
                 (void)qt_program_stats(p, nullptr, &scratch, nullptr);
