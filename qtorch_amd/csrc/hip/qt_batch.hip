@@ -227,11 +227,7 @@ __device__ __forceinline__ void run_step(PA A, PB B, V* C, const uint32_t* __res
         const uint32_t g0 = gs[0], g1 = gs[1], g2 = gs[2], g3 = gs[3];
         const int a0 = int(g0 & 0xffffu), a1 = int(g1 & 0xffffu), a2 = int(g2 & 0xffffu), a3 = int(g3 & 0xffffu);
         const int b0 = int(g0 >> 16), b1 = int(g1 >> 16), b2 = int(g2 >> 16), b3 = int(g3 >> 16);
-        // the next output's table entry is loaded while this one is computed
-        uint32_t f_next = lane < nC ? fc[lane] : 0u;
-        for (int c = lane; c < nC; c += LANES) {
-            const uint32_t f = f_next;
-            if (c + LANES < nC) f_next = fc[c + LANES];
+        auto one = [&](int c, uint32_t f) {
             const auto pa = A + int(f & 0xffffu);
             const auto pb = B + int(f >> 16);
             const V x0 = pa[a0], y0 = pb[b0], x1 = pa[a1], y1 = pb[b1];
@@ -242,7 +238,34 @@ __device__ __forceinline__ void run_step(PA A, PB B, V* C, const uint32_t* __res
             acc.mac(x2, y2);
             acc.mac(x3, y3);
             put(c, acc);
+        };
+        // two outputs per lane per iteration (16 operand loads in flight); the next
+        // pair's table entries are loaded while this pair is computed
+        int c = lane;
+        uint32_t f0 = c < nC ? fc[c] : 0u, f1 = c + LANES < nC ? fc[c + LANES] : 0u;
+        for (; c + LANES < nC; c += 2 * LANES) {
+            const uint32_t g0 = f0, g1 = f1;
+            if (c + 2 * LANES < nC) f0 = fc[c + 2 * LANES];
+            if (c + 3 * LANES < nC) f1 = fc[c + 3 * LANES];
+            const auto pa0 = A + int(g0 & 0xffffu), pa1 = A + int(g1 & 0xffffu);
+            const auto pb0 = B + int(g0 >> 16), pb1 = B + int(g1 >> 16);
+            const V x0 = pa0[a0], y0 = pb0[b0], x1 = pa0[a1], y1 = pb0[b1];
+            const V x2 = pa0[a2], y2 = pb0[b2], x3 = pa0[a3], y3 = pb0[b3];
+            const V u0 = pa1[a0], v0 = pb1[b0], u1 = pa1[a1], v1 = pb1[b1];
+            const V u2 = pa1[a2], v2 = pb1[b2], u3 = pa1[a3], v3 = pb1[b3];
+            Acc<V, MODE> acc, acd;
+            acc.mac(x0, y0);
+            acd.mac(u0, v0);
+            acc.mac(x1, y1);
+            acd.mac(u1, v1);
+            acc.mac(x2, y2);
+            acd.mac(u2, v2);
+            acc.mac(x3, y3);
+            acd.mac(u3, v3);
+            put(c, acc);
+            put(c + LANES, acd);
         }
+        if (c < nC) one(c, f0);
         return;
     }
     uint32_t f_next = lane < nC ? fc[lane] : 0u;
