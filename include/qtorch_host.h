@@ -131,6 +131,7 @@ int qth_netbatch_cmacs(qth_netbatch* b, int64_t* cmacs);   /* complex MACs per n
 int qth_netbatch_destroy(qth_netbatch* b);
 
 /* ---- serving: continuous batching of QAOA-energy and amplitude requests ----------
+ * (no counterpart in the reference, which evaluates one network at a time)
  * Requests are submitted from any thread and finish asynchronously.  Each request's
  * network is planned once per structure and then runs as a program instance in the
  * next launch of an idle stream (or, when its tensors exceed a program's limit, on
@@ -166,6 +167,7 @@ int qth_serve_plan(int kind, const char* text, int p, int ref_map, const char* b
                    int* route, int64_t* planned_bytes, int64_t* cmacs);
 
 /* ---- one large network, sliced --------------------------------------------------
+ * (no counterpart in the reference, whose ContractionTools contract one unsliced network)
  * Planned once in `order` (QTH_ORDER_*), then wires are sliced until every tensor
  * has at most max_bits index bits; the D^s slices run step by step (MFMA GEMMs for
  * large steps) in the given form (a complex64 form may be or-ed with QT_RESCALE for

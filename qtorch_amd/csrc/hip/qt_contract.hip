@@ -361,18 +361,35 @@ __global__ __launch_bounds__(kThreads) void k_gemm_c128(const double2* __restric
                 are[bm] = as_re[kl][mm];
                 aim[bm] = as_im[kl][mm];
             }
+            double bre[2], bim[2];
 #pragma unroll
             for (int bn = 0; bn < 2; ++bn) {
                 const int nn = wn * 32 + bn * 16 + li;
-                const double bre = bs_re[kl][nn], bim = bs_im[kl][nn];
-#pragma unroll
-                for (int bm = 0; bm < 2; ++bm) {
-                    acc_re[bn][bm] = __builtin_amdgcn_mfma_f64_16x16x4f64(bre, are[bm], acc_re[bn][bm], 0, 0, 0);
-                    acc_re[bn][bm] = __builtin_amdgcn_mfma_f64_16x16x4f64(-bim, aim[bm], acc_re[bn][bm], 0, 0, 0);
-                    acc_im[bn][bm] = __builtin_amdgcn_mfma_f64_16x16x4f64(bim, are[bm], acc_im[bn][bm], 0, 0, 0);
-                    acc_im[bn][bm] = __builtin_amdgcn_mfma_f64_16x16x4f64(bre, aim[bm], acc_im[bn][bm], 0, 0, 0);
-                }
+                bre[bn] = bs_re[kl][nn];
+                bim[bn] = bs_im[kl][nn];
             }
+            // the four products of a block go to two accumulators; issuing them in
+            // four rounds over the blocks keeps dependent MFMAs apart
+#pragma unroll
+            for (int bn = 0; bn < 2; ++bn)
+#pragma unroll
+                for (int bm = 0; bm < 2; ++bm)
+                    acc_re[bn][bm] = __builtin_amdgcn_mfma_f64_16x16x4f64(bre[bn], are[bm], acc_re[bn][bm], 0, 0, 0);
+#pragma unroll
+            for (int bn = 0; bn < 2; ++bn)
+#pragma unroll
+                for (int bm = 0; bm < 2; ++bm)
+                    acc_im[bn][bm] = __builtin_amdgcn_mfma_f64_16x16x4f64(bim[bn], are[bm], acc_im[bn][bm], 0, 0, 0);
+#pragma unroll
+            for (int bn = 0; bn < 2; ++bn)
+#pragma unroll
+                for (int bm = 0; bm < 2; ++bm)
+                    acc_re[bn][bm] = __builtin_amdgcn_mfma_f64_16x16x4f64(-bim[bn], aim[bm], acc_re[bn][bm], 0, 0, 0);
+#pragma unroll
+            for (int bn = 0; bn < 2; ++bn)
+#pragma unroll
+                for (int bm = 0; bm < 2; ++bm)
+                    acc_im[bn][bm] = __builtin_amdgcn_mfma_f64_16x16x4f64(bre[bn], aim[bm], acc_im[bn][bm], 0, 0, 0);
         }
         __syncthreads();
     }
@@ -460,32 +477,30 @@ __global__ __launch_bounds__(kThreads) void k_gemm_c64(const float2* __restrict_
             for (int bm = 0; bm < 2; ++bm)
 #pragma unroll
                 for (int q = 0; q < 4; ++q) a[bm][q] = ld(sa + q * kPlaneB, wm * 32 + bm * 16 + li);
+            bf8_t b[2][6];  // per bn: re_hi, re_lo, im_hi, im_lo, -im_hi, -im_lo
 #pragma unroll
             for (int bn = 0; bn < 2; ++bn) {
-                bf8_t b[4];
 #pragma unroll
-                for (int q = 0; q < 4; ++q) b[q] = ld(sb + q * kPlaneB, wn * 32 + bn * 16 + li);
-                const bf8_t nbih = neg(b[2]), nbil = neg(b[3]);
-#pragma unroll
-                for (int bm = 0; bm < 2; ++bm) {
-                    f4_t re = acc0[bn][bm], im = acc1[bn][bm];
-                    // re += br ar - bi ai; im += bi ar + br ai (each product hi*hi + hi*lo + lo*hi)
-                    re = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[0], a[bm][0], re, 0, 0, 0);
-                    re = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[0], a[bm][1], re, 0, 0, 0);
-                    re = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[1], a[bm][0], re, 0, 0, 0);
-                    re = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nbih, a[bm][2], re, 0, 0, 0);
-                    re = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nbih, a[bm][3], re, 0, 0, 0);
-                    re = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nbil, a[bm][2], re, 0, 0, 0);
-                    im = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[2], a[bm][0], im, 0, 0, 0);
-                    im = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[2], a[bm][1], im, 0, 0, 0);
-                    im = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[3], a[bm][0], im, 0, 0, 0);
-                    im = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[0], a[bm][2], im, 0, 0, 0);
-                    im = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[0], a[bm][3], im, 0, 0, 0);
-                    im = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[1], a[bm][2], im, 0, 0, 0);
-                    acc0[bn][bm] = re;
-                    acc1[bn][bm] = im;
-                }
+                for (int q = 0; q < 4; ++q) b[bn][q] = ld(sb + q * kPlaneB, wn * 32 + bn * 16 + li);
+                b[bn][4] = neg(b[bn][2]);
+                b[bn][5] = neg(b[bn][3]);
             }
+            // re += br ar - bi ai; im += bi ar + br ai, each product hi*hi + hi*lo + lo*hi;
+            // every round touches each of the eight accumulators once, so no MFMA waits
+            // for the one before it.  (B operand index, A operand index) per round:
+            constexpr int kRe[6][2] = {{0, 0}, {0, 1}, {1, 0}, {4, 2}, {4, 3}, {5, 2}};
+            constexpr int kIm[6][2] = {{2, 0}, {2, 1}, {3, 0}, {0, 2}, {0, 3}, {1, 2}};
+#pragma unroll
+            for (int t = 0; t < 6; ++t)
+#pragma unroll
+                for (int bn = 0; bn < 2; ++bn)
+#pragma unroll
+                    for (int bm = 0; bm < 2; ++bm) {
+                        acc0[bn][bm] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[bn][kRe[t][0]], a[bm][kRe[t][1]],
+                                                                               acc0[bn][bm], 0, 0, 0);
+                        acc1[bn][bm] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[bn][kIm[t][0]], a[bm][kIm[t][1]],
+                                                                               acc1[bn][bm], 0, 0, 0);
+                    }
         } else {
             float* sa = reinterpret_cast<float*>(smem);  // re plane, im plane of A', then of B'
             float* sb = sa + 2 * kPlaneF;
@@ -509,23 +524,38 @@ __global__ __launch_bounds__(kThreads) void k_gemm_c64(const float2* __restrict_
                     are[bm] = sa[o + wm * 32 + bm * 16 + li];
                     aim[bm] = sa[kPlaneF + o + wm * 32 + bm * 16 + li];
                 }
+                float bre[2], bim[2];
 #pragma unroll
                 for (int bn = 0; bn < 2; ++bn) {
-                    const float bre = sb[o + wn * 32 + bn * 16 + li], bim = sb[kPlaneF + o + wn * 32 + bn * 16 + li];
+                    bre[bn] = sb[o + wn * 32 + bn * 16 + li];
+                    bim[bn] = sb[kPlaneF + o + wn * 32 + bn * 16 + li];
+                }
+                // rounds over the four blocks (dependent MFMAs four apart)
+                if constexpr (MODE == MODE_3M) {
 #pragma unroll
-                    for (int bm = 0; bm < 2; ++bm) {
-                        if constexpr (MODE == MODE_3M) {
-                            acc0[bn][bm] = __builtin_amdgcn_mfma_f32_16x16x4f32(bre, are[bm], acc0[bn][bm], 0, 0, 0);
-                            acc1[bn][bm] = __builtin_amdgcn_mfma_f32_16x16x4f32(bim, aim[bm], acc1[bn][bm], 0, 0, 0);
-                            acc2[bn][bm] = __builtin_amdgcn_mfma_f32_16x16x4f32(bre + bim, are[bm] + aim[bm],
+                    for (int bn = 0; bn < 2; ++bn)
+#pragma unroll
+                        for (int bm = 0; bm < 2; ++bm) {
+                            acc0[bn][bm] = __builtin_amdgcn_mfma_f32_16x16x4f32(bre[bn], are[bm], acc0[bn][bm], 0, 0, 0);
+                            acc1[bn][bm] = __builtin_amdgcn_mfma_f32_16x16x4f32(bim[bn], aim[bm], acc1[bn][bm], 0, 0, 0);
+                            acc2[bn][bm] = __builtin_amdgcn_mfma_f32_16x16x4f32(bre[bn] + bim[bn], are[bm] + aim[bm],
                                                                                acc2[bn][bm], 0, 0, 0);
-                        } else {
-                            acc0[bn][bm] = __builtin_amdgcn_mfma_f32_16x16x4f32(bre, are[bm], acc0[bn][bm], 0, 0, 0);
-                            acc0[bn][bm] = __builtin_amdgcn_mfma_f32_16x16x4f32(-bim, aim[bm], acc0[bn][bm], 0, 0, 0);
-                            acc1[bn][bm] = __builtin_amdgcn_mfma_f32_16x16x4f32(bim, are[bm], acc1[bn][bm], 0, 0, 0);
-                            acc1[bn][bm] = __builtin_amdgcn_mfma_f32_16x16x4f32(bre, aim[bm], acc1[bn][bm], 0, 0, 0);
                         }
-                    }
+                } else {
+#pragma unroll
+                    for (int bn = 0; bn < 2; ++bn)
+#pragma unroll
+                        for (int bm = 0; bm < 2; ++bm) {
+                            acc0[bn][bm] = __builtin_amdgcn_mfma_f32_16x16x4f32(bre[bn], are[bm], acc0[bn][bm], 0, 0, 0);
+                            acc1[bn][bm] = __builtin_amdgcn_mfma_f32_16x16x4f32(bim[bn], are[bm], acc1[bn][bm], 0, 0, 0);
+                        }
+#pragma unroll
+                    for (int bn = 0; bn < 2; ++bn)
+#pragma unroll
+                        for (int bm = 0; bm < 2; ++bm) {
+                            acc0[bn][bm] = __builtin_amdgcn_mfma_f32_16x16x4f32(-bim[bn], aim[bm], acc0[bn][bm], 0, 0, 0);
+                            acc1[bn][bm] = __builtin_amdgcn_mfma_f32_16x16x4f32(bre[bn], aim[bm], acc1[bn][bm], 0, 0, 0);
+                        }
                 }
             }
         }
@@ -662,6 +692,135 @@ qt_status permute(Ctx* c, const void* in, void* out, int dtype, const PermPlan& 
 }
 
 namespace {
+
+// Large complex64 GEMMs in the 4M / 3M forms: 128x128 output tile per workgroup,
+// four waves of 64x64 (4x4 blocks of v_mfma_f32_16x16x4f32), k-tiles of 16; the next
+// k-tile is loaded into registers while the current one is multiplied from LDS.
+// LDS rows are 144 floats apart, so the four k-rows a wave reads at once sit in
+// different bank quarters.
+constexpr int BT = 128, BK = 16, BROW = BT + 16;
+
+template <int MODE>
+__global__ __launch_bounds__(kThreads) void k_gemm_c64_big(const float2* __restrict__ Ap,
+                                                           const float2* __restrict__ Bp, float2* __restrict__ C,
+                                                           int64_t M, int64_t N, int64_t K) {
+    __shared__ float sa[2][BK * BROW], sb[2][BK * BROW];  // [re|im][k][row]
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int li = lane & 15, lk = lane >> 4;
+    const int64_t tiles_m = M / BT;
+    const int64_t tile = int64_t(blockIdx.x) + int64_t(blockIdx.y) * gridDim.x;
+    if (tile >= tiles_m * (N / BT)) return;
+    const int64_t m0 = (tile % tiles_m) * BT, n0 = (tile / tiles_m) * BT;
+    constexpr int NA = MODE == MODE_3M ? 3 : 2;
+    f4_t acc[NA][4][4];
+#pragma unroll
+    for (int q = 0; q < NA; ++q)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[q][i][j] = f4_t{0, 0, 0, 0};
+    // each thread moves 8 elements of A' and 8 of B' per k-tile: element e = tid + 256 q
+    float2 ra[8], rb[8];
+    auto fetch = [&](int64_t k0) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int e = tid + kThreads * q, kk = e / BT, rr = e % BT;
+            ra[q] = Ap[(k0 + kk) * M + m0 + rr];
+            rb[q] = Bp[(k0 + kk) * N + n0 + rr];
+        }
+    };
+    fetch(0);
+    for (int64_t k0 = 0; k0 < K; k0 += BK) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int e = tid + kThreads * q, o = (e / BT) * BROW + e % BT;
+            sa[0][o] = ra[q].x;
+            sa[1][o] = ra[q].y;
+            sb[0][o] = rb[q].x;
+            sb[1][o] = rb[q].y;
+        }
+        __syncthreads();
+        if (k0 + BK < K) fetch(k0 + BK);
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 4) {
+            const int o = (kk + lk) * BROW;
+            float ar[4], ai[4], br[4], bi[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                ar[i] = sa[0][o + wm * 64 + i * 16 + li];
+                ai[i] = sa[1][o + wm * 64 + i * 16 + li];
+                br[i] = sb[0][o + wn * 64 + i * 16 + li];
+                bi[i] = sb[1][o + wn * 64 + i * 16 + li];
+            }
+            // one round of MFMAs over all 16 blocks per product, so that no MFMA waits
+            // for the one just before it
+            if constexpr (MODE == MODE_3M) {
+                float sr[4], si[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    sr[i] = br[i] + bi[i];
+                    si[i] = ar[i] + ai[i];
+                }
+#pragma unroll
+                for (int bn = 0; bn < 4; ++bn)
+#pragma unroll
+                    for (int bm = 0; bm < 4; ++bm)
+                        acc[0][bn][bm] = __builtin_amdgcn_mfma_f32_16x16x4f32(br[bn], ar[bm], acc[0][bn][bm], 0, 0, 0);
+#pragma unroll
+                for (int bn = 0; bn < 4; ++bn)
+#pragma unroll
+                    for (int bm = 0; bm < 4; ++bm)
+                        acc[1][bn][bm] = __builtin_amdgcn_mfma_f32_16x16x4f32(bi[bn], ai[bm], acc[1][bn][bm], 0, 0, 0);
+#pragma unroll
+                for (int bn = 0; bn < 4; ++bn)
+#pragma unroll
+                    for (int bm = 0; bm < 4; ++bm)
+                        acc[2][bn][bm] = __builtin_amdgcn_mfma_f32_16x16x4f32(sr[bn], si[bm], acc[2][bn][bm], 0, 0, 0);
+            } else {
+#pragma unroll
+                for (int bn = 0; bn < 4; ++bn)
+#pragma unroll
+                    for (int bm = 0; bm < 4; ++bm)
+                        acc[0][bn][bm] = __builtin_amdgcn_mfma_f32_16x16x4f32(br[bn], ar[bm], acc[0][bn][bm], 0, 0, 0);
+#pragma unroll
+                for (int bn = 0; bn < 4; ++bn)
+#pragma unroll
+                    for (int bm = 0; bm < 4; ++bm)
+                        acc[1][bn][bm] = __builtin_amdgcn_mfma_f32_16x16x4f32(bi[bn], ar[bm], acc[1][bn][bm], 0, 0, 0);
+#pragma unroll
+                for (int bn = 0; bn < 4; ++bn)
+#pragma unroll
+                    for (int bm = 0; bm < 4; ++bm)
+                        acc[0][bn][bm] = __builtin_amdgcn_mfma_f32_16x16x4f32(-bi[bn], ai[bm], acc[0][bn][bm], 0, 0, 0);
+#pragma unroll
+                for (int bn = 0; bn < 4; ++bn)
+#pragma unroll
+                    for (int bm = 0; bm < 4; ++bm)
+                        acc[1][bn][bm] = __builtin_amdgcn_mfma_f32_16x16x4f32(br[bn], ai[bm], acc[1][bn][bm], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int bn = 0; bn < 4; ++bn)
+#pragma unroll
+        for (int bm = 0; bm < 4; ++bm)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t n = n0 + wn * 64 + bn * 16 + 4 * lk + r;
+                const int64_t mm = m0 + wm * 64 + bm * 16 + li;
+                float2 v;
+                if constexpr (MODE == MODE_3M) {
+                    v.x = acc[0][bn][bm][r] - acc[1][bn][bm][r];
+                    v.y = acc[2][bn][bm][r] - acc[0][bn][bm][r] - acc[1][bn][bm][r];
+                } else {
+                    v.x = acc[0][bn][bm][r];
+                    v.y = acc[1][bn][bm][r];
+                }
+                C[mm + M * n] = v;
+            }
+}
 
 __global__ void k_absmax_c64(const float2* __restrict__ x, int64_t n, uint32_t* __restrict__ out) {
     float m = 0.0f;
@@ -824,9 +983,23 @@ qt_status run_gemm(Ctx* c, const Tensor* A, const Tensor* B, Tensor* C, const Bi
                                static_cast<double2*>(C->data), M, N, K);
             s = check_hip(c, hipGetLastError(), "k_gemm_c128");
         } else {
-            hipLaunchKernelGGL((k_gemm_c64<MODE>), dim3(gx, gy), dim3(kThreads), 0, c->stream, ap, bp,
-                               static_cast<float2*>(C->data), M, N, K);
-            s = check_hip(c, hipGetLastError(), "k_gemm_c64");
+            bool done = false;
+            if constexpr (MODE == MODE_4M || MODE == MODE_3M) {
+                if (M >= BT && N >= BT && K % BK == 0) {
+                    const int64_t big = (M / BT) * (N / BT);
+                    const unsigned bx = unsigned(big < 65536 ? big : 65536);
+                    const unsigned by = unsigned((big + bx - 1) / bx);
+                    hipLaunchKernelGGL((k_gemm_c64_big<MODE>), dim3(bx, by), dim3(kThreads), 0, c->stream, ap, bp,
+                                       static_cast<float2*>(C->data), M, N, K);
+                    s = check_hip(c, hipGetLastError(), "k_gemm_c64_big");
+                    done = true;
+                }
+            }
+            if (!done) {
+                hipLaunchKernelGGL((k_gemm_c64<MODE>), dim3(gx, gy), dim3(kThreads), 0, c->stream, ap, bp,
+                                   static_cast<float2*>(C->data), M, N, K);
+                s = check_hip(c, hipGetLastError(), "k_gemm_c64");
+            }
         }
     }
     // stream-ordered reuse: later work on this stream may take these blocks

@@ -6,8 +6,8 @@ oracle/build_ref.sh) in this container.
 Every case records the input files (QASM + measurement text; the reference's own
 sample files are copied as data), the contraction sequence the reference chose
 (its `mCreatedFrom` record, src/preprocess.h:41-46) or the line-graph order it was
-given, and the value it printed at %.17g.  The GPU box never runs this script and
-never reads /root/reference.
+given, and the value it printed at %.17g.  Only this script reads the reference
+(where its sources are); the tests read the committed fixtures.
 
 Usage:  python tests/golden/make_golden.py   (needs oracle/_ref built)
 """
