@@ -129,7 +129,8 @@ qt_status qt_tensor_exponent(qt_ctx* ctx, const qt_tensor* t, int32_t* exponent)
  * src/ContractionTools.h:219-388, ContractGivenSequence :392-411) compiled for the
  * device.  Operand ids: leaves 0..n_leaves-1, then the result of step i is id
  * n_leaves + i; every intermediate is consumed exactly once and the last step
- * yields the network's scalar.
+ * yields the network's scalar (or, for a subtree of a larger network, a tensor
+ * that a batch writes to per-instance output tensors, qt_batch_set_outputs).
  *   step_desc : 6 ints per step: a, b, rank_a, rank_b, k, rank_c
  *   step_axes : per step pair_a[k], pair_b[k], free_src[rank_c] (the same conventions
  *               as qt_contract_pair), concatenated
@@ -167,6 +168,15 @@ qt_status qt_batch_write_leaves(qt_batch* b, int64_t offset, const double* value
  * Waits for the batch's work in flight. */
 qt_status qt_batch_set_instances(qt_batch* b, int n, const int32_t* prog, const int64_t* param_base,
                                  int n_groups, const int32_t* group, double c0, double c1);
+/* Output tensors of the current instances (one per instance, in instance order, set
+ * again after every qt_batch_set_instances): each instance's last step writes its
+ * whole result (tensor or scalar) there, in the batch's dtype, in the step's axis
+ * order (its free_src), instead of the FP64 value array of qt_batch_results.  Needed
+ * when a program ends in a tensor; such batches run with 64 or 256 lanes per
+ * network.  The tensors are written on the context's stream by qt_batch_launch and
+ * can feed qt_contract_pair calls queued after it.  No counterpart in the
+ * reference: it lets the small subtrees of a large network run as one launch. */
+qt_status qt_batch_set_outputs(qt_batch* b, int n, qt_tensor* const* outs);
 /* Executor shape: lanes per network (4, 8, 16 or 32 = that many lanes of a
  * wavefront per network, several networks of one program side by side; 64 = one
  * wavefront; 256 = one workgroup) and workgroups per compute unit (workgroups are

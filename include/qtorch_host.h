@@ -177,8 +177,14 @@ typedef struct qth_sliced qth_sliced;
 int qth_sliced_create(const char* qasm, const char* measure, int dim_log2, int order, uint64_t seed, int max_bits,
                       qth_sliced** out);
 /* plan: sliced wires, peak bits, slices, complex MACs per slice, of those in
- * GEMM-shaped steps, peak bytes of a slice (complex128) */
-int qth_sliced_plan(qth_sliced* s, int64_t* values6);
+ * GEMM-shaped steps, peak bytes of a slice (complex128), fused subtrees, steps in
+ * them (small steps whose subtrees run as device programs, one batch launch per
+ * chunk of slices) */
+int qth_sliced_plan(qth_sliced* s, int64_t* values8);
+/* slices_per_launch = 0: every step of a slice runs through qt_contract_pair (no
+ * fused subtrees); < 0 (default): fused, slices per launch bounded by memory;
+ * > 0: fused, at most that many slices per launch. */
+int qth_sliced_set_fusion(qth_sliced* s, int64_t slices_per_launch);
 int qth_sliced_run(qth_sliced* s, qt_ctx* ctx, int precision, double* re, double* im, double* ms);
 int qth_sliced_destroy(qth_sliced* s);
 

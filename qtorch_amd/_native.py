@@ -72,6 +72,7 @@ _HIP_SIGS = {
     "qt_batch_set_leaves": (_c.c_int, [_vp, _dp, _c.c_int64]),
     "qt_batch_set_instances": (_c.c_int, [_vp, _c.c_int, _i32p, _i64p, _c.c_int, _i32p, _c.c_double,
                                           _c.c_double]),
+    "qt_batch_set_outputs": (_c.c_int, [_vp, _c.c_int, _c.POINTER(_vp)]),
     "qt_batch_config": (_c.c_int, [_vp, _c.c_int, _c.c_int]),
     "qt_batch_config_ex": (_c.c_int, [_vp, _c.c_int, _c.c_int, _c.c_int]),
     "qt_batch_autotune": (_c.c_int, [_vp, _c.POINTER(_c.c_int), _c.POINTER(_c.c_int), _c.POINTER(_c.c_float)]),
@@ -143,6 +144,7 @@ _HOST_SIGS = {
     "qth_sliced_run": (_c.c_int, [_vp, _vp, _c.c_int, _c.POINTER(_c.c_double), _c.POINTER(_c.c_double),
                                   _c.POINTER(_c.c_double)]),
     "qth_sliced_destroy": (_c.c_int, [_vp]),
+    "qth_sliced_set_fusion": (_c.c_int, [_vp, _c.c_int64]),
     "qth_serve_plan": (_c.c_int, [_c.c_int, _c.c_char_p, _c.c_int, _c.c_int, _c.c_char_p, _c.c_int,
                                   _c.POINTER(_c.c_int), _c.POINTER(_c.c_int64), _c.POINTER(_c.c_int64)]),
 }
@@ -714,10 +716,16 @@ class SlicedNetwork:
         self.handle = h
 
     def plan(self):
-        v = (ctypes.c_int64 * 6)()
+        v = (ctypes.c_int64 * 8)()
         _host_check(host.qth_sliced_plan(self.handle, v))
         return {"sliced_wires": v[0], "peak_bits": v[1], "slices": v[2], "cmacs_per_slice": v[3],
-                "gemm_cmacs_per_slice": v[4], "peak_bytes": v[5]}
+                "gemm_cmacs_per_slice": v[4], "peak_bytes": v[5], "fused_subtrees": v[6], "fused_steps": v[7]}
+
+    def set_fusion(self, slices_per_launch: int = -1) -> None:
+        """-1 (default): small subtrees run as device programs, one batch launch per
+        chunk of slices sized by memory; n > 0: at most n slices per launch; 0: every
+        step through qt_contract_pair."""
+        _host_check(host.qth_sliced_set_fusion(self.handle, int(slices_per_launch)))
 
     def run(self, ctx: Context, precision: int = QT_PLAIN_C128):
         re, im, ms = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()

@@ -70,6 +70,7 @@ struct qt_program {
     int64_t cmacs = 0;
     int32_t scratch_elems = 0;
     int32_t max_tensor_elems = 0;
+    int32_t out_elems = 1;  // elements of the last step's result
 };
 
 struct qt_batch {
@@ -110,6 +111,11 @@ struct qt_batch {
     int32_t slot_stride = 0;      // elements between the scratch areas of two executors
     int32_t pstage = 0, cstage = 0;  // LDS leaf areas: parameter block per slot, constant leaves per wave
     std::vector<int32_t> h_prog;  // program of each instance (host copy, for chunking)
+    std::vector<int32_t> prog_out_elems;  // elements of each program's result
+    void** d_tout = nullptr;      // per instance: output tensor of the last step (qt_batch_set_outputs)
+    size_t cap_tout = 0;
+    bool needs_tout = false;      // an instance's program ends in a tensor
+    bool tensor_out = false;      // output tensors are set for the current instances
     int2* d_chunks = nullptr;
     int n_chunks = 0;
     int n_cus = 0;
@@ -298,13 +304,16 @@ __device__ __forceinline__ void run_step(PA A, PB B, V* C, const uint32_t* __res
     }
 }
 
-template <class V, int MODE, int LANES, bool GLOBAL_SCRATCH>
+// With TOUT, every instance's last step writes its whole result (a tensor, or a
+// scalar) to the instance's output tensor tout[i] instead of the FP64 value array.
+template <class V, int MODE, int LANES, bool GLOBAL_SCRATCH, bool TOUT>
 __global__ __launch_bounds__(kWg) void k_batch(const ProgHdr* __restrict__ progs, const StepDesc* __restrict__ steps,
                                                const uint32_t* __restrict__ tabs,
                                                const int64_t* __restrict__ leaf_refs, const V* __restrict__ pool,
                                                const Instance* __restrict__ inst, int n_inst,
                                                int32_t* __restrict__ counter, double2* __restrict__ out,
-                                               V* __restrict__ gscratch, int scratch_elems) {
+                                               V* __restrict__ gscratch, int scratch_elems,
+                                               V* const* __restrict__ tout) {
     extern __shared__ unsigned char lds_raw[];
     __shared__ int wg_next;
     constexpr int kExec = kWg / LANES;
@@ -339,6 +348,12 @@ __global__ __launch_bounds__(kWg) void k_batch(const ProgHdr* __restrict__ progs
             const uint32_t* gs = fc + nC;
             double2* fin = (st.flags & 1) ? out + i : nullptr;
             V* C = mem + st.c;
+            if constexpr (TOUT) {
+                if (st.flags & 1) {
+                    fin = nullptr;
+                    C = tout[i];
+                }
+            }
             auto leaf = [&](int code) -> const V* {
                 const int64_t r = refs[-1 - code];
                 const int64_t off =
@@ -540,7 +555,11 @@ qt_status launch_grouped_t(qt_batch* b) {
 
 template <class V, int MODE>
 qt_status launch_batch_t(qt_batch* b) {
-    if (b->lanes < 64) return launch_grouped_t<V, MODE>(b);
+    if (b->lanes < 64) {
+        if (b->tensor_out)
+            return set_error(b->ctx, QT_INVALID_INPUT, "batch: tensor outputs need 64 or 256 lanes per network");
+        return launch_grouped_t<V, MODE>(b);
+    }
     Ctx* c = b->ctx;
     const int kExec = kWg / b->lanes;
     const unsigned grid = unsigned(b->n_cus * b->wgs_per_cu);
@@ -548,18 +567,32 @@ qt_status launch_batch_t(qt_batch* b) {
     V* gs = static_cast<V*>(b->d_global_scratch);
     const V* pool = static_cast<const V*>(b->d_pool);
     auto d_out = reinterpret_cast<double2*>(b->d_out);
-#define QT_LAUNCH(L, G)                                                                                         \
-    hipLaunchKernelGGL((k_batch<V, MODE, L, G>), dim3(grid), dim3(kWg), lds, c->stream, b->d_progs, b->d_steps, \
-                       b->d_tabs, b->d_leaf_refs, pool, b->d_inst, b->n_inst, b->d_counter, d_out, gs,         \
-                       b->scratch_elems)
+    V* const* tout = reinterpret_cast<V* const*>(b->d_tout);
+#define QT_LAUNCH_T(L, G, T)                                                                                       \
+    hipLaunchKernelGGL((k_batch<V, MODE, L, G, T>), dim3(grid), dim3(kWg), lds, c->stream, b->d_progs, b->d_steps, \
+                       b->d_tabs, b->d_leaf_refs, pool, b->d_inst, b->n_inst, b->d_counter, d_out, gs,            \
+                       b->scratch_elems, tout)
+#define QT_LAUNCH(L, G)              \
+    if (b->tensor_out) {             \
+        QT_LAUNCH_T(L, G, true);     \
+    } else {                         \
+        QT_LAUNCH_T(L, G, false);    \
+    }
     if (b->use_global_scratch) {
-        if (b->lanes == 64) QT_LAUNCH(64, true);
-        else QT_LAUNCH(256, true);
+        if (b->lanes == 64) {
+            QT_LAUNCH(64, true);
+        } else {
+            QT_LAUNCH(256, true);
+        }
     } else {
-        if (b->lanes == 64) QT_LAUNCH(64, false);
-        else QT_LAUNCH(256, false);
+        if (b->lanes == 64) {
+            QT_LAUNCH(64, false);
+        } else {
+            QT_LAUNCH(256, false);
+        }
     }
 #undef QT_LAUNCH
+#undef QT_LAUNCH_T
     return check_hip(c, hipGetLastError(), "k_batch");
 }
 
@@ -567,6 +600,8 @@ qt_status launch_batch(qt_batch* b) {
     Ctx* c = b->ctx;
     // a failed reconfiguration leaves scratch or chunk arrays that do not fit the shape
     if (!b->shape_ok) return set_error(c, QT_INVALID_INPUT, "batch: executor shape not configured (last change failed)");
+    if (b->needs_tout && !b->tensor_out)
+        return set_error(c, QT_INVALID_INPUT, "batch: programs with tensor results need qt_batch_set_outputs");
     hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, c->stream, b->d_counter);
     qt_status s = check_hip(c, hipGetLastError(), "k_reset_counter");
     if (s != QT_OK) return s;
@@ -708,6 +743,7 @@ qt_status append_programs(qt_batch* b, int n, qt_program* const* progs, const in
         b->max_param_ref_end = std::max(b->max_param_ref_end, pend);
         b->scratch_elems = std::max(b->scratch_elems, (p->scratch_elems + 15) / 16 * 16);
         b->prog_cmacs.push_back(p->cmacs);
+        b->prog_out_elems.push_back(p->out_elems);
         b->h_hdr.push_back(h);
     }
     b->n_programs += n;
@@ -900,8 +936,6 @@ qt_status qt_program_create(qt_ctx* cx, int dim_log2, int n_leaves, const int32_
     for (int s = 0; s + 1 < n_steps; ++s)
         if (last_use[size_t(s)] < 0)
             return set_error(ctx, QT_INVALID_INPUT, "qt_program_create: an intermediate is never consumed");
-    if (rank_of[size_t(n_leaves + n_steps - 1)] != 0)
-        return set_error(ctx, QT_INVALID_INPUT, "qt_program_create: the last step must produce a scalar");
 
     // scratch slots: first fit over live intervals (an intermediate lives from its
     // creating step until the step that consumes it)
@@ -988,6 +1022,7 @@ qt_status qt_program_create(qt_ctx* cx, int dim_log2, int n_leaves, const int32_
         p->steps.push_back(st);
         p->cmacs += int64_t(1) << ((rc + k) * dl);
         p->max_tensor_elems = std::max(p->max_tensor_elems, int32_t(1) << (rc * dl));
+        if (s + 1 == n_steps) p->out_elems = int32_t(1) << (rc * dl);
     }
     *out = p.release();
     return QT_OK;
@@ -1088,6 +1123,7 @@ qt_status qt_batch_set_instances(qt_batch* b, int n, const int32_t* prog, const 
     // validate everything before any state changes
     std::vector<qt::Instance> inst(static_cast<size_t>(n));
     int64_t cmacs = 0;
+    bool needs_tout = false;
     for (int i = 0; i < n; ++i) {
         if (prog[i] < 0 || prog[i] >= b->n_programs || param_base[i] < 0 ||
             param_base[i] + b->prog_param_end[size_t(prog[i])] > b->pool_elems ||
@@ -1097,6 +1133,7 @@ qt_status qt_batch_set_instances(qt_batch* b, int n, const int32_t* prog, const 
             return set_error(ctx, QT_INVALID_INPUT, "qt_batch_set_instances: bad group");
         inst[size_t(i)] = qt::Instance{prog[i], 0, param_base[i]};
         cmacs += b->prog_cmacs[size_t(prog[i])];
+        needs_tout = needs_tout || b->prog_out_elems[size_t(prog[i])] > 1;
     }
     // group g's members in instance order (group -1: no group)
     std::vector<int32_t> seg(size_t(n_groups) + 1, 0), members;
@@ -1122,10 +1159,51 @@ qt_status qt_batch_set_instances(qt_batch* b, int n, const int32_t* prog, const 
     if (s != QT_OK) return s;
     b->n_inst = n;
     b->n_groups = n_groups;
+    b->needs_tout = needs_tout;
+    b->tensor_out = false;  // output tensors belong to the instances they were set for
     b->cmacs_total = cmacs;
     b->c0 = c0;
     b->c1 = c1;
     return qt::configure_scratch(b);  // rebuilds the chunk list, drops a captured graph
+}
+
+qt_status qt_batch_set_outputs(qt_batch* b, int n, qt_tensor* const* outs) {
+    if (!b || !outs) return QT_INVALID_INPUT;
+    auto* ctx = b->ctx;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    if (n != b->n_inst || n == 0)
+        return set_error(ctx, QT_INVALID_INPUT, "qt_batch_set_outputs: one output tensor per instance");
+    const int dtype = b->precision == QT_PLAIN_C128 ? QT_DTYPE_C128 : QT_DTYPE_C64;
+    std::vector<void*> ptrs(static_cast<size_t>(n));
+    for (int i = 0; i < n; ++i) {
+        if (!outs[i]) return set_error(ctx, QT_INVALID_INPUT, "qt_batch_set_outputs: null tensor");
+        const qt::Tensor& t = outs[i]->impl;
+        if (t.dtype != dtype || t.numel != int64_t(b->prog_out_elems[size_t(b->h_prog[size_t(i)])]))
+            return set_error(ctx, QT_INVALID_INPUT, "qt_batch_set_outputs: tensor " + std::to_string(i) +
+                                                        " does not match its program's result");
+        for (int j = 0; j < i; ++j)
+            if (ptrs[size_t(j)] == t.data)
+                return set_error(ctx, QT_INVALID_INPUT, "qt_batch_set_outputs: two instances share a tensor");
+        ptrs[size_t(i)] = t.data;
+    }
+    (void)hipStreamSynchronize(ctx->stream);  // a launch in flight reads the pointer array
+    const size_t cap0 = b->cap_tout;
+    qt_status s = qt::dev_put(ctx, &b->d_tout, &b->cap_tout, ptrs.data(), ptrs.size());
+    if (s != QT_OK) {
+        b->tensor_out = false;
+        return s;
+    }
+    for (int i = 0; i < n; ++i) qt::release_exponent(ctx, &outs[i]->impl);  // results are plain values
+    // the kernel variant changes with the first outputs; a grown array moves
+    if (!b->tensor_out || b->cap_tout != cap0) {
+        b->tensor_out = true;
+        if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+        if (b->graph) (void)hipGraphDestroy(b->graph);
+        b->graph_exec = nullptr;
+        b->graph = nullptr;
+    }
+    return QT_OK;
 }
 
 qt_status qt_batch_config(qt_batch* b, int lanes, int wgs_per_cu) {
@@ -1279,7 +1357,7 @@ qt_status qt_batch_destroy(qt_batch* b) {
         for (void* p : {(void*)b->d_progs, (void*)b->d_steps, (void*)b->d_tabs, (void*)b->d_leaf_refs, b->d_pool,
                         (void*)b->d_inst, (void*)b->d_out, (void*)b->d_counter, (void*)b->d_seg, (void*)b->d_members,
                         (void*)b->d_group, b->d_global_scratch, (void*)b->d_chunks, b->d_pool_stage,
-                        (void*)b->d_leaf_lds, (void*)b->d_csts})
+                        (void*)b->d_leaf_lds, (void*)b->d_csts, (void*)b->d_tout})
             if (p) (void)hipFree(p);
         --b->ctx->live;
     }

@@ -555,6 +555,14 @@ int qth_sliced_plan(qth_sliced* s, int64_t* v) {
     v[3] = p.cmacs_per_slice;
     v[4] = p.gemm_cmacs_per_slice;
     v[5] = p.peak_bytes;
+    v[6] = p.fused_subtrees;
+    v[7] = p.fused_steps;
+    return QTH_OK;
+}
+
+int qth_sliced_set_fusion(qth_sliced* s, int64_t slices_per_launch) {
+    if (!s) return QTH_INVALID_FUNCTION_INPUT;
+    s->n->SetFusion(slices_per_launch);
     return QTH_OK;
 }
 

@@ -7,9 +7,10 @@
 // removes that axis from every tensor that carries it (leaves are cut to the
 // sub-tensor, steps lose the pair or the free axis), so all D^s slices are the same
 // smaller network with different leaf values; sum_v over a wire's values is the
-// contraction the wire stood for.  Each slice runs step by step through
-// qt_contract_pair (MFMA GEMMs for large steps) and its scalar is added in FP64 on
-// the device.
+// contraction the wire stood for.  The small subtrees of every slice run as device
+// programs in one batch launch per chunk of slices; the remaining steps of each
+// slice run through qt_contract_pair (MFMA GEMMs for large steps) and its scalar is
+// added in FP64 on the device.
 #include "sliced.hpp"
 
 #include <algorithm>
@@ -148,6 +149,137 @@ SlicedNetwork::SlicedNetwork(const std::string& qasm, const std::string& measure
             if (op >= n_leaves_) live -= bytes[size_t(op)];
     }
     plan_.peak_bytes = leaves + peak;
+    // sliced axes of every leaf
+    cut_.resize(size_t(n_leaves_));
+    leaf_rank_.resize(size_t(n_leaves_));
+    for (int i = 0; i < n_leaves_; ++i) {
+        const auto& w = leaf_wires_[size_t(i)];
+        int r = 0;
+        for (size_t j = 0; j < w.size(); ++j) {
+            const auto it = std::find(sliced_.begin(), sliced_.end(), w[j]);
+            if (it != sliced_.end()) cut_[size_t(i)].push_back({int(j), int(it - sliced_.begin())});
+            else ++r;
+        }
+        leaf_rank_[size_t(i)] = r;
+    }
+    PlanFusion();
+}
+
+// Small steps (result <= 2^12 elements, operands <= 2^14, <= 2^16 complex MACs) whose
+// operands are leaves or small steps form subtrees; each maximal one becomes a device
+// program whose last step writes its result tensor.  One batch launch runs every
+// subtree of a chunk of slices (instances = subtree x slice; a slice's cut leaves
+// are its instance's parameter block), replacing one launch per small step.
+void SlicedNetwork::PlanFusion() {
+    constexpr int kOutBits = 12, kOperandBits = 14, kMacBits = 16;
+    const int n = int(steps_.size());
+    step_fused_.assign(size_t(n), 0);
+    std::vector<int> consumer(size_t(n), -1);
+    int fused = 0;
+    for (int s = 0; s < n; ++s) {
+        const Step& st = steps_[size_t(s)];
+        bool ok = st.rank_c * dl_ <= kOutBits && st.rank_a * dl_ <= kOperandBits && st.rank_b * dl_ <= kOperandBits &&
+                  (st.rank_c + st.k) * dl_ <= kMacBits;
+        for (int op : {st.a, st.b})
+            if (op >= n_leaves_) {
+                consumer[size_t(op - n_leaves_)] = s;
+                ok = ok && step_fused_[size_t(op - n_leaves_)];
+            }
+        step_fused_[size_t(s)] = ok ? 1 : 0;
+        if (ok) ++fused;
+    }
+    leaf_direct_.assign(size_t(n_leaves_), 0);
+    for (int s = 0; s < n; ++s)
+        if (!step_fused_[size_t(s)])
+            for (int op : {steps_[size_t(s)].a, steps_[size_t(s)].b})
+                if (op < n_leaves_) leaf_direct_[size_t(op)] = 1;
+    for (int s = 0; s < n; ++s)
+        if (step_fused_[size_t(s)] && (consumer[size_t(s)] < 0 || !step_fused_[size_t(consumer[size_t(s)])]))
+            roots_.push_back(s);
+    for (int root : roots_) {
+        // the subtree's steps in plan order (a topological order) and its leaves
+        std::vector<int> sub;
+        std::vector<int> stack{root};
+        while (!stack.empty()) {
+            const int s = stack.back();
+            stack.pop_back();
+            sub.push_back(s);
+            for (int op : {steps_[size_t(s)].a, steps_[size_t(s)].b})
+                if (op >= n_leaves_) stack.push_back(op - n_leaves_);
+        }
+        std::sort(sub.begin(), sub.end());
+        std::unordered_map<int, int> local;  // operand id -> program operand id
+        std::vector<int32_t> leaf_ranks, desc, axes;
+        std::vector<int64_t> leaf_refs;
+        auto operand = [&](int op) {
+            if (op >= n_leaves_) return local.at(op);
+            auto it = local.find(op);
+            if (it != local.end()) return it->second;
+            const int64_t elems = int64_t(1) << (leaf_rank_[size_t(op)] * dl_);
+            if (cut_[size_t(op)].empty()) {
+                leaf_refs.push_back(int64_t(const_pool_.size() / 2));
+                for (const auto& v : leaf_vals_[size_t(op)]) {
+                    const_pool_.push_back(v.real());
+                    const_pool_.push_back(v.imag());
+                }
+            } else {
+                leaf_refs.push_back(QT_LEAF_PARAM | block_elems_);
+                param_leaves_.push_back({op, block_elems_});
+                block_elems_ += elems;
+            }
+            leaf_ranks.push_back(leaf_rank_[size_t(op)]);
+            const int id = int(leaf_ranks.size()) - 1;
+            local[op] = id;
+            return id;
+        };
+        // leaves first (program ids: leaves, then steps)
+        for (int s : sub) {
+            if (steps_[size_t(s)].a < n_leaves_) operand(steps_[size_t(s)].a);
+            if (steps_[size_t(s)].b < n_leaves_) operand(steps_[size_t(s)].b);
+        }
+        const int nl = int(leaf_ranks.size());
+        for (size_t j = 0; j < sub.size(); ++j) {
+            const Step& st = steps_[size_t(sub[j])];
+            desc.insert(desc.end(), {operand(st.a), operand(st.b), st.rank_a, st.rank_b, st.k, st.rank_c});
+            axes.insert(axes.end(), st.pair_a.begin(), st.pair_a.end());
+            axes.insert(axes.end(), st.pair_b.begin(), st.pair_b.end());
+            axes.insert(axes.end(), st.free_src.begin(), st.free_src.end());
+            local[n_leaves_ + sub[j]] = nl + int(j);
+        }
+        qt_program* p = nullptr;
+        if (qt_program_create(nullptr, dl_, nl, leaf_ranks.data(), leaf_refs.data(), int(sub.size()), desc.data(),
+                              axes.data(), &p) != QT_OK)
+            throw ContractionFailure();
+        progs_.emplace_back(p, [](qt_program* q) { qt_program_destroy(q); });
+        root_bytes_c128_ += int64_t(16) << (steps_[size_t(root)].rank_c * dl_);
+    }
+    plan_.fused_subtrees = int(roots_.size());
+    plan_.fused_steps = fused;
+}
+
+void SlicedNetwork::Extract(int leaf, int64_t slice, double* dst) const {
+    const auto& src = leaf_vals_[size_t(leaf)];
+    const int rank = int(leaf_wires_[size_t(leaf)].size());
+    const int64_t D = int64_t(1) << dl_;
+    const int64_t n = int64_t(1) << (leaf_rank_[size_t(leaf)] * dl_);
+    // fixed part of the source index, then the kept axes' digits
+    int64_t base = 0;
+    std::vector<int> kept;
+    for (int ax = 0; ax < rank; ++ax) {
+        bool is_cut = false;
+        for (const auto& [axis, digit] : cut_[size_t(leaf)])
+            if (axis == ax) {
+                is_cut = true;
+                base += ((slice >> (digit * dl_)) & (D - 1)) << (ax * dl_);
+            }
+        if (!is_cut) kept.push_back(ax);
+    }
+    for (int64_t e = 0; e < n; ++e) {
+        int64_t idx = base;
+        for (size_t k = 0; k < kept.size(); ++k) idx += ((e >> (int(k) * dl_)) & (D - 1)) << (kept[k] * dl_);
+        dst[2 * e] = src[size_t(idx)].real();
+        dst[2 * e + 1] = src[size_t(idx)].imag();
+    }
 }
 
 std::complex<double> SlicedNetwork::Run(qt_ctx* ctx, int precision, SliceStats* stats) const {
@@ -158,12 +290,12 @@ std::complex<double> SlicedNetwork::Run(qt_ctx* ctx, int precision, SliceStats* 
         (!c128 && base != QT_C64_4M && base != QT_C64_3M && base != QT_C64_BF16X3))
         throw InvalidFunctionInput();
     const int dtype = c128 ? QT_DTYPE_C128 : QT_DTYPE_C64;
-    const int form = precision;
-    const int D = 1 << dl_;
     const auto t0 = std::chrono::steady_clock::now();
+    const bool fuse = fuse_slices_ != 0 && !roots_.empty();
+    const int n_roots = int(roots_.size());
     qt_tensor* acc = nullptr;
-    CheckQt(ctx, qt_tensor_alloc(ctx, 0, dl_, QT_DTYPE_C128, &acc), "qt_tensor_alloc");
-    std::vector<qt_tensor*> ops(size_t(n_leaves_) + steps_.size(), nullptr);
+    qt_batch* batch = nullptr;
+    std::vector<qt_tensor*> ops(size_t(n_leaves_) + steps_.size(), nullptr), outs;
     auto release = [&] {
         for (auto*& t : ops)
             if (t) {
@@ -171,79 +303,122 @@ std::complex<double> SlicedNetwork::Run(qt_ctx* ctx, int precision, SliceStats* 
                 t = nullptr;
             }
     };
+    auto release_outs = [&] {
+        for (auto*& t : outs)
+            if (t) {
+                qt_tensor_free(ctx, t);
+                t = nullptr;
+            }
+    };
     try {
-        // sliced axes of every leaf
-        std::vector<std::vector<std::pair<int, int>>> cut(static_cast<size_t>(n_leaves_));  // (axis, slice digit)
-        std::vector<int> ranks(static_cast<size_t>(n_leaves_));
-        for (int i = 0; i < n_leaves_; ++i) {
-            const auto& w = leaf_wires_[size_t(i)];
-            int r = 0;
-            for (size_t j = 0; j < w.size(); ++j) {
-                const auto it = std::find(sliced_.begin(), sliced_.end(), w[j]);
-                if (it != sliced_.end()) cut[size_t(i)].push_back({int(j), int(it - sliced_.begin())});
-                else ++r;
-            }
-            ranks[size_t(i)] = r;
+        CheckQt(ctx, qt_tensor_alloc(ctx, 0, dl_, QT_DTYPE_C128, &acc), "qt_tensor_alloc");
+        int64_t const_elems = 0, chunk = 1;
+        if (fuse) {
+            std::vector<qt_program*> pp;
+            for (const auto& p : progs_) pp.push_back(p.get());
+            CheckQt(ctx, qt_batch_create(ctx, n_roots, pp.data(), base, &batch), "qt_batch_create");
+            const_elems = std::max<int64_t>(1, int64_t(const_pool_.size() / 2));
+            std::vector<double> cp(const_pool_);
+            cp.resize(size_t(2 * const_elems), 0.0);
+            CheckQt(ctx, qt_batch_write_leaves(batch, 0, cp.data(), const_elems), "qt_batch_write_leaves");
+            // slices per launch: their subtree results stay within 256 MiB
+            const int64_t per_slice = std::max<int64_t>(1, root_bytes_c128_ / (c128 ? 1 : 2));
+            chunk = std::max<int64_t>(1, std::min<int64_t>({plan_.slices, 4096, (int64_t(256) << 20) / per_slice}));
+            if (fuse_slices_ > 0) chunk = std::min(chunk, fuse_slices_);
         }
-        std::vector<std::vector<double>> host(static_cast<size_t>(n_leaves_));
-        std::vector<std::vector<float>> host32(static_cast<size_t>(n_leaves_));
-        std::vector<const void*> ptrs(static_cast<size_t>(n_leaves_));
-        for (int64_t slice = 0; slice < plan_.slices; ++slice) {
-            for (int i = 0; i < n_leaves_; ++i) {
-                const auto& src = leaf_vals_[size_t(i)];
-                const int rank = int(leaf_wires_[size_t(i)].size());
-                const int64_t n = int64_t(1) << (ranks[size_t(i)] * dl_);
-                auto& h = host[size_t(i)];
-                h.assign(size_t(2 * n), 0.0);
-                // fixed part of the source index, then the kept axes' digits
-                int64_t base = 0;
-                for (const auto& [axis, digit] : cut[size_t(i)]) {
-                    const int64_t v = (slice >> (digit * dl_)) & (D - 1);
-                    base += v << (axis * dl_);
+        // leaves that unfused steps read directly, uploaded per slice
+        std::vector<int> direct;
+        for (int i = 0; i < n_leaves_; ++i)
+            if (!fuse || leaf_direct_[size_t(i)]) direct.push_back(i);
+        std::vector<int> ranks;
+        for (int i : direct) ranks.push_back(leaf_rank_[size_t(i)]);
+        std::vector<std::vector<double>> host(direct.size());
+        std::vector<std::vector<float>> host32(direct.size());
+        std::vector<const void*> ptrs(direct.size());
+        std::vector<qt_tensor*> made(direct.size());
+        std::vector<double> block;
+        for (int64_t s0 = 0; s0 < plan_.slices; s0 += chunk) {
+            const int64_t n_sl = std::min(chunk, plan_.slices - s0);
+            if (fuse) {
+                // every subtree of every slice of the chunk in one launch
+                if (block_elems_ > 0) {
+                    block.assign(size_t(2 * n_sl * block_elems_), 0.0);
+                    for (int64_t j = 0; j < n_sl; ++j)
+                        for (const auto& [leaf, off] : param_leaves_)
+                            Extract(leaf, s0 + j, block.data() + 2 * (j * block_elems_ + off));
+                    CheckQt(ctx, qt_batch_write_leaves(batch, const_elems, block.data(), n_sl * block_elems_),
+                            "qt_batch_write_leaves");
                 }
-                for (int64_t e = 0; e < n; ++e) {
-                    int64_t idx = base;
-                    int kept = 0;
-                    for (int ax = 0; ax < rank; ++ax) {
-                        bool is_cut = false;
-                        for (const auto& c : cut[size_t(i)]) is_cut |= c.first == ax;
-                        if (is_cut) continue;
-                        idx += ((e >> (kept * dl_)) & (D - 1)) << (ax * dl_);
-                        ++kept;
+                const int n_inst = int(n_sl) * n_roots;
+                std::vector<int32_t> prog(static_cast<size_t>(n_inst));
+                std::vector<int64_t> pbase(static_cast<size_t>(n_inst));
+                outs.assign(size_t(n_inst), nullptr);
+                for (int64_t j = 0; j < n_sl; ++j)
+                    for (int r = 0; r < n_roots; ++r) {
+                        const size_t i = size_t(j * n_roots + r);
+                        prog[i] = r;
+                        pbase[i] = const_elems + j * block_elems_;
+                        CheckQt(ctx, qt_tensor_alloc_empty(ctx, steps_[size_t(roots_[size_t(r)])].rank_c, dl_, dtype,
+                                                           &outs[i]),
+                                "qt_tensor_alloc_empty");
                     }
-                    h[size_t(2 * e)] = src[size_t(idx)].real();
-                    h[size_t(2 * e + 1)] = src[size_t(idx)].imag();
-                }
-                if (c128) {
-                    ptrs[size_t(i)] = h.data();
-                } else {
-                    host32[size_t(i)].assign(h.begin(), h.end());
-                    ptrs[size_t(i)] = host32[size_t(i)].data();
-                }
+                CheckQt(ctx, qt_batch_set_instances(batch, n_inst, prog.data(), pbase.data(), 0, nullptr, 0.0, 1.0),
+                        "qt_batch_set_instances");
+                CheckQt(ctx, qt_batch_set_outputs(batch, n_inst, outs.data()), "qt_batch_set_outputs");
+                CheckQt(ctx, qt_batch_launch(batch), "qt_batch_launch");
             }
-            CheckQt(ctx, qt_tensor_create_many(ctx, n_leaves_, ranks.data(), dl_, dtype, ptrs.data(), ops.data()),
-                    "qt_tensor_create_many");
-            for (size_t s = 0; s < steps_.size(); ++s) {
-                const Step& st = steps_[s];
-                qt_tensor* c = nullptr;
-                CheckQt(ctx, qt_tensor_alloc_empty(ctx, st.rank_c, dl_, dtype, &c), "qt_tensor_alloc_empty");
-                ops[size_t(n_leaves_) + s] = c;
-                CheckQt(ctx,
-                        qt_contract_pair(ctx, ops[size_t(st.a)], st.rank_a, ops[size_t(st.b)], st.rank_b, st.k,
-                                         st.pair_a.data(), st.pair_b.data(), st.rank_c, st.free_src.data(), c, form,
-                                         nullptr),
-                        "qt_contract_pair");
-                for (int op : {st.a, st.b}) {
-                    qt_tensor_free(ctx, ops[size_t(op)]);
-                    ops[size_t(op)] = nullptr;
+            for (int64_t j = 0; j < n_sl; ++j) {
+                const int64_t slice = s0 + j;
+                for (size_t d = 0; d < direct.size(); ++d) {
+                    const int leaf = direct[d];
+                    const int64_t n = int64_t(1) << (leaf_rank_[size_t(leaf)] * dl_);
+                    host[d].assign(size_t(2 * n), 0.0);
+                    Extract(leaf, slice, host[d].data());
+                    if (c128) {
+                        ptrs[d] = host[d].data();
+                    } else {
+                        host32[d].assign(host[d].begin(), host[d].end());
+                        ptrs[d] = host32[d].data();
+                    }
                 }
+                if (!direct.empty()) {
+                    CheckQt(ctx, qt_tensor_create_many(ctx, int(direct.size()), ranks.data(), dl_, dtype, ptrs.data(),
+                                                       made.data()),
+                            "qt_tensor_create_many");
+                    for (size_t d = 0; d < direct.size(); ++d) ops[size_t(direct[d])] = made[d];
+                }
+                if (fuse)
+                    for (int r = 0; r < n_roots; ++r) {
+                        auto*& t = outs[size_t(j * n_roots + r)];
+                        ops[size_t(n_leaves_ + roots_[size_t(r)])] = t;
+                        t = nullptr;
+                    }
+                for (size_t s = 0; s < steps_.size(); ++s) {
+                    if (fuse && step_fused_[s]) continue;
+                    const Step& st = steps_[s];
+                    qt_tensor* c = nullptr;
+                    CheckQt(ctx, qt_tensor_alloc_empty(ctx, st.rank_c, dl_, dtype, &c), "qt_tensor_alloc_empty");
+                    ops[size_t(n_leaves_) + s] = c;
+                    CheckQt(ctx,
+                            qt_contract_pair(ctx, ops[size_t(st.a)], st.rank_a, ops[size_t(st.b)], st.rank_b, st.k,
+                                             st.pair_a.data(), st.pair_b.data(), st.rank_c, st.free_src.data(), c,
+                                             precision, nullptr),
+                            "qt_contract_pair");
+                    for (int op : {st.a, st.b}) {
+                        qt_tensor_free(ctx, ops[size_t(op)]);
+                        ops[size_t(op)] = nullptr;
+                    }
+                }
+                CheckQt(ctx, qt_tensor_accumulate(ctx, ops.back(), acc), "qt_tensor_accumulate");
+                release();
             }
-            CheckQt(ctx, qt_tensor_accumulate(ctx, ops.back(), acc), "qt_tensor_accumulate");
-            release();
         }
         double re = 0, im = 0;
         CheckQt(ctx, qt_tensor_read_scalar(ctx, acc, &re, &im), "qt_tensor_read_scalar");
         qt_tensor_free(ctx, acc);
+        acc = nullptr;
+        if (batch) qt_batch_destroy(batch);
+        batch = nullptr;
         if (stats) {
             *stats = plan_;
             stats->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -251,7 +426,9 @@ std::complex<double> SlicedNetwork::Run(qt_ctx* ctx, int precision, SliceStats* 
         return {re, im};
     } catch (...) {
         release();
-        qt_tensor_free(ctx, acc);
+        release_outs();
+        if (acc) qt_tensor_free(ctx, acc);
+        if (batch) qt_batch_destroy(batch);
         throw;
     }
 }

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <complex>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -16,6 +17,8 @@ struct SliceStats {
     int64_t cmacs_per_slice = 0;   // complex MACs of one slice (the reference's counter)
     int64_t gemm_cmacs_per_slice = 0;  // of those, in steps shaped for the MFMA GEMM
     int64_t peak_bytes = 0;        // device bytes of one slice's live tensors
+    int fused_subtrees = 0;        // subtrees of small steps run as device programs
+    int fused_steps = 0;           // steps inside them (of a slice)
     double ms = 0;                 // wall time of Run (device work included)
 };
 
@@ -27,11 +30,14 @@ class SlicedNetwork {
 public:
     SlicedNetwork(const std::string& qasm, const std::string& measure, int dim_log2, int order, uint64_t seed,
                   int max_bits);
-    // precision: QT_PLAIN_C128 (complex128), or a complex64 form, optionally or-ed
-    // with QT_RESCALE (every step rescaled by a power of two: no FP32 underflow for
-    // any depth, at two extra passes per step).  Contracts every slice on ctx's stream.
+    // precision: QT_PLAIN_C128 (complex128), or a complex64 form (optionally with
+    // QT_RESCALE: large steps rescaled by powers of two).  Contracts every slice on
+    // ctx's stream.
     std::complex<double> Run(qt_ctx* ctx, int precision, SliceStats* stats) const;
     const SliceStats& Plan() const { return plan_; }
+    // slices_per_launch = 0: every step through qt_contract_pair; < 0 (default): fused
+    // subtrees, slices per launch chosen by memory; > 0: at most that many per launch.
+    void SetFusion(int64_t slices_per_launch) { fuse_slices_ = slices_per_launch; }
 
 private:
     struct Step {
@@ -39,13 +45,29 @@ private:
         int rank_a, rank_b, k, rank_c;
         std::vector<int32_t> pair_a, pair_b, free_src;
     };
+    void PlanFusion();
+    // leaf i's sub-tensor for `slice` as interleaved FP64 (re, im) at dst
+    void Extract(int leaf, int64_t slice, double* dst) const;
+
     int dl_ = 2;
     int n_leaves_ = 0;
     std::vector<std::vector<std::complex<double>>> leaf_vals_;  // unsliced
     std::vector<std::vector<int>> leaf_wires_;                  // wire id of each axis
     std::vector<int> sliced_;                                   // sliced wire ids
+    std::vector<std::vector<std::pair<int, int>>> cut_;         // per leaf: (axis, sliced-wire index)
+    std::vector<int> leaf_rank_;                                // per leaf: rank without the sliced axes
     std::vector<Step> steps_;                                   // with the sliced axes removed
     SliceStats plan_;
+    // fused subtrees: the maximal subtrees of small steps, one device program each
+    int64_t fuse_slices_ = -1;
+    std::vector<char> step_fused_;                  // per step
+    std::vector<char> leaf_direct_;                 // per leaf: an operand of an unfused step
+    std::vector<int> roots_;                        // root step of each subtree
+    std::vector<std::shared_ptr<qt_program>> progs_;  // one per root
+    std::vector<double> const_pool_;                // FP64 values of their uncut leaves
+    std::vector<std::pair<int, int64_t>> param_leaves_;  // (cut leaf, offset in a slice's block)
+    int64_t block_elems_ = 0;                       // leaf elements of one slice's block
+    int64_t root_bytes_c128_ = 0;                   // subtree results of one slice, complex128
 };
 
 }  // namespace qtorch

@@ -54,3 +54,41 @@ def test_sliced_liouville_qft(ctx):
     assert sn.plan()["sliced_wires"] > 0
     v, _ = sn.run(ctx)
     assert abs(v - 1 / 16) <= 1e-13
+
+
+@pytest.mark.parametrize("prec,tol", [(qa.QT_PLAIN_C128, 1e-12), (qa.QT_C64_4M, 1e-5),
+                                      (qa.QT_C64_4M | qa.QT_RESCALE, 1e-5)])
+def test_fused_subtrees_match_step_by_step(ctx, prec, tol):
+    """Small subtrees run as device programs (one batch launch per chunk of slices,
+    results written to tensors that the remaining steps read): the same value as
+    running every step through qt_contract_pair, with all slices in one launch and
+    with three slices per launch, and the state vector's amplitude."""
+    q = wl.grid_random_circuit(4, 5, 8, seed=4)
+    bits = wl.bitstring(20, 6)
+    sn = qa.SlicedNetwork(q, bits, dim_log2=1, order=qa.ORDER_LINEGRAPH, max_bits=14)
+    plan = sn.plan()
+    assert plan["fused_subtrees"] >= 2 and plan["fused_steps"] > 0 and plan["slices"] >= 4
+    sn.set_fusion(0)
+    v_step, _ = sn.run(ctx, prec)
+    sn.set_fusion(-1)
+    v_all, _ = sn.run(ctx, prec)
+    sn.set_fusion(3)
+    v_three, _ = sn.run(ctx, prec)
+    psi, n = sv.simulate(q)
+    ref = psi[_amp_index(bits)]
+    scale = max(abs(ref), 2.0 ** (-n / 2))
+    for v in (v_step, v_all, v_three):
+        assert abs(v - ref) <= max(tol, 1e-10) * scale, (v, ref)
+    assert abs(v_all - v_step) <= tol * scale
+    assert v_three == v_all  # per-slice values do not depend on the chunking
+
+
+def test_fused_subtrees_liouville(ctx):
+    """Dimension-4 wires: a measured QFT5 network, fused and step by step, equals the
+    closed form |<0..0|QFT|x>|^2 = 1/32."""
+    sn = qa.SlicedNetwork(wl.qft_qasm(5), "1 0 1 1 0", dim_log2=2, order=qa.ORDER_GREEDY, max_bits=8)
+    assert sn.plan()["fused_subtrees"] >= 1
+    v_fused, _ = sn.run(ctx)
+    sn.set_fusion(0)
+    v_step, _ = sn.run(ctx)
+    assert abs(v_fused - 1 / 32) <= 1e-13 and abs(v_step - 1 / 32) <= 1e-13

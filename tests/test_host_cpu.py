@@ -254,3 +254,29 @@ def test_sliced_plan_bounds():
         assert p["cmacs_per_slice"] * p["slices"] >= whole["cmacs_per_slice"] // 4
     with pytest.raises(qa.InvalidFunctionInput):
         qa.SlicedNetwork(q, bits, dim_log2=3, max_bits=8)
+
+
+def test_sliced_fusion_plan_counts():
+    """The fused subtrees of an unsliced network, recounted from the plan: a step is
+    small when its result has <= 12 index bits, its operands <= 14 and its complex
+    MACs <= 2^16, and it is fused when its operands are leaves or fused steps; a
+    subtree's root is a fused step whose consumer is not fused (or the last step)."""
+    from qtorch_amd import workloads as wl
+    q = wl.grid_random_circuit(4, 5, 8, seed=2)
+    bits = wl.bitstring(20, 8)
+    got = qa.SlicedNetwork(q, bits, dim_log2=1, order=qa.ORDER_LINEGRAPH, max_bits=40).plan()
+    net = qa.Network(q, bits, None, dim_log2=1)
+    net.order(qa.ORDER_LINEGRAPH, 0)
+    plan = net.plan()
+    step_of = {int(r[2]): i for i, r in enumerate(plan) if int(r[2]) >= 0}
+    fused, consumer = [], {}
+    for i, (a, b, c, ra, rb, k, rc, _) in enumerate(plan.tolist()):
+        ok = rc <= 12 and ra <= 14 and rb <= 14 and rc + k <= 16
+        for op in (a, b):
+            if op in step_of:
+                consumer[step_of[op]] = i
+                ok = ok and fused[step_of[op]]
+        fused.append(ok)
+    roots = [i for i in range(len(plan)) if fused[i] and (i not in consumer or not fused[consumer[i]])]
+    assert got["fused_steps"] == sum(fused) > 0
+    assert got["fused_subtrees"] == len(roots) >= 1
