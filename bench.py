@@ -39,6 +39,8 @@ FP32_PEAK_TFLOPS = 157.3  # MI355X dense FP32 (vector and matrix)
 # benchmark; the factors are what the counters read for a 1 GiB device copy in the
 # same process (profiles/k_batch_qaoa_p1_3reg_n30.md)
 PMC_BYTES_PER_NETWORK = (2.0 * 14451.6 + 6544.0) * 1024 / 368640
+# the same for the specialised executor (generated straight-line code, qt_batch_specialised)
+PMC_BYTES_PER_NETWORK_SPECIALISED = (2.0 * 11935.2 + 5760.0) * 1024 / 368640
 
 
 def angle_sets(n: int, seed: int = 2024):
@@ -352,7 +354,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-forms", action="store_true", help="skip the other forms' accuracy and speed")
-    ap.add_argument("--exec", default="", help="executor shape 'lanes,wgs_per_cu' instead of the autotuner's")
+    ap.add_argument("--exec", default="", help="table executor shape 'lanes,wgs_per_cu' instead of the autotuner's")
+    ap.add_argument("--executor", default="specialised", choices=["specialised", "tables"],
+                    help="qaoa_p1: generated straight-line code (complex64 4M and complex128) or the table executor")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run plain device copies of this many MiB (calibrates HBM byte counters)")
     ap.add_argument("--workload", default="qaoa_p1", choices=["qaoa_p1", "qft8", "grid", "uneven", "serving"])
@@ -402,7 +406,19 @@ def main():
         b.config(lanes, wgs)
     else:
         lanes, wgs, _ = b.autotune()
-    b.capture()
+    # the table executor's time on the same launches, for comparison
+    b.launch()
+    ctx.sync()
+    tables_ms = b.time(max(5, args.steps // 5))
+    specialised = args.executor == "specialised" and args.form in ("c64_4m", "c128")
+    compile_ms = b.compile() if specialised else None
+    if specialised:
+        lanes, wgs = 1, 0  # one lane per network, one wavefront per 64 networks
+    try:
+        b.capture()
+        graph = True
+    except qa.DeviceFailure:  # launches stay direct
+        graph = False
     for _ in range(args.warmup):
         b.launch()
     ctx.sync()
@@ -420,7 +436,8 @@ def main():
         b.launch()
     ctx.sync()
     wall = time.perf_counter() - t0
-    kernel_ms = b.time(args.steps)  # HIP events on the engine's stream around the same launches
+    launch_ms = b.time(args.steps)  # HIP events on the engine's stream around whole launches
+    kernel_ms = b.time_kernel(args.steps)  # the same for the contraction kernel alone
     energies, zz = b.results()
     info = b.info()
     value = ne * args.sets * args.steps / wall
@@ -476,8 +493,21 @@ def main():
                  "order": "reference stochastic (seeded), one pairwise contraction per launch"}
 
     flops_per_launch = 8.0 * info["cmacs"]
-    achieved = flops_per_launch / (kernel_ms * 1e-3) / 1e12
-    traffic = PMC_BYTES_PER_NETWORK * ne * args.sets if lanes < 64 and not info["global_scratch"] else None
+    algorithmic_tflops = flops_per_launch / (kernel_ms * 1e-3) / 1e12
+    if specialised:
+        # the generated code drops terms with exact-zero constant factors and values
+        # that never reach a result: its executed FLOPs (2 per FMA) face the peak
+        _, fmas = b.specialised()
+        executed_flops = 2.0 * fmas
+        achieved = executed_flops / (kernel_ms * 1e-3) / 1e12
+        traffic = (PMC_BYTES_PER_NETWORK_SPECIALISED * ne * args.sets
+                   if PMC_BYTES_PER_NETWORK_SPECIALISED is not None else None)
+        kernel_name = "qt_batch_specialised (generated straight-line code, one lane per network)"
+    else:
+        executed_flops = flops_per_launch
+        achieved = algorithmic_tflops
+        traffic = PMC_BYTES_PER_NETWORK * ne * args.sets if lanes < 64 and not info["global_scratch"] else None
+        kernel_name = "k_batch_grouped (persistent table executor)" if lanes < 64 else "k_batch (persistent table executor)"
     res = {
         "metric": "QAOA edge-evals/s (p99 step latency), complex64 batched vs complex128 plain, rel err <= 1e-5",
         "value": value, "unit": "edge-evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -485,8 +515,11 @@ def main():
         "vs_baseline": None, "dtype": "c64" if args.form != "c128" else "c128", "data": "synthetic",
         "config": {"workload": "qaoa_p1_maxcut_3regular_n30_edges45", "angle_sets_per_step": args.sets,
                    "edge_networks_per_step": ne * args.sets, "form": args.form,
-                   "order": "greedy", "executor_lanes": lanes, "wgs_per_cu": wgs,
-                   "global_scratch": info["global_scratch"], "staged_leaves": info["staged_leaves"]},
+                   "order": "greedy", "executor": "specialised" if specialised else "tables",
+                   "executor_lanes": lanes, "wgs_per_cu": wgs,
+                   "global_scratch": info["global_scratch"] and not specialised,
+                   "staged_leaves": info["staged_leaves"] and not specialised},
+        "compile_ms": compile_ms, "table_executor_ms_per_launch": tables_ms, "hip_graph": graph,
         "p50_step_ms": 1e3 * float(np.percentile(lat, 50)), "p99_step_ms": 1e3 * float(np.percentile(lat, 99)),
         "accuracy": {"max_edge_rel_err_vs_c128": edge_err, "edges_within_1e-5": within,
                      "max_energy_rel_err_vs_c128": energy_err,
@@ -497,8 +530,11 @@ def main():
         "speedup_vs_plain_c128": (value / plain["value"]) if plain else None,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS, "traffic": traffic,
-                     "kernel": "k_batch (persistent fused contraction)", "kernel_ms_per_launch": kernel_ms,
-                     "algorithmic_flops_per_launch": flops_per_launch},
+                     "kernel": kernel_name, "kernel_ms_per_launch": kernel_ms,
+                     "launch_ms_with_group_sums": launch_ms,
+                     "executed_flops_per_launch": executed_flops,
+                     "algorithmic_flops_per_launch": flops_per_launch,
+                     "algorithmic_tflops": algorithmic_tflops},
     }
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(dgf, bg_ref)

@@ -203,11 +203,43 @@ qt_status qt_batch_capture(qt_batch* b);
 qt_status qt_batch_results(qt_batch* b, double* values, double* group_sums);
 /* Mean time of `reps` launches, by HIP events on the context's stream. */
 qt_status qt_batch_time(qt_batch* b, int reps, float* ms_per_launch);
+/* The same for the contraction kernel alone (no group sums; the table executors'
+ * work-counter reset, a one-thread kernel, is included): the dominant kernel's time
+ * per launch. */
+qt_status qt_batch_time_kernel(qt_batch* b, int reps, float* ms_per_launch);
 /* complex MACs of one launch, executor shape, flags (1: intermediates in device
  * memory, 2: leaves staged in LDS) and scratch elements per executor */
 qt_status qt_batch_info(const qt_batch* b, int64_t* cmacs, int* lanes, int* wgs_per_cu,
                         int* global_scratch, int* scratch_elems);
 qt_status qt_batch_destroy(qt_batch* b);
+
+/* ---- specialised executors -------------------------------------------------------------
+ * No counterpart in the reference (its ContractIndices walks index vectors per MAC,
+ * src/Network.h:876-971): for a batch whose programs stay fixed while instances
+ * change only their parameter blocks, qt_batch_compile generates one kernel in which
+ * every program is straight-line code (each output element an explicit FMA chain in
+ * the order of the table executors, constant leaves as literals with exact-zero
+ * terms dropped, intermediates in registers, one lane per instance), compiles it for
+ * the device at run time (hipRTC) and makes qt_batch_launch run it.  Only the
+ * complex64 4M and complex128 forms, only programs that end in scalars, and at most
+ * 250,000 FMAs of generated code in all (QT_INVALID_INPUT otherwise; the table
+ * executor keeps running).  The
+ * constants are read from the pool now: a later pool write that changes them, new
+ * programs, or an explicit qt_batch_config / qt_batch_config_ex drop the generated
+ * kernel (the table executor runs again).  *compile_ms (if non-NULL): generation,
+ * compilation and loading time. */
+qt_status qt_batch_compile(qt_batch* b, float* compile_ms);
+/* *active = 1 while the generated kernel is the one qt_batch_launch runs;
+ * *fmas_per_launch: FMAs its code executes for the current instances (each a real
+ * multiply-add of 2 FLOPs; terms with exact-zero constant factors and values that
+ * never reach the result are not executed) */
+qt_status qt_batch_specialised_info(const qt_batch* b, int* active, int64_t* fmas_per_launch);
+/* Host only: the HIP source qt_batch_compile would generate for these programs, with
+ * program i's constant leaf references offset by const_base[i] into `pool`
+ * (interleaved FP64, pool_elems elements).  Writes at most cap bytes (NUL-terminated)
+ * to buf (may be NULL) and the full length to *len. */
+qt_status qt_program_source(int n, qt_program* const* progs, const int64_t* const_base, const double* pool,
+                            int64_t pool_elems, int precision, char* buf, size_t cap, size_t* len);
 
 #ifdef __cplusplus
 }

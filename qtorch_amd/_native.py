@@ -73,6 +73,11 @@ _HIP_SIGS = {
     "qt_batch_set_instances": (_c.c_int, [_vp, _c.c_int, _i32p, _i64p, _c.c_int, _i32p, _c.c_double,
                                           _c.c_double]),
     "qt_batch_set_outputs": (_c.c_int, [_vp, _c.c_int, _c.POINTER(_vp)]),
+    "qt_batch_compile": (_c.c_int, [_vp, _c.POINTER(_c.c_float)]),
+    "qt_batch_time_kernel": (_c.c_int, [_vp, _c.c_int, _c.POINTER(_c.c_float)]),
+    "qt_batch_specialised_info": (_c.c_int, [_vp, _c.POINTER(_c.c_int), _c.POINTER(_c.c_int64)]),
+    "qt_program_source": (_c.c_int, [_c.c_int, _c.POINTER(_vp), _i64p, _dp, _c.c_int64, _c.c_int, _c.c_char_p,
+                                     _c.c_size_t, _c.POINTER(_c.c_size_t)]),
     "qt_batch_config": (_c.c_int, [_vp, _c.c_int, _c.c_int]),
     "qt_batch_config_ex": (_c.c_int, [_vp, _c.c_int, _c.c_int, _c.c_int]),
     "qt_batch_autotune": (_c.c_int, [_vp, _c.POINTER(_c.c_int), _c.POINTER(_c.c_int), _c.POINTER(_c.c_float)]),
@@ -145,6 +150,9 @@ _HOST_SIGS = {
                                   _c.POINTER(_c.c_double)]),
     "qth_sliced_destroy": (_c.c_int, [_vp]),
     "qth_sliced_set_fusion": (_c.c_int, [_vp, _c.c_int64]),
+    "qth_qaoa_source": (_c.c_int, [_c.c_char_p, _c.c_int, _c.c_int, _c.c_int, _c.c_uint64, _c.c_int, _c.c_int,
+                                   _c.c_int, _c.c_char_p, _c.c_size_t, _c.POINTER(_c.c_size_t)]),
+    "qth_qaoa_param_block": (_c.c_int, [_c.c_int, _dp, _dp]),
     "qth_serve_plan": (_c.c_int, [_c.c_int, _c.c_char_p, _c.c_int, _c.c_int, _c.c_char_p, _c.c_int,
                                   _c.POINTER(_c.c_int), _c.POINTER(_c.c_int64), _c.POINTER(_c.c_int64)]),
 }
@@ -500,6 +508,26 @@ class _DeviceBatch:
         self.ctx.check(hip.qt_batch_time(self.dev, reps, ctypes.byref(ms)), "qt_batch_time")
         return ms.value
 
+    def time_kernel(self, reps: int) -> float:
+        """Mean ms of the contraction kernel alone (no group sums)."""
+        ms = ctypes.c_float()
+        self.ctx.check(hip.qt_batch_time_kernel(self.dev, reps, ctypes.byref(ms)), "qt_batch_time_kernel")
+        return ms.value
+
+    def compile(self) -> float:
+        """Generate, compile (hipRTC) and load straight-line code for this batch's
+        programs; later launches run it.  Returns the milliseconds it took."""
+        ms = ctypes.c_float()
+        self.ctx.check(hip.qt_batch_compile(self.dev, ctypes.byref(ms)), "qt_batch_compile")
+        return ms.value
+
+    def specialised(self):
+        """(active, FMAs per launch) of the generated executor."""
+        a, f = ctypes.c_int(), ctypes.c_int64()
+        self.ctx.check(hip.qt_batch_specialised_info(self.dev, ctypes.byref(a), ctypes.byref(f)),
+                       "qt_batch_specialised_info")
+        return bool(a.value), f.value
+
     def info(self):
         cm = ctypes.c_int64()
         l, w, g, se = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
@@ -548,6 +576,25 @@ class QaoaBatch(_DeviceBatch):
             self.close()
         except Exception:
             pass
+
+
+def qaoa_source(dgf: str, p: int, first_edge: int, n_edges: int, precision: int = QT_C64_4M, ref_map: bool = True,
+                order: int = ORDER_GREEDY, seed: int = 0) -> str:
+    """Specialised-executor source of some QAOA edge programs (host only)."""
+    n = ctypes.c_size_t()
+    args = (dgf.encode(), p, 1 if ref_map else 0, order, ctypes.c_uint64(seed), precision, first_edge, n_edges)
+    _host_check(host.qth_qaoa_source(*args, None, 0, ctypes.byref(n)))
+    buf = ctypes.create_string_buffer(n.value + 1)
+    _host_check(host.qth_qaoa_source(*args, buf, n.value + 1, ctypes.byref(n)))
+    return buf.value.decode()
+
+
+def qaoa_param_block(p: int, betas_gammas: Sequence[float]) -> np.ndarray:
+    """The 32 p complex parameter elements of one angle set (complex128 array)."""
+    bg = np.ascontiguousarray(betas_gammas, dtype=np.float64)
+    out = np.zeros(64 * p)
+    _host_check(host.qth_qaoa_param_block(p, bg.ctypes.data_as(_dp), out.ctypes.data_as(_dp)))
+    return out.view(np.complex128)
 
 
 class NetBatch(_DeviceBatch):

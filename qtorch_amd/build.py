@@ -42,7 +42,7 @@ def build_hip():
     os.makedirs(LIB, exist_ok=True)
     srcs = sorted(glob.glob(os.path.join(PKG, "csrc", "hip", "*.hip")))
     out = os.path.join(LIB, "libqtorch_hip.so")
-    _run([HIPCC, *HIP_FLAGS, "-shared", "-I", os.path.join(REPO, "include"), *srcs, "-o", out + ".tmp"])
+    _run([HIPCC, *HIP_FLAGS, "-shared", "-I", os.path.join(REPO, "include"), *srcs, "-lhiprtc", "-o", out + ".tmp"])
     os.replace(out + ".tmp", out)
     return out
 

@@ -21,125 +21,7 @@
 #include <numeric>
 #include <vector>
 
-#include "qt_internal.h"
-
-namespace qt {
-
-struct ProgHdr {
-    int32_t step_begin, n_steps;
-    int32_t leaf_begin, n_leaves;
-    int32_t scratch_elems;
-    int32_t flags;
-    int64_t const_base;  // pool offset added to the program's constant leaf references
-    int32_t cst_begin, n_cst;  // the program's distinct constant leaves (CstEntry list)
-    int32_t param_elems;       // extent of an instance's parameter block that it reads
-    int32_t pad;
-};
-
-// A distinct constant leaf of a program and its place in a wave's LDS leaf area.
-struct CstEntry {
-    int64_t src;  // pool offset relative to the program's const_base
-    int32_t n;    // elements
-    int32_t dst;  // element offset in the wave's constant area
-};
-
-struct StepDesc {
-    int32_t a, b;       // >= 0: scratch element offset; < 0: leaf -(1 + index)
-    int32_t c;          // scratch element offset of the result (final step: unused)
-    int32_t tab;        // offset of the step's tables in the table array (uint32 units)
-    int32_t nc_bits;    // output elements = 1 << nc_bits
-    int32_t k_bits;     // summed elements = 1 << k_bits
-    int32_t flags;      // bit 0: final step
-    int32_t pad;
-};
-
-struct Instance {
-    int32_t prog;
-    int32_t pad;
-    int64_t param_base;
-};
-
-}  // namespace qt
-
-struct qt_program {
-    std::vector<qt::StepDesc> steps;  // offsets relative to this program's table block
-    std::vector<uint32_t> tables;
-    std::vector<int64_t> leaf_refs;
-    std::vector<int32_t> leaf_ranks;
-    int dim_log2 = 2;
-    int64_t cmacs = 0;
-    int32_t scratch_elems = 0;
-    int32_t max_tensor_elems = 0;
-    int32_t out_elems = 1;  // elements of the last step's result
-};
-
-struct qt_batch {
-    qt::Ctx* ctx = nullptr;
-    int precision = QT_C64_4M;
-    int n_programs = 0;
-    int32_t scratch_elems = 0;  // max over programs
-    int64_t cmacs_total = 0;    // per-instance sum, for instances set
-    std::vector<int64_t> prog_cmacs;
-    // device arrays
-    qt::ProgHdr* d_progs = nullptr;
-    qt::StepDesc* d_steps = nullptr;
-    uint32_t* d_tabs = nullptr;
-    int64_t* d_leaf_refs = nullptr;
-    void* d_pool = nullptr;  // leaf pool in the batch precision
-    int64_t pool_elems = 0;
-    int64_t max_leaf_ref_end = 0;  // absolute part
-    int64_t max_param_ref_end = 0; // relative part
-    qt::Instance* d_inst = nullptr;
-    int n_inst = 0;
-    double* d_out = nullptr;       // 2 * n_inst
-    size_t cap_inst = 0, cap_out = 0, cap_seg = 0, cap_members = 0, cap_group = 0, cap_chunks = 0;
-    int32_t* d_counter = nullptr;
-    // group sums
-    int32_t* d_seg = nullptr;
-    int32_t* d_members = nullptr;
-    double* d_group = nullptr;
-    int n_groups = 0;
-    double c0 = 0.0, c1 = 1.0;
-    void* d_global_scratch = nullptr;
-    size_t global_scratch_bytes = 0;
-    // launch configuration
-    int lanes = 64;  // 256: a workgroup per instance; 64: a wavefront; < 64: lanes per instance in a wave
-    int wgs_per_cu = 4;
-    int scratch_place = QT_SCRATCH_AUTO;  // with QT_SCRATCH_STAGE_LEAVES or-ed in
-    bool stage_leaves = false;
-    bool shape_ok = false;  // device arrays fit the executor shape
-    int32_t slot_stride = 0;      // elements between the scratch areas of two executors
-    int32_t pstage = 0, cstage = 0;  // LDS leaf areas: parameter block per slot, constant leaves per wave
-    std::vector<int32_t> h_prog;  // program of each instance (host copy, for chunking)
-    std::vector<int32_t> prog_out_elems;  // elements of each program's result
-    void** d_tout = nullptr;      // per instance: output tensor of the last step (qt_batch_set_outputs)
-    size_t cap_tout = 0;
-    bool needs_tout = false;      // an instance's program ends in a tensor
-    bool tensor_out = false;      // output tensors are set for the current instances
-    int2* d_chunks = nullptr;
-    int n_chunks = 0;
-    int n_cus = 0;
-    bool use_global_scratch = false;
-    bool rescale = false;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t graph_exec = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_done = nullptr;
-    // host copies of the device program arrays (programs can be appended)
-    std::vector<qt::ProgHdr> h_hdr;
-    std::vector<qt::StepDesc> h_steps;
-    std::vector<uint32_t> h_tabs;
-    std::vector<int64_t> h_refs;
-    std::vector<int32_t> h_leaf_lds;     // per leaf: offset in the constant area, or (1 << 31) | parameter offset
-    std::vector<qt::CstEntry> h_csts;
-    int32_t* d_leaf_lds = nullptr;
-    qt::CstEntry* d_csts = nullptr;
-    size_t cap_leaf_lds = 0, cap_csts = 0;
-    std::vector<int64_t> prog_const_end, prog_param_end;  // pool extents each program reads
-    size_t cap_progs = 0, cap_steps = 0, cap_tabs = 0, cap_refs = 0;
-    int64_t pool_cap = 0;           // elements allocated for the pool
-    void* d_pool_stage = nullptr;   // FP64 staging of pool writes for complex64 batches
-    int64_t pool_stage_cap = 0;
-};
+#include "qt_batch_impl.h"
 
 namespace qt {
 namespace {
@@ -462,6 +344,32 @@ __global__ __launch_bounds__(64) void k_batch_grouped(const ProgHdr* __restrict_
 
 __global__ void k_reset_counter(int32_t* c) { *c = 0; }
 
+// The same sums when every group has m members and member j of group g is instance
+// j * n_groups + g (angle set g of edge j in an edge-major batch).  A workgroup takes
+// 32 groups; its 8 slices of 32 threads each add the members j = slice, slice + 8,
+// ... in order (neighbouring threads read neighbouring values), then one thread per
+// group adds the 8 partial sums in slice order: a fixed order, whatever the timing.
+constexpr int kSumGroups = 32, kSumSlices = 8;
+__global__ __launch_bounds__(kSumGroups* kSumSlices) void k_group_sum_strided(const double2* __restrict__ out,
+                                                                             double* __restrict__ sums, int n_groups,
+                                                                             int m, double c0, double c1) {
+    __shared__ double part[kSumSlices][kSumGroups];
+    const int gl = int(threadIdx.x) % kSumGroups, slice = int(threadIdx.x) / kSumGroups;
+    const int g = int(blockIdx.x) * kSumGroups + gl;
+    double acc = 0.0;
+    if (g < n_groups) {
+#pragma unroll 4
+        for (int j = slice; j < m; j += kSumSlices) acc += c0 + c1 * out[int64_t(j) * n_groups + g].x;
+    }
+    part[slice][gl] = acc;
+    __syncthreads();
+    if (slice == 0 && g < n_groups) {
+        double t = part[0][gl];
+        for (int k = 1; k < kSumSlices; ++k) t += part[k][gl];
+        sums[g] = t;
+    }
+}
+
 // FP64 group sums: group g = sum over its members of c0 + c1 * Re(value).
 __global__ void k_group_sum(const double2* __restrict__ out, const int32_t* __restrict__ seg,
                             const int32_t* __restrict__ members, double* __restrict__ sums, int n_groups, double c0,
@@ -596,22 +504,38 @@ qt_status launch_batch_t(qt_batch* b) {
     return check_hip(c, hipGetLastError(), "k_batch");
 }
 
+qt_status launch_tables(qt_batch* b) {
+    switch (b->precision) {
+        case QT_PLAIN_C128: return launch_batch_t<double2, QT_PLAIN_C128>(b);
+        case QT_C64_3M: return launch_batch_t<float2, QT_C64_3M>(b);
+        case QT_C64_BF16X3: return launch_batch_t<float2, QT_C64_BF16X3>(b);
+        default: return launch_batch_t<float2, QT_C64_4M>(b);
+    }
+}
+
 qt_status launch_batch(qt_batch* b) {
     Ctx* c = b->ctx;
     // a failed reconfiguration leaves scratch or chunk arrays that do not fit the shape
     if (!b->shape_ok) return set_error(c, QT_INVALID_INPUT, "batch: executor shape not configured (last change failed)");
     if (b->needs_tout && !b->tensor_out)
         return set_error(c, QT_INVALID_INPUT, "batch: programs with tensor results need qt_batch_set_outputs");
-    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, c->stream, b->d_counter);
-    qt_status s = check_hip(c, hipGetLastError(), "k_reset_counter");
-    if (s != QT_OK) return s;
-    switch (b->precision) {
-        case QT_PLAIN_C128: s = launch_batch_t<double2, QT_PLAIN_C128>(b); break;
-        case QT_C64_3M: s = launch_batch_t<float2, QT_C64_3M>(b); break;
-        case QT_C64_BF16X3: s = launch_batch_t<float2, QT_C64_BF16X3>(b); break;
-        default: s = launch_batch_t<float2, QT_C64_4M>(b); break;
+    qt_status s = QT_OK;
+    if (b->specialised) {
+        s = launch_specialised(b);
+    } else {
+        hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, c->stream, b->d_counter);
+        s = check_hip(c, hipGetLastError(), "k_reset_counter");
+        if (s != QT_OK) return s;
+        s = launch_tables(b);
     }
     if (s != QT_OK || b->n_groups == 0) return s;
+    if (b->group_stride_m > 0) {
+        hipLaunchKernelGGL(k_group_sum_strided, dim3(unsigned((b->n_groups + kSumGroups - 1) / kSumGroups)),
+                           dim3(kSumGroups * kSumSlices), 0, c->stream,
+                           reinterpret_cast<const double2*>(b->d_out), b->d_group, b->n_groups, b->group_stride_m,
+                           b->c0, b->c1);
+        return check_hip(c, hipGetLastError(), "k_group_sum_strided");
+    }
     hipLaunchKernelGGL(k_group_sum, dim3(unsigned(b->n_groups)), dim3(64), 0, c->stream,
                        reinterpret_cast<const double2*>(b->d_out), b->d_seg, b->d_members, b->d_group, b->n_groups,
                        b->c0, b->c1);
@@ -747,6 +671,7 @@ qt_status append_programs(qt_batch* b, int n, qt_program* const* progs, const in
         b->h_hdr.push_back(h);
     }
     b->n_programs += n;
+    drop_specialised(b);  // generated for the previous program set
     (void)hipStreamSynchronize(c->stream);  // a launch in flight reads the arrays
     qt_status s = dev_put(c, &b->d_progs, &b->cap_progs, b->h_hdr.data(), b->h_hdr.size());
     if (s == QT_OK) s = dev_put(c, &b->d_steps, &b->cap_steps, b->h_steps.data(), b->h_steps.size());
@@ -762,6 +687,10 @@ qt_status append_programs(qt_batch* b, int n, qt_program* const* progs, const in
 // batches), growing the pool (contents kept) when needed.  The stream is idle.
 qt_status write_pool(qt_batch* b, int64_t offset, const double* values, int64_t n) {
     Ctx* c = b->ctx;
+    if (b->specialised && specialised_consts_change(b, offset, values, n)) {
+        drop_specialised(b);  // its literals are the old constants
+        if (b->graph_exec) (void)configure_scratch(b);  // the captured graph runs the dropped kernel
+    }
     const size_t es = elem_size(b->precision);
     const int64_t need = offset + n;
     if (need > b->pool_cap) {
@@ -1146,8 +1075,19 @@ qt_status qt_batch_set_instances(qt_batch* b, int n, const int32_t* prog, const 
         for (int i = 0; i < n; ++i)
             if (group[i] >= 0) members[size_t(fill[size_t(group[i])]++)] = i;
     }
+    // groups laid out as member j of group g = instance j * n_groups + g
+    int stride_m = 0;
+    if (n_groups > 0 && n % n_groups == 0) {
+        const int m = n / n_groups;
+        bool strided = int(members.size()) == n;
+        for (int g = 0; g < n_groups && strided; ++g)
+            for (int j = 0; j < m && strided; ++j)
+                strided = seg[size_t(g)] == g * m && members[size_t(g * m + j)] == j * n_groups + g;
+        if (strided) stride_m = m;
+    }
     b->n_inst = 0;
     b->n_groups = 0;
+    b->group_stride_m = 0;
     b->h_prog.assign(prog, prog + n);
     qt_status s = qt::dev_put(ctx, &b->d_inst, &b->cap_inst, inst.data(), inst.size());
     if (s == QT_OK) s = qt::dev_put<double>(ctx, &b->d_out, &b->cap_out, nullptr, size_t(std::max(n, 1)) * 2);
@@ -1159,11 +1099,16 @@ qt_status qt_batch_set_instances(qt_batch* b, int n, const int32_t* prog, const 
     if (s != QT_OK) return s;
     b->n_inst = n;
     b->n_groups = n_groups;
+    b->group_stride_m = stride_m;
     b->needs_tout = needs_tout;
     b->tensor_out = false;  // output tensors belong to the instances they were set for
     b->cmacs_total = cmacs;
     b->c0 = c0;
     b->c1 = c1;
+    if (b->specialised) {
+        s = qt::specialised_chunks(b);
+        if (s != QT_OK) qt::drop_specialised(b);
+    }
     return qt::configure_scratch(b);  // rebuilds the chunk list, drops a captured graph
 }
 
@@ -1215,6 +1160,7 @@ qt_status qt_batch_config_ex(qt_batch* b, int lanes, int wgs_per_cu, int scratch
         wgs_per_cu < 1 || wgs_per_cu > 16 || (scratch & ~(3 | QT_SCRATCH_STAGE_LEAVES)) != 0 || (scratch & 3) == 3)
         return QT_INVALID_INPUT;
     std::lock_guard<std::recursive_mutex> g(b->ctx->mu);
+    qt::drop_specialised(b);  // an explicit table-executor shape replaces generated code
     const int l0 = b->lanes, w0 = b->wgs_per_cu, s0 = b->scratch_place;
     b->lanes = lanes;
     b->wgs_per_cu = wgs_per_cu;
@@ -1302,6 +1248,34 @@ qt_status qt_batch_time(qt_batch* b, int reps, float* ms_per_launch) {
     return s;
 }
 
+qt_status qt_batch_time_kernel(qt_batch* b, int reps, float* ms_per_launch) {
+    if (!b || reps < 1 || !ms_per_launch) return QT_INVALID_INPUT;
+    auto* ctx = b->ctx;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    *ms_per_launch = 0;
+    if (b->n_inst == 0) return QT_OK;
+    if (!b->shape_ok) return set_error(ctx, QT_INVALID_INPUT, "batch: executor shape not configured (last change failed)");
+    if (b->needs_tout && !b->tensor_out)
+        return set_error(ctx, QT_INVALID_INPUT, "batch: programs with tensor results need qt_batch_set_outputs");
+    qt_status s = check_hip(ctx, hipEventRecord(b->ev0, ctx->stream), "event record");
+    for (int r = 0; r < reps && s == QT_OK; ++r) {
+        if (b->specialised) {
+            s = qt::launch_specialised(b);
+        } else {
+            hipLaunchKernelGGL(qt::k_reset_counter, dim3(1), dim3(1), 0, ctx->stream, b->d_counter);
+            s = check_hip(ctx, hipGetLastError(), "k_reset_counter");
+            if (s == QT_OK) s = qt::launch_tables(b);
+        }
+    }
+    if (s == QT_OK) s = check_hip(ctx, hipEventRecord(b->ev1, ctx->stream), "event record");
+    if (s == QT_OK) s = check_hip(ctx, hipEventSynchronize(b->ev1), "event sync");
+    float ms = 0;
+    if (s == QT_OK) s = check_hip(ctx, hipEventElapsedTime(&ms, b->ev0, b->ev1), "elapsed");
+    *ms_per_launch = ms / float(reps);
+    return s;
+}
+
 qt_status qt_batch_autotune(qt_batch* b, int* lanes, int* wgs_per_cu, float* ms) {
     if (!b) return QT_INVALID_INPUT;
     std::lock_guard<std::recursive_mutex> g(b->ctx->mu);
@@ -1354,10 +1328,11 @@ qt_status qt_batch_destroy(qt_batch* b) {
         if (b->ev0) (void)hipEventDestroy(b->ev0);
         if (b->ev1) (void)hipEventDestroy(b->ev1);
         if (b->ev_done) (void)hipEventDestroy(b->ev_done);
+        qt::drop_specialised(b);
         for (void* p : {(void*)b->d_progs, (void*)b->d_steps, (void*)b->d_tabs, (void*)b->d_leaf_refs, b->d_pool,
                         (void*)b->d_inst, (void*)b->d_out, (void*)b->d_counter, (void*)b->d_seg, (void*)b->d_members,
                         (void*)b->d_group, b->d_global_scratch, (void*)b->d_chunks, b->d_pool_stage,
-                        (void*)b->d_leaf_lds, (void*)b->d_csts, (void*)b->d_tout})
+                        (void*)b->d_leaf_lds, (void*)b->d_csts, (void*)b->d_tout, (void*)b->d_schunks})
             if (p) (void)hipFree(p);
         --b->ctx->live;
     }

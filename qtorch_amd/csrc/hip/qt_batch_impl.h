@@ -1,0 +1,149 @@
+// Internal structures of programs and batches, shared by qt_batch.hip (table
+// executors) and qt_codegen.hip (specialised executors).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "qt_internal.h"
+
+namespace qt {
+
+struct ProgHdr {
+    int32_t step_begin, n_steps;
+    int32_t leaf_begin, n_leaves;
+    int32_t scratch_elems;
+    int32_t flags;
+    int64_t const_base;  // pool offset added to the program's constant leaf references
+    int32_t cst_begin, n_cst;  // the program's distinct constant leaves (CstEntry list)
+    int32_t param_elems;       // extent of an instance's parameter block that it reads
+    int32_t pad;
+};
+
+// A distinct constant leaf of a program and its place in a wave's LDS leaf area.
+struct CstEntry {
+    int64_t src;  // pool offset relative to the program's const_base
+    int32_t n;    // elements
+    int32_t dst;  // element offset in the wave's constant area
+};
+
+struct StepDesc {
+    int32_t a, b;       // >= 0: scratch element offset; < 0: leaf -(1 + index)
+    int32_t c;          // scratch element offset of the result (final step: unused)
+    int32_t tab;        // offset of the step's tables in the table array (uint32 units)
+    int32_t nc_bits;    // output elements = 1 << nc_bits
+    int32_t k_bits;     // summed elements = 1 << k_bits
+    int32_t flags;      // bit 0: final step
+    int32_t pad;
+};
+
+struct Instance {
+    int32_t prog;
+    int32_t pad;
+    int64_t param_base;
+};
+
+}  // namespace qt
+
+struct qt_program {
+    std::vector<qt::StepDesc> steps;  // offsets relative to this program's table block
+    std::vector<uint32_t> tables;
+    std::vector<int64_t> leaf_refs;
+    std::vector<int32_t> leaf_ranks;
+    int dim_log2 = 2;
+    int64_t cmacs = 0;
+    int32_t scratch_elems = 0;
+    int32_t max_tensor_elems = 0;
+    int32_t out_elems = 1;  // elements of the last step's result
+};
+
+struct qt_batch {
+    qt::Ctx* ctx = nullptr;
+    int precision = QT_C64_4M;
+    int n_programs = 0;
+    int32_t scratch_elems = 0;  // max over programs
+    int64_t cmacs_total = 0;    // per-instance sum, for instances set
+    std::vector<int64_t> prog_cmacs;
+    // device arrays
+    qt::ProgHdr* d_progs = nullptr;
+    qt::StepDesc* d_steps = nullptr;
+    uint32_t* d_tabs = nullptr;
+    int64_t* d_leaf_refs = nullptr;
+    void* d_pool = nullptr;  // leaf pool in the batch precision
+    int64_t pool_elems = 0;
+    int64_t max_leaf_ref_end = 0;  // absolute part
+    int64_t max_param_ref_end = 0; // relative part
+    qt::Instance* d_inst = nullptr;
+    int n_inst = 0;
+    double* d_out = nullptr;       // 2 * n_inst
+    size_t cap_inst = 0, cap_out = 0, cap_seg = 0, cap_members = 0, cap_group = 0, cap_chunks = 0;
+    int32_t* d_counter = nullptr;
+    // group sums
+    int32_t* d_seg = nullptr;
+    int32_t* d_members = nullptr;
+    double* d_group = nullptr;
+    int n_groups = 0;
+    int group_stride_m = 0;  // > 0: group g = instances j * n_groups + g, j < m (k_group_sum_strided)
+    double c0 = 0.0, c1 = 1.0;
+    void* d_global_scratch = nullptr;
+    size_t global_scratch_bytes = 0;
+    // launch configuration
+    int lanes = 64;  // 256: a workgroup per instance; 64: a wavefront; < 64: lanes per instance in a wave
+    int wgs_per_cu = 4;
+    int scratch_place = QT_SCRATCH_AUTO;  // with QT_SCRATCH_STAGE_LEAVES or-ed in
+    bool stage_leaves = false;
+    bool shape_ok = false;  // device arrays fit the executor shape
+    int32_t slot_stride = 0;      // elements between the scratch areas of two executors
+    int32_t pstage = 0, cstage = 0;  // LDS leaf areas: parameter block per slot, constant leaves per wave
+    std::vector<int32_t> h_prog;  // program of each instance (host copy, for chunking)
+    std::vector<int32_t> prog_out_elems;  // elements of each program's result
+    void** d_tout = nullptr;      // per instance: output tensor of the last step (qt_batch_set_outputs)
+    size_t cap_tout = 0;
+    bool needs_tout = false;      // an instance's program ends in a tensor
+    bool tensor_out = false;      // output tensors are set for the current instances
+    int2* d_chunks = nullptr;
+    int n_chunks = 0;
+    int n_cus = 0;
+    bool use_global_scratch = false;
+    bool rescale = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_done = nullptr;
+    // host copies of the device program arrays (programs can be appended)
+    std::vector<qt::ProgHdr> h_hdr;
+    std::vector<qt::StepDesc> h_steps;
+    std::vector<uint32_t> h_tabs;
+    std::vector<int64_t> h_refs;
+    std::vector<int32_t> h_leaf_lds;     // per leaf: offset in the constant area, or (1 << 31) | parameter offset
+    std::vector<qt::CstEntry> h_csts;
+    int32_t* d_leaf_lds = nullptr;
+    qt::CstEntry* d_csts = nullptr;
+    size_t cap_leaf_lds = 0, cap_csts = 0;
+    std::vector<int64_t> prog_const_end, prog_param_end;  // pool extents each program reads
+    size_t cap_progs = 0, cap_steps = 0, cap_tabs = 0, cap_refs = 0;
+    int64_t pool_cap = 0;           // elements allocated for the pool
+    void* d_pool_stage = nullptr;   // FP64 staging of pool writes for complex64 batches
+    int64_t pool_stage_cap = 0;
+    // specialised executor (qt_batch_compile): generated kernel and its chunks of up to
+    // 64 consecutive instances of one program (one lane each)
+    hipModule_t module = nullptr;
+    hipFunction_t kernel = nullptr;
+    bool specialised = false;
+    std::vector<double> spec_consts;  // constant leaves the kernel was generated with (FP64)
+    std::vector<int64_t> spec_fmas;   // FMAs per instance of each program in the generated code
+    int2* d_schunks = nullptr;
+    size_t cap_schunks = 0;
+    int n_schunks = 0;
+};
+
+namespace qt {
+// specialised executors (qt_codegen.hip)
+qt_status specialised_chunks(qt_batch* b);
+void drop_specialised(qt_batch* b);
+qt_status launch_specialised(qt_batch* b);
+// a pool write of n values at offset would change constants the kernel embeds
+bool specialised_consts_change(const qt_batch* b, int64_t offset, const double* values, int64_t n);
+}  // namespace qt

@@ -1,0 +1,485 @@
+// Specialised executors: the programs of a batch compiled to straight-line code.
+//
+// The table executors (qt_batch.hip) read every step's index maps from packed offset
+// tables and gather operands from LDS, so a complex MAC costs table loads, address
+// arithmetic and two LDS gathers besides its four FMAs.  When a batch's programs are
+// fixed and its instances differ only in their parameter blocks (the edge light
+// cones of one graph run for many angle sets), qt_batch_compile writes one HIP
+// kernel in which every program is a branch of straight-line code:
+//   * each output element of each step is an explicit chain of FMAs over named
+//     values, in the order and with the operand roles of the table executors' Acc
+//     (so results agree bit for bit, up to the sign of zero);
+//   * constant leaves become literals and terms with an exact-zero constant factor
+//     are dropped;
+//   * parameter leaves are loaded once per instance; intermediates are local values
+//     that the compiler keeps in registers (no LDS, no tables);
+//   * one lane runs one instance, a wavefront up to 64 consecutive instances of one
+//     program (the branch is wave-uniform).
+// The kernel is compiled at run time for the device's architecture (hipRTC) and
+// loaded as a module; the constant leaves are read from the batch's pool when it is
+// compiled (writes over them afterwards drop the compiled kernel).
+#include <hip/hiprtc.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <set>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "qt_batch_impl.h"
+
+namespace qt {
+namespace {
+
+// Straight-line code grows with the programs' work: beyond this many FMAs in total
+// (the 45 light cones of a p = 1 QAOA graph need about 90 thousand, compiled in about
+// ten seconds) the compile time and code size stop paying off.
+constexpr int64_t kMaxSpecialisedFmas = 250000;
+
+struct ProgView {
+    const StepDesc* steps = nullptr;
+    int n_steps = 0;
+    const uint32_t* tabs = nullptr;  // steps[i].tab indexes this array
+    const int64_t* refs = nullptr;
+    int n_leaves = 0;
+    int64_t const_base = 0;
+};
+
+// One part (real or imaginary) of an operand element: a literal or a variable.
+struct Part {
+    bool lit = true;
+    double v = 0;          // literal value (already rounded for complex64)
+    std::string name;      // variable
+};
+struct Val {
+    Part re, im;
+    int step = -1, elem = -1;  // intermediate element it reads (liveness), else -1
+};
+
+class Writer {
+public:
+    explicit Writer(bool c128) : c128_(c128) {}
+    std::string lit(double x) const {
+        char buf[64];
+        if (c128_) std::snprintf(buf, sizeof buf, "%a", x);
+        else std::snprintf(buf, sizeof buf, "%af", double(float(x)));
+        return std::signbit(x) ? "(" + std::string(buf) + ")" : std::string(buf);
+    }
+    std::string str(const Part& p) const { return p.lit ? lit(p.v) : p.name; }
+    std::string neg(const Part& p) const { return p.lit ? lit(-p.v) : "(-" + p.name + ")"; }
+    const char* fma() const { return c128_ ? "__builtin_fma" : "__builtin_fmaf"; }
+    const char* real() const { return c128_ ? "double" : "float"; }
+
+private:
+    bool c128_;
+};
+
+bool zero(const Part& p) { return p.lit && p.v == 0.0; }
+
+// The four FMAs of Acc<V, 4M / C128>::mac(a, b) in their order: re += a.x b.x;
+// re += -a.y b.y; im += a.x b.y; im += a.y b.x.  A term with an exact-zero literal
+// factor adds exactly zero and is dropped.
+struct Term {
+    Val a, b;
+    bool f[4];
+};
+
+struct Program {
+    std::string text;
+    int64_t fmas = 0;  // FMAs the generated code executes per instance
+};
+
+// Source of one program: a device function prog<id>(const real* P, double* out).
+bool emit_program(int id, const ProgView& pv, bool c128, const double* pool, int64_t pool_elems, Program* out,
+                  std::string* err) {
+    Writer w(c128);
+    const int64_t kParam = QT_LEAF_PARAM;
+    const std::string tag = "program " + std::to_string(id) + ": ";
+    // which step's element each scratch element holds (the table executors' reuse)
+    std::map<int32_t, std::pair<int, int>> scratch;
+    // per step, per output element: its terms; parts that stay exactly zero
+    std::vector<std::vector<std::vector<Term>>> terms(size_t(pv.n_steps));
+    std::vector<std::vector<std::pair<bool, bool>>> has(size_t(pv.n_steps));  // (re non-empty, im non-empty)
+    auto operand = [&](int32_t code, int64_t idx, Val* v) -> bool {
+        if (code >= 0) {
+            auto it = scratch.find(code + int32_t(idx));
+            if (it == scratch.end()) {
+                *err = tag + "read of an unwritten scratch element";
+                return false;
+            }
+            const int st = it->second.first, e = it->second.second;
+            const auto h = has[size_t(st)][size_t(e)];
+            const std::string base = "t" + std::to_string(st) + "_" + std::to_string(e);
+            v->re = h.first ? Part{false, 0, base + "r"} : Part{};
+            v->im = h.second ? Part{false, 0, base + "i"} : Part{};
+            v->step = st;
+            v->elem = e;
+            return true;
+        }
+        const int li = -1 - code;
+        if (li < 0 || li >= pv.n_leaves) {
+            *err = tag + "bad leaf";
+            return false;
+        }
+        const int64_t r = pv.refs[li];
+        if (r & kParam) {
+            const int64_t e = (r & ~kParam) + idx;
+            v->re = Part{false, 0, "p" + std::to_string(e) + "r"};
+            v->im = Part{false, 0, "p" + std::to_string(e) + "i"};
+            return true;
+        }
+        const int64_t e = pv.const_base + r + idx;
+        if (e < 0 || e >= pool_elems) {
+            *err = tag + "constant leaf outside the pool";
+            return false;
+        }
+        v->re = Part{true, c128 ? pool[2 * e] : double(float(pool[2 * e])), ""};
+        v->im = Part{true, c128 ? pool[2 * e + 1] : double(float(pool[2 * e + 1])), ""};
+        return true;
+    };
+    // pass 1: the terms of every output element, with zeros propagated
+    int64_t n_terms = 0;
+    for (int si = 0; si < pv.n_steps; ++si) {
+        const StepDesc& st = pv.steps[si];
+        const int64_t nC = int64_t(1) << st.nc_bits, K = int64_t(1) << st.k_bits;
+        const uint32_t* fc = pv.tabs + st.tab;
+        const uint32_t* gs = fc + nC;
+        const bool final_step = (st.flags & 1) != 0;
+        if (final_step && (nC != 1 || si + 1 != pv.n_steps)) {
+            *err = tag + "a specialised program must end in a scalar";
+            return false;
+        }
+        auto& ts = terms[size_t(si)];
+        auto& hs = has[size_t(si)];
+        ts.resize(size_t(nC));
+        hs.resize(size_t(nC));
+        for (int64_t c = 0; c < nC; ++c) {
+            const uint32_t oa = fc[c] & 0xffffu, ob = fc[c] >> 16;
+            bool hr = false, hi = false;
+            for (int64_t s = 0; s < K; ++s) {
+                Term t;
+                if (!operand(st.a, int64_t(oa + (gs[s] & 0xffffu)), &t.a) ||
+                    !operand(st.b, int64_t(ob + (gs[s] >> 16)), &t.b))
+                    return false;
+                t.f[0] = !zero(t.a.re) && !zero(t.b.re);
+                t.f[1] = !zero(t.a.im) && !zero(t.b.im);
+                t.f[2] = !zero(t.a.re) && !zero(t.b.im);
+                t.f[3] = !zero(t.a.im) && !zero(t.b.re);
+                if (!(t.f[0] || t.f[1] || t.f[2] || t.f[3])) continue;
+                hr = hr || t.f[0] || t.f[1];
+                hi = hi || t.f[2] || t.f[3];
+                ts[size_t(c)].push_back(t);
+                if (++n_terms > kMaxSpecialisedFmas) {  // bounded memory for programs that will not fit
+                    *err = tag + "too large to specialise; the table executor runs it";
+                    return false;
+                }
+            }
+            hs[size_t(c)] = {hr, hi};
+        }
+        if (!final_step)
+            for (int64_t c = 0; c < nC; ++c) scratch[st.c + int32_t(c)] = {si, int(c)};
+    }
+    // pass 2: liveness from the final scalar backwards
+    std::vector<std::vector<char>> live(size_t(pv.n_steps));
+    for (int si = 0; si < pv.n_steps; ++si) live[size_t(si)].assign(terms[size_t(si)].size(), 0);
+    live.back()[0] = 1;
+    for (int si = pv.n_steps - 1; si >= 0; --si)
+        for (size_t c = 0; c < terms[size_t(si)].size(); ++c) {
+            if (!live[size_t(si)][c]) continue;
+            for (const Term& t : terms[size_t(si)][c])
+                for (const Val* v : {&t.a, &t.b})
+                    if (v->step >= 0) live[size_t(v->step)][size_t(v->elem)] = 1;
+        }
+    // pass 3: code
+    std::ostringstream body;
+    std::set<std::string> params;
+    for (int si = 0; si < pv.n_steps; ++si)
+        for (size_t c = 0; c < terms[size_t(si)].size(); ++c) {
+            if (!live[size_t(si)][c]) continue;
+            const auto h = has[size_t(si)][c];
+            const bool final_step = si + 1 == pv.n_steps;
+            const std::string base = "t" + std::to_string(si) + "_" + std::to_string(c);
+            const std::string vr = base + "r", vi = base + "i";
+            if (h.first || final_step) body << "  " << w.real() << " " << vr << " = 0;\n";
+            if (h.second || final_step) body << "  " << w.real() << " " << vi << " = 0;\n";
+            for (const Term& t : terms[size_t(si)][c]) {
+                for (const Val* v : {&t.a, &t.b})
+                    for (const Part* p : {&v->re, &v->im})
+                        if (!p->lit && p->name[0] == 'p') params.insert(p->name.substr(0, p->name.size() - 1));
+                const std::string f = w.fma();
+                if (t.f[0]) body << "  " << vr << " = " << f << "(" << w.str(t.a.re) << ", " << w.str(t.b.re) << ", " << vr << ");\n";
+                if (t.f[1]) body << "  " << vr << " = " << f << "(" << w.neg(t.a.im) << ", " << w.str(t.b.im) << ", " << vr << ");\n";
+                if (t.f[2]) body << "  " << vi << " = " << f << "(" << w.str(t.a.re) << ", " << w.str(t.b.im) << ", " << vi << ");\n";
+                if (t.f[3]) body << "  " << vi << " = " << f << "(" << w.str(t.a.im) << ", " << w.str(t.b.re) << ", " << vi << ");\n";
+                out->fmas += int64_t(t.f[0]) + t.f[1] + t.f[2] + t.f[3];
+            }
+            if (final_step) body << "  out[0] = double(" << vr << ");\n  out[1] = double(" << vi << ");\n";
+        }
+    std::ostringstream os;
+    os << "static __device__ __attribute__((always_inline)) void prog" << id << "(const " << w.real()
+       << "* __restrict__ P, double* __restrict__ out) {\n";
+    for (const std::string& p : params) {
+        const std::string e = p.substr(1);
+        os << "  const " << w.real() << " " << p << "r = P[2 * " << e << "], " << p << "i = P[2 * " << e << " + 1];\n";
+    }
+    os << body.str() << "}\n\n";
+    out->text = os.str();
+    return true;
+}
+
+std::string generate(const std::vector<ProgView>& progs, bool c128, const double* pool, int64_t pool_elems,
+                     std::vector<int64_t>* fmas, std::string* err) {
+    std::ostringstream os;
+    os << "// specialised executor: " << progs.size() << " programs, " << (c128 ? "complex128" : "complex64 4M")
+       << "\n\n";
+    if (fmas) fmas->clear();
+    int64_t total = 0;
+    for (size_t i = 0; i < progs.size(); ++i) {
+        Program p;
+        if (!emit_program(int(i), progs[i], c128, pool, pool_elems, &p, err)) return std::string();
+        total += p.fmas;
+        if (total > kMaxSpecialisedFmas) {
+            *err = "programs too large to specialise (more than " + std::to_string(kMaxSpecialisedFmas) +
+                   " FMAs of straight-line code); the table executor runs them";
+            return std::string();
+        }
+        os << p.text;
+        if (fmas) fmas->push_back(p.fmas);
+    }
+    const char* real = c128 ? "double" : "float";
+    // Instance records: int32 program, int32 pad, int64 parameter base (16 bytes)
+    os << "extern \"C\" __global__ __launch_bounds__(64) void qt_batch_specialised(const " << real
+       << "* __restrict__ pool, const long long* __restrict__ inst, const int* __restrict__ chunks, int n_chunks,\n"
+       << "    double* __restrict__ out) {\n"
+       // workgroups go round-robin to the 8 XCDs: XCD x takes the contiguous chunk range
+       // [x per, (x + 1) per), so the waves resident on a CU run few distinct programs
+       // and share its instruction cache
+       << "  const int per = int(gridDim.x) / 8;\n"
+       << "  const int ci = int(blockIdx.x % 8u) * per + int(blockIdx.x / 8u);\n"
+       << "  if (ci >= n_chunks) return;\n"
+       << "  const int first = chunks[2 * ci], count = chunks[2 * ci + 1];\n"
+       << "  if (int(threadIdx.x) >= count) return;\n"
+       << "  const int i = first + int(threadIdx.x);\n"
+       << "  const int prog = __builtin_amdgcn_readfirstlane(reinterpret_cast<const int*>(inst)[4 * first]);\n"
+       << "  const " << real << "* P = pool + 2 * inst[2 * i + 1];\n"
+       << "  double* o = out + 2 * i;\n"
+       << "  switch (prog) {\n";
+    for (size_t i = 0; i < progs.size(); ++i) os << "    case " << i << ": prog" << i << "(P, o); break;\n";
+    os << "    default: break;\n  }\n}\n";
+    return os.str();
+}
+
+std::vector<ProgView> batch_views(const qt_batch* b) {
+    std::vector<ProgView> v;
+    for (const auto& h : b->h_hdr) {
+        ProgView pv;
+        pv.steps = b->h_steps.data() + h.step_begin;
+        pv.n_steps = h.n_steps;
+        pv.tabs = b->h_tabs.data();
+        pv.refs = b->h_refs.data() + h.leaf_begin;
+        pv.n_leaves = h.n_leaves;
+        pv.const_base = h.const_base;
+        v.push_back(pv);
+    }
+    return v;
+}
+
+}  // namespace
+
+// Chunks of up to 64 consecutive instances of one program (one lane each).
+qt_status specialised_chunks(qt_batch* b) {
+    std::vector<int2> ch;
+    for (int i = 0; i < b->n_inst;) {
+        int j = i + 1;
+        while (j < b->n_inst && j - i < 64 && b->h_prog[size_t(j)] == b->h_prog[size_t(i)]) ++j;
+        ch.push_back(int2{i, j - i});
+        i = j;
+    }
+    b->n_schunks = int(ch.size());
+    if (ch.empty()) return QT_OK;
+    if (ch.size() > b->cap_schunks) {
+        if (b->d_schunks) (void)hipFree(b->d_schunks);
+        b->d_schunks = nullptr;
+        b->cap_schunks = 0;
+        qt_status s = check_hip(b->ctx, hipMalloc(reinterpret_cast<void**>(&b->d_schunks), ch.size() * sizeof(int2)),
+                                "specialised chunks");
+        if (s != QT_OK) return s;
+        b->cap_schunks = ch.size();
+    }
+    return check_hip(b->ctx, hipMemcpy(b->d_schunks, ch.data(), ch.size() * sizeof(int2), hipMemcpyHostToDevice),
+                     "specialised chunks");
+}
+
+void drop_specialised(qt_batch* b) {
+    if (b->module) (void)hipModuleUnload(b->module);
+    b->module = nullptr;
+    b->kernel = nullptr;
+    b->specialised = false;
+    b->spec_consts.clear();
+}
+
+bool specialised_consts_change(const qt_batch* b, int64_t offset, const double* values, int64_t n) {
+    const int64_t n_const = int64_t(b->spec_consts.size() / 2);
+    const bool c128 = b->precision == QT_PLAIN_C128;
+    for (int64_t e = std::max<int64_t>(offset, 0); e < std::min(offset + n, n_const); ++e)
+        for (int t = 0; t < 2; ++t) {
+            const double v = values[2 * (e - offset) + t];
+            if ((c128 ? v : double(float(v))) != b->spec_consts[size_t(2 * e + t)]) return true;
+        }
+    return false;
+}
+
+qt_status launch_specialised(qt_batch* b) {
+    Ctx* c = b->ctx;
+    if (b->needs_tout) return set_error(c, QT_INVALID_INPUT, "batch: specialised programs end in scalars");
+    if (b->n_schunks == 0) return QT_OK;
+    const void* pool = b->d_pool;
+    const void* inst = b->d_inst;
+    const void* chunks = b->d_schunks;
+    int n_chunks = b->n_schunks;
+    void* out = b->d_out;
+    void* params[] = {&pool, &inst, &chunks, &n_chunks, &out};
+    const unsigned grid = unsigned((n_chunks + 7) / 8 * 8);  // a multiple of the XCD count
+    return check_hip(c, hipModuleLaunchKernel(b->kernel, grid, 1, 1, 64, 1, 1, 0, c->stream, params, nullptr),
+                     "specialised kernel");
+}
+
+}  // namespace qt
+
+using qt::check_hip;
+using qt::set_error;
+
+extern "C" {
+
+qt_status qt_program_source(int n, qt_program* const* progs, const int64_t* const_base, const double* pool,
+                            int64_t pool_elems, int precision, char* buf, size_t cap, size_t* len) {
+    if (n < 1 || !progs || !const_base || (pool_elems > 0 && !pool) || pool_elems < 0 || !len ||
+        (precision != QT_C64_4M && precision != QT_PLAIN_C128))
+        return QT_INVALID_INPUT;
+    std::vector<qt::ProgView> views;
+    for (int i = 0; i < n; ++i) {
+        if (!progs[i]) return QT_INVALID_INPUT;
+        qt::ProgView pv;
+        pv.steps = progs[i]->steps.data();
+        pv.n_steps = int(progs[i]->steps.size());
+        pv.tabs = progs[i]->tables.data();
+        pv.refs = progs[i]->leaf_refs.data();
+        pv.n_leaves = int(progs[i]->leaf_refs.size());
+        pv.const_base = const_base[i];
+        views.push_back(pv);
+    }
+    std::string err;
+    const std::string src = qt::generate(views, precision == QT_PLAIN_C128, pool, pool_elems, nullptr, &err);
+    if (src.empty()) return QT_INVALID_INPUT;
+    *len = src.size();
+    if (buf && cap > 0) {
+        const size_t m = std::min(cap - 1, src.size());
+        std::memcpy(buf, src.data(), m);
+        buf[m] = '\0';
+    }
+    return QT_OK;
+}
+
+qt_status qt_batch_compile(qt_batch* b, float* compile_ms) {
+    if (!b) return QT_INVALID_INPUT;
+    auto* ctx = b->ctx;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    if (b->precision != QT_C64_4M && b->precision != QT_PLAIN_C128)
+        return set_error(ctx, QT_INVALID_INPUT, "qt_batch_compile: only the complex64 4M and complex128 forms");
+    if (b->n_programs == 0 || b->pool_elems == 0)
+        return set_error(ctx, QT_INVALID_INPUT, "qt_batch_compile: set programs and leaves first");
+    for (int32_t oe : b->prog_out_elems)
+        if (oe != 1) return set_error(ctx, QT_INVALID_INPUT, "qt_batch_compile: programs must end in scalars");
+    const auto t0 = std::chrono::steady_clock::now();
+    // constant leaves as FP64 (complex64 pools convert exactly)
+    const int64_t n_const = std::min(b->pool_elems, b->max_leaf_ref_end);
+    std::vector<double> pool(size_t(2 * std::max<int64_t>(n_const, 1)), 0.0);
+    qt_status s = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    if (s == QT_OK && n_const > 0) {
+        if (b->precision == QT_PLAIN_C128) {
+            s = check_hip(ctx, hipMemcpy(pool.data(), b->d_pool, size_t(n_const) * 16, hipMemcpyDeviceToHost),
+                          "pool read");
+        } else {
+            std::vector<float> f(size_t(2 * n_const));
+            s = check_hip(ctx, hipMemcpy(f.data(), b->d_pool, size_t(n_const) * 8, hipMemcpyDeviceToHost), "pool read");
+            for (size_t i = 0; i < f.size(); ++i) pool[i] = double(f[i]);
+        }
+    }
+    if (s != QT_OK) return s;
+    std::string err;
+    std::vector<int64_t> fmas;
+    const std::string src =
+        qt::generate(qt::batch_views(b), b->precision == QT_PLAIN_C128, pool.data(), n_const, &fmas, &err);
+    if (src.empty()) return set_error(ctx, QT_INVALID_INPUT, "qt_batch_compile: " + err);
+    hipDeviceProp_t prop;
+    s = check_hip(ctx, hipGetDeviceProperties(&prop, ctx->device), "device properties");
+    if (s != QT_OK) return s;
+    hiprtcProgram prog = nullptr;
+    if (hiprtcCreateProgram(&prog, src.c_str(), "qt_batch_specialised.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS)
+        return set_error(ctx, QT_DEVICE, "qt_batch_compile: hiprtcCreateProgram failed");
+    const std::string arch = std::string("--offload-arch=") + prop.gcnArchName;
+    // packed FP32 instructions stay off, as for the library's own kernels
+    const char* opts[] = {arch.c_str(), "-O3", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+    const hiprtcResult rc = hiprtcCompileProgram(prog, 6, opts);
+    if (rc != HIPRTC_SUCCESS) {
+        size_t n = 0;
+        (void)hiprtcGetProgramLogSize(prog, &n);
+        std::string log(n + 1, '\0');
+        (void)hiprtcGetProgramLog(prog, &log[0]);
+        (void)hiprtcDestroyProgram(&prog);
+        return set_error(ctx, QT_DEVICE, std::string("qt_batch_compile: ") + hiprtcGetErrorString(rc) + ": " +
+                                             log.substr(0, 2000));
+    }
+    size_t code_size = 0;
+    (void)hiprtcGetCodeSize(prog, &code_size);
+    std::vector<char> code(code_size);
+    (void)hiprtcGetCode(prog, code.data());
+    (void)hiprtcDestroyProgram(&prog);
+    hipModule_t mod = nullptr;
+    hipFunction_t fn = nullptr;
+    s = check_hip(ctx, hipModuleLoadData(&mod, code.data()), "module load");
+    if (s == QT_OK) s = check_hip(ctx, hipModuleGetFunction(&fn, mod, "qt_batch_specialised"), "module function");
+    if (s != QT_OK) {
+        if (mod) (void)hipModuleUnload(mod);
+        return s;
+    }
+    qt::drop_specialised(b);
+    b->module = mod;
+    b->kernel = fn;
+    b->spec_consts.assign(pool.begin(), pool.begin() + 2 * n_const);
+    b->spec_fmas = fmas;
+    s = qt::specialised_chunks(b);
+    if (s != QT_OK) {
+        qt::drop_specialised(b);
+        return s;
+    }
+    b->specialised = true;
+    // a captured graph holds the table executor
+    if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->graph_exec = nullptr;
+    b->graph = nullptr;
+    if (compile_ms)
+        *compile_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return QT_OK;
+}
+
+qt_status qt_batch_specialised_info(const qt_batch* b, int* active, int64_t* fmas_per_launch) {
+    if (!b) return QT_INVALID_INPUT;
+    if (active) *active = b->specialised ? 1 : 0;
+    if (fmas_per_launch) {
+        int64_t n = 0;
+        if (b->specialised)
+            for (int i = 0; i < b->n_inst; ++i) n += b->spec_fmas[size_t(b->h_prog[size_t(i)])];
+        *fmas_per_launch = n;
+    }
+    return QT_OK;
+}
+
+}  // extern "C"

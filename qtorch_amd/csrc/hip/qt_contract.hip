@@ -822,20 +822,41 @@ __global__ __launch_bounds__(kThreads) void k_gemm_c64_big(const float2* __restr
             }
 }
 
-__global__ void k_absmax_c64(const float2* __restrict__ x, int64_t n, uint32_t* __restrict__ out) {
+// Largest |re|, |im| of a complex64 tensor in two launches and no atomics: each
+// workgroup of k_absmax_c64 writes the maximum of its grid-stride share to parts[];
+// every workgroup of k_rescale_c64 reduces the (at most kAbsParts) partial maxima
+// itself, then scales its share by the power of two that brings the maximum into
+// [0.5, 1).
+constexpr int kAbsParts = 1024;
+
+__device__ __forceinline__ float block_max(float m, float* lds) {
+    for (int d = 32; d >= 1; d >>= 1) m = fmaxf(m, __shfl_down(m, d, 64));
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = m;
+    __syncthreads();
+    m = lds[0];
+    for (int w = 1; w < kThreads / 64; ++w) m = fmaxf(m, lds[w]);
+    return m;
+}
+
+__global__ __launch_bounds__(kThreads) void k_absmax_c64(const float2* __restrict__ x, int64_t n,
+                                                         float* __restrict__ parts) {
+    __shared__ float lds[kThreads / 64];
     float m = 0.0f;
     const int64_t stride = int64_t(gridDim.x) * blockDim.x;
     for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
         m = fmaxf(m, fmaxf(fabsf(x[i].x), fabsf(x[i].y)));
-    for (int d = 32; d >= 1; d >>= 1) m = fmaxf(m, __shfl_down(m, d, 64));
-    // non-negative floats order like their bit patterns
-    if ((threadIdx.x & 63) == 0 && m > 0.0f) atomicMax(out, __float_as_uint(m));
+    m = block_max(m, lds);
+    if (threadIdx.x == 0) parts[blockIdx.x] = m;
 }
 
-__global__ void k_rescale_c64(float2* __restrict__ x, int64_t n, const uint32_t* __restrict__ absmax,
-                              const int32_t* __restrict__ ea, const int32_t* __restrict__ eb,
-                              int32_t* __restrict__ ec) {
-    const float m = __uint_as_float(*absmax);
+__global__ __launch_bounds__(kThreads) void k_rescale_c64(float2* __restrict__ x, int64_t n,
+                                                          const float* __restrict__ parts, int n_parts,
+                                                          const int32_t* __restrict__ ea,
+                                                          const int32_t* __restrict__ eb, int32_t* __restrict__ ec) {
+    __shared__ float lds[kThreads / 64];
+    float m = 0.0f;
+    for (int j = threadIdx.x; j < n_parts; j += kThreads) m = fmaxf(m, parts[j]);
+    m = block_max(m, lds);
     int e = 0;
     if (m > 0.0f && isfinite(m)) (void)frexpf(m, &e);  // m = f 2^e, f in [0.5, 1)
     const float s = ldexpf(1.0f, -e);                 // exact power of two
@@ -867,7 +888,6 @@ __global__ __launch_bounds__(kThreads) void k_rescale_small_c64(float2* __restri
     if (threadIdx.x == 0) *ec = (ea ? *ea : 0) + (eb ? *eb : 0) + e;
 }
 
-__global__ void k_set_u32(uint32_t* p, uint32_t v) { *p = v; }
 
 template <class V>
 __global__ void k_accumulate(const V* __restrict__ t, const int32_t* __restrict__ e, double2* __restrict__ acc) {
@@ -893,7 +913,8 @@ qt_status rescale_c64(Ctx* c, const Tensor* A, const Tensor* B, Tensor* C) {
     if (!c->d_exp) {
         c->exp_cap = 1 << 20;
         qt_status s = check_hip(c, hipMalloc(reinterpret_cast<void**>(&c->d_exp), size_t(c->exp_cap) * 4), "exponents");
-        if (s == QT_OK) s = check_hip(c, hipMalloc(reinterpret_cast<void**>(&c->d_absmax), 4), "absmax");
+        if (s == QT_OK)
+            s = check_hip(c, hipMalloc(reinterpret_cast<void**>(&c->d_absmax), kAbsParts * sizeof(float)), "absmax");
         if (s != QT_OK) return s;
         for (int32_t i = c->exp_cap - 1; i >= 0; --i) c->exp_free.push_back(i);
     }
@@ -911,11 +932,10 @@ qt_status rescale_c64(Ctx* c, const Tensor* A, const Tensor* B, Tensor* C) {
                            c->d_exp + C->exp_slot);
         return check_hip(c, hipGetLastError(), "k_rescale_small_c64");
     }
-    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, c->stream, c->d_absmax, 0u);
-    const unsigned grid = grid_for(n, kThreads, 4096);
-    hipLaunchKernelGGL(k_absmax_c64, dim3(grid), dim3(kThreads), 0, c->stream, x, n, c->d_absmax);
-    hipLaunchKernelGGL(k_rescale_c64, dim3(grid), dim3(kThreads), 0, c->stream, x, n, c->d_absmax, ea, eb,
-                       c->d_exp + C->exp_slot);
+    const unsigned parts = grid_for(n, kThreads, kAbsParts);
+    hipLaunchKernelGGL(k_absmax_c64, dim3(parts), dim3(kThreads), 0, c->stream, x, n, c->d_absmax);
+    hipLaunchKernelGGL(k_rescale_c64, dim3(grid_for(n, kThreads, 4096)), dim3(kThreads), 0, c->stream, x, n,
+                       c->d_absmax, int(parts), ea, eb, c->d_exp + C->exp_slot);
     return check_hip(c, hipGetLastError(), "k_rescale_c64");
 }
 

@@ -49,7 +49,7 @@ struct Ctx {
     int32_t* d_exp = nullptr;
     std::vector<int32_t> exp_free;
     int32_t exp_cap = 0;
-    uint32_t* d_absmax = nullptr;  // scratch of one rescale
+    float* d_absmax = nullptr;  // per-workgroup maxima of one rescale
 };
 
 struct Tensor {

@@ -332,6 +332,32 @@ int qth_qaoa_batch_destroy(qth_qaoa_batch* h) {
     return QTH_OK;
 }
 
+int qth_qaoa_source(const char* dgf, int p, int ref_map, int order, uint64_t seed, int precision, int first_edge,
+                    int n_edges, char* buf, size_t cap, size_t* len) {
+    if (!dgf || p < 1 || !len || first_edge < 0 || n_edges < 1) return QTH_INVALID_FUNCTION_INPUT;
+    return Guard([&] {
+        const ExtraData data = ExtraData::FromText(p, dgf, ref_map != 0);
+        LeafPool pool;
+        CompiledQaoa c = CompileQaoa(data, p, order, seed, pool);
+        if (first_edge + n_edges > int(c.progs.size())) throw InvalidFunctionInput();
+        const std::vector<int64_t> base(size_t(n_edges), 0);
+        const auto& d = pool.Data();
+        CheckQt(nullptr,
+                qt_program_source(n_edges, c.progs.data() + first_edge, base.data(),
+                                  reinterpret_cast<const double*>(d.data()), int64_t(d.size()), precision, buf, cap,
+                                  len),
+                "qt_program_source");
+    });
+}
+
+int qth_qaoa_param_block(int p, const double* bg, double* out) {
+    if (p < 1 || !bg || !out) return QTH_INVALID_FUNCTION_INPUT;
+    return Guard([&] {
+        const auto b = QaoaParamBlock(p, bg);
+        std::memcpy(out, b.data(), b.size() * sizeof(b[0]));
+    });
+}
+
 int qth_qaoa_optimize(qt_ctx* ctx, const char* dgf, int p, int precision, int n_random, int max_evals,
                       uint64_t seed, double* angles, double* energy, int* evaluations) {
     if (!ctx || !dgf || !angles) return QTH_INVALID_FUNCTION_INPUT;
