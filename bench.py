@@ -414,11 +414,20 @@ def main():
     compile_ms = b.compile() if specialised else None
     if specialised:
         lanes, wgs = 1, 0  # one lane per network, one wavefront per 64 networks
+    # direct launches or a captured HIP graph, whichever runs a step faster here
+    b.launch()
+    ctx.sync()
+    direct_ms = b.time(20)
     try:
         b.capture()
-        graph = True
-    except qa.DeviceFailure:  # launches stay direct
-        graph = False
+        b.launch()
+        ctx.sync()
+        graph_ms = b.time(20)
+    except qa.DeviceFailure:
+        graph_ms = None
+    graph = graph_ms is not None and graph_ms < direct_ms
+    if graph_ms is not None and not graph:
+        b.uncapture()
     for _ in range(args.warmup):
         b.launch()
     ctx.sync()
@@ -469,11 +478,21 @@ def main():
                 e_f, zz_f = bf.results()
             l_f, w_f, _ = bf.autotune()
             ms_f = bf.time(5)
+            spec_ms = None
+            if prec == qa.QT_PLAIN_C128:  # complex128 also has generated code
+                bf.compile()
+                bf.launch()
+                ctx.sync()
+                spec_ms = bf.time(10)
+                bf.config(l_f, w_f)
             rel_f = np.abs(zz_f - zz64) / np.maximum(np.abs(zz64), 1e-3)
             by_form[name] = {
                 "edge_evals_per_s": ne * args.sets / (ms_f * 1e-3), "kernel_ms": ms_f, "executor": [l_f, w_f],
                 "max_edge_rel_err_vs_c128": float(np.max(rel_f)), "edges_within_1e-5": float(np.mean(rel_f <= 1e-5)),
                 "max_energy_rel_err_vs_c128": float(np.max(np.abs(e_f - e64) / np.abs(e64)))}
+            if spec_ms is not None:
+                by_form[name]["specialised_launch_ms"] = spec_ms
+                by_form[name]["specialised_edge_evals_per_s"] = ne * args.sets / (spec_ms * 1e-3)
             if bf is not b64:
                 bf.close()
     b64.close()
@@ -520,6 +539,7 @@ def main():
                    "global_scratch": info["global_scratch"] and not specialised,
                    "staged_leaves": info["staged_leaves"] and not specialised},
         "compile_ms": compile_ms, "table_executor_ms_per_launch": tables_ms, "hip_graph": graph,
+        "launch_ms_direct": direct_ms, "launch_ms_graph": graph_ms,
         "p50_step_ms": 1e3 * float(np.percentile(lat, 50)), "p99_step_ms": 1e3 * float(np.percentile(lat, 99)),
         "accuracy": {"max_edge_rel_err_vs_c128": edge_err, "edges_within_1e-5": within,
                      "max_energy_rel_err_vs_c128": energy_err,

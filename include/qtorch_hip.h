@@ -199,6 +199,8 @@ qt_status qt_batch_poll(qt_batch* b, int* done);
 /* Captures the launch sequence (counter reset, persistent kernel, group sums) into a
  * HIP graph; later launches replay it. */
 qt_status qt_batch_capture(qt_batch* b);
+/* Drops the captured graph: launches are direct again. */
+qt_status qt_batch_uncapture(qt_batch* b);
 /* Synchronises and copies per-instance values (2 doubles each) and group sums. */
 qt_status qt_batch_results(qt_batch* b, double* values, double* group_sums);
 /* Mean time of `reps` launches, by HIP events on the context's stream. */
@@ -240,6 +242,11 @@ qt_status qt_batch_specialised_info(const qt_batch* b, int* active, int64_t* fma
  * to buf (may be NULL) and the full length to *len. */
 qt_status qt_program_source(int n, qt_program* const* progs, const int64_t* const_base, const double* pool,
                             int64_t pool_elems, int precision, char* buf, size_t cap, size_t* len);
+/* Host only: compiles such source for an architecture ("gfx950", or a device's full
+ * gcnArchName) with the options qt_batch_compile uses (-O3, packed FP32 instructions
+ * off); *len receives the code object's size, which is copied to code when cap is
+ * large enough (QT_DEVICE if compilation fails). */
+qt_status qt_program_compile(const char* source, const char* arch, void* code, size_t cap, size_t* len);
 
 #ifdef __cplusplus
 }

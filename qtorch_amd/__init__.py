@@ -11,7 +11,7 @@ from ._native import (  # noqa: F401
     QaoaBatch, DeviceFailure, EngineError, InvalidContractionMethod, InvalidFile,
     InvalidFileFormat, InvalidFunctionInput, InvalidUserContractionSequence, Network, QbbFailure, Tensor,
     contract_pair, declared_symbols, maxcut_final_string, qaoa_optimize, qaoa_edge_text, qaoa_energy_plain, qaoa_num_edges,
-    qaoa_param_block, qaoa_source,
+    qaoa_param_block, qaoa_source, program_compile,
     ROUTE_EAGER, ROUTE_MEDIUM, ROUTE_SMALL, SERVE_AMPLITUDE, SERVE_QAOA_ENERGY, Server, serve_plan,
 )
 

@@ -75,6 +75,8 @@ _HIP_SIGS = {
     "qt_batch_set_outputs": (_c.c_int, [_vp, _c.c_int, _c.POINTER(_vp)]),
     "qt_batch_compile": (_c.c_int, [_vp, _c.POINTER(_c.c_float)]),
     "qt_batch_time_kernel": (_c.c_int, [_vp, _c.c_int, _c.POINTER(_c.c_float)]),
+    "qt_batch_uncapture": (_c.c_int, [_vp]),
+    "qt_program_compile": (_c.c_int, [_c.c_char_p, _c.c_char_p, _vp, _c.c_size_t, _c.POINTER(_c.c_size_t)]),
     "qt_batch_specialised_info": (_c.c_int, [_vp, _c.POINTER(_c.c_int), _c.POINTER(_c.c_int64)]),
     "qt_program_source": (_c.c_int, [_c.c_int, _c.POINTER(_vp), _i64p, _dp, _c.c_int64, _c.c_int, _c.c_char_p,
                                      _c.c_size_t, _c.POINTER(_c.c_size_t)]),
@@ -493,6 +495,9 @@ class _DeviceBatch:
     def capture(self) -> None:
         self.ctx.check(hip.qt_batch_capture(self.dev), "qt_batch_capture")
 
+    def uncapture(self) -> None:
+        self.ctx.check(hip.qt_batch_uncapture(self.dev), "qt_batch_uncapture")
+
     def config(self, lanes: int, wgs_per_cu: int, scratch: int = 0) -> None:
         """Executor shape; scratch: 0 automatic, 1 LDS, 2 device memory."""
         self.ctx.check(hip.qt_batch_config_ex(self.dev, lanes, wgs_per_cu, scratch), "qt_batch_config_ex")
@@ -587,6 +592,19 @@ def qaoa_source(dgf: str, p: int, first_edge: int, n_edges: int, precision: int 
     buf = ctypes.create_string_buffer(n.value + 1)
     _host_check(host.qth_qaoa_source(*args, buf, n.value + 1, ctypes.byref(n)))
     return buf.value.decode()
+
+
+def program_compile(source: str, arch: str = "gfx950") -> bytes:
+    """Code object of generated source for an architecture (host only)."""
+    n = ctypes.c_size_t()
+    rc = hip.qt_program_compile(source.encode(), arch.encode(), None, 0, ctypes.byref(n))
+    if rc != QT_OK:
+        raise DeviceFailure(f"qt_program_compile: status {rc}")
+    buf = ctypes.create_string_buffer(n.value)
+    rc = hip.qt_program_compile(source.encode(), arch.encode(), buf, n.value, ctypes.byref(n))
+    if rc != QT_OK:
+        raise DeviceFailure(f"qt_program_compile: status {rc}")
+    return buf.raw[:n.value]
 
 
 def qaoa_param_block(p: int, betas_gammas: Sequence[float]) -> np.ndarray:

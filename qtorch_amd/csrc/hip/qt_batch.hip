@@ -1219,6 +1219,18 @@ qt_status qt_batch_capture(qt_batch* b) {
     return check_hip(ctx, hipGraphInstantiate(&b->graph_exec, graph, nullptr, nullptr, 0), "graph instantiate");
 }
 
+qt_status qt_batch_uncapture(qt_batch* b) {
+    if (!b) return QT_INVALID_INPUT;
+    std::lock_guard<std::recursive_mutex> g(b->ctx->mu);
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->stream);  // a replay in flight holds the graph
+    if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->graph_exec = nullptr;
+    b->graph = nullptr;
+    return QT_OK;
+}
+
 qt_status qt_batch_results(qt_batch* b, double* values, double* group_sums) {
     if (!b) return QT_INVALID_INPUT;
     auto* ctx = b->ctx;

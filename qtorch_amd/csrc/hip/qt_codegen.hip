@@ -27,6 +27,7 @@
 #include <cstring>
 #include <map>
 #include <set>
+#include <unordered_map>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -195,30 +196,51 @@ bool emit_program(int id, const ProgView& pv, bool c128, const double* pool, int
                 for (const Val* v : {&t.a, &t.b})
                     if (v->step >= 0) live[size_t(v->step)][size_t(v->elem)] = 1;
         }
-    // pass 3: code
+    // pass 3: code.  A chain of FMAs identical to one already emitted (same operands
+    // in the same order) is the same value: its variable is reused.
     std::ostringstream body;
     std::set<std::string> params;
+    std::unordered_map<std::string, std::string> alias, chain_var;
+    auto name = [&](const Part& p) -> std::string {
+        if (p.lit) return w.lit(p.v);
+        if (p.name[0] == 'p') params.insert(p.name.substr(0, p.name.size() - 1));
+        auto it = alias.find(p.name);
+        return it == alias.end() ? p.name : it->second;
+    };
+    auto neg = [&](const Part& p) -> std::string { return p.lit ? w.lit(-p.v) : "(-" + name(p) + ")"; };
+    auto emit = [&](const std::string& var, const std::vector<std::pair<std::string, std::string>>& ops) {
+        std::string key;
+        for (const auto& o : ops) key += o.first + "," + o.second + ";";
+        auto it = chain_var.find(key);
+        if (it != chain_var.end()) {
+            alias[var] = it->second;
+            return;
+        }
+        chain_var.emplace(key, var);
+        body << "  " << w.real() << " " << var << " = 0;\n";
+        for (const auto& o : ops)
+            body << "  " << var << " = " << w.fma() << "(" << o.first << ", " << o.second << ", " << var << ");\n";
+        out->fmas += int64_t(ops.size());
+    };
     for (int si = 0; si < pv.n_steps; ++si)
         for (size_t c = 0; c < terms[size_t(si)].size(); ++c) {
             if (!live[size_t(si)][c]) continue;
-            const auto h = has[size_t(si)][c];
-            const bool final_step = si + 1 == pv.n_steps;
             const std::string base = "t" + std::to_string(si) + "_" + std::to_string(c);
             const std::string vr = base + "r", vi = base + "i";
-            if (h.first || final_step) body << "  " << w.real() << " " << vr << " = 0;\n";
-            if (h.second || final_step) body << "  " << w.real() << " " << vi << " = 0;\n";
+            std::vector<std::pair<std::string, std::string>> ore, oim;
             for (const Term& t : terms[size_t(si)][c]) {
-                for (const Val* v : {&t.a, &t.b})
-                    for (const Part* p : {&v->re, &v->im})
-                        if (!p->lit && p->name[0] == 'p') params.insert(p->name.substr(0, p->name.size() - 1));
-                const std::string f = w.fma();
-                if (t.f[0]) body << "  " << vr << " = " << f << "(" << w.str(t.a.re) << ", " << w.str(t.b.re) << ", " << vr << ");\n";
-                if (t.f[1]) body << "  " << vr << " = " << f << "(" << w.neg(t.a.im) << ", " << w.str(t.b.im) << ", " << vr << ");\n";
-                if (t.f[2]) body << "  " << vi << " = " << f << "(" << w.str(t.a.re) << ", " << w.str(t.b.im) << ", " << vi << ");\n";
-                if (t.f[3]) body << "  " << vi << " = " << f << "(" << w.str(t.a.im) << ", " << w.str(t.b.re) << ", " << vi << ");\n";
-                out->fmas += int64_t(t.f[0]) + t.f[1] + t.f[2] + t.f[3];
+                if (t.f[0]) ore.push_back({name(t.a.re), name(t.b.re)});
+                if (t.f[1]) ore.push_back({neg(t.a.im), name(t.b.im)});
+                if (t.f[2]) oim.push_back({name(t.a.re), name(t.b.im)});
+                if (t.f[3]) oim.push_back({name(t.a.im), name(t.b.re)});
             }
-            if (final_step) body << "  out[0] = double(" << vr << ");\n  out[1] = double(" << vi << ");\n";
+            if (!ore.empty()) emit(vr, ore);
+            if (!oim.empty()) emit(vi, oim);
+            if (si + 1 == pv.n_steps) {
+                const std::string zero = c128 ? "0.0" : "0.0f";
+                body << "  out[0] = double(" << (ore.empty() ? zero : name(Part{false, 0, vr})) << ");\n";
+                body << "  out[1] = double(" << (oim.empty() ? zero : name(Part{false, 0, vi})) << ");\n";
+            }
         }
     std::ostringstream os;
     os << "static __device__ __attribute__((always_inline)) void prog" << id << "(const " << w.real()
@@ -287,6 +309,34 @@ std::vector<ProgView> batch_views(const qt_batch* b) {
         v.push_back(pv);
     }
     return v;
+}
+
+// Compiles generated source for `arch` (e.g. "gfx950" or a device's full gcnArchName)
+// with hipRTC; packed FP32 instructions stay off, as for the library's own kernels.
+bool rtc_compile(const std::string& src, const std::string& arch, std::vector<char>* code, std::string* err) {
+    hiprtcProgram prog = nullptr;
+    if (hiprtcCreateProgram(&prog, src.c_str(), "qt_batch_specialised.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS) {
+        *err = "hiprtcCreateProgram failed";
+        return false;
+    }
+    const std::string a = "--offload-arch=" + arch;
+    const char* opts[] = {a.c_str(), "-O3", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+    const hiprtcResult rc = hiprtcCompileProgram(prog, 6, opts);
+    if (rc != HIPRTC_SUCCESS) {
+        size_t n = 0;
+        (void)hiprtcGetProgramLogSize(prog, &n);
+        std::string log(n + 1, '\0');
+        (void)hiprtcGetProgramLog(prog, &log[0]);
+        (void)hiprtcDestroyProgram(&prog);
+        *err = std::string(hiprtcGetErrorString(rc)) + ": " + log.substr(0, 2000);
+        return false;
+    }
+    size_t size = 0;
+    (void)hiprtcGetCodeSize(prog, &size);
+    code->assign(size, 0);
+    (void)hiprtcGetCode(prog, code->data());
+    (void)hiprtcDestroyProgram(&prog);
+    return true;
 }
 
 }  // namespace
@@ -385,6 +435,16 @@ qt_status qt_program_source(int n, qt_program* const* progs, const int64_t* cons
     return QT_OK;
 }
 
+qt_status qt_program_compile(const char* source, const char* arch, void* code, size_t cap, size_t* len) {
+    if (!source || !arch || !len) return QT_INVALID_INPUT;
+    std::vector<char> co;
+    std::string err;
+    if (!qt::rtc_compile(source, arch, &co, &err)) return QT_DEVICE;
+    *len = co.size();
+    if (code && cap >= co.size()) std::memcpy(code, co.data(), co.size());
+    return QT_OK;
+}
+
 qt_status qt_batch_compile(qt_batch* b, float* compile_ms) {
     if (!b) return QT_INVALID_INPUT;
     auto* ctx = b->ctx;
@@ -420,27 +480,9 @@ qt_status qt_batch_compile(qt_batch* b, float* compile_ms) {
     hipDeviceProp_t prop;
     s = check_hip(ctx, hipGetDeviceProperties(&prop, ctx->device), "device properties");
     if (s != QT_OK) return s;
-    hiprtcProgram prog = nullptr;
-    if (hiprtcCreateProgram(&prog, src.c_str(), "qt_batch_specialised.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS)
-        return set_error(ctx, QT_DEVICE, "qt_batch_compile: hiprtcCreateProgram failed");
-    const std::string arch = std::string("--offload-arch=") + prop.gcnArchName;
-    // packed FP32 instructions stay off, as for the library's own kernels
-    const char* opts[] = {arch.c_str(), "-O3", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
-    const hiprtcResult rc = hiprtcCompileProgram(prog, 6, opts);
-    if (rc != HIPRTC_SUCCESS) {
-        size_t n = 0;
-        (void)hiprtcGetProgramLogSize(prog, &n);
-        std::string log(n + 1, '\0');
-        (void)hiprtcGetProgramLog(prog, &log[0]);
-        (void)hiprtcDestroyProgram(&prog);
-        return set_error(ctx, QT_DEVICE, std::string("qt_batch_compile: ") + hiprtcGetErrorString(rc) + ": " +
-                                             log.substr(0, 2000));
-    }
-    size_t code_size = 0;
-    (void)hiprtcGetCodeSize(prog, &code_size);
-    std::vector<char> code(code_size);
-    (void)hiprtcGetCode(prog, code.data());
-    (void)hiprtcDestroyProgram(&prog);
+    std::vector<char> code;
+    if (!qt::rtc_compile(src, prop.gcnArchName, &code, &err))
+        return set_error(ctx, QT_DEVICE, "qt_batch_compile: " + err);
     hipModule_t mod = nullptr;
     hipFunction_t fn = nullptr;
     s = check_hip(ctx, hipModuleLoadData(&mod, code.data()), "module load");

@@ -75,3 +75,25 @@ def test_generated_code_drops_zero_terms():
     src = qa.qaoa_source(dgf, 1, 0, 1, qa.QT_C64_4M)
     fmas = src.count("__builtin_fmaf(")
     assert 0 < fmas < 4 * 2473
+
+
+def test_generated_code_compiles_without_packed_fp32(tmp_path):
+    """The product's own compile path (hipRTC, the options qt_batch_compile uses) turns
+    generated code into gfx950 code with scalar FP32 FMAs only: no packed FP32
+    instructions, and as many FMA instructions as the generator counts (within the
+    compiler's own exact simplifications of multiply-adds by +-1)."""
+    import re
+    import shutil
+    objdump = shutil.which("llvm-objdump") or "/opt/rocm/llvm/bin/llvm-objdump"
+    if not os.path.exists(objdump):
+        pytest.skip("llvm-objdump not available")
+    dgf = open(os.path.join(GOLDEN, "graphs", "3reg_n30_s2024.dgf")).read()
+    src = qa.qaoa_source(dgf, 1, 0, 2, qa.QT_C64_4M)
+    co = tmp_path / "k.co"
+    co.write_bytes(qa.program_compile(src, "gfx950"))
+    asm = subprocess.run([objdump, "-d", str(co)], capture_output=True, text=True, check=True, timeout=120).stdout
+    assert "qt_batch_specialised" in asm
+    assert not re.search(r"\bv_pk_(fma|mul|add)_f32\b", asm)
+    n_fma = len(re.findall(r"\bv_fma(c|mk|ak)?_f32", asm))
+    generated = src.count("__builtin_fmaf(")
+    assert 0.9 * generated <= n_fma <= generated
