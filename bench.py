@@ -355,6 +355,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-forms", action="store_true", help="skip the other forms' accuracy and speed")
     ap.add_argument("--exec", default="", help="table executor shape 'lanes,wgs_per_cu' instead of the autotuner's")
+    ap.add_argument("--no-hermitian", action="store_true",
+                    help="specialised executor without conjugate symmetry (bit-identical to the table executor)")
     ap.add_argument("--executor", default="specialised", choices=["specialised", "tables"],
                     help="qaoa_p1: generated straight-line code (complex64 4M and complex128) or the table executor")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -411,7 +413,9 @@ def main():
     ctx.sync()
     tables_ms = b.time(max(5, args.steps // 5))
     specialised = args.executor == "specialised" and args.form in ("c64_4m", "c128")
-    compile_ms = b.compile() if specialised else None
+    # the light cones' leaves are gate superoperators U (x) U* and Hermitian measurement
+    # vectors: conjugate-symmetric, so the generated code computes half of each tensor
+    compile_ms = b.compile(hermitian=not args.no_hermitian) if specialised else None
     if specialised:
         lanes, wgs = 1, 0  # one lane per network, one wavefront per 64 networks
     # direct launches or a captured HIP graph, whichever runs a step faster here
@@ -480,7 +484,7 @@ def main():
             ms_f = bf.time(5)
             spec_ms = None
             if prec == qa.QT_PLAIN_C128:  # complex128 also has generated code
-                bf.compile()
+                bf.compile(hermitian=not args.no_hermitian)
                 bf.launch()
                 ctx.sync()
                 spec_ms = bf.time(10)
@@ -535,6 +539,7 @@ def main():
         "config": {"workload": "qaoa_p1_maxcut_3regular_n30_edges45", "angle_sets_per_step": args.sets,
                    "edge_networks_per_step": ne * args.sets, "form": args.form,
                    "order": "greedy", "executor": "specialised" if specialised else "tables",
+                   "conjugate_symmetry": bool(specialised and not args.no_hermitian),
                    "executor_lanes": lanes, "wgs_per_cu": wgs,
                    "global_scratch": info["global_scratch"] and not specialised,
                    "staged_leaves": info["staged_leaves"] and not specialised},

@@ -225,12 +225,21 @@ qt_status qt_batch_destroy(qt_batch* b);
  * the device at run time (hipRTC) and makes qt_batch_launch run it.  Only the
  * complex64 4M and complex128 forms, only programs that end in scalars, and at most
  * 250,000 FMAs of generated code in all (QT_INVALID_INPUT otherwise; the table
- * executor keeps running).  The
- * constants are read from the pool now: a later pool write that changes them, new
- * programs, or an explicit qt_batch_config / qt_batch_config_ex drop the generated
- * kernel (the table executor runs again).  *compile_ms (if non-NULL): generation,
- * compilation and loading time. */
-qt_status qt_batch_compile(qt_batch* b, float* compile_ms);
+ * executor keeps running).  The constants are read from the pool now: a later pool
+ * write that changes them, new programs, or an explicit qt_batch_config /
+ * qt_batch_config_ex drop the generated kernel (the table executor runs again).
+ * flags: 0, or QT_SPECIALISE_HERMITIAN for networks in qTorch's density-matrix picture
+ * (two index bits per axis, d = 2 ket + bra) whose every leaf satisfies
+ * T(x') = conj T(x), x' being x with ket and bra swapped on every axis (gate
+ * superoperators U (x) U*, Hermitian measurement vectors): every intermediate then has
+ * the same symmetry, so of each pair x, x' only one element is computed and
+ * self-mirror elements are real (about half the FMAs).  The constant leaves and the
+ * current instances' parameter blocks are checked (QT_INVALID_INPUT if one is not
+ * symmetric); parameter blocks written later must keep the symmetry.  Results then
+ * differ from the table executors' by rounding only.  *compile_ms (if non-NULL):
+ * generation, compilation and loading time. */
+enum { QT_SPECIALISE_HERMITIAN = 1 };
+qt_status qt_batch_compile(qt_batch* b, int flags, float* compile_ms);
 /* *active = 1 while the generated kernel is the one qt_batch_launch runs;
  * *fmas_per_launch: FMAs its code executes for the current instances (each a real
  * multiply-add of 2 FLOPs; terms with exact-zero constant factors and values that
@@ -241,7 +250,7 @@ qt_status qt_batch_specialised_info(const qt_batch* b, int* active, int64_t* fma
  * (interleaved FP64, pool_elems elements).  Writes at most cap bytes (NUL-terminated)
  * to buf (may be NULL) and the full length to *len. */
 qt_status qt_program_source(int n, qt_program* const* progs, const int64_t* const_base, const double* pool,
-                            int64_t pool_elems, int precision, char* buf, size_t cap, size_t* len);
+                            int64_t pool_elems, int precision, int flags, char* buf, size_t cap, size_t* len);
 /* Host only: compiles such source for an architecture ("gfx950", or a device's full
  * gcnArchName) with the options qt_batch_compile uses (-O3, packed FP32 instructions
  * off); *len receives the code object's size, which is copied to code when cap is

@@ -110,9 +110,10 @@ int64_t qth_qaoa_batch_cmacs(qth_qaoa_batch* b); /* complex MACs of one energy e
 int qth_qaoa_batch_destroy(qth_qaoa_batch* b);
 /* Host only: the specialised-executor source (qt_batch_compile) of the edge programs
  * first_edge .. first_edge + n_edges - 1, as program 0, 1, ... (precision
- * QT_C64_4M or QT_PLAIN_C128); *len receives the full length. */
-int qth_qaoa_source(const char* dgf, int p, int ref_map, int order, uint64_t seed, int precision, int first_edge,
-                    int n_edges, char* buf, size_t cap, size_t* len);
+ * QT_C64_4M or QT_PLAIN_C128; flags as qt_batch_compile's); *len receives the full
+ * length. */
+int qth_qaoa_source(const char* dgf, int p, int ref_map, int order, uint64_t seed, int precision, int flags,
+                    int first_edge, int n_edges, char* buf, size_t cap, size_t* len);
 /* The 32 p complex parameter elements (64 p doubles, interleaved) of one angle set
  * [betas..., gammas...]: Rz(-gamma_i) at element 32 i, Rx(2 beta_i) at 32 i + 16. */
 int qth_qaoa_param_block(int p, const double* betas_gammas, double* out);

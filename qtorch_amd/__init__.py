@@ -6,7 +6,7 @@ This package only binds them for Python callers (tests, benchmarks, services).
 """
 from ._native import (  # noqa: F401
     QT_C64_3M, QT_C64_4M, QT_C64_BF16X3, QT_DTYPE_C64, QT_DTYPE_C128, QT_FROM_A, QT_PLAIN_C128,
-    QT_SCRATCH_AUTO, QT_SCRATCH_GLOBAL, QT_SCRATCH_LDS, QT_SCRATCH_STAGE_LEAVES, QT_RESCALE, SlicedNetwork,
+    QT_SCRATCH_AUTO, QT_SCRATCH_GLOBAL, QT_SCRATCH_LDS, QT_SCRATCH_STAGE_LEAVES, QT_RESCALE, QT_SPECIALISE_HERMITIAN, SlicedNetwork,
     ORDER_GIVEN, ORDER_GREEDY, ORDER_LINEGRAPH, ORDER_STOCHASTIC, ContractionFailure, Context, NetBatch,
     QaoaBatch, DeviceFailure, EngineError, InvalidContractionMethod, InvalidFile,
     InvalidFileFormat, InvalidFunctionInput, InvalidUserContractionSequence, Network, QbbFailure, Tensor,

@@ -22,6 +22,7 @@ QT_OK, QT_INVALID_INPUT, QT_OOM, QT_DEVICE, QT_TIMEOUT, QT_UNSUPPORTED = range(6
 QT_DTYPE_C128, QT_DTYPE_C64 = 0, 1
 QT_PLAIN_C128, QT_C64_4M, QT_C64_3M, QT_C64_BF16X3 = 0, 1, 2, 3
 QT_SCRATCH_AUTO, QT_SCRATCH_LDS, QT_SCRATCH_GLOBAL, QT_SCRATCH_STAGE_LEAVES = 0, 1, 2, 4
+QT_SPECIALISE_HERMITIAN = 1
 QT_RESCALE = 1 << 8  # or-ed into a complex64 form: per-step power-of-two rescaling
 QT_FROM_A = 1 << 30
 
@@ -73,12 +74,12 @@ _HIP_SIGS = {
     "qt_batch_set_instances": (_c.c_int, [_vp, _c.c_int, _i32p, _i64p, _c.c_int, _i32p, _c.c_double,
                                           _c.c_double]),
     "qt_batch_set_outputs": (_c.c_int, [_vp, _c.c_int, _c.POINTER(_vp)]),
-    "qt_batch_compile": (_c.c_int, [_vp, _c.POINTER(_c.c_float)]),
+    "qt_batch_compile": (_c.c_int, [_vp, _c.c_int, _c.POINTER(_c.c_float)]),
     "qt_batch_time_kernel": (_c.c_int, [_vp, _c.c_int, _c.POINTER(_c.c_float)]),
     "qt_batch_uncapture": (_c.c_int, [_vp]),
     "qt_program_compile": (_c.c_int, [_c.c_char_p, _c.c_char_p, _vp, _c.c_size_t, _c.POINTER(_c.c_size_t)]),
     "qt_batch_specialised_info": (_c.c_int, [_vp, _c.POINTER(_c.c_int), _c.POINTER(_c.c_int64)]),
-    "qt_program_source": (_c.c_int, [_c.c_int, _c.POINTER(_vp), _i64p, _dp, _c.c_int64, _c.c_int, _c.c_char_p,
+    "qt_program_source": (_c.c_int, [_c.c_int, _c.POINTER(_vp), _i64p, _dp, _c.c_int64, _c.c_int, _c.c_int, _c.c_char_p,
                                      _c.c_size_t, _c.POINTER(_c.c_size_t)]),
     "qt_batch_config": (_c.c_int, [_vp, _c.c_int, _c.c_int]),
     "qt_batch_config_ex": (_c.c_int, [_vp, _c.c_int, _c.c_int, _c.c_int]),
@@ -153,7 +154,7 @@ _HOST_SIGS = {
     "qth_sliced_destroy": (_c.c_int, [_vp]),
     "qth_sliced_set_fusion": (_c.c_int, [_vp, _c.c_int64]),
     "qth_qaoa_source": (_c.c_int, [_c.c_char_p, _c.c_int, _c.c_int, _c.c_int, _c.c_uint64, _c.c_int, _c.c_int,
-                                   _c.c_int, _c.c_char_p, _c.c_size_t, _c.POINTER(_c.c_size_t)]),
+                                   _c.c_int, _c.c_int, _c.c_char_p, _c.c_size_t, _c.POINTER(_c.c_size_t)]),
     "qth_qaoa_param_block": (_c.c_int, [_c.c_int, _dp, _dp]),
     "qth_serve_plan": (_c.c_int, [_c.c_int, _c.c_char_p, _c.c_int, _c.c_int, _c.c_char_p, _c.c_int,
                                   _c.POINTER(_c.c_int), _c.POINTER(_c.c_int64), _c.POINTER(_c.c_int64)]),
@@ -519,11 +520,14 @@ class _DeviceBatch:
         self.ctx.check(hip.qt_batch_time_kernel(self.dev, reps, ctypes.byref(ms)), "qt_batch_time_kernel")
         return ms.value
 
-    def compile(self) -> float:
+    def compile(self, hermitian: bool = False) -> float:
         """Generate, compile (hipRTC) and load straight-line code for this batch's
-        programs; later launches run it.  Returns the milliseconds it took."""
+        programs; later launches run it.  hermitian: the leaves are conjugate-symmetric
+        density-matrix-picture tensors (QT_SPECIALISE_HERMITIAN).  Returns the
+        milliseconds it took."""
         ms = ctypes.c_float()
-        self.ctx.check(hip.qt_batch_compile(self.dev, ctypes.byref(ms)), "qt_batch_compile")
+        self.ctx.check(hip.qt_batch_compile(self.dev, QT_SPECIALISE_HERMITIAN if hermitian else 0, ctypes.byref(ms)),
+                       "qt_batch_compile")
         return ms.value
 
     def specialised(self):
@@ -584,10 +588,11 @@ class QaoaBatch(_DeviceBatch):
 
 
 def qaoa_source(dgf: str, p: int, first_edge: int, n_edges: int, precision: int = QT_C64_4M, ref_map: bool = True,
-                order: int = ORDER_GREEDY, seed: int = 0) -> str:
+                order: int = ORDER_GREEDY, seed: int = 0, hermitian: bool = False) -> str:
     """Specialised-executor source of some QAOA edge programs (host only)."""
     n = ctypes.c_size_t()
-    args = (dgf.encode(), p, 1 if ref_map else 0, order, ctypes.c_uint64(seed), precision, first_edge, n_edges)
+    args = (dgf.encode(), p, 1 if ref_map else 0, order, ctypes.c_uint64(seed), precision,
+            QT_SPECIALISE_HERMITIAN if hermitian else 0, first_edge, n_edges)
     _host_check(host.qth_qaoa_source(*args, None, 0, ctypes.byref(n)))
     buf = ctypes.create_string_buffer(n.value + 1)
     _host_check(host.qth_qaoa_source(*args, buf, n.value + 1, ctypes.byref(n)))

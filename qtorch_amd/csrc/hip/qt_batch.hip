@@ -346,10 +346,11 @@ __global__ void k_reset_counter(int32_t* c) { *c = 0; }
 
 // The same sums when every group has m members and member j of group g is instance
 // j * n_groups + g (angle set g of edge j in an edge-major batch).  A workgroup takes
-// 32 groups; its 8 slices of 32 threads each add the members j = slice, slice + 8,
-// ... in order (neighbouring threads read neighbouring values), then one thread per
-// group adds the 8 partial sums in slice order: a fixed order, whatever the timing.
-constexpr int kSumGroups = 32, kSumSlices = 8;
+// 32 groups; its 16 slices of 32 threads each add the members j = slice, slice + 16,
+// ... in order (neighbouring threads read neighbouring values, every load of a
+// thread in flight at once), then one thread per group adds the 16 partial sums in
+// slice order: a fixed order, whatever the timing.
+constexpr int kSumGroups = 32, kSumSlices = 16;
 __global__ __launch_bounds__(kSumGroups* kSumSlices) void k_group_sum_strided(const double2* __restrict__ out,
                                                                              double* __restrict__ sums, int n_groups,
                                                                              int m, double c0, double c1) {
@@ -534,12 +535,14 @@ qt_status launch_batch(qt_batch* b) {
                            dim3(kSumGroups * kSumSlices), 0, c->stream,
                            reinterpret_cast<const double2*>(b->d_out), b->d_group, b->n_groups, b->group_stride_m,
                            b->c0, b->c1);
-        return check_hip(c, hipGetLastError(), "k_group_sum_strided");
+        s = check_hip(c, hipGetLastError(), "k_group_sum_strided");
+    } else {
+        hipLaunchKernelGGL(k_group_sum, dim3(unsigned(b->n_groups)), dim3(64), 0, c->stream,
+                           reinterpret_cast<const double2*>(b->d_out), b->d_seg, b->d_members, b->d_group, b->n_groups,
+                           b->c0, b->c1);
+        s = check_hip(c, hipGetLastError(), "k_group_sum");
     }
-    hipLaunchKernelGGL(k_group_sum, dim3(unsigned(b->n_groups)), dim3(64), 0, c->stream,
-                       reinterpret_cast<const double2*>(b->d_out), b->d_seg, b->d_members, b->d_group, b->n_groups,
-                       b->c0, b->c1);
-    return check_hip(c, hipGetLastError(), "k_group_sum");
+    return s;
 }
 
 size_t elem_size(int precision) { return precision == QT_PLAIN_C128 ? 16 : 8; }
@@ -659,6 +662,7 @@ qt_status append_programs(qt_batch* b, int n, qt_program* const* progs, const in
         }
         h.n_cst = int32_t(b->h_csts.size()) - h.cst_begin;
         h.param_elems = int32_t(pend);
+        h.pad = p->dim_log2;
         b->pstage = std::max(b->pstage, int32_t(pend));
         b->cstage = std::max(b->cstage, cst_elems);
         b->prog_const_end.push_back(cend);
@@ -687,9 +691,22 @@ qt_status append_programs(qt_batch* b, int n, qt_program* const* progs, const in
 // batches), growing the pool (contents kept) when needed.  The stream is idle.
 qt_status write_pool(qt_batch* b, int64_t offset, const double* values, int64_t n) {
     Ctx* c = b->ctx;
-    if (b->specialised && specialised_consts_change(b, offset, values, n)) {
-        drop_specialised(b);  // its literals are the old constants
-        if (b->graph_exec) (void)configure_scratch(b);  // the captured graph runs the dropped kernel
+    if (b->specialised) {
+        bool keep = !specialised_consts_change(b, offset, values, n);  // its literals are the constants
+        if (keep) {
+            // the host mirror follows the pool; the parameter blocks must keep what the
+            // generated code assumes of them
+            auto& m = b->h_pool_mirror;
+            if (int64_t(m.size()) < 2 * (offset + n)) m.resize(size_t(2 * (offset + n)), 0.0);
+            const bool c128 = b->precision == QT_PLAIN_C128;
+            for (int64_t i = 0; i < 2 * n; ++i)
+                m[size_t(2 * offset + i)] = c128 ? values[i] : double(float(values[i]));
+            keep = specialised_params_hold(b);
+        }
+        if (!keep) {
+            drop_specialised(b);
+            if (b->graph_exec) (void)configure_scratch(b);  // the captured graph runs the dropped kernel
+        }
     }
     const size_t es = elem_size(b->precision);
     const int64_t need = offset + n;
@@ -1089,6 +1106,7 @@ qt_status qt_batch_set_instances(qt_batch* b, int n, const int32_t* prog, const 
     b->n_groups = 0;
     b->group_stride_m = 0;
     b->h_prog.assign(prog, prog + n);
+    b->h_pbase.assign(param_base, param_base + n);
     qt_status s = qt::dev_put(ctx, &b->d_inst, &b->cap_inst, inst.data(), inst.size());
     if (s == QT_OK) s = qt::dev_put<double>(ctx, &b->d_out, &b->cap_out, nullptr, size_t(std::max(n, 1)) * 2);
     if (s == QT_OK && n_groups > 0) {
@@ -1107,7 +1125,8 @@ qt_status qt_batch_set_instances(qt_batch* b, int n, const int32_t* prog, const 
     b->c1 = c1;
     if (b->specialised) {
         s = qt::specialised_chunks(b);
-        if (s != QT_OK) qt::drop_specialised(b);
+        if (s != QT_OK || !qt::specialised_params_hold(b)) qt::drop_specialised(b);  // the table executor runs
+        s = QT_OK;
     }
     return qt::configure_scratch(b);  // rebuilds the chunk list, drops a captured graph
 }

@@ -6,11 +6,16 @@
 
 #include <cstdint>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "qt_internal.h"
 
 namespace qt {
+
+// Parameter elements (offset in the instance's block) that hold one value in every
+// instance of a program: (offset, re, im); generated code has them as literals.
+using ParamLits = std::vector<std::tuple<int64_t, double, double>>;
 
 struct ProgHdr {
     int32_t step_begin, n_steps;
@@ -20,7 +25,7 @@ struct ProgHdr {
     int64_t const_base;  // pool offset added to the program's constant leaf references
     int32_t cst_begin, n_cst;  // the program's distinct constant leaves (CstEntry list)
     int32_t param_elems;       // extent of an instance's parameter block that it reads
-    int32_t pad;
+    int32_t pad;               // index bits per axis (host use only)
 };
 
 // A distinct constant leaf of a program and its place in a wave's LDS leaf area.
@@ -134,9 +139,14 @@ struct qt_batch {
     bool specialised = false;
     std::vector<double> spec_consts;  // constant leaves the kernel was generated with (FP64)
     std::vector<int64_t> spec_fmas;   // FMAs per instance of each program in the generated code
+    bool spec_hermitian = false;      // generated with conjugate-symmetric leaves
+    std::vector<qt::ParamLits> spec_param_lits;  // per program: parameter elements generated as literals
+    std::vector<double> h_pool_mirror;  // host copy of the pool while generated code runs (complex64 values rounded)
+    std::vector<int64_t> h_pbase;     // parameter base of each instance (host copy)
     int2* d_schunks = nullptr;
     size_t cap_schunks = 0;
     int n_schunks = 0;
+    int spec_grid_m = 0, spec_grid_sb = 0;  // > 0: chunks form an edge-major grid (programs x 64-instance blocks)
 };
 
 namespace qt {
@@ -146,4 +156,7 @@ void drop_specialised(qt_batch* b);
 qt_status launch_specialised(qt_batch* b);
 // a pool write of n values at offset would change constants the kernel embeds
 bool specialised_consts_change(const qt_batch* b, int64_t offset, const double* values, int64_t n);
+// the generated kernel's assumptions on parameter blocks hold for the current
+// instances and the pool's host mirror
+bool specialised_params_hold(const qt_batch* b);
 }  // namespace qt

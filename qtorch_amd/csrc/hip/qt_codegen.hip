@@ -27,6 +27,7 @@
 #include <cstring>
 #include <map>
 #include <set>
+#include <tuple>
 #include <unordered_map>
 #include <sstream>
 #include <string>
@@ -43,12 +44,14 @@ namespace {
 constexpr int64_t kMaxSpecialisedFmas = 250000;
 
 struct ProgView {
+    const ParamLits* param_lits = nullptr;
     const StepDesc* steps = nullptr;
     int n_steps = 0;
     const uint32_t* tabs = nullptr;  // steps[i].tab indexes this array
     const int64_t* refs = nullptr;
     int n_leaves = 0;
     int64_t const_base = 0;
+    int dl = 2;  // index bits per axis
 };
 
 // One part (real or imaginary) of an operand element: a literal or a variable.
@@ -56,6 +59,7 @@ struct Part {
     bool lit = true;
     double v = 0;          // literal value (already rounded for complex64)
     std::string name;      // variable
+    bool neg = false;      // the variable's negation (conjugate mirror elements)
 };
 struct Val {
     Part re, im;
@@ -71,8 +75,8 @@ public:
         else std::snprintf(buf, sizeof buf, "%af", double(float(x)));
         return std::signbit(x) ? "(" + std::string(buf) + ")" : std::string(buf);
     }
-    std::string str(const Part& p) const { return p.lit ? lit(p.v) : p.name; }
-    std::string neg(const Part& p) const { return p.lit ? lit(-p.v) : "(-" + p.name + ")"; }
+    std::string str(const Part& p) const { return p.lit ? lit(p.v) : (p.neg ? "(-" + p.name + ")" : p.name); }
+    std::string neg(const Part& p) const { return p.lit ? lit(-p.v) : (p.neg ? p.name : "(-" + p.name + ")"); }
     const char* fma() const { return c128_ ? "__builtin_fma" : "__builtin_fmaf"; }
     const char* real() const { return c128_ ? "double" : "float"; }
 
@@ -96,8 +100,19 @@ struct Program {
 };
 
 // Source of one program: a device function prog<id>(const real* P, double* out).
-bool emit_program(int id, const ProgView& pv, bool c128, const double* pool, int64_t pool_elems, Program* out,
-                  std::string* err) {
+// The conjugate mirror of a flat index with two bits per axis (d = 2 ket + bra): ket
+// and bra bits swapped on every axis.
+inline int64_t mirror_index(int64_t x) {
+    const int64_t lo = 0x5555555555555555LL;
+    return ((x & lo) << 1) | ((x >> 1) & lo);
+}
+
+// With hermitian set, every leaf satisfies T(mirror(x)) = conj T(x) (superoperators
+// U (x) U* of gates, Hermitian measurement vectors), and so does every intermediate:
+// of each mirror pair only the lower index is computed (the other is its conjugate)
+// and self-mirror elements are real (their imaginary chains are not computed).
+bool emit_program(int id, const ProgView& pv, bool c128, bool hermitian, const double* pool, int64_t pool_elems,
+                  Program* out, std::string* err) {
     Writer w(c128);
     const int64_t kParam = QT_LEAF_PARAM;
     const std::string tag = "program " + std::to_string(id) + ": ";
@@ -106,6 +121,14 @@ bool emit_program(int id, const ProgView& pv, bool c128, const double* pool, int
     // per step, per output element: its terms; parts that stay exactly zero
     std::vector<std::vector<std::vector<Term>>> terms(size_t(pv.n_steps));
     std::vector<std::vector<std::pair<bool, bool>>> has(size_t(pv.n_steps));  // (re non-empty, im non-empty)
+    std::vector<std::vector<int>> mirror(size_t(pv.n_steps));  // computed element whose conjugate it is, or -1
+    if (hermitian && pv.dl != 2) {
+        *err = tag + "conjugate symmetry needs two index bits per axis";
+        return false;
+    }
+    std::unordered_map<int64_t, std::pair<double, double>> plit;
+    if (pv.param_lits)
+        for (const auto& [off, re, im] : *pv.param_lits) plit[off] = {re, im};
     auto operand = [&](int32_t code, int64_t idx, Val* v) -> bool {
         if (code >= 0) {
             auto it = scratch.find(code + int32_t(idx));
@@ -113,11 +136,17 @@ bool emit_program(int id, const ProgView& pv, bool c128, const double* pool, int
                 *err = tag + "read of an unwritten scratch element";
                 return false;
             }
-            const int st = it->second.first, e = it->second.second;
+            const int st = it->second.first;
+            int e = it->second.second;
+            bool conj = false;
+            if (mirror[size_t(st)][size_t(e)] >= 0) {
+                e = mirror[size_t(st)][size_t(e)];
+                conj = true;
+            }
             const auto h = has[size_t(st)][size_t(e)];
             const std::string base = "t" + std::to_string(st) + "_" + std::to_string(e);
-            v->re = h.first ? Part{false, 0, base + "r"} : Part{};
-            v->im = h.second ? Part{false, 0, base + "i"} : Part{};
+            v->re = h.first ? Part{false, 0, base + "r", false} : Part{};
+            v->im = h.second ? Part{false, 0, base + "i", conj} : Part{};
             v->step = st;
             v->elem = e;
             return true;
@@ -130,6 +159,12 @@ bool emit_program(int id, const ProgView& pv, bool c128, const double* pool, int
         const int64_t r = pv.refs[li];
         if (r & kParam) {
             const int64_t e = (r & ~kParam) + idx;
+            auto lit = plit.find(e);
+            if (lit != plit.end()) {  // the same in every instance
+                v->re = Part{true, lit->second.first, "", false};
+                v->im = Part{true, lit->second.second, "", false};
+                return true;
+            }
             v->re = Part{false, 0, "p" + std::to_string(e) + "r"};
             v->im = Part{false, 0, "p" + std::to_string(e) + "i"};
             return true;
@@ -143,6 +178,35 @@ bool emit_program(int id, const ProgView& pv, bool c128, const double* pool, int
         v->im = Part{true, c128 ? pool[2 * e + 1] : double(float(pool[2 * e + 1])), ""};
         return true;
     };
+    if (hermitian) {
+        // every constant leaf must be conjugate-symmetric (to FP64 rounding)
+        std::vector<int> leaf_bits(size_t(pv.n_leaves), -1);
+        for (int si = 0; si < pv.n_steps; ++si) {
+            const StepDesc& st = pv.steps[si];
+            if (st.a < 0 && -1 - st.a < pv.n_leaves) leaf_bits[size_t(-1 - st.a)] = st.pad & 0xff;
+            if (st.b < 0 && -1 - st.b < pv.n_leaves) leaf_bits[size_t(-1 - st.b)] = (st.pad >> 8) & 0xff;
+        }
+        for (int li = 0; li < pv.n_leaves; ++li) {
+            const int64_t r = pv.refs[li];
+            if ((r & kParam) || leaf_bits[size_t(li)] < 0) continue;
+            const int64_t n = int64_t(1) << leaf_bits[size_t(li)], base = pv.const_base + r;
+            if (base < 0 || base + n > pool_elems) {
+                *err = tag + "constant leaf outside the pool";
+                return false;
+            }
+            double mx = 0;
+            for (int64_t e = 0; e < n; ++e)
+                mx = std::max({mx, std::fabs(pool[2 * (base + e)]), std::fabs(pool[2 * (base + e) + 1])});
+            for (int64_t e = 0; e < n; ++e) {
+                const int64_t m = mirror_index(e);
+                if (std::fabs(pool[2 * (base + m)] - pool[2 * (base + e)]) > 1e-10 * mx ||
+                    std::fabs(pool[2 * (base + m) + 1] + pool[2 * (base + e) + 1]) > 1e-10 * mx) {
+                    *err = tag + "a constant leaf is not conjugate-symmetric";
+                    return false;
+                }
+            }
+        }
+    }
     // pass 1: the terms of every output element, with zeros propagated
     int64_t n_terms = 0;
     for (int si = 0; si < pv.n_steps; ++si) {
@@ -159,7 +223,15 @@ bool emit_program(int id, const ProgView& pv, bool c128, const double* pool, int
         auto& hs = has[size_t(si)];
         ts.resize(size_t(nC));
         hs.resize(size_t(nC));
+        mirror[size_t(si)].assign(size_t(nC), -1);
         for (int64_t c = 0; c < nC; ++c) {
+            const int64_t mc = hermitian ? mirror_index(c) : c;
+            if (mc < c) {  // the conjugate of an element computed before
+                mirror[size_t(si)][size_t(c)] = int(mc);
+                hs[size_t(c)] = hs[size_t(mc)];
+                continue;
+            }
+            const bool real_only = hermitian && mc == c;
             const uint32_t oa = fc[c] & 0xffffu, ob = fc[c] >> 16;
             bool hr = false, hi = false;
             for (int64_t s = 0; s < K; ++s) {
@@ -169,8 +241,8 @@ bool emit_program(int id, const ProgView& pv, bool c128, const double* pool, int
                     return false;
                 t.f[0] = !zero(t.a.re) && !zero(t.b.re);
                 t.f[1] = !zero(t.a.im) && !zero(t.b.im);
-                t.f[2] = !zero(t.a.re) && !zero(t.b.im);
-                t.f[3] = !zero(t.a.im) && !zero(t.b.re);
+                t.f[2] = !real_only && !zero(t.a.re) && !zero(t.b.im);
+                t.f[3] = !real_only && !zero(t.a.im) && !zero(t.b.re);
                 if (!(t.f[0] || t.f[1] || t.f[2] || t.f[3])) continue;
                 hr = hr || t.f[0] || t.f[1];
                 hi = hi || t.f[2] || t.f[3];
@@ -201,13 +273,19 @@ bool emit_program(int id, const ProgView& pv, bool c128, const double* pool, int
     std::ostringstream body;
     std::set<std::string> params;
     std::unordered_map<std::string, std::string> alias, chain_var;
-    auto name = [&](const Part& p) -> std::string {
-        if (p.lit) return w.lit(p.v);
+    auto plain = [&](const Part& p) -> std::string {
         if (p.name[0] == 'p') params.insert(p.name.substr(0, p.name.size() - 1));
         auto it = alias.find(p.name);
         return it == alias.end() ? p.name : it->second;
     };
-    auto neg = [&](const Part& p) -> std::string { return p.lit ? w.lit(-p.v) : "(-" + name(p) + ")"; };
+    auto name = [&](const Part& p) -> std::string {
+        if (p.lit) return w.lit(p.v);
+        return p.neg ? "(-" + plain(p) + ")" : plain(p);
+    };
+    auto neg = [&](const Part& p) -> std::string {
+        if (p.lit) return w.lit(-p.v);
+        return p.neg ? plain(p) : "(-" + plain(p) + ")";
+    };
     auto emit = [&](const std::string& var, const std::vector<std::pair<std::string, std::string>>& ops) {
         std::string key;
         for (const auto& o : ops) key += o.first + "," + o.second + ";";
@@ -224,7 +302,7 @@ bool emit_program(int id, const ProgView& pv, bool c128, const double* pool, int
     };
     for (int si = 0; si < pv.n_steps; ++si)
         for (size_t c = 0; c < terms[size_t(si)].size(); ++c) {
-            if (!live[size_t(si)][c]) continue;
+            if (!live[size_t(si)][c] || mirror[size_t(si)][c] >= 0) continue;
             const std::string base = "t" + std::to_string(si) + "_" + std::to_string(c);
             const std::string vr = base + "r", vi = base + "i";
             std::vector<std::pair<std::string, std::string>> ore, oim;
@@ -254,16 +332,16 @@ bool emit_program(int id, const ProgView& pv, bool c128, const double* pool, int
     return true;
 }
 
-std::string generate(const std::vector<ProgView>& progs, bool c128, const double* pool, int64_t pool_elems,
-                     std::vector<int64_t>* fmas, std::string* err) {
+std::string generate(const std::vector<ProgView>& progs, bool c128, bool hermitian, const double* pool,
+                     int64_t pool_elems, std::vector<int64_t>* fmas, std::string* err) {
     std::ostringstream os;
     os << "// specialised executor: " << progs.size() << " programs, " << (c128 ? "complex128" : "complex64 4M")
-       << "\n\n";
+       << (hermitian ? ", conjugate-symmetric leaves" : "") << "\n\n";
     if (fmas) fmas->clear();
     int64_t total = 0;
     for (size_t i = 0; i < progs.size(); ++i) {
         Program p;
-        if (!emit_program(int(i), progs[i], c128, pool, pool_elems, &p, err)) return std::string();
+        if (!emit_program(int(i), progs[i], c128, hermitian, pool, pool_elems, &p, err)) return std::string();
         total += p.fmas;
         if (total > kMaxSpecialisedFmas) {
             *err = "programs too large to specialise (more than " + std::to_string(kMaxSpecialisedFmas) +
@@ -277,12 +355,24 @@ std::string generate(const std::vector<ProgView>& progs, bool c128, const double
     // Instance records: int32 program, int32 pad, int64 parameter base (16 bytes)
     os << "extern \"C\" __global__ __launch_bounds__(64) void qt_batch_specialised(const " << real
        << "* __restrict__ pool, const long long* __restrict__ inst, const int* __restrict__ chunks, int n_chunks,\n"
-       << "    double* __restrict__ out) {\n"
-       // workgroups go round-robin to the 8 XCDs: XCD x takes the contiguous chunk range
-       // [x per, (x + 1) per), so the waves resident on a CU run few distinct programs
-       // and share its instruction cache
-       << "  const int per = int(gridDim.x) / 8;\n"
-       << "  const int ci = int(blockIdx.x % 8u) * per + int(blockIdx.x / 8u);\n"
+       << "    double* __restrict__ out, int grid_m, int grid_sb) {\n"
+       // Workgroups go round-robin to the 8 XCDs.  Chunks in an edge-major grid (chunk
+       // j * grid_sb + sb: program j, block sb of 64 instances sharing parameter blocks
+       // with block sb of every other program): XCD x owns the blocks
+       // [x per, (x + 1) per) and runs them in tiles of T blocks, all programs of a tile
+       // before the next, so a tile's parameter blocks stay in that XCD's L2.  Otherwise
+       // XCD x takes the contiguous chunk range [x per, (x + 1) per), so the waves
+       // resident on a CU run few distinct programs.
+       << "  int ci;\n"
+       << "  if (grid_sb > 0) {\n"
+       << "    const int per = grid_sb / 8, T = per < 16 ? per : 16;\n"
+       << "    const int x = int(blockIdx.x % 8u), k = int(blockIdx.x / 8u);\n"
+       << "    const int tile = k / (T * grid_m), r = k % (T * grid_m);\n"
+       << "    ci = (r / T) * grid_sb + x * per + tile * T + r % T;\n"
+       << "  } else {\n"
+       << "    const int per = int(gridDim.x) / 8;\n"
+       << "    ci = int(blockIdx.x % 8u) * per + int(blockIdx.x / 8u);\n"
+       << "  }\n"
        << "  if (ci >= n_chunks) return;\n"
        << "  const int first = chunks[2 * ci], count = chunks[2 * ci + 1];\n"
        << "  if (int(threadIdx.x) >= count) return;\n"
@@ -298,14 +388,17 @@ std::string generate(const std::vector<ProgView>& progs, bool c128, const double
 
 std::vector<ProgView> batch_views(const qt_batch* b) {
     std::vector<ProgView> v;
-    for (const auto& h : b->h_hdr) {
+    for (size_t p = 0; p < b->h_hdr.size(); ++p) {
+        const ProgHdr& h = b->h_hdr[p];
         ProgView pv;
+        if (p < b->spec_param_lits.size()) pv.param_lits = &b->spec_param_lits[p];
         pv.steps = b->h_steps.data() + h.step_begin;
         pv.n_steps = h.n_steps;
         pv.tabs = b->h_tabs.data();
         pv.refs = b->h_refs.data() + h.leaf_begin;
         pv.n_leaves = h.n_leaves;
         pv.const_base = h.const_base;
+        pv.dl = h.pad;  // the program's index bits per axis
         v.push_back(pv);
     }
     return v;
@@ -339,7 +432,104 @@ bool rtc_compile(const std::string& src, const std::string& arch, std::vector<ch
     return true;
 }
 
+// Parameter leaves of a program: (offset in the instance's block, element bits).
+std::vector<std::pair<int64_t, int>> param_leaves(const qt_batch* b, const ProgHdr& h) {
+    std::vector<std::pair<int64_t, int>> out;
+    for (int si = 0; si < h.n_steps; ++si) {
+        const StepDesc& st = b->h_steps[size_t(h.step_begin + si)];
+        for (int side = 0; side < 2; ++side) {
+            const int32_t code = side == 0 ? st.a : st.b;
+            if (code >= 0) continue;
+            const int64_t r = b->h_refs[size_t(h.leaf_begin + (-1 - code))];
+            if (r & QT_LEAF_PARAM)
+                out.push_back({r & ~QT_LEAF_PARAM, side == 0 ? (st.pad & 0xff) : ((st.pad >> 8) & 0xff)});
+        }
+    }
+    return out;
+}
+
+// Per program, the parameter elements with one value across all current instances.
+std::vector<ParamLits> collect_param_lits(const qt_batch* b, const std::vector<double>& pool) {
+    std::vector<ParamLits> lits(b->h_hdr.size());
+    std::vector<std::vector<int64_t>> bases(b->h_hdr.size());
+    for (int i = 0; i < b->n_inst; ++i) bases[size_t(b->h_prog[size_t(i)])].push_back(b->h_pbase[size_t(i)]);
+    const int64_t n_pool = int64_t(pool.size() / 2);
+    for (size_t p = 0; p < b->h_hdr.size(); ++p) {
+        auto& bs = bases[p];
+        std::sort(bs.begin(), bs.end());
+        bs.erase(std::unique(bs.begin(), bs.end()), bs.end());
+        if (bs.empty()) continue;
+        std::set<int64_t> done;
+        for (const auto& [r, bits] : param_leaves(b, b->h_hdr[p]))
+            for (int64_t e = r; e < r + (int64_t(1) << bits); ++e) {
+                if (!done.insert(e).second) continue;
+                bool same = true;
+                for (int64_t base : bs) {
+                    if (base + e >= n_pool) {
+                        same = false;
+                        break;
+                    }
+                    same = pool[size_t(2 * (base + e))] == pool[size_t(2 * (bs[0] + e))] &&
+                           pool[size_t(2 * (base + e) + 1)] == pool[size_t(2 * (bs[0] + e) + 1)];
+                    if (!same) break;
+                }
+                if (same) lits[p].push_back({e, pool[size_t(2 * (bs[0] + e))], pool[size_t(2 * (bs[0] + e) + 1)]});
+            }
+    }
+    return lits;
+}
+
+// Every parameter leaf of every instance satisfies T(mirror(x)) = conj T(x) (to
+// rounding): the leaves' element counts come from the steps that read them.
+bool params_conjugate_symmetric(const qt_batch* b, const std::vector<double>& pool) {
+    std::set<std::pair<int64_t, int>> checked;  // (pool offset, element bits)
+    for (int i = 0; i < b->n_inst; ++i) {
+        const ProgHdr& h = b->h_hdr[size_t(b->h_prog[size_t(i)])];
+        const int64_t pbase = b->h_pbase[size_t(i)];
+        for (int si = 0; si < h.n_steps; ++si) {
+            const StepDesc& st = b->h_steps[size_t(h.step_begin + si)];
+            for (int side = 0; side < 2; ++side) {
+                const int32_t code = side == 0 ? st.a : st.b;
+                if (code >= 0) continue;
+                const int64_t r = b->h_refs[size_t(h.leaf_begin + (-1 - code))];
+                if (!(r & QT_LEAF_PARAM)) continue;
+                const int bits = side == 0 ? (st.pad & 0xff) : ((st.pad >> 8) & 0xff);
+                const int64_t base = pbase + (r & ~QT_LEAF_PARAM), n = int64_t(1) << bits;
+                if (!checked.insert({base, bits}).second) continue;
+                if (base + n > int64_t(pool.size() / 2)) return false;
+                double mx = 0;
+                for (int64_t e = 0; e < n; ++e)
+                    mx = std::max({mx, std::fabs(pool[size_t(2 * (base + e))]), std::fabs(pool[size_t(2 * (base + e) + 1)])});
+                for (int64_t e = 0; e < n; ++e) {
+                    const int64_t m = mirror_index(e);
+                    if (std::fabs(pool[size_t(2 * (base + m))] - pool[size_t(2 * (base + e))]) > 1e-6 * mx ||
+                        std::fabs(pool[size_t(2 * (base + m) + 1)] + pool[size_t(2 * (base + e) + 1)]) > 1e-6 * mx)
+                        return false;
+                }
+            }
+        }
+    }
+    return true;
+}
+
 }  // namespace
+
+// The generated kernel's assumptions about parameter blocks still hold for the
+// current instances and the pool's host mirror: its parameter literals, and conjugate
+// symmetry when it was generated with it.
+bool specialised_params_hold(const qt_batch* b) {
+    const std::vector<double>& pool = b->h_pool_mirror;
+    const int64_t n_pool = int64_t(pool.size() / 2);
+    for (int i = 0; i < b->n_inst; ++i) {
+        const size_t p = size_t(b->h_prog[size_t(i)]);
+        const int64_t base = b->h_pbase[size_t(i)];
+        if (p >= b->spec_param_lits.size()) return false;
+        for (const auto& [e, re, im] : b->spec_param_lits[p])
+            if (base + e >= n_pool || pool[size_t(2 * (base + e))] != re || pool[size_t(2 * (base + e) + 1)] != im)
+                return false;
+    }
+    return !b->spec_hermitian || params_conjugate_symmetric(b, pool);
+}
 
 // Chunks of up to 64 consecutive instances of one program (one lane each).
 qt_status specialised_chunks(qt_batch* b) {
@@ -351,6 +541,21 @@ qt_status specialised_chunks(qt_batch* b) {
         i = j;
     }
     b->n_schunks = int(ch.size());
+    // an edge-major grid of full chunks (member j of group g = instance j n_groups + g)
+    // whose parameter blocks exceed an XCD's L2 (4 MiB): run in the tiled order
+    b->spec_grid_m = b->spec_grid_sb = 0;
+    int64_t param_end = 0;
+    for (int64_t e : b->prog_param_end) param_end = std::max(param_end, e);
+    const int64_t param_bytes = int64_t(b->n_groups) * param_end * (b->precision == QT_PLAIN_C128 ? 16 : 8);
+    if (b->group_stride_m > 0 && b->n_groups % 64 == 0 && param_bytes > (int64_t(4) << 20)) {
+        const int m = b->group_stride_m, n_sb = b->n_groups / 64, per = n_sb / 8;
+        bool grid = n_sb % 8 == 0 && per > 0 && (per < 16 || per % 16 == 0) && int(ch.size()) == m * n_sb;
+        for (size_t k = 0; k < ch.size() && grid; ++k) grid = ch[k].x == int(64 * k) && ch[k].y == 64;
+        if (grid) {
+            b->spec_grid_m = m;
+            b->spec_grid_sb = n_sb;
+        }
+    }
     if (ch.empty()) return QT_OK;
     if (ch.size() > b->cap_schunks) {
         if (b->d_schunks) (void)hipFree(b->d_schunks);
@@ -371,6 +576,8 @@ void drop_specialised(qt_batch* b) {
     b->kernel = nullptr;
     b->specialised = false;
     b->spec_consts.clear();
+    b->spec_param_lits.clear();
+    std::vector<double>().swap(b->h_pool_mirror);
 }
 
 bool specialised_consts_change(const qt_batch* b, int64_t offset, const double* values, int64_t n) {
@@ -393,8 +600,9 @@ qt_status launch_specialised(qt_batch* b) {
     const void* chunks = b->d_schunks;
     int n_chunks = b->n_schunks;
     void* out = b->d_out;
-    void* params[] = {&pool, &inst, &chunks, &n_chunks, &out};
-    const unsigned grid = unsigned((n_chunks + 7) / 8 * 8);  // a multiple of the XCD count
+    int grid_m = b->spec_grid_m, grid_sb = b->spec_grid_sb;
+    void* params[] = {&pool, &inst, &chunks, &n_chunks, &out, &grid_m, &grid_sb};
+    const unsigned grid = unsigned((n_chunks + 7) / 8 * 8);  // a multiple of the XCD count (exact for a grid)
     return check_hip(c, hipModuleLaunchKernel(b->kernel, grid, 1, 1, 64, 1, 1, 0, c->stream, params, nullptr),
                      "specialised kernel");
 }
@@ -407,7 +615,8 @@ using qt::set_error;
 extern "C" {
 
 qt_status qt_program_source(int n, qt_program* const* progs, const int64_t* const_base, const double* pool,
-                            int64_t pool_elems, int precision, char* buf, size_t cap, size_t* len) {
+                            int64_t pool_elems, int precision, int flags, char* buf, size_t cap, size_t* len) {
+    if ((flags & ~QT_SPECIALISE_HERMITIAN) != 0) return QT_INVALID_INPUT;
     if (n < 1 || !progs || !const_base || (pool_elems > 0 && !pool) || pool_elems < 0 || !len ||
         (precision != QT_C64_4M && precision != QT_PLAIN_C128))
         return QT_INVALID_INPUT;
@@ -421,10 +630,12 @@ qt_status qt_program_source(int n, qt_program* const* progs, const int64_t* cons
         pv.refs = progs[i]->leaf_refs.data();
         pv.n_leaves = int(progs[i]->leaf_refs.size());
         pv.const_base = const_base[i];
+        pv.dl = progs[i]->dim_log2;
         views.push_back(pv);
     }
     std::string err;
-    const std::string src = qt::generate(views, precision == QT_PLAIN_C128, pool, pool_elems, nullptr, &err);
+    const std::string src = qt::generate(views, precision == QT_PLAIN_C128, (flags & QT_SPECIALISE_HERMITIAN) != 0,
+                                         pool, pool_elems, nullptr, &err);
     if (src.empty()) return QT_INVALID_INPUT;
     *len = src.size();
     if (buf && cap > 0) {
@@ -445,8 +656,9 @@ qt_status qt_program_compile(const char* source, const char* arch, void* code, s
     return QT_OK;
 }
 
-qt_status qt_batch_compile(qt_batch* b, float* compile_ms) {
-    if (!b) return QT_INVALID_INPUT;
+qt_status qt_batch_compile(qt_batch* b, int flags, float* compile_ms) {
+    if (!b || (flags & ~QT_SPECIALISE_HERMITIAN) != 0) return QT_INVALID_INPUT;
+    const bool hermitian = (flags & QT_SPECIALISE_HERMITIAN) != 0;
     auto* ctx = b->ctx;
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
     (void)hipSetDevice(ctx->device);
@@ -457,8 +669,11 @@ qt_status qt_batch_compile(qt_batch* b, float* compile_ms) {
     for (int32_t oe : b->prog_out_elems)
         if (oe != 1) return set_error(ctx, QT_INVALID_INPUT, "qt_batch_compile: programs must end in scalars");
     const auto t0 = std::chrono::steady_clock::now();
-    // constant leaves as FP64 (complex64 pools convert exactly)
-    const int64_t n_const = std::min(b->pool_elems, b->max_leaf_ref_end);
+    // constant leaves as FP64 (complex64 pools convert exactly); with conjugate
+    // symmetry the whole pool, to check the instances' parameter blocks too
+    // the whole pool: constants, and the parameter blocks (their common elements
+    // become literals; conjugate symmetry is checked on them)
+    const int64_t n_const = b->pool_elems;
     std::vector<double> pool(size_t(2 * std::max<int64_t>(n_const, 1)), 0.0);
     qt_status s = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync");
     if (s == QT_OK && n_const > 0) {
@@ -472,29 +687,43 @@ qt_status qt_batch_compile(qt_batch* b, float* compile_ms) {
         }
     }
     if (s != QT_OK) return s;
+    if (hermitian && !qt::params_conjugate_symmetric(b, pool))
+        return set_error(ctx, QT_INVALID_INPUT, "qt_batch_compile: a parameter block is not conjugate-symmetric");
+    std::vector<qt::ParamLits> lits = qt::collect_param_lits(b, pool);
+    std::swap(lits, b->spec_param_lits);  // read by batch_views
     std::string err;
     std::vector<int64_t> fmas;
     const std::string src =
-        qt::generate(qt::batch_views(b), b->precision == QT_PLAIN_C128, pool.data(), n_const, &fmas, &err);
-    if (src.empty()) return set_error(ctx, QT_INVALID_INPUT, "qt_batch_compile: " + err);
+        qt::generate(qt::batch_views(b), b->precision == QT_PLAIN_C128, hermitian, pool.data(), n_const, &fmas, &err);
+    if (src.empty()) {
+        std::swap(lits, b->spec_param_lits);  // keep the state of the kernel that runs
+        return set_error(ctx, QT_INVALID_INPUT, "qt_batch_compile: " + err);
+    }
     hipDeviceProp_t prop;
     s = check_hip(ctx, hipGetDeviceProperties(&prop, ctx->device), "device properties");
     if (s != QT_OK) return s;
     std::vector<char> code;
-    if (!qt::rtc_compile(src, prop.gcnArchName, &code, &err))
+    if (!qt::rtc_compile(src, prop.gcnArchName, &code, &err)) {
+        std::swap(lits, b->spec_param_lits);
         return set_error(ctx, QT_DEVICE, "qt_batch_compile: " + err);
+    }
     hipModule_t mod = nullptr;
     hipFunction_t fn = nullptr;
     s = check_hip(ctx, hipModuleLoadData(&mod, code.data()), "module load");
     if (s == QT_OK) s = check_hip(ctx, hipModuleGetFunction(&fn, mod, "qt_batch_specialised"), "module function");
     if (s != QT_OK) {
         if (mod) (void)hipModuleUnload(mod);
+        std::swap(lits, b->spec_param_lits);
         return s;
     }
+    std::vector<qt::ParamLits> new_lits = std::move(b->spec_param_lits);
     qt::drop_specialised(b);
+    b->spec_param_lits = std::move(new_lits);
+    b->h_pool_mirror = pool;
     b->module = mod;
     b->kernel = fn;
-    b->spec_consts.assign(pool.begin(), pool.begin() + 2 * n_const);
+    b->spec_consts.assign(pool.begin(), pool.begin() + 2 * std::min(n_const, b->max_leaf_ref_end));
+    b->spec_hermitian = hermitian;
     b->spec_fmas = fmas;
     s = qt::specialised_chunks(b);
     if (s != QT_OK) {

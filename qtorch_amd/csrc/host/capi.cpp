@@ -332,8 +332,8 @@ int qth_qaoa_batch_destroy(qth_qaoa_batch* h) {
     return QTH_OK;
 }
 
-int qth_qaoa_source(const char* dgf, int p, int ref_map, int order, uint64_t seed, int precision, int first_edge,
-                    int n_edges, char* buf, size_t cap, size_t* len) {
+int qth_qaoa_source(const char* dgf, int p, int ref_map, int order, uint64_t seed, int precision, int flags,
+                    int first_edge, int n_edges, char* buf, size_t cap, size_t* len) {
     if (!dgf || p < 1 || !len || first_edge < 0 || n_edges < 1) return QTH_INVALID_FUNCTION_INPUT;
     return Guard([&] {
         const ExtraData data = ExtraData::FromText(p, dgf, ref_map != 0);
@@ -344,8 +344,8 @@ int qth_qaoa_source(const char* dgf, int p, int ref_map, int order, uint64_t see
         const auto& d = pool.Data();
         CheckQt(nullptr,
                 qt_program_source(n_edges, c.progs.data() + first_edge, base.data(),
-                                  reinterpret_cast<const double*>(d.data()), int64_t(d.size()), precision, buf, cap,
-                                  len),
+                                  reinterpret_cast<const double*>(d.data()), int64_t(d.size()), precision, flags, buf,
+                                  cap, len),
                 "qt_program_source");
     });
 }

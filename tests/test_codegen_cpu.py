@@ -45,16 +45,19 @@ def _build(src, n, real, tmp_path, name):
     return ctypes.CDLL(str(so))
 
 
+@pytest.mark.parametrize("hermitian", [False, True])
 @pytest.mark.parametrize("precision", [qa.QT_PLAIN_C128, qa.QT_C64_4M])
-def test_generated_code_matches_reference(tmp_path, precision):
+def test_generated_code_matches_reference(tmp_path, precision, hermitian):
+    """With hermitian, only one element of each conjugate pair of every intermediate is
+    computed (the other is its conjugate) and self-mirror elements are real."""
     dgf = open(os.path.join(GOLDEN, "graphs", "3reg_n30_s2024.dgf")).read()
     zz_ref, bg = _golden()
     first, n = 0, 6
-    src = qa.qaoa_source(dgf, 1, first, n, precision)
+    src = qa.qaoa_source(dgf, 1, first, n, precision, hermitian=hermitian)
     assert "qt_batch_specialised" in src and src.count("prog") >= n
     c128 = precision == qa.QT_PLAIN_C128
     real = "double" if c128 else "float"
-    lib = _build(src, n, real, tmp_path, "c128" if c128 else "c64")
+    lib = _build(src, n, real, tmp_path, ("c128" if c128 else "c64") + ("h" if hermitian else ""))
     block = qa.qaoa_param_block(1, bg)
     P = np.ascontiguousarray(block.view(np.float64) if c128 else block.view(np.float64).astype(np.float32))
     out = np.zeros(2)
@@ -70,11 +73,13 @@ def test_generated_code_matches_reference(tmp_path, precision):
 
 def test_generated_code_drops_zero_terms():
     """Constant gates in the Liouville picture are sparse: the generated code runs far
-    fewer FMAs than 4 per complex MAC of the plan's device steps."""
+    fewer FMAs than 4 per complex MAC of the plan's device steps; conjugate symmetry
+    roughly halves them again."""
     dgf = open(os.path.join(GOLDEN, "graphs", "3reg_n30_s2024.dgf")).read()
-    src = qa.qaoa_source(dgf, 1, 0, 1, qa.QT_C64_4M)
-    fmas = src.count("__builtin_fmaf(")
+    fmas = qa.qaoa_source(dgf, 1, 0, 1, qa.QT_C64_4M).count("__builtin_fmaf(")
+    herm = qa.qaoa_source(dgf, 1, 0, 1, qa.QT_C64_4M, hermitian=True).count("__builtin_fmaf(")
     assert 0 < fmas < 4 * 2473
+    assert 0 < herm < 0.6 * fmas
 
 
 def test_generated_code_compiles_without_packed_fp32(tmp_path):

@@ -33,12 +33,15 @@ def _sets(n, seed):
     return np.stack([rng.uniform(0, math.pi / 2, n), -2 * rng.uniform(0, math.pi, n)], axis=1)
 
 
-@pytest.mark.parametrize("precision,ref_map", [(qa.QT_C64_4M, True), (qa.QT_C64_4M, False),
-                                               (qa.QT_PLAIN_C128, True)])
-def test_specialised_equals_tables_bitwise(ctx, precision, ref_map):
-    """200 angle sets x 45 edges (partial wavefront chunks at every edge's end)."""
+@pytest.mark.parametrize("precision,ref_map,n_sets", [(qa.QT_C64_4M, True, 200), (qa.QT_C64_4M, False, 200),
+                                                      (qa.QT_PLAIN_C128, True, 200), (qa.QT_C64_4M, True, 32768),
+                                                      (qa.QT_PLAIN_C128, False, 1024)])
+def test_specialised_equals_tables_bitwise(ctx, precision, ref_map, n_sets):
+    """200 angle sets x 45 edges (partial wavefront chunks at every edge's end); 1024
+    sets form a grid of full chunks; 32768 sets' parameter blocks (8 MiB) exceed an
+    XCD's L2 and run in the XCD-tiled order."""
     b = qa.QaoaBatch(ctx, _graph(), 1, ref_map=ref_map, precision=precision)
-    b.set_angles(_sets(200, 3))
+    b.set_angles(_sets(n_sets, 3))
     b.launch()
     e0, z0 = b.results()
     ms = b.compile()
@@ -108,3 +111,75 @@ def test_specialise_refuses_what_it_cannot_run(ctx):
     e2, _ = b2.results()
     assert np.isfinite(e2[0])
     b2.close()
+
+
+@pytest.mark.parametrize("precision,tol", [(qa.QT_C64_4M, 2e-4), (qa.QT_PLAIN_C128, 1e-12)])
+def test_conjugate_symmetric_specialisation(ctx, precision, tol):
+    """QT_SPECIALISE_HERMITIAN: one element of each conjugate pair computed; values
+    within rounding of the complex128 table executor (per edge with the 1e-3 floor;
+    energies 1e-5 / 1e-12 relative), for several sets of angles and after new ones."""
+    ref = qa.QaoaBatch(ctx, _graph(), 1, ref_map=True, precision=qa.QT_PLAIN_C128)
+    b = qa.QaoaBatch(ctx, _graph(), 1, ref_map=True, precision=precision)
+    for k, sets in enumerate((_sets(130, 7), _sets(64, 8))):
+        ref.set_angles(sets)
+        ref.launch()
+        e_ref, z_ref = ref.results()
+        b.set_angles(sets)
+        if k == 0:
+            b.compile(hermitian=True)
+        assert b.specialised()[0]
+        b.launch()
+        e, z = b.results()
+        err = np.abs(z - z_ref) / np.maximum(np.abs(z_ref), 1e-3)
+        assert err.max() <= tol, err.max()
+        assert np.max(np.abs(e - e_ref) / np.abs(e_ref)) <= (1e-5 if precision == qa.QT_C64_4M else 1e-12)
+    b.close()
+    ref.close()
+
+
+def test_conjugate_symmetry_is_checked(ctx):
+    """Amplitude-picture programs (one bit per axis) cannot use the symmetry: refused,
+    and the table executor keeps running."""
+    from qtorch_amd import workloads as wl
+    q = wl.grid_random_circuit(2, 3, 4, seed=1)
+    nb = qa.NetBatch(ctx, [q, q], [wl.bitstring(6, 1), wl.bitstring(6, 2)], precision=qa.QT_C64_4M, dim_log2=1)
+    nb.launch()
+    v0 = nb.results()
+    with pytest.raises(qa.InvalidFunctionInput):
+        nb.compile(hermitian=True)
+    nb.compile()
+    nb.launch()
+    v1 = nb.results()
+    assert np.array_equal(v0, v1)
+    nb.close()
+
+
+def test_parameter_literals_are_guarded(ctx):
+    """Parameter elements equal in every instance at compile time become literals (with
+    one angle set, all of them).  New parameter blocks that break that drop the
+    generated kernel: the table executor runs and the values stay right; blocks that
+    keep it (the same set again) leave the kernel in place."""
+    one = _sets(1, 11)
+    b = qa.QaoaBatch(ctx, _graph(), 1, ref_map=True, precision=qa.QT_C64_4M)
+    b.set_angles(one)
+    b.launch()
+    e_tab, z_tab = b.results()
+    b.compile()
+    b.launch()
+    e1, z1 = b.results()
+    assert b.specialised()[0]
+    assert np.array_equal(z1, z_tab) and np.array_equal(e1, e_tab)
+    b.set_angles(one)  # the same block: literals still hold
+    assert b.specialised()[0]
+    many = _sets(50, 12)
+    b.set_angles(many)  # different angles: the literals no longer hold
+    assert not b.specialised()[0]
+    b.launch()
+    e2, z2 = b.results()
+    ref = qa.QaoaBatch(ctx, _graph(), 1, ref_map=True, precision=qa.QT_C64_4M)
+    ref.set_angles(many)
+    ref.launch()
+    e3, z3 = ref.results()
+    assert np.array_equal(z2, z3) and np.array_equal(e2, e3)
+    ref.close()
+    b.close()
