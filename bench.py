@@ -434,19 +434,24 @@ def main():
         b.uncapture()
     for _ in range(args.warmup):
         b.launch()
+    b.flush()
     ctx.sync()
     ne = b.n_edges
-    # per-step latency (launch + completion), then the throughput run
+    # per-step latency (launch + completion with its energy sums), then the throughput
+    # run; with direct launches of the specialised kernel each launch also sums the
+    # previous launch's energies, and the last launch's sums run at the flush
     lat = []
     for _ in range(min(args.steps, 20)):
         t0 = time.perf_counter()
         b.launch()
+        b.flush()
         ctx.sync()
         lat.append(time.perf_counter() - t0)
     ctx.sync()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         b.launch()
+    b.flush()
     ctx.sync()
     wall = time.perf_counter() - t0
     launch_ms = b.time(args.steps)  # HIP events on the engine's stream around whole launches

@@ -203,6 +203,13 @@ qt_status qt_batch_capture(qt_batch* b);
 qt_status qt_batch_uncapture(qt_batch* b);
 /* Synchronises and copies per-instance values (2 doubles each) and group sums. */
 qt_status qt_batch_results(qt_batch* b, double* values, double* group_sums);
+/* A specialised batch with group sums of the edge-major layout (member j of group g =
+ * instance j * n_groups + g) launched directly leaves each launch's sums to its next
+ * launch's kernel (the results alternate between two buffers); this launches the last
+ * launch's sums now, asynchronously.  qt_batch_results, qt_batch_poll, qt_batch_time
+ * and qt_batch_capture do it themselves; a caller that waits on the stream alone
+ * calls it first.  No counterpart in the reference. */
+qt_status qt_batch_flush(qt_batch* b);
 /* Mean time of `reps` launches, by HIP events on the context's stream. */
 qt_status qt_batch_time(qt_batch* b, int reps, float* ms_per_launch);
 /* The same for the contraction kernel alone (no group sums; the table executors'

@@ -77,6 +77,7 @@ _HIP_SIGS = {
     "qt_batch_compile": (_c.c_int, [_vp, _c.c_int, _c.POINTER(_c.c_float)]),
     "qt_batch_time_kernel": (_c.c_int, [_vp, _c.c_int, _c.POINTER(_c.c_float)]),
     "qt_batch_uncapture": (_c.c_int, [_vp]),
+    "qt_batch_flush": (_c.c_int, [_vp]),
     "qt_program_compile": (_c.c_int, [_c.c_char_p, _c.c_char_p, _vp, _c.c_size_t, _c.POINTER(_c.c_size_t)]),
     "qt_batch_specialised_info": (_c.c_int, [_vp, _c.POINTER(_c.c_int), _c.POINTER(_c.c_int64)]),
     "qt_program_source": (_c.c_int, [_c.c_int, _c.POINTER(_vp), _i64p, _dp, _c.c_int64, _c.c_int, _c.c_int, _c.c_char_p,
@@ -495,6 +496,16 @@ class _DeviceBatch:
 
     def capture(self) -> None:
         self.ctx.check(hip.qt_batch_capture(self.dev), "qt_batch_capture")
+
+    def poll(self) -> bool:
+        """True once the last launch (with its group sums) has finished; never blocks."""
+        d = ctypes.c_int()
+        self.ctx.check(hip.qt_batch_poll(self.dev, ctypes.byref(d)), "qt_batch_poll")
+        return bool(d.value)
+
+    def flush(self) -> None:
+        """Launch the group sums still pending for the last launch (asynchronous)."""
+        self.ctx.check(hip.qt_batch_flush(self.dev), "qt_batch_flush")
 
     def uncapture(self) -> None:
         self.ctx.check(hip.qt_batch_uncapture(self.dev), "qt_batch_uncapture")

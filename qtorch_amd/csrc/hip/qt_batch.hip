@@ -360,7 +360,7 @@ __global__ __launch_bounds__(kSumGroups* kSumSlices) void k_group_sum_strided(co
     double acc = 0.0;
     if (g < n_groups) {
 #pragma unroll 4
-        for (int j = slice; j < m; j += kSumSlices) acc += c0 + c1 * out[int64_t(j) * n_groups + g].x;
+        for (int j = slice; j < m; j += kSumSlices) acc += __builtin_fma(c1, out[int64_t(j) * n_groups + g].x, c0);
     }
     part[slice][gl] = acc;
     __syncthreads();
@@ -514,7 +514,7 @@ qt_status launch_tables(qt_batch* b) {
     }
 }
 
-qt_status launch_batch(qt_batch* b) {
+qt_status launch_batch(qt_batch* b, bool capture = false) {
     Ctx* c = b->ctx;
     // a failed reconfiguration leaves scratch or chunk arrays that do not fit the shape
     if (!b->shape_ok) return set_error(c, QT_INVALID_INPUT, "batch: executor shape not configured (last change failed)");
@@ -522,7 +522,10 @@ qt_status launch_batch(qt_batch* b) {
         return set_error(c, QT_INVALID_INPUT, "batch: programs with tensor results need qt_batch_set_outputs");
     qt_status s = QT_OK;
     if (b->specialised) {
-        s = launch_specialised(b);
+        // direct launches leave the group sums to the next launch (or a flush)
+        const bool defer = !capture && b->group_stride_m > 0 && b->d_out_prev;
+        s = launch_specialised(b, defer);
+        if (s != QT_OK || defer) return s;
     } else {
         hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, c->stream, b->d_counter);
         s = check_hip(c, hipGetLastError(), "k_reset_counter");
@@ -530,19 +533,7 @@ qt_status launch_batch(qt_batch* b) {
         s = launch_tables(b);
     }
     if (s != QT_OK || b->n_groups == 0) return s;
-    if (b->group_stride_m > 0) {
-        hipLaunchKernelGGL(k_group_sum_strided, dim3(unsigned((b->n_groups + kSumGroups - 1) / kSumGroups)),
-                           dim3(kSumGroups * kSumSlices), 0, c->stream,
-                           reinterpret_cast<const double2*>(b->d_out), b->d_group, b->n_groups, b->group_stride_m,
-                           b->c0, b->c1);
-        s = check_hip(c, hipGetLastError(), "k_group_sum_strided");
-    } else {
-        hipLaunchKernelGGL(k_group_sum, dim3(unsigned(b->n_groups)), dim3(64), 0, c->stream,
-                           reinterpret_cast<const double2*>(b->d_out), b->d_seg, b->d_members, b->d_group, b->n_groups,
-                           b->c0, b->c1);
-        s = check_hip(c, hipGetLastError(), "k_group_sum");
-    }
-    return s;
+    return launch_group_sums(b);
 }
 
 size_t elem_size(int precision) { return precision == QT_PLAIN_C128 ? 16 : 8; }
@@ -827,6 +818,31 @@ bool reorder_steps(int dl, int n_leaves, int n_steps, const int32_t* desc, const
 }
 
 }  // namespace
+
+qt_status launch_group_sums(qt_batch* b) {
+    Ctx* c = b->ctx;
+    qt_status s;
+    if (b->group_stride_m > 0) {
+        hipLaunchKernelGGL(k_group_sum_strided, dim3(unsigned((b->n_groups + kSumGroups - 1) / kSumGroups)),
+                           dim3(kSumGroups * kSumSlices), 0, c->stream,
+                           reinterpret_cast<const double2*>(b->d_out), b->d_group, b->n_groups, b->group_stride_m,
+                           b->c0, b->c1);
+        s = check_hip(c, hipGetLastError(), "k_group_sum_strided");
+    } else {
+        hipLaunchKernelGGL(k_group_sum, dim3(unsigned(b->n_groups)), dim3(64), 0, c->stream,
+                           reinterpret_cast<const double2*>(b->d_out), b->d_seg, b->d_members, b->d_group, b->n_groups,
+                           b->c0, b->c1);
+        s = check_hip(c, hipGetLastError(), "k_group_sum");
+    }
+    return s;
+}
+
+qt_status flush_sums(qt_batch* b) {
+    if (!b->sums_pending) return QT_OK;
+    b->sums_pending = false;
+    return launch_group_sums(b);
+}
+
 }  // namespace qt
 
 using qt::check_hip;
@@ -1044,6 +1060,7 @@ qt_status qt_batch_set_leaves(qt_batch* b, const double* pool, int64_t n_elems) 
         return set_error(ctx, QT_INVALID_INPUT, "qt_batch_set_leaves: pool smaller than the leaf references");
     b->pool_elems = 0;
     b->n_inst = 0;  // parameter bases were checked against the old pool: set instances again
+    b->sums_pending = false;
     (void)qt::configure_scratch(b);
     return qt::write_pool(b, 0, pool, n_elems);
 }
@@ -1108,7 +1125,10 @@ qt_status qt_batch_set_instances(qt_batch* b, int n, const int32_t* prog, const 
     b->h_prog.assign(prog, prog + n);
     b->h_pbase.assign(param_base, param_base + n);
     qt_status s = qt::dev_put(ctx, &b->d_inst, &b->cap_inst, inst.data(), inst.size());
+    b->sums_pending = false;  // the previous instances' results are gone
     if (s == QT_OK) s = qt::dev_put<double>(ctx, &b->d_out, &b->cap_out, nullptr, size_t(std::max(n, 1)) * 2);
+    if (s == QT_OK && n_groups > 0)
+        s = qt::dev_put<double>(ctx, &b->d_out_prev, &b->cap_out_prev, nullptr, size_t(std::max(n, 1)) * 2);
     if (s == QT_OK && n_groups > 0) {
         s = qt::dev_put(ctx, &b->d_seg, &b->cap_seg, seg.data(), seg.size());
         if (s == QT_OK) s = qt::dev_put(ctx, &b->d_members, &b->cap_members, members.data(), members.size());
@@ -1209,6 +1229,12 @@ qt_status qt_batch_launch(qt_batch* b) {
 qt_status qt_batch_poll(qt_batch* b, int* done) {
     if (!b || !done) return QT_INVALID_INPUT;
     std::lock_guard<std::recursive_mutex> g(b->ctx->mu);
+    if (b->sums_pending) {  // the last launch is done once its group sums are
+        (void)hipSetDevice(b->ctx->device);
+        qt_status s = qt::flush_sums(b);
+        if (s == QT_OK) s = check_hip(b->ctx, hipEventRecord(b->ev_done, b->ctx->stream), "event record");
+        if (s != QT_OK) return s;
+    }
     const hipError_t e = hipEventQuery(b->ev_done);
     if (e == hipErrorNotReady) {
         *done = 0;
@@ -1224,9 +1250,11 @@ qt_status qt_batch_capture(qt_batch* b) {
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
     (void)hipSetDevice(ctx->device);
     if (b->graph_exec) return QT_OK;
-    qt_status s = check_hip(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "begin capture");
+    qt_status s = qt::flush_sums(b);
     if (s != QT_OK) return s;
-    qt_status ls = qt::launch_batch(b);
+    s = check_hip(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "begin capture");
+    if (s != QT_OK) return s;
+    qt_status ls = qt::launch_batch(b, true);
     hipGraph_t graph = nullptr;
     s = check_hip(ctx, hipStreamEndCapture(ctx->stream, &graph), "end capture");
     if (ls != QT_OK) s = ls;
@@ -1250,12 +1278,20 @@ qt_status qt_batch_uncapture(qt_batch* b) {
     return QT_OK;
 }
 
+qt_status qt_batch_flush(qt_batch* b) {
+    if (!b) return QT_INVALID_INPUT;
+    std::lock_guard<std::recursive_mutex> g(b->ctx->mu);
+    (void)hipSetDevice(b->ctx->device);
+    return qt::flush_sums(b);
+}
+
 qt_status qt_batch_results(qt_batch* b, double* values, double* group_sums) {
     if (!b) return QT_INVALID_INPUT;
     auto* ctx = b->ctx;
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
     (void)hipSetDevice(ctx->device);
-    qt_status s = check_hip(ctx, hipStreamSynchronize(ctx->stream), "batch sync");
+    qt_status s = qt::flush_sums(b);
+    if (s == QT_OK) s = check_hip(ctx, hipStreamSynchronize(ctx->stream), "batch sync");
     if (s == QT_OK && values && b->n_inst > 0)
         s = check_hip(ctx, hipMemcpy(values, b->d_out, size_t(b->n_inst) * 16, hipMemcpyDeviceToHost), "results");
     if (s == QT_OK && group_sums && b->n_groups > 0)
@@ -1271,6 +1307,7 @@ qt_status qt_batch_time(qt_batch* b, int reps, float* ms_per_launch) {
     (void)hipSetDevice(ctx->device);
     qt_status s = check_hip(ctx, hipEventRecord(b->ev0, ctx->stream), "event record");
     for (int r = 0; r < reps && s == QT_OK; ++r) s = qt_batch_launch(b);
+    if (s == QT_OK) s = qt::flush_sums(b);  // the last launch's sums are part of its work
     if (s == QT_OK) s = check_hip(ctx, hipEventRecord(b->ev1, ctx->stream), "event record");
     if (s == QT_OK) s = check_hip(ctx, hipEventSynchronize(b->ev1), "event sync");
     float ms = 0;
@@ -1289,10 +1326,11 @@ qt_status qt_batch_time_kernel(qt_batch* b, int reps, float* ms_per_launch) {
     if (!b->shape_ok) return set_error(ctx, QT_INVALID_INPUT, "batch: executor shape not configured (last change failed)");
     if (b->needs_tout && !b->tensor_out)
         return set_error(ctx, QT_INVALID_INPUT, "batch: programs with tensor results need qt_batch_set_outputs");
-    qt_status s = check_hip(ctx, hipEventRecord(b->ev0, ctx->stream), "event record");
+    qt_status s = qt::flush_sums(b);  // its kernels rewrite d_out with the same values
+    if (s == QT_OK) s = check_hip(ctx, hipEventRecord(b->ev0, ctx->stream), "event record");
     for (int r = 0; r < reps && s == QT_OK; ++r) {
         if (b->specialised) {
-            s = qt::launch_specialised(b);
+            s = qt::launch_specialised(b, false);
         } else {
             hipLaunchKernelGGL(qt::k_reset_counter, dim3(1), dim3(1), 0, ctx->stream, b->d_counter);
             s = check_hip(ctx, hipGetLastError(), "k_reset_counter");
@@ -1363,7 +1401,8 @@ qt_status qt_batch_destroy(qt_batch* b) {
         for (void* p : {(void*)b->d_progs, (void*)b->d_steps, (void*)b->d_tabs, (void*)b->d_leaf_refs, b->d_pool,
                         (void*)b->d_inst, (void*)b->d_out, (void*)b->d_counter, (void*)b->d_seg, (void*)b->d_members,
                         (void*)b->d_group, b->d_global_scratch, (void*)b->d_chunks, b->d_pool_stage,
-                        (void*)b->d_leaf_lds, (void*)b->d_csts, (void*)b->d_tout, (void*)b->d_schunks})
+                        (void*)b->d_leaf_lds, (void*)b->d_csts, (void*)b->d_tout, (void*)b->d_schunks,
+                        (void*)b->d_out_prev})
             if (p) (void)hipFree(p);
         --b->ctx->live;
     }

@@ -147,13 +147,23 @@ struct qt_batch {
     size_t cap_schunks = 0;
     int n_schunks = 0;
     int spec_grid_m = 0, spec_grid_sb = 0;  // > 0: chunks form an edge-major grid (programs x 64-instance blocks)
+    // Deferred group sums (specialised kernel, strided groups, direct launches): each
+    // launch's kernel also sums the previous launch's results, kept in the other result
+    // buffer; the last launch's sums run when results are read, polled or flushed.
+    double* d_out_prev = nullptr;
+    size_t cap_out_prev = 0;
+    bool sums_pending = false;  // d_out's group sums have not been launched yet
 };
 
 namespace qt {
 // specialised executors (qt_codegen.hip)
 qt_status specialised_chunks(qt_batch* b);
 void drop_specialised(qt_batch* b);
-qt_status launch_specialised(qt_batch* b);
+qt_status launch_specialised(qt_batch* b, bool defer_sums);
+// group sums of d_out on the context's stream (qt_batch.hip)
+qt_status launch_group_sums(qt_batch* b);
+// launches the group sums still pending for the last launch
+qt_status flush_sums(qt_batch* b);
 // a pool write of n values at offset would change constants the kernel embeds
 bool specialised_consts_change(const qt_batch* b, int64_t offset, const double* values, int64_t n);
 // the generated kernel's assumptions on parameter blocks hold for the current

@@ -355,7 +355,29 @@ std::string generate(const std::vector<ProgView>& progs, bool c128, bool hermiti
     // Instance records: int32 program, int32 pad, int64 parameter base (16 bytes)
     os << "extern \"C\" __global__ __launch_bounds__(64) void qt_batch_specialised(const " << real
        << "* __restrict__ pool, const long long* __restrict__ inst, const int* __restrict__ chunks, int n_chunks,\n"
-       << "    double* __restrict__ out, int grid_m, int grid_sb) {\n"
+       << "    double* __restrict__ out, int grid_m, int grid_sb, const double* __restrict__ prev,\n"
+       << "    double* __restrict__ sums, int n_groups, int m, double c0, double c1, int n_sum_blocks) {\n"
+       // the first n_sum_blocks workgroups add up the previous launch's results per
+       // group (member j of group g = instance j n_groups + g), in the order of the
+       // library's strided group-sum kernel: 16 slices j = slice, slice + 16, ..., then
+       // the slices in order
+       << "  if (int(blockIdx.x) < n_sum_blocks) {  // 4 groups x 16 slices per workgroup\n"
+       << "    __shared__ double part[16][4];\n"
+       << "    const int gl = int(threadIdx.x) & 3, slice = int(threadIdx.x) >> 2;\n"
+       << "    const int g = int(blockIdx.x) * 4 + gl;\n"
+       << "    double p = 0.0;\n"
+       << "    if (g < n_groups)\n"
+       << "      for (int j = slice; j < m; j += 16) p += __builtin_fma(c1, prev[2 * ((long long)j * n_groups + g)], c0);\n"
+       << "    part[slice][gl] = p;\n"
+       << "    __syncthreads();\n"
+       << "    if (slice == 0 && g < n_groups) {\n"
+       << "      double t = part[0][gl];\n"
+       << "      for (int k = 1; k < 16; ++k) t += part[k][gl];\n"
+       << "      sums[g] = t;\n"
+       << "    }\n"
+       << "    return;\n"
+       << "  }\n"
+       << "  const unsigned bid = blockIdx.x - unsigned(n_sum_blocks), nb = gridDim.x - unsigned(n_sum_blocks);\n"
        // Workgroups go round-robin to the 8 XCDs.  Chunks in an edge-major grid (chunk
        // j * grid_sb + sb: program j, block sb of 64 instances sharing parameter blocks
        // with block sb of every other program): XCD x owns the blocks
@@ -366,12 +388,12 @@ std::string generate(const std::vector<ProgView>& progs, bool c128, bool hermiti
        << "  int ci;\n"
        << "  if (grid_sb > 0) {\n"
        << "    const int per = grid_sb / 8, T = per < 16 ? per : 16;\n"
-       << "    const int x = int(blockIdx.x % 8u), k = int(blockIdx.x / 8u);\n"
+       << "    const int x = int(bid % 8u), k = int(bid / 8u);\n"
        << "    const int tile = k / (T * grid_m), r = k % (T * grid_m);\n"
        << "    ci = (r / T) * grid_sb + x * per + tile * T + r % T;\n"
        << "  } else {\n"
-       << "    const int per = int(gridDim.x) / 8;\n"
-       << "    ci = int(blockIdx.x % 8u) * per + int(blockIdx.x / 8u);\n"
+       << "    const int per = int(nb) / 8;\n"
+       << "    ci = int(bid % 8u) * per + int(bid / 8u);\n"
        << "  }\n"
        << "  if (ci >= n_chunks) return;\n"
        << "  const int first = chunks[2 * ci], count = chunks[2 * ci + 1];\n"
@@ -571,6 +593,7 @@ qt_status specialised_chunks(qt_batch* b) {
 }
 
 void drop_specialised(qt_batch* b) {
+    if (b->sums_pending) (void)flush_sums(b);  // the table executor's launches sum their own results
     if (b->module) (void)hipModuleUnload(b->module);
     b->module = nullptr;
     b->kernel = nullptr;
@@ -591,18 +614,37 @@ bool specialised_consts_change(const qt_batch* b, int64_t offset, const double* 
     return false;
 }
 
-qt_status launch_specialised(qt_batch* b) {
+qt_status launch_specialised(qt_batch* b, bool defer_sums) {
     Ctx* c = b->ctx;
     if (b->needs_tout) return set_error(c, QT_INVALID_INPUT, "batch: specialised programs end in scalars");
     if (b->n_schunks == 0) return QT_OK;
+    // deferred group sums: this launch writes the other buffer and sums the previous
+    // launch's results (when they are still unsummed) in its first workgroups
+    const double* prev = nullptr;
+    int n_sum_blocks = 0;
+    if (defer_sums) {
+        std::swap(b->d_out, b->d_out_prev);
+        std::swap(b->cap_out, b->cap_out_prev);  // capacities follow their buffers
+        if (b->sums_pending) {
+            prev = b->d_out_prev;
+            n_sum_blocks = (b->n_groups + 3) / 4;
+            n_sum_blocks = (n_sum_blocks + 7) / 8 * 8;  // keeps the XCD of each chunk workgroup
+        }
+        b->sums_pending = true;
+    }
+    double* sums = b->d_group;
+    int n_groups = b->n_groups, m = b->group_stride_m;
+    double c0 = b->c0, c1 = b->c1;
     const void* pool = b->d_pool;
     const void* inst = b->d_inst;
     const void* chunks = b->d_schunks;
     int n_chunks = b->n_schunks;
     void* out = b->d_out;
     int grid_m = b->spec_grid_m, grid_sb = b->spec_grid_sb;
-    void* params[] = {&pool, &inst, &chunks, &n_chunks, &out, &grid_m, &grid_sb};
-    const unsigned grid = unsigned((n_chunks + 7) / 8 * 8);  // a multiple of the XCD count (exact for a grid)
+    void* params[] = {&pool, &inst, &chunks, &n_chunks, &out, &grid_m, &grid_sb, &prev,
+                      &sums, &n_groups, &m, &c0, &c1, &n_sum_blocks};
+    // a multiple of the XCD count (exact for a grid), after the sum workgroups
+    const unsigned grid = unsigned(n_sum_blocks + (n_chunks + 7) / 8 * 8);
     return check_hip(c, hipModuleLaunchKernel(b->kernel, grid, 1, 1, 64, 1, 1, 0, c->stream, params, nullptr),
                      "specialised kernel");
 }

@@ -23,6 +23,8 @@ PRELUDE = r"""
 struct qt_index3 { unsigned x, y, z; };
 static qt_index3 blockIdx, threadIdx, gridDim;
 #define __builtin_amdgcn_readfirstlane(x) (x)
+#define __shared__ static
+#define __syncthreads() ((void)0)
 """
 
 

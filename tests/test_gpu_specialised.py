@@ -183,3 +183,41 @@ def test_parameter_literals_are_guarded(ctx):
     assert np.array_equal(z2, z3) and np.array_equal(e2, e3)
     ref.close()
     b.close()
+
+
+def test_deferred_group_sums(ctx):
+    """Direct launches of the specialised kernel sum the previous launch's energies in
+    their own first workgroups; results, poll and flush launch the last one's.  Energies
+    after launches with different angle sets equal the table executor's bit for bit."""
+    import time
+    graph = _graph()
+    b = qa.QaoaBatch(ctx, graph, 1, ref_map=True, precision=qa.QT_C64_4M)
+    t = qa.QaoaBatch(ctx, graph, 1, ref_map=True, precision=qa.QT_C64_4M)
+    b.set_angles(_sets(256, 21))
+    b.compile()
+    for seed in (22, 23):
+        sets = _sets(256, seed)
+        b.set_angles(sets)
+        t.set_angles(sets)
+        for _ in range(3):  # consecutive launches: each sums the previous one
+            b.launch()
+        e, z = b.results()
+        t.launch()
+        e_t, z_t = t.results()
+        assert np.array_equal(z, z_t) and np.array_equal(e, e_t)
+    b.launch()
+    b.launch()
+    done = 0
+    t0 = time.time()
+    while not done and time.time() - t0 < 30:
+        done = b.poll()
+    assert done
+    e2, _ = b.results()
+    assert np.array_equal(e2, e_t)
+    b.launch()
+    b.flush()
+    ctx.sync()
+    e3, _ = b.results()
+    assert np.array_equal(e3, e_t)
+    b.close()
+    t.close()
