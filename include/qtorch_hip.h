@@ -212,9 +212,11 @@ qt_status qt_batch_results(qt_batch* b, double* values, double* group_sums);
 qt_status qt_batch_flush(qt_batch* b);
 /* Mean time of `reps` launches, by HIP events on the context's stream. */
 qt_status qt_batch_time(qt_batch* b, int reps, float* ms_per_launch);
-/* The same for the contraction kernel alone (no group sums; the table executors'
- * work-counter reset, a one-thread kernel, is included): the dominant kernel's time
- * per launch. */
+/* The same for the contraction kernel alone, the dominant kernel's time per launch:
+ * for the table executors without the group sums (their work-counter reset, a
+ * one-thread kernel, is included); for a specialised kernel with deferred group sums
+ * as it runs in a stream of direct launches (each summing the previous launch's
+ * results in its first workgroups). */
 qt_status qt_batch_time_kernel(qt_batch* b, int reps, float* ms_per_launch);
 /* complex MACs of one launch, executor shape, flags (1: intermediates in device
  * memory, 2: leaves staged in LDS) and scratch elements per executor */

@@ -147,6 +147,8 @@ struct qt_batch {
     size_t cap_schunks = 0;
     int n_schunks = 0;
     int spec_grid_m = 0, spec_grid_sb = 0;  // > 0: chunks form an edge-major grid (programs x 64-instance blocks)
+    int64_t spec_pb0 = 0, spec_pb_stride = 0;  // parameter base of instance i = pb0 + stride (i mod pb_mod)
+    int spec_pb_mod = 0;                       // 0: read from the instance records
     // Deferred group sums (specialised kernel, strided groups, direct launches): each
     // launch's kernel also sums the previous launch's results, kept in the other result
     // buffer; the last launch's sums run when results are read, polled or flushed.

@@ -356,7 +356,8 @@ std::string generate(const std::vector<ProgView>& progs, bool c128, bool hermiti
     os << "extern \"C\" __global__ __launch_bounds__(64) void qt_batch_specialised(const " << real
        << "* __restrict__ pool, const long long* __restrict__ inst, const int* __restrict__ chunks, int n_chunks,\n"
        << "    double* __restrict__ out, int grid_m, int grid_sb, const double* __restrict__ prev,\n"
-       << "    double* __restrict__ sums, int n_groups, int m, double c0, double c1, int n_sum_blocks) {\n"
+       << "    double* __restrict__ sums, int n_groups, int m, double c0, double c1, int n_sum_blocks,\n"
+       << "    long long pb0, long long pb_stride, int pb_mod) {\n"
        // the first n_sum_blocks workgroups add up the previous launch's results per
        // group (member j of group g = instance j n_groups + g), in the order of the
        // library's strided group-sum kernel: 16 slices j = slice, slice + 16, ..., then
@@ -400,7 +401,10 @@ std::string generate(const std::vector<ProgView>& progs, bool c128, bool hermiti
        << "  if (int(threadIdx.x) >= count) return;\n"
        << "  const int i = first + int(threadIdx.x);\n"
        << "  const int prog = __builtin_amdgcn_readfirstlane(reinterpret_cast<const int*>(inst)[4 * first]);\n"
-       << "  const " << real << "* P = pool + 2 * inst[2 * i + 1];\n"
+       // parameter base: pb0 + pb_stride (i mod pb_mod) when every instance has that
+       // form (angle set i mod n_groups of an edge-major batch), else the record's
+       << "  const long long pbase = pb_mod > 0 ? pb0 + pb_stride * (long long)(i % pb_mod) : inst[2 * i + 1];\n"
+       << "  const " << real << "* P = pool + 2 * pbase;\n"
        << "  double* o = out + 2 * i;\n"
        << "  switch (prog) {\n";
     for (size_t i = 0; i < progs.size(); ++i) os << "    case " << i << ": prog" << i << "(P, o); break;\n";
@@ -563,6 +567,18 @@ qt_status specialised_chunks(qt_batch* b) {
         i = j;
     }
     b->n_schunks = int(ch.size());
+    // parameter bases of the form pb0 + stride (i mod n_groups): computed, not read
+    b->spec_pb_mod = 0;
+    if (b->n_groups > 0 && b->n_inst > 0 && b->n_inst % b->n_groups == 0) {
+        const int64_t pb0 = b->h_pbase[0], stride = b->n_groups > 1 ? b->h_pbase[1] - b->h_pbase[0] : 0;
+        bool affine = true;
+        for (int i = 0; i < b->n_inst && affine; ++i) affine = b->h_pbase[size_t(i)] == pb0 + stride * (i % b->n_groups);
+        if (affine) {
+            b->spec_pb0 = pb0;
+            b->spec_pb_stride = stride;
+            b->spec_pb_mod = b->n_groups;
+        }
+    }
     // an edge-major grid of full chunks (member j of group g = instance j n_groups + g)
     // whose parameter blocks exceed an XCD's L2 (4 MiB): run in the tiled order
     b->spec_grid_m = b->spec_grid_sb = 0;
@@ -641,8 +657,10 @@ qt_status launch_specialised(qt_batch* b, bool defer_sums) {
     int n_chunks = b->n_schunks;
     void* out = b->d_out;
     int grid_m = b->spec_grid_m, grid_sb = b->spec_grid_sb;
-    void* params[] = {&pool, &inst, &chunks, &n_chunks, &out, &grid_m, &grid_sb, &prev,
-                      &sums, &n_groups, &m, &c0, &c1, &n_sum_blocks};
+    long long pb0 = b->spec_pb0, pb_stride = b->spec_pb_stride;
+    int pb_mod = b->spec_pb_mod;
+    void* params[] = {&pool, &inst, &chunks, &n_chunks, &out, &grid_m, &grid_sb, &prev, &sums, &n_groups,
+                      &m, &c0, &c1, &n_sum_blocks, &pb0, &pb_stride, &pb_mod};
     // a multiple of the XCD count (exact for a grid), after the sum workgroups
     const unsigned grid = unsigned(n_sum_blocks + (n_chunks + 7) / 8 * 8);
     return check_hip(c, hipModuleLaunchKernel(b->kernel, grid, 1, 1, 64, 1, 1, 0, c->stream, params, nullptr),
