@@ -41,7 +41,7 @@ FP32_PEAK_TFLOPS = 157.3  # MI355X dense FP32 (vector and matrix)
 PMC_BYTES_PER_NETWORK = (2.0 * 14451.6 + 6544.0) * 1024 / 368640
 # the same for the specialised executor (generated straight-line code, qt_batch_specialised)
 # (as it runs in the timed loop: each launch also sums the previous launch's energies)
-PMC_BYTES_PER_NETWORK_SPECIALISED = (2.0 * 17362.4 + 5824.0) * 1024 / 368640
+PMC_BYTES_PER_NETWORK_SPECIALISED = (2.0 * 14490.2 + 5824.0) * 1024 / 368640
 
 
 def angle_sets(n: int, seed: int = 2024):
