@@ -416,7 +416,12 @@ def main():
     specialised = args.executor == "specialised" and args.form in ("c64_4m", "c128")
     # the light cones' leaves are gate superoperators U (x) U* and Hermitian measurement
     # vectors: conjugate-symmetric, so the generated code computes half of each tensor
-    compile_ms = b.compile(hermitian=not args.no_hermitian) if specialised else None
+    compile_ms, compile_error = None, None
+    if specialised:
+        try:
+            compile_ms = b.compile(hermitian=not args.no_hermitian)
+        except (qa.DeviceFailure, qa.InvalidFunctionInput) as exc:  # the table executor runs
+            specialised, compile_error = False, str(exc)
     if specialised:
         lanes, wgs = 1, 0  # one lane per network, one wavefront per 64 networks
     # direct launches or a captured HIP graph, whichever runs a step faster here
@@ -489,7 +494,7 @@ def main():
             l_f, w_f, _ = bf.autotune()
             ms_f = bf.time(5)
             spec_ms = None
-            if prec == qa.QT_PLAIN_C128:  # complex128 also has generated code
+            if prec == qa.QT_PLAIN_C128 and specialised:  # complex128 also has generated code
                 bf.compile(hermitian=not args.no_hermitian)
                 bf.launch()
                 ctx.sync()
@@ -549,7 +554,8 @@ def main():
                    "executor_lanes": lanes, "wgs_per_cu": wgs,
                    "global_scratch": info["global_scratch"] and not specialised,
                    "staged_leaves": info["staged_leaves"] and not specialised},
-        "compile_ms": compile_ms, "table_executor_ms_per_launch": tables_ms, "hip_graph": graph,
+        "compile_ms": compile_ms, "compile_error": compile_error, "table_executor_ms_per_launch": tables_ms,
+        "hip_graph": graph,
         "launch_ms_direct": direct_ms, "launch_ms_graph": graph_ms,
         "p50_step_ms": 1e3 * float(np.percentile(lat, 50)), "p99_step_ms": 1e3 * float(np.percentile(lat, 99)),
         "accuracy": {"max_edge_rel_err_vs_c128": edge_err, "edges_within_1e-5": within,

@@ -393,10 +393,10 @@ __global__ void k_to_c64(const double2* __restrict__ in, float2* __restrict__ ou
 template <class T>
 qt_status dev_put(Ctx* c, T** dst, size_t* cap, const T* src, size_t n) {
     if (n > *cap) {
-        if (*dst) (void)hipFree(*dst);
+        batch_free(c, *dst);
         *dst = nullptr;
         *cap = 0;
-        qt_status s = check_hip(c, hipMalloc(reinterpret_cast<void**>(dst), n * sizeof(T)), "batch alloc");
+        qt_status s = check_hip(c, batch_malloc(c, reinterpret_cast<void**>(dst), n * sizeof(T)), "batch alloc");
         if (s != QT_OK) return s;
         *cap = n;
     }
@@ -407,11 +407,11 @@ qt_status dev_put(Ctx* c, T** dst, size_t* cap, const T* src, size_t n) {
 template <class T>
 qt_status dev_upload(Ctx* c, T** dst, const T* src, size_t n) {
     if (*dst) {
-        (void)hipFree(*dst);
+        batch_free(c, *dst);
         *dst = nullptr;
     }
     if (n == 0) return QT_OK;
-    qt_status s = check_hip(c, hipMalloc(reinterpret_cast<void**>(dst), n * sizeof(T)), "batch alloc");
+    qt_status s = check_hip(c, batch_malloc(c, reinterpret_cast<void**>(dst), n * sizeof(T)), "batch alloc");
     if (s != QT_OK) return s;
     return check_hip(c, hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice), "batch upload");
 }
@@ -580,10 +580,10 @@ qt_status configure_scratch(qt_batch* b) {
         if (need > (size_t(4) << 30))
             return set_error(c, QT_OOM, "batch: scratch of this executor shape exceeds 4 GiB");
         if (need > b->global_scratch_bytes) {
-            if (b->d_global_scratch) (void)hipFree(b->d_global_scratch);
+            batch_free(c, b->d_global_scratch);
             b->d_global_scratch = nullptr;
             b->global_scratch_bytes = 0;
-            qt_status s = check_hip(c, hipMalloc(&b->d_global_scratch, need), "batch global scratch");
+            qt_status s = check_hip(c, batch_malloc(c, &b->d_global_scratch, need), "batch global scratch");
             if (s != QT_OK) return s;
             b->global_scratch_bytes = need;
         }
@@ -704,16 +704,16 @@ qt_status write_pool(qt_batch* b, int64_t offset, const double* values, int64_t 
     if (need > b->pool_cap) {
         const int64_t cap = std::max(need, b->pool_cap + b->pool_cap / 2);
         void* fresh = nullptr;
-        qt_status s = check_hip(c, hipMalloc(&fresh, size_t(cap) * es), "leaf pool");
+        qt_status s = check_hip(c, batch_malloc(c, &fresh, size_t(cap) * es), "leaf pool");
         if (s != QT_OK) return s;
         if (b->d_pool && b->pool_elems > 0)
             s = check_hip(c, hipMemcpy(fresh, b->d_pool, size_t(b->pool_elems) * es, hipMemcpyDeviceToDevice),
                           "leaf pool grow");
         if (s != QT_OK) {
-            (void)hipFree(fresh);
+            batch_free(c, fresh);
             return s;
         }
-        if (b->d_pool) (void)hipFree(b->d_pool);
+        batch_free(c, b->d_pool);
         b->d_pool = fresh;
         b->pool_cap = cap;
         if (b->graph_exec) (void)configure_scratch(b);  // the captured graph holds the old pool
@@ -724,10 +724,10 @@ qt_status write_pool(qt_batch* b, int64_t offset, const double* values, int64_t 
         s = check_hip(c, hipMemcpy(dst, values, size_t(n) * 16, hipMemcpyHostToDevice), "leaf pool copy");
     } else {
         if (n > b->pool_stage_cap) {
-            if (b->d_pool_stage) (void)hipFree(b->d_pool_stage);
+            batch_free(c, b->d_pool_stage);
             b->d_pool_stage = nullptr;
             b->pool_stage_cap = 0;
-            s = check_hip(c, hipMalloc(&b->d_pool_stage, size_t(n) * 16), "leaf pool staging");
+            s = check_hip(c, batch_malloc(c, &b->d_pool_stage, size_t(n) * 16), "leaf pool staging");
             if (s != QT_OK) return s;
             b->pool_stage_cap = n;
         }
@@ -1015,7 +1015,7 @@ qt_status qt_batch_create(qt_ctx* cx, int n_programs, qt_program* const* progs, 
     auto b = std::make_unique<qt_batch>();
     b->ctx = ctx;
     b->precision = precision;
-    qt_status s = check_hip(ctx, hipMalloc(reinterpret_cast<void**>(&b->d_counter), sizeof(int32_t)), "counter");
+    qt_status s = check_hip(ctx, qt::batch_malloc(ctx, reinterpret_cast<void**>(&b->d_counter), sizeof(int32_t)), "counter");
     if (s == QT_OK) s = check_hip(ctx, hipDeviceGetAttribute(&b->n_cus, hipDeviceAttributeMultiprocessorCount,
                                                               ctx->device), "CU count");
     if (s == QT_OK) s = check_hip(ctx, hipEventCreate(&b->ev0), "event");
@@ -1407,7 +1407,7 @@ qt_status qt_batch_destroy(qt_batch* b) {
                         (void*)b->d_group, b->d_global_scratch, (void*)b->d_chunks, b->d_pool_stage,
                         (void*)b->d_leaf_lds, (void*)b->d_csts, (void*)b->d_tout, (void*)b->d_schunks,
                         (void*)b->d_out_prev})
-            if (p) (void)hipFree(p);
+            qt::batch_free(b->ctx, p);
         --b->ctx->live;
     }
     delete b;

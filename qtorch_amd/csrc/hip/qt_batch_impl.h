@@ -158,6 +158,20 @@ struct qt_batch {
 };
 
 namespace qt {
+// Device arrays of a batch come from its context's caching allocator: no hipMalloc /
+// hipFree (which synchronises the whole device) once the cache is warm.  A block is
+// returned after the context's stream has drained, so work in flight never sees it
+// reused.
+inline hipError_t batch_malloc(Ctx* c, void** p, size_t bytes) {
+    hipError_t e = hipSuccess;
+    *p = c->pool.alloc(bytes, &e);
+    return *p ? hipSuccess : (e == hipSuccess ? hipErrorOutOfMemory : e);
+}
+inline void batch_free(Ctx* c, void* p) {
+    if (!p) return;
+    (void)hipStreamSynchronize(c->stream);
+    c->pool.release(p);
+}
 // specialised executors (qt_codegen.hip)
 qt_status specialised_chunks(qt_batch* b);
 void drop_specialised(qt_batch* b);

@@ -596,10 +596,10 @@ qt_status specialised_chunks(qt_batch* b) {
     }
     if (ch.empty()) return QT_OK;
     if (ch.size() > b->cap_schunks) {
-        if (b->d_schunks) (void)hipFree(b->d_schunks);
+        batch_free(b->ctx, b->d_schunks);
         b->d_schunks = nullptr;
         b->cap_schunks = 0;
-        qt_status s = check_hip(b->ctx, hipMalloc(reinterpret_cast<void**>(&b->d_schunks), ch.size() * sizeof(int2)),
+        qt_status s = check_hip(b->ctx, batch_malloc(b->ctx, reinterpret_cast<void**>(&b->d_schunks), ch.size() * sizeof(int2)),
                                 "specialised chunks");
         if (s != QT_OK) return s;
         b->cap_schunks = ch.size();
