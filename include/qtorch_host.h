@@ -144,7 +144,8 @@ int qth_netbatch_destroy(qth_netbatch* b);
  * Requests are submitted from any thread and finish asynchronously.  Each request's
  * network is planned once per structure and then runs as a program instance in the
  * next launch of an idle stream (or, when its tensors exceed a program's limit, on
- * the drop-in complex128 path), admitted while the planned device bytes of the
+ * the large-network path in complex128: sliced, small subtrees fused, MFMA GEMMs),
+ * admitted while the planned device bytes of the
  * requests in flight fit hbm_budget.  Results do not depend on what else shares a
  * launch. */
 typedef struct qth_server qth_server;

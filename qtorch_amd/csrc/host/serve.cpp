@@ -10,9 +10,11 @@
 // their planned device bytes fit the budget, writes their parameters, and launches
 // them; requests that arrive meanwhile go to the next idle stream, so the work of
 // consecutive launches overlaps on the device.  Networks too large for a program
-// (tensors above 2^16 elements) run on a separate thread through the drop-in
-// complex128 path.  Every request's result is computed by the same program, in
-// the same order, whatever else shares its launch.
+// (tensors above 2^16 elements) run on a separate thread and context through the
+// large-network path in complex128 (sliced to a bound on index bits, small subtrees
+// fused into one batch launch, MFMA GEMMs for the large steps).  Every request's
+// result is computed by the same program, in the same order, whatever else shares
+// its launch.
 #include "serve.hpp"
 
 #include <algorithm>
@@ -21,6 +23,7 @@
 
 #include "batch.hpp"
 #include "qtorch_host.h"
+#include "sliced.hpp"
 
 namespace qtorch {
 
@@ -35,6 +38,9 @@ struct Server::Structure {
     int64_t scratch_elems = 0;            // largest program's
     int64_t cmacs = 0;                    // per request
     int64_t planned_bytes = 0;            // per request
+    // large-network route: each network planned once (line-graph order, sliced, small
+    // subtrees fused); a request copies it and sets its own leaf values
+    std::vector<std::shared_ptr<SlicedNetwork>> eager_nets;
     std::string error;                    // set when the request cannot run
     ~Structure() {
         for (auto* p : progs) qt_program_destroy(p);
@@ -54,31 +60,6 @@ namespace {
 constexpr int kBatchClasses = 2;  // small, medium
 
 int64_t ElemBytes(int precision) { return precision == QT_PLAIN_C128 ? 16 : 8; }
-
-// Peak device bytes of a recorded plan run by the drop-in path: every leaf
-// uploaded, intermediates live from their step to the step that consumes them.
-int64_t PlanPeakBytes(const Network& net) {
-    const int dl = net.GetDimLog2();
-    const int n_orig = net.GetNumOriginalNodes();
-    int64_t leaves = 0;
-    for (int i = 0; i < n_orig; ++i) leaves += int64_t(16) << (dl * net.GetAllNodes()[size_t(i)]->mRank);
-    int64_t live = 0, peak = 0;
-    std::unordered_map<int, int64_t> size;
-    for (const auto& s : net.GetPlan()) {
-        const int64_t c = int64_t(16) << (dl * s.rank_c);
-        live += c;
-        peak = std::max(peak, live);
-        for (int id : {s.a_id, s.b_id}) {
-            auto it = size.find(id);
-            if (it != size.end()) {
-                live -= it->second;
-                size.erase(it);
-            }
-        }
-        if (s.c_id >= 0) size[s.c_id] = c;
-    }
-    return leaves + peak;
-}
 
 std::shared_ptr<Server::Structure> BuildStructure(const ServeRequest& r, const ServeOptions& opt) {
     auto st = std::make_shared<Server::Structure>();
@@ -102,16 +83,18 @@ std::shared_ptr<Server::Structure> BuildStructure(const ServeRequest& r, const S
                 // LDS (small route) or in the batch's fixed scratch area (medium route)
                 st->planned_bytes = st->param_elems * es + 32 * int64_t(st->progs.size());
             } catch (const DeviceFailure&) {  // a light cone's tensors exceed a program's limit
+                // the large-network path: each edge network planned once (line-graph
+                // order, sliced to opt.eager_max_bits), its small subtrees fused
                 st->route = kRouteEager;
                 int64_t peak = 0;
                 for (size_t e = 0; e < st->data.pairs.size(); ++e) {
                     const std::string q = QaoaEdgeQasm(st->data, int(e), std::vector<double>(size_t(2 * r.p), 0.5));
                     const std::string m = QaoaEdgeMeasure(st->data, int(e));
-                    NetworkOptions o;
-                    Network net(q, &m, o);
-                    GreedyContract(net);
-                    peak = std::max(peak, PlanPeakBytes(net));
-                    st->cmacs += net.getNumFloatOps();
+                    auto sn = std::make_shared<SlicedNetwork>(q, m, 2, kOrderLineGraph, 0, opt.eager_max_bits);
+                    peak = std::max(peak, sn->Plan().peak_bytes);
+                    st->cmacs += sn->Plan().cmacs_per_slice * sn->Plan().slices;
+                    st->eager_nets.push_back(std::move(sn));
+                    if (st->cmacs > opt.max_request_cmacs) break;  // refused at admission anyway
                 }
                 st->planned_bytes = peak;
             }
@@ -148,8 +131,11 @@ std::shared_ptr<Server::Structure> BuildStructure(const ServeRequest& r, const S
                 st->route = scratch <= opt.small_scratch ? kRouteSmall : kRouteMedium;
                 st->planned_bytes = st->param_elems * es + 32;
             } catch (const DeviceFailure&) {
-                st->route = kRouteEager;
-                st->planned_bytes = PlanPeakBytes(net);
+                st->route = kRouteEager;  // the large-network path, line-graph order, sliced
+                auto sn = std::make_shared<SlicedNetwork>(r.text, zeros, 1, kOrderLineGraph, 0, opt.eager_max_bits);
+                st->planned_bytes = sn->Plan().peak_bytes;
+                st->cmacs = sn->Plan().cmacs_per_slice * sn->Plan().slices;
+                st->eager_nets.push_back(std::move(sn));
             }
         } else {
             throw InvalidFunctionInput();
@@ -277,11 +263,15 @@ int64_t Server::Submit(ServeRequest r) {
     p->id = next_id_++;
     ++stats_.submitted;
     ++outstanding_;
-    if (!p->st || !p->st->error.empty() || p->st->planned_bytes > opt_.hbm_budget) {
+    const bool too_big = p->st && (p->st->planned_bytes > opt_.hbm_budget ||
+                                   (p->st->route == kRouteEager && p->st->cmacs > opt_.max_request_cmacs));
+    if (!p->st || !p->st->error.empty() || too_big) {
         ServeResult res;
         res.status = !p->st ? QTH_INVALID_FUNCTION_INPUT : !p->st->error.empty() ? StatusOf(p->st->error) : QTH_OTHER;
-        res.error = !p->st ? "bad angles or bitstring"
-                           : !p->st->error.empty() ? p->st->error : "planned device memory exceeds the budget";
+        res.error = !p->st                   ? "bad angles or bitstring"
+                    : !p->st->error.empty()  ? p->st->error
+                    : p->st->planned_bytes > opt_.hbm_budget ? "planned device memory exceeds the budget"
+                                                             : "planned work exceeds the per-request limit";
         if (p->st) res.planned_bytes = p->st->planned_bytes;
         ++stats_.rejected;
         lk.unlock();
@@ -550,25 +540,21 @@ void Server::EagerLoop() {
         res.route = kRouteEager;
         res.planned_bytes = p->st->planned_bytes;
         try {
+            // the large-network path in complex128: fused small subtrees, MFMA GEMMs for
+            // the large steps, FP64 sums of slices
             if (p->req.kind == kServeQaoaEnergy) {
                 double f = 0.0;
                 for (size_t e = 0; e < p->st->data.pairs.size(); ++e) {
-                    const std::string q = QaoaEdgeQasm(p->st->data, int(e), p->req.angles);
-                    const std::string m = QaoaEdgeMeasure(p->st->data, int(e));
-                    NetworkOptions o;
-                    o.ctx = eager_ctx_;
-                    Network net(q, &m, o);
-                    GreedyContract(net);
-                    f += 0.5 * (1.0 - net.GetFinalValue().real());  // src/maxcut.cpp:196
+                    SlicedNetwork sn(*p->st->eager_nets[e]);  // the plan; this request's angles
+                    sn.SetLeafValues(QaoaEdgeQasm(p->st->data, int(e), p->req.angles),
+                                     QaoaEdgeMeasure(p->st->data, int(e)));
+                    f += 0.5 * (1.0 - sn.Run(eager_ctx_, QT_PLAIN_C128, nullptr).real());  // src/maxcut.cpp:196
                 }
                 res.re = f;
             } else {
-                NetworkOptions o;
-                o.ctx = eager_ctx_;
-                o.dim_log2 = 1;
-                Network net(p->req.text, &p->req.bits, o);
-                GreedyContract(net);
-                const auto v = net.GetFinalValue();
+                SlicedNetwork sn(*p->st->eager_nets[0]);
+                sn.SetLeafValues(p->req.text, p->req.bits);
+                const auto v = sn.Run(eager_ctx_, QT_PLAIN_C128, nullptr);
                 res.re = v.real();
                 res.im = v.imag();
             }

@@ -27,6 +27,8 @@ struct ServeOptions {
     int64_t small_scratch = 768;        // elements: programs up to this run in lane-grouped waves (LDS)
     int small_lanes = 8, small_wgs = 4; // executor shape of the small route
     int64_t param_capacity = 1 << 22;   // parameter elements per slot and launch
+    int eager_max_bits = 28;            // large-network path: index bits per tensor after slicing
+    int64_t max_request_cmacs = int64_t(1) << 44;  // large-network path: planned complex MACs of a request
 };
 
 enum ServeKind { kServeQaoaEnergy = 0, kServeAmplitude = 1 };

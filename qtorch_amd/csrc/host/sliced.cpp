@@ -217,6 +217,7 @@ void SlicedNetwork::PlanFusion() {
             if (it != local.end()) return it->second;
             const int64_t elems = int64_t(1) << (leaf_rank_[size_t(op)] * dl_);
             if (cut_[size_t(op)].empty()) {
+                const_leaves_.push_back({op, int64_t(const_pool_.size() / 2)});
                 leaf_refs.push_back(int64_t(const_pool_.size() / 2));
                 for (const auto& v : leaf_vals_[size_t(op)]) {
                     const_pool_.push_back(v.real());
@@ -255,6 +256,29 @@ void SlicedNetwork::PlanFusion() {
     }
     plan_.fused_subtrees = int(roots_.size());
     plan_.fused_steps = fused;
+}
+
+void SlicedNetwork::SetLeafValues(const std::string& qasm, const std::string& measure) {
+    NetworkOptions o;
+    o.dim_log2 = dl_;
+    Network net(qasm, &measure, o);
+    if (net.GetNumOriginalNodes() != n_leaves_) throw InvalidFunctionInput();
+    std::vector<std::vector<std::complex<double>>> vals;
+    for (int i = 0; i < n_leaves_; ++i) {
+        const auto& node = net.GetAllNodes()[size_t(i)];
+        if (node->mRank != int(leaf_wires_[size_t(i)].size()) ||
+            node->mHostVals.size() != leaf_vals_[size_t(i)].size())
+            throw InvalidFunctionInput();
+        vals.push_back(node->mHostVals);
+    }
+    leaf_vals_ = std::move(vals);
+    for (const auto& [leaf, off] : const_leaves_) {
+        const auto& v = leaf_vals_[size_t(leaf)];
+        for (size_t e = 0; e < v.size(); ++e) {
+            const_pool_[size_t(2 * (off + int64_t(e)))] = v[e].real();
+            const_pool_[size_t(2 * (off + int64_t(e)) + 1)] = v[e].imag();
+        }
+    }
 }
 
 void SlicedNetwork::Extract(int leaf, int64_t slice, double* dst) const {

@@ -38,6 +38,10 @@ public:
     // slices_per_launch = 0: every step through qt_contract_pair; < 0 (default): fused
     // subtrees, slices per launch chosen by memory; > 0: at most that many per launch.
     void SetFusion(int64_t slices_per_launch) { fuse_slices_ = slices_per_launch; }
+    // New leaf values for the same network structure (the plan, slicing and fused
+    // programs depend on the structure only): the nodes of (qasm, measure) parsed as
+    // in the constructor must have the ranks of this network's leaves.
+    void SetLeafValues(const std::string& qasm, const std::string& measure);
 
 private:
     struct Step {
@@ -65,6 +69,7 @@ private:
     std::vector<int> roots_;                        // root step of each subtree
     std::vector<std::shared_ptr<qt_program>> progs_;  // one per root
     std::vector<double> const_pool_;                // FP64 values of their uncut leaves
+    std::vector<std::pair<int, int64_t>> const_leaves_;  // (uncut leaf, its offset in const_pool_)
     std::vector<std::pair<int, int64_t>> param_leaves_;  // (cut leaf, offset in a slice's block)
     int64_t block_elems_ = 0;                       // leaf elements of one slice's block
     int64_t root_bytes_c128_ = 0;                   // subtree results of one slice, complex128

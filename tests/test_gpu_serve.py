@@ -129,10 +129,11 @@ def test_served_values_match_references(ctx):
         srv.close()
 
 
-def test_large_network_takes_the_drop_in_path():
+def test_large_network_takes_the_large_network_path():
     """A 20-qubit depth-14 grid amplitude has tensors beyond a program's limit: it runs
-    on the complex128 path (matching a state-vector simulation to 1e-10) while small
-    requests keep flowing, and its planned bytes are counted at admission."""
+    on the complex128 large-network path (sliced, fused small subtrees; matching a
+    state-vector simulation to 1e-10) while small requests keep flowing, and its
+    planned bytes are counted at admission."""
     q = wl.grid_random_circuit(4, 5, 14, seed=1)
     bits = wl.bitstring(20, 3)
     plan = qa.serve_plan(qa.SERVE_AMPLITUDE, q, bits=bits)
@@ -171,3 +172,41 @@ def test_admission_rejects_what_cannot_fit():
     assert r1["status"] == 9 and r2["status"] == 0
     assert bad["status"] != 0
     assert st["rejected"] == 2
+
+
+@pytest.mark.parametrize("n,d,p", [(8, 3, 3), (10, 4, 2)])
+def test_large_light_cones_served_exactly(n, d, p):
+    """p = 3 and 4-regular p = 2 light cones exceed a program's tensors: they run on the
+    large-network path (planned once per structure, each request's angles set into the
+    plan) in complex128; the energy matches a state-vector simulation of every edge's
+    light-cone circuit to 1e-10, for two angle sets in a row."""
+    dgf = wl.dgf_text(wl.regular_graph(n, d, seed=31))
+    assert qa.serve_plan(qa.SERVE_QAOA_ENERGY, dgf, p)["route"] == "eager"
+    ne, _ = qa.qaoa_num_edges(dgf)
+    srv = qa.Server(slots=1)
+    try:
+        for s in (1, 2):
+            ang = wl.qaoa_angles(p, s)
+            res = srv.wait(srv.submit_qaoa(dgf, p, ang))
+            assert res["status"] == 0 and res["route"] == "eager", res
+            ref = 0.0
+            for e in range(ne):
+                q, m = qa.qaoa_edge_text(dgf, p, False, e, ang)
+                psi, nq = sv.simulate(q)
+                ref += 0.5 * (1.0 - sv.expectation(psi, nq, m).real)
+            assert abs(res["value"].real - ref) <= 1e-10 * abs(ref), (res, ref)
+    finally:
+        srv.close()
+
+
+def test_admission_rejects_too_much_work():
+    """A request whose planned work exceeds the per-request limit (here p = 3 light
+    cones of a 20-vertex graph, about 1e18 complex MACs) is refused without running."""
+    dgf = wl.dgf_text(wl.regular_graph(20, 3, seed=920))
+    srv = qa.Server(slots=1)
+    try:
+        res = srv.wait(srv.submit_qaoa(dgf, 3, wl.qaoa_angles(3, 1)))
+        st = srv.stats()
+    finally:
+        srv.close()
+    assert res["status"] == 9 and st["rejected"] == 1

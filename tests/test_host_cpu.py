@@ -214,8 +214,9 @@ def test_program_schedule_keeps_scratch_small():
 def test_serving_plan_routes_and_bytes():
     """Serving plans each request on the host: small light cones run in lane-grouped
     waves, larger programs in workgroup executors, networks with tensors beyond a
-    program's 2^16 elements on the drop-in path, whose planned bytes cover at least
-    its largest intermediate (complex128)."""
+    program's 2^16 elements on the large-network path (line-graph order, sliced),
+    whose planned bytes cover at least its largest intermediate (complex128) and whose
+    MAC count is the plan's."""
     from qtorch_amd import workloads as wl
     with open(os.path.join(GOLDEN_DIR, "graphs", "3reg_n30_s2024.dgf")) as f:
         dgf = f.read()
@@ -227,7 +228,7 @@ def test_serving_plan_routes_and_bytes():
     bits = wl.bitstring(20, 3)
     big = qa.serve_plan(qa.SERVE_AMPLITUDE, q, bits=bits)
     net = qa.Network(q, bits, None, dim_log2=1)
-    net.order(qa.ORDER_GREEDY, 0)
+    net.order(qa.ORDER_LINEGRAPH, 0)
     peak_rank = int(net.plan()[:, 6].max())
     assert peak_rank > 16 and big["route"] == "eager"
     assert big["planned_bytes"] >= 16 * 2 ** peak_rank and big["cmacs"] == net.flops()
