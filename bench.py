@@ -270,11 +270,18 @@ def run_uneven(args, qa, np):
 
 
 def run_serving(args, qa, np):
-    """Poisson arrivals at a fixed rate: 70% QAOA energies (3-regular graphs with
-    p = 1 or 2, 4-regular with p = 1; 20-200 vertices) and 30% single amplitudes of
-    grid random circuits (16-24 qubits, depth 8), from a pool of structures planned
-    before the timed region; requests run in the continuous-batching server (complex64
-    4M).  Latency = submission to result."""
+    """Poisson arrivals at a fixed rate, from a pool of structures planned before the
+    timed region; requests run in the continuous-batching server (complex64 4M for
+    the batched routes, complex128 on the large-network route).  Latency = submission
+    to result.
+    * mix "light": 70% QAOA energies (3-regular graphs with p = 1 or 2, 4-regular with
+      p = 1; 20-200 vertices) and 30% single amplitudes of grid random circuits (16-24
+      qubits, depth 8): every request fits a batched program.
+    * mix "config": 70% QAOA energies on 3- and 4-regular graphs of 20-200 vertices with
+      p = 1, 2 (45 %, 45 %) and 3 (10 %), 30% amplitudes of grid circuits of 24-48
+      qubits at depth 8 and 14; light cones and circuits beyond a program's tensors run
+      on the large-network route, and requests whose planned work exceeds the
+      per-request limit (p = 3 on these graphs, 30+ qubits at depth 14) are refused."""
     from qtorch_amd import workloads as wl
     rng = random.Random(args.seed)
     graphs = []
@@ -282,20 +289,29 @@ def run_serving(args, qa, np):
         d = 3 if i % 3 else 4
         n = int(round(math.exp(rng.uniform(math.log(20), math.log(200)))))
         n += (n * d) % 2
-        graphs.append((wl.dgf_text(wl.regular_graph(n, d, seed=500 + i)), d))
-    circuits = [(wl.grid_random_circuit(r, c, 8, seed=600 + i), r * c) for i, (r, c) in
-                enumerate([(4, 4), (4, 5), (4, 6), (3, 6)])]
+        graphs.append((wl.dgf_text(wl.regular_graph(n, d, seed=500 + i)), d, n))
+    if args.mix == "config":
+        shapes = [(4, 6, 8), (5, 6, 8), (6, 6, 8), (6, 8, 8), (4, 6, 14), (6, 8, 14)]
+    else:
+        shapes = [(4, 4, 8), (4, 5, 8), (4, 6, 8), (3, 6, 8)]
+    circuits = [(wl.grid_random_circuit(r, c, dep, seed=600 + i), r * c, f"amplitude {r}x{c} d{dep}")
+                for i, (r, c, dep) in enumerate(shapes)]
     reqs = []
     n_req = int(args.qps * args.duration)
     for _ in range(n_req):
         if rng.random() < 0.7:
-            dgf, d = rng.choice(graphs)
-            p = rng.choice([1, 2]) if d == 3 else 1
-            reqs.append(("qaoa", dgf, p, wl.qaoa_angles(p, rng.randrange(1 << 30))))
+            dgf, d, n = rng.choice(graphs)
+            if args.mix == "config":
+                p = rng.choices([1, 2, 3], weights=[45, 45, 10])[0]
+            else:
+                p = rng.choice([1, 2]) if d == 3 else 1
+            reqs.append(("qaoa", dgf, p, wl.qaoa_angles(p, rng.randrange(1 << 30)), f"qaoa {d}-regular p={p}"))
         else:
-            q, n = rng.choice(circuits)
-            reqs.append(("amp", q, wl.bitstring(n, rng.randrange(1 << 30))))
-    srv = qa.Server(slots=2, precision=qa.QT_C64_4M)
+            q, n, label = rng.choice(circuits)
+            reqs.append(("amp", q, wl.bitstring(n, rng.randrange(1 << 30)), None, label))
+    # 64 GiB of the 288 GiB HBM for the requests admitted at once: large light cones
+    # (8-9 GiB of live tensors per edge network) run up to four edge networks side by side
+    srv = qa.Server(slots=2, precision=qa.QT_C64_4M, hbm_budget=64 << 30)
 
     def submit(r):
         return srv.submit_qaoa(r[1], r[2], r[3]) if r[0] == "qaoa" else srv.submit_amplitude(r[1], r[2])
@@ -306,7 +322,10 @@ def run_serving(args, qa, np):
         key = (r[0], r[1], r[2] if r[0] == "qaoa" else 0)
         if key not in seen:
             seen.add(key)
-            srv.wait(submit(r))
+            t = time.perf_counter()
+            w = srv.wait(submit(r))
+            print(f"structure {len(seen)}: {r[4]} route {w['route']} status {w['status']} "
+                  f"{1e3 * (time.perf_counter() - t):.1f} ms", file=sys.stderr, flush=True)
     base = srv.stats()
     arrivals = np.cumsum(np.random.default_rng(args.seed).exponential(1.0 / args.qps, n_req))
     ids = [0] * n_req
@@ -316,16 +335,33 @@ def run_serving(args, qa, np):
         if delay > 0:
             time.sleep(delay)
         ids[k] = submit(reqs[k])
-    res = [srv.wait(i) for i in ids]
+    print(f"submitted {n_req} requests in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    res = []
+    for k, i in enumerate(ids):
+        res.append(srv.wait(i))
+        if (k + 1) % max(n_req // 10, 1) == 0:
+            print(f"{k + 1} results at {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0
     st = srv.stats()
     srv.close()
-    lat = np.array([r["latency_ms"] for r in res])
+    lat = np.array([r["latency_ms"] for r in res if r["status"] == 0])
     ok = sum(1 for r in res if r["status"] == 0)
-    routes = {k: sum(1 for r in res if r["route"] == k) for k in ("small", "medium", "eager")}
+    routes = {k: sum(1 for r in res if r["route"] == k and r["status"] == 0) for k in ("small", "medium", "eager")}
+    classes = {}
+    for r, q in zip(res, reqs):
+        c = classes.setdefault(q[4], {"requests": 0, "refused": 0, "lat": []})
+        c["requests"] += 1
+        if r["status"] == 0:
+            c["lat"].append(r["latency_ms"])
+        else:
+            c["refused"] += 1
+    for c in classes.values():
+        lt = c.pop("lat")
+        c["p50_ms"] = float(np.percentile(lt, 50)) if lt else None
+        c["p99_ms"] = float(np.percentile(lt, 99)) if lt else None
     # the FP64 plain path on a sample of the same requests
     ctx = qa.Context(0)
-    sample = [k for k in range(n_req) if reqs[k][0] == "qaoa"][:10]
+    sample = [k for k in range(n_req) if reqs[k][0] == "qaoa" and res[k]["status"] == 0 and reqs[k][2] == 1][:10]
     t1 = time.perf_counter()
     errs = []
     for k in sample:
@@ -336,10 +372,11 @@ def run_serving(args, qa, np):
     return {"metric": "serving requests/s (p50/p99 latency) under Poisson arrivals", "value": ok / wall,
             "unit": "requests/s", "n_gpus": 1, "steps": n_req, "warmup": len(seen), "ms_per_step": 1e3 * wall / n_req,
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "c64", "data": "synthetic",
-            "config": {"workload": "serving_mix_qaoa70_amplitude30", "qps": args.qps, "duration_s": args.duration,
-                       "slots": 2, "structures": len(seen)},
+            "config": {"workload": "serving_mix_qaoa70_amplitude30", "mix": args.mix, "qps": args.qps,
+                       "duration_s": args.duration, "slots": 2, "hbm_budget_gib": 64, "structures": len(seen)},
             "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-            "completed": ok, "routes": routes, "edge_evals_per_s": edge_evals / wall,
+            "completed": ok, "refused": n_req - ok, "routes": routes, "classes": classes,
+            "edge_evals_per_s": edge_evals / wall,
             "launches": st["launches"] - base["launches"], "overlap_ms": st["overlap_ms"] - base["overlap_ms"],
             "busy_ms": st["busy_ms"] - base["busy_ms"], "peak_planned_bytes": st["peak_planned_bytes"],
             "plain_c128_ms_per_qaoa_request": plain_ms, "max_energy_rel_err_vs_plain_c128": max(errs) if errs else None}
@@ -368,6 +405,7 @@ def main():
     ap.add_argument("--networks", type=int, default=512, help="uneven: edge networks")
     ap.add_argument("--streams", type=int, default=4, help="uneven: host threads / streams for the large networks")
     ap.add_argument("--qps", type=float, default=400.0, help="serving: arrival rate")
+    ap.add_argument("--mix", default="light", choices=["light", "config"], help="serving: request mix")
     ap.add_argument("--duration", type=float, default=10.0, help="serving: seconds of arrivals")
     args = ap.parse_args()
     if args.gpus != 1:

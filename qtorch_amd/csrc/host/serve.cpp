@@ -18,7 +18,9 @@
 #include "serve.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <exception>
 #include <sstream>
 
 #include "batch.hpp"
@@ -41,6 +43,7 @@ struct Server::Structure {
     // large-network route: each network planned once (line-graph order, sliced, small
     // subtrees fused); a request copies it and sets its own leaf values
     std::vector<std::shared_ptr<SlicedNetwork>> eager_nets;
+    int eager_workers = 1;                // edge networks of a request run side by side
     std::string error;                    // set when the request cannot run
     ~Structure() {
         for (auto* p : progs) qt_program_destroy(p);
@@ -60,6 +63,13 @@ namespace {
 constexpr int kBatchClasses = 2;  // small, medium
 
 int64_t ElemBytes(int precision) { return precision == QT_PLAIN_C128 ? 16 : 8; }
+
+// a + b * c, saturating at INT64_MAX (planned work of very large networks overflows)
+int64_t SatMulAdd(int64_t a, int64_t b, int64_t c) {
+    int64_t bc = 0, r = 0;
+    if (__builtin_mul_overflow(b, c, &bc) || __builtin_add_overflow(a, bc, &r)) return INT64_MAX;
+    return r;
+}
 
 std::shared_ptr<Server::Structure> BuildStructure(const ServeRequest& r, const ServeOptions& opt) {
     auto st = std::make_shared<Server::Structure>();
@@ -92,11 +102,16 @@ std::shared_ptr<Server::Structure> BuildStructure(const ServeRequest& r, const S
                     const std::string m = QaoaEdgeMeasure(st->data, int(e));
                     auto sn = std::make_shared<SlicedNetwork>(q, m, 2, kOrderLineGraph, 0, opt.eager_max_bits);
                     peak = std::max(peak, sn->Plan().peak_bytes);
-                    st->cmacs += sn->Plan().cmacs_per_slice * sn->Plan().slices;
+                    st->cmacs = SatMulAdd(st->cmacs, sn->Plan().cmacs_per_slice, sn->Plan().slices);
                     st->eager_nets.push_back(std::move(sn));
                     if (st->cmacs > opt.max_request_cmacs) break;  // refused at admission anyway
                 }
-                st->planned_bytes = peak;
+                // edge networks run on up to eager_workers streams at once, as many as
+                // the budget holds (at least one)
+                const int64_t fit = peak > 0 ? opt.hbm_budget / peak : int64_t(opt.eager_workers);
+                st->eager_workers = int(std::max<int64_t>(
+                    1, std::min<int64_t>({fit, int64_t(opt.eager_workers), int64_t(st->eager_nets.size())})));
+                st->planned_bytes = SatMulAdd(0, peak, st->eager_workers);
             }
         } else if (r.kind == kServeAmplitude) {
             st->text = r.text;
@@ -134,7 +149,7 @@ std::shared_ptr<Server::Structure> BuildStructure(const ServeRequest& r, const S
                 st->route = kRouteEager;  // the large-network path, line-graph order, sliced
                 auto sn = std::make_shared<SlicedNetwork>(r.text, zeros, 1, kOrderLineGraph, 0, opt.eager_max_bits);
                 st->planned_bytes = sn->Plan().peak_bytes;
-                st->cmacs = sn->Plan().cmacs_per_slice * sn->Plan().slices;
+                st->cmacs = SatMulAdd(0, sn->Plan().cmacs_per_slice, sn->Plan().slices);
                 st->eager_nets.push_back(std::move(sn));
             }
         } else {
@@ -175,7 +190,8 @@ struct Server::Slot {
 };
 
 Server::Server(const ServeOptions& opt) : opt_(opt), t0_(Clock::now()) {
-    if (opt_.slots < 1 || opt_.max_requests_per_launch < 1 || opt_.hbm_budget < 1 || opt_.param_capacity < 64)
+    if (opt_.slots < 1 || opt_.max_requests_per_launch < 1 || opt_.hbm_budget < 1 || opt_.param_capacity < 64 ||
+        opt_.eager_workers < 1 || opt_.eager_workers > 16)
         throw InvalidFunctionInput();
     try {
         for (int i = 0; i < opt_.slots; ++i) {
@@ -192,14 +208,18 @@ Server::Server(const ServeOptions& opt) : opt_(opt), t0_(Clock::now()) {
             }
             slots_.push_back(std::move(s));
         }
-        if (qt_ctx_create(opt_.device, &eager_ctx_) != QT_OK) throw DeviceFailure("serve: no device context");
+        for (int w = 0; w < opt_.eager_workers; ++w) {
+            qt_ctx* c = nullptr;
+            if (qt_ctx_create(opt_.device, &c) != QT_OK) throw DeviceFailure("serve: no device context");
+            eager_ctxs_.push_back(c);
+        }
     } catch (...) {
         for (auto& s : slots_) {
             for (auto* b : s->batch)
                 if (b) qt_batch_destroy(b);
             if (s->ctx) qt_ctx_destroy(s->ctx);
         }
-        if (eager_ctx_) qt_ctx_destroy(eager_ctx_);
+        for (auto* c : eager_ctxs_) qt_ctx_destroy(c);
         throw;
     }
     loop_ = std::thread([this] { Loop(); });
@@ -219,7 +239,7 @@ Server::~Server() {
             if (b) qt_batch_destroy(b);
         qt_ctx_destroy(s->ctx);
     }
-    qt_ctx_destroy(eager_ctx_);
+    for (auto* c : eager_ctxs_) qt_ctx_destroy(c);
 }
 
 std::shared_ptr<Server::Structure> Server::GetStructure(const ServeRequest& r) {
@@ -543,18 +563,41 @@ void Server::EagerLoop() {
             // the large-network path in complex128: fused small subtrees, MFMA GEMMs for
             // the large steps, FP64 sums of slices
             if (p->req.kind == kServeQaoaEnergy) {
+                // edge networks on eager_workers threads, one stream each; the per-edge
+                // values are summed in edge order afterwards, so the energy does not
+                // depend on which worker ran which edge
+                const int n_edges = int(p->st->data.pairs.size());
+                std::vector<double> zz(size_t(n_edges), 0.0);
+                std::atomic<int> next{0};
+                std::vector<std::exception_ptr> errs(eager_ctxs_.size());
+                auto work = [&](size_t w) {
+                    try {
+                        for (int e = next++; e < n_edges; e = next++) {
+                            SlicedNetwork sn(*p->st->eager_nets[size_t(e)]);  // the plan; this request's angles
+                            sn.SetLeafValues(QaoaEdgeQasm(p->st->data, e, p->req.angles),
+                                             QaoaEdgeMeasure(p->st->data, e));
+                            zz[size_t(e)] = sn.Run(eager_ctxs_[w], QT_PLAIN_C128, nullptr).real();
+                        }
+                    } catch (...) {
+                        errs[w] = std::current_exception();
+                        next = n_edges;
+                    }
+                };
+                std::vector<std::thread> pool;
+                const size_t n_workers =
+                    std::min({eager_ctxs_.size(), size_t(n_edges), size_t(p->st->eager_workers)});
+                for (size_t w = 1; w < n_workers; ++w) pool.emplace_back(work, w);
+                work(0);
+                for (auto& t : pool) t.join();
+                for (auto& ep : errs)
+                    if (ep) std::rethrow_exception(ep);
                 double f = 0.0;
-                for (size_t e = 0; e < p->st->data.pairs.size(); ++e) {
-                    SlicedNetwork sn(*p->st->eager_nets[e]);  // the plan; this request's angles
-                    sn.SetLeafValues(QaoaEdgeQasm(p->st->data, int(e), p->req.angles),
-                                     QaoaEdgeMeasure(p->st->data, int(e)));
-                    f += 0.5 * (1.0 - sn.Run(eager_ctx_, QT_PLAIN_C128, nullptr).real());  // src/maxcut.cpp:196
-                }
+                for (double v : zz) f += 0.5 * (1.0 - v);  // src/maxcut.cpp:196
                 res.re = f;
             } else {
                 SlicedNetwork sn(*p->st->eager_nets[0]);
                 sn.SetLeafValues(p->req.text, p->req.bits);
-                const auto v = sn.Run(eager_ctx_, QT_PLAIN_C128, nullptr);
+                const auto v = sn.Run(eager_ctxs_[0], QT_PLAIN_C128, nullptr);
                 res.re = v.real();
                 res.im = v.imag();
             }

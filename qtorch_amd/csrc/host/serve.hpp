@@ -29,6 +29,7 @@ struct ServeOptions {
     int64_t param_capacity = 1 << 22;   // parameter elements per slot and launch
     int eager_max_bits = 28;            // large-network path: index bits per tensor after slicing
     int64_t max_request_cmacs = int64_t(1) << 44;  // large-network path: planned complex MACs of a request
+    int eager_workers = 4;              // large-network path: edge networks of a request run side by side
 };
 
 enum ServeKind { kServeQaoaEnergy = 0, kServeAmplitude = 1 };
@@ -104,7 +105,7 @@ private:
     std::map<std::string, std::shared_ptr<Structure>> structures_;
     std::mutex struct_mu_;
     std::vector<std::unique_ptr<Slot>> slots_;
-    qt_ctx* eager_ctx_ = nullptr;
+    std::vector<qt_ctx*> eager_ctxs_;  // one stream per large-network worker
     std::vector<double> latencies_;
     int64_t next_id_ = 1, in_flight_bytes_ = 0, outstanding_ = 0;
     ServeStats stats_;

@@ -124,14 +124,17 @@ SlicedNetwork::SlicedNetwork(const std::string& qasm, const std::string& measure
         steps_.push_back(st);
         // statistics of one slice
         const int64_t macs = int64_t(1) << ((st.rank_c + st.k) * dl_);
-        plan_.cmacs_per_slice += macs;
+        // saturates at INT64_MAX rather than wrapping (a work bound compares against it)
+        if (__builtin_add_overflow(plan_.cmacs_per_slice, macs, &plan_.cmacs_per_slice))
+            plan_.cmacs_per_slice = INT64_MAX;
         int from_a = 0;
         while (from_a < st.rank_c && (st.free_src[size_t(from_a)] & QT_FROM_A)) ++from_a;
         bool split = true;
         for (int i = from_a; i < st.rank_c; ++i)
             if (st.free_src[size_t(i)] & QT_FROM_A) split = false;
         if (split && from_a * dl_ >= 6 && (st.rank_c - from_a) * dl_ >= 6 && st.k * dl_ >= 5)
-            plan_.gemm_cmacs_per_slice += macs;
+            if (__builtin_add_overflow(plan_.gemm_cmacs_per_slice, macs, &plan_.gemm_cmacs_per_slice))
+                plan_.gemm_cmacs_per_slice = INT64_MAX;
     }
     plan_.sliced_wires = int(sliced_.size());
     plan_.slices = int64_t(1) << (int(sliced_.size()) * dl_);
