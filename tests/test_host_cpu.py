@@ -238,6 +238,16 @@ def test_serving_plan_routes_and_bytes():
         qa.serve_plan(qa.SERVE_AMPLITUDE, wl.qft_qasm(4), bits="01x1")
 
 
+def test_serving_planned_work_saturates():
+    """The planned work of a request too large to run (48 qubits at depth 14: slices
+    times MACs per slice is beyond 2^63) saturates at 2^63 - 1 instead of wrapping
+    to a negative count, which the per-request work limit would then admit."""
+    from qtorch_amd import workloads as wl
+    q = wl.grid_random_circuit(6, 8, 14, seed=605)
+    plan = qa.serve_plan(qa.SERVE_AMPLITUDE, q, bits=wl.bitstring(48, 7))
+    assert plan["route"] == "eager" and plan["cmacs"] == 2 ** 63 - 1
+
+
 def test_sliced_plan_bounds():
     """Slicing keeps every tensor of a slice within max_bits index bits; D^s slices of
     one shape; without slicing the per-slice MAC count is the plan's own."""
