@@ -146,39 +146,64 @@ def run_qft8(args, qa, np):
 
 def run_grid(args, qa, np):
     """Precision stress: one amplitude of seeded grid random circuits (RX/RY/RZ layers,
-    CZ on alternating grid-edge patterns) in the amplitude picture, line-graph order,
-    sliced to at most --max-bits index bits per tensor; every form against complex128."""
+    CZ on alternating grid-edge patterns) in the amplitude picture, sliced to at most
+    --max-bits index bits per tensor; every form against complex128.  6x8 depth 8 and
+    4x6 depth 14 in the line-graph order, every slice; 6x8 depth 14 in the stochastic
+    order sliced to 28 bits (2^27 slices of 2^38 MACs each, beyond reach as a whole):
+    the first --grid-sample slices that are not zero by structure, whose sum each form
+    computes."""
     from qtorch_amd import workloads as wl
     ctx = qa.Context(0)
-    cases = [(6, 8, 8), (4, 6, 14)]
+    cases = [(6, 8, 8, qa.ORDER_LINEGRAPH, args.max_bits, None), (4, 6, 14, qa.ORDER_LINEGRAPH, args.max_bits, None),
+             (6, 8, 14, qa.ORDER_STOCHASTIC, 28, args.grid_sample)]
     out = []
-    for r, c, d in cases:
+    chosen = [int(x) for x in args.grid_cases.split(",")] if args.grid_cases else range(len(cases))
+    for i, (r, c, d, order, max_bits, sample) in enumerate(cases):
+        if sample == 0 or i not in chosen:
+            continue
         q = wl.grid_random_circuit(r, c, d, seed=2024)
         bits = wl.bitstring(r * c, args.seed)
         t0 = time.perf_counter()
-        sn = qa.SlicedNetwork(q, bits, dim_log2=1, order=qa.ORDER_LINEGRAPH, max_bits=args.max_bits)
+        sn = qa.SlicedNetwork(q, bits, dim_log2=1, order=order, seed=2, max_bits=max_bits)
         plan_s = time.perf_counter() - t0
         plan = sn.plan()
+        print(f"grid {r}x{c} depth {d}: planned in {plan_s:.1f} s, {plan['slices']} slices", file=sys.stderr,
+              flush=True)
+        n_slices = plan["slices"]
+        if sample is not None:
+            # the first slices that are not zero by the structure alone (diagonal
+            # gates tie sliced wires; |0> states and output bits fix them)
+            picked = sn.nonzero_slices(sample)
+            n_slices = len(picked)
+            sn.set_slices(picked)
         res = {}
         ref = None
-        for prec in (qa.QT_PLAIN_C128, qa.QT_C64_4M, qa.QT_C64_3M, qa.QT_C64_BF16X3, qa.QT_C64_4M | qa.QT_RESCALE):
+        forms = (qa.QT_PLAIN_C128, qa.QT_C64_4M, qa.QT_C64_3M, qa.QT_C64_BF16X3, qa.QT_C64_4M | qa.QT_RESCALE)
+        for prec in forms[:args.grid_forms]:
             sn.run(ctx, prec)  # warm the allocator
             v, ms = sn.run(ctx, prec)
             if prec == qa.QT_PLAIN_C128:
                 ref = v
-            flops = 8.0 * plan["cmacs_per_slice"] * plan["slices"]
+            flops = 8.0 * plan["cmacs_per_slice"] * n_slices
             name = _form_name(qa, prec & ~qa.QT_RESCALE) + ("_rescaled" if prec & qa.QT_RESCALE else "")
             res[name] = {"amplitude": [v.real, v.imag], "ms": ms, "tflops": flops / (ms * 1e-3) / 1e12,
-                         "rel_err_vs_c128": abs(v - ref) / abs(ref)}
-        out.append({"circuit": f"grid_{r}x{c}_depth{d}", "qubits": r * c, "bits": bits, "plan": plan,
-                    "plan_s": plan_s, "forms": res})
+                         "rel_err_vs_c128": abs(v - ref) / abs(ref) if ref != 0 else None}
+            print(f"  {name}: {ms:.1f} ms", file=sys.stderr, flush=True)
+        case = {"circuit": f"grid_{r}x{c}_depth{d}", "qubits": r * c, "bits": bits, "plan": plan,
+                "order": "stochastic (seed 2)" if order == qa.ORDER_STOCHASTIC else "line-graph min-fill",
+                "plan_s": plan_s, "forms": res}
+        if sample is not None:
+            case["slices_contracted"] = picked
+            case["value_is"] = f"sum of {n_slices} of {plan['slices']} slices (the first ones not zero by structure)"
+        out.append(case)
+        sn.close()
     best = max(out[0]["forms"].items(), key=lambda kv: kv[1]["tflops"])
     return {"metric": "grid amplitude contraction TFLOP/s by form (error vs complex128)",
             "value": best[1]["tflops"], "unit": "TFLOP/s", "n_gpus": 1, "steps": 1, "warmup": 1,
             "ms_per_step": best[1]["ms"], "higher_is_better": True, "scaling": "strong", "vs_baseline": None,
             "dtype": best[0], "data": "synthetic",
             "config": {"workload": "grid_random_circuit_amplitudes", "max_bits": args.max_bits,
-                       "order": "line-graph min-fill", "cases": [o["circuit"] for o in out]},
+                       "cases": [o["circuit"] for o in out]},
             "cases": out}
 
 
@@ -402,6 +427,10 @@ def main():
     ap.add_argument("--workload", default="qaoa_p1", choices=["qaoa_p1", "qft8", "grid", "uneven", "serving"])
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--max-bits", type=int, default=22, help="grid: index bits per tensor after slicing")
+    ap.add_argument("--grid-sample", type=int, default=8, help="grid: slices of the 6x8 depth-14 circuit (0: skip)")
+    ap.add_argument("--grid-cases", default="", help="grid: comma-separated case indices (0: 6x8 d8, 1: 4x6 d14, "
+                    "2: 6x8 d14 sample); default all")
+    ap.add_argument("--grid-forms", type=int, default=5, help="grid: the first N of c128, 4M, 3M, bf16x3, 4M rescaled")
     ap.add_argument("--networks", type=int, default=512, help="uneven: edge networks")
     ap.add_argument("--streams", type=int, default=4, help="uneven: host threads / streams for the large networks")
     ap.add_argument("--qps", type=float, default=400.0, help="serving: arrival rate")

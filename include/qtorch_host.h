@@ -195,6 +195,16 @@ int qth_sliced_plan(qth_sliced* s, int64_t* values8);
  * fused subtrees); < 0 (default): fused, slices per launch bounded by memory;
  * > 0: fused, at most that many slices per launch. */
 int qth_sliced_set_fusion(qth_sliced* s, int64_t slices_per_launch);
+/* Later runs contract only slices [first, first + count) (count < 0: to the last
+ * slice) and return their sum: a bounded sample of a network whose full slice count
+ * is out of reach.  No reference counterpart (the reference does not slice). */
+int qth_sliced_set_range(qth_sliced* s, int64_t first, int64_t count);
+/* Later runs contract exactly slices[0..n) (each in [0, slices)) and return their sum. */
+int qth_sliced_set_slices(qth_sliced* s, const int64_t* slices, int64_t n);
+/* Up to max_count slices, in increasing order, in which no leaf's sub-tensor is all
+ * zero (the others contribute exactly zero: diagonal gates tie the wires around
+ * them); written to out[0..*n_out).  Host only. */
+int qth_sliced_nonzero_slices(qth_sliced* s, int64_t max_count, int64_t* out, int64_t* n_out);
 int qth_sliced_run(qth_sliced* s, qt_ctx* ctx, int precision, double* re, double* im, double* ms);
 int qth_sliced_destroy(qth_sliced* s);
 

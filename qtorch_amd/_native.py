@@ -154,6 +154,9 @@ _HOST_SIGS = {
                                   _c.POINTER(_c.c_double)]),
     "qth_sliced_destroy": (_c.c_int, [_vp]),
     "qth_sliced_set_fusion": (_c.c_int, [_vp, _c.c_int64]),
+    "qth_sliced_set_range": (_c.c_int, [_vp, _c.c_int64, _c.c_int64]),
+    "qth_sliced_set_slices": (_c.c_int, [_vp, _c.POINTER(_c.c_int64), _c.c_int64]),
+    "qth_sliced_nonzero_slices": (_c.c_int, [_vp, _c.c_int64, _c.POINTER(_c.c_int64), _c.POINTER(_c.c_int64)]),
     "qth_qaoa_source": (_c.c_int, [_c.c_char_p, _c.c_int, _c.c_int, _c.c_int, _c.c_uint64, _c.c_int, _c.c_int,
                                    _c.c_int, _c.c_int, _c.c_char_p, _c.c_size_t, _c.POINTER(_c.c_size_t)]),
     "qth_qaoa_param_block": (_c.c_int, [_c.c_int, _dp, _dp]),
@@ -807,6 +810,24 @@ class SlicedNetwork:
         chunk of slices sized by memory; n > 0: at most n slices per launch; 0: every
         step through qt_contract_pair."""
         _host_check(host.qth_sliced_set_fusion(self.handle, int(slices_per_launch)))
+
+    def set_range(self, first: int, count: int = -1) -> None:
+        """Later runs contract only slices [first, first + count) (count < 0: to the
+        last) and return their sum."""
+        _host_check(host.qth_sliced_set_range(self.handle, int(first), int(count)))
+
+    def set_slices(self, slices: Sequence[int]) -> None:
+        """Later runs contract exactly these slices and return their sum."""
+        arr = (ctypes.c_int64 * len(slices))(*[int(x) for x in slices])
+        _host_check(host.qth_sliced_set_slices(self.handle, arr, len(slices)))
+
+    def nonzero_slices(self, max_count: int) -> List[int]:
+        """Up to max_count slices, in increasing order, in which no leaf's sub-tensor
+        is all zero (every other slice is exactly zero).  Host only."""
+        out = (ctypes.c_int64 * max(int(max_count), 1))()
+        n = ctypes.c_int64()
+        _host_check(host.qth_sliced_nonzero_slices(self.handle, int(max_count), out, ctypes.byref(n)))
+        return list(out[:n.value])
 
     def run(self, ctx: Context, precision: int = QT_PLAIN_C128):
         re, im, ms = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()

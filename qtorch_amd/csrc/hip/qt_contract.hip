@@ -8,6 +8,10 @@
 //                     plain complex128 form floating-point contraction is off, so
 //                     results are bit-identical to the reference's loop.  Covers
 //                     every shape; chosen for small summed extents (memory-bound).
+//   k_skinny        : at most 5 free bits on one side of C: one lane per element of
+//                     the large side holds all outputs of the small side, summed in
+//                     k_direct's order (bit-identical to it); the large operand is
+//                     read once instead of once per small-side output.
 //   k_reduce_*      : small outputs with a large summed extent (e.g. a scalar from
 //                     two big tensors): the summed index is split over workgroups,
 //                     wavefront/LDS reductions, then an ordered second pass.
@@ -179,6 +183,78 @@ __global__ __launch_bounds__(kThreads) void k_direct(const V* __restrict__ A, co
             }
         }
         if (c < nC) C[c] = finish<V, MODE>(acc);
+    }
+}
+
+// Steps with at most 5 free bits on one side of C (a gate, or a thin tensor, against a
+// large tensor; C = [A's free bits low | B's free bits high]).  One lane per element
+// of the large side's part of C holds all 2^NB outputs of the small side, so the large
+// operand is read once (k_direct reads it once per small-side output).  Each output is
+// summed in k_direct's order with the same arithmetic: bit-identical to k_direct.
+// SMALL_B: B is the small side, output o of lane t is C[t + (o << lo_bits)];
+// otherwise A is (lo_bits == NB), output o of lane t is C[o + (t << NB)].
+template <class V, int MODE, int NB, bool SMALL_B>
+__global__ __launch_bounds__(kThreads) void k_skinny(const V* __restrict__ A, const V* __restrict__ B,
+                                                     V* __restrict__ C, BitMap m, int lo_bits, int64_t n_big) {
+    constexpr int NO = 1 << NB;
+    __shared__ int64_t ga[kThreads], gb[kThreads];
+    __shared__ int64_t small_off[NO];
+    const int tid = threadIdx.x;
+    if (tid < NO) {
+        int64_t oa, ob;
+        if constexpr (SMALL_B) {
+            c_offsets(m, int64_t(tid) << lo_bits, lo_bits, m.nbits_c, oa, ob);
+            small_off[tid] = ob;
+        } else {
+            c_offsets(m, tid, 0, NB, oa, ob);
+            small_off[tid] = oa;
+        }
+    }
+    const int64_t nS = int64_t(1) << m.nbits_s;
+    const int64_t stride = int64_t(gridDim.x) * kThreads;
+    for (int64_t base = int64_t(blockIdx.x) * kThreads; base < n_big; base += stride) {
+        const int64_t t = base + tid;
+        int64_t big_off;
+        {
+            int64_t oa, ob;
+            if constexpr (SMALL_B) {
+                c_offsets(m, t, 0, lo_bits, oa, ob);
+                big_off = oa;
+            } else {
+                c_offsets(m, t << NB, NB, m.nbits_c, oa, ob);
+                big_off = ob;
+            }
+        }
+        Acc<V, MODE> acc[NO];
+        for (int64_t s0 = 0; s0 < nS; s0 += kThreads) {
+            __syncthreads();
+            if (s0 + tid < nS) {
+                int64_t a, b;
+                s_offsets(m, s0 + tid, a, b);
+                ga[tid] = a;
+                gb[tid] = b;
+            }
+            __syncthreads();
+            const int n_s = int(nS - s0 < kThreads ? nS - s0 : kThreads);
+            if (t < n_big)
+                for (int j = 0; j < n_s; ++j) {
+                    if constexpr (SMALL_B) {
+                        const V x = A[big_off + ga[j]];
+#pragma unroll
+                        for (int o = 0; o < NO; ++o) mac<V, MODE>(acc[o], x, B[small_off[o] + gb[j]]);
+                    } else {
+                        const V y = B[big_off + gb[j]];
+#pragma unroll
+                        for (int o = 0; o < NO; ++o) mac<V, MODE>(acc[o], A[small_off[o] + ga[j]], y);
+                    }
+                }
+        }
+        if (t < n_big)
+#pragma unroll
+            for (int o = 0; o < NO; ++o) {
+                const int64_t idx = SMALL_B ? t + (int64_t(o) << lo_bits) : int64_t(o) + (t << NB);
+                C[idx] = finish<V, MODE>(acc[o]);
+            }
     }
 }
 
@@ -600,6 +676,34 @@ qt_status run_direct(Ctx* c, const V* A, const V* B, V* C, const BitMap& m) {
     const unsigned grid = grid_for(nC, kThreads, 16384);
     hipLaunchKernelGGL((k_direct<V, MODE>), dim3(grid), dim3(kThreads), 0, c->stream, A, B, C, m, nC);
     return check_hip(c, hipGetLastError(), "k_direct");
+}
+
+template <class V, int MODE, int NB>
+qt_status launch_skinny(Ctx* c, const V* A, const V* B, V* C, const BitMap& m, int mbits, bool small_b) {
+    const int64_t n_big = int64_t(1) << (m.nbits_c - NB);
+    const unsigned grid = grid_for(n_big, kThreads, 16384);
+    if (small_b)
+        hipLaunchKernelGGL((k_skinny<V, MODE, NB, true>), dim3(grid), dim3(kThreads), 0, c->stream, A, B, C, m,
+                           mbits, n_big);
+    else
+        hipLaunchKernelGGL((k_skinny<V, MODE, NB, false>), dim3(grid), dim3(kThreads), 0, c->stream, A, B, C, m,
+                           mbits, n_big);
+    return check_hip(c, hipGetLastError(), "k_skinny");
+}
+
+// C = [A part (mbits) | B part]; the side with at most 5 bits is the small one
+template <class V, int MODE>
+qt_status run_skinny(Ctx* c, const V* A, const V* B, V* C, const BitMap& m, int mbits) {
+    const int nbits_n = m.nbits_c - mbits;
+    const bool small_b = nbits_n <= 5;
+    switch (small_b ? nbits_n : mbits) {
+        case 0: return launch_skinny<V, MODE, 0>(c, A, B, C, m, mbits, small_b);
+        case 1: return launch_skinny<V, MODE, 1>(c, A, B, C, m, mbits, small_b);
+        case 2: return launch_skinny<V, MODE, 2>(c, A, B, C, m, mbits, small_b);
+        case 3: return launch_skinny<V, MODE, 3>(c, A, B, C, m, mbits, small_b);
+        case 4: return launch_skinny<V, MODE, 4>(c, A, B, C, m, mbits, small_b);
+        default: return launch_skinny<V, MODE, 5>(c, A, B, C, m, mbits, small_b);
+    }
 }
 
 template <class V, int MODE>
@@ -1028,14 +1132,23 @@ qt_status run_gemm(Ctx* c, const Tensor* A, const Tensor* B, Tensor* C, const Bi
     return s;
 }
 
+// a split C (A's free bits low, B's high) with at most 5 bits on one side and at
+// least 10 on the other
+inline bool skinny_shape(const BitMap& m, bool split, int mbits) {
+    const int nbits_n = m.nbits_c - mbits;
+    return split && std::min(mbits, nbits_n) <= 5 && std::max(mbits, nbits_n) >= 10;
+}
+
 template <class V, int MODE>
-qt_status dispatch_small(Ctx* c, const Tensor* A, const Tensor* B, Tensor* C, const BitMap& m) {
+qt_status dispatch_small(Ctx* c, const Tensor* A, const Tensor* B, Tensor* C, const BitMap& m, bool split,
+                         int mbits) {
     const V* a = static_cast<const V*>(A->data);
     const V* b = static_cast<const V*>(B->data);
     V* cc = static_cast<V*>(C->data);
     const int64_t nC = int64_t(1) << m.nbits_c;
     const int64_t nS = int64_t(1) << m.nbits_s;
     if (nC <= kReduceMaxOut && nS >= 8192) return run_reduce<V, MODE>(c, a, b, cc, m);
+    if (skinny_shape(m, split, mbits)) return run_skinny<V, MODE>(c, a, b, cc, m, mbits);
     return run_direct<V, MODE>(c, a, b, cc, m);
 }
 
@@ -1053,17 +1166,19 @@ qt_status contract_pair(Ctx* c, const Tensor* A, const Tensor* B, Tensor* C, con
     const bool gemm = split && mbits >= 6 && nbits_n >= 6;
     if (A->dtype == QT_DTYPE_C128) {
         if (gemm && m.nbits_s >= 4) return run_gemm<double2, MODE_PLAIN>(c, A, B, C, m, mbits);
-        return dispatch_small<double2, MODE_PLAIN>(c, A, B, C, m);
+        return dispatch_small<double2, MODE_PLAIN>(c, A, B, C, m, split, mbits);
     }
     const bool gemm64 = gemm && m.nbits_s >= 5;  // k-tiles of 32
     switch (precision) {
         case QT_C64_4M:
-            return gemm64 ? run_gemm<float2, MODE_4M>(c, A, B, C, m, mbits) : dispatch_small<float2, MODE_4M>(c, A, B, C, m);
+            return gemm64 ? run_gemm<float2, MODE_4M>(c, A, B, C, m, mbits)
+                          : dispatch_small<float2, MODE_4M>(c, A, B, C, m, split, mbits);
         case QT_C64_3M:
-            return gemm64 ? run_gemm<float2, MODE_3M>(c, A, B, C, m, mbits) : dispatch_small<float2, MODE_3M>(c, A, B, C, m);
+            return gemm64 ? run_gemm<float2, MODE_3M>(c, A, B, C, m, mbits)
+                          : dispatch_small<float2, MODE_3M>(c, A, B, C, m, split, mbits);
         default:
             return gemm64 ? run_gemm<float2, MODE_BF16X3>(c, A, B, C, m, mbits)
-                          : dispatch_small<float2, MODE_BF16X3>(c, A, B, C, m);
+                          : dispatch_small<float2, MODE_BF16X3>(c, A, B, C, m, split, mbits);
     }
 }
 

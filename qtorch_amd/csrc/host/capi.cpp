@@ -592,6 +592,25 @@ int qth_sliced_set_fusion(qth_sliced* s, int64_t slices_per_launch) {
     return QTH_OK;
 }
 
+int qth_sliced_set_range(qth_sliced* s, int64_t first, int64_t count) {
+    if (!s) return QTH_INVALID_FUNCTION_INPUT;
+    return Guard([&] { s->n->SetSliceRange(first, count); });
+}
+
+int qth_sliced_set_slices(qth_sliced* s, const int64_t* slices, int64_t n) {
+    if (!s || !slices || n < 1) return QTH_INVALID_FUNCTION_INPUT;
+    return Guard([&] { s->n->SetSliceList(std::vector<int64_t>(slices, slices + n)); });
+}
+
+int qth_sliced_nonzero_slices(qth_sliced* s, int64_t max_count, int64_t* out, int64_t* n_out) {
+    if (!s || !out || !n_out || max_count < 1) return QTH_INVALID_FUNCTION_INPUT;
+    return Guard([&] {
+        const auto v = s->n->NonzeroSlices(max_count);
+        std::copy(v.begin(), v.end(), out);
+        *n_out = int64_t(v.size());
+    });
+}
+
 int qth_sliced_run(qth_sliced* s, qt_ctx* ctx, int precision, double* re, double* im, double* ms) {
     if (!s || !ctx) return QTH_INVALID_FUNCTION_INPUT;
     return Guard([&] {

@@ -22,6 +22,7 @@
 #include <cmath>
 #include <exception>
 #include <sstream>
+#include <system_error>
 
 #include "batch.hpp"
 #include "qtorch_host.h"
@@ -586,7 +587,11 @@ void Server::EagerLoop() {
                 std::vector<std::thread> pool;
                 const size_t n_workers =
                     std::min({eager_ctxs_.size(), size_t(n_edges), size_t(p->st->eager_workers)});
-                for (size_t w = 1; w < n_workers; ++w) pool.emplace_back(work, w);
+                try {
+                    for (size_t w = 1; w < n_workers; ++w) pool.emplace_back(work, w);
+                } catch (const std::system_error&) {
+                    // fewer threads: the ones started and this one still take every edge
+                }
                 work(0);
                 for (auto& t : pool) t.join();
                 for (auto& ep : errs)

@@ -309,6 +309,119 @@ void SlicedNetwork::Extract(int leaf, int64_t slice, double* dst) const {
     }
 }
 
+void SlicedNetwork::SetSliceRange(int64_t first, int64_t count) {
+    if (first < 0 || first >= plan_.slices || count == 0) throw InvalidFunctionInput();
+    slice_first_ = first;
+    slice_count_ = count;
+    slice_list_.clear();
+}
+
+void SlicedNetwork::SetSliceList(const std::vector<int64_t>& slices) {
+    if (slices.empty()) throw InvalidFunctionInput();
+    for (int64_t s : slices)
+        if (s < 0 || s >= plan_.slices) throw InvalidFunctionInput();
+    slice_list_ = slices;
+}
+
+std::vector<int64_t> SlicedNetwork::NonzeroSlices(int64_t max_count, int64_t budget) const {
+    std::vector<int64_t> found;
+    const int n_digits = int(sliced_.size());
+    if (max_count <= 0) return found;
+    if (n_digits == 0) return {0};
+    const int64_t D = int64_t(1) << dl_;
+    // leaves checked once their lowest cut digit is set (digits are set from the
+    // highest down, so found slices come in increasing order)
+    std::vector<std::vector<int>> check_at(static_cast<size_t>(n_digits));
+    for (int leaf = 0; leaf < n_leaves_; ++leaf) {
+        int lo = n_digits;
+        for (const auto& [axis, digit] : cut_[size_t(leaf)]) lo = std::min(lo, digit);
+        if (lo < n_digits) check_at[size_t(lo)].push_back(leaf);
+    }
+    // wires tied by diagonal leaves (nonzero only where two axes agree: CZ, RZ, ...)
+    // carry one value; a leaf that is zero for some value of an axis rules that
+    // value out for the axis's class (a |0> state, an output bit)
+    int n_wires = 0;
+    for (const auto& w : leaf_wires_)
+        for (int x : w) n_wires = std::max(n_wires, x + 1);
+    std::vector<int> parent(static_cast<size_t>(n_wires));
+    for (int i = 0; i < n_wires; ++i) parent[size_t(i)] = i;
+    auto find = [&](int x) {
+        while (parent[size_t(x)] != x) x = parent[size_t(x)] = parent[size_t(parent[size_t(x)])];
+        return x;
+    };
+    std::vector<uint32_t> allowed(static_cast<size_t>(n_wires), (1u << D) - 1);
+    for (int leaf = 0; leaf < n_leaves_; ++leaf) {
+        const auto& w = leaf_wires_[size_t(leaf)];
+        const auto& vals = leaf_vals_[size_t(leaf)];
+        const int r = int(w.size());
+        std::vector<uint32_t> mask(size_t(r), 0);
+        std::vector<char> untied(size_t(r * r), 0);
+        bool any = false;
+        for (size_t idx = 0; idx < vals.size(); ++idx) {
+            if (vals[idx] == std::complex<double>(0.0, 0.0)) continue;
+            any = true;
+            for (int a = 0; a < r; ++a) {
+                const int64_t da = (int64_t(idx) >> (a * dl_)) & (D - 1);
+                mask[size_t(a)] |= 1u << da;
+                for (int b = a + 1; b < r; ++b)
+                    if (da != ((int64_t(idx) >> (b * dl_)) & (D - 1))) untied[size_t(a * r + b)] = 1;
+            }
+        }
+        for (int a = 0; a < r; ++a) allowed[size_t(w[size_t(a)])] &= mask[size_t(a)];
+        if (any)
+            for (int a = 0; a < r; ++a)
+                for (int b = a + 1; b < r; ++b)
+                    if (!untied[size_t(a * r + b)]) parent[size_t(find(w[size_t(a)]))] = find(w[size_t(b)]);
+    }
+    std::vector<uint32_t> class_allowed(static_cast<size_t>(n_wires), (1u << D) - 1);
+    for (int x = 0; x < n_wires; ++x) class_allowed[size_t(find(x))] &= allowed[size_t(x)];
+    // for each digit, the next higher digit whose wire is in the same class (or -1)
+    std::vector<int> tied_above(static_cast<size_t>(n_digits), -1);
+    for (int d = 0; d < n_digits; ++d)
+        for (int e = d + 1; e < n_digits && tied_above[size_t(d)] < 0; ++e)
+            if (find(sliced_[size_t(d)]) == find(sliced_[size_t(e)])) tied_above[size_t(d)] = e;
+    std::vector<double> sub;
+    auto leaf_nonzero = [&](int leaf, int64_t slice) {
+        sub.assign(size_t(2) << (leaf_rank_[size_t(leaf)] * dl_), 0.0);
+        Extract(leaf, slice, sub.data());
+        for (double v : sub)
+            if (v != 0.0) return true;
+        return false;
+    };
+    int64_t visits = 0;
+    // depth-first over digits n_digits-1 .. 0
+    std::vector<int64_t> value(size_t(n_digits) + 1, 0);
+    int digit = n_digits - 1;
+    value[size_t(digit)] = -1;
+    int64_t slice = 0;
+    while (digit < n_digits && int64_t(found.size()) < max_count && visits < budget) {
+        const int64_t shift = int64_t(digit) * dl_;
+        slice &= ~((D - 1) << shift);
+        if (++value[size_t(digit)] == D) {  // digit exhausted: back up
+            ++digit;
+            continue;
+        }
+        ++visits;
+        slice |= value[size_t(digit)] << shift;
+        if (!((class_allowed[size_t(find(sliced_[size_t(digit)]))] >> value[size_t(digit)]) & 1u)) continue;
+        if (tied_above[size_t(digit)] >= 0 && value[size_t(tied_above[size_t(digit)])] != value[size_t(digit)]) continue;
+        bool ok = true;
+        for (int leaf : check_at[size_t(digit)])
+            if (!leaf_nonzero(leaf, slice)) {
+                ok = false;
+                break;
+            }
+        if (!ok) continue;
+        if (digit == 0) {
+            found.push_back(slice);
+            continue;
+        }
+        --digit;
+        value[size_t(digit)] = -1;
+    }
+    return found;
+}
+
 std::complex<double> SlicedNetwork::Run(qt_ctx* ctx, int precision, SliceStats* stats) const {
     if (!ctx) throw DeviceFailure("SlicedNetwork::Run needs a device context");
     const int base = precision & ~QT_RESCALE;
@@ -364,15 +477,23 @@ std::complex<double> SlicedNetwork::Run(qt_ctx* ctx, int precision, SliceStats* 
         std::vector<const void*> ptrs(direct.size());
         std::vector<qt_tensor*> made(direct.size());
         std::vector<double> block;
-        for (int64_t s0 = 0; s0 < plan_.slices; s0 += chunk) {
-            const int64_t n_sl = std::min(chunk, plan_.slices - s0);
+        // the slices contracted: the list, else the range (every slice by default)
+        const bool listed = !slice_list_.empty();
+        const int64_t s_begin = listed ? 0 : slice_first_;
+        const int64_t s_end = listed ? int64_t(slice_list_.size())
+                              : slice_count_ < 0 || slice_count_ > plan_.slices - slice_first_
+                                  ? plan_.slices
+                                  : slice_first_ + slice_count_;
+        auto slice_id = [&](int64_t i) { return listed ? slice_list_[size_t(i)] : i; };
+        for (int64_t s0 = s_begin; s0 < s_end; s0 += chunk) {
+            const int64_t n_sl = std::min(chunk, s_end - s0);
             if (fuse) {
                 // every subtree of every slice of the chunk in one launch
                 if (block_elems_ > 0) {
                     block.assign(size_t(2 * n_sl * block_elems_), 0.0);
                     for (int64_t j = 0; j < n_sl; ++j)
                         for (const auto& [leaf, off] : param_leaves_)
-                            Extract(leaf, s0 + j, block.data() + 2 * (j * block_elems_ + off));
+                            Extract(leaf, slice_id(s0 + j), block.data() + 2 * (j * block_elems_ + off));
                     CheckQt(ctx, qt_batch_write_leaves(batch, const_elems, block.data(), n_sl * block_elems_),
                             "qt_batch_write_leaves");
                 }
@@ -395,7 +516,7 @@ std::complex<double> SlicedNetwork::Run(qt_ctx* ctx, int precision, SliceStats* 
                 CheckQt(ctx, qt_batch_launch(batch), "qt_batch_launch");
             }
             for (int64_t j = 0; j < n_sl; ++j) {
-                const int64_t slice = s0 + j;
+                const int64_t slice = slice_id(s0 + j);
                 for (size_t d = 0; d < direct.size(); ++d) {
                     const int leaf = direct[d];
                     const int64_t n = int64_t(1) << (leaf_rank_[size_t(leaf)] * dl_);

@@ -38,6 +38,18 @@ public:
     // slices_per_launch = 0: every step through qt_contract_pair; < 0 (default): fused
     // subtrees, slices per launch chosen by memory; > 0: at most that many per launch.
     void SetFusion(int64_t slices_per_launch) { fuse_slices_ = slices_per_launch; }
+    // Run contracts only slices [first, first + count) of the plan's slices (count < 0:
+    // to the last one) and returns their sum; the default is every slice.
+    void SetSliceRange(int64_t first, int64_t count);
+    // Run contracts exactly these slices (each in [0, slices)) and returns their sum.
+    void SetSliceList(const std::vector<int64_t>& slices);
+    // The first max_count slices, in increasing order, that are not zero for one of
+    // these reasons: a leaf's sub-tensor is all zero, or two sliced wires tied by a
+    // chain of diagonal leaves (CZ, RZ, ...) take different values, or a wire takes a
+    // value its chain rules out (a |0> state, an output bit).  Every slice left out
+    // contributes exactly zero.  Stops early after visiting `budget` partial
+    // assignments of the sliced wires.
+    std::vector<int64_t> NonzeroSlices(int64_t max_count, int64_t budget = int64_t(1) << 24) const;
     // New leaf values for the same network structure (the plan, slicing and fused
     // programs depend on the structure only): the nodes of (qasm, measure) parsed as
     // in the constructor must have the ranks of this network's leaves.
@@ -64,6 +76,8 @@ private:
     SliceStats plan_;
     // fused subtrees: the maximal subtrees of small steps, one device program each
     int64_t fuse_slices_ = -1;
+    int64_t slice_first_ = 0, slice_count_ = -1;
+    std::vector<int64_t> slice_list_;  // non-empty: the slices Run contracts
     std::vector<char> step_fused_;                  // per step
     std::vector<char> leaf_direct_;                 // per leaf: an operand of an unfused step
     std::vector<int> roots_;                        // root step of each subtree

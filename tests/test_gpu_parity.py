@@ -51,6 +51,8 @@ SHAPES = [
     (7, 7, 7, 2), (8, 7, 7, 2), (7, 8, 7, 2),          # split reduction (small output, large sum)
     (5, 5, 2, 2), (6, 6, 3, 2), (7, 6, 3, 2), (6, 7, 4, 2),  # permute + MFMA GEMM
     (10, 10, 5, 1), (12, 9, 4, 1), (14, 14, 14, 1), (6, 4, 2, 1),  # dimension-2 tensors
+    (8, 2, 1, 2), (2, 8, 1, 2), (7, 4, 2, 2), (10, 6, 5, 2),  # skinny: <= 5 free bits on one side
+    (12, 3, 2, 1), (3, 12, 1, 1), (14, 8, 3, 1), (8, 14, 3, 1), (12, 2, 2, 1),
 ]
 
 
@@ -63,6 +65,8 @@ def _regime(free, k, d):
         return "gemm"
     if nc <= 6 and ns >= 13:
         return "reduce"
+    if split and min(na * d, nc - na * d) <= 5 and max(na * d, nc - na * d) >= 10:
+        return "skinny"
     return "direct"
 
 
@@ -79,7 +83,7 @@ def test_contract_pair_plain_vs_oracle(ctx, ra, rb, k, d):
         got = tc.download()
         ref = qo.contract_indices(a, ra, b, rb, free, pairs, d)
         assert macs == 1 << (d * (len(free) + k))
-        if _regime(free, k, d) == "direct":
+        if _regime(free, k, d) in ("direct", "skinny"):  # the reference's summation order
             assert np.array_equal(got, ref), (ra, rb, k, d, interleave)
         else:
             np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
@@ -87,7 +91,8 @@ def test_contract_pair_plain_vs_oracle(ctx, ra, rb, k, d):
 
 @pytest.mark.parametrize("prec,tol", [(qa.QT_C64_4M, 2e-6), (qa.QT_C64_3M, 4e-6), (qa.QT_C64_BF16X3, 1e-4)])
 @pytest.mark.parametrize("ra,rb,k,d", [(4, 4, 2, 2), (7, 7, 7, 2), (6, 6, 3, 2), (10, 10, 5, 1),
-                                      (12, 12, 6, 1), (14, 13, 7, 1), (8, 8, 3, 2), (9, 9, 4, 2)])
+                                      (12, 12, 6, 1), (14, 13, 7, 1), (8, 8, 3, 2), (9, 9, 4, 2),
+                                      (8, 2, 1, 2), (3, 12, 1, 1), (10, 6, 5, 2)])
 def test_contract_pair_c64_forms(ctx, prec, tol, ra, rb, k, d):
     """complex64 forms against the FP64 oracle: error relative to the largest |C|.
     Shapes with 64+ free elements on each side and 32+ summed run on the MFMA GEMM

@@ -92,3 +92,57 @@ def test_fused_subtrees_liouville(ctx):
     sn.set_fusion(0)
     v_step, _ = sn.run(ctx)
     assert abs(v_fused - 1 / 32) <= 1e-13 and abs(v_step - 1 / 32) <= 1e-13
+
+
+@pytest.mark.parametrize("fusion", [-1, 0])
+def test_slice_ranges_sum_to_the_whole(ctx, fusion):
+    """Slices [0, k) plus slices [k, end) sum to the value of every slice (the
+    state-vector amplitude, pinned by the test above); a count past the end stops at
+    the last slice; the full range is the default."""
+    q = wl.grid_random_circuit(4, 4, 8, seed=11)
+    bits = wl.bitstring(16, 5)
+    sn = qa.SlicedNetwork(q, bits, dim_log2=1, order=qa.ORDER_LINEGRAPH, max_bits=8)
+    sn.set_fusion(fusion)
+    n = sn.plan()["slices"]
+    assert n >= 4
+    whole, _ = sn.run(ctx)
+    k = n // 3
+    sn.set_range(0, k)
+    a, _ = sn.run(ctx)
+    sn.set_range(k, -1)
+    b, _ = sn.run(ctx)
+    sn.set_range(k, n)  # a count past the end stops at the last slice
+    b2, _ = sn.run(ctx)
+    assert b2 == b
+    assert abs((a + b) - whole) <= 1e-12 * abs(whole), (a, b, whole)
+    assert abs(a) > 0 and abs(b) > 0
+    sn.set_range(0, -1)
+    again, _ = sn.run(ctx)
+    assert again == whole
+    with pytest.raises(qa.InvalidFunctionInput):
+        sn.set_range(n, 1)
+    with pytest.raises(qa.InvalidFunctionInput):
+        sn.set_range(0, 0)
+
+
+def test_nonzero_slices_cover_the_amplitude(ctx):
+    """Slices left out by nonzero_slices contract to exactly zero, and the listed
+    slices sum to the amplitude of every slice."""
+    q = wl.grid_random_circuit(4, 4, 10, seed=3)
+    bits = wl.bitstring(16, 5)
+    sn = qa.SlicedNetwork(q, bits, dim_log2=1, order=qa.ORDER_GREEDY, max_bits=8)
+    n = sn.plan()["slices"]
+    assert n <= 512
+    whole, _ = sn.run(ctx)
+    listed = sn.nonzero_slices(n)
+    assert 0 < len(listed) < n
+    skipped = sorted(set(range(n)) - set(listed))
+    sn.set_slices(skipped)
+    zero, _ = sn.run(ctx)
+    assert zero == 0
+    for s in skipped[:8]:
+        sn.set_range(s, 1)
+        assert sn.run(ctx)[0] == 0
+    sn.set_slices(listed)
+    part, _ = sn.run(ctx)
+    assert abs(part - whole) <= 1e-12 * abs(whole), (part, whole)

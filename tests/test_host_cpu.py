@@ -248,6 +248,25 @@ def test_serving_planned_work_saturates():
     assert plan["route"] == "eager" and plan["cmacs"] == 2 ** 63 - 1
 
 
+def test_nonzero_slices_listing():
+    """nonzero_slices lists slices in increasing order, within range, a shorter
+    request giving a prefix of a longer one; a grid circuit's CZ gates tie sliced
+    wires, so some slices are left out."""
+    from qtorch_amd import workloads as wl
+    q = wl.grid_random_circuit(4, 4, 10, seed=3)
+    sn = qa.SlicedNetwork(q, wl.bitstring(16, 5), dim_log2=1, order=qa.ORDER_GREEDY, max_bits=8)
+    n = sn.plan()["slices"]
+    full = sn.nonzero_slices(n)
+    assert full == sorted(set(full)) and all(0 <= s < n for s in full)
+    assert 0 < len(full) < n
+    assert sn.nonzero_slices(3) == full[:3]
+    with pytest.raises(qa.InvalidFunctionInput):
+        sn.set_slices([n])
+    with pytest.raises(qa.InvalidFunctionInput):
+        sn.set_range(n, 1)
+    sn.close()
+
+
 def test_sliced_plan_bounds():
     """Slicing keeps every tensor of a slice within max_bits index bits; D^s slices of
     one shape; without slicing the per-slice MAC count is the plan's own."""
