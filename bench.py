@@ -432,7 +432,7 @@ def main():
                     "2: 6x8 d14 sample); default all")
     ap.add_argument("--grid-forms", type=int, default=5, help="grid: the first N of c128, 4M, 3M, bf16x3, 4M rescaled")
     ap.add_argument("--networks", type=int, default=512, help="uneven: edge networks")
-    ap.add_argument("--streams", type=int, default=4, help="uneven: host threads / streams for the large networks")
+    ap.add_argument("--streams", type=int, default=8, help="uneven: host threads / streams for the large networks")
     ap.add_argument("--qps", type=float, default=400.0, help="serving: arrival rate")
     ap.add_argument("--mix", default="light", choices=["light", "config"], help="serving: request mix")
     ap.add_argument("--duration", type=float, default=10.0, help="serving: seconds of arrivals")
