@@ -89,6 +89,27 @@ def test_contract_pair_plain_vs_oracle(ctx, ra, rb, k, d):
             np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
 
 
+@pytest.mark.parametrize("prec", [qa.QT_PLAIN_C128, qa.QT_C64_4M])
+def test_gemm_with_many_permute_tiles(ctx, prec):
+    """A 22-bit operand whose summed axes are its lowest: the GEMM path permutes it in
+    2^12 or more LDS tiles (a two-dimensional grid of workgroups)."""
+    rng = np.random.default_rng(5)
+    ra, rb, d = 11, 5, 2
+    pairs = [(0, 1), (1, 0)]
+    free = [(True, i) for i in range(2, ra)] + [(False, j) for j in range(2, rb)]
+    dt = np.complex128 if prec == qa.QT_PLAIN_C128 else np.complex64
+    a = _rand(rng, 1 << (d * ra), dt)
+    b = _rand(rng, 1 << (d * rb), dt)
+    assert _regime(free, len(pairs), d) == "gemm"
+    ta = qa.Tensor(ctx, ra, d, qa.QT_DTYPE_C128 if dt == np.complex128 else qa.QT_DTYPE_C64).upload(a)
+    tb = qa.Tensor(ctx, rb, d, qa.QT_DTYPE_C128 if dt == np.complex128 else qa.QT_DTYPE_C64).upload(b)
+    tc, _ = qa.contract_pair(ctx, ta, tb, pairs, free, prec)
+    got = tc.download().astype(np.complex128)
+    ref = qo.contract_indices(a.astype(np.complex128), ra, b.astype(np.complex128), rb, free, pairs, d)
+    tol = 1e-12 if prec == qa.QT_PLAIN_C128 else 2e-6 * np.sqrt(16)
+    assert np.abs(got - ref).max() <= tol * np.abs(ref).max()
+
+
 @pytest.mark.parametrize("prec,tol", [(qa.QT_C64_4M, 2e-6), (qa.QT_C64_3M, 4e-6), (qa.QT_C64_BF16X3, 1e-4)])
 @pytest.mark.parametrize("ra,rb,k,d", [(4, 4, 2, 2), (7, 7, 7, 2), (6, 6, 3, 2), (10, 10, 5, 1),
                                       (12, 12, 6, 1), (14, 13, 7, 1), (8, 8, 3, 2), (9, 9, 4, 2),
