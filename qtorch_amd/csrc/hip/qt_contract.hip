@@ -752,11 +752,11 @@ bool is_identity(const int* src, int n) {
 
 }  // namespace
 
-PermPlan make_perm_plan(int nbits, const int* src) {
+PermPlan make_perm_plan(int nbits, const int* src, int tl_max) {
     PermPlan p;
     std::memset(&p, 0, sizeof p);
     p.nbits = nbits;
-    const int tl = nbits < 5 ? nbits : 5;
+    const int tl = nbits < tl_max ? nbits : tl_max;
     p.tl = tl;
     int inv[64];
     for (int j = 0; j < nbits; ++j) inv[src[j]] = j;
@@ -1068,6 +1068,8 @@ namespace {
 // Large steps: permute into GEMM layout (skipped when already there), MFMA GEMM.
 template <class V, int MODE>
 qt_status run_gemm(Ctx* c, const Tensor* A, const Tensor* B, Tensor* C, const BitMap& m, int mbits) {
+    // permute runs of 512 B in either dtype: 32 complex128 or 64 complex64 elements
+    constexpr int kPermLowBits = sizeof(V) == 8 ? 6 : 5;
     const int nbits_n = m.nbits_c - mbits;
     const int sb = m.nbits_s;
     int src_a[64], src_b[64];
@@ -1084,7 +1086,7 @@ qt_status run_gemm(Ctx* c, const Tensor* A, const Tensor* B, Tensor* C, const Bi
         hipError_t e;
         abuf = c->pool.alloc(A->bytes, &e);
         if (!abuf) return check_hip(c, e, "gemm scratch A");
-        s = permute(c, A->data, abuf, A->dtype, make_perm_plan(mbits + sb, src_a));
+        s = permute(c, A->data, abuf, A->dtype, make_perm_plan(mbits + sb, src_a, kPermLowBits));
         ap = static_cast<const V*>(abuf);
     }
     if (s == QT_OK && !is_identity(src_b, nbits_n + sb)) {
@@ -1094,7 +1096,7 @@ qt_status run_gemm(Ctx* c, const Tensor* A, const Tensor* B, Tensor* C, const Bi
             c->pool.release(abuf);
             return check_hip(c, e, "gemm scratch B");
         }
-        s = permute(c, B->data, bbuf, B->dtype, make_perm_plan(nbits_n + sb, src_b));
+        s = permute(c, B->data, bbuf, B->dtype, make_perm_plan(nbits_n + sb, src_b, kPermLowBits));
         bp = static_cast<const V*>(bbuf);
     }
     if (s == QT_OK) {

@@ -112,7 +112,8 @@ qt_status read_exponent(Ctx* c, const Tensor* t, int32_t* e);
 void release_exponent(Ctx* c, Tensor* t);
 // acc (complex128 scalar) += t (scalar, exponent applied), in FP64 on the stream.
 qt_status accumulate_scalar(Ctx* c, const Tensor* t, Tensor* acc);
-PermPlan make_perm_plan(int nbits, const int* src);
+// tl_max: input-low / output-low bits of a tile (contiguous runs of 2^tl_max elements)
+PermPlan make_perm_plan(int nbits, const int* src, int tl_max = 5);
 
 }  // namespace qt
 
