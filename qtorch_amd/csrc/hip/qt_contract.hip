@@ -1068,8 +1068,9 @@ namespace {
 // Large steps: permute into GEMM layout (skipped when already there), MFMA GEMM.
 template <class V, int MODE>
 qt_status run_gemm(Ctx* c, const Tensor* A, const Tensor* B, Tensor* C, const BitMap& m, int mbits) {
-    // permute runs of 512 B in either dtype: 32 complex128 or 64 complex64 elements
-    constexpr int kPermLowBits = sizeof(V) == 8 ? 6 : 5;
+    // permute runs of 64 elements (1 KB complex128, 512 B complex64); tiles of up to
+    // 2^12 elements (64 KB of LDS in complex128)
+    constexpr int kPermLowBits = 6;
     const int nbits_n = m.nbits_c - mbits;
     const int sb = m.nbits_s;
     int src_a[64], src_b[64];
