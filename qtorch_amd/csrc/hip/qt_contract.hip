@@ -414,15 +414,16 @@ __global__ __launch_bounds__(kThreads) void k_gemm_c128(const double2* __restric
         }
     const int li = lane & 15, lk = lane >> 4;
     for (int64_t k0 = 0; k0 < K; k0 += GK) {
+        // a last k-tile beyond K (K = 8) is filled with zeros, which add nothing
         for (int e = tid; e < GK * GM; e += kThreads) {
             const int kk = e / GM, mm = e % GM;
-            const double2 v = Ap[(k0 + kk) * M + m0 + mm];
+            const double2 v = k0 + kk < K ? Ap[(k0 + kk) * M + m0 + mm] : double2{0.0, 0.0};
             as_re[kk][mm] = v.x;
             as_im[kk][mm] = v.y;
         }
         for (int e = tid; e < GK * GN; e += kThreads) {
             const int kk = e / GN, nn = e % GN;
-            const double2 v = Bp[(k0 + kk) * N + n0 + nn];
+            const double2 v = k0 + kk < K ? Bp[(k0 + kk) * N + n0 + nn] : double2{0.0, 0.0};
             bs_re[kk][nn] = v.x;
             bs_im[kk][nn] = v.y;
         }
@@ -522,8 +523,9 @@ __global__ __launch_bounds__(kThreads) void k_gemm_c64(const float2* __restrict_
             uint16_t* sb = sa + 4 * kPlaneB;
             for (int e = tid; e < HK * GM; e += kThreads) {
                 const int kk = e / GM, rr = e % GM;
-                const float2 va = Ap[(k0 + kk) * M + m0 + rr];
-                const float2 vb = Bp[(k0 + kk) * N + n0 + rr];
+                const bool in_k = k0 + kk < K;  // a last k-tile beyond K is zero-filled
+                const float2 va = in_k ? Ap[(k0 + kk) * M + m0 + rr] : float2{0.0f, 0.0f};
+                const float2 vb = in_k ? Bp[(k0 + kk) * N + n0 + rr] : float2{0.0f, 0.0f};
                 const int o = rr * kRowB + kk;
                 const float arh = bf16_round(va.x), aih = bf16_round(va.y);
                 const float brh = bf16_round(vb.x), bih = bf16_round(vb.y);
@@ -582,8 +584,9 @@ __global__ __launch_bounds__(kThreads) void k_gemm_c64(const float2* __restrict_
             float* sb = sa + 2 * kPlaneF;
             for (int e = tid; e < HK * GM; e += kThreads) {
                 const int kk = e / GM, rr = e % GM;
-                const float2 va = Ap[(k0 + kk) * M + m0 + rr];
-                const float2 vb = Bp[(k0 + kk) * N + n0 + rr];
+                const bool in_k = k0 + kk < K;  // a last k-tile beyond K is zero-filled
+                const float2 va = in_k ? Ap[(k0 + kk) * M + m0 + rr] : float2{0.0f, 0.0f};
+                const float2 vb = in_k ? Bp[(k0 + kk) * N + n0 + rr] : float2{0.0f, 0.0f};
                 const int o = kk * (GM + 1) + rr;
                 sa[o] = va.x;
                 sa[kPlaneF + o] = va.y;
@@ -1167,11 +1170,17 @@ qt_status contract_pair(Ctx* c, const Tensor* A, const Tensor* B, Tensor* C, con
         if (m.c_src[i] < 64) split = false;
     const int nbits_n = m.nbits_c - mbits;
     const bool gemm = split && mbits >= 6 && nbits_n >= 6;
+    // short summed extents (8, or 16 in complex64: below a k-tile) take the GEMM, with
+    // the last k-tile zero-filled, when the output is large: k_direct would re-read
+    // the operands once per output element of the other side (measured: from 2^16
+    // outputs in complex64, from 2^24 in complex128)
     if (A->dtype == QT_DTYPE_C128) {
-        if (gemm && m.nbits_s >= 4) return run_gemm<double2, MODE_PLAIN>(c, A, B, C, m, mbits);
+        const bool short_k = m.nbits_s >= 3 && m.nbits_c >= 24;
+        if (gemm && (m.nbits_s >= 4 || short_k)) return run_gemm<double2, MODE_PLAIN>(c, A, B, C, m, mbits);
         return dispatch_small<double2, MODE_PLAIN>(c, A, B, C, m, split, mbits);
     }
-    const bool gemm64 = gemm && m.nbits_s >= 5;  // k-tiles of 32
+    const bool short_k = m.nbits_s >= 3 && m.nbits_c >= 16;
+    const bool gemm64 = gemm && (m.nbits_s >= 5 || short_k);  // k-tiles of 32
     switch (precision) {
         case QT_C64_4M:
             return gemm64 ? run_gemm<float2, MODE_4M>(c, A, B, C, m, mbits)

@@ -53,6 +53,7 @@ SHAPES = [
     (10, 10, 5, 1), (12, 9, 4, 1), (14, 14, 14, 1), (6, 4, 2, 1),  # dimension-2 tensors
     (8, 2, 1, 2), (2, 8, 1, 2), (7, 4, 2, 2), (10, 6, 5, 2),  # skinny: <= 5 free bits on one side
     (12, 3, 2, 1), (3, 12, 1, 1), (14, 8, 3, 1), (8, 14, 3, 1), (12, 2, 2, 1),
+    (15, 15, 3, 1),  # GEMM with a summed extent of 8 (zero-filled k-tile)
 ]
 
 
@@ -61,7 +62,7 @@ def _regime(free, k, d):
     na = sum(1 for f in free if f[0])
     split = all(f[0] for f in free[:na])
     nc, ns = len(free) * d, k * d
-    if split and na * d >= 6 and (len(free) - na) * d >= 6 and ns >= 4:
+    if split and na * d >= 6 and (len(free) - na) * d >= 6 and (ns >= 4 or (ns >= 3 and nc >= 24)):
         return "gemm"
     if nc <= 6 and ns >= 13:
         return "reduce"
@@ -113,7 +114,7 @@ def test_gemm_with_many_permute_tiles(ctx, prec):
 @pytest.mark.parametrize("prec,tol", [(qa.QT_C64_4M, 2e-6), (qa.QT_C64_3M, 4e-6), (qa.QT_C64_BF16X3, 1e-4)])
 @pytest.mark.parametrize("ra,rb,k,d", [(4, 4, 2, 2), (7, 7, 7, 2), (6, 6, 3, 2), (10, 10, 5, 1),
                                       (12, 12, 6, 1), (14, 13, 7, 1), (8, 8, 3, 2), (9, 9, 4, 2),
-                                      (8, 2, 1, 2), (3, 12, 1, 1), (10, 6, 5, 2)])
+                                      (8, 2, 1, 2), (3, 12, 1, 1), (10, 6, 5, 2), (11, 11, 3, 1), (12, 12, 4, 1)])
 def test_contract_pair_c64_forms(ctx, prec, tol, ra, rb, k, d):
     """complex64 forms against the FP64 oracle: error relative to the largest |C|.
     Shapes with 64+ free elements on each side and 32+ summed run on the MFMA GEMM
