@@ -189,15 +189,19 @@ __global__ __launch_bounds__(kThreads) void k_direct(const V* __restrict__ A, co
 // Steps with at most 5 free bits on one side of C (a gate, or a thin tensor, against a
 // large tensor; C = [A's free bits low | B's free bits high]).  One lane per element
 // of the large side's part of C holds all 2^NB outputs of the small side, so the large
-// operand is read once (k_direct reads it once per small-side output).  Each output is
-// summed in k_direct's order with the same arithmetic: bit-identical to k_direct.
+// operand is read once (k_direct reads it once per small-side output); the small
+// operand's elements for a chunk of summed indices are staged in LDS and read there
+// by every lane at once.  Each output is summed in k_direct's order with the same
+// arithmetic: bit-identical to k_direct.
 // SMALL_B: B is the small side, output o of lane t is C[t + (o << lo_bits)];
 // otherwise A is (lo_bits == NB), output o of lane t is C[o + (t << NB)].
 template <class V, int MODE, int NB, bool SMALL_B>
 __global__ __launch_bounds__(kThreads) void k_skinny(const V* __restrict__ A, const V* __restrict__ B,
                                                      V* __restrict__ C, BitMap m, int lo_bits, int64_t n_big) {
     constexpr int NO = 1 << NB;
-    __shared__ int64_t ga[kThreads], gb[kThreads];
+    constexpr int SC = NB >= 5 ? 32 : 64;  // summed indices per staged chunk (<= 16 KB complex128)
+    __shared__ int64_t g_big[SC];          // the large operand's offset of each summed index
+    __shared__ V ys[SC][NO];               // the small operand's elements
     __shared__ int64_t small_off[NO];
     const int tid = threadIdx.x;
     if (tid < NO) {
@@ -226,26 +230,24 @@ __global__ __launch_bounds__(kThreads) void k_skinny(const V* __restrict__ A, co
             }
         }
         Acc<V, MODE> acc[NO];
-        for (int64_t s0 = 0; s0 < nS; s0 += kThreads) {
-            __syncthreads();
-            if (s0 + tid < nS) {
+        for (int64_t s0 = 0; s0 < nS; s0 += SC) {
+            const int n_s = int(nS - s0 < SC ? nS - s0 : SC);
+            __syncthreads();  // small_off is written; the previous chunk's reads are done
+            for (int e = tid; e < n_s * NO; e += kThreads) {
+                const int j = e / NO, o = e % NO;
                 int64_t a, b;
-                s_offsets(m, s0 + tid, a, b);
-                ga[tid] = a;
-                gb[tid] = b;
+                s_offsets(m, s0 + j, a, b);
+                if (o == 0) g_big[j] = SMALL_B ? a : b;
+                ys[j][o] = SMALL_B ? B[small_off[o] + b] : A[small_off[o] + a];
             }
             __syncthreads();
-            const int n_s = int(nS - s0 < kThreads ? nS - s0 : kThreads);
             if (t < n_big)
                 for (int j = 0; j < n_s; ++j) {
-                    if constexpr (SMALL_B) {
-                        const V x = A[big_off + ga[j]];
+                    const V x = SMALL_B ? A[big_off + g_big[j]] : B[big_off + g_big[j]];
 #pragma unroll
-                        for (int o = 0; o < NO; ++o) mac<V, MODE>(acc[o], x, B[small_off[o] + gb[j]]);
-                    } else {
-                        const V y = B[big_off + gb[j]];
-#pragma unroll
-                        for (int o = 0; o < NO; ++o) mac<V, MODE>(acc[o], A[small_off[o] + ga[j]], y);
+                    for (int o = 0; o < NO; ++o) {
+                        if constexpr (SMALL_B) mac<V, MODE>(acc[o], x, ys[j][o]);
+                        else mac<V, MODE>(acc[o], ys[j][o], x);
                     }
                 }
         }
