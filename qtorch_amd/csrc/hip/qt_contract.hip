@@ -415,21 +415,33 @@ __global__ __launch_bounds__(kThreads) void k_gemm_c128(const double2* __restric
             acc_im[i][j] = d4_t{0, 0, 0, 0};
         }
     const int li = lane & 15, lk = lane >> 4;
-    for (int64_t k0 = 0; k0 < K; k0 += GK) {
-        // a last k-tile beyond K (K = 8) is filled with zeros, which add nothing
-        for (int e = tid; e < GK * GM; e += kThreads) {
-            const int kk = e / GM, mm = e % GM;
-            const double2 v = k0 + kk < K ? Ap[(k0 + kk) * M + m0 + mm] : double2{0.0, 0.0};
-            as_re[kk][mm] = v.x;
-            as_im[kk][mm] = v.y;
+    // each thread stages kPer elements of A' and of B' per k-tile; the next k-tile is
+    // loaded into registers while the current one is multiplied from LDS.  A last
+    // k-tile beyond K (K = 8) is filled with zeros, which add nothing.
+    constexpr int kPer = GK * GM / kThreads;
+    static_assert(GK * GM == GK * GN && kPer * kThreads == GK * GM, "tile staging");
+    double2 ra[kPer], rb[kPer];
+    auto fetch = [&](int64_t k0) {
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+            const int e = tid + q * kThreads, kk = e / GM, rr = e % GM;
+            const bool in_k = k0 + kk < K;
+            ra[q] = in_k ? Ap[(k0 + kk) * M + m0 + rr] : double2{0.0, 0.0};
+            rb[q] = in_k ? Bp[(k0 + kk) * N + n0 + rr] : double2{0.0, 0.0};
         }
-        for (int e = tid; e < GK * GN; e += kThreads) {
-            const int kk = e / GN, nn = e % GN;
-            const double2 v = k0 + kk < K ? Bp[(k0 + kk) * N + n0 + nn] : double2{0.0, 0.0};
-            bs_re[kk][nn] = v.x;
-            bs_im[kk][nn] = v.y;
+    };
+    fetch(0);
+    for (int64_t k0 = 0; k0 < K; k0 += GK) {
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+            const int e = tid + q * kThreads, kk = e / GM, rr = e % GM;
+            as_re[kk][rr] = ra[q].x;
+            as_im[kk][rr] = ra[q].y;
+            bs_re[kk][rr] = rb[q].x;
+            bs_im[kk][rr] = rb[q].y;
         }
         __syncthreads();
+        if (k0 + GK < K) fetch(k0 + GK);
 #pragma unroll
         for (int kk = 0; kk < GK; kk += 4) {
             const int kl = kk + lk;
