@@ -42,7 +42,7 @@ struct Server::Structure {
     int64_t cmacs = 0;                    // per request
     int64_t planned_bytes = 0;            // per request
     // large-network route: each network planned once (line-graph order, sliced, small
-    // subtrees fused); a request copies it and sets its own leaf values
+    // subtrees fused); a request runs it with its own leaf values
     std::vector<std::shared_ptr<SlicedNetwork>> eager_nets;
     int eager_workers = 1;                // edge networks of a request run side by side
     std::string error;                    // set when the request cannot run
@@ -574,10 +574,11 @@ void Server::EagerLoop() {
                 auto work = [&](size_t w) {
                     try {
                         for (int e = next++; e < n_edges; e = next++) {
-                            SlicedNetwork sn(*p->st->eager_nets[size_t(e)]);  // the plan; this request's angles
-                            sn.SetLeafValues(QaoaEdgeQasm(p->st->data, e, p->req.angles),
-                                             QaoaEdgeMeasure(p->st->data, e));
-                            zz[size_t(e)] = sn.Run(eager_ctxs_[w], QT_PLAIN_C128, nullptr).real();
+                            // the shared plan, run with this request's leaf values
+                            const SlicedNetwork& sn = *p->st->eager_nets[size_t(e)];
+                            const auto leaves = sn.MakeLeafState(QaoaEdgeQasm(p->st->data, e, p->req.angles),
+                                                                 QaoaEdgeMeasure(p->st->data, e));
+                            zz[size_t(e)] = sn.Run(eager_ctxs_[w], QT_PLAIN_C128, nullptr, leaves).real();
                         }
                     } catch (...) {
                         errs[w] = std::current_exception();
@@ -600,9 +601,8 @@ void Server::EagerLoop() {
                 for (double v : zz) f += 0.5 * (1.0 - v);  // src/maxcut.cpp:196
                 res.re = f;
             } else {
-                SlicedNetwork sn(*p->st->eager_nets[0]);
-                sn.SetLeafValues(p->req.text, p->req.bits);
-                const auto v = sn.Run(eager_ctxs_[0], QT_PLAIN_C128, nullptr);
+                const SlicedNetwork& sn = *p->st->eager_nets[0];
+                const auto v = sn.Run(eager_ctxs_[0], QT_PLAIN_C128, nullptr, sn.MakeLeafState(p->req.text, p->req.bits));
                 res.re = v.real();
                 res.im = v.imag();
             }

@@ -261,31 +261,40 @@ void SlicedNetwork::PlanFusion() {
     plan_.fused_steps = fused;
 }
 
-void SlicedNetwork::SetLeafValues(const std::string& qasm, const std::string& measure) {
+SlicedNetwork::LeafState SlicedNetwork::MakeLeafState(const std::string& qasm, const std::string& measure) const {
     NetworkOptions o;
     o.dim_log2 = dl_;
     Network net(qasm, &measure, o);
     if (net.GetNumOriginalNodes() != n_leaves_) throw InvalidFunctionInput();
-    std::vector<std::vector<std::complex<double>>> vals;
+    LeafState ls;
+    ls.vals.reserve(size_t(n_leaves_));
     for (int i = 0; i < n_leaves_; ++i) {
         const auto& node = net.GetAllNodes()[size_t(i)];
         if (node->mRank != int(leaf_wires_[size_t(i)].size()) ||
             node->mHostVals.size() != leaf_vals_[size_t(i)].size())
             throw InvalidFunctionInput();
-        vals.push_back(node->mHostVals);
+        ls.vals.push_back(node->mHostVals);
     }
-    leaf_vals_ = std::move(vals);
+    ls.const_pool = const_pool_;
     for (const auto& [leaf, off] : const_leaves_) {
-        const auto& v = leaf_vals_[size_t(leaf)];
+        const auto& v = ls.vals[size_t(leaf)];
         for (size_t e = 0; e < v.size(); ++e) {
-            const_pool_[size_t(2 * (off + int64_t(e)))] = v[e].real();
-            const_pool_[size_t(2 * (off + int64_t(e)) + 1)] = v[e].imag();
+            ls.const_pool[size_t(2 * (off + int64_t(e)))] = v[e].real();
+            ls.const_pool[size_t(2 * (off + int64_t(e)) + 1)] = v[e].imag();
         }
     }
+    return ls;
 }
 
-void SlicedNetwork::Extract(int leaf, int64_t slice, double* dst) const {
-    const auto& src = leaf_vals_[size_t(leaf)];
+void SlicedNetwork::SetLeafValues(const std::string& qasm, const std::string& measure) {
+    LeafState ls = MakeLeafState(qasm, measure);
+    leaf_vals_ = std::move(ls.vals);
+    const_pool_ = std::move(ls.const_pool);
+}
+
+void SlicedNetwork::Extract(const std::vector<std::vector<std::complex<double>>>& vals, int leaf, int64_t slice,
+                            double* dst) const {
+    const auto& src = vals[size_t(leaf)];
     const int rank = int(leaf_wires_[size_t(leaf)].size());
     const int64_t D = int64_t(1) << dl_;
     const int64_t n = int64_t(1) << (leaf_rank_[size_t(leaf)] * dl_);
@@ -423,6 +432,18 @@ std::vector<int64_t> SlicedNetwork::NonzeroSlices(int64_t max_count, int64_t bud
 }
 
 std::complex<double> SlicedNetwork::Run(qt_ctx* ctx, int precision, SliceStats* stats) const {
+    return RunWith(ctx, precision, stats, leaf_vals_, const_pool_);
+}
+
+std::complex<double> SlicedNetwork::Run(qt_ctx* ctx, int precision, SliceStats* stats, const LeafState& leaves) const {
+    if (leaves.vals.size() != leaf_vals_.size() || leaves.const_pool.size() != const_pool_.size())
+        throw InvalidFunctionInput();
+    return RunWith(ctx, precision, stats, leaves.vals, leaves.const_pool);
+}
+
+std::complex<double> SlicedNetwork::RunWith(qt_ctx* ctx, int precision, SliceStats* stats,
+                                            const std::vector<std::vector<std::complex<double>>>& vals,
+                                            const std::vector<double>& const_pool) const {
     if (!ctx) throw DeviceFailure("SlicedNetwork::Run needs a device context");
     const int base = precision & ~QT_RESCALE;
     const bool c128 = base == QT_PLAIN_C128;
@@ -457,8 +478,8 @@ std::complex<double> SlicedNetwork::Run(qt_ctx* ctx, int precision, SliceStats* 
             std::vector<qt_program*> pp;
             for (const auto& p : progs_) pp.push_back(p.get());
             CheckQt(ctx, qt_batch_create(ctx, n_roots, pp.data(), base, &batch), "qt_batch_create");
-            const_elems = std::max<int64_t>(1, int64_t(const_pool_.size() / 2));
-            std::vector<double> cp(const_pool_);
+            const_elems = std::max<int64_t>(1, int64_t(const_pool.size() / 2));
+            std::vector<double> cp(const_pool);
             cp.resize(size_t(2 * const_elems), 0.0);
             CheckQt(ctx, qt_batch_write_leaves(batch, 0, cp.data(), const_elems), "qt_batch_write_leaves");
             // slices per launch: their subtree results stay within 256 MiB
@@ -493,7 +514,7 @@ std::complex<double> SlicedNetwork::Run(qt_ctx* ctx, int precision, SliceStats* 
                     block.assign(size_t(2 * n_sl * block_elems_), 0.0);
                     for (int64_t j = 0; j < n_sl; ++j)
                         for (const auto& [leaf, off] : param_leaves_)
-                            Extract(leaf, slice_id(s0 + j), block.data() + 2 * (j * block_elems_ + off));
+                            Extract(vals, leaf, slice_id(s0 + j), block.data() + 2 * (j * block_elems_ + off));
                     CheckQt(ctx, qt_batch_write_leaves(batch, const_elems, block.data(), n_sl * block_elems_),
                             "qt_batch_write_leaves");
                 }
@@ -521,7 +542,7 @@ std::complex<double> SlicedNetwork::Run(qt_ctx* ctx, int precision, SliceStats* 
                     const int leaf = direct[d];
                     const int64_t n = int64_t(1) << (leaf_rank_[size_t(leaf)] * dl_);
                     host[d].assign(size_t(2 * n), 0.0);
-                    Extract(leaf, slice, host[d].data());
+                    Extract(vals, leaf, slice, host[d].data());
                     if (c128) {
                         ptrs[d] = host[d].data();
                     } else {

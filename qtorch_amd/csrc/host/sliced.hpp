@@ -34,6 +34,16 @@ public:
     // QT_RESCALE: large steps rescaled by powers of two).  Contracts every slice on
     // ctx's stream.
     std::complex<double> Run(qt_ctx* ctx, int precision, SliceStats* stats) const;
+    // Leaf values of (qasm, measure) for this network's structure (parsed as in the
+    // constructor), held apart from the network: several threads can run the same
+    // planned network with their own values at once.
+    struct LeafState {
+        std::vector<std::vector<std::complex<double>>> vals;
+        std::vector<double> const_pool;  // the fused programs' constant leaves, FP64 pairs
+    };
+    LeafState MakeLeafState(const std::string& qasm, const std::string& measure) const;
+    // Run with those leaf values instead of the network's own.
+    std::complex<double> Run(qt_ctx* ctx, int precision, SliceStats* stats, const LeafState& leaves) const;
     const SliceStats& Plan() const { return plan_; }
     // slices_per_launch = 0: every step through qt_contract_pair; < 0 (default): fused
     // subtrees, slices per launch chosen by memory; > 0: at most that many per launch.
@@ -63,7 +73,12 @@ private:
     };
     void PlanFusion();
     // leaf i's sub-tensor for `slice` as interleaved FP64 (re, im) at dst
-    void Extract(int leaf, int64_t slice, double* dst) const;
+    void Extract(int leaf, int64_t slice, double* dst) const { Extract(leaf_vals_, leaf, slice, dst); }
+    void Extract(const std::vector<std::vector<std::complex<double>>>& vals, int leaf, int64_t slice,
+                 double* dst) const;
+    std::complex<double> RunWith(qt_ctx* ctx, int precision, SliceStats* stats,
+                                 const std::vector<std::vector<std::complex<double>>>& vals,
+                                 const std::vector<double>& const_pool) const;
 
     int dl_ = 2;
     int n_leaves_ = 0;
