@@ -335,7 +335,8 @@ def run_serving(args, qa, np):
             q, n, label = rng.choice(circuits)
             reqs.append(("amp", q, wl.bitstring(n, rng.randrange(1 << 30)), None, label))
     # 64 GiB of the 288 GiB HBM for the requests admitted at once: large light cones
-    # (8-9 GiB of live tensors per edge network) run up to four edge networks side by side
+    # (8-9 GiB of live tensors per edge network) run two edge networks side by side, two
+    # such requests at a time
     srv = qa.Server(slots=2, precision=qa.QT_C64_4M, hbm_budget=64 << 30)
 
     def submit(r):

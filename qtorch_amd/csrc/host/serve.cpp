@@ -109,9 +109,10 @@ std::shared_ptr<Server::Structure> BuildStructure(const ServeRequest& r, const S
                 }
                 // edge networks run on up to eager_workers streams at once, as many as
                 // the budget holds (at least one)
-                const int64_t fit = peak > 0 ? opt.hbm_budget / peak : int64_t(opt.eager_workers);
+                const int64_t per_loop = std::max(1, opt.eager_workers / std::max(1, opt.eager_loops));
+                const int64_t fit = peak > 0 ? opt.hbm_budget / peak : per_loop;
                 st->eager_workers = int(std::max<int64_t>(
-                    1, std::min<int64_t>({fit, int64_t(opt.eager_workers), int64_t(st->eager_nets.size())})));
+                    1, std::min<int64_t>({fit, per_loop, int64_t(st->eager_nets.size())})));
                 st->planned_bytes = SatMulAdd(0, peak, st->eager_workers);
             }
         } else if (r.kind == kServeAmplitude) {
@@ -192,7 +193,8 @@ struct Server::Slot {
 
 Server::Server(const ServeOptions& opt) : opt_(opt), t0_(Clock::now()) {
     if (opt_.slots < 1 || opt_.max_requests_per_launch < 1 || opt_.hbm_budget < 1 || opt_.param_capacity < 64 ||
-        opt_.eager_workers < 1 || opt_.eager_workers > 16)
+        opt_.eager_workers < 1 || opt_.eager_workers > 16 || opt_.eager_loops < 1 ||
+        opt_.eager_loops > opt_.eager_workers)
         throw InvalidFunctionInput();
     try {
         for (int i = 0; i < opt_.slots; ++i) {
@@ -224,7 +226,7 @@ Server::Server(const ServeOptions& opt) : opt_(opt), t0_(Clock::now()) {
         throw;
     }
     loop_ = std::thread([this] { Loop(); });
-    eager_ = std::thread([this] { EagerLoop(); });
+    for (int k = 0; k < opt_.eager_loops; ++k) eager_threads_.emplace_back([this, k] { EagerLoop(k); });
 }
 
 Server::~Server() {
@@ -234,7 +236,8 @@ Server::~Server() {
     }
     cv_.notify_all();
     if (loop_.joinable()) loop_.join();
-    if (eager_.joinable()) eager_.join();
+    for (auto& t : eager_threads_)
+        if (t.joinable()) t.join();
     for (auto& s : slots_) {
         for (auto* b : s->batch)
             if (b) qt_batch_destroy(b);
@@ -539,21 +542,26 @@ void Server::Loop() {
     }
 }
 
-void Server::EagerLoop() {
+void Server::EagerLoop(int k) {
+    // this loop's streams: eager_workers / eager_loops of them
+    const size_t per = size_t(std::max(1, opt_.eager_workers / opt_.eager_loops));
+    qt_ctx* const* ctxs = eager_ctxs_.data() + size_t(k) * per;
     while (true) {
         std::shared_ptr<Pending> p;
         {
             std::unique_lock<std::mutex> lk(mu_);
             cv_.wait(lk, [&] { return stop_ || !eager_queue_.empty(); });
             if (eager_queue_.empty()) return;
-            // admission: waits until its planned bytes fit beside the batched work
-            if (in_flight_bytes_ + eager_queue_.front()->st->planned_bytes > opt_.hbm_budget && busy_slots_ > 0) {
+            // admission: waits until its planned bytes fit beside the work in flight
+            if (in_flight_bytes_ + eager_queue_.front()->st->planned_bytes > opt_.hbm_budget &&
+                (busy_slots_ > 0 || eager_running_ > 0)) {
                 lk.unlock();
                 std::this_thread::sleep_for(std::chrono::microseconds(50));
                 continue;
             }
             p = eager_queue_.front();
             eager_queue_.pop_front();
+            ++eager_running_;
             in_flight_bytes_ += p->st->planned_bytes;
             stats_.peak_planned_bytes = std::max(stats_.peak_planned_bytes, in_flight_bytes_);
         }
@@ -570,7 +578,7 @@ void Server::EagerLoop() {
                 const int n_edges = int(p->st->data.pairs.size());
                 std::vector<double> zz(size_t(n_edges), 0.0);
                 std::atomic<int> next{0};
-                std::vector<std::exception_ptr> errs(eager_ctxs_.size());
+                std::vector<std::exception_ptr> errs(per);
                 auto work = [&](size_t w) {
                     try {
                         for (int e = next++; e < n_edges; e = next++) {
@@ -578,7 +586,7 @@ void Server::EagerLoop() {
                             const SlicedNetwork& sn = *p->st->eager_nets[size_t(e)];
                             const auto leaves = sn.MakeLeafState(QaoaEdgeQasm(p->st->data, e, p->req.angles),
                                                                  QaoaEdgeMeasure(p->st->data, e));
-                            zz[size_t(e)] = sn.Run(eager_ctxs_[w], QT_PLAIN_C128, nullptr, leaves).real();
+                            zz[size_t(e)] = sn.Run(ctxs[w], QT_PLAIN_C128, nullptr, leaves).real();
                         }
                     } catch (...) {
                         errs[w] = std::current_exception();
@@ -587,7 +595,7 @@ void Server::EagerLoop() {
                 };
                 std::vector<std::thread> pool;
                 const size_t n_workers =
-                    std::min({eager_ctxs_.size(), size_t(n_edges), size_t(p->st->eager_workers)});
+                    std::min({per, size_t(n_edges), size_t(p->st->eager_workers)});
                 try {
                     for (size_t w = 1; w < n_workers; ++w) pool.emplace_back(work, w);
                 } catch (const std::system_error&) {
@@ -602,7 +610,7 @@ void Server::EagerLoop() {
                 res.re = f;
             } else {
                 const SlicedNetwork& sn = *p->st->eager_nets[0];
-                const auto v = sn.Run(eager_ctxs_[0], QT_PLAIN_C128, nullptr, sn.MakeLeafState(p->req.text, p->req.bits));
+                const auto v = sn.Run(ctxs[0], QT_PLAIN_C128, nullptr, sn.MakeLeafState(p->req.text, p->req.bits));
                 res.re = v.real();
                 res.im = v.imag();
             }
@@ -613,6 +621,7 @@ void Server::EagerLoop() {
         {
             std::lock_guard<std::mutex> g(mu_);
             in_flight_bytes_ -= p->st->planned_bytes;
+            --eager_running_;
         }
         Complete(*p, res);
     }

@@ -29,7 +29,8 @@ struct ServeOptions {
     int64_t param_capacity = 1 << 22;   // parameter elements per slot and launch
     int eager_max_bits = 28;            // large-network path: index bits per tensor after slicing
     int64_t max_request_cmacs = int64_t(1) << 44;  // large-network path: planned complex MACs of a request
-    int eager_workers = 4;              // large-network path: edge networks of a request run side by side
+    int eager_workers = 4;              // large-network path: streams (edge networks run side by side)
+    int eager_loops = 2;                // large-network requests in flight at once (sharing those streams)
 };
 
 enum ServeKind { kServeQaoaEnergy = 0, kServeAmplitude = 1 };
@@ -92,7 +93,7 @@ private:
     using Clock = std::chrono::steady_clock;
     std::shared_ptr<Structure> GetStructure(const ServeRequest& r);
     void Loop();
-    void EagerLoop();
+    void EagerLoop(int k);
     bool StartIteration(Slot& s);
     bool FinishIteration(Slot& s, bool wait);
     void Complete(Pending& p, ServeResult res);
@@ -113,7 +114,9 @@ private:
     Clock::time_point t0_;
     double last_event_ms_ = 0;
     int busy_slots_ = 0;
-    std::thread loop_, eager_;
+    std::thread loop_;
+    std::vector<std::thread> eager_threads_;
+    int eager_running_ = 0;  // large-network requests being contracted
 };
 
 }  // namespace qtorch
