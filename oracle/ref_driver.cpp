@@ -17,10 +17,21 @@
 //   nwires <qasm> <meas> <0|1>           number of line-graph nodes (distinct wires)
 //   time   <listfile> <reps>             wall time of Contract(Stochastic) over every
 //                                        "<qasm> <meas>" line of listfile (CPU baseline)
+//   cones  <dgf> <p> <bg_0> .. <bg_2p-1> ExtraData(p, dgf) (src/maxcut.h:31-186): per
+//                                        edge CONE e qubitsNeeded, its realIterations
+//                                        as IT a b, and the QASM text F_p writes for it
+//                                        (src/maxcut.cpp:172-178) produced by the
+//                                        reference's own outputInitialPlusStateToFile
+//                                        and applyU_CsThenU_Bs (src/maxcut.h:191-233)
+//                                        between QASM_BEGIN / QASM_END
 #include "ContractionTools.h"
 #include "LineGraph.h"
+#include "maxcut.h"   // needs nlopt-2.4.2/api on the include path (headers only)
 
 #include <chrono>
+#include <cstdlib>
+#include <stdexcept>
+#include <unistd.h>
 #include <cstdio>
 #include <fstream>
 #include <sstream>
@@ -128,6 +139,33 @@ int main(int argc, char** argv) {
             auto t1 = std::chrono::steady_clock::now();
             std::printf("TIME %.9f\nJOBS %zu\nCHECKSUM %.17g\n",
                         std::chrono::duration<double>(t1 - t0).count(), jobs.size() * reps, sum);
+        } else if (cmd == "cones" && argc >= 4) {
+            const int p = std::atoi(argv[3]);
+            if (argc < 4 + 2 * p) throw std::runtime_error("cones needs 2p angles");
+            std::vector<double> bg;
+            for (int i = 0; i < 2 * p; ++i) bg.push_back(std::strtod(argv[4 + i], nullptr));
+            ExtraData ed(p, argv[2]);
+            char tmpl[] = "/tmp/ref_cones_XXXXXX";
+            const int fd = mkstemp(tmpl);
+            if (fd < 0) throw std::runtime_error("mkstemp failed");
+            close(fd);
+            for (size_t e = 0; e < ed.pairs.size(); ++e) {
+                const int nq = ed.qubitsNeeded[e];
+                std::printf("CONE %zu %d\n", e, nq);
+                for (const auto& it : ed.realIterations[e]) std::printf("IT %d %d\n", it.first, it.second);
+                {
+                    // the same calls F_p makes per edge (src/maxcut.cpp:172-178)
+                    std::ofstream q(tmpl);
+                    q << nq << std::endl;
+                    outputInitialPlusStateToFile(q, nq);
+                    applyU_CsThenU_Bs(ed.realIterations[e], ed.p, bg, nq, q);
+                }
+                std::ifstream back(tmpl);
+                std::stringstream text;
+                text << back.rdbuf();
+                std::printf("QASM_BEGIN\n%sQASM_END\n", text.str().c_str());
+            }
+            std::remove(tmpl);
         } else {
             std::fprintf(stderr, "bad command\n");
             rc = 2;

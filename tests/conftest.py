@@ -18,3 +18,12 @@ def golden():
     import json
     with open(os.path.join(REPO, "tests", "golden", "cases.json")) as f:
         return json.load(f)["cases"]
+
+
+@pytest.fixture(scope="session")
+def ref_cones():
+    """Light cones and per-edge F_p texts written by the reference's own ExtraData and
+    emission functions (src/maxcut.h:31-233), recorded by tests/golden/make_golden.py."""
+    import json
+    with open(os.path.join(REPO, "tests", "golden", "cones.json")) as f:
+        return json.load(f)["graphs"]

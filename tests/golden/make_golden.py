@@ -9,10 +9,18 @@ sample files are copied as data), the contraction sequence the reference chose
 given, and the value it printed at %.17g.  Only this script reads the reference
 (where its sources are); the tests read the committed fixtures.
 
-Usage:  python tests/golden/make_golden.py   (needs oracle/_ref built)
+The QAOA light cones and the per-edge QASM text F_p writes come from the
+reference's own ExtraData and emission functions (src/maxcut.h:31-233) through the
+driver's `cones` command: `cones.json` holds, per graph and p, every edge's
+qubitsNeeded and realIterations, and per angle set the SHA-256 of every edge's
+QASM text (the full text for a few edges).
+
+Usage:  python tests/golden/make_golden.py           (needs oracle/_ref built)
+        python tests/golden/make_golden.py --cones   (rewrites cones.json only)
 """
 from __future__ import annotations
 
+import hashlib
 import json
 import math
 import os
@@ -152,8 +160,94 @@ def regular_graph(n, d, seed):
             return [(a, b) if rng.random() < 0.5 else (b, a) for a, b in e]
 
 
+def ref_cones(dgf_path, p, bg):
+    """Runs the reference's ExtraData(p, dgf) and its emission for one angle set:
+    [(qubits_needed, real_iterations, qasm_text)] per edge."""
+    out = subprocess.run([DRIVER, "cones", dgf_path, str(p)] + ["%.17g" % x for x in bg],
+                         capture_output=True, text=True, timeout=600)
+    if out.returncode != 0:
+        raise RuntimeError(f"cones {dgf_path} p={p}: rc={out.returncode}\n{out.stdout}\n{out.stderr}")
+    cones, cur, qasm = [], None, None
+    for line in out.stdout.splitlines(keepends=True):
+        if qasm is not None:
+            if line == "QASM_END\n":
+                cur[2] = "".join(qasm)
+                qasm = None
+            else:
+                qasm.append(line)
+            continue
+        parts = line.split()
+        if parts[0] == "CONE":
+            assert int(parts[1]) == len(cones)
+            cur = [int(parts[2]), [], None]
+            cones.append(cur)
+        elif parts[0] == "IT":
+            cur[1].append([int(parts[1]), int(parts[2])])
+        elif parts[0] == "QASM_BEGIN":
+            qasm = []
+    return [tuple(c) for c in cones]
+
+
+# graphs and angle sets whose light cones and emitted text are pinned; the angle sets
+# exercise ostream's 6-significant-digit printing (exponent forms, rounding, signs)
+CONE_GRAPHS = ["graphs/3reg_n30_s2024.dgf", "samples/3regRand30Node50.dgf",
+               "samples/4regRand30Node26.dgf"]
+CONE_ANGLES = {
+    1: [[0.3, -1.1], [1.2345678e-5, -123456.789], [0.1234565, 2.5e-7], [-0.0, 3.14159265358979]],
+    2: [[0.392699, 0.392699, 0.785399, 0.785399], [0.3, 0.2, -0.7, 1.1],
+        [1e-7, -2.0000005, 98765.4321, -1e-12]],
+    3: [[0.1, 0.2, 0.3, -0.4, -0.5, 0.6]],
+}
+
+
+def golden_qaoa_angles():
+    """The angle set of the qaoa3_* cases: gamma ~ U[0, pi], beta ~ U[0, pi/2] seeded,
+    as [beta, -2 gamma] (exp(-i gamma ZZ), RX(2 beta) in the reference's terms)."""
+    arng = random.Random(77)
+    gamma = arng.uniform(0, math.pi)
+    beta = arng.uniform(0, math.pi / 2)
+    return gamma, beta, [beta, -2.0 * gamma]
+
+
+def write_cones():
+    doc = {"generator": "tests/golden/make_golden.py --cones",
+           "reference": "ExtraData + outputInitialPlusStateToFile + applyU_CsThenU_Bs "
+                        "(src/maxcut.h:31-233) through oracle/_ref/ref_driver_O2 cones",
+           "graphs": []}
+    for rel in CONE_GRAPHS:
+        path = os.path.join(HERE, rel)
+        for p in (1, 2, 3):
+            if p == 3 and not rel.startswith("graphs/"):
+                continue
+            entry = {"dgf": rel, "p": p, "sets": []}
+            sets = list(CONE_ANGLES[p])
+            if rel == "graphs/3reg_n30_s2024.dgf" and p == 1:
+                sets.insert(0, golden_qaoa_angles()[2])
+            for bg in sets:
+                cones = ref_cones(path, p, bg)
+                if "qubits" not in entry:
+                    entry["qubits"] = [c[0] for c in cones]
+                    entry["iterations"] = [c[1] for c in cones]
+                assert entry["qubits"] == [c[0] for c in cones]
+                assert entry["iterations"] == [c[1] for c in cones]
+                entry["sets"].append({
+                    "betas_gammas": bg,
+                    "sha256": [hashlib.sha256(c[2].encode()).hexdigest()[:32] for c in cones],
+                    "text": {str(e): cones[e][2] for e in (0, len(cones) - 1)},
+                })
+            doc["graphs"].append(entry)
+            print(f"cones {rel} p={p}: {len(entry['qubits'])} edges, qubits "
+                  f"{min(entry['qubits'])}-{max(entry['qubits'])}", flush=True)
+    with open(os.path.join(HERE, "cones.json"), "w") as f:
+        json.dump(doc, f, indent=0)
+
+
 def main():
     if not os.path.exists(DRIVER):
+        sys.exit("build oracle/_ref first: oracle/build_ref.sh")
+    if "--cones" in sys.argv:
+        write_cones()
+        return
         sys.exit("build oracle/_ref first: oracle/build_ref.sh")
     os.makedirs(SAMPLES, exist_ok=True)
     # reference sample data used by the reference's own tests
@@ -300,26 +394,25 @@ def main():
     write(os.path.join(HERE, "graphs", "3reg_n30_s2024.dgf"),
           "c seeded random 3-regular graph, 30 vertices (tests/golden/make_golden.py)\n" +
           "".join(f"e {a} {b}\n" for a, b in edges))
-    arng = random.Random(77)
-    gamma = arng.uniform(0, math.pi)
-    beta = arng.uniform(0, math.pi / 2)
-    bg = [beta, -2.0 * gamma]   # exp(-i gamma ZZ) and RX(2 beta) in the reference's terms
+    gamma, beta, bg = golden_qaoa_angles()
+    dgf3 = os.path.join(HERE, "graphs", "3reg_n30_s2024.dgf")
     for compat in (True, False):
-        cones = qo.light_cones(edges, 1, reference_qubit_map=compat)
+        # the reference map's text is the reference's own emission; the corrected map
+        # (pair.second -> 1 as well) has no reference counterpart and is the oracle's
+        cones = ref_cones(dgf3, 1, bg) if compat else qo.light_cones(edges, 1, reference_qubit_map=False)
         tag = "ref" if compat else "fix"
-        for e, (nq, real, _) in enumerate(cones):
-            q = qo.qaoa_edge_qasm(nq, real, 1, bg)
+        for e, cone in enumerate(cones):
+            nq, real = cone[0], cone[1]
+            q = cone[2] if compat else qo.qaoa_edge_qasm(nq, real, 1, bg)
             add(f"qaoa3_{tag}_e{e}", f"circuits/qaoa3_{tag}_e{e}.qasm", qo.qaoa_edge_measure(nq),
                 "stoch", {"edge": e, "gamma": gamma, "beta": beta, "betas_gammas": bg,
                           "qubit_map": tag}, qasm_text=q)
 
     # --- a sample graph from the reference, p=2, reference map ---------------------------
-    with open(os.path.join(SAMPLES, "3regRand30Node50.dgf")) as f:
-        sample_edges = qo.read_dgf(f.read())
-    cones2 = qo.light_cones(sample_edges, 2, reference_qubit_map=True)
+    cones2 = ref_cones(os.path.join(SAMPLES, "3regRand30Node50.dgf"), 2,
+                       [0.392699, 0.392699, 0.785399, 0.785399])
     for e in range(0, len(cones2), 9):
-        nq, real, _ = cones2[e]
-        q = qo.qaoa_edge_qasm(nq, real, 2, [0.392699, 0.392699, 0.785399, 0.785399])
+        nq, q = cones2[e][0], cones2[e][2]
         add(f"qaoa_sample_p2_e{e}", f"circuits/qaoa_sample_p2_e{e}.qasm",
             qo.qaoa_edge_measure(nq), "stoch", {"edge": e}, qasm_text=q)
 
@@ -339,6 +432,7 @@ def main():
                    "cases": cases}, f, indent=0)
     shutil.rmtree(work)
     print(f"wrote {len(cases)} cases")
+    write_cones()
 
 
 if __name__ == "__main__":

@@ -129,7 +129,27 @@ def test_linegraph_minfill_completes(golden):
         assert net.counts()["uncontracted"] == 1
 
 
-def test_qaoa_edge_text_matches_reference_emission(golden):
+def test_qaoa_edge_text_byte_identical_to_reference(ref_cones):
+    """The product's per-edge F_p text (reference qubit map) equals the text the
+    reference's ExtraData + emission wrote, for every edge, graph, p and angle set
+    recorded in cones.json (src/maxcut.h:31-233, src/maxcut.cpp:172-189)."""
+    import hashlib
+    n = 0
+    for g in ref_cones:
+        with open(os.path.join(GOLDEN_DIR, g["dgf"])) as f:
+            dgf = f.read()
+        for s in g["sets"]:
+            for e, want in enumerate(s["sha256"]):
+                q, m = qa.qaoa_edge_text(dgf, g["p"], True, e, s["betas_gammas"])
+                if str(e) in s["text"]:
+                    assert q == s["text"][str(e)], (g["dgf"], g["p"], e)
+                assert hashlib.sha256(q.encode()).hexdigest()[:32] == want, (g["dgf"], g["p"], e)
+                assert m == "".join("Z " if i < 2 else "T " for i in range(g["qubits"][e]))
+                n += 1
+    assert n > 1000
+
+
+def test_qaoa_edge_text_matches_golden_cases(golden):
     with open(os.path.join(GOLDEN_DIR, "graphs", "3reg_n30_s2024.dgf")) as f:
         dgf = f.read()
     for case in golden:

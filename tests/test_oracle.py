@@ -10,6 +10,7 @@ is compared against it.
 * the pairwise arithmetic equals numpy.einsum on random tensors (index layout,
   operand order and digit significance), and a planted transposition fails it.
 """
+import hashlib
 import math
 import os
 
@@ -219,3 +220,61 @@ def test_qft8_closed_form(golden):
     for case in golden:
         if case["name"].startswith("qft8_"):
             assert abs(case["value"][0] - 1 / 256) < 1e-14
+
+
+def _sha(text):
+    return hashlib.sha256(text.encode()).hexdigest()[:32]
+
+
+def test_light_cones_match_reference_extradata(ref_cones):
+    """The oracle's light cones in the reference's qubit map equal ExtraData's
+    qubitsNeeded and realIterations (src/maxcut.h:105-186) for every edge of three
+    graphs at p = 1, 2 (and 3 on one graph), and its per-edge QASM text is
+    byte-identical to what outputInitialPlusStateToFile + applyU_CsThenU_Bs wrote
+    (src/maxcut.h:191-233) for angle sets that exercise 6-digit printing."""
+    n = 0
+    for g in ref_cones:
+        with open(os.path.join(GOLDEN_DIR, g["dgf"])) as f:
+            edges = qo.read_dgf(f.read())
+        cones = qo.light_cones(edges, g["p"], reference_qubit_map=True)
+        assert [c[0] for c in cones] == g["qubits"], g["dgf"]
+        assert [[list(t) for t in c[1]] for c in cones] == g["iterations"], g["dgf"]
+        for s in g["sets"]:
+            for e, (nq, real, _) in enumerate(cones):
+                text = qo.qaoa_edge_qasm(nq, real, g["p"], s["betas_gammas"])
+                if str(e) in s["text"]:
+                    assert text == s["text"][str(e)], (g["dgf"], g["p"], e)
+                assert _sha(text) == s["sha256"][e], (g["dgf"], g["p"], e, s["betas_gammas"])
+                n += 1
+    assert n > 1000
+
+
+def test_qaoa_fixture_circuits_are_reference_emission(ref_cones, golden):
+    """The qaoa3_ref_* and qaoa_sample_p2_* circuits the golden cases contract are the
+    reference's own per-edge texts."""
+    by_key = {(g["dgf"], g["p"]): g for g in ref_cones}
+    g1 = by_key[("graphs/3reg_n30_s2024.dgf", 1)]
+    g2 = by_key[("samples/3regRand30Node50.dgf", 2)]
+    n = 0
+    for case in golden:
+        if case["name"].startswith("qaoa3_ref_"):
+            s = [s for s in g1["sets"] if s["betas_gammas"] == case["betas_gammas"]][0]
+            want = s["sha256"][case["edge"]]
+        elif case["name"].startswith("qaoa_sample_p2_"):
+            want = g2["sets"][0]["sha256"][case["edge"]]
+        else:
+            continue
+        with open(os.path.join(GOLDEN_DIR, case["qasm"])) as f:
+            assert _sha(f.read()) == want, case["name"]
+        n += 1
+    assert n == 50
+
+
+def test_light_cone_check_catches_planted_mistake(ref_cones):
+    """A plausible restatement error (the reference map with pair.second also fixed to
+    1, i.e. the corrected map) fails the ExtraData comparison on these graphs."""
+    g = [g for g in ref_cones if g["dgf"] == "samples/3regRand30Node50.dgf" and g["p"] == 1][0]
+    with open(os.path.join(GOLDEN_DIR, g["dgf"])) as f:
+        edges = qo.read_dgf(f.read())
+    fixed = qo.light_cones(edges, 1, reference_qubit_map=False)
+    assert [[list(t) for t in c[1]] for c in fixed] != g["iterations"]
