@@ -33,15 +33,35 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
 GRAPH = os.path.join(REPO, "tests", "golden", "graphs", "3reg_n30_s2024.dgf")
+METRIC = "QAOA edge-evals/s & p99 latency, FP32-MFMA vs FP64 plain, rel err\u22641e-5"
 FP32_PEAK_TFLOPS = 157.3  # MI355X dense FP32 (vector and matrix)
-# HBM bytes per edge network of the lane-grouped kernel with intermediates in LDS,
-# from rocprofv3 --pmc FETCH_SIZE (scaled x2.0) and WRITE_SIZE (x1.0) runs of this
-# benchmark; the factors are what the counters read for a 1 GiB device copy in the
-# same process (profiles/k_batch_qaoa_p1_3reg_n30.md)
-PMC_BYTES_PER_NETWORK = (2.0 * 14451.6 + 6544.0) * 1024 / 368640
-# the same for the specialised executor (generated straight-line code, qt_batch_specialised)
-# (as it runs in the timed loop: each launch also sums the previous launch's energies)
-PMC_BYTES_PER_NETWORK_SPECIALISED = (2.0 * 14490.2 + 5824.0) * 1024 / 368640
+# MI355X vector FMA rate without packed FP32 instructions (never emitted here): FP64
+# and FP32 both 78.6 TFLOP/s.  The specialised kernel is plain VALU FMA chains.
+VALU_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+# HBM bytes per edge network measured by rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
+# runs of this benchmark (each scaled by what the counter reads for a 1 GiB device
+# copy in the same process), keyed by (kernel, form, angle sets per launch); the
+# profile file that holds the counts is named beside the figure.
+PMC_BYTES_PER_NETWORK = {
+    ("tables", "c64_4m", 8192): ((2.0 * 14451.6 + 6544.0) * 1024 / 368640, "profiles/k_batch_qaoa_p1_3reg_n30.md"),
+    ("specialised", "c64_4m", 8192): ((2.0 * 14490.2 + 5824.0) * 1024 / 368640,
+                                      "profiles/qt_batch_specialised_qaoa_p1_3reg_n30.md"),
+}
+
+
+def host_info():
+    """CPU model and the cores this process may use (for the CPU baseline)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity_cores": len(os.sched_getaffinity(0))}
 
 
 def angle_sets(n: int, seed: int = 2024):
@@ -79,10 +99,16 @@ def cpu_baseline(dgf: str, bg, target_s: float = 12.0):
     out = subprocess.run([drv, "time", lst, str(reps)], capture_output=True, text=True, timeout=900).stdout
     t = float(out.split("TIME")[1].split()[0])
     jobs = int(out.split("JOBS")[1].split()[0])
-    res = {"value": jobs / t, "unit": "edge-evals/s", "cores": 8, "kind": "reference",
+    # ParallelContract runs its two partitions on two std::threads, then one; the
+    # per-MAC fan-out to 8 threads needs an output rank >= 8, which these light
+    # cones never reach: at most 2 threads run at once
+    res = {"value": jobs / t, "unit": "edge-evals/s", "cores": 2, "kind": "reference",
            "sample": f"{jobs} edge networks ({reps} energy evaluations of the 45-edge graph) through the "
-                     f"reference's ContractionTools::Contract(Stochastic), built -O2 from its sources; it "
-                     f"spawns up to 8 std::threads (2 partition threads, 8 for output rank >= 8); {t:.1f} s"}
+                     f"reference's ContractionTools::Contract(Stochastic), F_p's per-edge loop run serially; "
+                     f"{t:.1f} s",
+           "threads": "2 (ParallelContract's partition threads; no output rank reaches the 8-thread fan-out)",
+           "compiler": "g++ -O2 -g -std=c++11 -pthread (the reference's Makefile builds without -O)",
+           **host_info()}
     # upper bound for the host: independent copies on every core (edges spread over cores)
     cores = min(len(os.sched_getaffinity(0)), 16)
     per = max(1, int(0.25 * target_s / max(t1, 1e-3)))  # about a quarter of the sample's time per copy
@@ -414,7 +440,9 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--sets", type=int, default=8192, help="angle sets per step")
-    ap.add_argument("--form", default="c64_4m", choices=["c64_4m", "c64_3m", "c64_bf16x3", "c128"])
+    ap.add_argument("--form", default="c128", choices=["c64_4m", "c64_3m", "c64_bf16x3", "c128"])
+    ap.add_argument("--latency-steps", type=int, default=2000,
+                    help="steps launched and waited for one at a time for p50 / p99")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-forms", action="store_true", help="skip the other forms' accuracy and speed")
@@ -515,7 +543,7 @@ def main():
     # run; with direct launches of the specialised kernel each launch also sums the
     # previous launch's energies, and the last launch's sums run at the flush
     lat = []
-    for _ in range(min(args.steps, 20)):
+    for _ in range(args.latency_steps):
         t0 = time.perf_counter()
         b.launch()
         b.flush()
@@ -596,27 +624,34 @@ def main():
 
     flops_per_launch = 8.0 * info["cmacs"]
     algorithmic_tflops = flops_per_launch / (kernel_ms * 1e-3) / 1e12
+    n_networks = ne * args.sets
+    elem = 16 if args.form == "c128" else 8
     if specialised:
         # the generated code drops terms with exact-zero constant factors and values
-        # that never reach a result: its executed FLOPs (2 per FMA) face the peak
+        # that never reach a result: its executed FLOPs (2 per FMA) face the VALU peak
         _, fmas = b.specialised()
         executed_flops = 2.0 * fmas
         achieved = executed_flops / (kernel_ms * 1e-3) / 1e12
-        traffic = (PMC_BYTES_PER_NETWORK_SPECIALISED * ne * args.sets
-                   if PMC_BYTES_PER_NETWORK_SPECIALISED is not None else None)
-        kernel_name = "qt_batch_specialised (generated straight-line code, one lane per network)"
+        bound, peak = "valu", VALU_PEAK_TFLOPS
+        kernel_name = "qt_batch_specialised (generated straight-line FMA chains, one lane per network)"
     else:
         executed_flops = flops_per_launch
         achieved = algorithmic_tflops
-        traffic = PMC_BYTES_PER_NETWORK * ne * args.sets if lanes < 64 and not info["global_scratch"] else None
+        bound, peak = "valu", VALU_PEAK_TFLOPS if args.form == "c128" else FP32_PEAK_TFLOPS
         kernel_name = "k_batch_grouped (persistent table executor)" if lanes < 64 else "k_batch (persistent table executor)"
+    # algorithmic HBM bytes of one launch: every instance's parameter block read once
+    # per angle set, one result written per network, the previous launch's results
+    # read by the deferred energy sums, one FP64 energy per angle set
+    param_bytes = args.sets * 32 * elem  # the 32 p parameter elements of an angle set (p = 1)
+    alg_bytes = param_bytes + n_networks * 16 * (2 if specialised else 1) + args.sets * 8
+    pmc = PMC_BYTES_PER_NETWORK.get(("specialised" if specialised else "tables", args.form, args.sets))
     res = {
-        "metric": "QAOA edge-evals/s (p99 step latency), complex64 batched vs complex128 plain, rel err <= 1e-5",
+        "metric": METRIC,
         "value": value, "unit": "edge-evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": 1e3 * wall / args.steps, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "c64" if args.form != "c128" else "c128", "data": "synthetic",
         "config": {"workload": "qaoa_p1_maxcut_3regular_n30_edges45", "angle_sets_per_step": args.sets,
-                   "edge_networks_per_step": ne * args.sets, "form": args.form,
+                   "edge_networks_per_step": n_networks, "form": args.form,
                    "order": "greedy", "executor": "specialised" if specialised else "tables",
                    "conjugate_symmetry": bool(specialised and not args.no_hermitian),
                    "executor_lanes": lanes, "wgs_per_cu": wgs,
@@ -625,21 +660,31 @@ def main():
         "compile_ms": compile_ms, "compile_error": compile_error, "table_executor_ms_per_launch": tables_ms,
         "hip_graph": graph,
         "launch_ms_direct": direct_ms, "launch_ms_graph": graph_ms,
+        "latency_steps": len(lat),
         "p50_step_ms": 1e3 * float(np.percentile(lat, 50)), "p99_step_ms": 1e3 * float(np.percentile(lat, 99)),
         "accuracy": {"max_edge_rel_err_vs_c128": edge_err, "edges_within_1e-5": within,
                      "max_energy_rel_err_vs_c128": energy_err,
                      "max_edge_rel_err_vs_reference": ref_err, "c128_batched_vs_reference": c128_vs_ref,
-                     "floor": "|d| / max(|x|, 1e-3)"},
+                     "floor": "|d| / max(|x|, 1e-3)",
+                     "compared": f"{args.sets} angle sets x {ne} edges against the complex128 table executor; "
+                                 f"set 0 against the reference's own values"},
         "other_forms": by_form,
         "plain_c128_gpu": plain,
         "speedup_vs_plain_c128": (value / plain["value"]) if plain else None,
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": traffic,
+        "roofline": {"bound": bound, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak,
+                     "traffic": pmc[0] * n_networks if pmc else None,
+                     "traffic_source": pmc[1] if pmc else None,
                      "kernel": kernel_name, "kernel_ms_per_launch": kernel_ms,
                      "launch_ms_with_group_sums": launch_ms,
                      "executed_flops_per_launch": executed_flops,
-                     "algorithmic_flops_per_launch": flops_per_launch,
-                     "algorithmic_tflops": algorithmic_tflops},
+                     "algorithmic_bytes_per_launch": alg_bytes,
+                     "hbm_achieved_gbs": alg_bytes / (kernel_ms * 1e-3) / 1e9,
+                     "hbm_frac": alg_bytes / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "peak_note": "VALU FMA rate without packed FP32 (FP64 = FP32 = 78.6 TFLOP/s on MI355X); "
+                                  "the kernel issues no MFMA",
+                     "dense_equivalent_flops_per_launch": flops_per_launch,
+                     "dense_equivalent_tflops": algorithmic_tflops},
     }
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(dgf, bg_ref)

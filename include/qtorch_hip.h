@@ -54,7 +54,8 @@ typedef enum {
 /* Or-ed into a complex64 precision of qt_contract_pair: after the step, C is scaled
  * by a power of two so that its largest |re|, |im| lies in [0.5, 1), and the
  * exponent is kept on the device beside C (value = data * 2^exponent, exponents of A
- * and B included), so that long chains of small amplitudes cannot underflow FP32. */
+ * and B included), so that long chains of small amplitudes cannot underflow FP32.
+ * A step without the flag whose operands carry exponents gives C their sum. */
 #define QT_RESCALE (1 << 8)
 
 /* Bit 30 of a free-axis code marks an axis of A (else of B). */
@@ -65,6 +66,14 @@ typedef enum {
 /* Creates a context on HIP device `device` (stream, caching allocator). */
 qt_status qt_ctx_create(int device, qt_ctx** out);
 qt_status qt_ctx_destroy(qt_ctx* ctx);
+/* Time limit for work on the context, counted from this call: once `seconds` have
+ * passed, qt_contract_pair and qt_batch_launch return QT_TIMEOUT without queueing
+ * anything (checked when each call is made, i.e. between pairwise steps; work
+ * already queued completes).  seconds <= 0 removes the limit.  It replaces the
+ * reference's totTimer / maxTime checks (src/Network.h:50-51,899-910,
+ * src/ContractionTools.h:271,332), which stop a contraction once maxTime seconds
+ * have passed and end in ContractionFailure (src/ContractionTools.h:380-386). */
+qt_status qt_ctx_set_time_limit(qt_ctx* ctx, double seconds);
 /* Blocks until every call issued on the context has finished. */
 qt_status qt_sync(qt_ctx* ctx);
 /* Message of the last error on the context ("" if none).  Owned by the context. */

@@ -38,6 +38,25 @@ typedef struct qth_network qth_network;
 
 const char* qth_last_error(void);
 
+/* The reference's process-wide totTimer / maxTime (src/Network.h:50-51, src/Timer.h):
+ * once the timer is started and maxTime seconds have passed, contractions stop
+ * before their next pairwise step and return QTH_CONTRACTION_FAILURE
+ * (src/Network.h:899-910, src/ContractionTools.h:271,332,380-386).  maxTime
+ * starts at 60; the timer is only started by qth_timer_start and qth_preprocess. */
+int qth_set_max_time(double seconds);
+double qth_get_max_time(void);
+int qth_timer_start(void);
+int qth_timer_reset(void);
+double qth_timer_elapsed(void);
+/* preProcess (src/preprocess.h:28-52): up to 100 seeded stochastic contractions
+ * (attempt i with seed + i), each timed from a fresh timer start with maxTime =
+ * time_threshold; *found = 1 and the first in-time contraction's mCreatedFrom
+ * record (2 ints per step, *n steps; up to cap written) when one finishes within
+ * time_threshold seconds, else *found = 0.  ctx NULL: plan-only contractions.
+ * maxTime keeps time_threshold afterwards, as in the reference. */
+int qth_preprocess(qt_ctx* ctx, const char* qasm, const char* measure, double time_threshold, uint64_t seed,
+                   int32_t* pairs, int cap, int* n, int* found);
+
 /* Network(qasm, measure) (src/Network.h:148-151, 246-304) from in-memory text;
  * measure may be NULL (every qubit traced); base_dir resolves def1/def2 paths. */
 int qth_network_create(qt_ctx* ctx, const char* qasm, const char* measure, const char* base_dir,
@@ -47,6 +66,20 @@ int qth_network_create(qt_ctx* ctx, const char* qasm, const char* measure, const
  * output bitstring x and the value is <x|U|0...0>). */
 int qth_network_create_ex(qt_ctx* ctx, const char* qasm, const char* measure, const char* base_dir,
                           int dim_log2, qth_network** out);
+/* Same, with the output wires of open_qubits[0..n_open) left open (no measurement
+ * or trace node; the reference has no counterpart, where every qubit ends in one, and
+ * gets a probability vector from one measured network per output string,
+ * src/Network.h:199-240): the contraction ends in a tensor over those wires instead
+ * of a scalar (qth_network_result_tensor).  Contract it with QTH_ORDER_GREEDY
+ * (qth_network_order) or a given sequence. */
+int qth_network_create_open(qt_ctx* ctx, const char* qasm, const char* measure, const char* base_dir, int dim_log2,
+                            const int32_t* open_qubits, int n_open, qth_network** out);
+/* The tensor a finished network with open qubits ends in: D^rank interleaved values
+ * (up to cap_elems written) and the qubit of each axis (axis_qubits, rank entries;
+ * digit i of the flat index belongs to axis_qubits[i]).  For D = 4 the digit of a
+ * qubit is 2 ket + bra of the output density matrix, so P(x) is the element whose
+ * digits are 3 x_q.  A network without open qubits gives its scalar (rank 0). */
+int qth_network_result_tensor(qth_network* net, double* values, int64_t cap_elems, int32_t* axis_qubits, int* rank);
 int qth_network_destroy(qth_network* net);
 /* Network::ReduceCircuit (src/Network.h:1015-1118) */
 int qth_network_reduce(qth_network* net);
@@ -93,8 +126,14 @@ int qth_qaoa_energy_plain(qt_ctx* ctx, const char* dgf, int p, int ref_map, cons
 
 /* ---- batched evaluation (device programs, include/qtorch_hip.h qt_batch_*) -------------
  * order: 0 = given sequences (ContractGivenSequence), 1 = greedy cost order,
- *        2 = the reference's stochastic order (seeded), 3 = line-graph min-fill order. */
-enum { QTH_ORDER_GIVEN = 0, QTH_ORDER_GREEDY = 1, QTH_ORDER_STOCHASTIC = 2, QTH_ORDER_LINEGRAPH = 3 };
+ *        2 = the reference's stochastic order (seeded), 3 = line-graph min-fill order,
+ *        4 = circuit sweep (no reference counterpart): local merges, then one growing
+ *            tensor absorbing the rest along a line through the qubits (Fiedler order
+ *            of their interaction graph); for a sliced amplitude network it also
+ *            splits diagonal two-qubit gates (CZ, CPHASE, CRk) into two rank-3
+ *            tensors joined by a bond. */
+enum { QTH_ORDER_GIVEN = 0, QTH_ORDER_GREEDY = 1, QTH_ORDER_STOCHASTIC = 2, QTH_ORDER_LINEGRAPH = 3,
+       QTH_ORDER_SWEEP = 4 };
 
 /* F_p (src/maxcut.cpp:162-204) for many angle sets per launch: one program per edge
  * light cone; instances = angle sets x edges; FP64 energy per angle set. */
@@ -127,6 +166,10 @@ int qth_qaoa_optimize(qt_ctx* ctx, const char* dgf, int p, int precision, int n_
  * projector probabilities; bits receives one int per vertex. */
 int qth_maxcut_final_string(qt_ctx* ctx, const char* dgf, int p, const double* betas_gammas, uint64_t seed,
                             int* bits, int* cut_edges);
+/* Same, also writing the conditional probability P(bit i = 0 | the bits chosen
+ * before it) that decided each bit (cond_probs: one double per vertex; may be NULL). */
+int qth_maxcut_final_string_ex(qt_ctx* ctx, const char* dgf, int p, const double* betas_gammas, uint64_t seed,
+                               int* bits, int* cut_edges, double* cond_probs);
 
 /* Any list of networks (QASM + measurement text) in one batch.  given / given_len:
  * concatenated node-id pairs per network for QTH_ORDER_GIVEN (else NULL). */

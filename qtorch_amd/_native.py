@@ -46,6 +46,7 @@ _HIP_SIGS = {
     "qt_ctx_create": (_c.c_int, [_c.c_int, _c.POINTER(_vp)]),
     "qt_ctx_destroy": (_c.c_int, [_vp]),
     "qt_sync": (_c.c_int, [_vp]),
+    "qt_ctx_set_time_limit": (_c.c_int, [_vp, _c.c_double]),
     "qt_last_error": (_c.c_char_p, [_vp]),
     "qt_mem_reserved": (_c.c_int64, [_vp]),
     "qt_tensor_alloc": (_c.c_int, [_vp, _c.c_int, _c.c_int, _c.c_int, _c.POINTER(_vp)]),
@@ -95,8 +96,18 @@ _HIP_SIGS = {
 }
 _HOST_SIGS = {
     "qth_last_error": (_c.c_char_p, []),
+    "qth_set_max_time": (_c.c_int, [_c.c_double]),
+    "qth_get_max_time": (_c.c_double, []),
+    "qth_timer_start": (_c.c_int, []),
+    "qth_timer_reset": (_c.c_int, []),
+    "qth_timer_elapsed": (_c.c_double, []),
+    "qth_preprocess": (_c.c_int, [_vp, _c.c_char_p, _c.c_char_p, _c.c_double, _c.c_uint64, _i32p, _c.c_int,
+                                  _c.POINTER(_c.c_int), _c.POINTER(_c.c_int)]),
     "qth_network_create": (_c.c_int, [_vp, _c.c_char_p, _c.c_char_p, _c.c_char_p, _c.POINTER(_vp)]),
     "qth_network_create_ex": (_c.c_int, [_vp, _c.c_char_p, _c.c_char_p, _c.c_char_p, _c.c_int, _c.POINTER(_vp)]),
+    "qth_network_create_open": (_c.c_int, [_vp, _c.c_char_p, _c.c_char_p, _c.c_char_p, _c.c_int, _i32p, _c.c_int,
+                                           _c.POINTER(_vp)]),
+    "qth_network_result_tensor": (_c.c_int, [_vp, _dp, _c.c_int64, _i32p, _c.POINTER(_c.c_int)]),
     "qth_network_destroy": (_c.c_int, [_vp]),
     "qth_network_reduce": (_c.c_int, [_vp]),
     "qth_contract_nodes": (_c.c_int, [_vp, _c.c_int, _c.c_int, _c.c_int, _c.POINTER(_c.c_int)]),
@@ -131,6 +142,8 @@ _HOST_SIGS = {
                                      _dp, _c.POINTER(_c.c_int)]),
     "qth_maxcut_final_string": (_c.c_int, [_vp, _c.c_char_p, _c.c_int, _dp, _c.c_uint64, _c.POINTER(_c.c_int),
                                            _c.POINTER(_c.c_int)]),
+    "qth_maxcut_final_string_ex": (_c.c_int, [_vp, _c.c_char_p, _c.c_int, _dp, _c.c_uint64, _c.POINTER(_c.c_int),
+                                              _c.POINTER(_c.c_int), _dp]),
     "qth_netbatch_device": (_vp, [_vp]),
     "qth_netbatch_results": (_c.c_int, [_vp, _dp]),
     "qth_netbatch_cmacs": (_c.c_int, [_vp, _i64p]),
@@ -164,7 +177,7 @@ _HOST_SIGS = {
                                   _c.POINTER(_c.c_int), _c.POINTER(_c.c_int64), _c.POINTER(_c.c_int64)]),
 }
 
-ORDER_GIVEN, ORDER_GREEDY, ORDER_STOCHASTIC, ORDER_LINEGRAPH = 0, 1, 2, 3
+ORDER_GIVEN, ORDER_GREEDY, ORDER_STOCHASTIC, ORDER_LINEGRAPH, ORDER_SWEEP = 0, 1, 2, 3, 4
 
 
 def _missing(name: str):
@@ -260,10 +273,17 @@ class Context:
         msg = hip.qt_last_error(self.handle).decode(errors="replace")
         if rc == QT_INVALID_INPUT:
             raise InvalidFunctionInput(f"{what}: {msg}")
+        if rc == QT_TIMEOUT:
+            raise ContractionFailure(f"{what}: {msg}")
         raise DeviceFailure(f"{what}: status {rc}: {msg}")
 
     def sync(self) -> None:
         self.check(hip.qt_sync(self.handle), "qt_sync")
+
+    def set_time_limit(self, seconds: float) -> None:
+        """Work requested more than `seconds` from now fails with ContractionFailure
+        (QT_TIMEOUT); seconds <= 0 removes the limit."""
+        self.check(hip.qt_ctx_set_time_limit(self.handle, float(seconds)), "qt_ctx_set_time_limit")
 
     def mem_reserved(self) -> int:
         return int(hip.qt_mem_reserved(self.handle))
@@ -366,13 +386,46 @@ class Network:
     values)."""
 
     def __init__(self, qasm: str, measure: Optional[str], ctx: Optional[Context] = None, base_dir: str = "",
-                 dim_log2: int = 2):
+                 dim_log2: int = 2, open_qubits: Optional[Sequence[int]] = None):
         h = _vp()
-        _host_check(host.qth_network_create_ex(ctx.handle if ctx else None, qasm.encode(),
-                                               measure.encode() if measure is not None else None,
-                                               base_dir.encode(), dim_log2, ctypes.byref(h)))
+        m = measure.encode() if measure is not None else None
+        if open_qubits:
+            oq = list(open_qubits)
+            _host_check(host.qth_network_create_open(ctx.handle if ctx else None, qasm.encode(), m,
+                                                     base_dir.encode(), dim_log2, _i32(oq), len(oq), ctypes.byref(h)))
+        else:
+            _host_check(host.qth_network_create_ex(ctx.handle if ctx else None, qasm.encode(), m,
+                                                   base_dir.encode(), dim_log2, ctypes.byref(h)))
         self.handle = h
         self.ctx = ctx
+        self.dim_log2 = dim_log2
+
+    def result_tensor(self) -> Tuple[np.ndarray, List[int]]:
+        """(values, axis qubits) of the tensor a finished network with open qubits ends
+        in (a scalar and [] for a measured network)."""
+        rank = ctypes.c_int(0)
+        axes = (ctypes.c_int32 * 64)()
+        _host_check(host.qth_network_result_tensor(self.handle, None, 0, axes, ctypes.byref(rank)))
+        n = 1 << (self.dim_log2 * rank.value)
+        out = np.zeros(2 * n, dtype=np.float64)
+        _host_check(host.qth_network_result_tensor(self.handle, out.ctypes.data_as(_dp), n, axes, ctypes.byref(rank)))
+        return out.view(np.complex128), list(axes[:rank.value])
+
+    def probabilities(self) -> np.ndarray:
+        """P(x) for every output string x of the open qubits (x's bit j = the j-th open
+        qubit in increasing qubit order): the diagonal of the output density matrix
+        (digits 3 x_q, Liouville picture) or |amplitude|^2 (amplitude picture)."""
+        vals, axq = self.result_tensor()
+        qs = sorted(axq)
+        n = len(qs)
+        x = np.arange(1 << n)
+        idx = np.zeros(1 << n, dtype=np.int64)
+        per = 3 if self.dim_log2 == 2 else 1
+        for axis, q in enumerate(axq):
+            bit = (x >> qs.index(q)) & 1
+            idx += per * bit << (self.dim_log2 * axis)
+        v = vals[idx]
+        return v.real if self.dim_log2 == 2 else np.abs(v) ** 2
 
     def close(self):
         if self.handle:
@@ -461,6 +514,45 @@ class Network:
         out = np.zeros((max(n.value, 1), 8), dtype=np.int32)
         _host_check(host.qth_plan(self.handle, out.ctypes.data_as(_i32p), n.value, ctypes.byref(n)))
         return out[:n.value]
+
+
+def set_max_time(seconds: float) -> None:
+    """The reference's maxTime (src/Network.h:51): contractions stop with
+    ContractionFailure once the started timer passes it."""
+    _host_check(host.qth_set_max_time(float(seconds)))
+
+
+def get_max_time() -> float:
+    return float(host.qth_get_max_time())
+
+
+def timer_start() -> None:
+    """totTimer.start() (src/Timer.h)."""
+    _host_check(host.qth_timer_start())
+
+
+def timer_reset() -> None:
+    _host_check(host.qth_timer_reset())
+
+
+def timer_elapsed() -> float:
+    return float(host.qth_timer_elapsed())
+
+
+def preprocess(qasm: str, measure: Optional[str], time_threshold: float, seed: int = 0,
+               ctx: Optional[Context] = None) -> Tuple[bool, List[Tuple[int, int]]]:
+    """preProcess (src/preprocess.h:28-52): (found, mCreatedFrom record) of the first of
+    up to 100 seeded stochastic contractions that finishes within time_threshold s."""
+    n, found = ctypes.c_int(0), ctypes.c_int(0)
+    h = ctx.handle if ctx is not None else None
+    m = measure.encode() if measure is not None else None
+    cap = 1 << 16  # steps: a network with more nodes cannot be recorded here
+    out = (ctypes.c_int32 * (2 * cap))()
+    _host_check(host.qth_preprocess(h, qasm.encode(), m, float(time_threshold), ctypes.c_uint64(seed), out, cap,
+                                    ctypes.byref(n), ctypes.byref(found)))
+    if n.value > cap:
+        raise InvalidFunctionInput(f"preprocess: {n.value} steps exceed the {cap} this binding records")
+    return bool(found.value), [(out[2 * i], out[2 * i + 1]) for i in range(n.value)]
 
 
 def qaoa_edge_text(dgf: str, p: int, ref_map: bool, e: int, betas_gammas: Sequence[float]) -> Tuple[str, str]:
@@ -691,14 +783,19 @@ def qaoa_optimize(ctx: Context, dgf: str, p: int, precision: int = QT_C64_4M, n_
     return ang, e.value, n.value
 
 
-def maxcut_final_string(ctx: Context, dgf: str, p: int, betas_gammas: Sequence[float], seed: int = 0):
-    """Greedy final cut (src/maxcut.cpp:29-140); returns (bits, cut edges)."""
+def maxcut_final_string(ctx: Context, dgf: str, p: int, betas_gammas: Sequence[float], seed: int = 0,
+                        with_probs: bool = False):
+    """Greedy final cut (src/maxcut.cpp:29-140); returns (bits, cut edges), and the
+    conditional probability that decided each bit when with_probs."""
     _, nv = qaoa_num_edges(dgf)
     bg = (ctypes.c_double * len(betas_gammas))(*betas_gammas)
     bits = (ctypes.c_int * nv)()
     cut = ctypes.c_int()
-    _host_check(host.qth_maxcut_final_string(ctx.handle, dgf.encode(), p, bg, ctypes.c_uint64(seed), bits,
-                                             ctypes.byref(cut)))
+    probs = (ctypes.c_double * nv)()
+    _host_check(host.qth_maxcut_final_string_ex(ctx.handle, dgf.encode(), p, bg, ctypes.c_uint64(seed), bits,
+                                                ctypes.byref(cut), probs))
+    if with_probs:
+        return list(bits), cut.value, list(probs)
     return list(bits), cut.value
 
 

@@ -1218,9 +1218,11 @@ qt_status qt_batch_launch(qt_batch* b) {
     if (!b) return QT_INVALID_INPUT;
     auto* ctx = b->ctx;
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    qt_status s = qt::check_deadline(ctx, "qt_batch_launch");
+    if (s != QT_OK) return s;
     (void)hipSetDevice(ctx->device);
     if (b->n_inst == 0) return QT_OK;
-    const qt_status s = b->graph_exec ? check_hip(ctx, hipGraphLaunch(b->graph_exec, ctx->stream), "hipGraphLaunch")
+    s = b->graph_exec ? check_hip(ctx, hipGraphLaunch(b->graph_exec, ctx->stream), "hipGraphLaunch")
                                       : qt::launch_batch(b);
     if (s != QT_OK) return s;
     return check_hip(ctx, hipEventRecord(b->ev_done, ctx->stream), "event record");

@@ -724,14 +724,18 @@ qt_status run_skinny(Ctx* c, const V* A, const V* B, V* C, const BitMap& m, int 
 }
 
 template <class V, int MODE>
-qt_status run_reduce(Ctx* c, const V* A, const V* B, V* C, const BitMap& m) {
+qt_status run_reduce(Ctx* c, const V* A, const V* B, V* C, const BitMap& m, const Workspace* ws) {
     const int64_t nC = int64_t(1) << m.nbits_c;
     const int64_t nS = int64_t(1) << m.nbits_s;
     int64_t chunk = kThreads * 16;
     while (nS / chunk > 2048) chunk <<= 1;
     const int64_t nblocks = (nS + chunk - 1) / chunk;
     const size_t need = size_t(nblocks * nC) * sizeof(V);
-    if (c->partials_bytes < need) {
+    V* part = nullptr;
+    if (ws) {
+        if (need > ws->bytes) return set_error(c, QT_INVALID_INPUT, "split reduction: partial sums exceed the workspace");
+        part = reinterpret_cast<V*>(ws->base);
+    } else if (c->partials_bytes < need) {
         if (c->partials) {
             (void)hipStreamSynchronize(c->stream);
             (void)hipFree(c->partials);
@@ -742,7 +746,7 @@ qt_status run_reduce(Ctx* c, const V* A, const V* B, V* C, const BitMap& m) {
         if (s != QT_OK) return s;
         c->partials_bytes = need;
     }
-    V* part = static_cast<V*>(c->partials);
+    if (!part) part = static_cast<V*>(c->partials);
     hipLaunchKernelGGL((k_reduce_partial<V, MODE>), dim3(unsigned(nblocks)), dim3(kThreads), 0,
                        c->stream, A, B, part, m, nC, chunk);
     qt_status s = check_hip(c, hipGetLastError(), "k_reduce_partial");
@@ -1030,7 +1034,10 @@ qt_status accumulate_scalar(Ctx* c, const Tensor* t, Tensor* acc) {
     return check_hip(c, hipGetLastError(), "k_accumulate");
 }
 
-qt_status rescale_c64(Ctx* c, const Tensor* A, const Tensor* B, Tensor* C) {
+namespace {
+
+// The device exponent table (first use) and a slot of it for C.
+qt_status acquire_exponent(Ctx* c, Tensor* C) {
     if (!c->d_exp) {
         c->exp_cap = 1 << 20;
         qt_status s = check_hip(c, hipMalloc(reinterpret_cast<void**>(&c->d_exp), size_t(c->exp_cap) * 4), "exponents");
@@ -1044,6 +1051,32 @@ qt_status rescale_c64(Ctx* c, const Tensor* A, const Tensor* B, Tensor* C) {
         C->exp_slot = c->exp_free.back();
         c->exp_free.pop_back();
     }
+    return QT_OK;
+}
+
+__global__ void k_carry_exponent(const int32_t* __restrict__ ea, const int32_t* __restrict__ eb,
+                                 int32_t* __restrict__ ec) {
+    *ec = (ea ? *ea : 0) + (eb ? *eb : 0);
+}
+
+}  // namespace
+
+qt_status carry_exponent(Ctx* c, const Tensor* A, const Tensor* B, Tensor* C) {
+    if (A->exp_slot < 0 && B->exp_slot < 0) {
+        release_exponent(c, C);  // C holds plain values
+        return QT_OK;
+    }
+    qt_status s = acquire_exponent(c, C);
+    if (s != QT_OK) return s;
+    const int32_t* ea = A->exp_slot >= 0 ? c->d_exp + A->exp_slot : nullptr;
+    const int32_t* eb = B->exp_slot >= 0 ? c->d_exp + B->exp_slot : nullptr;
+    hipLaunchKernelGGL(k_carry_exponent, dim3(1), dim3(1), 0, c->stream, ea, eb, c->d_exp + C->exp_slot);
+    return check_hip(c, hipGetLastError(), "k_carry_exponent");
+}
+
+qt_status rescale_c64(Ctx* c, const Tensor* A, const Tensor* B, Tensor* C) {
+    qt_status st = acquire_exponent(c, C);
+    if (st != QT_OK) return st;
     const int64_t n = C->numel;
     float2* x = static_cast<float2*>(C->data);
     const int32_t* ea = A->exp_slot >= 0 ? c->d_exp + A->exp_slot : nullptr;
@@ -1084,7 +1117,8 @@ namespace {
 
 // Large steps: permute into GEMM layout (skipped when already there), MFMA GEMM.
 template <class V, int MODE>
-qt_status run_gemm(Ctx* c, const Tensor* A, const Tensor* B, Tensor* C, const BitMap& m, int mbits) {
+qt_status run_gemm(Ctx* c, const Tensor* A, const Tensor* B, Tensor* C, const BitMap& m, int mbits,
+                   const Workspace* ws) {
     // permute runs of 64 elements (1 KB complex128, 512 B complex64); tiles of up to
     // 2^12 elements (64 KB of LDS in complex128)
     constexpr int kPermLowBits = 6;
@@ -1099,19 +1133,24 @@ qt_status run_gemm(Ctx* c, const Tensor* A, const Tensor* B, Tensor* C, const Bi
     const V* bp = static_cast<const V*>(B->data);
     void* abuf = nullptr;
     void* bbuf = nullptr;
+    const bool perm_a = !is_identity(src_a, mbits + sb), perm_b = !is_identity(src_b, nbits_n + sb);
+    // a workspace holds A's permute at its start and B's after it (256-byte aligned)
+    const size_t ws_b = perm_a ? (A->bytes + 255) / 256 * 256 : 0;
+    if (ws && ws_b + (perm_b ? B->bytes : 0) > ws->bytes)
+        return set_error(c, QT_INVALID_INPUT, "gemm: operand permutes exceed the workspace");
     qt_status s = QT_OK;
-    if (!is_identity(src_a, mbits + sb)) {
-        hipError_t e;
-        abuf = c->pool.alloc(A->bytes, &e);
+    if (perm_a) {
+        hipError_t e = hipSuccess;
+        abuf = ws ? static_cast<void*>(ws->base) : c->pool.alloc(A->bytes, &e);
         if (!abuf) return check_hip(c, e, "gemm scratch A");
         s = permute(c, A->data, abuf, A->dtype, make_perm_plan(mbits + sb, src_a, kPermLowBits));
         ap = static_cast<const V*>(abuf);
     }
-    if (s == QT_OK && !is_identity(src_b, nbits_n + sb)) {
-        hipError_t e;
-        bbuf = c->pool.alloc(B->bytes, &e);
+    if (s == QT_OK && perm_b) {
+        hipError_t e = hipSuccess;
+        bbuf = ws ? static_cast<void*>(ws->base + ws_b) : c->pool.alloc(B->bytes, &e);
         if (!bbuf) {
-            c->pool.release(abuf);
+            if (!ws) c->pool.release(abuf);
             return check_hip(c, e, "gemm scratch B");
         }
         s = permute(c, B->data, bbuf, B->dtype, make_perm_plan(nbits_n + sb, src_b, kPermLowBits));
@@ -1147,8 +1186,10 @@ qt_status run_gemm(Ctx* c, const Tensor* A, const Tensor* B, Tensor* C, const Bi
         }
     }
     // stream-ordered reuse: later work on this stream may take these blocks
-    c->pool.release(abuf);
-    c->pool.release(bbuf);
+    if (!ws) {
+        c->pool.release(abuf);
+        c->pool.release(bbuf);
+    }
     return s;
 }
 
@@ -1161,13 +1202,13 @@ inline bool skinny_shape(const BitMap& m, bool split, int mbits) {
 
 template <class V, int MODE>
 qt_status dispatch_small(Ctx* c, const Tensor* A, const Tensor* B, Tensor* C, const BitMap& m, bool split,
-                         int mbits) {
+                         int mbits, const Workspace* ws) {
     const V* a = static_cast<const V*>(A->data);
     const V* b = static_cast<const V*>(B->data);
     V* cc = static_cast<V*>(C->data);
     const int64_t nC = int64_t(1) << m.nbits_c;
     const int64_t nS = int64_t(1) << m.nbits_s;
-    if (nC <= kReduceMaxOut && nS >= 8192) return run_reduce<V, MODE>(c, a, b, cc, m);
+    if (nC <= kReduceMaxOut && nS >= 8192) return run_reduce<V, MODE>(c, a, b, cc, m, ws);
     if (skinny_shape(m, split, mbits)) return run_skinny<V, MODE>(c, a, b, cc, m, mbits);
     return run_direct<V, MODE>(c, a, b, cc, m);
 }
@@ -1175,7 +1216,7 @@ qt_status dispatch_small(Ctx* c, const Tensor* A, const Tensor* B, Tensor* C, co
 }  // namespace
 
 qt_status contract_pair(Ctx* c, const Tensor* A, const Tensor* B, Tensor* C, const BitMap& m,
-                        int precision) {
+                        int precision, const Workspace* ws) {
     // GEMM shape: C's low bits all from A, high bits all from B
     int mbits = 0;
     while (mbits < m.nbits_c && m.c_src[mbits] < 64) ++mbits;
@@ -1190,21 +1231,21 @@ qt_status contract_pair(Ctx* c, const Tensor* A, const Tensor* B, Tensor* C, con
     // outputs in complex64, from 2^24 in complex128)
     if (A->dtype == QT_DTYPE_C128) {
         const bool short_k = m.nbits_s >= 3 && m.nbits_c >= 24;
-        if (gemm && (m.nbits_s >= 4 || short_k)) return run_gemm<double2, MODE_PLAIN>(c, A, B, C, m, mbits);
-        return dispatch_small<double2, MODE_PLAIN>(c, A, B, C, m, split, mbits);
+        if (gemm && (m.nbits_s >= 4 || short_k)) return run_gemm<double2, MODE_PLAIN>(c, A, B, C, m, mbits, ws);
+        return dispatch_small<double2, MODE_PLAIN>(c, A, B, C, m, split, mbits, ws);
     }
     const bool short_k = m.nbits_s >= 3 && m.nbits_c >= 16;
     const bool gemm64 = gemm && (m.nbits_s >= 5 || short_k);  // k-tiles of 32
     switch (precision) {
         case QT_C64_4M:
-            return gemm64 ? run_gemm<float2, MODE_4M>(c, A, B, C, m, mbits)
-                          : dispatch_small<float2, MODE_4M>(c, A, B, C, m, split, mbits);
+            return gemm64 ? run_gemm<float2, MODE_4M>(c, A, B, C, m, mbits, ws)
+                          : dispatch_small<float2, MODE_4M>(c, A, B, C, m, split, mbits, ws);
         case QT_C64_3M:
-            return gemm64 ? run_gemm<float2, MODE_3M>(c, A, B, C, m, mbits)
-                          : dispatch_small<float2, MODE_3M>(c, A, B, C, m, split, mbits);
+            return gemm64 ? run_gemm<float2, MODE_3M>(c, A, B, C, m, mbits, ws)
+                          : dispatch_small<float2, MODE_3M>(c, A, B, C, m, split, mbits, ws);
         default:
-            return gemm64 ? run_gemm<float2, MODE_BF16X3>(c, A, B, C, m, mbits)
-                          : dispatch_small<float2, MODE_BF16X3>(c, A, B, C, m, split, mbits);
+            return gemm64 ? run_gemm<float2, MODE_BF16X3>(c, A, B, C, m, mbits, ws)
+                          : dispatch_small<float2, MODE_BF16X3>(c, A, B, C, m, split, mbits, ws);
     }
 }
 

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cstdint>
 #include <map>
 #include <memory>
@@ -50,7 +51,13 @@ struct Ctx {
     std::vector<int32_t> exp_free;
     int32_t exp_cap = 0;
     float* d_absmax = nullptr;  // per-workgroup maxima of one rescale
+    // qt_ctx_set_time_limit: work requested after this instant returns QT_TIMEOUT
+    bool has_deadline = false;
+    std::chrono::steady_clock::time_point deadline{};
 };
+
+// QT_TIMEOUT (with the message set) when the context's time limit has passed.
+qt_status check_deadline(Ctx* c, const char* what);
 
 struct Tensor {
     void* data = nullptr;
@@ -101,8 +108,16 @@ qt_status check_hip(Ctx* c, hipError_t e, const char* what);
 
 // Kernel launchers (qt_contract.hip)
 qt_status launch_fill_zero(Ctx* c, void* p, size_t bytes);
+// Device scratch a contraction may use instead of allocating: the GEMM operand
+// permutes and the split reduction's partial sums come from [base, base + bytes).
+struct Workspace {
+    char* base = nullptr;
+    size_t bytes = 0;
+};
+// ws == nullptr: scratch from the context's caching allocator.  With a workspace,
+// a step whose scratch exceeds it returns QT_INVALID_INPUT before launching anything.
 qt_status contract_pair(Ctx* c, const Tensor* A, const Tensor* B, Tensor* C, const BitMap& m,
-                        int precision);
+                        int precision, const Workspace* ws = nullptr);
 qt_status permute(Ctx* c, const void* in, void* out, int dtype, const PermPlan& p);
 // Scales complex64 C by 2^-e so that its largest |re|, |im| lies in [0.5, 1) and
 // records exp(C) = exp(A) + exp(B) + e on the device (no host synchronisation).
@@ -110,6 +125,9 @@ qt_status rescale_c64(Ctx* c, const Tensor* A, const Tensor* B, Tensor* C);
 // Reads a tensor's exponent (0 if never rescaled); synchronises.
 qt_status read_exponent(Ctx* c, const Tensor* t, int32_t* e);
 void release_exponent(Ctx* c, Tensor* t);
+// exp(C) = exp(A) + exp(B) on the device, for a step without rescaling whose
+// operands carry exponents from earlier rescaled steps.
+qt_status carry_exponent(Ctx* c, const Tensor* A, const Tensor* B, Tensor* C);
 // acc (complex128 scalar) += t (scalar, exponent applied), in FP64 on the stream.
 qt_status accumulate_scalar(Ctx* c, const Tensor* t, Tensor* acc);
 // tl_max: input-low / output-low bits of a tile (contiguous runs of 2^tl_max elements)

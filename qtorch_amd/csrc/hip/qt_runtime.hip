@@ -1,6 +1,8 @@
 // Context, caching allocator and tensor entry points of libqtorch_hip.so.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 
 #include "qt_internal.h"
@@ -67,6 +69,11 @@ qt_status set_error(Ctx* c, qt_status s, const std::string& msg) {
     return s;
 }
 
+qt_status check_deadline(Ctx* c, const char* what) {
+    if (!c->has_deadline || std::chrono::steady_clock::now() < c->deadline) return QT_OK;
+    return set_error(c, QT_TIMEOUT, std::string(what) + ": the context's time limit has passed");
+}
+
 qt_status check_hip(Ctx* c, hipError_t e, const char* what) {
     if (e == hipSuccess) return QT_OK;
     (void)hipGetLastError();
@@ -120,6 +127,17 @@ qt_status qt_ctx_destroy(qt_ctx* c) {
         (void)hipStreamDestroy(c->impl.stream);
     }
     delete c;
+    return QT_OK;
+}
+
+qt_status qt_ctx_set_time_limit(qt_ctx* c, double seconds) {
+    if (!c || !(seconds == seconds)) return QT_INVALID_INPUT;
+    std::lock_guard<std::recursive_mutex> g(c->impl.mu);
+    c->impl.has_deadline = seconds > 0;
+    if (c->impl.has_deadline)
+        c->impl.deadline = std::chrono::steady_clock::now() +
+                           std::chrono::duration_cast<std::chrono::steady_clock::duration>(
+                               std::chrono::duration<double>(std::min(seconds, 1e9)));
     return QT_OK;
 }
 
@@ -359,10 +377,16 @@ qt_status qt_tensor_alloc_empty(qt_ctx* c, int rank, int dim_log2, int dtype, qt
 int64_t qt_tensor_numel(const qt_tensor* t) { return t ? t->impl.numel : 0; }
 int qt_tensor_rank(const qt_tensor* t) { return t ? t->impl.rank : -1; }
 
-qt_status qt_contract_pair(qt_ctx* c, const qt_tensor* A, int rank_a, const qt_tensor* B,
-                           int rank_b, int k, const int32_t* pair_a, const int32_t* pair_b,
-                           int rank_c, const int32_t* free_src, qt_tensor* C, int precision,
-                           int64_t* cmacs) {
+}  // extern "C"
+
+namespace {
+
+// qt_contract_pair: argument checks, the bit map, the launch.  ws != nullptr: the
+// step's scratch comes from the workspace and nothing is allocated (QT_RESCALE, whose
+// exponent table is allocated on first use, is refused then).
+qt_status contract_pair_entry(qt_ctx* c, const qt_tensor* A, int rank_a, const qt_tensor* B, int rank_b, int k,
+                              const int32_t* pair_a, const int32_t* pair_b, int rank_c, const int32_t* free_src,
+                              qt_tensor* C, int precision, int64_t* cmacs, const qt::Workspace* ws) {
     if (!c) return QT_INVALID_INPUT;
     auto* ctx = &c->impl;
     if (!A || !B || !C || (k > 0 && (!pair_a || !pair_b)) || (rank_c > 0 && !free_src))
@@ -382,6 +406,8 @@ qt_status qt_contract_pair(qt_ctx* c, const qt_tensor* A, int rank_a, const qt_t
     const bool c128 = a.dtype == QT_DTYPE_C128;
     const bool rescale = (precision & QT_RESCALE) != 0;
     precision &= ~QT_RESCALE;
+    if (ws && rescale)
+        return set_error(ctx, QT_INVALID_INPUT, "qt_contract_pair: QT_RESCALE needs the context's allocator");
     if ((c128 && (precision != QT_PLAIN_C128 || rescale)) ||
         (!c128 && precision != QT_C64_4M && precision != QT_C64_3M && precision != QT_C64_BF16X3))
         return set_error(ctx, QT_INVALID_INPUT, "qt_contract_pair: precision does not fit dtype");
@@ -389,7 +415,8 @@ qt_status qt_contract_pair(qt_ctx* c, const qt_tensor* A, int rank_a, const qt_t
     if ((rank_c + k) * dl > 62)
         return set_error(ctx, QT_INVALID_INPUT, "qt_contract_pair: index space too large");
     // every axis of A and of B is used exactly once (summed or free)
-    std::vector<int> used_a(rank_a, 0), used_b(rank_b, 0);
+    if (rank_a > 64 || rank_b > 64) return set_error(ctx, QT_INVALID_INPUT, "qt_contract_pair: rank above 64");
+    int used_a[64] = {0}, used_b[64] = {0};
     qt::BitMap m;
     std::memset(&m, 0, sizeof m);
     m.nbits_c = rank_c * dl;
@@ -414,12 +441,26 @@ qt_status qt_contract_pair(qt_ctx* c, const qt_tensor* A, int rank_a, const qt_t
             m.c_src[i * dl + t] = uint8_t(from_a ? ax * dl + t : 64 + ax * dl + t);
     }
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    qt_status s = qt::check_deadline(ctx, "qt_contract_pair");
+    if (s != QT_OK) return s;
     (void)hipSetDevice(ctx->device);
-    qt_status s = qt::contract_pair(ctx, &a, &b, &cc, m, precision);
-    if (s == QT_OK && rescale) s = qt::rescale_c64(ctx, &a, &b, &cc);
-    else if (s == QT_OK) qt::release_exponent(ctx, &cc);  // C holds plain values
+    s = qt::contract_pair(ctx, &a, &b, &cc, m, precision, ws);
+    // without QT_RESCALE, C carries its operands' exponents (0 when they have none)
+    if (s == QT_OK) s = rescale ? qt::rescale_c64(ctx, &a, &b, &cc) : qt::carry_exponent(ctx, &a, &b, &cc);
     if (s == QT_OK && cmacs) *cmacs = int64_t(1) << ((rank_c + k) * dl);
     return s;
+}
+
+}  // namespace
+
+extern "C" {
+
+qt_status qt_contract_pair(qt_ctx* c, const qt_tensor* A, int rank_a, const qt_tensor* B,
+                           int rank_b, int k, const int32_t* pair_a, const int32_t* pair_b,
+                           int rank_c, const int32_t* free_src, qt_tensor* C, int precision,
+                           int64_t* cmacs) {
+    return contract_pair_entry(c, A, rank_a, B, rank_b, k, pair_a, pair_b, rank_c, free_src, C, precision, cmacs,
+                               nullptr);
 }
 
 }  // extern "C"

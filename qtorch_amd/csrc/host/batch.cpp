@@ -2,8 +2,10 @@
 // reference's orderings, or a greedy cost order), compiled into device programs
 // and run many at a time by the persistent batch kernel (include/qtorch_hip.h).
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <map>
+#include <random>
 #include <sstream>
 #include <unordered_map>
 
@@ -14,6 +16,7 @@ namespace qtorch {
 void CheckQt(qt_ctx* ctx, qt_status s, const char* what) {
     if (s == QT_OK) return;
     if (s == QT_INVALID_INPUT) throw DeviceFailure(std::string(what) + ": invalid input: " + qt_last_error(ctx));
+    if (s == QT_TIMEOUT) throw ContractionFailure();
     throw DeviceFailure(std::string(what) + ": " + qt_last_error(ctx));
 }
 
@@ -67,6 +70,187 @@ void GreedyContract(Network& net) {
             best_b = sc[1];
         }
         net.ContractNodes(best_a, best_b, 1 << 20);
+    }
+}
+
+namespace {
+
+// Second eigenvector of the weighted Laplacian L = D - W of an n-vertex graph (the
+// Fiedler vector) by power iteration on c I - L with the constant vector projected
+// out; c bounds L's spectrum (Gershgorin).  Deterministic start.
+std::vector<double> FiedlerVector(int n, const std::vector<std::vector<std::pair<int, double>>>& adj) {
+    std::vector<double> deg(static_cast<size_t>(n), 0.0), x(static_cast<size_t>(n)), y(static_cast<size_t>(n));
+    double c = 1.0;
+    for (int i = 0; i < n; ++i) {
+        for (const auto& [j, w] : adj[size_t(i)]) deg[size_t(i)] += w;
+        c = std::max(c, 2.0 * deg[size_t(i)]);
+    }
+    for (int i = 0; i < n; ++i) x[size_t(i)] = double(i) - 0.5 * double(n - 1) + 0.01 * std::sin(double(i));
+    for (int it = 0; it < 20000; ++it) {
+        double mean = 0.0;
+        for (double v : x) mean += v;
+        mean /= n;
+        for (auto& v : x) v -= mean;
+        for (int i = 0; i < n; ++i) {
+            double lx = deg[size_t(i)] * x[size_t(i)];
+            for (const auto& [j, w] : adj[size_t(i)]) lx -= w * x[size_t(j)];
+            y[size_t(i)] = c * x[size_t(i)] - lx;
+        }
+        double norm = 0.0;
+        for (double v : y) norm += v * v;
+        norm = std::sqrt(norm);
+        if (norm == 0.0) break;
+        for (int i = 0; i < n; ++i) x[size_t(i)] = y[size_t(i)] / norm;
+    }
+    return x;
+}
+
+}  // namespace
+
+// Circuit sweep order (no reference counterpart; the reference's orderings are
+// ParallelContract and LGContract).  The qubits are arranged on a line by the
+// Fiedler vector of their interaction graph (qubits joined by multi-qubit tensors),
+// which puts few interactions across every cut of the line; each leaf gets the key
+// (its qubit's place on that line, its place along the qubit's wire).  First every
+// pair of connected tensors whose result is no larger than its larger operand and
+// has at most 4 axes is merged (single-qubit gates into their neighbours; a cluster
+// keeps its smallest key); then the clusters are absorbed one by one, in key order,
+// into one growing tensor whose axes are the wires cut by the sweep line (a cluster
+// not yet connected to it waits until it is).
+void SweepContract(Network& net, uint64_t variant) {
+    const auto& all = net.GetAllNodes();
+    const int n_q = net.GetNumQubits(), n_orig = net.GetNumOriginalNodes();
+    if (n_q < 1) throw InvalidFunctionInput();
+    std::unordered_map<const Node*, int> orig_of;
+    for (int i = 0; i < n_orig; ++i) orig_of[all[size_t(i)].get()] = i;
+    auto qubit_of = [&](const Node* n) {
+        const auto& wn = n->GetWireNumber();
+        return wn.empty() ? 0 : wn[0];
+    };
+    // interaction graph: qubits of the two ends of every wire between leaves on
+    // different qubits, and the qubits of each multi-qubit leaf
+    std::vector<std::map<int, double>> w(static_cast<size_t>(n_q));
+    auto link = [&](int a, int b) {
+        if (a == b || a < 0 || b < 0 || a >= n_q || b >= n_q) return;
+        w[size_t(a)][b] += 1.0;
+        w[size_t(b)][a] += 1.0;
+    };
+    for (int i = 0; i < n_orig; ++i) {
+        const Node* n = all[size_t(i)].get();
+        const auto& wn = n->GetWireNumber();
+        for (size_t j = 1; j < wn.size(); ++j) link(wn[0], wn[j]);
+        for (const auto& wp : const_cast<Node*>(n)->GetWires()) {
+            const auto a = wp->GetNodeA().lock(), b = wp->GetNodeB().lock();
+            if (a && b) link(qubit_of(a.get()), qubit_of(b.get()));
+        }
+    }
+    std::vector<std::vector<std::pair<int, double>>> adj(static_cast<size_t>(n_q));
+    for (int q = 0; q < n_q; ++q)
+        for (const auto& [r, x] : w[size_t(q)]) adj[size_t(q)].push_back({r, x});
+    std::vector<double> fv = FiedlerVector(n_q, adj);
+    if (variant > 0) {
+        // another line: the Fiedler values perturbed by a seeded fraction of their
+        // spread (neighbours on the line may swap), possibly reversed
+        std::mt19937_64 rng(variant);
+        std::normal_distribution<double> noise(0.0, 1.0);
+        const auto [lo, hi] = std::minmax_element(fv.begin(), fv.end());
+        const double scale = (*hi - *lo) / double(n_q) * (0.5 + double(variant % 4));
+        const double sign = (variant / 4) % 2 ? -1.0 : 1.0;
+        for (auto& v : fv) v = sign * v + scale * noise(rng);
+    }
+    std::vector<int> line(static_cast<size_t>(n_q));
+    for (int q = 0; q < n_q; ++q) line[size_t(q)] = q;
+    std::stable_sort(line.begin(), line.end(), [&](int a, int b) { return fv[size_t(a)] < fv[size_t(b)]; });
+    std::vector<int> place(static_cast<size_t>(n_q));
+    for (int i = 0; i < n_q; ++i) place[size_t(line[size_t(i)])] = i;
+    // keys of the leaves: (place of the qubit, place along its wire)
+    std::unordered_map<const Node*, std::pair<int, int>> key;
+    const auto& by_wire = net.GetNodesByWire();
+    for (int q = 0; q < n_q; ++q)
+        for (size_t t = 0; t < by_wire[size_t(q)].size(); ++t) {
+            const Node* n = by_wire[size_t(q)][t].get();
+            const std::pair<int, int> k{place[size_t(q)], int(t)};
+            auto it = key.find(n);
+            if (it == key.end() || k < it->second) key[n] = k;
+        }
+    for (int i = 0; i < n_orig; ++i) key.emplace(all[size_t(i)].get(), std::make_pair(n_q, i));
+    auto neighbours = [](const std::shared_ptr<Node>& a) {
+        std::vector<std::shared_ptr<Node>> out;
+        for (const auto& wp : a->GetWires()) {
+            if (wp->IsContracted()) continue;
+            auto b = wp->GetNodeA().lock() == a ? wp->GetNodeB().lock() : wp->GetNodeA().lock();
+            if (b && b != a && !b->mContracted && std::find(out.begin(), out.end(), b) == out.end()) out.push_back(b);
+        }
+        return out;
+    };
+    auto contract = [&](const std::shared_ptr<Node>& a, const std::shared_ptr<Node>& b) {
+        const auto k = std::min(key.at(a.get()), key.at(b.get()));
+        auto c = net.ContractNodes(a, b, 1 << 20);
+        if (c) key[c.get()] = k;
+        return c;
+    };
+    // phase 1: merges that do not grow a tensor beyond its larger operand or 4 axes
+    bool changed = true;
+    while (changed && !net.IsDone()) {
+        changed = false;
+        auto un = net.GetUncontractedNodes();
+        std::sort(un.begin(), un.end(), [&](const auto& x, const auto& y) { return key.at(x.get()) < key.at(y.get()); });
+        for (const auto& a : un) {
+            if (a->mContracted) continue;
+            for (const auto& b : neighbours(a)) {
+                const int k = SharedWires(a, b), rc = a->mRank + b->mRank - 2 * k;
+                if (rc <= std::max(a->mRank, b->mRank) && rc <= 4) {
+                    contract(a, b);
+                    changed = true;
+                    break;
+                }
+            }
+            if (net.IsDone()) break;
+        }
+    }
+    if (net.IsDone()) return;
+    // phase 2: the sweep
+    auto rest = net.GetUncontractedNodes();
+    std::sort(rest.begin(), rest.end(), [&](const auto& x, const auto& y) { return key.at(x.get()) < key.at(y.get()); });
+    std::shared_ptr<Node> acc = rest.front();
+    std::vector<std::shared_ptr<Node>> waiting(rest.begin() + 1, rest.end());  // in key order
+    while (!net.IsDone()) {
+        bool progressed = false;
+        for (size_t i = 0; i < waiting.size(); ++i) {
+            if (SharedWires(acc, waiting[i]) == 0) continue;
+            auto b = waiting[i];
+            waiting.erase(waiting.begin() + long(i));
+            auto c = contract(acc, b);
+            if (net.IsDone()) return;
+            acc = c;
+            progressed = true;
+            break;
+        }
+        if (progressed) continue;
+        // only disconnected parts are left: contract them on their own, then the scalars
+        if (waiting.empty()) throw ContractionFailure();
+        std::shared_ptr<Node> other;
+        for (const auto& x : waiting)
+            if (x->mRank == 0) other = x;
+        if (acc->mRank == 0 && other) {
+            waiting.erase(std::find(waiting.begin(), waiting.end(), other));
+            auto c = contract(acc, other);
+            if (net.IsDone()) return;
+            acc = c;
+            continue;
+        }
+        // contract the first waiting cluster with its neighbours until it is a scalar
+        auto x = waiting.front();
+        auto nb = neighbours(x);
+        auto it = std::find_if(waiting.begin() + 1, waiting.end(),
+                               [&](const auto& y) { return std::find(nb.begin(), nb.end(), y) != nb.end(); });
+        if (it == waiting.end()) throw ContractionFailure();
+        auto y = *it;
+        waiting.erase(it);
+        waiting.erase(waiting.begin());
+        auto c = contract(x, y);
+        if (net.IsDone()) return;
+        waiting.insert(waiting.begin(), c);
     }
 }
 
@@ -244,6 +428,8 @@ void OrderNetwork(Network& net, int order, uint64_t seed, const std::vector<std:
             net.ContractNodes(all[size_t(p.first)], all[size_t(p.second)], 100);
         }
         if (!net.IsDone()) throw ContractionFailure();
+    } else if (order == kOrderSweep) {
+        SweepContract(net, 0);
     } else if (order == kOrderLineGraph) {
         std::shared_ptr<Network> alias(&net, [](Network*) {});
         LineGraph lg(alias);

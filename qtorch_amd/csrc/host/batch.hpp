@@ -10,10 +10,14 @@
 
 namespace qtorch {
 
-enum { kOrderGiven = 0, kOrderGreedy = 1, kOrderStochastic = 2, kOrderLineGraph = 3 };
+enum { kOrderGiven = 0, kOrderGreedy = 1, kOrderStochastic = 2, kOrderLineGraph = 3, kOrderSweep = 4 };
 
 // Greedy cost order on a (plan-only or device) network.
 void GreedyContract(Network& net);
+// Circuit sweep order: local merges, then one growing tensor absorbing the rest in
+// the order of a line through the qubits (see batch.cpp).  variant 0 takes the line
+// of the interaction graph's Fiedler vector; others perturb it (seeded).
+void SweepContract(Network& net, uint64_t variant);
 // Contracts a network in the requested order (records its plan).
 void OrderNetwork(Network& net, int order, uint64_t seed, const std::vector<std::pair<int, int>>* given);
 

@@ -92,6 +92,44 @@ extern "C" {
 
 const char* qth_last_error(void) { return g_err.c_str(); }
 
+int qth_set_max_time(double seconds) {
+    if (!(seconds == seconds)) return QTH_INVALID_FUNCTION_INPUT;
+    maxTime = seconds;
+    return QTH_OK;
+}
+
+double qth_get_max_time(void) { return maxTime; }
+
+int qth_timer_start(void) {
+    totTimer = Timer();
+    totTimer.start();
+    return QTH_OK;
+}
+
+int qth_timer_reset(void) {
+    totTimer.reset();
+    return QTH_OK;
+}
+
+double qth_timer_elapsed(void) { return totTimer.getElapsed(); }
+
+int qth_preprocess(qt_ctx* ctx, const char* qasm, const char* measure, double time_threshold, uint64_t seed,
+                   int32_t* pairs, int cap, int* n, int* found) {
+    if (!qasm || !n || !found || cap < 0 || (cap > 0 && !pairs)) return QTH_INVALID_FUNCTION_INPUT;
+    *n = 0;
+    *found = 0;
+    return Guard([&] {
+        std::vector<std::pair<int, int>> seq;
+        const std::string m = measure ? measure : "";
+        *found = preProcess(std::string(qasm), measure ? &m : nullptr, ctx, seed, seq, time_threshold) ? 1 : 0;
+        *n = int(seq.size());
+        for (int i = 0; i < *n && i < cap; ++i) {
+            pairs[2 * i] = seq[size_t(i)].first;
+            pairs[2 * i + 1] = seq[size_t(i)].second;
+        }
+    });
+}
+
 int qth_network_create(qt_ctx* ctx, const char* qasm, const char* measure, const char* base_dir,
                        qth_network** out) {
     if (!qasm || !out) return QTH_INVALID_FUNCTION_INPUT;
@@ -120,6 +158,39 @@ int qth_network_create_ex(qt_ctx* ctx, const char* qasm, const char* measure, co
         auto h = std::make_unique<qth_network>();
         h->net = std::make_shared<Network>(std::string(qasm), measure ? &m : nullptr, opt);
         *out = h.release();
+    });
+}
+
+int qth_network_create_open(qt_ctx* ctx, const char* qasm, const char* measure, const char* base_dir, int dim_log2,
+                            const int32_t* open_qubits, int n_open, qth_network** out) {
+    if (!qasm || !out || (dim_log2 != 1 && dim_log2 != 2) || n_open < 0 || (n_open > 0 && !open_qubits))
+        return QTH_INVALID_FUNCTION_INPUT;
+    *out = nullptr;
+    return Guard([&] {
+        NetworkOptions opt;
+        opt.ctx = ctx;
+        opt.base_dir = base_dir ? base_dir : "";
+        opt.dim_log2 = dim_log2;
+        opt.open_qubits.assign(open_qubits, open_qubits + n_open);
+        std::string m = measure ? measure : "";
+        auto h = std::make_unique<qth_network>();
+        h->net = std::make_shared<Network>(std::string(qasm), measure ? &m : nullptr, opt);
+        *out = h.release();
+    });
+}
+
+int qth_network_result_tensor(qth_network* h, double* values, int64_t cap_elems, int32_t* axis_qubits, int* rank) {
+    if (!h || !rank || cap_elems < 0 || (cap_elems > 0 && !values)) return QTH_INVALID_FUNCTION_INPUT;
+    return Guard([&] {
+        std::vector<int> q;
+        const auto v = h->net->GetFinalTensor(&q);
+        *rank = int(q.size());
+        if (axis_qubits)
+            for (size_t i = 0; i < q.size(); ++i) axis_qubits[i] = q[i];
+        for (int64_t i = 0; i < cap_elems && i < int64_t(v.size()); ++i) {
+            values[2 * i] = v[size_t(i)].real();
+            values[2 * i + 1] = v[size_t(i)].imag();
+        }
     });
 }
 
@@ -375,6 +446,17 @@ int qth_maxcut_final_string(qt_ctx* ctx, const char* dgf, int p, const double* b
     return Guard([&] {
         const auto b = MaxcutFinalString(ctx, dgf, p, std::vector<double>(bg, bg + 2 * p), seed, cut_edges);
         std::memcpy(bits, b.data(), b.size() * sizeof(int));
+    });
+}
+
+int qth_maxcut_final_string_ex(qt_ctx* ctx, const char* dgf, int p, const double* bg, uint64_t seed, int* bits,
+                               int* cut_edges, double* cond_probs) {
+    if (!ctx || !dgf || !bg || !bits || p < 1) return QTH_INVALID_FUNCTION_INPUT;
+    return Guard([&] {
+        std::vector<double> cp;
+        const auto b = MaxcutFinalString(ctx, dgf, p, std::vector<double>(bg, bg + 2 * p), seed, cut_edges, &cp);
+        std::memcpy(bits, b.data(), b.size() * sizeof(int));
+        if (cond_probs) std::memcpy(cond_probs, cp.data(), cp.size() * sizeof(double));
     });
 }
 

@@ -102,6 +102,7 @@ std::shared_ptr<Network> ContractionTools::ParallelContract(std::mt19937& gen) {
             const int one = dist(gen);
             const int two = dist(gen);
             if (one == two) continue;
+            if (totTimer.getElapsed() > maxTime) throw ContractionFailure();  // :271, :380-386
             auto result = net->ContractNodes(part[size_t(one)], part[size_t(two)], 1);
             if (result) {
                 Splice(part, one, two, result);
@@ -128,6 +129,7 @@ std::shared_ptr<Network> ContractionTools::ParallelContract(std::mt19937& gen) {
         const int one = dist(gen);
         const int two = dist(gen);
         if (one == two || left[size_t(one)]->mContracted || left[size_t(two)]->mContracted) continue;
+        if (totTimer.getElapsed() > maxTime) throw ContractionFailure();  // :332, :380-386
         auto result = net->ContractNodes(left[size_t(one)], left[size_t(two)], threshold);
         if (result) {
             Splice(left, one, two, result);
@@ -197,6 +199,39 @@ std::shared_ptr<Network> ContractionTools::ContractUserDefinedSequenceOfWires(co
         if (ss >> a >> b) seq.push_back({a, b});
     }
     return ContractUserDefinedSequence(seq);
+}
+
+// preProcess (src/preprocess.h:28-52)
+bool preProcess(const std::string& qasm_text, const std::string* measure_text, qt_ctx* ctx, uint64_t seed,
+                std::vector<std::pair<int, int>>& optimalContractionSequence, double timeThreshold) {
+    maxTime = timeThreshold;
+    for (int i = 0; i < 100; ++i) {
+        totTimer = Timer();
+        totTimer.start();
+        NetworkOptions opt;
+        opt.ctx = ctx;
+        ContractionTools p(std::make_shared<Network>(qasm_text, measure_text, opt));
+        p.SetSeed(seed + uint64_t(i));
+        std::shared_ptr<Network> temp = p.Contract(Stochastic);
+        if (totTimer.getElapsed() <= timeThreshold) {
+            const auto r = RecordSequence(*temp);
+            optimalContractionSequence.insert(optimalContractionSequence.end(), r.begin(), r.end());
+            totTimer.reset();
+            return true;
+        }
+    }
+    return false;
+}
+
+bool preProcess(const std::string& fileName, std::vector<std::pair<int, int>>& optimalContractionSequence,
+                double timeThreshold) {
+    std::ifstream in(fileName);
+    if (!in) throw InvalidFile();
+    std::stringstream ss;
+    ss << in.rdbuf();
+    // the reference measures with "measureTest.txt", which it never writes: every qubit traced
+    return preProcess(ss.str(), nullptr, DefaultContext(), std::random_device{}(), optimalContractionSequence,
+                      timeThreshold);
 }
 
 }  // namespace qtorch

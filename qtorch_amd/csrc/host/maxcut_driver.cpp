@@ -123,7 +123,7 @@ OptimizeResult MaxcutOptimalAngles(qt_ctx* ctx, const std::string& dgf, int p, i
 // coin instead of rand()).  Every probability is one full-circuit contraction on the
 // device, in the greedy order planned once for the circuit.
 std::vector<int> MaxcutFinalString(qt_ctx* ctx, const std::string& dgf, int p, const std::vector<double>& bg,
-                                   uint64_t seed, int* cut_edges) {
+                                   uint64_t seed, int* cut_edges, std::vector<double>* cond_probs) {
     ExtraData d = ExtraData::FromText(p, dgf, true);
     std::ostringstream q;
     q << d.numQubits << std::endl;
@@ -149,6 +149,7 @@ std::vector<int> MaxcutFinalString(qt_ctx* ctx, const std::string& dgf, int p, c
         auto net = std::make_shared<Network>(qasm, &m, opt);
         GreedyContract(*net);
         const double cond = net->GetFinalValue().real() / current;
+        if (cond_probs) cond_probs->push_back(cond);
         if (cond > 0.5) {
             current *= cond;
             bits.push_back(0);

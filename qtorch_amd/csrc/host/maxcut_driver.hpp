@@ -17,6 +17,7 @@ struct OptimizeResult {
 OptimizeResult MaxcutOptimalAngles(qt_ctx* ctx, const std::string& dgf, int p, int precision, int n_random,
                                    int max_evals, uint64_t seed);
 std::vector<int> MaxcutFinalString(qt_ctx* ctx, const std::string& dgf, int p, const std::vector<double>& bg,
-                                   uint64_t seed, int* cut_edges);
+                                   uint64_t seed, int* cut_edges,
+                                   std::vector<double>* cond_probs = nullptr);
 
 }  // namespace qtorch

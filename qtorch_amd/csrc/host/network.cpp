@@ -12,6 +12,9 @@
 
 namespace qtorch {
 
+Timer totTimer;
+double maxTime = 60.0;  // src/Network.h:51
+
 namespace {
 
 constexpr double kPi = 3.14159265358979323846;  // src/Node.h:45
@@ -266,6 +269,7 @@ std::vector<std::string> Tokens(const std::string& line) {
 void Check(qt_ctx* ctx, qt_status s, const char* what) {
     if (s == QT_OK) return;
     if (s == QT_INVALID_INPUT) throw InvalidFunctionInput();
+    if (s == QT_TIMEOUT) throw ContractionFailure();
     throw DeviceFailure(std::string(what) + ": " + qt_last_error(ctx));
 }
 
@@ -357,6 +361,7 @@ void Network::Reset() {  // src/Network.h:154-171
     mOneQubitGates.clear();
     mTwoQubitGates.clear();
     mPlan.clear();
+    mFinalTensor.reset();
     ParseNetwork(mQasmText, mHasMeasure ? &mMeasureText : nullptr);
 }
 
@@ -406,7 +411,13 @@ void Network::ParseNetwork(const std::string& qasm, const std::string* measure) 
             meas[q++] = ch;
         }
     }
+    std::vector<char> open(size_t(mNumberOfQubits), 0);
+    for (int q : mOpt.open_qubits) {
+        if (q < 0 || q >= mNumberOfQubits || open[size_t(q)]) throw InvalidFunctionInput();
+        open[size_t(q)] = 1;
+    }
     for (int i = 0; i < mNumberOfQubits; ++i) {
+        if (open[size_t(i)]) continue;  // the hanging wire stays an open axis
         std::shared_ptr<Node> n;
         if (mOpt.dim_log2 == 2) {
             n = MeasureLeaf(meas[size_t(i)]);
@@ -461,6 +472,34 @@ void Network::AttachTwoQubit(const std::shared_ptr<Node>& n, int q1, int q2) {
     mNodesByWire[size_t(q2)].push_back(n);
 }
 
+// diag(1, 1, 1, d) on (q1, q2) = sum_k |k><k| (x) diag(1, d^k): on q1 a rank-3 tensor
+// [in, out, k] = delta(in, out) delta(in, k), on q2 [in, out, k] = delta(in, out) d^(in k);
+// the bond k joins them.  Both are leaves, q1's first.
+void Network::AttachSplitDiagonal(GateType type, const std::string& name, cd d, int q1, int q2) {
+    if (q1 > mNumberOfQubits - 1 || q2 > mNumberOfQubits - 1 || q1 == q2 || q1 < 0 || q2 < 0)
+        throw InvalidFileFormat();
+    auto a = Leaf(3, type, name + " (control half)", 1);
+    auto b = Leaf(3, type, name + " (target half)", 1);
+    a->mHostVals[0 + 2 * 0 + 4 * 0] = 1.0;
+    a->mHostVals[1 + 2 * 1 + 4 * 1] = 1.0;
+    for (int x = 0; x < 2; ++x)
+        for (int k = 0; k < 2; ++k) b->mHostVals[size_t(x + 2 * x + 4 * k)] = (x && k) ? d : cd(1.0);
+    auto bond = std::make_shared<Wire>(a, b, q1);
+    for (auto [n, q] : {std::make_pair(a, q1), std::make_pair(b, q2)}) {
+        n->GetWires().push_back(mHanging[size_t(q)]);
+        mHanging[size_t(q)]->SetNodeB(n);
+        auto w = std::make_shared<Wire>(n, nullptr, q);
+        mHanging[size_t(q)] = w;
+        n->GetWires().push_back(w);
+        n->GetWires().push_back(bond);
+        n->AddWireNumber(q);
+        n->mIndexOfPreviousNode = int(mNodesByWire[size_t(q)].size()) - 1;
+        mNodesByWire[size_t(q)].push_back(n);
+        mAllNodes.push_back(n);
+        n->mID = int(mAllNodes.size()) - 1;
+    }
+}
+
 // ParseNode (src/Network.h:326-707)
 void Network::ParseNode(const std::string& line) {
     const auto t = Tokens(line);
@@ -488,6 +527,16 @@ void Network::ParseNode(const std::string& line) {
         AttachOneQubit(n, std::stoi(arg(1)));
     } else if (g == "CNOT" || g == "SWAP" || g == "CZ" || g == "CRk") {
         const int q1 = std::stoi(arg(1)), q2 = std::stoi(arg(2));
+        if (mOpt.dim_log2 == 1 && mOpt.split_diagonal && (g == "CZ" || g == "CRk")) {
+            if (g == "CZ") {
+                AttachSplitDiagonal(GateType::CZ, "CZ", cd(-1.0), q1, q2);
+            } else {
+                if (q1 > mNumberOfQubits - 1 || q2 > mNumberOfQubits - 1 || q1 == q2 || q1 < 0) throw InvalidFileFormat();
+                const double th = 2.0 * kPi / std::pow(2, q1 + 1.0);  // as AmpTwoQubitGate
+                AttachSplitDiagonal(GateType::CRK, "CRk", cd(std::cos(th), std::sin(th)), q1, q2);
+            }
+            return;
+        }
         if (mOpt.dim_log2 == 1) {
             if (g == "CRk" && (q1 > mNumberOfQubits - 1 || q2 > mNumberOfQubits - 1 || q1 == q2))
                 throw InvalidFileFormat();
@@ -502,6 +551,11 @@ void Network::ParseNode(const std::string& line) {
         AttachTwoQubit(n, q1, q2);
     } else if (g == "CPHASE") {
         const double angle = std::stod(arg(1));  // src/Network.h:615
+        if (mOpt.dim_log2 == 1 && mOpt.split_diagonal) {
+            AttachSplitDiagonal(GateType::CPHASE, "CPhase", cd(std::cos(angle), std::sin(angle)), std::stoi(arg(2)),
+                                std::stoi(arg(3)));
+            return;
+        }
         n = mOpt.dim_log2 == 1 ? AmpTwoQubitGate("CPHASE", angle, 0) : CPhaseLeaf(angle);
         AttachTwoQubit(n, std::stoi(arg(2)), std::stoi(arg(3)));
     } else if (g == "def1") {
@@ -571,9 +625,12 @@ void Network::ContractNetworkLinearly() {
     }
 }
 
-// ContractNodes (src/Network.h:715-864)
+// ContractNodes (src/Network.h:715-864).  The bookkeeping runs under mLocker, the
+// device step outside it, as the reference runs ContractIndices unlocked
+// (:831-837), so threads contracting disjoint pairs of one network overlap.
 std::shared_ptr<Node> Network::ContractNodes(std::shared_ptr<Node> A, std::shared_ptr<Node> B, int threshold) {
     if (!A || !B) throw InvalidFunctionInput();
+    std::unique_lock<std::mutex> lock(mLocker);
     if (A->mContracted || B->mContracted) return nullptr;
     auto touches = [](const std::shared_ptr<Wire>& w, const std::shared_ptr<Node>& n) {
         return w->GetNodeB().lock() == n || w->GetNodeA().lock() == n;
@@ -603,6 +660,9 @@ std::shared_ptr<Node> Network::ContractNodes(std::shared_ptr<Node> A, std::share
     // the reference compares size_t against int: a negative limit wraps to a huge one
     const size_t limit = static_cast<size_t>(std::max(A->mRank, B->mRank) + threshold);
     if ((ia.empty() && !ic.empty()) || remaining.size() > limit) return nullptr;
+    // out of time: stop before the network changes
+    if (totTimer.getElapsed() > maxTime) throw ContractionFailure();
+    if (!A->HasData() || !B->HasData()) throw InvalidFunctionInput();
     std::vector<std::pair<int, int>> pairs;
     for (size_t i = 0; i < ia.size() && i < ib.size(); ++i) pairs.push_back({ia[i], ib[i]});
     auto C = std::make_shared<Node>(int(ic.size()));
@@ -622,7 +682,28 @@ std::shared_ptr<Node> Network::ContractNodes(std::shared_ptr<Node> A, std::share
     B->mContracted = true;
     A->mContracted = true;
     for (auto& w : connected) w->SetIsContracted(true);
-    ContractIndices(ic, pairs, A, B, C);
+    lock.unlock();
+    PlanStep st;
+    cd value(std::numeric_limits<double>::quiet_NaN(), std::numeric_limits<double>::quiet_NaN());
+    ContractIndices(ic, pairs, A, B, C, &st, &value);
+    lock.lock();
+    // the counter, the final-value rule and the done test of ContractIndices
+    // (src/Network.h:884-885, 961-969)
+    mNumFloatOps += 1LL << (mOpt.dim_log2 * (ic.size() + pairs.size()));
+    st.c_id = int(mAllNodes.size());
+    mPlan.push_back(std::move(st));
+    if (ic.empty()) {
+        if ((std::abs(mFinalVal.real()) <= 1.0e-30 && std::abs(mFinalVal.imag()) <= 1.0e-30) ||
+            (A->mRank == 0 && B->mRank == 0))
+            mFinalVal = value;
+        if (mUncontracted.size() == 2) {
+            mDone = true;
+            mPlan.back().c_id = -1;
+        }
+    } else if (!mOpt.open_qubits.empty() && mUncontracted.size() == 2) {
+        mDone = true;  // open qubits: the last step leaves their tensor (kept in C)
+        mPlan.back().c_id = -1;
+    }
     if (mDone) {
         auto t = std::make_shared<Node>(0);
         t->mID = int(mAllNodes.size());
@@ -632,6 +713,7 @@ std::shared_ptr<Node> Network::ContractNodes(std::shared_ptr<Node> A, std::share
         FindAndReplace(B, C);
         A->ClearNodeData();
         B->ClearNodeData();
+        if (!ic.empty()) mFinalTensor = C;
         return nullptr;
     }
     C->mID = int(mAllNodes.size());
@@ -647,13 +729,11 @@ std::shared_ptr<Node> Network::ContractNodes(std::shared_ptr<Node> A, std::share
 // ContractIndices (src/Network.h:876-971): the arithmetic runs on the device.
 void Network::ContractIndices(const std::vector<std::pair<bool, int>>& free,
                               const std::vector<std::pair<int, int>>& sum, const std::shared_ptr<Node>& A,
-                              const std::shared_ptr<Node>& B, const std::shared_ptr<Node>& C) {
-    mNumFloatOps += 1LL << (mOpt.dim_log2 * (free.size() + sum.size()));
-    if (!A->HasData() || !B->HasData()) throw InvalidFunctionInput();
-    PlanStep st;
+                              const std::shared_ptr<Node>& B, const std::shared_ptr<Node>& C, PlanStep* step,
+                              cd* value) {
+    PlanStep& st = *step;
     st.a_id = A->mID;
     st.b_id = B->mID;
-    st.c_id = int(mAllNodes.size());
     st.rank_a = A->mRank;
     st.rank_b = B->mRank;
     st.k = int(sum.size());
@@ -663,35 +743,24 @@ void Network::ContractIndices(const std::vector<std::pair<bool, int>>& free,
         st.pair_b.push_back(p.second);
     }
     for (const auto& f : free) st.free_src.push_back(f.first ? (f.second | QT_FROM_A) : f.second);
-    cd value(std::numeric_limits<double>::quiet_NaN(), std::numeric_limits<double>::quiet_NaN());
-    if (mCtx) {
-        EnsureOnDevice(A);
-        EnsureOnDevice(B);
-        qt_tensor* t = nullptr;
-        Check(mCtx, qt_tensor_alloc_empty(mCtx, C->mRank, mOpt.dim_log2, QT_DTYPE_C128, &t), "alloc C");
-        C->mDev = t;
-        C->mCtx = mCtx;
-        Check(mCtx,
-              qt_contract_pair(mCtx, A->mDev, A->mRank, B->mDev, B->mRank, st.k, st.pair_a.data(),
-                               st.pair_b.data(), st.rank_c, st.free_src.data(), t, QT_PLAIN_C128, nullptr),
-              "contract");
-        if (free.empty()) {
-            double re = 0, im = 0;
-            Check(mCtx, qt_tensor_read_scalar(mCtx, t, &re, &im), "scalar");
-            value = cd(re, im);
-        }
-    } else {
+    if (!mCtx) {
         C->mPlanned = true;
+        return;
     }
-    mPlan.push_back(std::move(st));
+    EnsureOnDevice(A);
+    EnsureOnDevice(B);
+    qt_tensor* t = nullptr;
+    Check(mCtx, qt_tensor_alloc_empty(mCtx, C->mRank, mOpt.dim_log2, QT_DTYPE_C128, &t), "alloc C");
+    C->mDev = t;
+    C->mCtx = mCtx;
+    Check(mCtx,
+          qt_contract_pair(mCtx, A->mDev, A->mRank, B->mDev, B->mRank, st.k, st.pair_a.data(), st.pair_b.data(),
+                           st.rank_c, st.free_src.data(), t, QT_PLAIN_C128, nullptr),
+          "contract");
     if (free.empty()) {
-        if ((std::abs(mFinalVal.real()) <= 1.0e-30 && std::abs(mFinalVal.imag()) <= 1.0e-30) ||
-            (A->mRank == 0 && B->mRank == 0))
-            mFinalVal = value;
-        if (mUncontracted.size() == 2) {
-            mDone = true;
-            mPlan.back().c_id = -1;
-        }
+        double re = 0, im = 0;
+        Check(mCtx, qt_tensor_read_scalar(mCtx, t, &re, &im), "scalar");
+        *value = cd(re, im);
     }
 }
 
@@ -799,6 +868,19 @@ void Network::ReduceCircuit() {
         found = false;
     }
     mNodesByWire = std::move(out);
+}
+
+std::vector<std::complex<double>> Network::GetFinalTensor(std::vector<int>* axis_qubits) const {
+    if (!mDone) throw InvalidFunctionInput();
+    if (axis_qubits) axis_qubits->clear();
+    if (!mFinalTensor) return {mFinalVal};
+    if (axis_qubits)
+        for (const auto& w : mFinalTensor->GetWires()) axis_qubits->push_back(w->GetQubitNumber());
+    if (!mCtx) {
+        const double nan = std::numeric_limits<double>::quiet_NaN();
+        return std::vector<cd>(size_t(1) << (mOpt.dim_log2 * mFinalTensor->mRank), cd(nan, nan));
+    }
+    return mFinalTensor->GetTensorVals();
 }
 
 std::vector<std::pair<int, int>> RecordSequence(const Network& net) {

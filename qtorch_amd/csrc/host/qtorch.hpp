@@ -11,10 +11,13 @@
 // tests).  Stochastic ordering takes an explicit seed.
 #pragma once
 
+#include <chrono>
 #include <complex>
 #include <cstdint>
+#include <ctime>
 #include <exception>
 #include <memory>
+#include <mutex>
 #include <random>
 #include <string>
 #include <unordered_map>
@@ -63,6 +66,37 @@ public:
 private:
     std::string msg_;
 };
+
+// src/Timer.h:24-67: wall-clock seconds since start() (0 until started).
+class Timer {
+public:
+    void start() {
+        mStart = std::chrono::steady_clock::now();
+        mCPUClockStart = std::clock();
+        mStarted = true;
+    }
+    void reset() { mStarted = false; }
+    double getElapsed() const {
+        if (!mStarted) return 0.0;
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - mStart).count();
+    }
+    double getCPUElapsed() const { return double(std::clock() - mCPUClockStart) / CLOCKS_PER_SEC; }
+
+private:
+    std::chrono::steady_clock::time_point mStart{};
+    std::clock_t mCPUClockStart = 0;
+    bool mStarted = false;
+};
+
+// The reference's process-wide contraction timer and limit (src/Network.h:50-51).
+// Once totTimer has been started and getElapsed() > maxTime, ContractNodes throws
+// ContractionFailure before it changes the network, i.e. between pairwise steps
+// (the reference tests them per MAC inside ContractIndices, src/Network.h:899-910,
+// and before each stochastic pick, src/ContractionTools.h:271,332, and ends in
+// ContractionFailure, :380-386).  The timer is only started by preProcess or by
+// the caller.
+extern Timer totTimer;
+extern double maxTime;
 
 enum class GateType {
     CNOT, SWAP, HADAMARD, RX, RY, RZ, X, Y, Z, PHASE, DEPOLARIZER, CRK, CZ, CPHASE,
@@ -158,6 +192,15 @@ struct NetworkOptions {
     int dim_log2 = 2;
     std::string base_dir;      // resolves relative def1/def2 gate-file paths ("" = cwd)
     bool verbose = false;      // the reference's std::cout progress messages
+    // Amplitude picture only: every diagonal two-qubit gate diag(1, 1, 1, d) (CZ,
+    // CPHASE, CRk) becomes two rank-3 tensors joined by a dimension-2 bond (the
+    // reference has no such split): a cut through the gate then costs one index bit
+    // instead of two.  Node ids after the first such gate differ from the reference's.
+    bool split_diagonal = false;
+    // Qubits whose output wire stays open (no measurement or trace node; no reference
+    // counterpart, where every qubit ends in one): the contraction ends in a tensor
+    // over these wires instead of a scalar (GetFinalTensor).
+    std::vector<int> open_qubits;
 };
 
 // src/Network.h:54-140
@@ -188,6 +231,10 @@ public:
     long long getNumFloatOps() const noexcept { return mNumFloatOps; }
 
     // engine extensions
+    // Values of the tensor a network with open qubits ends in, and the qubit of each
+    // of its axes (digit i of the flat index belongs to axis_qubits[i]); a finished
+    // network without open qubits gives its scalar.
+    std::vector<std::complex<double>> GetFinalTensor(std::vector<int>* axis_qubits) const;
     const std::vector<PlanStep>& GetPlan() const noexcept { return mPlan; }
     bool IsPlanOnly() const noexcept { return mCtx == nullptr; }
     qt_ctx* GetContext() const noexcept { return mCtx; }
@@ -200,10 +247,13 @@ private:
     void ParseNode(const std::string& line);
     void AttachOneQubit(const std::shared_ptr<Node>& n, int q);
     void AttachTwoQubit(const std::shared_ptr<Node>& n, int q1, int q2);
+    void AttachSplitDiagonal(GateType type, const std::string& name, std::complex<double> d, int q1, int q2);
+    // The arithmetic of one step on the device (no bookkeeping: it runs outside
+    // mLocker); *value receives the scalar of a rank-0 result.
     void ContractIndices(const std::vector<std::pair<bool, int>>& toNotSumOn,
                          const std::vector<std::pair<int, int>>& toSumOn,
                          const std::shared_ptr<Node>& A, const std::shared_ptr<Node>& B,
-                         const std::shared_ptr<Node>& C);
+                         const std::shared_ptr<Node>& C, PlanStep* step, std::complex<double>* value);
     void EnsureOnDevice(const std::shared_ptr<Node>& n);
     void UploadLeaves();
     void FindAndRemove(const std::shared_ptr<Node>& n);
@@ -229,6 +279,10 @@ private:
     long long mNumFloatOps = 0;
     int mNumberOfThreads = 8;
     std::vector<PlanStep> mPlan;
+    std::shared_ptr<Node> mFinalTensor;  // open qubits: the tensor the network ends in
+    // guards the network's bookkeeping (wires, node lists, ids, plan, final value)
+    // when several threads contract disjoint pairs (src/Network.h:718-862)
+    std::mutex mLocker;
 };
 
 enum ContractionType { Stochastic, FromEdges, CostContractSimple, CostContractBruteForce };
@@ -268,6 +322,20 @@ private:
 
 // The mCreatedFrom record of a contracted network (src/preprocess.h:41-46).
 std::vector<std::pair<int, int>> RecordSequence(const Network& net);
+
+// preProcess (src/preprocess.h:28-52): up to 100 stochastic contractions of the
+// circuit, each timed from a fresh start of totTimer with maxTime = timeThreshold;
+// the first that finishes within timeThreshold seconds gives its mCreatedFrom record
+// in optimalContractionSequence and true (totTimer is then reset), else false.  A
+// contraction that runs out of time throws ContractionFailure, as the reference's
+// does.  maxTime keeps timeThreshold afterwards, as in the reference.
+bool preProcess(const std::string& fileName, std::vector<std::pair<int, int>>& optimalContractionSequence,
+                double timeThreshold);
+// The same on in-memory text and a context (nullptr: plan-only), with seeded orders
+// (attempt i uses seed + i); measure_text nullptr traces every qubit, as the
+// reference's missing measureTest.txt does.
+bool preProcess(const std::string& qasm_text, const std::string* measure_text, qt_ctx* ctx, uint64_t seed,
+                std::vector<std::pair<int, int>>& optimalContractionSequence, double timeThreshold);
 
 // src/LineGraph.h:44-89
 class LineGraph {

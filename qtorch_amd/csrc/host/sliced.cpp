@@ -21,12 +21,19 @@
 
 namespace qtorch {
 
+namespace {
+// lines through the qubits the sweep order tries (the plan with the least work after
+// slicing is kept)
+constexpr uint64_t kSweepVariants = 16;
+}  // namespace
+
 SlicedNetwork::SlicedNetwork(const std::string& qasm, const std::string& measure, int dim_log2, int order,
                              uint64_t seed, int max_bits)
-    : dl_(dim_log2) {
+    : dl_(dim_log2), split_diagonal_(order == kOrderSweep && dim_log2 == 1) {
     if ((dim_log2 != 1 && dim_log2 != 2) || max_bits < 1 || max_bits > 40) throw InvalidFunctionInput();
     NetworkOptions o;
     o.dim_log2 = dim_log2;
+    o.split_diagonal = split_diagonal_;
     Network net(qasm, &measure, o);
     n_leaves_ = net.GetNumOriginalNodes();
     std::unordered_map<const Wire*, int> wire_id;
@@ -38,57 +45,128 @@ SlicedNetwork::SlicedNetwork(const std::string& qasm, const std::string& measure
         if (int(w.size()) != node->mRank) throw InvalidFunctionInput();
         leaf_wires_.push_back(w);
     }
+    // One candidate plan: the unsliced steps in the order, the wires of every operand
+    // (leaves, then step results) and the wires the slicer fixes.
+    struct Candidate {
+        std::vector<Step> full;
+        std::vector<std::vector<int>> wires;
+        std::vector<char> is_sliced;
+        std::vector<int> sliced;
+        long double work = 0;  // slices x complex MACs per slice
+    };
+    auto plan_candidate = [&](Network& cn) {
+        Candidate cd;
+        const auto& plan = cn.GetPlan();
+        if (plan.empty()) throw InvalidFunctionInput();
+        cd.wires = leaf_wires_;
+        std::unordered_map<int, int> operand_of_node;
+        for (int i = 0; i < n_leaves_; ++i) operand_of_node[i] = i;
+        for (const auto& ps : plan) {
+            Step st;
+            st.a = operand_of_node.at(ps.a_id);
+            st.b = operand_of_node.at(ps.b_id);
+            st.rank_a = ps.rank_a;
+            st.rank_b = ps.rank_b;
+            st.k = ps.k;
+            st.rank_c = ps.rank_c;
+            st.pair_a = ps.pair_a;
+            st.pair_b = ps.pair_b;
+            st.free_src = ps.free_src;
+            const auto& wa = cd.wires[size_t(st.a)];
+            const auto& wb = cd.wires[size_t(st.b)];
+            if (int(wa.size()) != st.rank_a || int(wb.size()) != st.rank_b) throw ContractionFailure();
+            std::vector<int> wc;
+            for (int32_t f : st.free_src)
+                wc.push_back((f & QT_FROM_A) ? wa[size_t(f & ~QT_FROM_A)] : wb[size_t(f)]);
+            const int id = n_leaves_ + int(cd.full.size());
+            if (ps.c_id >= 0) operand_of_node[ps.c_id] = id;
+            cd.wires.push_back(wc);
+            cd.full.push_back(st);
+        }
+        // slicing: fix the wire that occurs in most of the tensors above the limit
+        // (weighted by how far above it they are); among equals, the one that leaves
+        // the least work per slice
+        cd.is_sliced.assign(wire_id.size(), 0);
+        auto bits_of = [&](const std::vector<int>& w) {
+            int n = 0;
+            for (int x : w) n += cd.is_sliced[size_t(x)] ? 0 : 1;
+            return n * dl_;
+        };
+        auto slice_macs = [&]() {
+            long double m = 0;
+            for (size_t s2 = 0; s2 < cd.full.size(); ++s2) {
+                const Step& f = cd.full[s2];
+                int bits = bits_of(cd.wires[size_t(n_leaves_) + s2]);
+                for (int32_t pa : f.pair_a)
+                    if (!cd.is_sliced[size_t(cd.wires[size_t(f.a)][size_t(pa)])]) bits += dl_;
+                m += std::ldexp(1.0L, bits);
+            }
+            return m;
+        };
+        while (true) {
+            int worst = 0;
+            for (const auto& w : cd.wires) worst = std::max(worst, bits_of(w));
+            if (worst <= max_bits) break;
+            std::vector<int64_t> score(wire_id.size(), 0);
+            for (const auto& w : cd.wires) {
+                const int over = bits_of(w) - max_bits;
+                if (over > 0)
+                    for (int x : w)
+                        if (!cd.is_sliced[size_t(x)]) score[size_t(x)] += int64_t(1) << std::min(over, 20);
+            }
+            const int64_t top = *std::max_element(score.begin(), score.end());
+            int pick = -1;
+            long double best_macs = 0;
+            for (size_t x = 0; x < score.size(); ++x) {
+                if (score[x] != top || top == 0) continue;
+                cd.is_sliced[x] = 1;
+                const long double m = slice_macs();
+                cd.is_sliced[x] = 0;
+                if (pick < 0 || m < best_macs) {
+                    pick = int(x);
+                    best_macs = m;
+                }
+            }
+            if (pick < 0) throw InvalidFunctionInput();
+            cd.is_sliced[size_t(pick)] = 1;
+            cd.sliced.push_back(pick);
+            if (int(cd.sliced.size()) * dl_ > 40) throw InvalidFunctionInput();  // more than 2^40 slices
+        }
+        cd.work = std::ldexp(slice_macs(), int(cd.sliced.size()) * dl_);
+        return cd;
+    };
     OrderNetwork(net, order, seed, nullptr);
-    const auto& plan = net.GetPlan();
-    if (plan.empty()) throw InvalidFunctionInput();
-    // wires of every operand (leaves, then step results) and the unsliced steps
-    std::vector<std::vector<int>> wires(leaf_wires_);
-    std::unordered_map<int, int> operand_of_node;
-    for (int i = 0; i < n_leaves_; ++i) operand_of_node[i] = i;
-    std::vector<Step> full;
-    for (const auto& ps : plan) {
-        Step st;
-        st.a = operand_of_node.at(ps.a_id);
-        st.b = operand_of_node.at(ps.b_id);
-        st.rank_a = ps.rank_a;
-        st.rank_b = ps.rank_b;
-        st.k = ps.k;
-        st.rank_c = ps.rank_c;
-        st.pair_a = ps.pair_a;
-        st.pair_b = ps.pair_b;
-        st.free_src = ps.free_src;
-        const auto& wa = wires[size_t(st.a)];
-        const auto& wb = wires[size_t(st.b)];
-        if (int(wa.size()) != st.rank_a || int(wb.size()) != st.rank_b) throw ContractionFailure();
-        std::vector<int> wc;
-        for (int32_t f : st.free_src)
-            wc.push_back((f & QT_FROM_A) ? wa[size_t(f & ~QT_FROM_A)] : wb[size_t(f)]);
-        const int id = n_leaves_ + int(full.size());
-        if (ps.c_id >= 0) operand_of_node[ps.c_id] = id;
-        wires.push_back(wc);
-        full.push_back(st);
+    Candidate best;
+    bool have = false;
+    try {
+        best = plan_candidate(net);
+        have = true;
+    } catch (const InvalidFunctionInput&) {  // more than 2^40 slices
+        if (order != kOrderSweep) throw;
     }
-    // slicing: fix the wire that occurs in most of the tensors above the limit
-    std::vector<char> is_sliced(wire_id.size(), 0);
+    if (order == kOrderSweep) {
+        // other lines through the qubits: the sweep's width depends on it
+        for (uint64_t v = 1; v < kSweepVariants; ++v) {
+            Network cn(qasm, &measure, o);
+            SweepContract(cn, seed * kSweepVariants + v);
+            try {
+                Candidate c = plan_candidate(cn);
+                if (!have || c.work < best.work) best = std::move(c);
+                have = true;
+            } catch (const InvalidFunctionInput&) {
+            }
+        }
+        if (!have) throw InvalidFunctionInput();
+    }
+    std::vector<Step> full = std::move(best.full);
+    std::vector<std::vector<int>> wires = std::move(best.wires);
+    std::vector<char> is_sliced = std::move(best.is_sliced);
+    sliced_ = std::move(best.sliced);
     auto bits_of = [&](const std::vector<int>& w) {
         int n = 0;
         for (int x : w) n += is_sliced[size_t(x)] ? 0 : 1;
         return n * dl_;
     };
-    while (true) {
-        int worst = 0;
-        for (const auto& w : wires) worst = std::max(worst, bits_of(w));
-        if (worst <= max_bits) break;
-        std::vector<int> count(wire_id.size(), 0);
-        for (const auto& w : wires)
-            if (bits_of(w) == worst)
-                for (int x : w)
-                    if (!is_sliced[size_t(x)]) ++count[size_t(x)];
-        const int pick = int(std::max_element(count.begin(), count.end()) - count.begin());
-        is_sliced[size_t(pick)] = 1;
-        sliced_.push_back(pick);
-        if (int(sliced_.size()) * dl_ > 40) throw InvalidFunctionInput();  // more than 2^40 slices
-    }
     // the sliced steps: axes renumbered without the sliced wires
     auto kept = [&](const std::vector<int>& w) {
         std::vector<int> idx(w.size(), -1);
@@ -264,6 +342,7 @@ void SlicedNetwork::PlanFusion() {
 SlicedNetwork::LeafState SlicedNetwork::MakeLeafState(const std::string& qasm, const std::string& measure) const {
     NetworkOptions o;
     o.dim_log2 = dl_;
+    o.split_diagonal = split_diagonal_;
     Network net(qasm, &measure, o);
     if (net.GetNumOriginalNodes() != n_leaves_) throw InvalidFunctionInput();
     LeafState ls;

@@ -81,6 +81,9 @@ private:
                                  const std::vector<double>& const_pool) const;
 
     int dl_ = 2;
+    // the sweep order in the amplitude picture parses diagonal two-qubit gates as two
+    // rank-3 halves (NetworkOptions::split_diagonal)
+    bool split_diagonal_ = false;
     int n_leaves_ = 0;
     std::vector<std::vector<std::complex<double>>> leaf_vals_;  // unsliced
     std::vector<std::vector<int>> leaf_wires_;                  // wire id of each axis

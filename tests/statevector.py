@@ -134,3 +134,71 @@ def expectation(psi, n, measure_text: str) -> complex:
         if ch in ops:
             phi = apply_1q(phi, ops[ch], q, n)
     return complex(np.vdot(psi, phi))
+
+
+def _apply_1q_batch(psi, u, q, n):
+    """psi: (S, 2^n); u: (S, 2, 2) or (2, 2)."""
+    s = psi.shape[0]
+    t = np.moveaxis(psi.reshape([s] + [2] * n), 1 + n - 1 - q, -1)
+    shape = t.shape
+    t = t.reshape(s, -1, 2)
+    t = np.einsum("sab,skb->ska", u, t) if u.ndim == 3 else t @ u.T
+    return np.moveaxis(t.reshape(shape), -1, 1 + n - 1 - q).reshape(s, -1)
+
+
+def _apply_2q_batch(psi, u, q1, q2, n):
+    s = psi.shape[0]
+    t = psi.reshape([s] + [2] * n)
+    t = np.moveaxis(t, [1 + n - 1 - q1, 1 + n - 1 - q2], [-2, -1])
+    shape = t.shape
+    t = t.reshape(s, -1, 4) @ u.T
+    return np.moveaxis(t.reshape(shape), [-2, -1], [1 + n - 1 - q1, 1 + n - 1 - q2]).reshape(s, -1)
+
+
+def simulate_batch(qasm_texts):
+    """State vectors of circuits that share their gate sequence and differ only in
+    the angles of Rx / Ry / Rz / PHASE gates (the edge light cones of one graph for
+    many angle sets): (S, 2^n) and n.  Same conventions as simulate()."""
+    split = [t.split("\n") for t in qasm_texts]
+    n = int(split[0][0].strip().split()[0])
+    s = len(split)
+    psi = np.zeros((s, 2 ** n), dtype=np.complex128)
+    psi[:, 0] = 1.0
+    rot = {"Rx": _rx, "RX": _rx, "Ry": _ry, "RY": _ry, "Rz": _rz, "RZ": _rz, "PHASE": _rz}
+    for li in range(1, len(split[0])):
+        line = split[0][li]
+        if not line or line.startswith("#"):
+            continue
+        t = [x for x in line.split(" ") if x != ""]
+        g = t[0]
+        if g in rot:
+            us = np.empty((s, 2, 2), dtype=np.complex128)
+            cache = {}
+            for k in range(s):
+                tok = split[k][li].split(" ")[1]
+                if tok not in cache:
+                    cache[tok] = rot[g](_f32(tok))
+                us[k] = cache[tok]
+            psi = _apply_1q_batch(psi, us, int(t[2]), n)
+        elif g in ("H", "X", "Y", "Z"):
+            psi = _apply_1q_batch(psi, {"H": _H, "X": _X, "Y": _Y, "Z": _Z}[g], int(t[1]), n)
+        elif g == "CNOT":
+            psi = _apply_2q_batch(psi, np.eye(4, dtype=np.complex128)[[0, 1, 3, 2]], int(t[1]), int(t[2]), n)
+        elif g == "CZ":
+            psi = _apply_2q_batch(psi, np.diag([1, 1, 1, -1]).astype(np.complex128), int(t[1]), int(t[2]), n)
+        else:
+            raise ValueError(g)
+    return psi, n
+
+
+def expectation_batch(psi, n, measure_text: str):
+    """Tr(M rho) per row of a batch of state vectors (see expectation())."""
+    chars = [c for c in measure_text if not c.isspace()]
+    ops = {"X": _X, "Y": _Y, "Z": _Z, "0": np.diag([1, 0]).astype(np.complex128),
+           "1": np.diag([0, 1]).astype(np.complex128)}
+    phi = psi.copy()
+    for q in range(n):
+        ch = chars[q] if q < len(chars) else "T"
+        if ch in ops:
+            phi = _apply_1q_batch(phi, ops[ch], q, n)
+    return np.einsum("sk,sk->s", psi.conj(), phi)

@@ -62,3 +62,44 @@ def test_qaoa_optimize_and_final_cut(ctx):
     edges = [tuple(map(int, l.split()[1:3])) for l in dgf.splitlines() if l.startswith("e")]
     assert cut == sum(1 for a, b in edges if bits[a] != bits[b])
     assert cut >= len(edges) // 2
+
+
+@pytest.mark.parametrize("n,p,seed", [(10, 1, 4), (8, 2, 6)])
+def test_final_cut_against_state_vector(ctx, n, p, seed):
+    """maxcutGetFinalString (src/maxcut.cpp:29-140) on small 3-regular graphs: every
+    conditional probability P(bit i = 0 | the bits before) that the device computes
+    from one full-circuit network per qubit, and the bitstring they decide, against a
+    state vector of the same circuit text (H on every qubit; p x [CNOT, Rz(-gamma),
+    CNOT per edge; Rx(2 beta) per qubit], angles printed at 6 significant digits)."""
+    from qtorch_amd import workloads as wl
+    from tests import statevector as sv
+    edges = wl.regular_graph(n, 3, seed)
+    dgf = wl.dgf_text(edges)
+    bg = wl.qaoa_angles(p, seed)
+    bits, cut, probs = qa.maxcut_final_string(ctx, dgf, p, bg, seed=0, with_probs=True)
+    lines = [str(n)] + [f"H {i}" for i in range(n)]
+    for i in range(p):
+        for a, b in edges:
+            lines += [f"CNOT {a} {b}", f"Rz {-bg[i + p]:g} {b}", f"CNOT {a} {b}"]
+        lines += [f"Rx {bg[i] * 2.0:g} {j}" for j in range(n)]
+    psi, nq = sv.simulate("\n".join(lines) + "\n")
+    prob = np.abs(psi) ** 2
+    idx = np.arange(1 << n)
+    chosen, current = [], 1.0
+    for i in range(n):
+        mask = np.ones(1 << n, dtype=bool)
+        for j, b in enumerate(chosen):
+            mask &= ((idx >> j) & 1) == b
+        mask &= ((idx >> i) & 1) == 0
+        cond = prob[mask].sum() / current
+        assert abs(probs[i] - cond) <= 1e-10, (i, probs[i], cond)
+        if abs(cond - 0.5) <= 1e-9:
+            # a tie up to rounding (the first bit always is: the QAOA state is symmetric
+            # under flipping every bit): the device's choice stands
+            b = bits[i]
+        else:
+            b = 0 if cond > 0.5 else 1
+        chosen.append(b)
+        current *= cond if b == 0 else 1.0 - cond
+    assert bits == chosen
+    assert cut == sum(1 for a, b in edges if bits[a] != bits[b])

@@ -162,6 +162,63 @@ def test_rescaled_chain_does_not_underflow(ctx):
             assert got == 0
 
 
+def test_plain_step_after_rescaled_steps_keeps_exponents(ctx):
+    """A step without QT_RESCALE whose operands carry exponents from rescaled steps
+    gives C the sum of their exponents (its data are then the plain product of the
+    scaled operands): the value read back equals the all-rescaled chain's."""
+    rng = np.random.default_rng(5)
+    scale = 2.0 ** -40
+    v0 = (_rand(rng, 2) * scale).astype(np.complex64)
+    mats = [(_rand(rng, 4) * scale).astype(np.complex64) for _ in range(3)]
+    ref = v0.astype(np.complex128)
+    for m in mats:
+        ref = ref @ m.astype(np.complex128).reshape(2, 2, order="F")
+    t = qa.Tensor(ctx, 1, 1, qa.QT_DTYPE_C64).upload(v0)
+    for i, m in enumerate(mats):
+        prec = qa.QT_C64_4M | (qa.QT_RESCALE if i < 2 else 0)  # the last step is plain
+        tm = qa.Tensor(ctx, 2, 1, qa.QT_DTYPE_C64).upload(m)
+        t, _ = qa.contract_pair(ctx, t, tm, [(0, 0)], [(False, 1)], prec)
+    e = t.exponent()
+    assert e < -60
+    got = t.download().astype(np.complex128) * 2.0 ** e
+    assert np.allclose(got, ref, rtol=1e-5, atol=0), (got, ref)
+    # two rescaled operands into one plain step: both exponents are carried
+    tu = qa.Tensor(ctx, 1, 1, qa.QT_DTYPE_C64).upload((_rand(rng, 2) * scale).astype(np.complex64))
+    tv = qa.Tensor(ctx, 2, 1, qa.QT_DTYPE_C64).upload((_rand(rng, 4) * scale).astype(np.complex64))
+    su, _ = qa.contract_pair(ctx, tu, tv, [(0, 0)], [(False, 1)], qa.QT_C64_4M | qa.QT_RESCALE)
+    s_, _ = qa.contract_pair(ctx, t, su, [(0, 0)], [], qa.QT_C64_4M)
+    assert s_.exponent() == t.exponent() + su.exponent()
+    ref_s = complex(np.sum(got * (su.download().astype(np.complex128) * 2.0 ** su.exponent())))
+    assert abs(s_.scalar() - ref_s) <= 1e-5 * abs(ref_s)
+
+
+def test_context_time_limit_returns_timeout(ctx):
+    """qt_ctx_set_time_limit: once the limit has passed, qt_contract_pair and batch
+    launches return QT_TIMEOUT (ContractionFailure) before queueing anything; removing
+    the limit restores them."""
+    import time
+    rng = np.random.default_rng(6)
+    a = qa.Tensor(ctx, 2, 2).upload(_rand(rng, 16))
+    b = qa.Tensor(ctx, 2, 2).upload(_rand(rng, 16))
+    c0, _ = qa.contract_pair(ctx, a, b, [(0, 1)], [(True, 1), (False, 0)])
+    ctx.set_time_limit(1e-6)
+    time.sleep(0.01)
+    try:
+        with pytest.raises(qa.ContractionFailure):
+            qa.contract_pair(ctx, a, b, [(0, 1)], [(True, 1), (False, 0)])
+    finally:
+        ctx.set_time_limit(0)
+    c1, _ = qa.contract_pair(ctx, a, b, [(0, 1)], [(True, 1), (False, 0)])
+    assert np.array_equal(c1.download(), c0.download())
+    # a generous limit does not get in the way
+    ctx.set_time_limit(3600)
+    try:
+        c2, _ = qa.contract_pair(ctx, a, b, [(0, 1)], [(True, 1), (False, 0)])
+        assert np.array_equal(c2.download(), c0.download())
+    finally:
+        ctx.set_time_limit(0)
+
+
 def test_device_fp64_accumulation(ctx):
     """Rank-0 results of any form are summed in FP64 on the device."""
     rng = np.random.default_rng(4)
@@ -247,3 +304,27 @@ def test_qaoa_p1_energy_plain_vs_reference(ctx, golden):
     ref_zz = np.array([c["value"][0] for c in ref_cases])
     assert np.all(np.abs(zz - ref_zz) <= 1e-12 * np.abs(ref_zz) + 1e-15)
     assert abs(energy - np.sum(0.5 * (1 - ref_zz))) < 1e-12 * abs(energy)
+
+
+def test_teleportation_known_answer(ctx, tmp_path):
+    """The reference's teleportation test (src/tests.cpp:1433-1526): a one-qubit gate
+    file [[alpha, 0], [beta, 0]] prepares alpha|0> + beta|1> on qubit 0, then H 1,
+    CNOT 1 2, CNOT 0 1, H 0; the projectors 0 0 0 / 0 0 1 / 1 0 0 / 1 0 1 give
+    |alpha/2|^2, |beta/2|^2, |alpha/2|^2, |beta/2|^2.  Ten seeded states, written as
+    the reference writes them (std::complex at the default 6 digits), contracted on
+    the device in seeded stochastic orders; the reference's tolerance is 1e-5, here
+    1e-12 against the closed form of the printed values."""
+    rng = np.random.default_rng(1433)
+    for i in range(10):
+        t = rng.uniform(-1, 1, 4)
+        t /= np.sqrt(np.sum(t * t))
+        alpha, beta = complex(t[0], t[1]), complex(t[2], t[3])
+        gate = tmp_path / "testTeleportGate.gate"
+        gate.write_text(f"({alpha.real:g},{alpha.imag:g}) 0 ({beta.real:g},{beta.imag:g}) 0")
+        a_w = complex(float(f"{alpha.real:g}"), float(f"{alpha.imag:g}"))
+        b_w = complex(float(f"{beta.real:g}"), float(f"{beta.imag:g}"))
+        qasm = "3\ndef1 AB testTeleportGate.gate\nAB 0\nH 1\nCNOT 1 2\nCNOT 0 1\nH 0\n"
+        for meas, amp in (("0 0 0", a_w), ("0 0 1", b_w), ("1 0 0", a_w), ("1 0 1", b_w)):
+            net = qa.Network(qasm, meas, ctx, base_dir=str(tmp_path))
+            v = net.contract_stochastic(10 * i + len(meas))
+            assert abs(v.real - abs(amp / 2) ** 2) <= 1e-12 and abs(v.imag) <= 1e-12, (i, meas, v, amp)

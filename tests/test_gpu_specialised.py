@@ -89,6 +89,33 @@ def test_specialised_against_reference_values(ctx, golden):
         b.close()
 
 
+def test_headline_form_on_random_angle_sets(ctx):
+    """The form and executor the benchmark reports (complex128, specialised, conjugate
+    symmetry, greedy order) on 1024 seeded random angle sets x 45 edges: every edge
+    within 1e-12 of an independent FP64 state vector of the reference's own edge
+    circuit (|d| <= 1e-12 max(|x|, 1e-3), far inside the metric's 1e-5), and every
+    energy F = sum 0.5 (1 - <ZZ>) within 1e-12 relative."""
+    from tests import statevector as sv
+    dgf = _graph()
+    sets = _sets(1024, 2024)
+    b = qa.QaoaBatch(ctx, dgf, 1, ref_map=True, precision=qa.QT_PLAIN_C128, order=qa.ORDER_GREEDY)
+    b.set_angles(sets)
+    b.compile(hermitian=True)
+    assert b.specialised()[0]
+    b.launch()
+    energies, zz = b.results()
+    ref = np.empty_like(zz)
+    for e in range(b.n_edges):
+        texts = [qa.qaoa_edge_text(dgf, 1, True, e, list(bg)) for bg in sets]
+        psi, n = sv.simulate_batch([t[0] for t in texts])
+        ref[:, e] = sv.expectation_batch(psi, n, texts[0][1]).real
+    err = np.abs(zz - ref) / np.maximum(np.abs(ref), 1e-3)
+    assert err.max() <= 1e-12, err.max()
+    e_ref = np.sum(0.5 * (1.0 - ref), axis=1)
+    assert np.max(np.abs(energies - e_ref) / np.abs(e_ref)) <= 1e-12
+    b.close()
+
+
 def test_specialise_refuses_what_it_cannot_run(ctx):
     """Forms without generated code, programs too large to specialise: the error is
     reported and the table executor keeps running."""

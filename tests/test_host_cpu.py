@@ -330,3 +330,133 @@ def test_sliced_fusion_plan_counts():
     roots = [i for i in range(len(plan)) if fused[i] and (i not in consumer or not fused[consumer[i]])]
     assert got["fused_steps"] == sum(fused) > 0
     assert got["fused_subtrees"] == len(roots) >= 1
+
+
+def test_max_time_stops_contractions_with_contraction_failure():
+    """totTimer / maxTime (src/Network.h:50-51): once the started timer passes maxTime,
+    the stochastic contraction stops before its next step and raises
+    ContractionFailure (src/ContractionTools.h:271,332,380-386); a stopped timer never
+    times out, and the default limit is the reference's 60 s."""
+    q = _ghz_qasm(12)
+    assert qa.get_max_time() == 60.0
+    try:
+        qa.set_max_time(0.0)
+        qa.timer_start()
+        net = qa.Network(q, None, None)
+        with pytest.raises(qa.ContractionFailure):
+            net.contract_stochastic(3)
+        assert net.counts()["uncontracted"] == net.counts()["original"]  # stopped before the first step
+        # ContractNodes itself checks between steps: a user sequence stops too
+        net2 = qa.Network(q, None, None)
+        with pytest.raises(qa.ContractionFailure):
+            net2.contract_given([(0, 12)])
+        qa.timer_reset()  # not started: no limit
+        net3 = qa.Network(q, None, None)
+        net3.contract_stochastic(3)
+        assert net3.counts()["done"]
+    finally:
+        qa.timer_reset()
+        qa.set_max_time(60.0)
+
+
+def _ghz_qasm(n):
+    return f"{n}\nH 0\n" + "".join(f"CNOT {i} {i + 1}\n" for i in range(n - 1)) + "".join(
+        f"Rz 0.{i + 1} {i}\n" for i in range(n))
+
+
+def test_preprocess_finds_a_sequence_in_time():
+    """preProcess (src/preprocess.h:28-52): the first seeded stochastic contraction
+    that finishes within the threshold gives its mCreatedFrom record, which replays
+    in the oracle to the state-vector value; with no time at all it raises
+    ContractionFailure; maxTime keeps the threshold afterwards, as in the reference."""
+    q = _ghz_qasm(10)
+    meas = "Z Z 0 1 T T T T T Z"
+    try:
+        found, seq = qa.preprocess(q, meas, 30.0, seed=11)
+        assert found and len(seq) > 0
+        assert qa.get_max_time() == 30.0
+        assert qa.timer_elapsed() == 0.0  # reset after success
+        onet = qo.Network(q, meas)
+        val = onet.contract_given_sequence(seq)
+        psi, n = sv.simulate(q)
+        ref = sv.expectation(psi, n, meas)
+        assert abs(val - ref) < 1e-12 * max(1.0, abs(ref))
+        # the same seed gives the same record
+        assert qa.preprocess(q, meas, 30.0, seed=11)[1] == seq
+        with pytest.raises(qa.ContractionFailure):
+            qa.preprocess(q, meas, 0.0, seed=11)
+    finally:
+        qa.timer_reset()
+        qa.set_max_time(60.0)
+
+
+def test_contract_nodes_from_two_threads():
+    """ContractNodes guards its bookkeeping like the reference's mLocker
+    (src/Network.h:718-862), so two threads can contract disjoint pairs of one network
+    (src/ContractionTools.h:359-368): two independent registers are contracted
+    concurrently, one thread each, following each register's own recorded order; the
+    node ids stay unique and dense, the counter is the sum of both, and the merged
+    record replays to the same counter."""
+    import threading
+    n, layers = 24, 3
+
+    def reg_gates(q0):
+        out = []
+        for l in range(layers):
+            out += [f"H {q0 + i}" for i in range(n)]
+            out += [f"CNOT {q0 + i} {q0 + i + 1}" for i in range(l % 2, n - 1, 2)]
+            out += [f"Rz 0.{l + 3} {q0 + i}" for i in range(n)]
+        return out
+
+    ga, gb = reg_gates(0), reg_gates(n)
+    assert len(ga) == len(gb)
+    g = len(ga)
+    # each register alone: its recorded order (local ids)
+    solo_q = f"{n}\n" + "\n".join(reg_gates(0)) + "\n"
+    solo = qa.Network(solo_q, None, None)
+    solo.contract_stochastic(5)
+    local_seq, solo_flops = solo.record(), solo.flops()
+    both_q = f"{2 * n}\n" + "\n".join(ga + gb) + "\n"
+
+    def to_global(reg, lid, created):
+        if lid < n:
+            return reg * n + lid                      # initial states
+        if lid < n + g:
+            return 2 * n + reg * g + (lid - n)        # gates
+        if lid < n + g + n:
+            return 2 * n + 2 * g + reg * n + (lid - n - g)  # trace nodes
+        return created[lid]
+
+    for rep in range(5):
+        net = qa.Network(both_q, None, None)
+        finals = [None, None]
+        errors = []
+
+        def run(reg):
+            try:
+                created = {}
+                nxt = n + g + n
+                for a, b in local_seq:
+                    rid = net.contract_nodes(to_global(reg, a, created), to_global(reg, b, created), 100)
+                    assert rid >= 0
+                    created[nxt] = rid
+                    nxt += 1
+                finals[reg] = rid
+            except Exception as exc:  # reported below
+                errors.append(exc)
+
+        ts = [threading.Thread(target=run, args=(r,)) for r in (0, 1)]
+        for t in ts:
+            t.start()
+        for t in ts:
+            t.join()
+        assert not errors, errors
+        assert net.contract_nodes(finals[0], finals[1], 100) == -1  # scalar x scalar ends the network
+        c = net.counts()
+        assert c["done"] and c["all"] == c["original"] + 2 * len(local_seq) + 1
+        assert net.flops() == 2 * solo_flops + 1
+        rec = net.record()
+        assert len(rec) == 2 * len(local_seq) + 1
+        replay = qa.Network(both_q, None, None)
+        replay.contract_given(rec)
+        assert replay.flops() == net.flops() and replay.counts()["done"]
