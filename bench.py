@@ -177,11 +177,14 @@ def run_grid(args, qa, np):
     4x6 depth 14 in the line-graph order, every slice; 6x8 depth 14 in the stochastic
     order sliced to 28 bits (2^27 slices of 2^38 MACs each, beyond reach as a whole):
     the first --grid-sample slices that are not zero by structure, whose sum each form
-    computes."""
+    computes; and 6x8 depth 14 whole, in the circuit sweep order with CZ split into
+    bonded halves: unsliced (29-bit peak: one 4 GiB complex64 tensor) and sliced to 28
+    bits as configured (64 slices)."""
     from qtorch_amd import workloads as wl
     ctx = qa.Context(0)
     cases = [(6, 8, 8, qa.ORDER_LINEGRAPH, args.max_bits, None), (4, 6, 14, qa.ORDER_LINEGRAPH, args.max_bits, None),
-             (6, 8, 14, qa.ORDER_STOCHASTIC, 28, args.grid_sample)]
+             (6, 8, 14, qa.ORDER_STOCHASTIC, 28, args.grid_sample),
+             (6, 8, 14, qa.ORDER_SWEEP, 29, None), (6, 8, 14, qa.ORDER_SWEEP, 28, None)]
     out = []
     chosen = [int(x) for x in args.grid_cases.split(",")] if args.grid_cases else range(len(cases))
     for i, (r, c, d, order, max_bits, sample) in enumerate(cases):
@@ -190,7 +193,8 @@ def run_grid(args, qa, np):
         q = wl.grid_random_circuit(r, c, d, seed=2024)
         bits = wl.bitstring(r * c, args.seed)
         t0 = time.perf_counter()
-        sn = qa.SlicedNetwork(q, bits, dim_log2=1, order=order, seed=2, max_bits=max_bits)
+        sn = qa.SlicedNetwork(q, bits, dim_log2=1, order=order, seed=0 if order == qa.ORDER_SWEEP else 2,
+                              max_bits=max_bits)
         plan_s = time.perf_counter() - t0
         plan = sn.plan()
         print(f"grid {r}x{c} depth {d}: planned in {plan_s:.1f} s, {plan['slices']} slices", file=sys.stderr,
@@ -205,7 +209,8 @@ def run_grid(args, qa, np):
         res = {}
         ref = None
         forms = (qa.QT_PLAIN_C128, qa.QT_C64_4M, qa.QT_C64_3M, qa.QT_C64_BF16X3, qa.QT_C64_4M | qa.QT_RESCALE)
-        for prec in forms[:args.grid_forms]:
+        n_forms = args.grid_forms if plan["slices"] * plan["cmacs_per_slice"] < 2 ** 40 else min(args.grid_forms, 2)
+        for prec in forms[:n_forms]:
             sn.run(ctx, prec)  # warm the allocator
             v, ms = sn.run(ctx, prec)
             if prec == qa.QT_PLAIN_C128:
@@ -216,7 +221,8 @@ def run_grid(args, qa, np):
                          "rel_err_vs_c128": abs(v - ref) / abs(ref) if ref != 0 else None}
             print(f"  {name}: {ms:.1f} ms", file=sys.stderr, flush=True)
         case = {"circuit": f"grid_{r}x{c}_depth{d}", "qubits": r * c, "bits": bits, "plan": plan,
-                "order": "stochastic (seed 2)" if order == qa.ORDER_STOCHASTIC else "line-graph min-fill",
+                "order": {qa.ORDER_STOCHASTIC: "stochastic (seed 2)", qa.ORDER_SWEEP: "circuit sweep, CZ split"}.get(
+                    order, "line-graph min-fill"),
                 "plan_s": plan_s, "forms": res}
         if sample is not None:
             case["slices_contracted"] = picked
@@ -234,90 +240,64 @@ def run_grid(args, qa, np):
 
 
 def run_uneven(args, qa, np):
-    """512 QAOA p=2 edge light-cone networks from seeded 4-regular graphs of 20-120
-    vertices (log-uniform): networks whose tensors fit a device program (<= 2^16
-    elements) run in one persistent, graph-captured, autotuned launch; the larger ones
-    step by step with MFMA GEMMs; all in complex64 4M against the same networks
-    launched one pairwise contraction at a time in complex128."""
-    from qtorch_amd import workloads as wl
+    """Uneven batching as configured: 512 seeded 4-regular graphs of 20-120 vertices
+    (log-uniform), QAOA p = 2 with seeded angles per graph, every edge's light cone
+    (the correct qubit map), FP64 energy per graph.  Light cones whose tensors fit a
+    device program run in graph-captured persistent table-executor launches, the rest
+    on the sliced path with MFMA GEMMs on several streams; complex64 4M.  Compared with
+    complex128 one pairwise contraction per launch, one network at a time, on a
+    seeded sample of the networks (and every edge of the smallest graphs)."""
+    from qtorch_amd import uneven, workloads as wl
     rng = random.Random(args.seed)
-    ctx = qa.Context(0)
-    nets = []
-    g = 0
-    while len(nets) < args.networks:
+    graphs = []
+    for g in range(args.graphs):
         n = int(round(math.exp(rng.uniform(math.log(20), math.log(120)))))
         n += n % 2
-        dgf = wl.dgf_text(wl.regular_graph(n, 4, seed=7000 + g))
-        g += 1
-        ne, _ = qa.qaoa_num_edges(dgf)
-        ang = wl.qaoa_angles(2, rng.randrange(1 << 30))
-        for e in rng.sample(range(ne), min(ne, 8)):
-            q, m = qa.qaoa_edge_text(dgf, 2, False, e, ang)
-            nets.append((n, q, m))
-            if len(nets) == args.networks:
-                break
+        graphs.append((wl.dgf_text(wl.regular_graph(n, 4, seed=7000 + g)), 2, wl.qaoa_angles(2, 9000 + g)))
+    ctx = qa.Context(0)
+    u = uneven.UnevenEnergies(ctx, graphs, precision=qa.QT_C64_4M, streams=args.streams)
+    print(f"uneven: {len(u.nets)} light cones of {len(graphs)} graphs planned in {u.planning_s:.1f} s "
+          f"({len(u.small)} batched in {len(u.batches)} batches, {len(u.big)} large)", file=sys.stderr, flush=True)
+    lanes, wgs = u.configure()
+    energies, values, t_small, t_big = u.run()
+    # complex128, one pairwise contraction per launch, one network at a time: a seeded
+    # sample of the networks, and every edge of the three smallest graphs
+    sizes = [int(qa.qaoa_num_edges(d)[1]) for d, _, _ in graphs]
+    smallest = sorted(range(len(graphs)), key=lambda g: sizes[g])[:3]
+    sample = sorted(set(rng.sample(range(len(u.nets)), min(args.networks, len(u.nets)))) |
+                    {i for i, x in enumerate(u.nets) if x[0] in smallest})
     t0 = time.perf_counter()
-    small, big = [], []
-    for i, (n, q, m) in enumerate(nets):
-        net = qa.Network(q, m, None)
-        net.order(qa.ORDER_GREEDY, 0)
-        (small if int(net.plan()[:, 6].max()) <= 8 else big).append(i)
-        net.close()
-    sliced = {i: qa.SlicedNetwork(nets[i][1], nets[i][2], dim_log2=2, order=qa.ORDER_LINEGRAPH, max_bits=28)
-              for i in big}
-    plan_s = time.perf_counter() - t0
-    nb = qa.NetBatch(ctx, [nets[i][1] for i in small], [nets[i][2] for i in small], precision=qa.QT_C64_4M)
-    lanes, wgs, _ = nb.autotune()
-    nb.capture()
-    nb.launch()
-    ctx.sync()
-    t0 = time.perf_counter()
-    nb.launch()
-    ctx.sync()
-    t_small = time.perf_counter() - t0
-    v_small = nb.results()
-    # the large networks run step by step; four host threads, each with its own
-    # context (HIP stream), keep four of them in flight so their small launches overlap
-    from concurrent.futures import ThreadPoolExecutor
-    ctxs = [qa.Context(0) for _ in range(args.streams)]
-    order = sorted(big, key=lambda i: -sliced[i].plan()["cmacs_per_slice"])
-    shards = [order[k::args.streams] for k in range(args.streams)]
-
-    def work(k):
-        return {i: sliced[i].run(ctxs[k], qa.QT_C64_4M)[0] for i in shards[k]}
-
-    for k in range(args.streams):  # warm each context's allocator
-        if shards[k]:
-            sliced[shards[k][0]].run(ctxs[k], qa.QT_C64_4M)
-    t0 = time.perf_counter()
-    v_big = {}
-    with ThreadPoolExecutor(args.streams) as pool:
-        for part in pool.map(work, range(args.streams)):
-            v_big.update(part)
-    t_big = time.perf_counter() - t0
-    # one pairwise contraction per launch, complex128, the same greedy order
-    t0 = time.perf_counter()
-    ref = []
-    for n, q, m in nets:
+    ref = {}
+    for i in sample:
+        _, _, q, m = u.nets[i]
         net = qa.Network(q, m, ctx)
-        net.order(qa.ORDER_GREEDY, 0)
-        ref.append(net.value())
+        net.order(qa.ORDER_LINEGRAPH, 0)
+        ref[i] = net.value()
         net.close()
-    t_one = time.perf_counter() - t0
-    got = {i: v for i, v in zip(small, v_small)}
-    got.update(v_big)
-    err = max(abs(got[i] - ref[i]) / max(abs(ref[i]), 1e-3) for i in range(len(nets)))
+    t_one = (time.perf_counter() - t0) / len(sample)
+    err = max(abs(values[i] - ref[i]) / max(abs(ref[i]), 1e-3) for i in sample)
+    e_err = 0.0
+    for g in smallest:
+        e_ref = sum(0.5 * (1.0 - ref[i].real) for i, x in enumerate(u.nets) if x[0] == g)
+        e_err = max(e_err, abs(energies[g] - e_ref) / abs(e_ref))
     total = t_small + t_big
-    return {"metric": "QAOA p=2 uneven light-cone networks per second (batched + GEMM path vs one at a time)",
-            "value": len(nets) / total, "unit": "networks/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+    n_nets = len(u.nets)
+    u.close()
+    return {"metric": "QAOA p=2 light-cone networks per second, 512 graphs, every edge (batched + GEMM path)",
+            "value": n_nets / total, "unit": "networks/s", "n_gpus": 1, "steps": 1, "warmup": 1,
             "ms_per_step": 1e3 * total, "higher_is_better": True, "scaling": "strong", "vs_baseline": None,
             "dtype": "c64", "data": "synthetic",
-            "config": {"workload": "qaoa_p2_4regular_n20_120_edge_networks", "networks": len(nets),
-                       "batched": len(small), "gemm_path": len(big), "executor": [lanes, wgs],
-                       "gemm_path_streams": args.streams},
-            "batched_ms": 1e3 * t_small, "gemm_path_ms": 1e3 * t_big,
-            "one_at_a_time_c128_ms": 1e3 * t_one, "speedup_vs_one_at_a_time": t_one / total,
-            "max_rel_err_vs_c128": err, "floor": "|d| / max(|x|, 1e-3)", "host_planning_s": plan_s}
+            "config": {"workload": "qaoa_p2_4regular_n20_120_512graphs_every_edge", "graphs": len(graphs),
+                       "networks": n_nets, "batched": len(u.small), "batches": len(u.batches),
+                       "gemm_path": len(u.big), "executor": [lanes, wgs], "gemm_path_streams": args.streams},
+            "batched_ms": 1e3 * t_small, "gemm_path_ms": 1e3 * t_big, "host_planning_s": u.planning_s,
+            "graph_energies_per_s": len(graphs) / total,
+            "one_at_a_time_c128_ms_per_network": 1e3 * t_one,
+            "one_at_a_time_c128_ms_extrapolated": 1e3 * t_one * n_nets,
+            "speedup_vs_one_at_a_time": t_one * n_nets / total,
+            "reference_sample": len(sample), "max_rel_err_vs_c128": err, "floor": "|d| / max(|x|, 1e-3)",
+            "max_energy_rel_err_vs_c128": e_err, "energy_checked_graphs": [sizes[g] for g in smallest],
+            "energies_head": [float(x) for x in energies[:8]]}
 
 
 def run_serving(args, qa, np):
@@ -439,8 +419,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--sets", type=int, default=8192, help="angle sets per step")
+    ap.add_argument("--sets", type=int, default=131072, help="angle sets per step")
     ap.add_argument("--form", default="c128", choices=["c64_4m", "c64_3m", "c64_bf16x3", "c128"])
+    ap.add_argument("--per-edge", action="store_true",
+                    help="timed launches also write every edge's <ZZ> (default: energies only)")
     ap.add_argument("--latency-steps", type=int, default=2000,
                     help="steps launched and waited for one at a time for p50 / p99")
     ap.add_argument("--no-cpu", action="store_true")
@@ -460,7 +442,8 @@ def main():
     ap.add_argument("--grid-cases", default="", help="grid: comma-separated case indices (0: 6x8 d8, 1: 4x6 d14, "
                     "2: 6x8 d14 sample); default all")
     ap.add_argument("--grid-forms", type=int, default=5, help="grid: the first N of c128, 4M, 3M, bf16x3, 4M rescaled")
-    ap.add_argument("--networks", type=int, default=512, help="uneven: edge networks")
+    ap.add_argument("--networks", type=int, default=256, help="uneven: sampled networks checked one at a time")
+    ap.add_argument("--graphs", type=int, default=512, help="uneven: graphs")
     ap.add_argument("--streams", type=int, default=8, help="uneven: host threads / streams for the large networks")
     ap.add_argument("--qps", type=float, default=400.0, help="serving: arrival rate")
     ap.add_argument("--mix", default="light", choices=["light", "config"], help="serving: request mix")
@@ -520,6 +503,10 @@ def main():
             specialised, compile_error = False, str(exc)
     if specialised:
         lanes, wgs = 1, 0  # one lane per network, one wavefront per 64 networks
+    # the timed steps need the energies only: per-edge values are not written
+    # (group-sums-only launches, bit-identical energies); accuracy below is checked on
+    # a launch that writes every edge's value
+    sums_only = specialised and not args.per_edge and b.sums_only(True)
     # direct launches or a captured HIP graph, whichever runs a step faster here
     b.launch()
     ctx.sync()
@@ -558,7 +545,18 @@ def main():
     wall = time.perf_counter() - t0
     launch_ms = b.time(args.steps)  # HIP events on the engine's stream around whole launches
     kernel_ms = b.time_kernel(args.steps)  # the same for the contraction kernel alone
-    energies, zz = b.results()
+    energies, _ = b.results()
+    if sums_only:  # one launch that writes every edge's value, for the accuracy checks
+        b.sums_only(False)
+        if graph:
+            b.uncapture()
+        b.launch()
+        energies_v, zz = b.results()
+        sums_only_bitwise = bool(np.array_equal(energies, energies_v))
+    else:
+        energies_v, zz = b.results()
+        sums_only_bitwise = None
+    energies = energies_v
     info = b.info()
     value = ne * args.sets * args.steps / wall
 
@@ -633,7 +631,9 @@ def main():
         executed_flops = 2.0 * fmas
         achieved = executed_flops / (kernel_ms * 1e-3) / 1e12
         bound, peak = "valu", VALU_PEAK_TFLOPS
-        kernel_name = "qt_batch_specialised (generated straight-line FMA chains, one lane per network)"
+        kernel_name = ("qt_batch_specialised_sums (generated straight-line FMA chains, one lane per angle set and "
+                       "slice of edges, energies only)" if sums_only else
+                       "qt_batch_specialised (generated straight-line FMA chains, one lane per network)")
     else:
         executed_flops = flops_per_launch
         achieved = algorithmic_tflops
@@ -643,7 +643,10 @@ def main():
     # per angle set, one result written per network, the previous launch's results
     # read by the deferred energy sums, one FP64 energy per angle set
     param_bytes = args.sets * 32 * elem  # the 32 p parameter elements of an angle set (p = 1)
-    alg_bytes = param_bytes + n_networks * 16 * (2 if specialised else 1) + args.sets * 8
+    if sums_only:  # 16 partial sums per set written, read back by the next launch's sums
+        alg_bytes = param_bytes + args.sets * 16 * 8 * 2 + args.sets * 8
+    else:
+        alg_bytes = param_bytes + n_networks * 16 * (2 if specialised else 1) + args.sets * 8
     pmc = PMC_BYTES_PER_NETWORK.get(("specialised" if specialised else "tables", args.form, args.sets))
     res = {
         "metric": METRIC,
@@ -654,6 +657,7 @@ def main():
                    "edge_networks_per_step": n_networks, "form": args.form,
                    "order": "greedy", "executor": "specialised" if specialised else "tables",
                    "conjugate_symmetry": bool(specialised and not args.no_hermitian),
+                   "energies_only": bool(sums_only),
                    "executor_lanes": lanes, "wgs_per_cu": wgs,
                    "global_scratch": info["global_scratch"] and not specialised,
                    "staged_leaves": info["staged_leaves"] and not specialised},
@@ -666,6 +670,7 @@ def main():
                      "max_energy_rel_err_vs_c128": energy_err,
                      "max_edge_rel_err_vs_reference": ref_err, "c128_batched_vs_reference": c128_vs_ref,
                      "floor": "|d| / max(|x|, 1e-3)",
+                     "energies_only_equal_summed_values": sums_only_bitwise,
                      "compared": f"{args.sets} angle sets x {ne} edges against the complex128 table executor; "
                                  f"set 0 against the reference's own values"},
         "other_forms": by_form,

@@ -80,6 +80,9 @@ qt_status qt_sync(qt_ctx* ctx);
 const char* qt_last_error(const qt_ctx* ctx);
 /* Device bytes held by the context's allocator (in use + cached). */
 int64_t qt_mem_reserved(const qt_ctx* ctx);
+/* Returns the allocator's cached (unused) blocks to the device after the context's
+ * stream has drained; tensors in use keep their memory. */
+qt_status qt_ctx_trim(qt_ctx* ctx);
 
 /* ---- tensors (replace Node::mVals, src/Node.h:112-113,137,151,160) ---------------- */
 
@@ -204,6 +207,21 @@ qt_status qt_batch_autotune(qt_batch* b, int* lanes, int* wgs_per_cu, float* ms)
 /* Asynchronous run of every instance (replays the captured graph if there is one). */
 qt_status qt_batch_launch(qt_batch* b);
 /* *done = 1 once the last launch has finished (0 while it runs); never blocks. */
+/* Group sums only (no reference counterpart): launches of a compiled batch whose
+ * groups are strided (qt_batch_set_instances with n_groups and member j of group g =
+ * instance j n_groups + g), each member running one program in every group, compute
+ * the group sums without writing per-instance values: a generated kernel runs, per
+ * wavefront, 64 groups through the members j = s, s + 16, ... of one slice s and
+ * adds c0 + c1 Re(value) in that order, the order of the ordinary group sums, so the
+ * sums are bit-identical to theirs.  qt_batch_results then accepts values == NULL
+ * only.  *active (may be NULL) = 1 when launches currently take this path (compiled,
+ * with instances that fit); otherwise launches run as before.  on = 0 restores
+ * per-instance values. */
+/* Device bytes a batch holds for its programs and leaf pool (device arrays of the
+ * program store and the pool, allocated capacity): what stays resident between
+ * launches as programs are added. */
+int64_t qt_batch_resident_bytes(const qt_batch* b);
+qt_status qt_batch_set_sums_only(qt_batch* b, int on, int* active);
 qt_status qt_batch_poll(qt_batch* b, int* done);
 /* Captures the launch sequence (counter reset, persistent kernel, group sums) into a
  * HIP graph; later launches replay it. */

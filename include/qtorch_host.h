@@ -213,6 +213,11 @@ int qth_server_drain(qth_server* s);
  * ms: p50, p99, mean latency, time with two or more launches in flight, time with
  * any launch in flight; peak: largest planned device bytes admitted at once. */
 int qth_server_stats(qth_server* s, int64_t* counts, double* ms, int64_t* peak);
+/* Device bytes of the programs and constants resident in the slots' batches (counted
+ * against hbm_budget with the planned bytes of the requests in flight), and how many
+ * times a slot dropped its programs because they exceeded a quarter of the budget per
+ * slot (structures are added again on their next use). */
+int qth_server_memory(qth_server* s, int64_t* resident_bytes, int64_t* slot_resets);
 /* Route and planned device bytes of a request (planning only; no device needed).
  * kind QTH_SERVE_QAOA_ENERGY: text = .dgf, p, ref_map; QTH_SERVE_AMPLITUDE: text =
  * QASM, bits = the output bitstring. */
@@ -248,6 +253,13 @@ int qth_sliced_set_slices(qth_sliced* s, const int64_t* slices, int64_t n);
  * zero (the others contribute exactly zero: diagonal gates tie the wires around
  * them); written to out[0..*n_out).  Host only. */
 int qth_sliced_nonzero_slices(qth_sliced* s, int64_t max_count, int64_t* out, int64_t* n_out);
+/* The same with the search budget (partial assignments of the sliced wires visited;
+ * qth_sliced_nonzero_slices uses 2^24 and returns QTH_OTHER, with what it found
+ * written, when the budget runs out first): *complete = 1 when the list is full
+ * or every slice was visited, 0 when the budget ran out (the list may then miss
+ * nonzero slices). */
+int qth_sliced_nonzero_slices_ex(qth_sliced* s, int64_t max_count, int64_t budget, int64_t* out, int64_t* n_out,
+                                 int* complete);
 int qth_sliced_run(qth_sliced* s, qt_ctx* ctx, int precision, double* re, double* im, double* ms);
 int qth_sliced_destroy(qth_sliced* s);
 

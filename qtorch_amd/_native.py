@@ -49,6 +49,8 @@ _HIP_SIGS = {
     "qt_ctx_set_time_limit": (_c.c_int, [_vp, _c.c_double]),
     "qt_last_error": (_c.c_char_p, [_vp]),
     "qt_mem_reserved": (_c.c_int64, [_vp]),
+    "qt_ctx_trim": (_c.c_int, [_vp]),
+    "qt_batch_resident_bytes": (_c.c_int64, [_vp]),
     "qt_tensor_alloc": (_c.c_int, [_vp, _c.c_int, _c.c_int, _c.c_int, _c.POINTER(_vp)]),
     "qt_tensor_alloc_empty": (_c.c_int, [_vp, _c.c_int, _c.c_int, _c.c_int, _c.POINTER(_vp)]),
     "qt_tensor_upload": (_c.c_int, [_vp, _vp, _vp, _c.c_int64]),
@@ -66,6 +68,7 @@ _HIP_SIGS = {
     "qt_batch_add_programs": (_c.c_int, [_vp, _c.c_int, _c.POINTER(_vp), _i64p, _i32p]),
     "qt_batch_write_leaves": (_c.c_int, [_vp, _c.c_int64, _dp, _c.c_int64]),
     "qt_batch_poll": (_c.c_int, [_vp, _c.POINTER(_c.c_int)]),
+    "qt_batch_set_sums_only": (_c.c_int, [_vp, _c.c_int, _c.POINTER(_c.c_int)]),
     "qt_program_create": (_c.c_int, [_vp, _c.c_int, _c.c_int, _i32p, _i64p, _c.c_int, _i32p, _i32p,
                                      _c.POINTER(_vp)]),
     "qt_program_stats": (_c.c_int, [_vp, _i64p, _i64p, _c.POINTER(_c.c_int)]),
@@ -170,6 +173,9 @@ _HOST_SIGS = {
     "qth_sliced_set_range": (_c.c_int, [_vp, _c.c_int64, _c.c_int64]),
     "qth_sliced_set_slices": (_c.c_int, [_vp, _c.POINTER(_c.c_int64), _c.c_int64]),
     "qth_sliced_nonzero_slices": (_c.c_int, [_vp, _c.c_int64, _c.POINTER(_c.c_int64), _c.POINTER(_c.c_int64)]),
+    "qth_sliced_nonzero_slices_ex": (_c.c_int, [_vp, _c.c_int64, _c.c_int64, _c.POINTER(_c.c_int64),
+                                                _c.POINTER(_c.c_int64), _c.POINTER(_c.c_int)]),
+    "qth_server_memory": (_c.c_int, [_vp, _c.POINTER(_c.c_int64), _c.POINTER(_c.c_int64)]),
     "qth_qaoa_source": (_c.c_int, [_c.c_char_p, _c.c_int, _c.c_int, _c.c_int, _c.c_uint64, _c.c_int, _c.c_int,
                                    _c.c_int, _c.c_int, _c.c_char_p, _c.c_size_t, _c.POINTER(_c.c_size_t)]),
     "qth_qaoa_param_block": (_c.c_int, [_c.c_int, _dp, _dp]),
@@ -266,6 +272,7 @@ class Context:
         if rc != QT_OK:
             raise DeviceFailure(f"qt_ctx_create(device={device}) failed with status {rc}: no usable HIP device")
         self.handle = h
+        self.device = device
 
     def check(self, rc: int, what: str = "") -> None:
         if rc == QT_OK:
@@ -636,6 +643,14 @@ class _DeviceBatch:
                        "qt_batch_compile")
         return ms.value
 
+    def sums_only(self, on: bool = True) -> bool:
+        """Group sums only (qt_batch_set_sums_only): launches of the compiled batch write
+        no per-instance values; returns whether launches now take that path."""
+        a = ctypes.c_int()
+        self.ctx.check(hip.qt_batch_set_sums_only(self.dev, 1 if on else 0, ctypes.byref(a)), "qt_batch_set_sums_only")
+        self._sums_only = bool(on)
+        return bool(a.value)
+
     def specialised(self):
         """(active, FMAs per launch) of the generated executor."""
         a, f = ctypes.c_int(), ctypes.c_int64()
@@ -673,7 +688,11 @@ class QaoaBatch(_DeviceBatch):
         self.n_sets = bg.shape[0]
 
     def results(self):
+        """(energies, per-edge <ZZ>); the second is None after group-sums-only launches."""
         e = np.zeros(self.n_sets)
+        if getattr(self, "_sums_only", False):
+            _host_check(host.qth_qaoa_batch_results(self.handle, e.ctypes.data_as(_dp), None))
+            return e, None
         zz = np.zeros(self.n_sets * self.n_edges)
         _host_check(host.qth_qaoa_batch_results(self.handle, e.ctypes.data_as(_dp), zz.ctypes.data_as(_dp)))
         return e, zz.reshape(self.n_sets, self.n_edges)
@@ -866,9 +885,11 @@ class Server:
         m = (ctypes.c_double * 5)()
         peak = ctypes.c_int64()
         _host_check(host.qth_server_stats(self.handle, c, m, ctypes.byref(peak)))
+        res_b, resets = ctypes.c_int64(), ctypes.c_int64()
+        _host_check(host.qth_server_memory(self.handle, ctypes.byref(res_b), ctypes.byref(resets)))
         return {"submitted": c[0], "completed": c[1], "rejected": c[2], "launches": c[3], "edge_evals": c[4],
                 "p50_ms": m[0], "p99_ms": m[1], "mean_ms": m[2], "overlap_ms": m[3], "busy_ms": m[4],
-                "peak_planned_bytes": peak.value}
+                "peak_planned_bytes": peak.value, "resident_bytes": res_b.value, "slot_resets": resets.value}
 
     def close(self) -> None:
         if self.handle:
@@ -920,11 +941,22 @@ class SlicedNetwork:
 
     def nonzero_slices(self, max_count: int) -> List[int]:
         """Up to max_count slices, in increasing order, in which no leaf's sub-tensor
-        is all zero (every other slice is exactly zero).  Host only."""
+        is all zero (every other slice is exactly zero).  Host only; raises when the
+        search budget runs out first."""
         out = (ctypes.c_int64 * max(int(max_count), 1))()
         n = ctypes.c_int64()
         _host_check(host.qth_sliced_nonzero_slices(self.handle, int(max_count), out, ctypes.byref(n)))
         return list(out[:n.value])
+
+    def nonzero_slices_ex(self, max_count: int, budget: int = 1 << 24) -> Tuple[List[int], bool]:
+        """(slices, complete): complete is False when the budget of partial assignments
+        ran out before the list was full or every slice was visited."""
+        out = (ctypes.c_int64 * max(int(max_count), 1))()
+        n = ctypes.c_int64()
+        done = ctypes.c_int()
+        _host_check(host.qth_sliced_nonzero_slices_ex(self.handle, int(max_count), int(budget), out, ctypes.byref(n),
+                                                      ctypes.byref(done)))
+        return list(out[:n.value]), bool(done.value)
 
     def run(self, ctx: Context, precision: int = QT_PLAIN_C128):
         re, im, ms = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()

@@ -520,7 +520,13 @@ qt_status launch_batch(qt_batch* b, bool capture = false) {
     if (!b->shape_ok) return set_error(c, QT_INVALID_INPUT, "batch: executor shape not configured (last change failed)");
     if (b->needs_tout && !b->tensor_out)
         return set_error(c, QT_INVALID_INPUT, "batch: programs with tensor results need qt_batch_set_outputs");
+    if (b->specialised && b->sums_only && b->spec_sums_ok) return launch_specialised_sums(b, capture);
     qt_status s = QT_OK;
+    if (b->sums_pending && b->pending_parts) {  // sum the partial sums before d_out is overwritten
+        s = flush_sums(b);
+        if (s != QT_OK) return s;
+    }
+    b->values_valid = true;
     if (b->specialised) {
         // direct launches leave the group sums to the next launch (or a flush)
         const bool defer = !capture && b->group_stride_m > 0 && b->d_out_prev;
@@ -840,6 +846,10 @@ qt_status launch_group_sums(qt_batch* b) {
 qt_status flush_sums(qt_batch* b) {
     if (!b->sums_pending) return QT_OK;
     b->sums_pending = false;
+    if (b->pending_parts) {  // a group-sums-only launch's partial sums
+        b->pending_parts = false;
+        return launch_part_sums(b);
+    }
     return launch_group_sums(b);
 }
 
@@ -1061,6 +1071,7 @@ qt_status qt_batch_set_leaves(qt_batch* b, const double* pool, int64_t n_elems) 
     b->pool_elems = 0;
     b->n_inst = 0;  // parameter bases were checked against the old pool: set instances again
     b->sums_pending = false;
+    b->pending_parts = false;
     (void)qt::configure_scratch(b);
     return qt::write_pool(b, 0, pool, n_elems);
 }
@@ -1126,6 +1137,7 @@ qt_status qt_batch_set_instances(qt_batch* b, int n, const int32_t* prog, const 
     b->h_pbase.assign(param_base, param_base + n);
     qt_status s = qt::dev_put(ctx, &b->d_inst, &b->cap_inst, inst.data(), inst.size());
     b->sums_pending = false;  // the previous instances' results are gone
+    b->pending_parts = false;
     if (s == QT_OK) s = qt::dev_put<double>(ctx, &b->d_out, &b->cap_out, nullptr, size_t(std::max(n, 1)) * 2);
     if (s == QT_OK && n_groups > 0)
         s = qt::dev_put<double>(ctx, &b->d_out_prev, &b->cap_out_prev, nullptr, size_t(std::max(n, 1)) * 2);
@@ -1228,6 +1240,32 @@ qt_status qt_batch_launch(qt_batch* b) {
     return check_hip(ctx, hipEventRecord(b->ev_done, ctx->stream), "event record");
 }
 
+int64_t qt_batch_resident_bytes(const qt_batch* b) {
+    if (!b) return 0;
+    const int64_t elem = b->precision == QT_PLAIN_C128 ? 16 : 8;
+    return int64_t(b->cap_progs * sizeof(qt::ProgHdr) + b->cap_steps * sizeof(qt::StepDesc) +
+                   b->cap_tabs * sizeof(uint32_t) + b->cap_refs * sizeof(int64_t) +
+                   b->cap_leaf_lds * sizeof(int32_t) + b->cap_csts * sizeof(qt::CstEntry)) +
+           b->pool_cap * elem;
+}
+
+qt_status qt_batch_set_sums_only(qt_batch* b, int on, int* active) {
+    if (!b) return QT_INVALID_INPUT;
+    auto* ctx = b->ctx;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    if (b->sums_only != (on != 0)) {
+        b->sums_only = on != 0;
+        // a captured graph holds the other kind of launch
+        if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+        if (b->graph) (void)hipGraphDestroy(b->graph);
+        b->graph_exec = nullptr;
+        b->graph = nullptr;
+    }
+    if (active) *active = (b->specialised && b->sums_only && b->spec_sums_ok) ? 1 : 0;
+    return QT_OK;
+}
+
 qt_status qt_batch_poll(qt_batch* b, int* done) {
     if (!b || !done) return QT_INVALID_INPUT;
     std::lock_guard<std::recursive_mutex> g(b->ctx->mu);
@@ -1253,6 +1291,7 @@ qt_status qt_batch_capture(qt_batch* b) {
     (void)hipSetDevice(ctx->device);
     if (b->graph_exec) return QT_OK;
     qt_status s = qt::flush_sums(b);
+    if (s == QT_OK && b->specialised && b->sums_only && b->spec_sums_ok) s = qt::reserve_parts(b);
     if (s != QT_OK) return s;
     s = check_hip(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "begin capture");
     if (s != QT_OK) return s;
@@ -1292,6 +1331,8 @@ qt_status qt_batch_results(qt_batch* b, double* values, double* group_sums) {
     auto* ctx = b->ctx;
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
     (void)hipSetDevice(ctx->device);
+    if (values && !b->values_valid)
+        return set_error(ctx, QT_INVALID_INPUT, "qt_batch_results: group-sums-only launches keep no per-instance values");
     qt_status s = qt::flush_sums(b);
     if (s == QT_OK) s = check_hip(ctx, hipStreamSynchronize(ctx->stream), "batch sync");
     if (s == QT_OK && values && b->n_inst > 0)
@@ -1330,11 +1371,14 @@ qt_status qt_batch_time_kernel(qt_batch* b, int reps, float* ms_per_launch) {
         return set_error(ctx, QT_INVALID_INPUT, "batch: programs with tensor results need qt_batch_set_outputs");
     // a specialised kernel is timed as it runs in a stream of direct launches: each
     // launch also sums the previous launch's results (one untimed launch first)
+    const bool sums_only = b->specialised && b->sums_only && b->spec_sums_ok;
     const bool defer = b->specialised && b->group_stride_m > 0 && b->d_out_prev;
-    qt_status s = defer ? qt::launch_specialised(b, true) : QT_OK;
+    qt_status s = sums_only ? qt::launch_specialised_sums(b, false) : defer ? qt::launch_specialised(b, true) : QT_OK;
     if (s == QT_OK) s = check_hip(ctx, hipEventRecord(b->ev0, ctx->stream), "event record");
     for (int r = 0; r < reps && s == QT_OK; ++r) {
-        if (b->specialised) {
+        if (sums_only) {
+            s = qt::launch_specialised_sums(b, false);
+        } else if (b->specialised) {
             s = qt::launch_specialised(b, defer);
         } else {
             hipLaunchKernelGGL(qt::k_reset_counter, dim3(1), dim3(1), 0, ctx->stream, b->d_counter);

@@ -155,6 +155,18 @@ struct qt_batch {
     double* d_out_prev = nullptr;
     size_t cap_out_prev = 0;
     bool sums_pending = false;  // d_out's group sums have not been launched yet
+    // Group sums only (qt_batch_set_sums_only): for an edge-major batch whose members
+    // j < m each run one program in every group, a second generated kernel has each
+    // wavefront run 64 groups through the members j = s, s + 16, ... of one slice s,
+    // adding c0 + c1 Re(value) in that order (the strided group sums' order), and
+    // writes 16 partial sums per group; the sums over the slices run in the next
+    // launch's first workgroups (or at a flush).  No per-instance value is written.
+    hipFunction_t kernel_sums = nullptr;
+    std::vector<int32_t> spec_member_prog;  // the program of member j the kernel was generated with
+    bool sums_only = false;                 // requested by the caller
+    bool spec_sums_ok = false;              // the instances fit the generated sums kernel
+    bool pending_parts = false;             // the pending sums are over partial sums (d_out)
+    bool values_valid = true;               // d_out holds the last launch's per-instance values
 };
 
 namespace qt {
@@ -176,6 +188,15 @@ inline void batch_free(Ctx* c, void* p) {
 qt_status specialised_chunks(qt_batch* b);
 void drop_specialised(qt_batch* b);
 qt_status launch_specialised(qt_batch* b, bool defer_sums);
+// group sums only: partial sums per slice, the sums over slices deferred (or, when
+// captured, launched right after)
+qt_status launch_specialised_sums(qt_batch* b, bool capture);
+// the sums of the pending partial sums (a sums-only launch's), into d_group
+qt_status launch_part_sums(qt_batch* b);
+// whether the current instances fit the generated sums kernel (sets spec_sums_ok)
+void check_sums_fit(qt_batch* b);
+// sizes d_out for the partial sums before a capture (no allocation inside one)
+qt_status reserve_parts(qt_batch* b);
 // group sums of d_out on the context's stream (qt_batch.hip)
 qt_status launch_group_sums(qt_batch* b);
 // launches the group sums still pending for the last launch

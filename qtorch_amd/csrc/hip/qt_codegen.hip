@@ -42,6 +42,10 @@ namespace {
 // (the 45 light cones of a p = 1 QAOA graph need about 90 thousand, compiled in about
 // ten seconds) the compile time and code size stop paying off.
 constexpr int64_t kMaxSpecialisedFmas = 250000;
+// Blocks of 64 groups per tile of the group-sums-only kernel: 64 x 64 angle sets'
+// parameter blocks (2 MiB of complex128 p = 1 blocks) stay in an XCD's 4 MiB L2 while
+// all 16 slices run over them.
+constexpr int kSumsTile = 64;
 
 struct ProgView {
     const ParamLits* param_lits = nullptr;
@@ -333,7 +337,8 @@ bool emit_program(int id, const ProgView& pv, bool c128, bool hermitian, const d
 }
 
 std::string generate(const std::vector<ProgView>& progs, bool c128, bool hermitian, const double* pool,
-                     int64_t pool_elems, std::vector<int64_t>* fmas, std::string* err) {
+                     int64_t pool_elems, std::vector<int64_t>* fmas, std::string* err,
+                     const std::vector<int32_t>* member_prog = nullptr) {
     std::ostringstream os;
     os << "// specialised executor: " << progs.size() << " programs, " << (c128 ? "complex128" : "complex64 4M")
        << (hermitian ? ", conjugate-symmetric leaves" : "") << "\n\n";
@@ -409,6 +414,53 @@ std::string generate(const std::vector<ProgView>& progs, bool c128, bool hermiti
        << "  switch (prog) {\n";
     for (size_t i = 0; i < progs.size(); ++i) os << "    case " << i << ": prog" << i << "(P, o); break;\n";
     os << "    default: break;\n  }\n}\n";
+    if (member_prog && !member_prog->empty()) {
+        // Group sums only.  The first n_sum_blocks workgroups add up the previous
+        // launch's 16 partial sums of 64 groups each, in slice order; then workgroup
+        // (slice s, block of 64 groups) runs members j = s, s + 16, ... of its groups
+        // and adds c0 + c1 Re(value) in that order: the strided group sums' order, so
+        // the energies equal the table executor's bit for bit.  Workgroups go
+        // round-robin to the 8 XCDs: XCD x runs slices 2x and 2x + 1 (few programs per
+        // XCD for the instruction cache) of every block of groups.
+        const int m = int(member_prog->size());
+        os << "\nextern \"C\" __global__ __launch_bounds__(64) void qt_batch_specialised_sums(const " << real
+           << "* __restrict__ pool, int n_groups, double* __restrict__ part, const double* __restrict__ prev,\n"
+           << "    double* __restrict__ sums, int n_sum_blocks, double c0, double c1, long long pb0, long long pb_stride) {\n"
+           << "  if (int(blockIdx.x) < n_sum_blocks) {\n"
+           << "    const int g = int(blockIdx.x) * 64 + int(threadIdx.x);\n"
+           << "    if (g < n_groups) {\n"
+           << "      double t = prev[g];\n"
+           << "      for (int k = 1; k < 16; ++k) t += prev[(long long)k * n_groups + g];\n"
+           << "      sums[g] = t;\n"
+           << "    }\n"
+           << "    return;\n"
+           << "  }\n"
+           // XCD x (workgroups go round-robin to the 8 XCDs) owns the blocks of 64 groups
+           // [x per, (x + 1) per) and runs them in tiles of T blocks, every slice of a
+           // tile before the next tile: a tile's parameter blocks (T x 64 sets) stay in
+           // that XCD's L2 for all 16 slices, and the waves in flight run few slices'
+           // code at a time
+           << "  const int bid = int(blockIdx.x) - n_sum_blocks;\n"
+           << "  const int n_sb = (n_groups + 63) / 64, per = (n_sb + 7) / 8, T = per < " << kSumsTile << " ? per : "
+           << kSumsTile << ";\n"
+           << "  const int x = bid % 8, k = bid / 8;\n"
+           << "  const int t = k / (16 * T), r = k % (16 * T);\n"
+           << "  const int s = r / T, local = t * T + r % T;\n"
+           << "  if (local >= per) return;\n"
+           << "  const int g = (x * per + local) * 64 + int(threadIdx.x);\n"
+           << "  if (g >= n_groups) return;\n"
+           << "  const " << real << "* P = pool + 2 * (pb0 + pb_stride * (long long)g);\n"
+           << "  double acc = 0.0, o[2];\n"
+           << "  switch (s) {\n";
+        for (int sl = 0; sl < 16; ++sl) {
+            os << "    case " << sl << ":";
+            for (int j = sl; j < m; j += 16)
+                os << " prog" << (*member_prog)[size_t(j)] << "(P, o); acc += __builtin_fma(c1, o[0], c0);";
+            os << " break;\n";
+        }
+        os << "    default: break;\n  }\n"
+           << "  part[(long long)s * n_groups + g] = acc;\n}\n";
+    }
     return os.str();
 }
 
@@ -594,6 +646,7 @@ qt_status specialised_chunks(qt_batch* b) {
             b->spec_grid_sb = n_sb;
         }
     }
+    check_sums_fit(b);
     if (ch.empty()) return QT_OK;
     if (ch.size() > b->cap_schunks) {
         batch_free(b->ctx, b->d_schunks);
@@ -613,6 +666,9 @@ void drop_specialised(qt_batch* b) {
     if (b->module) (void)hipModuleUnload(b->module);
     b->module = nullptr;
     b->kernel = nullptr;
+    b->kernel_sums = nullptr;
+    b->spec_member_prog.clear();
+    b->spec_sums_ok = false;
     b->specialised = false;
     b->spec_consts.clear();
     b->spec_param_lits.clear();
@@ -630,8 +686,111 @@ bool specialised_consts_change(const qt_batch* b, int64_t offset, const double* 
     return false;
 }
 
+void check_sums_fit(qt_batch* b) {
+    b->spec_sums_ok = false;
+    const int m = int(b->spec_member_prog.size());
+    if (!b->kernel_sums || m == 0 || m != b->group_stride_m || b->spec_pb_mod != b->n_groups || b->n_groups == 0 ||
+        int64_t(b->n_inst) != int64_t(m) * b->n_groups)
+        return;
+    for (int j = 0; j < m; ++j)
+        for (int g = 0; g < b->n_groups; ++g)
+            if (b->h_prog[size_t(j) * size_t(b->n_groups) + size_t(g)] != b->spec_member_prog[size_t(j)]) return;
+    b->spec_sums_ok = true;
+}
+
+qt_status launch_part_sums(qt_batch* b) {
+    Ctx* c = b->ctx;
+    const void* pool = b->d_pool;
+    int n_groups = b->n_groups;
+    void* part = nullptr;
+    const void* prev = b->d_out;
+    double* sums = b->d_group;
+    int n_sum_blocks = (b->n_groups + 63) / 64;
+    double c0 = b->c0, c1 = b->c1;
+    long long pb0 = b->spec_pb0, pb_stride = b->spec_pb_stride;
+    void* params[] = {&pool, &n_groups, &part, &prev, &sums, &n_sum_blocks, &c0, &c1, &pb0, &pb_stride};
+    return check_hip(c, hipModuleLaunchKernel(b->kernel_sums, unsigned(n_sum_blocks), 1, 1, 64, 1, 1, 0, c->stream,
+                                              params, nullptr),
+                     "specialised sums");
+}
+
+qt_status launch_specialised_sums(qt_batch* b, bool capture) {
+    Ctx* c = b->ctx;
+    if (b->sums_pending && !b->pending_parts) {  // a previous launch's per-instance values
+        qt_status s = flush_sums(b);
+        if (s != QT_OK) return s;
+    }
+    const size_t need = size_t(16) * size_t(b->n_groups);
+    const int n_sb = (b->n_groups + 63) / 64;
+    // partial sums of this launch in d_out, the previous one's in d_out_prev
+    if (!capture) {
+        std::swap(b->d_out, b->d_out_prev);
+        std::swap(b->cap_out, b->cap_out_prev);
+    }
+    for (auto [buf, cap] : {std::make_pair(&b->d_out, &b->cap_out), std::make_pair(&b->d_out_prev, &b->cap_out_prev)}) {
+        if (*cap >= need) continue;
+        if (buf == &b->d_out_prev && (b->sums_pending || capture)) continue;  // holds pending partial sums
+        if (capture) return set_error(c, QT_INVALID_INPUT, "capture: partial-sum buffer not reserved");
+        batch_free(c, *buf);
+        *buf = nullptr;
+        *cap = 0;
+        qt_status s = check_hip(c, batch_malloc(c, reinterpret_cast<void**>(buf), need * sizeof(double)),
+                                "partial sums");
+        if (s != QT_OK) return s;
+        *cap = need;
+    }
+    const double* prev = nullptr;
+    int n_sum_blocks = 0;
+    if (!capture && b->sums_pending) {
+        prev = b->d_out_prev;
+        n_sum_blocks = (n_sb + 7) / 8 * 8;  // keeps the XCD of each slice workgroup
+    }
+    const void* pool = b->d_pool;
+    int n_groups = b->n_groups;
+    void* part = b->d_out;
+    double* sums = b->d_group;
+    double c0 = b->c0, c1 = b->c1;
+    long long pb0 = b->spec_pb0, pb_stride = b->spec_pb_stride;
+    void* params[] = {&pool, &n_groups, &part, &prev, &sums, &n_sum_blocks, &c0, &c1, &pb0, &pb_stride};
+    // compute workgroups: 8 XCDs x n_tiles tiles x 16 slices x T blocks (the kernel's order)
+    const int per = (n_sb + 7) / 8, T = std::min(per, kSumsTile), n_tiles = (per + T - 1) / T;
+    const unsigned grid = unsigned(n_sum_blocks + 8 * 16 * T * n_tiles);
+    qt_status s = check_hip(c, hipModuleLaunchKernel(b->kernel_sums, grid, 1, 1, 64, 1, 1, 0, c->stream, params,
+                                                     nullptr),
+                            "specialised sums");
+    if (s != QT_OK) return s;
+    b->values_valid = false;
+    b->sums_pending = true;
+    b->pending_parts = true;
+    if (capture) {  // no deferral inside a graph: the sums follow at once
+        s = launch_part_sums(b);
+        b->sums_pending = false;
+        b->pending_parts = false;
+    }
+    return s;
+}
+
+qt_status reserve_parts(qt_batch* b) {
+    const size_t need = size_t(16) * size_t(b->n_groups);
+    if (b->cap_out >= need) return QT_OK;
+    qt_status s = flush_sums(b);  // whatever d_out holds is summed first
+    if (s != QT_OK) return s;
+    batch_free(b->ctx, b->d_out);
+    b->d_out = nullptr;
+    b->cap_out = 0;
+    s = check_hip(b->ctx, batch_malloc(b->ctx, reinterpret_cast<void**>(&b->d_out), need * sizeof(double)),
+                  "partial sums");
+    if (s == QT_OK) b->cap_out = need;
+    return s;
+}
+
 qt_status launch_specialised(qt_batch* b, bool defer_sums) {
     Ctx* c = b->ctx;
+    if (b->sums_pending && b->pending_parts) {  // partial sums of a sums-only launch
+        qt_status s = flush_sums(b);
+        if (s != QT_OK) return s;
+    }
+    b->values_valid = true;
     if (b->needs_tout) return set_error(c, QT_INVALID_INPUT, "batch: specialised programs end in scalars");
     if (b->n_schunks == 0) return QT_OK;
     // deferred group sums: this launch writes the other buffer and sums the previous
@@ -753,8 +912,21 @@ qt_status qt_batch_compile(qt_batch* b, int flags, float* compile_ms) {
     std::swap(lits, b->spec_param_lits);  // read by batch_views
     std::string err;
     std::vector<int64_t> fmas;
-    const std::string src =
-        qt::generate(qt::batch_views(b), b->precision == QT_PLAIN_C128, hermitian, pool.data(), n_const, &fmas, &err);
+    // an edge-major batch whose members each run one program in every group also gets
+    // the group-sums-only kernel
+    std::vector<int32_t> member_prog;
+    if (b->group_stride_m > 0 && b->n_groups > 0 && int64_t(b->n_inst) == int64_t(b->group_stride_m) * b->n_groups) {
+        member_prog.resize(size_t(b->group_stride_m));
+        bool ok = true;
+        for (int j = 0; j < b->group_stride_m && ok; ++j) {
+            member_prog[size_t(j)] = b->h_prog[size_t(j) * size_t(b->n_groups)];
+            for (int g = 1; g < b->n_groups && ok; ++g)
+                ok = b->h_prog[size_t(j) * size_t(b->n_groups) + size_t(g)] == member_prog[size_t(j)];
+        }
+        if (!ok) member_prog.clear();
+    }
+    const std::string src = qt::generate(qt::batch_views(b), b->precision == QT_PLAIN_C128, hermitian, pool.data(),
+                                         n_const, &fmas, &err, &member_prog);
     if (src.empty()) {
         std::swap(lits, b->spec_param_lits);  // keep the state of the kernel that runs
         return set_error(ctx, QT_INVALID_INPUT, "qt_batch_compile: " + err);
@@ -771,6 +943,9 @@ qt_status qt_batch_compile(qt_batch* b, int flags, float* compile_ms) {
     hipFunction_t fn = nullptr;
     s = check_hip(ctx, hipModuleLoadData(&mod, code.data()), "module load");
     if (s == QT_OK) s = check_hip(ctx, hipModuleGetFunction(&fn, mod, "qt_batch_specialised"), "module function");
+    hipFunction_t fn_sums = nullptr;
+    if (s == QT_OK && !member_prog.empty())
+        s = check_hip(ctx, hipModuleGetFunction(&fn_sums, mod, "qt_batch_specialised_sums"), "module function");
     if (s != QT_OK) {
         if (mod) (void)hipModuleUnload(mod);
         std::swap(lits, b->spec_param_lits);
@@ -782,6 +957,8 @@ qt_status qt_batch_compile(qt_batch* b, int flags, float* compile_ms) {
     b->h_pool_mirror = pool;
     b->module = mod;
     b->kernel = fn;
+    b->kernel_sums = fn_sums;
+    b->spec_member_prog = member_prog;
     b->spec_consts.assign(pool.begin(), pool.begin() + 2 * std::min(n_const, b->max_leaf_ref_end));
     b->spec_hermitian = hermitian;
     b->spec_fmas = fmas;

@@ -152,6 +152,15 @@ const char* qt_last_error(const qt_ctx* c) { return c ? c->impl.err.c_str() : "n
 
 int64_t qt_mem_reserved(const qt_ctx* c) { return c ? c->impl.pool.reserved() : 0; }
 
+qt_status qt_ctx_trim(qt_ctx* c) {
+    if (!c) return QT_INVALID_INPUT;
+    std::lock_guard<std::recursive_mutex> g(c->impl.mu);
+    (void)hipSetDevice(c->impl.device);
+    qt_status s = check_hip(&c->impl, hipStreamSynchronize(c->impl.stream), "trim sync");
+    if (s == QT_OK) c->impl.pool.trim();
+    return s;
+}
+
 qt_status qt_tensor_alloc(qt_ctx* c, int rank, int dim_log2, int dtype, qt_tensor** out) {
     if (!c || !out) return QT_INVALID_INPUT;
     *out = nullptr;

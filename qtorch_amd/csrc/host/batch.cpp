@@ -556,6 +556,10 @@ void QaoaBatch::SetAngles(int n_sets, const double* bg) {
 }
 
 void QaoaBatch::Results(double* energies, double* zz) {
+    if (!zz) {  // energies only (also after group-sums-only launches)
+        CheckQt(ctx_, qt_batch_results(batch_, nullptr, energies), "qt_batch_results");
+        return;
+    }
     std::vector<double> vals(size_t(2 * n_sets_) * data_.pairs.size());
     CheckQt(ctx_, qt_batch_results(batch_, vals.data(), energies), "qt_batch_results");
     const size_t ne = data_.pairs.size(), ns = size_t(n_sets_);

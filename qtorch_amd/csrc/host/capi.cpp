@@ -596,6 +596,15 @@ int qth_server_drain(qth_server* s) {
     return Guard([&] { s->s->Drain(); });
 }
 
+int qth_server_memory(qth_server* s, int64_t* resident_bytes, int64_t* slot_resets) {
+    if (!s) return QTH_INVALID_FUNCTION_INPUT;
+    return Guard([&] {
+        const ServeStats st = s->s->Stats();
+        if (resident_bytes) *resident_bytes = st.resident_bytes;
+        if (slot_resets) *slot_resets = st.slot_resets;
+    });
+}
+
 int qth_server_stats(qth_server* s, int64_t* counts, double* ms, int64_t* peak) {
     if (!s) return QTH_INVALID_FUNCTION_INPUT;
     return Guard([&] {
@@ -684,13 +693,27 @@ int qth_sliced_set_slices(qth_sliced* s, const int64_t* slices, int64_t n) {
     return Guard([&] { s->n->SetSliceList(std::vector<int64_t>(slices, slices + n)); });
 }
 
-int qth_sliced_nonzero_slices(qth_sliced* s, int64_t max_count, int64_t* out, int64_t* n_out) {
-    if (!s || !out || !n_out || max_count < 1) return QTH_INVALID_FUNCTION_INPUT;
+int qth_sliced_nonzero_slices_ex(qth_sliced* s, int64_t max_count, int64_t budget, int64_t* out, int64_t* n_out,
+                                 int* complete) {
+    if (!s || !out || !n_out || max_count < 1 || budget < 1) return QTH_INVALID_FUNCTION_INPUT;
     return Guard([&] {
-        const auto v = s->n->NonzeroSlices(max_count);
+        bool done = true;
+        const auto v = s->n->NonzeroSlices(max_count, budget, &done);
         std::copy(v.begin(), v.end(), out);
         *n_out = int64_t(v.size());
+        if (complete) *complete = done ? 1 : 0;
     });
+}
+
+int qth_sliced_nonzero_slices(qth_sliced* s, int64_t max_count, int64_t* out, int64_t* n_out) {
+    int complete = 1;
+    const int rc = qth_sliced_nonzero_slices_ex(s, max_count, int64_t(1) << 24, out, n_out, &complete);
+    if (rc == QTH_OK && !complete) {
+        g_err = "qth_sliced_nonzero_slices: the search budget ran out before max_count slices were found; the list "
+                "written is incomplete (qth_sliced_nonzero_slices_ex takes a larger budget)";
+        return QTH_OTHER;
+    }
+    return rc;
 }
 
 int qth_sliced_run(qth_sliced* s, qt_ctx* ctx, int precision, double* re, double* im, double* ms) {

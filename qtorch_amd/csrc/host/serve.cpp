@@ -44,6 +44,11 @@ struct Server::Structure {
     // large-network route: each network planned once (line-graph order, sliced, small
     // subtrees fused); a request runs it with its own leaf values
     std::vector<std::shared_ptr<SlicedNetwork>> eager_nets;
+    // QAOA on that route: per edge network its leaf values for placeholder angles and
+    // its angle-dependent leaves (leaf, 32 layer + 0 for Rz / 16 for Rx: the offset of
+    // the gate's values in QaoaParamBlock); a request overwrites only those
+    std::vector<SlicedNetwork::LeafState> eager_base;
+    std::vector<std::vector<std::pair<int, int>>> eager_param_leaves;
     int eager_workers = 1;                // edge networks of a request run side by side
     std::string error;                    // set when the request cannot run
     ~Structure() {
@@ -103,6 +108,20 @@ std::shared_ptr<Server::Structure> BuildStructure(const ServeRequest& r, const S
                     const std::string m = QaoaEdgeMeasure(st->data, int(e));
                     auto sn = std::make_shared<SlicedNetwork>(q, m, 2, kOrderLineGraph, 0, opt.eager_max_bits);
                     peak = std::max(peak, sn->Plan().peak_bytes);
+                    {  // the angle-dependent leaves, numbered as CompileQaoa numbers their layers
+                        NetworkOptions po;
+                        Network net(q, &m, po);
+                        std::vector<std::pair<int, int>> pl;
+                        const int n_iter = int(st->data.realIterations[e].size()), nq = st->data.qubitsNeeded[e];
+                        int n_rz = 0, n_rx = 0;
+                        for (int i = 0; i < net.GetNumOriginalNodes(); ++i) {
+                            const auto ty = net.GetAllNodes()[size_t(i)]->GetTypeOfNode();
+                            if (ty == GateType::RZ) pl.push_back({i, 32 * (n_iter ? n_rz++ / n_iter : 0)});
+                            else if (ty == GateType::RX) pl.push_back({i, 32 * (n_rx++ / nq) + 16});
+                        }
+                        st->eager_param_leaves.push_back(std::move(pl));
+                        st->eager_base.push_back(sn->MakeLeafState(q, m));
+                    }
                     st->cmacs = SatMulAdd(st->cmacs, sn->Plan().cmacs_per_slice, sn->Plan().slices);
                     st->eager_nets.push_back(std::move(sn));
                     if (st->cmacs > opt.max_request_cmacs) break;  // refused at admission anyway
@@ -185,6 +204,7 @@ struct Server::Slot {
     bool busy = false;
     std::vector<std::shared_ptr<Pending>> running;
     int64_t running_bytes = 0;
+    int64_t resident = 0;  // device bytes of the programs and constants its batches hold
     // per batch: instance -> (request index in `running`, edge)
     std::vector<std::pair<int, int>> inst_of[kBatchClasses];
     int n_groups[kBatchClasses] = {0, 0};
@@ -194,8 +214,10 @@ struct Server::Slot {
 Server::Server(const ServeOptions& opt) : opt_(opt), t0_(Clock::now()) {
     if (opt_.slots < 1 || opt_.max_requests_per_launch < 1 || opt_.hbm_budget < 1 || opt_.param_capacity < 64 ||
         opt_.eager_workers < 1 || opt_.eager_workers > 16 || opt_.eager_loops < 1 ||
-        opt_.eager_loops > opt_.eager_workers)
+        opt_.eager_loops > opt_.eager_workers || opt_.max_structures < 1 || opt_.resident_limit < 1)
         throw InvalidFunctionInput();
+    // the slots' resident programs stay within a quarter of the budget
+    opt_.resident_limit = std::min(opt_.resident_limit, std::max<int64_t>(1, opt_.hbm_budget / (4 * opt_.slots)));
     try {
         for (int i = 0; i < opt_.slots; ++i) {
             auto s = std::make_unique<Slot>();
@@ -247,12 +269,41 @@ Server::~Server() {
 }
 
 std::shared_ptr<Server::Structure> Server::GetStructure(const ServeRequest& r) {
-    std::ostringstream key;
-    key << r.kind << '|' << r.p << '|' << r.ref_map << '|' << r.bits.size() << '|' << r.text;
-    std::lock_guard<std::mutex> g(struct_mu_);
-    auto& slot = structures_[key.str()];
-    if (!slot) slot = BuildStructure(r, opt_);
-    return slot;
+    std::ostringstream os;
+    os << r.kind << '|' << r.p << '|' << r.ref_map << '|' << r.bits.size() << '|' << r.text;
+    const std::string key = os.str();
+    std::promise<std::shared_ptr<Structure>> built;
+    std::shared_future<std::shared_ptr<Structure>> fut;
+    bool build = false;
+    {
+        std::lock_guard<std::mutex> g(struct_mu_);
+        auto it = structures_.find(key);
+        if (it == structures_.end()) {
+            if (int(structures_.size()) >= opt_.max_structures) {  // drop the least recently used ready one
+                auto lru = structures_.end();
+                for (auto jt = structures_.begin(); jt != structures_.end(); ++jt)
+                    if (jt->second.fut.wait_for(std::chrono::seconds(0)) == std::future_status::ready &&
+                        (lru == structures_.end() || jt->second.last_use < lru->second.last_use))
+                        lru = jt;
+                if (lru != structures_.end()) structures_.erase(lru);
+            }
+            fut = built.get_future().share();
+            it = structures_.emplace(key, CachedStructure{fut, 0}).first;
+            build = true;
+        }
+        it->second.last_use = ++struct_tick_;
+        fut = it->second.fut;
+    }
+    if (build) {  // planning runs unlocked: cached structures stay available meanwhile
+        try {
+            built.set_value(BuildStructure(r, opt_));
+        } catch (...) {
+            built.set_exception(std::current_exception());
+            std::lock_guard<std::mutex> g(struct_mu_);
+            structures_.erase(key);
+        }
+    }
+    return fut.get();
 }
 
 ServePlan PlanRequest(const ServeRequest& r, const ServeOptions& opt) {
@@ -351,6 +402,7 @@ void Server::Drain() {
 ServeStats Server::Stats() {
     std::lock_guard<std::mutex> g(mu_);
     ServeStats s = stats_;
+    s.resident_bytes = resident_bytes_;
     if (!latencies_.empty()) {
         std::vector<double> v = latencies_;
         std::sort(v.begin(), v.end());
@@ -370,7 +422,32 @@ ServeStats Server::Stats() {
 
 // ---- the loop ------------------------------------------------------------------------
 
+// An idle slot whose resident programs and constants exceed the limit drops them all:
+// its batches are created anew and structures are added again on their next use.
+void Server::ResetSlot(Slot& s) {
+    for (int c = 0; c < kBatchClasses; ++c) {
+        if (s.batch[c]) qt_batch_destroy(s.batch[c]);
+        s.batch[c] = nullptr;
+        s.first_prog[c].clear();
+        s.const_next[c] = opt_.param_capacity;
+    }
+    s.keep.clear();
+    (void)qt_ctx_trim(s.ctx);
+    for (int c = 0; c < kBatchClasses; ++c) {
+        CheckQt(s.ctx, qt_batch_create(s.ctx, 0, nullptr, opt_.precision, &s.batch[c]), "qt_batch_create");
+        CheckQt(s.ctx,
+                c == kRouteSmall ? qt_batch_config_ex(s.batch[c], opt_.small_lanes, opt_.small_wgs, QT_SCRATCH_AUTO)
+                                 : qt_batch_config_ex(s.batch[c], 256, 1, QT_SCRATCH_GLOBAL),
+                "qt_batch_config_ex");
+    }
+    std::lock_guard<std::mutex> g(mu_);
+    resident_bytes_ -= s.resident;
+    s.resident = 0;
+    ++stats_.slot_resets;
+}
+
 bool Server::StartIteration(Slot& s) {
+    if (s.resident > opt_.resident_limit) ResetSlot(s);
     std::vector<std::shared_ptr<Pending>> take;
     int64_t bytes = 0;
     {
@@ -379,7 +456,8 @@ bool Server::StartIteration(Slot& s) {
         while (!queue_.empty() && int(take.size()) < opt_.max_requests_per_launch) {
             auto& p = queue_.front();
             const int cls = p->st->route;
-            if (in_flight_bytes_ + bytes + p->st->planned_bytes > opt_.hbm_budget) break;
+            // resident programs and constants count against the budget too
+            if (in_flight_bytes_ + resident_bytes_ + bytes + p->st->planned_bytes > opt_.hbm_budget) break;
             if (params[cls] + int64_t(p->params.size()) > opt_.param_capacity) break;
             params[cls] += int64_t(p->params.size());
             bytes += p->st->planned_bytes;
@@ -425,6 +503,11 @@ bool Server::StartIteration(Slot& s) {
                             "qt_batch_add_programs");
                     it = s.first_prog[cls].emplace(p.st.get(), first).first;
                     s.keep.push_back(p.st);
+                    int64_t res = 0;
+                    for (auto* bb : s.batch) res += qt_batch_resident_bytes(bb);
+                    std::lock_guard<std::mutex> g(mu_);
+                    resident_bytes_ += res - s.resident;
+                    s.resident = res;
                 }
                 const int64_t pbase = int64_t(params.size() / 2);
                 params.insert(params.end(), reinterpret_cast<const double*>(p.params.data()),
@@ -553,7 +636,7 @@ void Server::EagerLoop(int k) {
             cv_.wait(lk, [&] { return stop_ || !eager_queue_.empty(); });
             if (eager_queue_.empty()) return;
             // admission: waits until its planned bytes fit beside the work in flight
-            if (in_flight_bytes_ + eager_queue_.front()->st->planned_bytes > opt_.hbm_budget &&
+            if (in_flight_bytes_ + resident_bytes_ + eager_queue_.front()->st->planned_bytes > opt_.hbm_budget &&
                 (busy_slots_ > 0 || eager_running_ > 0)) {
                 lk.unlock();
                 std::this_thread::sleep_for(std::chrono::microseconds(50));
@@ -577,15 +660,20 @@ void Server::EagerLoop(int k) {
                 // depend on which worker ran which edge
                 const int n_edges = int(p->st->data.pairs.size());
                 std::vector<double> zz(size_t(n_edges), 0.0);
+                const auto block = QaoaParamBlock(p->st->p, p->req.angles.data());
                 std::atomic<int> next{0};
                 std::vector<std::exception_ptr> errs(per);
                 auto work = [&](size_t w) {
                     try {
                         for (int e = next++; e < n_edges; e = next++) {
-                            // the shared plan, run with this request's leaf values
+                            // the shared plan, run with this request's leaf values: the
+                            // structure's own with the Rz / Rx leaves set from the angles
+                            // (the values parsing the request's QASM text would give)
                             const SlicedNetwork& sn = *p->st->eager_nets[size_t(e)];
-                            const auto leaves = sn.MakeLeafState(QaoaEdgeQasm(p->st->data, e, p->req.angles),
-                                                                 QaoaEdgeMeasure(p->st->data, e));
+                            SlicedNetwork::LeafState leaves = p->st->eager_base[size_t(e)];
+                            for (const auto& [leaf, off] : p->st->eager_param_leaves[size_t(e)])
+                                sn.SetLeaf(leaves, leaf,
+                                           std::vector<std::complex<double>>(block.begin() + off, block.begin() + off + 16));
                             zz[size_t(e)] = sn.Run(ctxs[w], QT_PLAIN_C128, nullptr, leaves).real();
                         }
                     } catch (...) {
@@ -618,6 +706,10 @@ void Server::EagerLoop(int k) {
             res.status = QTH_DEVICE;
             res.error = e.what();
         }
+        // this loop's streams keep cached blocks for the next request; beyond their share
+        // of the budget they are returned, so held memory stays near what is planned
+        for (size_t w = 0; w < per; ++w)
+            if (qt_mem_reserved(ctxs[w]) > opt_.hbm_budget / int64_t(2 * opt_.eager_workers)) (void)qt_ctx_trim(ctxs[w]);
         {
             std::lock_guard<std::mutex> g(mu_);
             in_flight_bytes_ -= p->st->planned_bytes;

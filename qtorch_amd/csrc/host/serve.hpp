@@ -5,6 +5,7 @@
 #include <chrono>
 #include <condition_variable>
 #include <deque>
+#include <future>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -31,6 +32,10 @@ struct ServeOptions {
     int64_t max_request_cmacs = int64_t(1) << 44;  // large-network path: planned complex MACs of a request
     int eager_workers = 4;              // large-network path: streams (edge networks run side by side)
     int eager_loops = 2;                // large-network requests in flight at once (sharing those streams)
+    int max_structures = 4096;          // planned structures kept on the host (least recently used dropped)
+    // device bytes of the programs and constants resident in a slot's batches: beyond
+    // this, an idle slot drops every program (structures are added again on use)
+    int64_t resident_limit = 2LL << 30;
 };
 
 enum ServeKind { kServeQaoaEnergy = 0, kServeAmplitude = 1 };
@@ -70,6 +75,8 @@ struct ServeStats {
     double busy_ms = 0;          // time with any launch in flight
     int64_t peak_planned_bytes = 0;
     int64_t edge_evals = 0;      // QAOA edge networks evaluated
+    int64_t resident_bytes = 0;  // programs and constants resident in the slots' batches
+    int64_t slot_resets = 0;     // times a slot dropped its programs (resident limit)
 };
 
 class Server {
@@ -95,6 +102,7 @@ private:
     void Loop();
     void EagerLoop(int k);
     bool StartIteration(Slot& s);
+    void ResetSlot(Slot& s);
     bool FinishIteration(Slot& s, bool wait);
     void Complete(Pending& p, ServeResult res);
 
@@ -103,12 +111,21 @@ private:
     std::condition_variable cv_, done_cv_;
     std::deque<std::shared_ptr<Pending>> queue_, eager_queue_;
     std::unordered_map<int64_t, ServeResult> results_;
-    std::map<std::string, std::shared_ptr<Structure>> structures_;
+    // planned structures by key: built outside struct_mu_ (other requests for the same
+    // key wait on the future, the rest go on); least recently used ones beyond
+    // opt_.max_structures are dropped (requests and slots that hold one keep it)
+    struct CachedStructure {
+        std::shared_future<std::shared_ptr<Structure>> fut;
+        uint64_t last_use = 0;
+    };
+    std::map<std::string, CachedStructure> structures_;
+    uint64_t struct_tick_ = 0;
     std::mutex struct_mu_;
     std::vector<std::unique_ptr<Slot>> slots_;
     std::vector<qt_ctx*> eager_ctxs_;  // one stream per large-network worker
     std::vector<double> latencies_;
     int64_t next_id_ = 1, in_flight_bytes_ = 0, outstanding_ = 0;
+    int64_t resident_bytes_ = 0;  // programs and constants resident in the slots' batches
     ServeStats stats_;
     bool stop_ = false;
     Clock::time_point t0_;

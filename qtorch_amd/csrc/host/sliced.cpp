@@ -217,19 +217,28 @@ SlicedNetwork::SlicedNetwork(const std::string& qasm, const std::string& measure
     plan_.sliced_wires = int(sliced_.size());
     plan_.slices = int64_t(1) << (int(sliced_.size()) * dl_);
     for (const auto& w : wires) plan_.peak_bits = std::max(plan_.peak_bits, bits_of(w));
-    // live bytes of one slice (complex128): leaves, then intermediates until consumed
+    // device bytes of one slice (complex128): leaves, then intermediates until consumed,
+    // plus each step's scratch while it runs (a GEMM-shaped step may permute copies of
+    // both operands); the split reduction's partial sums (at most 2 MiB) and the fused
+    // subtrees' outputs (PlanFusion) are added on top
     int64_t leaves = 0, live = 0, peak = 0;
     for (int i = 0; i < n_leaves_; ++i) leaves += int64_t(16) << bits_of(leaf_wires_[size_t(i)]);
     std::vector<int64_t> bytes(wires.size(), 0);
     for (size_t s = 0; s < steps_.size(); ++s) {
+        const Step& st = steps_[s];
         const int id = n_leaves_ + int(s);
-        bytes[size_t(id)] = int64_t(16) << (steps_[s].rank_c * dl_);
+        bytes[size_t(id)] = int64_t(16) << (st.rank_c * dl_);
         live += bytes[size_t(id)];
-        peak = std::max(peak, live);
-        for (int op : {steps_[s].a, steps_[s].b})
+        int from_a = 0;
+        while (from_a < st.rank_c && (st.free_src[size_t(from_a)] & QT_FROM_A)) ++from_a;
+        const bool gemm_shape = from_a * dl_ >= 6 && (st.rank_c - from_a) * dl_ >= 6 && st.k * dl_ >= 3;
+        const int64_t scratch =
+            gemm_shape ? (int64_t(16) << (st.rank_a * dl_)) + (int64_t(16) << (st.rank_b * dl_)) : 0;
+        peak = std::max(peak, live + scratch);
+        for (int op : {st.a, st.b})
             if (op >= n_leaves_) live -= bytes[size_t(op)];
     }
-    plan_.peak_bytes = leaves + peak;
+    plan_.peak_bytes = leaves + peak + (int64_t(2) << 20);
     // sliced axes of every leaf
     cut_.resize(size_t(n_leaves_));
     leaf_rank_.resize(size_t(n_leaves_));
@@ -337,6 +346,13 @@ void SlicedNetwork::PlanFusion() {
     }
     plan_.fused_subtrees = int(roots_.size());
     plan_.fused_steps = fused;
+    // a chunk of slices' subtree results (Run: at most 256 MiB, or one slice's), their
+    // parameter blocks and the batch's constants
+    if (!roots_.empty()) {
+        const int64_t chunk = std::max<int64_t>(
+            1, std::min<int64_t>({plan_.slices, 4096, (int64_t(256) << 20) / std::max<int64_t>(1, root_bytes_c128_)}));
+        plan_.peak_bytes += chunk * (root_bytes_c128_ + 16 * block_elems_) + int64_t(const_pool_.size()) * 8;
+    }
 }
 
 SlicedNetwork::LeafState SlicedNetwork::MakeLeafState(const std::string& qasm, const std::string& measure) const {
@@ -363,6 +379,19 @@ SlicedNetwork::LeafState SlicedNetwork::MakeLeafState(const std::string& qasm, c
         }
     }
     return ls;
+}
+
+void SlicedNetwork::SetLeaf(LeafState& ls, int leaf, const std::vector<std::complex<double>>& v) const {
+    if (leaf < 0 || leaf >= n_leaves_ || ls.vals.size() != size_t(n_leaves_) ||
+        v.size() != leaf_vals_[size_t(leaf)].size())
+        throw InvalidFunctionInput();
+    ls.vals[size_t(leaf)] = v;
+    for (const auto& [cl, off] : const_leaves_)
+        if (cl == leaf)
+            for (size_t e = 0; e < v.size(); ++e) {
+                ls.const_pool[size_t(2 * (off + int64_t(e)))] = v[e].real();
+                ls.const_pool[size_t(2 * (off + int64_t(e)) + 1)] = v[e].imag();
+            }
 }
 
 void SlicedNetwork::SetLeafValues(const std::string& qasm, const std::string& measure) {
@@ -411,9 +440,10 @@ void SlicedNetwork::SetSliceList(const std::vector<int64_t>& slices) {
     slice_list_ = slices;
 }
 
-std::vector<int64_t> SlicedNetwork::NonzeroSlices(int64_t max_count, int64_t budget) const {
+std::vector<int64_t> SlicedNetwork::NonzeroSlices(int64_t max_count, int64_t budget, bool* complete) const {
     std::vector<int64_t> found;
     const int n_digits = int(sliced_.size());
+    if (complete) *complete = true;
     if (max_count <= 0) return found;
     if (n_digits == 0) return {0};
     const int64_t D = int64_t(1) << dl_;
@@ -507,6 +537,8 @@ std::vector<int64_t> SlicedNetwork::NonzeroSlices(int64_t max_count, int64_t bud
         --digit;
         value[size_t(digit)] = -1;
     }
+    // complete: every slice visited, or the list is full; otherwise the budget ran out
+    if (complete) *complete = digit >= n_digits || int64_t(found.size()) >= max_count;
     return found;
 }
 

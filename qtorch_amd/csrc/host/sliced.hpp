@@ -42,6 +42,8 @@ public:
         std::vector<double> const_pool;  // the fused programs' constant leaves, FP64 pairs
     };
     LeafState MakeLeafState(const std::string& qasm, const std::string& measure) const;
+    // New values for one leaf of a leaf state (the fused programs' constants follow).
+    void SetLeaf(LeafState& ls, int leaf, const std::vector<std::complex<double>>& values) const;
     // Run with those leaf values instead of the network's own.
     std::complex<double> Run(qt_ctx* ctx, int precision, SliceStats* stats, const LeafState& leaves) const;
     const SliceStats& Plan() const { return plan_; }
@@ -58,8 +60,10 @@ public:
     // chain of diagonal leaves (CZ, RZ, ...) take different values, or a wire takes a
     // value its chain rules out (a |0> state, an output bit).  Every slice left out
     // contributes exactly zero.  Stops early after visiting `budget` partial
-    // assignments of the sliced wires.
-    std::vector<int64_t> NonzeroSlices(int64_t max_count, int64_t budget = int64_t(1) << 24) const;
+    // assignments of the sliced wires; *complete (if given) is then false, and true
+    // when the list is full or every slice was visited.
+    std::vector<int64_t> NonzeroSlices(int64_t max_count, int64_t budget = int64_t(1) << 24,
+                                       bool* complete = nullptr) const;
     // New leaf values for the same network structure (the plan, slicing and fused
     // programs depend on the structure only): the nodes of (qasm, measure) parsed as
     // in the constructor must have the ranks of this network's leaves.

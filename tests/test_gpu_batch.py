@@ -194,3 +194,33 @@ def test_p2_light_cones_batched(ctx):
     e16, zz16 = b.results()
     assert b.info()["global_scratch"]
     assert np.array_equal(zz16, zz) and np.array_equal(e16, e)
+
+
+@pytest.mark.parametrize("precision,tol,e_tol", [(qa.QT_C64_4M, 5e-5, 1e-5), (qa.QT_PLAIN_C128, 1e-12, 1e-12)])
+def test_uneven_p2_graph_energies(ctx, precision, tol, e_tol):
+    """Uneven batching as the benchmark runs it, on three 4-regular graphs (20, 24, 28
+    vertices, p = 2, their own angles): every edge light cone, the ones whose largest
+    tensor has rank <= 8 in batched launches captured in HIP graphs, the others on the
+    sliced GEMM path on two streams; every per-graph FP64 energy and every edge against
+    the drop-in complex128 path, one network at a time (edges with the 1e-3 floor;
+    complex64 4M edges of these 26-qubit light cones reach about 1.2e-5, the energies
+    stay within 1e-5)."""
+    from qtorch_amd import uneven
+    from qtorch_amd import workloads as wl
+    graphs = [(wl.dgf_text(wl.regular_graph(n, 4, seed=50 + n)), 2, wl.qaoa_angles(2, n)) for n in (20, 24, 28)]
+    u = uneven.UnevenEnergies(ctx, graphs, precision=precision, streams=2, chunk=16)
+    assert len(u.small) > 0 and len(u.big) > 0 and len(u.batches) >= 2
+    energies, values, _, _ = u.run()
+    ref_e = np.zeros(len(graphs))
+    for i, (g, e, q, m) in enumerate(u.nets):
+        net = qa.Network(q, m, ctx)
+        net.order(qa.ORDER_LINEGRAPH, 0)
+        ref = net.value()
+        net.close()
+        assert abs(values[i] - ref) <= tol * max(abs(ref), 1e-3), (g, e, values[i], ref)
+        ref_e[g] += 0.5 * (1.0 - ref.real)
+    assert np.max(np.abs(energies - ref_e) / np.abs(ref_e)) <= e_tol
+    # a second run (captured graphs replayed) gives the same values
+    energies2, values2, _, _ = u.run()
+    assert np.array_equal(values2, values) and np.array_equal(energies2, energies)
+    u.close()

@@ -210,3 +210,33 @@ def test_admission_rejects_too_much_work():
     finally:
         srv.close()
     assert res["status"] == 9 and st["rejected"] == 1
+
+
+def test_large_route_requests_in_flight_together():
+    """Three large-network requests of one structure with different angles, submitted
+    before any is waited for: two run at once (each on its own streams, sharing the
+    plan, each with its own leaf values set from its angles); every energy matches a
+    state vector of every edge's light cone to 1e-10, and the planned bytes admitted
+    at once never exceed the budget."""
+    dgf = wl.dgf_text(wl.regular_graph(10, 4, seed=31))
+    p = 2
+    assert qa.serve_plan(qa.SERVE_QAOA_ENERGY, dgf, p)["route"] == "eager"
+    ne, _ = qa.qaoa_num_edges(dgf)
+    budget = 16 << 30
+    srv = qa.Server(slots=1, hbm_budget=budget)
+    try:
+        angs = [wl.qaoa_angles(p, s) for s in (11, 12, 13)]
+        ids = [srv.submit_qaoa(dgf, p, a) for a in angs]
+        res = [srv.wait(i) for i in ids]
+        st = srv.stats()
+    finally:
+        srv.close()
+    for ang, r in zip(angs, res):
+        assert r["status"] == 0 and r["route"] == "eager", r
+        ref = 0.0
+        for e in range(ne):
+            q, m = qa.qaoa_edge_text(dgf, p, False, e, ang)
+            psi, nq = sv.simulate(q)
+            ref += 0.5 * (1.0 - sv.expectation(psi, nq, m).real)
+        assert abs(r["value"].real - ref) <= 1e-10 * abs(ref), (r, ref)
+    assert 0 < st["peak_planned_bytes"] <= budget

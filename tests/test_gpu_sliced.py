@@ -146,3 +146,56 @@ def test_nonzero_slices_cover_the_amplitude(ctx):
     sn.set_slices(listed)
     part, _ = sn.run(ctx)
     assert abs(part - whole) <= 1e-12 * abs(whole), (part, whole)
+
+
+@pytest.mark.parametrize("rows,cols,depth,max_bits", [(4, 4, 8, 6), (4, 5, 10, 10), (4, 5, 10, 40), (3, 6, 12, 9)])
+def test_sweep_order_with_split_cz_matches_statevector(ctx, rows, cols, depth, max_bits):
+    """The circuit sweep order with CZ split into two bonded rank-3 halves, unsliced and
+    sliced, in complex128 and 4M, against the state vector's amplitude."""
+    q = wl.grid_random_circuit(rows, cols, depth, seed=7)
+    n = rows * cols
+    bits = wl.bitstring(n, 3)
+    sn = qa.SlicedNetwork(q, bits, dim_log2=1, order=qa.ORDER_SWEEP, seed=0, max_bits=max_bits)
+    plan = sn.plan()
+    assert plan["peak_bits"] <= max_bits
+    psi, _ = sv.simulate(q)
+    ref = psi[_amp_index(bits)]
+    scale = max(abs(ref), 2.0 ** (-n / 2))
+    for prec, tol in ((qa.QT_PLAIN_C128, 1e-12), (qa.QT_C64_4M, 2e-5)):
+        v, _ = sn.run(ctx, prec)
+        assert abs(v - ref) <= tol * scale, (prec, v, ref)
+    sn.close()
+
+
+def test_split_diagonal_gates_match_statevector(ctx):
+    """CZ, CPHASE and CRk split into bonded halves (the sweep order) on a circuit that
+    uses all three, against the state vector."""
+    lines = ["6"] + [f"H {i}" for i in range(6)]
+    lines += ["CZ 0 1", "CPHASE 0.7 1 2", "CRk 2 3", "Rx 0.4 3", "CZ 3 4", "CPHASE -1.3 5 0", "CRk 1 5",
+              "Ry 1.1 2", "CZ 2 5", "Rz 0.9 4", "CPHASE 2.2 4 1"]
+    q = "\n".join(lines) + "\n"
+    psi, n = sv.simulate(q)
+    for bits in ("010110", "111000", "000001"):
+        sn = qa.SlicedNetwork(q, bits, dim_log2=1, order=qa.ORDER_SWEEP, seed=1, max_bits=40)
+        v, _ = sn.run(ctx, qa.QT_PLAIN_C128)
+        ref = psi[_amp_index(bits)]
+        assert abs(v - ref) <= 1e-13, (bits, v, ref)
+        sn.close()
+
+
+def test_grid_6x8_depth14_whole_amplitude(ctx):
+    """The precision-stress configuration computed whole: 48-qubit 6x8 depth-14 grid
+    circuit, one seeded output amplitude, circuit sweep order (29-bit peak, 2.5e11
+    complex MACs, unsliced), complex128 against 4M and 4M with per-step rescaling
+    (the sliced 28-bit plan, 64 slices, is compared with it in the benchmark)."""
+    q = wl.grid_random_circuit(6, 8, 14, seed=2024)
+    bits = wl.bitstring(48, 1)
+    sn = qa.SlicedNetwork(q, bits, dim_log2=1, order=qa.ORDER_SWEEP, seed=0, max_bits=29)
+    plan = sn.plan()
+    assert plan["slices"] == 1 and plan["peak_bits"] == 29 and plan["cmacs_per_slice"] < 2 ** 38
+    v128, _ = sn.run(ctx, qa.QT_PLAIN_C128)
+    assert 1e-9 < abs(v128) < 1e-6  # |amp|^2 ~ 2^-48 for a random circuit
+    for prec in (qa.QT_C64_4M, qa.QT_C64_4M | qa.QT_RESCALE):
+        v, _ = sn.run(ctx, prec)
+        assert abs(v - v128) <= 1e-5 * abs(v128), (prec, v, v128)
+    sn.close()

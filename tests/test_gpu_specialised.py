@@ -248,3 +248,41 @@ def test_deferred_group_sums(ctx):
     assert np.array_equal(e3, e_t)
     b.close()
     t.close()
+
+
+@pytest.mark.parametrize("precision,n_sets,hermitian", [(qa.QT_PLAIN_C128, 200, True), (qa.QT_C64_4M, 200, False),
+                                                        (qa.QT_PLAIN_C128, 32768, True)])
+def test_group_sums_only_equal_the_summed_values(ctx, precision, n_sets, hermitian):
+    """qt_batch_set_sums_only: the generated sums kernel runs 64 angle sets through the
+    edges of one slice (edges s, s + 16, s + 32) and writes 16 partial sums per set,
+    summed in the next launch; the energies equal, bit for bit, those of launches that
+    write every edge's value and sum them with the strided group sums, over
+    consecutive launches, captured-graph replays, new angle sets and back."""
+    b = qa.QaoaBatch(ctx, _graph(), 1, ref_map=True, precision=precision)
+    b.set_angles(_sets(n_sets, 21))
+    b.compile(hermitian=hermitian)
+    b.launch()
+    e_vals, zz = b.results()
+    assert zz is not None
+    assert b.sums_only(True)
+    for _ in range(3):  # consecutive launches: each sums the previous one's partial sums
+        b.launch()
+    e1, z1 = b.results()
+    assert z1 is None and np.array_equal(e1, e_vals)
+    sets2 = _sets(n_sets, 22)
+    b.set_angles(sets2)
+    assert b.sums_only(True)  # still active for the new instances
+    b.launch()
+    e2, _ = b.results()
+    b.capture()
+    for _ in range(3):
+        b.launch()
+        e3, _ = b.results()
+        assert np.array_equal(e3, e2)
+        ctx.sync()
+    assert not b.sums_only(False)
+    b.launch()
+    e4, z4 = b.results()
+    assert np.array_equal(e4, e2) and z4 is not None
+    assert np.array_equal(e4, np.sum(0.5 * (1.0 - z4), axis=1)) or np.max(np.abs(e4 - np.sum(0.5 * (1.0 - z4), axis=1))) <= 1e-12 * np.max(np.abs(e4))
+    b.close()

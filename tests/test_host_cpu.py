@@ -460,3 +460,37 @@ def test_contract_nodes_from_two_threads():
         replay = qa.Network(both_q, None, None)
         replay.contract_given(rec)
         assert replay.flops() == net.flops() and replay.counts()["done"]
+
+
+def test_sliced_planned_bytes_cover_step_scratch():
+    """A slice's planned device bytes cover, at every GEMM-shaped step, the live
+    operands and result plus the permuted copies of both operands that the GEMM may
+    make (and the split reduction's partial sums), not only the live tensors."""
+    from qtorch_amd import workloads as wl
+    q = wl.grid_random_circuit(5, 6, 10, seed=4)
+    bits = wl.bitstring(30, 2)
+    plan = qa.SlicedNetwork(q, bits, dim_log2=1, order=qa.ORDER_LINEGRAPH, max_bits=40).plan()
+    net = qa.Network(q, bits, None, dim_log2=1)
+    net.order(qa.ORDER_LINEGRAPH, 0)
+    steps = net.plan()
+    need = 0
+    for _, _, _, ra, rb, k, rc, _ in steps:
+        if (ra - k) >= 6 and (rb - k) >= 6 and k >= 3:  # dimension-2 wires: one bit per axis
+            need = max(need, 16 * (2 * (2 ** ra) + 2 * (2 ** rb) + 2 ** rc))
+    assert plan["gemm_cmacs_per_slice"] > 0 and need > 0
+    assert plan["peak_bytes"] >= need + (2 << 20), (plan["peak_bytes"], need)
+
+
+def test_nonzero_slices_report_a_truncated_search():
+    """qth_sliced_nonzero_slices_ex: a budget too small to finish says so (complete =
+    0); a sufficient one lists every nonzero slice and says it is complete; the plain
+    entry point returns an error instead of a silently short list."""
+    from qtorch_amd import workloads as wl
+    q = wl.grid_random_circuit(4, 4, 8, seed=11)
+    sn = qa.SlicedNetwork(q, wl.bitstring(16, 5), dim_log2=1, order=qa.ORDER_LINEGRAPH, max_bits=8)
+    full, done = sn.nonzero_slices_ex(1 << 20, budget=1 << 24)
+    assert done and 0 < len(full) <= sn.plan()["slices"]
+    part, done = sn.nonzero_slices_ex(1 << 20, budget=3)
+    assert not done and part == full[:len(part)]
+    capped, done = sn.nonzero_slices_ex(1, budget=1 << 24)
+    assert done and capped == full[:1]
