@@ -124,6 +124,23 @@ qt_status qt_contract_pair(qt_ctx* ctx, const qt_tensor* A, int rank_a, const qt
                            int rank_c, const int32_t* free_src, qt_tensor* C, int precision,
                            int64_t* cmacs);
 
+/* The same contraction without allocating: no hipMalloc, hipFree or other allocating
+ * call.  It has no counterpart in the reference; it is for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The caller passes A, B and an allocated C as for qt_contract_pair; the
+ * step's scratch (GEMM operand permutes: A's and B's sizes, 256-byte aligned; split
+ * reduction partial sums: at most 2 MiB) comes from a fixed 64 MiB device array per
+ * GPU, held from the first qt_ctx_create on that GPU until the process ends.  A step
+ * whose scratch exceeds 64 MiB returns QT_INVALID_INPUT before launching anything, as
+ * does QT_RESCALE (its exponent table is allocated on first use: use
+ * qt_contract_pair).  Every call on a GPU shares that GPU's array and nothing in it
+ * needs to start at zero; calls that overlap on one GPU (from several contexts or
+ * threads) race on it. */
+qt_status qt_contract_pair_noalloc(qt_ctx* ctx, const qt_tensor* A, int rank_a, const qt_tensor* B,
+                                   int rank_b, int k, const int32_t* pair_a, const int32_t* pair_b,
+                                   int rank_c, const int32_t* free_src, qt_tensor* C, int precision,
+                                   int64_t* cmacs);
+
 /* Reads the rank-0 result of a finished contraction (the mFinalVal read of
  * src/Network.h:961-969 and src/LineGraph.h:388), with a rescaled tensor's exponent
  * applied in FP64; synchronises. */

@@ -62,6 +62,8 @@ _HIP_SIGS = {
     "qt_tensor_rank": (_c.c_int, [_vp]),
     "qt_contract_pair": (_c.c_int, [_vp, _vp, _c.c_int, _vp, _c.c_int, _c.c_int, _i32p, _i32p, _c.c_int,
                                     _i32p, _vp, _c.c_int, _i64p]),
+    "qt_contract_pair_noalloc": (_c.c_int, [_vp, _vp, _c.c_int, _vp, _c.c_int, _c.c_int, _i32p, _i32p, _c.c_int,
+                                            _i32p, _vp, _c.c_int, _i64p]),
     "qt_tensor_read_scalar": (_c.c_int, [_vp, _vp, _dp, _dp]),
     "qt_tensor_accumulate": (_c.c_int, [_vp, _vp, _vp]),
     "qt_tensor_exponent": (_c.c_int, [_vp, _vp, _c.POINTER(_c.c_int32)]),

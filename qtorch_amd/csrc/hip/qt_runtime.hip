@@ -86,6 +86,32 @@ qt_status check_hip(Ctx* c, hipError_t e, const char* what) {
 using qt::check_hip;
 using qt::set_error;
 
+namespace {
+
+// Scratch of qt_contract_pair_noalloc: one array per device (each device holds its
+// own copy for the life of the process).  In host code `scratch` is only a symbol:
+// its device address on each device comes from hipGetSymbolAddress with that device
+// current, taken once per device when the first context on it is created.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) char scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The calling thread's current device is `device`.
+qt_status take_scratch_address(qt::Ctx* c, int device) {
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (device < 0 || device >= kMaxDevices) return QT_OK;  // the no-allocation entry refuses such devices
+    if (g_scratch_ptr[device]) return QT_OK;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) return check_hip(c, e != hipSuccess ? e : hipErrorInvalidSymbol, "scratch address");
+    g_scratch_ptr[device] = ptr;
+    return QT_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 qt_status qt_ctx_create(int device, qt_ctx** out) {
@@ -103,6 +129,11 @@ qt_status qt_ctx_create(int device, qt_ctx** out) {
     if (hipSetDevice(device) != hipSuccess ||
         hipStreamCreateWithFlags(&c->impl.stream, hipStreamNonBlocking) != hipSuccess) {
         (void)hipGetLastError();
+        delete c;
+        return QT_DEVICE;
+    }
+    if (take_scratch_address(&c->impl, device) != QT_OK) {
+        (void)hipStreamDestroy(c->impl.stream);
         delete c;
         return QT_DEVICE;
     }
@@ -390,9 +421,10 @@ int qt_tensor_rank(const qt_tensor* t) { return t ? t->impl.rank : -1; }
 
 namespace {
 
-// qt_contract_pair: argument checks, the bit map, the launch.  ws != nullptr: the
-// step's scratch comes from the workspace and nothing is allocated (QT_RESCALE, whose
-// exponent table is allocated on first use, is refused then).
+// qt_contract_pair and qt_contract_pair_noalloc: argument checks, the bit map, the
+// launch.  ws != nullptr: the step's scratch comes from the workspace and nothing
+// is allocated (QT_RESCALE, whose exponent table is allocated on first use, is
+// refused there).
 qt_status contract_pair_entry(qt_ctx* c, const qt_tensor* A, int rank_a, const qt_tensor* B, int rank_b, int k,
                               const int32_t* pair_a, const int32_t* pair_b, int rank_c, const int32_t* free_src,
                               qt_tensor* C, int precision, int64_t* cmacs, const qt::Workspace* ws) {
@@ -416,7 +448,7 @@ qt_status contract_pair_entry(qt_ctx* c, const qt_tensor* A, int rank_a, const q
     const bool rescale = (precision & QT_RESCALE) != 0;
     precision &= ~QT_RESCALE;
     if (ws && rescale)
-        return set_error(ctx, QT_INVALID_INPUT, "qt_contract_pair: QT_RESCALE needs the context's allocator");
+        return set_error(ctx, QT_INVALID_INPUT, "qt_contract_pair_noalloc: QT_RESCALE needs qt_contract_pair");
     if ((c128 && (precision != QT_PLAIN_C128 || rescale)) ||
         (!c128 && precision != QT_C64_4M && precision != QT_C64_3M && precision != QT_C64_BF16X3))
         return set_error(ctx, QT_INVALID_INPUT, "qt_contract_pair: precision does not fit dtype");
@@ -470,6 +502,23 @@ qt_status qt_contract_pair(qt_ctx* c, const qt_tensor* A, int rank_a, const qt_t
                            int64_t* cmacs) {
     return contract_pair_entry(c, A, rank_a, B, rank_b, k, pair_a, pair_b, rank_c, free_src, C, precision, cmacs,
                                nullptr);
+}
+
+qt_status qt_contract_pair_noalloc(qt_ctx* c, const qt_tensor* A, int rank_a, const qt_tensor* B, int rank_b, int k,
+                                   const int32_t* pair_a, const int32_t* pair_b, int rank_c, const int32_t* free_src,
+                                   qt_tensor* C, int precision, int64_t* cmacs) {
+    if (!c) return QT_INVALID_INPUT;
+    qt::Workspace ws;
+    {
+        std::lock_guard<std::mutex> g(g_scratch_mu);
+        const int dev = c->impl.device;
+        if (dev < 0 || dev >= kMaxDevices || !g_scratch_ptr[dev])
+            return set_error(&c->impl, QT_DEVICE, "qt_contract_pair_noalloc: no scratch address for the device");
+        ws.base = static_cast<char*>(g_scratch_ptr[dev]);
+    }
+    ws.bytes = kScratchBytes;
+    return contract_pair_entry(c, A, rank_a, B, rank_b, k, pair_a, pair_b, rank_c, free_src, C, precision, cmacs,
+                               &ws);
 }
 
 }  // extern "C"
