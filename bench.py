@@ -6,14 +6,16 @@ Workload: QAOA p=1 MaxCut on a seeded random 3-regular graph with 30 vertices
 light-cone network in qTorch's Liouville form, exactly the networks F_p builds
 (src/maxcut.cpp:162-204, reference qubit map).  One step = one energy evaluation
 of the graph for each of B angle sets (gamma ~ U[0, pi], beta ~ U[0, pi/2], seeded)
-= 45 * B edge networks, inputs resident on the device.
+= 45 * B edge networks (B = 131,072 by default), inputs resident on the device.
 
-value: edge-evals/s of the batched complex64 form (persistent kernel, greedy
-contraction order, HIP-graph replay, autotuned executor shape), with per-edge and
-per-energy error against complex128 (the FP64 oracle's values for the reference's
-angle set come from tests/golden/cases.json).  Beside it: the drop-in complex128
-path (reference stochastic order, one pairwise contraction per launch) on the same
-GPU, and the reference itself on this host's CPU cores.
+value: edge-evals/s of the complex128 specialised executor (generated straight-line
+code per edge program, conjugate symmetry, energies only: each step launches one
+kernel that also sums the previous step's partial energies), every edge within
+1e-12 of complex128 table-executor values and, for the reference's angle set, of the
+values the reference printed (tests/golden/cases.json).  Beside it: the complex64
+forms (4M, 3M, bf16x3) with their per-edge error, the drop-in complex128 path
+(reference stochastic order, one pairwise contraction per launch) on the same GPU,
+and the reference itself on this host's CPU cores.
 
 Prints one JSON line.  --gpus N with N > 1 is not supported (one GPU).
 """
@@ -47,6 +49,8 @@ PMC_BYTES_PER_NETWORK = {
     ("tables", "c64_4m", 8192): ((2.0 * 14451.6 + 6544.0) * 1024 / 368640, "profiles/k_batch_qaoa_p1_3reg_n30.md"),
     ("specialised", "c64_4m", 8192): ((2.0 * 14490.2 + 5824.0) * 1024 / 368640,
                                       "profiles/qt_batch_specialised_qaoa_p1_3reg_n30.md"),
+    ("specialised_sums", "c128", 131072): ((2.0 * 41809.8 + 17408.0) * 1024 / (131072 * 45),
+                                           "profiles/qt_batch_specialised_sums_qaoa_p1_3reg_n30.md"),
 }
 
 
@@ -647,7 +651,8 @@ def main():
         alg_bytes = param_bytes + args.sets * 16 * 8 * 2 + args.sets * 8
     else:
         alg_bytes = param_bytes + n_networks * 16 * (2 if specialised else 1) + args.sets * 8
-    pmc = PMC_BYTES_PER_NETWORK.get(("specialised" if specialised else "tables", args.form, args.sets))
+    pmc = PMC_BYTES_PER_NETWORK.get(("specialised_sums" if sums_only else "specialised" if specialised else "tables",
+                                     args.form, args.sets))
     res = {
         "metric": METRIC,
         "value": value, "unit": "edge-evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,

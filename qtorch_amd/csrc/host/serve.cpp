@@ -167,11 +167,30 @@ std::shared_ptr<Server::Structure> BuildStructure(const ServeRequest& r, const S
                 st->route = scratch <= opt.small_scratch ? kRouteSmall : kRouteMedium;
                 st->planned_bytes = st->param_elems * es + 32;
             } catch (const DeviceFailure&) {
-                st->route = kRouteEager;  // the large-network path, line-graph order, sliced
-                auto sn = std::make_shared<SlicedNetwork>(r.text, zeros, 1, kOrderLineGraph, 0, opt.eager_max_bits);
-                st->planned_bytes = sn->Plan().peak_bytes;
-                st->cmacs = SatMulAdd(0, sn->Plan().cmacs_per_slice, sn->Plan().slices);
-                st->eager_nets.push_back(std::move(sn));
+                // the large-network path: the line-graph order sliced to eager_max_bits, or
+                // the circuit sweep (CZ split into bonded halves) sliced to eager_max_bits
+                // or up to two bits more when the budget holds it, whichever plans the
+                // least work
+                st->route = kRouteEager;
+                std::shared_ptr<SlicedNetwork> best;
+                int64_t best_work = INT64_MAX;
+                auto consider = [&](int order, int max_bits) {
+                    try {
+                        auto sn = std::make_shared<SlicedNetwork>(r.text, zeros, 1, order, 0, max_bits);
+                        const int64_t work = SatMulAdd(0, sn->Plan().cmacs_per_slice, sn->Plan().slices);
+                        if (sn->Plan().peak_bytes <= opt.hbm_budget / 2 && (!best || work < best_work)) {
+                            best = sn;
+                            best_work = work;
+                        }
+                    } catch (const InvalidFunctionInput&) {  // more slices than a plan allows
+                    }
+                };
+                consider(kOrderLineGraph, opt.eager_max_bits);
+                for (int extra = 0; extra <= 2; ++extra) consider(kOrderSweep, opt.eager_max_bits + extra);
+                if (!best) best = std::make_shared<SlicedNetwork>(r.text, zeros, 1, kOrderLineGraph, 0, opt.eager_max_bits);
+                st->planned_bytes = best->Plan().peak_bytes;
+                st->cmacs = SatMulAdd(0, best->Plan().cmacs_per_slice, best->Plan().slices);
+                st->eager_nets.push_back(std::move(best));
             }
         } else {
             throw InvalidFunctionInput();

@@ -240,3 +240,62 @@ def test_large_route_requests_in_flight_together():
             ref += 0.5 * (1.0 - sv.expectation(psi, nq, m).real)
         assert abs(r["value"].real - ref) <= 1e-10 * abs(ref), (r, ref)
     assert 0 < st["peak_planned_bytes"] <= budget
+
+
+def test_configured_mix_at_a_low_rate():
+    """The configured request mix at a low rate: QAOA energies on 3- and 4-regular
+    graphs (p = 1, 2, 3) and amplitudes of grid circuits from 4x6 depth 8 to the
+    48-qubit 6x8 depth 14, submitted with Poisson arrivals.  Every request either
+    completes with the value computed on its own (QAOA: the drop-in complex128 path,
+    1e-5; amplitudes: the state vector to 24 qubits, else the same plan contracted
+    directly, 1e-10) or is refused for a stated reason (planned work or memory);
+    the 48-qubit depth-14 amplitude is served (circuit sweep plan); per-class p99
+    latencies come back."""
+    rng = random.Random(17)
+    ctx = qa.Context(0)
+    graphs = [(wl.dgf_text(wl.regular_graph(n, d, seed=500 + n)), d) for n, d in ((20, 3), (24, 4), (30, 3))]
+    circuits = [(wl.grid_random_circuit(r, c, dep, seed=600 + r * c + dep), r * c, f"{r}x{c} d{dep}")
+                for r, c, dep in ((4, 6, 8), (4, 6, 14), (6, 8, 14))]
+    reqs = []
+    for k in range(24):
+        if k % 3:
+            dgf, d = graphs[k % len(graphs)]
+            p = (1, 2, 1, 2, 3)[k % 5]
+            reqs.append(("qaoa", dgf, p, wl.qaoa_angles(p, 40 + k), f"qaoa {d}-regular p={p}"))
+        else:
+            q, n, label = circuits[(k // 3) % len(circuits)]
+            reqs.append(("amp", q, wl.bitstring(n, 70 + k), n, label))
+    srv = qa.Server(slots=2, precision=qa.QT_C64_4M, hbm_budget=64 << 30)
+    try:
+        ids = []
+        for r in reqs:
+            time.sleep(rng.expovariate(200.0))
+            ids.append(srv.submit_qaoa(r[1], r[2], r[3]) if r[0] == "qaoa" else srv.submit_amplitude(r[1], r[2]))
+        res = [srv.wait(i) for i in ids]
+        st = srv.stats()
+    finally:
+        srv.close()
+    lat = {}
+    served_48 = 0
+    for r, out in zip(reqs, res):
+        if out["status"] != 0:
+            assert r[0] == "qaoa" and r[2] == 3, (r[4], out)  # only p = 3 exceeds the per-request work limit
+            continue
+        lat.setdefault(r[4], []).append(out["latency_ms"])
+        if r[0] == "qaoa":
+            ref, _ = qa.qaoa_energy_plain(ctx, r[1], r[2], False, list(r[3]), seed=3)
+            assert abs(out["value"].real - ref) <= 1e-5 * abs(ref), (r[4], out, ref)
+        elif r[3] <= 24:
+            psi, _ = sv.simulate(r[1])
+            ref = psi[_amp_index(r[2])]
+            assert abs(out["value"] - ref) <= 1e-10 * max(abs(ref), 2.0 ** (-r[3] / 2)), (r[4], out, ref)
+        else:
+            served_48 += 1
+            sn = qa.SlicedNetwork(r[1], r[2], dim_log2=1, order=qa.ORDER_SWEEP, seed=0, max_bits=29)
+            ref, _ = sn.run(ctx, qa.QT_PLAIN_C128)
+            sn.close()
+            assert abs(out["value"] - ref) <= 1e-10 * abs(ref), (r[4], out, ref)
+    assert served_48 >= 1
+    p99 = {k: float(np.percentile(v, 99)) for k, v in lat.items()}
+    assert all(v > 0 for v in p99.values()) and st["completed"] == len(reqs)
+    ctx.close()

@@ -234,9 +234,10 @@ def test_program_schedule_keeps_scratch_small():
 def test_serving_plan_routes_and_bytes():
     """Serving plans each request on the host: small light cones run in lane-grouped
     waves, larger programs in workgroup executors, networks with tensors beyond a
-    program's 2^16 elements on the large-network path (line-graph order, sliced),
-    whose planned bytes cover at least its largest intermediate (complex128) and whose
-    MAC count is the plan's."""
+    program's 2^16 elements on the large-network path, whose plan is the one with the
+    least work among the line-graph order and the circuit sweep (sliced to 28 to 30
+    bits), whose planned bytes cover at least that plan's largest intermediate
+    (complex128) and whose MAC count is that plan's."""
     from qtorch_amd import workloads as wl
     with open(os.path.join(GOLDEN_DIR, "graphs", "3reg_n30_s2024.dgf")) as f:
         dgf = f.read()
@@ -249,23 +250,34 @@ def test_serving_plan_routes_and_bytes():
     big = qa.serve_plan(qa.SERVE_AMPLITUDE, q, bits=bits)
     net = qa.Network(q, bits, None, dim_log2=1)
     net.order(qa.ORDER_LINEGRAPH, 0)
-    peak_rank = int(net.plan()[:, 6].max())
-    assert peak_rank > 16 and big["route"] == "eager"
-    assert big["planned_bytes"] >= 16 * 2 ** peak_rank and big["cmacs"] == net.flops()
+    assert int(net.plan()[:, 6].max()) > 16 and big["route"] == "eager"
+    plans = [qa.SlicedNetwork(q, bits, dim_log2=1, order=qa.ORDER_LINEGRAPH, max_bits=28).plan()]
+    plans += [qa.SlicedNetwork(q, bits, dim_log2=1, order=qa.ORDER_SWEEP, max_bits=mb).plan() for mb in (28, 29, 30)]
+    work = [pl["cmacs_per_slice"] * pl["slices"] for pl in plans]
+    chosen = plans[work.index(min(work))]
+    assert big["cmacs"] == min(work) <= net.flops()
+    assert big["planned_bytes"] >= 16 * 2 ** chosen["peak_bits"]
     with pytest.raises(qa.InvalidFunctionInput):
         qa.serve_plan(7, dgf, 1)
     with pytest.raises(qa.InvalidFunctionInput):
         qa.serve_plan(qa.SERVE_AMPLITUDE, wl.qft_qasm(4), bits="01x1")
 
 
-def test_serving_planned_work_saturates():
-    """The planned work of a request too large to run (48 qubits at depth 14: slices
-    times MACs per slice is beyond 2^63) saturates at 2^63 - 1 instead of wrapping
-    to a negative count, which the per-request work limit would then admit."""
+def test_serving_plans_the_48_qubit_amplitude_by_sweep():
+    """48 qubits at depth 14: in the line-graph order sliced to 28 bits, slices times
+    MACs per slice is beyond 2^63 (the planner's count saturates rather than wrapping
+    to a negative number the work limit would admit); the serving planner takes the
+    circuit sweep instead (CZ split, sliced to fit half the default 16 GiB budget),
+    whose work is positive and within the per-request limit of 2^44 complex MACs."""
     from qtorch_amd import workloads as wl
     q = wl.grid_random_circuit(6, 8, 14, seed=605)
-    plan = qa.serve_plan(qa.SERVE_AMPLITUDE, q, bits=wl.bitstring(48, 7))
-    assert plan["route"] == "eager" and plan["cmacs"] == 2 ** 63 - 1
+    bits = wl.bitstring(48, 7)
+    lg = qa.SlicedNetwork(q, bits, dim_log2=1, order=qa.ORDER_LINEGRAPH, max_bits=28).plan()
+    assert lg["cmacs_per_slice"] * lg["slices"] > 2 ** 63
+    plan = qa.serve_plan(qa.SERVE_AMPLITUDE, q, bits=bits)
+    sweep = qa.SlicedNetwork(q, bits, dim_log2=1, order=qa.ORDER_SWEEP, max_bits=28).plan()
+    assert plan["route"] == "eager" and 0 < plan["cmacs"] <= 2 ** 44
+    assert plan["cmacs"] == sweep["cmacs_per_slice"] * sweep["slices"]
 
 
 def test_nonzero_slices_listing():
