@@ -530,23 +530,62 @@ def main():
     b.flush()
     ctx.sync()
     ne = b.n_edges
-    # per-step latency (launch + completion with its energy sums), then the throughput
-    # run; with direct launches of the specialised kernel each launch also sums the
-    # previous launch's energies, and the last launch's sums run at the flush
+    # an optimiser supplies new angles for every evaluation: each step first rewrites
+    # the sets' Rz / Rx blocks on the device from a fresh array of angles already in
+    # HBM (qth_qaoa_batch_update_angles), then launches.  The arrays are distinct per
+    # step; the latency run cycles through them.
+    import torch
+    fresh_host = np.stack([angle_sets(args.sets, seed=10_000 + i) for i in range(args.steps)])
+    fresh = torch.from_numpy(fresh_host).to("cuda:0")
+    torch.cuda.synchronize()
+    ptrs = [fresh[i].data_ptr() for i in range(args.steps)]
+    b.update_angles(device_ptr=ptrs[0], n_sets=args.sets)  # the first call checks the generated kernel's literals
+    b.launch()
+    b.flush()
+    ctx.sync()
+    # per-step latency (new angles, launch, completion with its energy sums), then the
+    # throughput run; with direct launches of the specialised kernel each launch also
+    # sums the previous launch's energies, and the last launch's sums run at the flush
     lat = []
-    for _ in range(args.latency_steps):
+    for i in range(args.latency_steps):
         t0 = time.perf_counter()
+        b.update_angles(device_ptr=ptrs[i % args.steps], n_sets=args.sets)
         b.launch()
         b.flush()
         ctx.sync()
         lat.append(time.perf_counter() - t0)
     ctx.sync()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for i in range(args.steps):
+        b.update_angles(device_ptr=ptrs[i], n_sets=args.sets)
         b.launch()
     b.flush()
     ctx.sync()
     wall = time.perf_counter() - t0
+    e_fresh, _ = b.results()  # the last step's angles
+    # the same steps with the angles left as they are (the same sets every step)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        b.launch()
+    b.flush()
+    ctx.sync()
+    wall_fixed = time.perf_counter() - t0
+    # the parameter kernel alone: updates queued back to back
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        b.update_angles(device_ptr=ptrs[i], n_sets=args.sets)
+    ctx.sync()
+    update_ms = 1e3 * (time.perf_counter() - t0) / args.steps
+    # the last fresh step's energies against networks rebuilt from the gates' text on
+    # the host (complex128 table executor)
+    bchk = qa.QaoaBatch(ctx, dgf, 1, ref_map=True, precision=qa.QT_PLAIN_C128, order=qa.ORDER_GREEDY)
+    bchk.set_angles(fresh_host[-1])
+    bchk.launch()
+    e_chk, _ = bchk.results()
+    bchk.close()
+    fresh_err = float(np.max(np.abs(e_fresh - e_chk) / np.abs(e_chk)))
+    b.update_angles(sets)  # back to the sets the accuracy checks below use
+    del fresh
     launch_ms = b.time(args.steps)  # HIP events on the engine's stream around whole launches
     kernel_ms = b.time_kernel(args.steps)  # the same for the contraction kernel alone
     energies, _ = b.results()
@@ -563,6 +602,7 @@ def main():
     energies = energies_v
     info = b.info()
     value = ne * args.sets * args.steps / wall
+    value_fixed = ne * args.sets * args.steps / wall_fixed
 
     # ---- accuracy against complex128 on the same plan, and against the reference ----
     b64 = qa.QaoaBatch(ctx, dgf, 1, ref_map=True, precision=qa.QT_PLAIN_C128, order=qa.ORDER_GREEDY)
@@ -669,6 +709,13 @@ def main():
         "compile_ms": compile_ms, "compile_error": compile_error, "table_executor_ms_per_launch": tables_ms,
         "hip_graph": graph,
         "launch_ms_direct": direct_ms, "launch_ms_graph": graph_ms,
+        "angles": "fresh every step: Rz / Rx blocks rebuilt on the device from a new array of angle sets "
+                  "in HBM before each launch (one array per step)",
+        "fresh_angles": {"param_update_ms": update_ms, "max_energy_rel_err_vs_host_rebuild_c128": fresh_err,
+                         "compared": f"the last step's {args.sets} energies against networks rebuilt from the gates' "
+                                     "text on the host (complex128 table executor)"},
+        "fixed_angles": {"value": value_fixed, "ms_per_step": 1e3 * wall_fixed / args.steps,
+                         "note": "the same angle sets every step (no parameter update)"},
         "latency_steps": len(lat),
         "p50_step_ms": 1e3 * float(np.percentile(lat, 50)), "p99_step_ms": 1e3 * float(np.percentile(lat, 99)),
         "accuracy": {"max_edge_rel_err_vs_c128": edge_err, "edges_within_1e-5": within,

@@ -240,6 +240,44 @@ qt_status qt_batch_launch(qt_batch* b);
 int64_t qt_batch_resident_bytes(const qt_batch* b);
 qt_status qt_batch_set_sums_only(qt_batch* b, int on, int* active);
 qt_status qt_batch_poll(qt_batch* b, int* done);
+
+/* ---- rotation gates whose angles change between launches ------------------------- */
+
+/* Gate kinds of qt_batch_rotation_params (Rz also stands for PHASE: same tables). */
+typedef enum { QT_ROT_RX = 0, QT_ROT_RY = 1, QT_ROT_RZ = 2 } qt_rotation_kind;
+
+/* One rotation gate of an angle set's parameter region. */
+typedef struct qt_rotation {
+    int32_t kind;   /* qt_rotation_kind */
+    int32_t angle;  /* index of the gate's angle in a set's row of angles */
+    double scale;   /* the gate's angle is scale * row[angle] (-1 for Rz(-gamma), 2 for Rx(2 beta)) */
+    int64_t offset; /* element offset of the gate's block in the set's parameter region */
+} qt_rotation;
+
+/* Writes the blocks of rotation gates whose angles come from an array, on the device:
+ * for every set a < n_sets and gate r < n_rot, the block of gate r with angle
+ * scale * angles[a * n_angles + angle] at pool element base + a * set_stride +
+ * rots[r].offset (16 elements: the superoperator over (in, out), dim_log2 = 2; or 4: the
+ * amplitude gate, dim_log2 = 1).  The values are those of a network built from the
+ * gate's text: the angle printed at the default ostream precision and parsed with
+ * std::stof (src/maxcut.h:211,217; src/Network.h:342), then the gate table in FP64
+ * (src/Node.h:197-898) -- the round trip is exact, the cos / sin values are the
+ * device's (within a few units in the last place of the host's).  It replaces
+ * rebuilding every light-cone network from text for each evaluation of new angles.
+ * angles: n_sets x n_angles doubles in device memory (angles_on_host = 0: read in
+ * stream order on the context's stream; writes to them from other streams must have
+ * completed) or in host memory (angles_on_host = 1: copied first).  Asynchronous.
+ * Blocks must lie inside the pool and not overlap (set_stride at least the extent
+ * of a set's blocks); at most 32 gates per set.  An angle that is not 0 and whose
+ * magnitude lies outside [1e-16, 1e15), or that is not finite, gives NaN values and
+ * makes the next qt_batch_results return QT_INVALID_INPUT.  A specialised executor
+ * whose literals the new blocks could change is dropped (the table executor runs). */
+qt_status qt_batch_rotation_params(qt_batch* b, int dim_log2, int n_rot, const qt_rotation* rots, int n_sets,
+                                   int n_angles, const double* angles, int angles_on_host, int64_t base,
+                                   int64_t set_stride);
+/* The float that the text round trip above gives x (the host instantiation of the
+ * routine the device runs): QT_INVALID_INPUT outside its range. */
+qt_status qt_angle_text_float(double x, float* out);
 /* Captures the launch sequence (counter reset, persistent kernel, group sums) into a
  * HIP graph; later launches replay it. */
 qt_status qt_batch_capture(qt_batch* b);

@@ -142,6 +142,14 @@ int qth_qaoa_batch_create(qt_ctx* ctx, const char* dgf, int p, int ref_map, int 
                           uint64_t seed, qth_qaoa_batch** out);
 /* bg: n_sets x 2p doubles, each set [betas..., gammas...] */
 int qth_qaoa_batch_set_angles(qth_qaoa_batch* b, int n_sets, const double* bg);
+/* New angles for the same number of sets, without rebuilding anything on the host:
+ * the Rz(-gamma) / Rx(2 beta) blocks are written on the device from bg
+ * (qt_batch_rotation_params), with the values the gates' text would give.  bg:
+ * n_sets x 2p doubles as above, in host memory (on_host = 1) or device memory
+ * (on_host = 0; read in order on the context's stream).  Asynchronous.  It replaces
+ * the reference's rebuild of every edge network from text for each evaluation
+ * (src/maxcut.cpp:162-204, src/maxcut.h:194-233). */
+int qth_qaoa_batch_update_angles(qth_qaoa_batch* b, int n_sets, const double* bg, int on_host);
 /* energies: n_sets; zz: n_sets x n_edges values of <Z Z> (either may be NULL); synchronises */
 int qth_qaoa_batch_results(qth_qaoa_batch* b, double* energies, double* zz);
 qt_batch* qth_qaoa_batch_device(qth_qaoa_batch* b);

@@ -12,7 +12,7 @@ from ._native import (  # noqa: F401
     InvalidFileFormat, InvalidFunctionInput, InvalidUserContractionSequence, Network, QbbFailure, Tensor,
     contract_pair, declared_symbols, maxcut_final_string, qaoa_optimize, qaoa_edge_text, qaoa_energy_plain, qaoa_num_edges,
     qaoa_param_block, qaoa_source, program_compile, set_max_time, get_max_time, timer_start, timer_reset,
-    timer_elapsed, preprocess,
+    timer_elapsed, preprocess, angle_text_float,
     ROUTE_EAGER, ROUTE_MEDIUM, ROUTE_SMALL, SERVE_AMPLITUDE, SERVE_QAOA_ENERGY, Server, serve_plan,
 )
 

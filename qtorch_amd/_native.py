@@ -71,6 +71,9 @@ _HIP_SIGS = {
     "qt_batch_write_leaves": (_c.c_int, [_vp, _c.c_int64, _dp, _c.c_int64]),
     "qt_batch_poll": (_c.c_int, [_vp, _c.POINTER(_c.c_int)]),
     "qt_batch_set_sums_only": (_c.c_int, [_vp, _c.c_int, _c.POINTER(_c.c_int)]),
+    "qt_batch_rotation_params": (_c.c_int, [_vp, _c.c_int, _c.c_int, _vp, _c.c_int, _c.c_int, _vp, _c.c_int,
+                                            _c.c_int64, _c.c_int64]),
+    "qt_angle_text_float": (_c.c_int, [_c.c_double, _c.POINTER(_c.c_float)]),
     "qt_program_create": (_c.c_int, [_vp, _c.c_int, _c.c_int, _i32p, _i64p, _c.c_int, _i32p, _i32p,
                                      _c.POINTER(_vp)]),
     "qt_program_stats": (_c.c_int, [_vp, _i64p, _i64p, _c.POINTER(_c.c_int)]),
@@ -137,6 +140,7 @@ _HOST_SIGS = {
                                          _c.POINTER(_vp)]),
     "qth_qaoa_batch_set_angles": (_c.c_int, [_vp, _c.c_int, _dp]),
     "qth_qaoa_batch_results": (_c.c_int, [_vp, _dp, _dp]),
+    "qth_qaoa_batch_update_angles": (_c.c_int, [_vp, _c.c_int, _vp, _c.c_int]),
     "qth_qaoa_batch_device": (_vp, [_vp]),
     "qth_qaoa_batch_cmacs": (_c.c_int64, [_vp]),
     "qth_qaoa_batch_destroy": (_c.c_int, [_vp]),
@@ -689,6 +693,16 @@ class QaoaBatch(_DeviceBatch):
         _host_check(host.qth_qaoa_batch_set_angles(self.handle, bg.shape[0], bg.ctypes.data_as(_dp)))
         self.n_sets = bg.shape[0]
 
+    def update_angles(self, bg=None, device_ptr: int = 0, n_sets: int = 0) -> None:
+        """New angles for the current sets, written on the device (qth_qaoa_batch_update_angles):
+        from a host array ``bg`` (n_sets x 2p), or from ``device_ptr``, the address of n_sets x 2p
+        doubles in device memory (e.g. a torch tensor's data_ptr(), its writes completed)."""
+        if bg is not None:
+            bg = np.ascontiguousarray(bg, dtype=np.float64).reshape(-1, 2 * self.p)
+            _host_check(host.qth_qaoa_batch_update_angles(self.handle, bg.shape[0], bg.ctypes.data, 1))
+        else:
+            _host_check(host.qth_qaoa_batch_update_angles(self.handle, n_sets, _c.c_void_p(device_ptr), 0))
+
     def results(self):
         """(energies, per-edge <ZZ>); the second is None after group-sums-only launches."""
         e = np.zeros(self.n_sets)
@@ -737,6 +751,16 @@ def program_compile(source: str, arch: str = "gfx950") -> bytes:
     if rc != QT_OK:
         raise DeviceFailure(f"qt_program_compile: status {rc}")
     return buf.raw[:n.value]
+
+
+def angle_text_float(x: float) -> float:
+    """qt_angle_text_float: the float that printing x at the default ostream precision
+    and parsing it with std::stof gives (the routine the device runs for rotation blocks)."""
+    f = ctypes.c_float(0.0)
+    st = hip.qt_angle_text_float(ctypes.c_double(x), ctypes.byref(f))
+    if st != 0:
+        raise InvalidFunctionInput(f"angle {x!r} outside the supported range")
+    return f.value
 
 
 def qaoa_param_block(p: int, betas_gammas: Sequence[float]) -> np.ndarray:

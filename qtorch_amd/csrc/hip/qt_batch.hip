@@ -690,6 +690,7 @@ qt_status write_pool(qt_batch* b, int64_t offset, const double* values, int64_t 
     Ctx* c = b->ctx;
     if (b->specialised) {
         bool keep = !specialised_consts_change(b, offset, values, n);  // its literals are the constants
+        if (keep && b->mirror_stale) keep = refresh_pool_mirror(b) == QT_OK;
         if (keep) {
             // the host mirror follows the pool; the parameter blocks must keep what the
             // generated code assumes of them
@@ -841,6 +842,10 @@ qt_status launch_group_sums(qt_batch* b) {
         s = check_hip(c, hipGetLastError(), "k_group_sum");
     }
     return s;
+}
+
+qt_status after_specialised_drop(qt_batch* b) {
+    return b->graph_exec ? configure_scratch(b) : QT_OK;  // the captured graph runs the dropped kernel
 }
 
 qt_status flush_sums(qt_batch* b) {
@@ -1155,8 +1160,10 @@ qt_status qt_batch_set_instances(qt_batch* b, int n, const int32_t* prog, const 
     b->cmacs_total = cmacs;
     b->c0 = c0;
     b->c1 = c1;
+    ++b->layout_gen;
     if (b->specialised) {
         s = qt::specialised_chunks(b);
+        if (s == QT_OK && b->mirror_stale) s = qt::refresh_pool_mirror(b);
         if (s != QT_OK || !qt::specialised_params_hold(b)) qt::drop_specialised(b);  // the table executor runs
         s = QT_OK;
     }
@@ -1335,6 +1342,17 @@ qt_status qt_batch_results(qt_batch* b, double* values, double* group_sums) {
         return set_error(ctx, QT_INVALID_INPUT, "qt_batch_results: group-sums-only launches keep no per-instance values");
     qt_status s = qt::flush_sums(b);
     if (s == QT_OK) s = check_hip(ctx, hipStreamSynchronize(ctx->stream), "batch sync");
+    if (s == QT_OK && b->params_unchecked) {
+        int bad = 0;
+        s = check_hip(ctx, hipMemcpy(&bad, b->d_param_status, sizeof bad, hipMemcpyDeviceToHost), "status");
+        b->params_unchecked = false;
+        if (s == QT_OK && bad) {
+            s = qt::launch_fill_zero(ctx, b->d_param_status, 256);
+            return set_error(ctx, QT_INVALID_INPUT,
+                             "qt_batch_results: an angle given to qt_batch_rotation_params is outside the "
+                             "supported range (0, or magnitude in [1e-16, 1e15))");
+        }
+    }
     if (s == QT_OK && values && b->n_inst > 0)
         s = check_hip(ctx, hipMemcpy(values, b->d_out, size_t(b->n_inst) * 16, hipMemcpyDeviceToHost), "results");
     if (s == QT_OK && group_sums && b->n_groups > 0)
@@ -1452,7 +1470,7 @@ qt_status qt_batch_destroy(qt_batch* b) {
                         (void*)b->d_inst, (void*)b->d_out, (void*)b->d_counter, (void*)b->d_seg, (void*)b->d_members,
                         (void*)b->d_group, b->d_global_scratch, (void*)b->d_chunks, b->d_pool_stage,
                         (void*)b->d_leaf_lds, (void*)b->d_csts, (void*)b->d_tout, (void*)b->d_schunks,
-                        (void*)b->d_out_prev})
+                        (void*)b->d_out_prev, (void*)b->d_param_status, (void*)b->d_angles})
             qt::batch_free(b->ctx, p);
         --b->ctx->live;
     }

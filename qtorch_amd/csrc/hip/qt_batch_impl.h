@@ -167,6 +167,15 @@ struct qt_batch {
     bool spec_sums_ok = false;              // the instances fit the generated sums kernel
     bool pending_parts = false;             // the pending sums are over partial sums (d_out)
     bool values_valid = true;               // d_out holds the last launch's per-instance values
+    // rotation blocks written on the device (qt_batch_rotation_params)
+    bool mirror_stale = false;        // h_pool_mirror misses device-written blocks
+    int* d_param_status = nullptr;    // set by the kernel for an angle out of range
+    bool params_unchecked = false;    // a rotation launch since the status was last read
+    double* d_angles = nullptr;       // host-given angles, copied
+    size_t cap_angles = 0;
+    uint64_t layout_gen = 1;          // changes with the instances or the generated kernel
+    uint64_t rot_checked_gen = 0;     // layout_gen and arguments the literal check last passed for
+    std::vector<char> rot_checked_args;
 };
 
 namespace qt {
@@ -206,4 +215,8 @@ bool specialised_consts_change(const qt_batch* b, int64_t offset, const double* 
 // the generated kernel's assumptions on parameter blocks hold for the current
 // instances and the pool's host mirror
 bool specialised_params_hold(const qt_batch* b);
+// re-reads h_pool_mirror from the device after device-side parameter writes (qt_params.hip)
+qt_status refresh_pool_mirror(qt_batch* b);
+// a dropped specialised executor: rebuilds the captured graph's launch state (qt_batch.hip)
+qt_status after_specialised_drop(qt_batch* b);
 }  // namespace qt

@@ -673,6 +673,8 @@ void drop_specialised(qt_batch* b) {
     b->spec_consts.clear();
     b->spec_param_lits.clear();
     std::vector<double>().swap(b->h_pool_mirror);
+    b->mirror_stale = false;
+    ++b->layout_gen;
 }
 
 bool specialised_consts_change(const qt_batch* b, int64_t offset, const double* values, int64_t n) {
@@ -955,6 +957,8 @@ qt_status qt_batch_compile(qt_batch* b, int flags, float* compile_ms) {
     qt::drop_specialised(b);
     b->spec_param_lits = std::move(new_lits);
     b->h_pool_mirror = pool;
+    b->mirror_stale = false;
+    ++b->layout_gen;
     b->module = mod;
     b->kernel = fn;
     b->kernel_sums = fn_sums;

@@ -260,6 +260,55 @@ __global__ __launch_bounds__(kThreads) void k_skinny(const V* __restrict__ A, co
     }
 }
 
+// k_skinny's steps when the large side's free bits keep their order in C (a gate on
+// a large tensor, as in a circuit sweep) and at most 4 bits are summed: lane t's
+// element of the large operand X sits at t with zero bits inserted at X's summed
+// positions (a few shifts instead of a walk over every bit of C), the small operand's
+// elements are uniform (scalar loads from the kernel's arguments' offsets), and each
+// lane issues its 2^NS loads of X before any arithmetic.  Same summation order and
+// arithmetic as k_skinny: bit-identical to it.
+struct ThinArgs {
+    int64_t n_big;
+    int32_t lo_bits;        // small B: output o of lane t is C[t + (o << lo_bits)]
+    int32_t n_ins;          // X's summed positions, ascending
+    int32_t ins_pos[4];
+    int64_t g_big[16];      // X offset of summed index j (k_direct's digit order)
+    int32_t g_small[16];    // small operand offset of summed index j
+    int32_t small_off[16];  // small operand offset of its free part o
+};
+
+template <class V, int MODE, int NS, int NB, bool SMALL_B>
+__global__ __launch_bounds__(kThreads) void k_thin(const V* __restrict__ X, const V* __restrict__ G,
+                                                   V* __restrict__ C, ThinArgs r) {
+    constexpr int NJ = 1 << NS, NO = 1 << NB;
+    const int64_t stride = int64_t(gridDim.x) * kThreads;
+    for (int64_t t = int64_t(blockIdx.x) * kThreads + threadIdx.x; t < r.n_big; t += stride) {
+        int64_t x = t;
+#pragma unroll
+        for (int i = 0; i < NS; ++i) {
+            const int p = r.ins_pos[i];
+            x = ((x >> p) << (p + 1)) | (x & ((int64_t(1) << p) - 1));
+        }
+        V xs[NJ];
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) xs[j] = X[x + r.g_big[j]];
+        Acc<V, MODE> acc[NO];
+#pragma unroll
+        for (int j = 0; j < NJ; ++j)
+#pragma unroll
+            for (int o = 0; o < NO; ++o) {
+                const V g = G[r.small_off[o] + r.g_small[j]];
+                if constexpr (SMALL_B) mac<V, MODE>(acc[o], xs[j], g);
+                else mac<V, MODE>(acc[o], g, xs[j]);
+            }
+#pragma unroll
+        for (int o = 0; o < NO; ++o) {
+            const int64_t idx = SMALL_B ? t + (int64_t(o) << r.lo_bits) : int64_t(o) + (t << NB);
+            C[idx] = finish<V, MODE>(acc[o]);
+        }
+    }
+}
+
 // Split reduction for outputs of at most kReduceMaxOut elements.  Block b sums the
 // summed-index chunk [b*chunk, (b+1)*chunk) for every output and writes
 // partials[b * nC + c]; k_reduce_final adds the partials in block order.
@@ -708,11 +757,92 @@ qt_status launch_skinny(Ctx* c, const V* A, const V* B, V* C, const BitMap& m, i
     return check_hip(c, hipGetLastError(), "k_skinny");
 }
 
+template <class V, int MODE, int NS, int NB>
+qt_status launch_thin_nb(Ctx* c, const V* X, const V* G, V* C, const ThinArgs& r, bool small_b) {
+    const unsigned grid = grid_for(r.n_big, kThreads, 32768);
+    if (small_b)
+        hipLaunchKernelGGL((k_thin<V, MODE, NS, NB, true>), dim3(grid), dim3(kThreads), 0, c->stream, X, G, C, r);
+    else
+        hipLaunchKernelGGL((k_thin<V, MODE, NS, NB, false>), dim3(grid), dim3(kThreads), 0, c->stream, X, G, C, r);
+    return check_hip(c, hipGetLastError(), "k_thin");
+}
+
+template <class V, int MODE, int NS>
+qt_status launch_thin_ns(Ctx* c, const V* X, const V* G, V* C, const ThinArgs& r, int nb, bool small_b) {
+    switch (nb) {
+        case 0: return launch_thin_nb<V, MODE, NS, 0>(c, X, G, C, r, small_b);
+        case 1: return launch_thin_nb<V, MODE, NS, 1>(c, X, G, C, r, small_b);
+        case 2: return launch_thin_nb<V, MODE, NS, 2>(c, X, G, C, r, small_b);
+        case 3: return launch_thin_nb<V, MODE, NS, 3>(c, X, G, C, r, small_b);
+        default: return launch_thin_nb<V, MODE, NS, 4>(c, X, G, C, r, small_b);
+    }
+}
+
+// k_thin when it applies (plain and 4M forms, 1-4 summed bits, at most 4 bits on the
+// small side, the large side's free bits in their own order in C); false otherwise.
+template <class V, int MODE>
+bool try_thin(Ctx* c, const V* A, const V* B, V* C, const BitMap& m, int mbits, bool small_b, qt_status* st) {
+    if constexpr (MODE != MODE_PLAIN && MODE != MODE_4M) {
+        return false;
+    } else {
+        const int nb = small_b ? m.nbits_c - mbits : mbits;
+        const int ns = m.nbits_s;
+        if (ns < 1 || ns > 4 || nb > 4) return false;
+        // the large side's C bits, in C order, must be its free bits in increasing order
+        const int lo = small_b ? 0 : nb, hi = small_b ? mbits : m.nbits_c;
+        int prev = -1;
+        for (int i = lo; i < hi; ++i) {
+            const int src = small_b ? m.c_src[i] : m.c_src[i] - 64;
+            if (src <= prev) return false;
+            prev = src;
+        }
+        ThinArgs r;
+        std::memset(&r, 0, sizeof r);
+        r.n_big = int64_t(1) << (hi - lo);
+        r.lo_bits = mbits;
+        int pos[4];
+        for (int d = 0; d < ns; ++d) pos[d] = small_b ? m.s_a[d] : m.s_b[d];
+        std::sort(pos, pos + ns);
+        r.n_ins = ns;
+        for (int d = 0; d < ns; ++d) r.ins_pos[d] = pos[d];
+        for (int j = 0; j < (1 << ns); ++j) {
+            int64_t gb = 0, gs = 0;
+            for (int d = 0; d < ns; ++d)
+                if ((j >> d) & 1) {
+                    gb += int64_t(1) << (small_b ? m.s_a[d] : m.s_b[d]);
+                    gs += int64_t(1) << (small_b ? m.s_b[d] : m.s_a[d]);
+                }
+            r.g_big[j] = gb;
+            r.g_small[j] = int32_t(gs);
+        }
+        for (int o = 0; o < (1 << nb); ++o) {
+            int64_t off = 0;
+            for (int i = 0; i < nb; ++i)
+                if ((o >> i) & 1) {
+                    const int src = small_b ? m.c_src[mbits + i] - 64 : m.c_src[i];
+                    off += int64_t(1) << src;
+                }
+            r.small_off[o] = int32_t(off);
+        }
+        const V* X = small_b ? A : B;
+        const V* G = small_b ? B : A;
+        switch (ns) {
+            case 1: *st = launch_thin_ns<V, MODE, 1>(c, X, G, C, r, nb, small_b); break;
+            case 2: *st = launch_thin_ns<V, MODE, 2>(c, X, G, C, r, nb, small_b); break;
+            case 3: *st = launch_thin_ns<V, MODE, 3>(c, X, G, C, r, nb, small_b); break;
+            default: *st = launch_thin_ns<V, MODE, 4>(c, X, G, C, r, nb, small_b); break;
+        }
+        return true;
+    }
+}
+
 // C = [A part (mbits) | B part]; the side with at most 5 bits is the small one
 template <class V, int MODE>
 qt_status run_skinny(Ctx* c, const V* A, const V* B, V* C, const BitMap& m, int mbits) {
     const int nbits_n = m.nbits_c - mbits;
     const bool small_b = nbits_n <= 5;
+    qt_status st = QT_OK;
+    if (try_thin<V, MODE>(c, A, B, C, m, mbits, small_b, &st)) return st;
     switch (small_b ? nbits_n : mbits) {
         case 0: return launch_skinny<V, MODE, 0>(c, A, B, C, m, mbits, small_b);
         case 1: return launch_skinny<V, MODE, 1>(c, A, B, C, m, mbits, small_b);

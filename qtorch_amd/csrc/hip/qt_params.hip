@@ -1,0 +1,482 @@
+// Rotation-gate parameter blocks written on the device from angles
+// (qt_batch_rotation_params).
+//
+// An optimiser's outer loop changes only the angles of Rx / Ry / Rz gates between
+// evaluations.  The reference rebuilds every light-cone network from text for each
+// evaluation: the angle is printed at the default ostream precision (6 significant
+// digits; src/maxcut.h:211,217), parsed back with std::stof (src/Network.h:342), and
+// the gate's superoperator is built from that float in FP64 (src/Node.h:197-898).
+// Here one thread per (angle set, gate) does the same arithmetic on the device and
+// writes the gate's block into the batch's leaf pool, so new angles cost a few
+// microseconds instead of a host rebuild and a pool upload.
+//
+// The text round trip is exact: angle_text_float() gives the float that strtof
+// returns for the %g text of x, bit for bit, without producing text (exact products
+// and remainders with FMA; argued inline).  The trigonometric values come from the
+// device's cos / sin in FP64, so a block element may differ from the host's glibc
+// value in the last bit or two.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "qt_batch_impl.h"
+
+namespace qt {
+
+namespace {
+
+// 10^k for 0 <= k <= 22: every such power is a double, and each product is exact.
+__host__ __device__ inline double pow10_exact(int k) {
+    double p = 1.0;
+    for (int i = 0; i < k; ++i) p *= 10.0;
+    return p;
+}
+
+// Sign of (hi + lo) - m, where hi + lo is an exact double-double (|lo| <= ulp(hi)/2).
+// hi - m is exact when m/2 <= hi <= 2m (Sterbenz); otherwise |hi - m| dwarfs |lo| and
+// its rounding keeps the sign.  Rounding of d + lo never changes a sign.
+__host__ __device__ inline int sign_dd_minus(double hi, double lo, double m) {
+    const double r = (hi - m) + lo;
+    return (r > 0) - (r < 0);
+}
+
+__host__ __device__ inline float next_float(float f, bool up) {  // f > 0, finite
+    uint32_t u;
+    __builtin_memcpy(&u, &f, 4);
+    u = up ? u + 1 : u - 1;
+    float g;
+    __builtin_memcpy(&g, &u, 4);
+    return g;
+}
+
+}  // namespace
+
+// The float std::stof returns for the text that `std::ostream << x` writes at the
+// default precision, i.e. for x rounded to 6 significant decimal digits (%g: ties
+// to even on the exact binary value), then rounded to the nearest float (ties to
+// even).  Exact for x == 0 and 1e-16 <= |x| < 1e15; *ok = 0 (and NaN) otherwise.
+__host__ __device__ float angle_text_float(double x, int* ok) {
+    *ok = 1;
+    if (x == 0.0) return __builtin_signbit(x) ? -0.0f : 0.0f;  // "-0" parses to -0.0f
+    const double a = fabs(x);
+    if (!(a >= 1e-16 && a < 1e15)) {
+        *ok = 0;
+        return __builtin_nanf("");
+    }
+    // N = a * 10^(5 - e) rounded to an integer (half to even), e = floor(log10(a)):
+    // 10^5 <= N <= 10^6.  log10's estimate is corrected by exact comparisons.
+    int e = int(floor(log10(a)));
+    double n_dig = 0;
+    for (int iter = 0; iter < 4; ++iter) {
+        const int m = 5 - e;  // in [-10, 22]
+        if (m >= 0) {
+            const double s = pow10_exact(m);
+            const double hi = a * s, lo = fma(a, s, -hi);  // a s = hi + lo exactly
+            if (sign_dd_minus(hi, lo, 1e5) < 0) {
+                --e;
+                continue;
+            }
+            if (sign_dd_minus(hi, lo, 1e6) >= 0) {
+                ++e;
+                continue;
+            }
+            // rint(hi) is the answer unless hi sits on a half: then lo decides (lo == 0
+            // is a tie, which rint already broke to even)
+            const double n0 = rint(hi), r = hi - n0;  // r exact: hi, n0 in [1e5, 1e6]
+            n_dig = n0;
+            if (r == 0.5 && lo > 0) n_dig = n0 + 1;
+            if (r == -0.5 && lo < 0) n_dig = n0 - 1;
+        } else {
+            const int j = -m;  // a >= 10^6; 10^(6 + j) <= 10^16 is exact
+            const double s = pow10_exact(j);
+            if (a < 1e5 * s) {
+                --e;
+                continue;
+            }
+            if (a >= pow10_exact(6 + j)) {
+                ++e;
+                continue;
+            }
+            // a / s rounded, then compared exactly with the neighbouring half-integers
+            // times s (M s = hi + lo exactly; a - hi is exact, a being that close)
+            double n0 = rint(a / s);
+            const double mu = n0 + 0.5, ml = n0 - 0.5;
+            const double hu = mu * s, lu = fma(mu, s, -hu);
+            const double hl = ml * s, ll = fma(ml, s, -hl);
+            const int cu = sign_dd_minus(hu, lu, a);  // sign of mu s - a
+            const int cl = sign_dd_minus(hl, ll, a);
+            if (cu < 0 || (cu == 0 && fmod(n0, 2.0) != 0)) n0 += 1;
+            else if (cl > 0 || (cl == 0 && fmod(n0, 2.0) != 0)) n0 -= 1;
+            n_dig = n0;
+        }
+        break;
+    }
+    if (n_dig == 0) {  // log10 was off by more than the loop corrects (not expected)
+        *ok = 0;
+        return __builtin_nanf("");
+    }
+    // the decimal value D = n_dig * 10^(e - 5), correctly rounded to float
+    const int k = e - 5;
+    float f;
+    if (k >= 0) {
+        f = float(n_dig * pow10_exact(k));  // n_dig 10^k <= 10^15: an exact double
+    } else {
+        const double p = pow10_exact(-k);  // -k <= 22
+        const double q = n_dig / p;
+        f = float(q);
+        // float(q) is right unless q is exactly halfway between two floats: the exact
+        // quotient may lie on either side of that midpoint (or on it)
+        if (double(f) != q) {
+            const float g = next_float(f, q > double(f));
+            const double mid = 0.5 * (double(f) + double(g));  // exact
+            if (mid == q) {
+                const double hi = mid * p, lo = fma(mid, p, -hi);  // mid p exactly
+                const int c = sign_dd_minus(hi, lo, n_dig);        // sign of mid - D (times p)
+                if (c < 0) f = fmaxf(f, g);                         // D above the midpoint
+                else if (c > 0) f = fminf(f, g);
+                // c == 0: a true tie, which float(q) broke to even
+            }
+        }
+    }
+    return x < 0 ? -f : f;
+}
+
+namespace {
+
+constexpr int kMaxRot = 32;
+
+// A call's gates (kernel argument: no device copy of the description).
+struct RotArgs {
+    int32_t n_rot, dl, n_angles, n_sets;
+    int64_t base, stride;
+    int32_t kind[kMaxRot];
+    int32_t angle[kMaxRot];
+    double scale[kMaxRot];
+    int64_t offset[kMaxRot];
+};
+
+// Values a gate block takes from its angle t (the parsed float, widened): the
+// expressions of the host's gate tables.
+struct RotScalars {
+    double c2, s2, h, ct, st;  // superoperators: cos^2(t/2), sin^2(t/2), sin t / 2, cos t, sin t
+};
+
+__host__ __device__ inline RotScalars rot_scalars(int dl, double t) {
+    RotScalars r;
+    if (dl == 2) {
+        const double ch = cos(t / 2.0), sh = sin(t / 2.0);
+        r.c2 = ch * ch;
+        r.s2 = sh * sh;
+        r.st = sin(t);
+        r.h = r.st / 2.0;
+        r.ct = cos(t);
+    } else {  // amplitudes: cos(t/2), sin(t/2)
+        r.c2 = cos(t / 2.0);
+        r.s2 = sin(t / 2.0);
+        r.h = r.st = r.ct = 0;
+    }
+    return r;
+}
+
+// Symbols of the superoperator tables (src/Node.h:561-898), one nibble per element
+// e = in + 4 out: the element's value in terms of the angle's scalars.
+enum : int { S_Z0 = 0, S_ONE, S_C2, S_S2, S_NS2, S_H, S_NH, S_IH, S_NIH, S_EM, S_EP };
+
+// Row `in` lists the symbols for out = 0..3; packed so element e sits at bits 4e.
+constexpr uint64_t pack_rows(const int (&t)[4][4]) {
+    uint64_t v = 0;
+    for (int in = 0; in < 4; ++in)
+        for (int out = 0; out < 4; ++out) v |= uint64_t(t[in][out]) << (4 * (in + 4 * out));
+    return v;
+}
+constexpr int kTabRx[4][4] = {{S_C2, S_IH, S_NIH, S_S2}, {S_IH, S_C2, S_S2, S_NIH},
+                              {S_NIH, S_S2, S_C2, S_IH}, {S_S2, S_NIH, S_IH, S_C2}};
+constexpr int kTabRy[4][4] = {{S_C2, S_H, S_H, S_S2}, {S_NH, S_C2, S_NS2, S_H},
+                              {S_NH, S_NS2, S_C2, S_H}, {S_S2, S_NH, S_NH, S_C2}};
+constexpr int kTabRz[4][4] = {{S_ONE, S_Z0, S_Z0, S_Z0}, {S_Z0, S_EM, S_Z0, S_Z0},
+                              {S_Z0, S_Z0, S_EP, S_Z0}, {S_Z0, S_Z0, S_Z0, S_ONE}};
+constexpr uint64_t kPackRx = pack_rows(kTabRx), kPackRy = pack_rows(kTabRy), kPackRz = pack_rows(kTabRz);
+
+// Element e of a gate block.  Superoperators (dl = 2): the tables above, with the
+// host's expressions (NS2 = -s2, NH = -(sin t)/2, NIH = -i (sin t)/2 are exact
+// negations).  Amplitude gates (dl = 1): element in + 2 out = U[out][in] with
+// Rx = [[c, -is], [-is, c]], Ry = [[c, -s], [s, c]], Rz = diag(c - is, c + is).
+__host__ __device__ inline double2 rot_element(int kind, int dl, const RotScalars& v, int e) {
+    if (dl == 1) {
+        const double c = v.c2, s = v.s2;
+        switch (kind) {
+            case QT_ROT_RX: return (e == 0 || e == 3) ? double2{c, 0} : double2{0, -s};
+            case QT_ROT_RY: return (e == 0 || e == 3) ? double2{c, 0} : (e == 1 ? double2{-s, 0} : double2{s, 0});
+            default: return e == 0 ? double2{c, -s} : (e == 3 ? double2{c, s} : double2{0, 0});
+        }
+    }
+    const uint64_t pack = kind == QT_ROT_RX ? kPackRx : kind == QT_ROT_RY ? kPackRy : kPackRz;
+    switch (int((pack >> (4 * e)) & 15)) {
+        case S_ONE: return double2{1.0, 0};
+        case S_C2: return double2{v.c2, 0};
+        case S_S2: return double2{v.s2, 0};
+        case S_NS2: return double2{-v.s2, 0};
+        case S_H: return double2{v.h, 0};
+        case S_NH: return double2{-v.h, 0};
+        case S_IH: return double2{0, v.h};
+        case S_NIH: return double2{0, -v.h};
+        case S_EM: return double2{v.ct, -v.st};
+        case S_EP: return double2{v.ct, v.st};
+        default: return double2{0, 0};
+    }
+}
+
+constexpr int kPairsPerBlock = 256;
+
+// One thread per (set, gate) computes the angle's scalars into LDS; then the
+// workgroup writes its blocks' elements with consecutive lanes on consecutive
+// elements.  An angle outside angle_text_float's range sets *status and writes NaN.
+template <class V>
+__global__ __launch_bounds__(kPairsPerBlock) void k_rotation_params(V* __restrict__ pool,
+                                                                    const double* __restrict__ angles,
+                                                                    RotArgs r, int* __restrict__ status) {
+    __shared__ RotScalars sc[kPairsPerBlock];
+    const int64_t n_pairs = int64_t(r.n_sets) * r.n_rot;
+    const int64_t first = int64_t(blockIdx.x) * kPairsPerBlock;
+    const int tid = threadIdx.x;
+    {
+        const int64_t q = first + tid;
+        if (q < n_pairs) {
+            const int64_t a = q / r.n_rot;
+            const int rot = int(q - a * r.n_rot);
+            int ok = 1;
+            const double x = r.scale[rot] * angles[a * r.n_angles + r.angle[rot]];
+            const float f = angle_text_float(x, &ok);
+            RotScalars v = rot_scalars(r.dl, double(f));
+            if (!ok) {
+                atomicOr(status, 1);
+                const double nan = __builtin_nan("");
+                v = RotScalars{nan, nan, nan, nan, nan};
+            }
+            sc[tid] = v;
+        }
+    }
+    __syncthreads();
+    const int nel_log2 = 2 * r.dl, nel = 1 << nel_log2;
+    const int64_t pairs_here = n_pairs - first < kPairsPerBlock ? n_pairs - first : kPairsPerBlock;
+    for (int64_t idx = tid; idx < pairs_here * nel; idx += kPairsPerBlock) {
+        const int lp = int(idx >> nel_log2), e = int(idx & (nel - 1));
+        const int64_t q = first + lp;
+        const int64_t a = q / r.n_rot;
+        const int rot = int(q - a * r.n_rot);
+        const double2 v = rot_element(r.kind[rot], r.dl, sc[lp], e);
+        V out;
+        out.x = decltype(out.x)(v.x);
+        out.y = decltype(out.y)(v.y);
+        pool[r.base + a * r.stride + r.offset[rot] + e] = out;
+    }
+}
+
+// Whether the blocks a call writes keep every parameter literal of the generated
+// kernel: a literal inside a written block must sit at an element that holds the same
+// value for every angle (a zero or a one of the table), with that value.
+bool literals_survive(const qt_batch* b, const RotArgs& r) {
+    const int nel = 1 << (2 * r.dl);
+    const bool c128 = b->precision == QT_PLAIN_C128;
+    // angle-independent elements of each gate: equal at two unrelated angles
+    std::vector<std::vector<std::pair<bool, double2>>> fixed(size_t(r.n_rot));
+    for (int g = 0; g < r.n_rot; ++g) {
+        const RotScalars v1 = rot_scalars(r.dl, 0.3), v2 = rot_scalars(r.dl, 1.1);
+        for (int e = 0; e < nel; ++e) {
+            double2 a = rot_element(r.kind[g], r.dl, v1, e), c = rot_element(r.kind[g], r.dl, v2, e);
+            if (!c128) {
+                a = double2{double(float(a.x)), double(float(a.y))};
+                c = double2{double(float(c.x)), double(float(c.y))};
+            }
+            fixed[size_t(g)].push_back({a.x == c.x && a.y == c.y, a});
+        }
+    }
+    // gate and element of each offset inside a set's region (blocks do not overlap)
+    std::vector<std::pair<int64_t, int>> starts;
+    for (int g = 0; g < r.n_rot; ++g) starts.push_back({r.offset[g], g});
+    std::sort(starts.begin(), starts.end());
+    const int64_t span = int64_t(r.n_sets) * r.stride;
+    // one element of one set's region: is a literal there kept?
+    auto ok_at = [&](int64_t o, double re, double im) {
+        auto it = std::upper_bound(starts.begin(), starts.end(), std::make_pair(o, INT32_MAX));
+        if (it == starts.begin()) return true;
+        --it;
+        const int64_t el = o - it->first;
+        if (el >= nel) return true;  // between blocks: not written
+        const auto& f = fixed[size_t(it->second)][size_t(el)];
+        return f.first && f.second.x == re && f.second.y == im;
+    };
+    // usual layout: every instance's block starts a set's region, so each program's
+    // literals land on the same offsets in every instance
+    bool aligned = true;
+    std::vector<char> used(b->spec_param_lits.size(), 0);
+    for (int i = 0; i < b->n_inst && aligned; ++i) {
+        const int64_t d = b->h_pbase[size_t(i)] - r.base;
+        aligned = d >= 0 && d % r.stride == 0 && d / r.stride < r.n_sets;
+        const size_t p = size_t(b->h_prog[size_t(i)]);
+        if (p < used.size()) used[p] = 1;
+    }
+    if (aligned) {
+        for (size_t p = 0; p < used.size(); ++p)
+            if (used[p])
+                for (const auto& [e, re, im] : b->spec_param_lits[p]) {
+                    if (e < 0 || e >= r.stride) {
+                        aligned = false;  // reaches into another set's region: check per instance
+                        break;
+                    }
+                    if (!ok_at(e, re, im)) return false;
+                }
+        if (aligned) return true;
+    }
+    for (int i = 0; i < b->n_inst; ++i) {
+        const size_t p = size_t(b->h_prog[size_t(i)]);
+        if (p >= b->spec_param_lits.size()) continue;
+        const int64_t pbase = b->h_pbase[size_t(i)];
+        for (const auto& [e, re, im] : b->spec_param_lits[p]) {
+            const int64_t q = pbase + e - r.base;
+            if (q < 0 || q >= span) continue;
+            if (!ok_at(q % r.stride, re, im)) return false;
+        }
+    }
+    return true;
+}
+
+}  // namespace
+
+qt_status refresh_pool_mirror(qt_batch* b) {
+    if (!b->mirror_stale) return QT_OK;
+    Ctx* c = b->ctx;
+    qt_status s = check_hip(c, hipStreamSynchronize(c->stream), "sync");
+    if (s != QT_OK) return s;
+    const int64_t n = b->pool_elems;
+    std::vector<double> m(size_t(2 * n), 0.0);
+    if (b->precision == QT_PLAIN_C128) {
+        s = check_hip(c, hipMemcpy(m.data(), b->d_pool, size_t(n) * 16, hipMemcpyDeviceToHost), "pool read");
+    } else {
+        std::vector<float> f(size_t(2 * n));
+        s = check_hip(c, hipMemcpy(f.data(), b->d_pool, size_t(n) * 8, hipMemcpyDeviceToHost), "pool read");
+        for (size_t i = 0; i < f.size(); ++i) m[i] = double(f[i]);
+    }
+    if (s != QT_OK) return s;
+    b->h_pool_mirror.swap(m);
+    b->mirror_stale = false;
+    return QT_OK;
+}
+
+}  // namespace qt
+
+using qt::check_hip;
+using qt::set_error;
+
+extern "C" {
+
+qt_status qt_angle_text_float(double x, float* out) {
+    if (!out) return QT_INVALID_INPUT;
+    int ok = 0;
+    *out = qt::angle_text_float(x, &ok);
+    return ok ? QT_OK : QT_INVALID_INPUT;
+}
+
+qt_status qt_batch_rotation_params(qt_batch* b, int dim_log2, int n_rot, const qt_rotation* rots, int n_sets,
+                                   int n_angles, const double* angles, int angles_on_host, int64_t base,
+                                   int64_t set_stride) {
+    if (!b) return QT_INVALID_INPUT;
+    auto* ctx = b->ctx;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    if ((dim_log2 != 1 && dim_log2 != 2) || n_rot < 1 || n_rot > qt::kMaxRot || !rots || n_sets < 1 ||
+        n_angles < 1 || !angles || base < 0 || set_stride < 1)
+        return set_error(ctx, QT_INVALID_INPUT, "qt_batch_rotation_params: bad arguments");
+    const int nel = 1 << (2 * dim_log2);
+    qt::RotArgs r;
+    std::memset(&r, 0, sizeof r);
+    r.n_rot = n_rot;
+    r.dl = dim_log2;
+    r.n_angles = n_angles;
+    r.n_sets = n_sets;
+    r.base = base;
+    r.stride = set_stride;
+    std::vector<std::pair<int64_t, int64_t>> blocks;
+    for (int i = 0; i < n_rot; ++i) {
+        const qt_rotation& q = rots[i];
+        if (q.kind < QT_ROT_RX || q.kind > QT_ROT_RZ || q.angle < 0 || q.angle >= n_angles || q.offset < 0 ||
+            !std::isfinite(q.scale))
+            return set_error(ctx, QT_INVALID_INPUT, "qt_batch_rotation_params: bad gate " + std::to_string(i));
+        r.kind[i] = q.kind;
+        r.angle[i] = q.angle;
+        r.scale[i] = q.scale;
+        r.offset[i] = q.offset;
+        blocks.push_back({q.offset, q.offset + nel});
+    }
+    std::sort(blocks.begin(), blocks.end());
+    for (size_t i = 1; i < blocks.size(); ++i)
+        if (blocks[i].first < blocks[i - 1].second)
+            return set_error(ctx, QT_INVALID_INPUT, "qt_batch_rotation_params: gate blocks overlap");
+    if (blocks.back().second > set_stride)
+        return set_error(ctx, QT_INVALID_INPUT, "qt_batch_rotation_params: set_stride below a set's blocks");
+    const int64_t lo = base + blocks.front().first;
+    const int64_t hi = base + (int64_t(n_sets) - 1) * set_stride + blocks.back().second;
+    if (hi > b->pool_elems || hi < lo)
+        return set_error(ctx, QT_INVALID_INPUT, "qt_batch_rotation_params: blocks outside the leaf pool");
+    // the generated kernel: its constants, conjugate symmetry (rotation superoperators
+    // keep it; amplitude gates have none) and literals
+    if (b->specialised) {
+        bool keep = lo >= int64_t(b->spec_consts.size() / 2) && !(b->spec_hermitian && dim_log2 != 2);
+        if (keep) {
+            const char* raw = reinterpret_cast<const char*>(&r);
+            const bool same = b->rot_checked_gen == b->layout_gen && b->rot_checked_args.size() == sizeof r &&
+                              std::memcmp(b->rot_checked_args.data(), raw, sizeof r) == 0;
+            if (!same) {
+                keep = qt::literals_survive(b, r);
+                if (keep) {
+                    b->rot_checked_gen = b->layout_gen;
+                    b->rot_checked_args.assign(raw, raw + sizeof r);
+                }
+            }
+        }
+        if (!keep) {
+            qt::drop_specialised(b);
+            qt_status s = qt::after_specialised_drop(b);
+            if (s != QT_OK) return s;
+        } else {
+            b->mirror_stale = true;
+        }
+    }
+    qt_status s = QT_OK;
+    if (!b->d_param_status) {
+        s = check_hip(ctx, qt::batch_malloc(ctx, reinterpret_cast<void**>(&b->d_param_status), 256), "status");
+        if (s == QT_OK) s = qt::launch_fill_zero(ctx, b->d_param_status, 256);
+        if (s != QT_OK) return s;
+    }
+    const double* d_ang = angles;
+    if (angles_on_host) {
+        const size_t n = size_t(n_sets) * size_t(n_angles);
+        if (n > b->cap_angles) {
+            qt::batch_free(ctx, b->d_angles);
+            b->d_angles = nullptr;
+            b->cap_angles = 0;
+            s = check_hip(ctx, qt::batch_malloc(ctx, reinterpret_cast<void**>(&b->d_angles), n * 8), "angles");
+            if (s != QT_OK) return s;
+            b->cap_angles = n;
+        }
+        // the previous call's kernel may still read the buffer: the copy is ordered
+        // after it on the stream, and the host array is taken before returning
+        s = check_hip(ctx, hipMemcpyAsync(b->d_angles, angles, n * 8, hipMemcpyHostToDevice, ctx->stream), "angles");
+        if (s == QT_OK) s = check_hip(ctx, hipStreamSynchronize(ctx->stream), "angles");
+        if (s != QT_OK) return s;
+        d_ang = b->d_angles;
+    }
+    const int64_t n_pairs = int64_t(n_sets) * n_rot;
+    const unsigned grid = unsigned((n_pairs + qt::kPairsPerBlock - 1) / qt::kPairsPerBlock);
+    if (b->precision == QT_PLAIN_C128)
+        hipLaunchKernelGGL(qt::k_rotation_params<double2>, dim3(grid), dim3(qt::kPairsPerBlock), 0, ctx->stream,
+                           static_cast<double2*>(b->d_pool), d_ang, r, b->d_param_status);
+    else
+        hipLaunchKernelGGL(qt::k_rotation_params<float2>, dim3(grid), dim3(qt::kPairsPerBlock), 0, ctx->stream,
+                           static_cast<float2*>(b->d_pool), d_ang, r, b->d_param_status);
+    s = check_hip(ctx, hipGetLastError(), "k_rotation_params");
+    if (s == QT_OK) b->params_unchecked = true;
+    return s;
+}
+
+}  // extern "C"

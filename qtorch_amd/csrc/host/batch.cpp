@@ -555,6 +555,20 @@ void QaoaBatch::SetAngles(int n_sets, const double* bg) {
             "qt_batch_set_instances");
 }
 
+void QaoaBatch::UpdateAngles(int n_sets, const double* bg, bool on_host) {
+    if (n_sets != n_sets_ || n_sets < 1 || !bg) throw InvalidFunctionInput();
+    // the parameter block of each set: layer i holds Rz(-gamma_i) at 32 i and
+    // Rx(2 beta_i) at 32 i + 16, as QaoaParamBlock writes it
+    std::vector<qt_rotation> rots;
+    for (int i = 0; i < p_; ++i) {
+        rots.push_back(qt_rotation{QT_ROT_RZ, p_ + i, -1.0, int64_t(32 * i)});
+        rots.push_back(qt_rotation{QT_ROT_RX, i, 2.0, int64_t(32 * i + 16)});
+    }
+    CheckQt(ctx_, qt_batch_rotation_params(batch_, 2, int(rots.size()), rots.data(), n_sets, 2 * p_, bg,
+                                           on_host ? 1 : 0, const_elems_, 32 * int64_t(p_)),
+            "qt_batch_rotation_params");
+}
+
 void QaoaBatch::Results(double* energies, double* zz) {
     if (!zz) {  // energies only (also after group-sums-only launches)
         CheckQt(ctx_, qt_batch_results(batch_, nullptr, energies), "qt_batch_results");

@@ -84,6 +84,10 @@ public:
     QaoaBatch& operator=(const QaoaBatch&) = delete;
     // bg: n_sets x 2p angles, each set laid out as [betas..., gammas...] (src/maxcut.cpp:155-157)
     void SetAngles(int n_sets, const double* bg);
+    // New angles for the n_sets sets already set (same layout as SetAngles): the
+    // Rz / Rx blocks are rebuilt on the device (qt_batch_rotation_params); bg is in
+    // host memory (on_host) or device memory.  Asynchronous.
+    void UpdateAngles(int n_sets, const double* bg, bool on_host);
     void Results(double* energies, double* zz);
     qt_batch* Batch() const { return batch_; }
     int NumEdges() const { return int(data_.pairs.size()); }

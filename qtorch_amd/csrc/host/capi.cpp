@@ -389,6 +389,11 @@ int qth_qaoa_batch_set_angles(qth_qaoa_batch* h, int n_sets, const double* bg) {
     return Guard([&] { h->b->SetAngles(n_sets, bg); });
 }
 
+int qth_qaoa_batch_update_angles(qth_qaoa_batch* h, int n_sets, const double* bg, int on_host) {
+    if (!h) return QTH_INVALID_FUNCTION_INPUT;
+    return Guard([&] { h->b->UpdateAngles(n_sets, bg, on_host != 0); });
+}
+
 int qth_qaoa_batch_results(qth_qaoa_batch* h, double* energies, double* zz) {
     if (!h) return QTH_INVALID_FUNCTION_INPUT;
     return Guard([&] { h->b->Results(energies, zz); });

@@ -90,6 +90,48 @@ def test_contract_pair_plain_vs_oracle(ctx, ra, rb, k, d):
             np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
 
 
+# (big rank, gate rank, big axes summed (in the gate's axis order), dim_log2)
+GATE_STEPS = [(20, 4, (0, 1), 1), (20, 4, (7, 2), 1), (20, 4, (18, 19), 1), (20, 3, (0,), 1), (20, 3, (19,), 1),
+              (20, 5, (3, 11, 6), 1), (21, 6, (1, 4, 9, 20), 1), (10, 2, (0,), 2), (10, 2, (9,), 2),
+              (10, 1, (4,), 2), (10, 4, (2, 7), 2)]
+
+
+@pytest.mark.parametrize("prec", [qa.QT_PLAIN_C128, qa.QT_C64_4M])
+@pytest.mark.parametrize("big_first", [True, False])
+@pytest.mark.parametrize("rbig,rg,pos,d", GATE_STEPS)
+def test_gate_on_large_tensor(ctx, rbig, rg, pos, d, big_first, prec):
+    """A gate (or a thin tensor) against a 2^20-element tensor whose free axes keep
+    their order, as every step of a circuit sweep: the large side's offset comes from
+    inserting zero bits at its summed positions (k_thin).  complex128 is bit-identical
+    to the oracle's loop in the reference's order (src/Network.h:876-971); 4M within
+    FP32 rounding.  Summed positions at the bottom, the top and unsorted."""
+    rng = np.random.default_rng(rbig * 100 + rg * 10 + sum(pos) + d)
+    dt = np.complex128 if prec == qa.QT_PLAIN_C128 else np.complex64
+    dtype = qa.QT_DTYPE_C128 if prec == qa.QT_PLAIN_C128 else qa.QT_DTYPE_C64
+    big = _rand(rng, 1 << (d * rbig), dt)
+    gate = _rand(rng, 1 << (d * rg), dt)
+    k = len(pos)
+    big_free = [a for a in range(rbig) if a not in pos]
+    if big_first:
+        ra, rb, a, b = rbig, rg, big, gate
+        pairs = [(p, i) for i, p in enumerate(pos)]
+        free = [(True, x) for x in big_free] + [(False, g) for g in range(k, rg)]
+    else:
+        ra, rb, a, b = rg, rbig, gate, big
+        pairs = [(i, p) for i, p in enumerate(pos)]
+        free = [(True, g) for g in range(k, rg)] + [(False, x) for x in big_free]
+    assert _regime(free, k, d) == "skinny"
+    ta = qa.Tensor(ctx, ra, d, dtype).upload(a)
+    tb = qa.Tensor(ctx, rb, d, dtype).upload(b)
+    tc, _ = qa.contract_pair(ctx, ta, tb, pairs, free, prec)
+    got = tc.download()
+    ref = qo.contract_indices(a.astype(np.complex128), ra, b.astype(np.complex128), rb, free, pairs, d)
+    if prec == qa.QT_PLAIN_C128:
+        assert np.array_equal(got, ref)
+    else:
+        assert np.abs(got.astype(np.complex128) - ref).max() <= 2e-6 * np.sqrt(1 << (d * k)) * np.abs(ref).max()
+
+
 @pytest.mark.parametrize("prec", [qa.QT_PLAIN_C128, qa.QT_C64_4M])
 def test_gemm_with_many_permute_tiles(ctx, prec):
     """A 22-bit operand whose summed axes are its lowest: the GEMM path permutes it in

@@ -286,3 +286,100 @@ def test_group_sums_only_equal_the_summed_values(ctx, precision, n_sets, hermiti
     assert np.array_equal(e4, e2) and z4 is not None
     assert np.array_equal(e4, np.sum(0.5 * (1.0 - z4), axis=1)) or np.max(np.abs(e4 - np.sum(0.5 * (1.0 - z4), axis=1))) <= 1e-12 * np.max(np.abs(e4))
     b.close()
+
+
+def _energies(ctx, precision, sets, p=1):
+    r = qa.QaoaBatch(ctx, _graph(), p, precision=precision)
+    r.set_angles(sets)
+    r.launch()
+    e, z = r.results()
+    r.close()
+    return e, z
+
+
+@pytest.mark.parametrize("precision,p,compiled", [(qa.QT_PLAIN_C128, 1, False), (qa.QT_PLAIN_C128, 1, True),
+                                                  (qa.QT_C64_4M, 1, True), (qa.QT_PLAIN_C128, 2, False),
+                                                  (qa.QT_C64_4M, 2, False)])
+def test_update_angles_matches_rebuilt_networks(ctx, precision, p, compiled):
+    """New angles written on the device (qth_qaoa_batch_update_angles ->
+    qt_batch_rotation_params) against networks rebuilt from the gates' text on the
+    host (QaoaBatch.set_angles): the parsed float angle is the same bit for bit, the
+    device's cos / sin differ from glibc's by at most a few units in the last place,
+    so complex128 per-edge values agree to 1e-13 and complex64 4M ones to its own
+    rounding.  Host- and device-resident angle arrays; the generated kernel stays."""
+    n = 300
+    b = qa.QaoaBatch(ctx, _graph(), p, precision=precision)
+    b.set_angles(_sets_p(n, p, 11))
+    if compiled:
+        b.compile(hermitian=True)
+    for it, seed in enumerate((12, 13)):
+        new = _sets_p(n, p, seed)
+        if it == 0:
+            b.update_angles(new)
+        else:
+            import torch
+            dev = torch.tensor(new, dtype=torch.float64, device="cuda")
+            torch.cuda.synchronize()
+            b.update_angles(device_ptr=dev.data_ptr(), n_sets=n)
+        b.launch()
+        e, z = b.results()
+        assert b.specialised()[0] == compiled
+        e_ref, z_ref = _energies(ctx, precision, new, p)
+        tol = 1e-13 if precision == qa.QT_PLAIN_C128 else 2e-6
+        assert np.abs(z - z_ref).max() <= tol, (it, np.abs(z - z_ref).max())
+        assert np.abs(e - e_ref).max() <= tol * np.abs(e_ref).max() * 4
+    b.close()
+
+
+def _sets_p(n, p, seed):
+    rng = np.random.default_rng(seed)
+    return np.concatenate([rng.uniform(0, math.pi / 2, (n, p)), rng.uniform(-math.pi, math.pi, (n, p))], axis=1)
+
+
+def test_update_angles_sums_only_and_literals(ctx):
+    """Energies-only launches follow device-written angles; a kernel generated from a
+    single angle set (every parameter element a literal) is dropped by the first
+    update that could change its literals, and the table executor's results follow."""
+    b = qa.QaoaBatch(ctx, _graph(), 1, precision=qa.QT_PLAIN_C128)
+    b.set_angles(_sets_p(512, 1, 21))
+    b.compile(hermitian=True)
+    assert b.sums_only(True)
+    new = _sets_p(512, 1, 22)
+    b.update_angles(new)
+    b.launch()
+    e, _ = b.results()
+    e_ref, _ = _energies(ctx, qa.QT_PLAIN_C128, new)
+    assert np.abs(e - e_ref).max() <= 1e-12 * np.abs(e_ref).max()
+    b.close()
+    one = qa.QaoaBatch(ctx, _graph(), 1, precision=qa.QT_PLAIN_C128)
+    one.set_angles(_sets_p(1, 1, 23))
+    one.compile()
+    assert one.specialised()[0]
+    new1 = _sets_p(1, 1, 24)
+    one.update_angles(new1)
+    assert not one.specialised()[0]
+    one.launch()
+    e1, z1 = one.results()
+    e1_ref, z1_ref = _energies(ctx, qa.QT_PLAIN_C128, new1)
+    assert np.abs(z1 - z1_ref).max() <= 1e-13
+    one.close()
+
+
+def test_update_angles_out_of_range_reports(ctx):
+    """An angle outside the exact text round trip's range (here 1e300) poisons its
+    set's blocks and makes the next results call fail; valid angles recover."""
+    b = qa.QaoaBatch(ctx, _graph(), 1, precision=qa.QT_PLAIN_C128)
+    sets = _sets_p(8, 1, 31)
+    b.set_angles(sets)
+    bad = sets.copy()
+    bad[3, 1] = 1e300
+    b.update_angles(bad)
+    b.launch()
+    with pytest.raises(qa.EngineError):
+        b.results()
+    b.update_angles(sets)
+    b.launch()
+    e, _ = b.results()
+    e_ref, _ = _energies(ctx, qa.QT_PLAIN_C128, sets)
+    assert np.abs(e - e_ref).max() <= 1e-12 * np.abs(e_ref).max()
+    b.close()
