@@ -688,6 +688,7 @@ qt_status append_programs(qt_batch* b, int n, qt_program* const* progs, const in
 // batches), growing the pool (contents kept) when needed.  The stream is idle.
 qt_status write_pool(qt_batch* b, int64_t offset, const double* values, int64_t n) {
     Ctx* c = b->ctx;
+    b->rot_full_valid = false;  // rotation blocks are written in full again next time
     if (b->specialised) {
         bool keep = !specialised_consts_change(b, offset, values, n);  // its literals are the constants
         if (keep && b->mirror_stale) keep = refresh_pool_mirror(b) == QT_OK;

@@ -13,9 +13,15 @@
 
 namespace qt {
 
-// Parameter elements (offset in the instance's block) that hold one value in every
-// instance of a program: (offset, re, im); generated code has them as literals.
-using ParamLits = std::vector<std::tuple<int64_t, double, double>>;
+// Parts of parameter elements (offset in the instance's block) that hold one value in
+// every instance of a program (a real part that is always 0, say, while the imaginary
+// part varies); generated code has them as literals.
+struct ParamLit {
+    int64_t off;
+    bool has_re, has_im;  // which parts are literals
+    double re, im;
+};
+using ParamLits = std::vector<ParamLit>;
 
 struct ProgHdr {
     int32_t step_begin, n_steps;
@@ -176,6 +182,8 @@ struct qt_batch {
     uint64_t layout_gen = 1;          // changes with the instances or the generated kernel
     uint64_t rot_checked_gen = 0;     // layout_gen and arguments the literal check last passed for
     std::vector<char> rot_checked_args;
+    bool rot_full_valid = false;      // every element of the rot_full_args layout holds its table value
+    std::vector<char> rot_full_args;
 };
 
 namespace qt {

@@ -130,9 +130,9 @@ bool emit_program(int id, const ProgView& pv, bool c128, bool hermitian, const d
         *err = tag + "conjugate symmetry needs two index bits per axis";
         return false;
     }
-    std::unordered_map<int64_t, std::pair<double, double>> plit;
+    std::unordered_map<int64_t, ParamLit> plit;
     if (pv.param_lits)
-        for (const auto& [off, re, im] : *pv.param_lits) plit[off] = {re, im};
+        for (const ParamLit& l : *pv.param_lits) plit[l.off] = l;
     auto operand = [&](int32_t code, int64_t idx, Val* v) -> bool {
         if (code >= 0) {
             auto it = scratch.find(code + int32_t(idx));
@@ -163,14 +163,13 @@ bool emit_program(int id, const ProgView& pv, bool c128, bool hermitian, const d
         const int64_t r = pv.refs[li];
         if (r & kParam) {
             const int64_t e = (r & ~kParam) + idx;
-            auto lit = plit.find(e);
-            if (lit != plit.end()) {  // the same in every instance
-                v->re = Part{true, lit->second.first, "", false};
-                v->im = Part{true, lit->second.second, "", false};
-                return true;
-            }
             v->re = Part{false, 0, "p" + std::to_string(e) + "r"};
             v->im = Part{false, 0, "p" + std::to_string(e) + "i"};
+            auto lit = plit.find(e);
+            if (lit != plit.end()) {  // parts that are the same in every instance
+                if (lit->second.has_re) v->re = Part{true, lit->second.re, "", false};
+                if (lit->second.has_im) v->im = Part{true, lit->second.im, "", false};
+            }
             return true;
         }
         const int64_t e = pv.const_base + r + idx;
@@ -541,17 +540,19 @@ std::vector<ParamLits> collect_param_lits(const qt_batch* b, const std::vector<d
         for (const auto& [r, bits] : param_leaves(b, b->h_hdr[p]))
             for (int64_t e = r; e < r + (int64_t(1) << bits); ++e) {
                 if (!done.insert(e).second) continue;
-                bool same = true;
+                bool same_re = true, same_im = true;
                 for (int64_t base : bs) {
                     if (base + e >= n_pool) {
-                        same = false;
+                        same_re = same_im = false;
                         break;
                     }
-                    same = pool[size_t(2 * (base + e))] == pool[size_t(2 * (bs[0] + e))] &&
-                           pool[size_t(2 * (base + e) + 1)] == pool[size_t(2 * (bs[0] + e) + 1)];
-                    if (!same) break;
+                    same_re = same_re && pool[size_t(2 * (base + e))] == pool[size_t(2 * (bs[0] + e))];
+                    same_im = same_im && pool[size_t(2 * (base + e) + 1)] == pool[size_t(2 * (bs[0] + e) + 1)];
+                    if (!same_re && !same_im) break;
                 }
-                if (same) lits[p].push_back({e, pool[size_t(2 * (bs[0] + e))], pool[size_t(2 * (bs[0] + e) + 1)]});
+                if (same_re || same_im)
+                    lits[p].push_back(ParamLit{e, same_re, same_im, pool[size_t(2 * (bs[0] + e))],
+                                               pool[size_t(2 * (bs[0] + e) + 1)]});
             }
     }
     return lits;
@@ -602,8 +603,9 @@ bool specialised_params_hold(const qt_batch* b) {
         const size_t p = size_t(b->h_prog[size_t(i)]);
         const int64_t base = b->h_pbase[size_t(i)];
         if (p >= b->spec_param_lits.size()) return false;
-        for (const auto& [e, re, im] : b->spec_param_lits[p])
-            if (base + e >= n_pool || pool[size_t(2 * (base + e))] != re || pool[size_t(2 * (base + e) + 1)] != im)
+        for (const ParamLit& l : b->spec_param_lits[p])
+            if (base + l.off >= n_pool || (l.has_re && pool[size_t(2 * (base + l.off))] != l.re) ||
+                (l.has_im && pool[size_t(2 * (base + l.off) + 1)] != l.im))
                 return false;
     }
     return !b->spec_hermitian || params_conjugate_symmetric(b, pool);

@@ -277,34 +277,80 @@ struct ThinArgs {
     int32_t small_off[16];  // small operand offset of its free part o
 };
 
-template <class V, int MODE, int NS, int NB, bool SMALL_B>
+// Two adjacent elements of V, loaded and stored as one access.
+template <class V>
+struct alignas(2 * sizeof(V)) Pair {
+    V v[2];
+};
+
+// E = 2 (X's bit 0 free): each lane takes the adjacent elements 2u, 2u + 1, read and
+// written as pairs (16-byte accesses in complex64, 32 in complex128).
+template <class V, int MODE, int NS, int NB, bool SMALL_B, int E>
 __global__ __launch_bounds__(kThreads) void k_thin(const V* __restrict__ X, const V* __restrict__ G,
                                                    V* __restrict__ C, ThinArgs r) {
     constexpr int NJ = 1 << NS, NO = 1 << NB;
     const int64_t stride = int64_t(gridDim.x) * kThreads;
-    for (int64_t t = int64_t(blockIdx.x) * kThreads + threadIdx.x; t < r.n_big; t += stride) {
+    const int64_t n_units = r.n_big / E;
+    for (int64_t u = int64_t(blockIdx.x) * kThreads + threadIdx.x; u < n_units; u += stride) {
+        const int64_t t = u * E;
         int64_t x = t;
 #pragma unroll
         for (int i = 0; i < NS; ++i) {
             const int p = r.ins_pos[i];
             x = ((x >> p) << (p + 1)) | (x & ((int64_t(1) << p) - 1));
         }
-        V xs[NJ];
+        V xs[NJ][E];
 #pragma unroll
-        for (int j = 0; j < NJ; ++j) xs[j] = X[x + r.g_big[j]];
-        Acc<V, MODE> acc[NO];
+        for (int j = 0; j < NJ; ++j) {
+            if constexpr (E == 2) {
+                const Pair<V> pr = *reinterpret_cast<const Pair<V>*>(X + x + r.g_big[j]);
+                xs[j][0] = pr.v[0];
+                xs[j][1] = pr.v[1];
+            } else {
+                xs[j][0] = X[x + r.g_big[j]];
+            }
+        }
+        Acc<V, MODE> acc[E][NO];
 #pragma unroll
         for (int j = 0; j < NJ; ++j)
 #pragma unroll
             for (int o = 0; o < NO; ++o) {
                 const V g = G[r.small_off[o] + r.g_small[j]];
-                if constexpr (SMALL_B) mac<V, MODE>(acc[o], xs[j], g);
-                else mac<V, MODE>(acc[o], g, xs[j]);
-            }
 #pragma unroll
-        for (int o = 0; o < NO; ++o) {
-            const int64_t idx = SMALL_B ? t + (int64_t(o) << r.lo_bits) : int64_t(o) + (t << NB);
-            C[idx] = finish<V, MODE>(acc[o]);
+                for (int e = 0; e < E; ++e) {
+                    if constexpr (SMALL_B) mac<V, MODE>(acc[e][o], xs[j][e], g);
+                    else mac<V, MODE>(acc[e][o], g, xs[j][e]);
+                }
+            }
+        if constexpr (SMALL_B) {
+#pragma unroll
+            for (int o = 0; o < NO; ++o) {
+                const int64_t idx = t + (int64_t(o) << r.lo_bits);
+                if constexpr (E == 2) {
+                    Pair<V> pr;
+                    pr.v[0] = finish<V, MODE>(acc[0][o]);
+                    pr.v[1] = finish<V, MODE>(acc[1][o]);
+                    *reinterpret_cast<Pair<V>*>(C + idx) = pr;
+                } else {
+                    C[idx] = finish<V, MODE>(acc[0][o]);
+                }
+            }
+        } else {
+            // the lane's E * NO outputs are contiguous: C[o + (t << NB)]
+#pragma unroll
+            for (int e = 0; e < E; ++e)
+#pragma unroll
+                for (int o = 0; o < NO; o += (NO >= 2 ? 2 : 1)) {
+                    const int64_t idx = int64_t(o) + ((t + e) << NB);
+                    if constexpr (NO >= 2) {
+                        Pair<V> pr;
+                        pr.v[0] = finish<V, MODE>(acc[e][o]);
+                        pr.v[1] = finish<V, MODE>(acc[e][o + 1]);
+                        *reinterpret_cast<Pair<V>*>(C + idx) = pr;
+                    } else {
+                        C[idx] = finish<V, MODE>(acc[e][o]);
+                    }
+                }
         }
     }
 }
@@ -401,6 +447,11 @@ __global__ __launch_bounds__(kThreads) void k_reduce_final(const V* __restrict__
 
 // LDS-staged bit permutation (see PermPlan).  Reads walk the input-low bits
 // contiguously, writes walk the output-low bits contiguously.
+// Tile element t sits at slot t with its low 6 bits xor-ed with its row t / 64: a read
+// that walks the tile 64 elements at a time (an input-low bit moved to an output-high
+// position) hits a different bank with every lane, and the tile stays 2^tbits slots.
+__device__ __forceinline__ int tile_slot(int t) { return t ^ ((t >> 6) & 63); }
+
 template <class V>
 __global__ __launch_bounds__(kThreads) void k_permute(const V* __restrict__ in, V* __restrict__ out,
                                                       PermPlan p) {
@@ -415,26 +466,42 @@ __global__ __launch_bounds__(kThreads) void k_permute(const V* __restrict__ in, 
         }
     const int tbits = p.tl + p.n_extra_in;
     const int n = 1 << tbits;
-    const int low_mask = (1 << p.tl) - 1;
-    for (int t = threadIdx.x; t < n; t += kThreads) {
-        int64_t off = base_in + (t & low_mask);
-        for (int j = 0; j < p.n_extra_in; ++j)
-            if ((t >> (p.tl + j)) & 1) off += int64_t(1) << p.extra_in_pos[j];
-        tile[t] = in[off];
-    }
+    const int tid = threadIdx.x;
+    // element t = tid + 256 q: the offsets split into the thread's part (tile bits
+    // 0-7, computed once) and the step's part (bits 8 and up, uniform)
+    auto in_bit = [&](int b) -> int64_t { return b < p.tl ? int64_t(1) << b : int64_t(1) << p.extra_in_pos[b - p.tl]; };
+    auto out_bit = [&](int b) -> int64_t { return b < p.tl ? int64_t(1) << b : int64_t(1) << p.extra_out_pos[b - p.tl]; };
+    auto tile_bit = [&](int b) -> int { return b < p.tl ? p.lowout_tile[b] : p.extra_out_tile[b - p.tl]; };
+    const int lo_bits = tbits < 8 ? tbits : 8;
+    int64_t in_t = base_in, out_t = base_out;
+    int slot_t = 0;
+    for (int b = 0; b < lo_bits; ++b)
+        if ((tid >> b) & 1) {
+            in_t += in_bit(b);
+            out_t += out_bit(b);
+            slot_t |= 1 << tile_bit(b);
+        }
+    const int steps = n > kThreads ? n / kThreads : 1;
+    if (tid < n)
+        for (int q = 0; q < steps; ++q) {
+            int64_t off = in_t;
+            for (int b = 8; b < tbits; ++b)
+                if ((q >> (b - 8)) & 1) off += in_bit(b);
+            const int t = tid + (q << 8);
+            tile[tile_slot(t)] = in[off];
+        }
     __syncthreads();
-    for (int u = threadIdx.x; u < n; u += kThreads) {
-        int t = 0;
-        int64_t off = base_out + (u & low_mask);
-        for (int j = 0; j < p.tl; ++j)
-            if ((u >> j) & 1) t |= 1 << p.lowout_tile[j];
-        for (int j = 0; j < p.n_extra_out; ++j)
-            if ((u >> (p.tl + j)) & 1) {
-                t |= 1 << p.extra_out_tile[j];
-                off += int64_t(1) << p.extra_out_pos[j];
-            }
-        out[off] = tile[t];
-    }
+    if (tid < n)
+        for (int q = 0; q < steps; ++q) {
+            int64_t off = out_t;
+            int t = slot_t;
+            for (int b = 8; b < tbits; ++b)
+                if ((q >> (b - 8)) & 1) {
+                    off += out_bit(b);
+                    t |= 1 << tile_bit(b);
+                }
+            out[off] = tile[tile_slot(t)];
+        }
 }
 
 // Complex FP64 GEMM on MFMA: C[m + M n] = sum_s A'[s][m] * B'[s][n].
@@ -759,11 +826,21 @@ qt_status launch_skinny(Ctx* c, const V* A, const V* B, V* C, const BitMap& m, i
 
 template <class V, int MODE, int NS, int NB>
 qt_status launch_thin_nb(Ctx* c, const V* X, const V* G, V* C, const ThinArgs& r, bool small_b) {
-    const unsigned grid = grid_for(r.n_big, kThreads, 32768);
-    if (small_b)
-        hipLaunchKernelGGL((k_thin<V, MODE, NS, NB, true>), dim3(grid), dim3(kThreads), 0, c->stream, X, G, C, r);
+    // pairs when X's bit 0 is free (the lowest summed position is above it) and the
+    // large side has at least 2 elements
+    const bool pairs = r.ins_pos[0] > 0 && r.n_big >= 2;
+    // one unit per lane, no grid-stride reuse: on MI355X streaming kernels reach their
+    // highest HBM rates with a workgroup per 256 units (a capped grid looping over the
+    // tensor measured 10-30 % slower)
+    const unsigned grid = grid_for(pairs ? r.n_big / 2 : r.n_big, kThreads, 1u << 30);
+    if (small_b && pairs)
+        hipLaunchKernelGGL((k_thin<V, MODE, NS, NB, true, 2>), dim3(grid), dim3(kThreads), 0, c->stream, X, G, C, r);
+    else if (small_b)
+        hipLaunchKernelGGL((k_thin<V, MODE, NS, NB, true, 1>), dim3(grid), dim3(kThreads), 0, c->stream, X, G, C, r);
+    else if (pairs)
+        hipLaunchKernelGGL((k_thin<V, MODE, NS, NB, false, 2>), dim3(grid), dim3(kThreads), 0, c->stream, X, G, C, r);
     else
-        hipLaunchKernelGGL((k_thin<V, MODE, NS, NB, false>), dim3(grid), dim3(kThreads), 0, c->stream, X, G, C, r);
+        hipLaunchKernelGGL((k_thin<V, MODE, NS, NB, false, 1>), dim3(grid), dim3(kThreads), 0, c->stream, X, G, C, r);
     return check_hip(c, hipGetLastError(), "k_thin");
 }
 

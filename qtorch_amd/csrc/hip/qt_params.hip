@@ -25,10 +25,13 @@ namespace qt {
 
 namespace {
 
-// 10^k for 0 <= k <= 22: every such power is a double, and each product is exact.
+// 10^k for 0 <= k <= 22 by squaring: every power of ten up to 10^22 is a double, so
+// each product that reaches the result is exact (10^32, formed last for k >= 16, is
+// never used).
 __host__ __device__ inline double pow10_exact(int k) {
-    double p = 1.0;
-    for (int i = 0; i < k; ++i) p *= 10.0;
+    double p = 1.0, b = 10.0;
+    for (; k > 0; k >>= 1, b *= b)
+        if (k & 1) p *= b;
     return p;
 }
 
@@ -155,6 +158,15 @@ struct RotArgs {
     int64_t offset[kMaxRot];
 };
 
+// The elements a launch writes per angle set, (gate << 4) | element: every element
+// the first time a layout is written, afterwards those whose value depends on the
+// angle (the others already hold their table value).
+constexpr int kMaxWrites = kMaxRot * 16;
+struct RotWrites {
+    int32_t n;
+    uint16_t w[kMaxWrites];
+};
+
 // Values a gate block takes from its angle t (the parsed float, widened): the
 // expressions of the host's gate tables.
 struct RotScalars {
@@ -163,16 +175,16 @@ struct RotScalars {
 
 __host__ __device__ inline RotScalars rot_scalars(int dl, double t) {
     RotScalars r;
+    double ch, sh;
+    sincos(t / 2.0, &sh, &ch);
     if (dl == 2) {
-        const double ch = cos(t / 2.0), sh = sin(t / 2.0);
         r.c2 = ch * ch;
         r.s2 = sh * sh;
-        r.st = sin(t);
+        sincos(t, &r.st, &r.ct);
         r.h = r.st / 2.0;
-        r.ct = cos(t);
     } else {  // amplitudes: cos(t/2), sin(t/2)
-        r.c2 = cos(t / 2.0);
-        r.s2 = sin(t / 2.0);
+        r.c2 = ch;
+        r.s2 = sh;
         r.h = r.st = r.ct = 0;
     }
     return r;
@@ -226,49 +238,76 @@ __host__ __device__ inline double2 rot_element(int kind, int dl, const RotScalar
     }
 }
 
-constexpr int kPairsPerBlock = 256;
+// 64 (set, gate) pairs per workgroup of 128 threads: many workgroups (4,096 for
+// 131,072 sets of two gates) and few elements per thread to write.
+constexpr int kPairsPerBlock = 64, kRotThreads = 128;
+
+// Element e of a superoperator block without branches (lanes of one wave hold gates of
+// different kinds): the symbol picks the value through selects.
+__device__ inline double2 rot_element_select(uint64_t pack, const RotScalars& v, int e) {
+    const int sym = int((pack >> (4 * e)) & 15);
+    double re = 0, im = 0;
+    re = sym == S_ONE ? 1.0 : re;
+    re = sym == S_C2 ? v.c2 : re;
+    re = sym == S_S2 ? v.s2 : re;
+    re = sym == S_NS2 ? -v.s2 : re;
+    re = sym == S_H ? v.h : re;
+    re = sym == S_NH ? -v.h : re;
+    re = (sym == S_EM || sym == S_EP) ? v.ct : re;
+    im = sym == S_IH ? v.h : im;
+    im = sym == S_NIH ? -v.h : im;
+    im = sym == S_EM ? -v.st : im;
+    im = sym == S_EP ? v.st : im;
+    return double2{re, im};
+}
 
 // One thread per (set, gate) computes the angle's scalars into LDS; then the
-// workgroup writes its blocks' elements with consecutive lanes on consecutive
-// elements.  An angle outside angle_text_float's range sets *status and writes NaN.
+// workgroup writes the listed elements of its sets' blocks, consecutive lanes on
+// consecutive list entries (the blocks of a set's gates, and of consecutive sets, are
+// usually adjacent in the pool), each lane selecting its element's value without
+// branches.  An angle outside angle_text_float's range sets *status and writes NaN.
 template <class V>
-__global__ __launch_bounds__(kPairsPerBlock) void k_rotation_params(V* __restrict__ pool,
+__global__ __launch_bounds__(kRotThreads) void k_rotation_params(V* __restrict__ pool,
                                                                     const double* __restrict__ angles,
-                                                                    RotArgs r, int* __restrict__ status) {
+                                                                    RotArgs r, RotWrites wr,
+                                                                    int* __restrict__ status) {
     __shared__ RotScalars sc[kPairsPerBlock];
-    const int64_t n_pairs = int64_t(r.n_sets) * r.n_rot;
-    const int64_t first = int64_t(blockIdx.x) * kPairsPerBlock;
+    const int sets_per_block = kPairsPerBlock / r.n_rot;
+    const int first_set = int(blockIdx.x) * sets_per_block;
+    const int sets_here = r.n_sets - first_set < sets_per_block ? r.n_sets - first_set : sets_per_block;
     const int tid = threadIdx.x;
-    {
-        const int64_t q = first + tid;
-        if (q < n_pairs) {
-            const int64_t a = q / r.n_rot;
-            const int rot = int(q - a * r.n_rot);
-            int ok = 1;
-            const double x = r.scale[rot] * angles[a * r.n_angles + r.angle[rot]];
-            const float f = angle_text_float(x, &ok);
-            RotScalars v = rot_scalars(r.dl, double(f));
-            if (!ok) {
-                atomicOr(status, 1);
-                const double nan = __builtin_nan("");
-                v = RotScalars{nan, nan, nan, nan, nan};
-            }
-            sc[tid] = v;
+    if (tid < sets_here * r.n_rot) {
+        const int a = first_set + tid / r.n_rot;
+        const int rot = tid % r.n_rot;
+        int ok = 1;
+        const double x = r.scale[rot] * angles[int64_t(a) * r.n_angles + r.angle[rot]];
+        const float f = angle_text_float(x, &ok);
+        RotScalars v = rot_scalars(r.dl, double(f));
+        if (!ok) {
+            atomicOr(status, 1);
+            const double nan = __builtin_nan("");
+            v = RotScalars{nan, nan, nan, nan, nan};
         }
+        sc[tid] = v;
     }
     __syncthreads();
-    const int nel_log2 = 2 * r.dl, nel = 1 << nel_log2;
-    const int64_t pairs_here = n_pairs - first < kPairsPerBlock ? n_pairs - first : kPairsPerBlock;
-    for (int64_t idx = tid; idx < pairs_here * nel; idx += kPairsPerBlock) {
-        const int lp = int(idx >> nel_log2), e = int(idx & (nel - 1));
-        const int64_t q = first + lp;
-        const int64_t a = q / r.n_rot;
-        const int rot = int(q - a * r.n_rot);
-        const double2 v = rot_element(r.kind[rot], r.dl, sc[lp], e);
+    for (int idx = tid; idx < sets_here * wr.n; idx += kRotThreads) {
+        const int la = idx / wr.n;
+        const int w = wr.w[idx - la * wr.n];
+        const int rot = w >> 4, e = w & 15;
+        const int kind = r.kind[rot];
+        const RotScalars& v = sc[la * r.n_rot + rot];
+        double2 val;
+        if (r.dl == 2) {
+            const uint64_t pack = kind == QT_ROT_RX ? kPackRx : kind == QT_ROT_RY ? kPackRy : kPackRz;
+            val = rot_element_select(pack, v, e);
+        } else {
+            val = rot_element(kind, 1, v, e);
+        }
         V out;
-        out.x = decltype(out.x)(v.x);
-        out.y = decltype(out.y)(v.y);
-        pool[r.base + a * r.stride + r.offset[rot] + e] = out;
+        out.x = decltype(out.x)(val.x);
+        out.y = decltype(out.y)(val.y);
+        pool[r.base + int64_t(first_set + la) * r.stride + r.offset[rot] + e] = out;
     }
 }
 
@@ -278,8 +317,12 @@ __global__ __launch_bounds__(kPairsPerBlock) void k_rotation_params(V* __restric
 bool literals_survive(const qt_batch* b, const RotArgs& r) {
     const int nel = 1 << (2 * r.dl);
     const bool c128 = b->precision == QT_PLAIN_C128;
-    // angle-independent elements of each gate: equal at two unrelated angles
-    std::vector<std::vector<std::pair<bool, double2>>> fixed(size_t(r.n_rot));
+    // angle-independent parts of each gate's elements: equal at two unrelated angles
+    struct Fixed {
+        bool re, im;
+        double2 v;
+    };
+    std::vector<std::vector<Fixed>> fixed(size_t(r.n_rot));
     for (int g = 0; g < r.n_rot; ++g) {
         const RotScalars v1 = rot_scalars(r.dl, 0.3), v2 = rot_scalars(r.dl, 1.1);
         for (int e = 0; e < nel; ++e) {
@@ -288,7 +331,7 @@ bool literals_survive(const qt_batch* b, const RotArgs& r) {
                 a = double2{double(float(a.x)), double(float(a.y))};
                 c = double2{double(float(c.x)), double(float(c.y))};
             }
-            fixed[size_t(g)].push_back({a.x == c.x && a.y == c.y, a});
+            fixed[size_t(g)].push_back(Fixed{a.x == c.x, a.y == c.y, a});
         }
     }
     // gate and element of each offset inside a set's region (blocks do not overlap)
@@ -297,14 +340,14 @@ bool literals_survive(const qt_batch* b, const RotArgs& r) {
     std::sort(starts.begin(), starts.end());
     const int64_t span = int64_t(r.n_sets) * r.stride;
     // one element of one set's region: is a literal there kept?
-    auto ok_at = [&](int64_t o, double re, double im) {
+    auto ok_at = [&](int64_t o, const ParamLit& l) {
         auto it = std::upper_bound(starts.begin(), starts.end(), std::make_pair(o, INT32_MAX));
         if (it == starts.begin()) return true;
         --it;
         const int64_t el = o - it->first;
         if (el >= nel) return true;  // between blocks: not written
-        const auto& f = fixed[size_t(it->second)][size_t(el)];
-        return f.first && f.second.x == re && f.second.y == im;
+        const Fixed& f = fixed[size_t(it->second)][size_t(el)];
+        return (!l.has_re || (f.re && f.v.x == l.re)) && (!l.has_im || (f.im && f.v.y == l.im));
     };
     // usual layout: every instance's block starts a set's region, so each program's
     // literals land on the same offsets in every instance
@@ -319,12 +362,12 @@ bool literals_survive(const qt_batch* b, const RotArgs& r) {
     if (aligned) {
         for (size_t p = 0; p < used.size(); ++p)
             if (used[p])
-                for (const auto& [e, re, im] : b->spec_param_lits[p]) {
-                    if (e < 0 || e >= r.stride) {
+                for (const ParamLit& l : b->spec_param_lits[p]) {
+                    if (l.off < 0 || l.off >= r.stride) {
                         aligned = false;  // reaches into another set's region: check per instance
                         break;
                     }
-                    if (!ok_at(e, re, im)) return false;
+                    if (!ok_at(l.off, l)) return false;
                 }
         if (aligned) return true;
     }
@@ -332,10 +375,10 @@ bool literals_survive(const qt_batch* b, const RotArgs& r) {
         const size_t p = size_t(b->h_prog[size_t(i)]);
         if (p >= b->spec_param_lits.size()) continue;
         const int64_t pbase = b->h_pbase[size_t(i)];
-        for (const auto& [e, re, im] : b->spec_param_lits[p]) {
-            const int64_t q = pbase + e - r.base;
+        for (const ParamLit& l : b->spec_param_lits[p]) {
+            const int64_t q = pbase + l.off - r.base;
             if (q < 0 || q >= span) continue;
-            if (!ok_at(q % r.stride, re, im)) return false;
+            if (!ok_at(q % r.stride, l)) return false;
         }
     }
     return true;
@@ -387,6 +430,8 @@ qt_status qt_batch_rotation_params(qt_batch* b, int dim_log2, int n_rot, const q
     if ((dim_log2 != 1 && dim_log2 != 2) || n_rot < 1 || n_rot > qt::kMaxRot || !rots || n_sets < 1 ||
         n_angles < 1 || !angles || base < 0 || set_stride < 1)
         return set_error(ctx, QT_INVALID_INPUT, "qt_batch_rotation_params: bad arguments");
+    if (int64_t(n_sets) * n_rot >= (int64_t(1) << 31))
+        return set_error(ctx, QT_INVALID_INPUT, "qt_batch_rotation_params: more than 2^31 gate blocks");
     const int nel = 1 << (2 * dim_log2);
     qt::RotArgs r;
     std::memset(&r, 0, sizeof r);
@@ -442,6 +487,23 @@ qt_status qt_batch_rotation_params(qt_batch* b, int dim_log2, int n_rot, const q
             b->mirror_stale = true;
         }
     }
+    // every element the first time this layout is written into the current pool,
+    // afterwards only the angle-dependent ones
+    qt::RotWrites wr;
+    std::memset(&wr, 0, sizeof wr);
+    {
+        const char* raw = reinterpret_cast<const char*>(&r);
+        const bool full_done = b->rot_full_valid && b->rot_full_args.size() == sizeof r &&
+                               std::memcmp(b->rot_full_args.data(), raw, sizeof r) == 0;
+        for (int g2 = 0; g2 < n_rot; ++g2) {
+            const qt::RotScalars v1 = qt::rot_scalars(dim_log2, 0.3), v2 = qt::rot_scalars(dim_log2, 1.1);
+            for (int e = 0; e < nel; ++e) {
+                const double2 x1 = qt::rot_element(r.kind[g2], dim_log2, v1, e);
+                const double2 x2 = qt::rot_element(r.kind[g2], dim_log2, v2, e);
+                if (!full_done || x1.x != x2.x || x1.y != x2.y) wr.w[wr.n++] = uint16_t((g2 << 4) | e);
+            }
+        }
+    }
     qt_status s = QT_OK;
     if (!b->d_param_status) {
         s = check_hip(ctx, qt::batch_malloc(ctx, reinterpret_cast<void**>(&b->d_param_status), 256), "status");
@@ -466,16 +528,21 @@ qt_status qt_batch_rotation_params(qt_batch* b, int dim_log2, int n_rot, const q
         if (s != QT_OK) return s;
         d_ang = b->d_angles;
     }
-    const int64_t n_pairs = int64_t(n_sets) * n_rot;
-    const unsigned grid = unsigned((n_pairs + qt::kPairsPerBlock - 1) / qt::kPairsPerBlock);
+    const int sets_per_block = qt::kPairsPerBlock / n_rot;
+    const unsigned grid = unsigned((n_sets + sets_per_block - 1) / sets_per_block);
     if (b->precision == QT_PLAIN_C128)
-        hipLaunchKernelGGL(qt::k_rotation_params<double2>, dim3(grid), dim3(qt::kPairsPerBlock), 0, ctx->stream,
-                           static_cast<double2*>(b->d_pool), d_ang, r, b->d_param_status);
+        hipLaunchKernelGGL(qt::k_rotation_params<double2>, dim3(grid), dim3(qt::kRotThreads), 0, ctx->stream,
+                           static_cast<double2*>(b->d_pool), d_ang, r, wr, b->d_param_status);
     else
-        hipLaunchKernelGGL(qt::k_rotation_params<float2>, dim3(grid), dim3(qt::kPairsPerBlock), 0, ctx->stream,
-                           static_cast<float2*>(b->d_pool), d_ang, r, b->d_param_status);
+        hipLaunchKernelGGL(qt::k_rotation_params<float2>, dim3(grid), dim3(qt::kRotThreads), 0, ctx->stream,
+                           static_cast<float2*>(b->d_pool), d_ang, r, wr, b->d_param_status);
     s = check_hip(ctx, hipGetLastError(), "k_rotation_params");
-    if (s == QT_OK) b->params_unchecked = true;
+    if (s == QT_OK) {
+        b->params_unchecked = true;
+        const char* raw = reinterpret_cast<const char*>(&r);
+        b->rot_full_args.assign(raw, raw + sizeof r);
+        b->rot_full_valid = true;
+    }
     return s;
 }
 
