@@ -277,80 +277,34 @@ struct ThinArgs {
     int32_t small_off[16];  // small operand offset of its free part o
 };
 
-// Two adjacent elements of V, loaded and stored as one access.
-template <class V>
-struct alignas(2 * sizeof(V)) Pair {
-    V v[2];
-};
-
-// E = 2 (X's bit 0 free): each lane takes the adjacent elements 2u, 2u + 1, read and
-// written as pairs (16-byte accesses in complex64, 32 in complex128).
-template <class V, int MODE, int NS, int NB, bool SMALL_B, int E>
+template <class V, int MODE, int NS, int NB, bool SMALL_B>
 __global__ __launch_bounds__(kThreads) void k_thin(const V* __restrict__ X, const V* __restrict__ G,
                                                    V* __restrict__ C, ThinArgs r) {
     constexpr int NJ = 1 << NS, NO = 1 << NB;
     const int64_t stride = int64_t(gridDim.x) * kThreads;
-    const int64_t n_units = r.n_big / E;
-    for (int64_t u = int64_t(blockIdx.x) * kThreads + threadIdx.x; u < n_units; u += stride) {
-        const int64_t t = u * E;
+    for (int64_t t = int64_t(blockIdx.x) * kThreads + threadIdx.x; t < r.n_big; t += stride) {
         int64_t x = t;
 #pragma unroll
         for (int i = 0; i < NS; ++i) {
             const int p = r.ins_pos[i];
             x = ((x >> p) << (p + 1)) | (x & ((int64_t(1) << p) - 1));
         }
-        V xs[NJ][E];
+        V xs[NJ];
 #pragma unroll
-        for (int j = 0; j < NJ; ++j) {
-            if constexpr (E == 2) {
-                const Pair<V> pr = *reinterpret_cast<const Pair<V>*>(X + x + r.g_big[j]);
-                xs[j][0] = pr.v[0];
-                xs[j][1] = pr.v[1];
-            } else {
-                xs[j][0] = X[x + r.g_big[j]];
-            }
-        }
-        Acc<V, MODE> acc[E][NO];
+        for (int j = 0; j < NJ; ++j) xs[j] = X[x + r.g_big[j]];
+        Acc<V, MODE> acc[NO];
 #pragma unroll
         for (int j = 0; j < NJ; ++j)
 #pragma unroll
             for (int o = 0; o < NO; ++o) {
                 const V g = G[r.small_off[o] + r.g_small[j]];
-#pragma unroll
-                for (int e = 0; e < E; ++e) {
-                    if constexpr (SMALL_B) mac<V, MODE>(acc[e][o], xs[j][e], g);
-                    else mac<V, MODE>(acc[e][o], g, xs[j][e]);
-                }
+                if constexpr (SMALL_B) mac<V, MODE>(acc[o], xs[j], g);
+                else mac<V, MODE>(acc[o], g, xs[j]);
             }
-        if constexpr (SMALL_B) {
 #pragma unroll
-            for (int o = 0; o < NO; ++o) {
-                const int64_t idx = t + (int64_t(o) << r.lo_bits);
-                if constexpr (E == 2) {
-                    Pair<V> pr;
-                    pr.v[0] = finish<V, MODE>(acc[0][o]);
-                    pr.v[1] = finish<V, MODE>(acc[1][o]);
-                    *reinterpret_cast<Pair<V>*>(C + idx) = pr;
-                } else {
-                    C[idx] = finish<V, MODE>(acc[0][o]);
-                }
-            }
-        } else {
-            // the lane's E * NO outputs are contiguous: C[o + (t << NB)]
-#pragma unroll
-            for (int e = 0; e < E; ++e)
-#pragma unroll
-                for (int o = 0; o < NO; o += (NO >= 2 ? 2 : 1)) {
-                    const int64_t idx = int64_t(o) + ((t + e) << NB);
-                    if constexpr (NO >= 2) {
-                        Pair<V> pr;
-                        pr.v[0] = finish<V, MODE>(acc[e][o]);
-                        pr.v[1] = finish<V, MODE>(acc[e][o + 1]);
-                        *reinterpret_cast<Pair<V>*>(C + idx) = pr;
-                    } else {
-                        C[idx] = finish<V, MODE>(acc[e][o]);
-                    }
-                }
+        for (int o = 0; o < NO; ++o) {
+            const int64_t idx = SMALL_B ? t + (int64_t(o) << r.lo_bits) : int64_t(o) + (t << NB);
+            C[idx] = finish<V, MODE>(acc[o]);
         }
     }
 }
@@ -826,21 +780,14 @@ qt_status launch_skinny(Ctx* c, const V* A, const V* B, V* C, const BitMap& m, i
 
 template <class V, int MODE, int NS, int NB>
 qt_status launch_thin_nb(Ctx* c, const V* X, const V* G, V* C, const ThinArgs& r, bool small_b) {
-    // pairs when X's bit 0 is free (the lowest summed position is above it) and the
-    // large side has at least 2 elements
-    const bool pairs = r.ins_pos[0] > 0 && r.n_big >= 2;
-    // one unit per lane, no grid-stride reuse: on MI355X streaming kernels reach their
-    // highest HBM rates with a workgroup per 256 units (a capped grid looping over the
-    // tensor measured 10-30 % slower)
-    const unsigned grid = grid_for(pairs ? r.n_big / 2 : r.n_big, kThreads, 1u << 30);
-    if (small_b && pairs)
-        hipLaunchKernelGGL((k_thin<V, MODE, NS, NB, true, 2>), dim3(grid), dim3(kThreads), 0, c->stream, X, G, C, r);
-    else if (small_b)
-        hipLaunchKernelGGL((k_thin<V, MODE, NS, NB, true, 1>), dim3(grid), dim3(kThreads), 0, c->stream, X, G, C, r);
-    else if (pairs)
-        hipLaunchKernelGGL((k_thin<V, MODE, NS, NB, false, 2>), dim3(grid), dim3(kThreads), 0, c->stream, X, G, C, r);
+    // one element per lane, no grid-stride reuse: on MI355X streaming kernels reach
+    // their highest HBM rates with a workgroup per 256 elements (a capped grid looping
+    // over the tensor measured 10-30 % slower, two adjacent elements per lane 5-8 %)
+    const unsigned grid = grid_for(r.n_big, kThreads, 1u << 30);
+    if (small_b)
+        hipLaunchKernelGGL((k_thin<V, MODE, NS, NB, true>), dim3(grid), dim3(kThreads), 0, c->stream, X, G, C, r);
     else
-        hipLaunchKernelGGL((k_thin<V, MODE, NS, NB, false, 1>), dim3(grid), dim3(kThreads), 0, c->stream, X, G, C, r);
+        hipLaunchKernelGGL((k_thin<V, MODE, NS, NB, false>), dim3(grid), dim3(kThreads), 0, c->stream, X, G, C, r);
     return check_hip(c, hipGetLastError(), "k_thin");
 }
 

@@ -173,19 +173,24 @@ struct RotScalars {
     double c2, s2, h, ct, st;  // superoperators: cos^2(t/2), sin^2(t/2), sin t / 2, cos t, sin t
 };
 
-__host__ __device__ inline RotScalars rot_scalars(int dl, double t) {
-    RotScalars r;
+// Only what the kind's table reads: Rz superoperators cos t and sin t; Rx and Ry
+// cos^2(t/2), sin^2(t/2) and sin t / 2; amplitude gates cos(t/2) and sin(t/2).
+__host__ __device__ inline RotScalars rot_scalars(int dl, int kind, double t) {
+    RotScalars r{0, 0, 0, 0, 0};
+    if (dl == 2 && kind == QT_ROT_RZ) {
+        sincos(t, &r.st, &r.ct);
+        return r;
+    }
     double ch, sh;
     sincos(t / 2.0, &sh, &ch);
     if (dl == 2) {
         r.c2 = ch * ch;
         r.s2 = sh * sh;
-        sincos(t, &r.st, &r.ct);
+        r.st = sin(t);
         r.h = r.st / 2.0;
-    } else {  // amplitudes: cos(t/2), sin(t/2)
+    } else {
         r.c2 = ch;
         r.s2 = sh;
-        r.h = r.st = r.ct = 0;
     }
     return r;
 }
@@ -238,9 +243,9 @@ __host__ __device__ inline double2 rot_element(int kind, int dl, const RotScalar
     }
 }
 
-// 64 (set, gate) pairs per workgroup of 128 threads: many workgroups (4,096 for
-// 131,072 sets of two gates) and few elements per thread to write.
-constexpr int kPairsPerBlock = 64, kRotThreads = 128;
+// 128 (set, gate) pairs per workgroup of 128 threads: 2,048 workgroups for 131,072
+// sets of two gates, one wavefront per gate kind.
+constexpr int kPairsPerBlock = 128, kRotThreads = 128;
 
 // Element e of a superoperator block without branches (lanes of one wave hold gates of
 // different kinds): the symbol picks the value through selects.
@@ -276,19 +281,20 @@ __global__ __launch_bounds__(kRotThreads) void k_rotation_params(V* __restrict__
     const int first_set = int(blockIdx.x) * sets_per_block;
     const int sets_here = r.n_sets - first_set < sets_per_block ? r.n_sets - first_set : sets_per_block;
     const int tid = threadIdx.x;
-    if (tid < sets_here * r.n_rot) {
-        const int a = first_set + tid / r.n_rot;
-        const int rot = tid % r.n_rot;
+    // gate-major within the workgroup, so that the lanes of a wavefront usually share
+    // a gate kind (and its branch of rot_scalars)
+    const int rot = tid / sets_per_block, la = tid % sets_per_block;
+    if (rot < r.n_rot && la < sets_here) {
         int ok = 1;
-        const double x = r.scale[rot] * angles[int64_t(a) * r.n_angles + r.angle[rot]];
+        const double x = r.scale[rot] * angles[int64_t(first_set + la) * r.n_angles + r.angle[rot]];
         const float f = angle_text_float(x, &ok);
-        RotScalars v = rot_scalars(r.dl, double(f));
+        RotScalars v = rot_scalars(r.dl, r.kind[rot], double(f));
         if (!ok) {
             atomicOr(status, 1);
             const double nan = __builtin_nan("");
             v = RotScalars{nan, nan, nan, nan, nan};
         }
-        sc[tid] = v;
+        sc[la * r.n_rot + rot] = v;
     }
     __syncthreads();
     for (int idx = tid; idx < sets_here * wr.n; idx += kRotThreads) {
@@ -324,7 +330,7 @@ bool literals_survive(const qt_batch* b, const RotArgs& r) {
     };
     std::vector<std::vector<Fixed>> fixed(size_t(r.n_rot));
     for (int g = 0; g < r.n_rot; ++g) {
-        const RotScalars v1 = rot_scalars(r.dl, 0.3), v2 = rot_scalars(r.dl, 1.1);
+        const RotScalars v1 = rot_scalars(r.dl, r.kind[g], 0.3), v2 = rot_scalars(r.dl, r.kind[g], 1.1);
         for (int e = 0; e < nel; ++e) {
             double2 a = rot_element(r.kind[g], r.dl, v1, e), c = rot_element(r.kind[g], r.dl, v2, e);
             if (!c128) {
@@ -496,7 +502,8 @@ qt_status qt_batch_rotation_params(qt_batch* b, int dim_log2, int n_rot, const q
         const bool full_done = b->rot_full_valid && b->rot_full_args.size() == sizeof r &&
                                std::memcmp(b->rot_full_args.data(), raw, sizeof r) == 0;
         for (int g2 = 0; g2 < n_rot; ++g2) {
-            const qt::RotScalars v1 = qt::rot_scalars(dim_log2, 0.3), v2 = qt::rot_scalars(dim_log2, 1.1);
+            const qt::RotScalars v1 = qt::rot_scalars(dim_log2, r.kind[g2], 0.3);
+            const qt::RotScalars v2 = qt::rot_scalars(dim_log2, r.kind[g2], 1.1);
             for (int e = 0; e < nel; ++e) {
                 const double2 x1 = qt::rot_element(r.kind[g2], dim_log2, v1, e);
                 const double2 x2 = qt::rot_element(r.kind[g2], dim_log2, v2, e);
