@@ -307,8 +307,8 @@ def run_uneven(args, qa, np):
 def run_serving(args, qa, np):
     """Poisson arrivals at a fixed rate, from a pool of structures planned before the
     timed region; requests run in the continuous-batching server (complex64 4M for
-    the batched routes, complex128 on the large-network route).  Latency = submission
-    to result.
+    the batched routes, complex128 on the large-network route unless --eager-form
+    says otherwise).  Latency = submission to result.
     * mix "light": 70% QAOA energies (3-regular graphs with p = 1 or 2, 4-regular with
       p = 1; 20-200 vertices) and 30% single amplitudes of grid random circuits (16-24
       qubits, depth 8): every request fits a batched program.
@@ -347,7 +347,8 @@ def run_serving(args, qa, np):
     # 64 GiB of the 288 GiB HBM for the requests admitted at once: large light cones
     # (8-9 GiB of live tensors per edge network) run two edge networks side by side, two
     # such requests at a time
-    srv = qa.Server(slots=2, precision=qa.QT_C64_4M, hbm_budget=64 << 30)
+    eager = {"c128": qa.QT_PLAIN_C128, "c64_4m": qa.QT_C64_4M}[args.eager_form]
+    srv = qa.Server(slots=2, precision=qa.QT_C64_4M, hbm_budget=64 << 30, eager_precision=eager)
 
     def submit(r):
         return srv.submit_qaoa(r[1], r[2], r[3]) if r[0] == "qaoa" else srv.submit_amplitude(r[1], r[2])
@@ -404,18 +405,36 @@ def run_serving(args, qa, np):
         e, _ = qa.qaoa_energy_plain(ctx, reqs[k][1], reqs[k][2], False, list(reqs[k][3]), seed=k)
         errs.append(abs(res[k]["value"].real - e) / abs(e))
     plain_ms = 1e3 * (time.perf_counter() - t1) / max(len(sample), 1)
+    # amplitudes served on the large-network route against complex128 (one request of
+    # each circuit, up to three), when that route runs a complex64 form
+    amp_errs = []
+    if args.eager_form != "c128":
+        done_circ = set()
+        for k in range(n_req):
+            r = reqs[k]
+            if r[0] != "amp" or res[k]["status"] != 0 or res[k]["route"] != "eager" or r[4] in done_circ:
+                continue
+            done_circ.add(r[4])
+            sn = qa.SlicedNetwork(r[1], r[2], dim_log2=1, order=qa.ORDER_SWEEP, max_bits=30)
+            ref, _ = sn.run(ctx, qa.QT_PLAIN_C128)
+            sn.close()
+            amp_errs.append(abs(res[k]["value"] - ref) / abs(ref))
+            if len(done_circ) == 3:
+                break
     edge_evals = st["edge_evals"] - base["edge_evals"]
     return {"metric": "serving requests/s (p50/p99 latency) under Poisson arrivals", "value": ok / wall,
             "unit": "requests/s", "n_gpus": 1, "steps": n_req, "warmup": len(seen), "ms_per_step": 1e3 * wall / n_req,
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "c64", "data": "synthetic",
             "config": {"workload": "serving_mix_qaoa70_amplitude30", "mix": args.mix, "qps": args.qps,
-                       "duration_s": args.duration, "slots": 2, "hbm_budget_gib": 64, "structures": len(seen)},
+                       "duration_s": args.duration, "slots": 2, "hbm_budget_gib": 64, "structures": len(seen),
+                       "large_network_form": args.eager_form},
             "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
             "completed": ok, "refused": n_req - ok, "routes": routes, "classes": classes,
             "edge_evals_per_s": edge_evals / wall,
             "launches": st["launches"] - base["launches"], "overlap_ms": st["overlap_ms"] - base["overlap_ms"],
             "busy_ms": st["busy_ms"] - base["busy_ms"], "peak_planned_bytes": st["peak_planned_bytes"],
-            "plain_c128_ms_per_qaoa_request": plain_ms, "max_energy_rel_err_vs_plain_c128": max(errs) if errs else None}
+            "plain_c128_ms_per_qaoa_request": plain_ms, "max_energy_rel_err_vs_plain_c128": max(errs) if errs else None,
+            "large_route_amplitude_rel_err_vs_c128": max(amp_errs) if amp_errs else None}
 
 
 def main():
@@ -452,6 +471,8 @@ def main():
     ap.add_argument("--qps", type=float, default=400.0, help="serving: arrival rate")
     ap.add_argument("--mix", default="light", choices=["light", "config"], help="serving: request mix")
     ap.add_argument("--duration", type=float, default=10.0, help="serving: seconds of arrivals")
+    ap.add_argument("--eager-form", default="c128", choices=["c128", "c64_4m"],
+                    help="serving: arithmetic form of the large-network route")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine measures one GPU; --gpus must be 1"}))

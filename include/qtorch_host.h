@@ -195,7 +195,8 @@ int qth_netbatch_destroy(qth_netbatch* b);
  * Requests are submitted from any thread and finish asynchronously.  Each request's
  * network is planned once per structure and then runs as a program instance in the
  * next launch of an idle stream (or, when its tensors exceed a program's limit, on
- * the large-network path in complex128: sliced, small subtrees fused, MFMA GEMMs),
+ * the large-network path, complex128 by default: sliced, small subtrees fused, MFMA
+ * GEMMs),
  * admitted while the planned device bytes of the
  * requests in flight fit hbm_budget.  Results do not depend on what else shares a
  * launch. */
@@ -204,6 +205,11 @@ enum { QTH_SERVE_QAOA_ENERGY = 0, QTH_SERVE_AMPLITUDE = 1 };
 enum { QTH_ROUTE_SMALL = 0, QTH_ROUTE_MEDIUM = 1, QTH_ROUTE_EAGER = 2 };
 int qth_server_create(int device, int slots, int precision, int64_t hbm_budget, int max_requests_per_launch,
                       qth_server** out);
+/* The same with the large-network path's arithmetic form (complex128 in
+ * qth_server_create; a complex64 form halves its bytes and runs its GEMMs on FP32
+ * MFMA, with that form's error). */
+int qth_server_create_ex(int device, int slots, int precision, int eager_precision, int64_t hbm_budget,
+                         int max_requests_per_launch, qth_server** out);
 int qth_server_destroy(qth_server* s); /* finishes the submitted requests first */
 /* QAOA energy F_p of a .dgf graph for one angle set [betas..., gammas...] (2p doubles). */
 int qth_server_submit_qaoa(qth_server* s, const char* dgf, int p, int ref_map, const double* betas_gammas,

@@ -158,6 +158,8 @@ _HOST_SIGS = {
     "qth_netbatch_cmacs": (_c.c_int, [_vp, _i64p]),
     "qth_netbatch_destroy": (_c.c_int, [_vp]),
     "qth_server_create": (_c.c_int, [_c.c_int, _c.c_int, _c.c_int, _c.c_int64, _c.c_int, _c.POINTER(_vp)]),
+    "qth_server_create_ex": (_c.c_int, [_c.c_int, _c.c_int, _c.c_int, _c.c_int, _c.c_int64, _c.c_int,
+                                        _c.POINTER(_vp)]),
     "qth_server_destroy": (_c.c_int, [_vp]),
     "qth_server_submit_qaoa": (_c.c_int, [_vp, _c.c_char_p, _c.c_int, _c.c_int, _c.POINTER(_c.c_double),
                                           _c.POINTER(_c.c_int64)]),
@@ -867,10 +869,10 @@ class Server:
     awaited by id."""
 
     def __init__(self, device: int = 0, slots: int = 2, precision: int = QT_C64_4M, hbm_budget: int = 16 << 30,
-                 max_requests_per_launch: int = 512):
+                 max_requests_per_launch: int = 512, eager_precision: int = QT_PLAIN_C128):
         h = _vp()
-        _host_check(host.qth_server_create(device, slots, precision, hbm_budget, max_requests_per_launch,
-                                           ctypes.byref(h)))
+        _host_check(host.qth_server_create_ex(device, slots, precision, eager_precision, hbm_budget,
+                                              max_requests_per_launch, ctypes.byref(h)))
         self.handle = h
 
     def submit_qaoa(self, dgf: str, p: int, betas_gammas: Sequence[float], ref_map: bool = False) -> int:

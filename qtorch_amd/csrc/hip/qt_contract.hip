@@ -802,62 +802,58 @@ qt_status launch_thin_ns(Ctx* c, const V* X, const V* G, V* C, const ThinArgs& r
     }
 }
 
-// k_thin when it applies (plain and 4M forms, 1-4 summed bits, at most 4 bits on the
-// small side, the large side's free bits in their own order in C); false otherwise.
+// k_thin when it applies (1-4 summed bits, at most 4 bits on the small side, the
+// large side's free bits in their own order in C); false otherwise.
 template <class V, int MODE>
 bool try_thin(Ctx* c, const V* A, const V* B, V* C, const BitMap& m, int mbits, bool small_b, qt_status* st) {
-    if constexpr (MODE != MODE_PLAIN && MODE != MODE_4M) {
-        return false;
-    } else {
-        const int nb = small_b ? m.nbits_c - mbits : mbits;
-        const int ns = m.nbits_s;
-        if (ns < 1 || ns > 4 || nb > 4) return false;
-        // the large side's C bits, in C order, must be its free bits in increasing order
-        const int lo = small_b ? 0 : nb, hi = small_b ? mbits : m.nbits_c;
-        int prev = -1;
-        for (int i = lo; i < hi; ++i) {
-            const int src = small_b ? m.c_src[i] : m.c_src[i] - 64;
-            if (src <= prev) return false;
-            prev = src;
-        }
-        ThinArgs r;
-        std::memset(&r, 0, sizeof r);
-        r.n_big = int64_t(1) << (hi - lo);
-        r.lo_bits = mbits;
-        int pos[4];
-        for (int d = 0; d < ns; ++d) pos[d] = small_b ? m.s_a[d] : m.s_b[d];
-        std::sort(pos, pos + ns);
-        r.n_ins = ns;
-        for (int d = 0; d < ns; ++d) r.ins_pos[d] = pos[d];
-        for (int j = 0; j < (1 << ns); ++j) {
-            int64_t gb = 0, gs = 0;
-            for (int d = 0; d < ns; ++d)
-                if ((j >> d) & 1) {
-                    gb += int64_t(1) << (small_b ? m.s_a[d] : m.s_b[d]);
-                    gs += int64_t(1) << (small_b ? m.s_b[d] : m.s_a[d]);
-                }
-            r.g_big[j] = gb;
-            r.g_small[j] = int32_t(gs);
-        }
-        for (int o = 0; o < (1 << nb); ++o) {
-            int64_t off = 0;
-            for (int i = 0; i < nb; ++i)
-                if ((o >> i) & 1) {
-                    const int src = small_b ? m.c_src[mbits + i] - 64 : m.c_src[i];
-                    off += int64_t(1) << src;
-                }
-            r.small_off[o] = int32_t(off);
-        }
-        const V* X = small_b ? A : B;
-        const V* G = small_b ? B : A;
-        switch (ns) {
-            case 1: *st = launch_thin_ns<V, MODE, 1>(c, X, G, C, r, nb, small_b); break;
-            case 2: *st = launch_thin_ns<V, MODE, 2>(c, X, G, C, r, nb, small_b); break;
-            case 3: *st = launch_thin_ns<V, MODE, 3>(c, X, G, C, r, nb, small_b); break;
-            default: *st = launch_thin_ns<V, MODE, 4>(c, X, G, C, r, nb, small_b); break;
-        }
-        return true;
+    const int nb = small_b ? m.nbits_c - mbits : mbits;
+    const int ns = m.nbits_s;
+    if (ns < 1 || ns > 4 || nb > 4) return false;
+    // the large side's C bits, in C order, must be its free bits in increasing order
+    const int lo = small_b ? 0 : nb, hi = small_b ? mbits : m.nbits_c;
+    int prev = -1;
+    for (int i = lo; i < hi; ++i) {
+        const int src = small_b ? m.c_src[i] : m.c_src[i] - 64;
+        if (src <= prev) return false;
+        prev = src;
     }
+    ThinArgs r;
+    std::memset(&r, 0, sizeof r);
+    r.n_big = int64_t(1) << (hi - lo);
+    r.lo_bits = mbits;
+    int pos[4];
+    for (int d = 0; d < ns; ++d) pos[d] = small_b ? m.s_a[d] : m.s_b[d];
+    std::sort(pos, pos + ns);
+    r.n_ins = ns;
+    for (int d = 0; d < ns; ++d) r.ins_pos[d] = pos[d];
+    for (int j = 0; j < (1 << ns); ++j) {
+        int64_t gb = 0, gs = 0;
+        for (int d = 0; d < ns; ++d)
+            if ((j >> d) & 1) {
+                gb += int64_t(1) << (small_b ? m.s_a[d] : m.s_b[d]);
+                gs += int64_t(1) << (small_b ? m.s_b[d] : m.s_a[d]);
+            }
+        r.g_big[j] = gb;
+        r.g_small[j] = int32_t(gs);
+    }
+    for (int o = 0; o < (1 << nb); ++o) {
+        int64_t off = 0;
+        for (int i = 0; i < nb; ++i)
+            if ((o >> i) & 1) {
+                const int src = small_b ? m.c_src[mbits + i] - 64 : m.c_src[i];
+                off += int64_t(1) << src;
+            }
+        r.small_off[o] = int32_t(off);
+    }
+    const V* X = small_b ? A : B;
+    const V* G = small_b ? B : A;
+    switch (ns) {
+        case 1: *st = launch_thin_ns<V, MODE, 1>(c, X, G, C, r, nb, small_b); break;
+        case 2: *st = launch_thin_ns<V, MODE, 2>(c, X, G, C, r, nb, small_b); break;
+        case 3: *st = launch_thin_ns<V, MODE, 3>(c, X, G, C, r, nb, small_b); break;
+        default: *st = launch_thin_ns<V, MODE, 4>(c, X, G, C, r, nb, small_b); break;
+    }
+    return true;
 }
 
 // C = [A part (mbits) | B part]; the side with at most 5 bits is the small one

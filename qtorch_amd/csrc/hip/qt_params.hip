@@ -277,10 +277,22 @@ __global__ __launch_bounds__(kRotThreads) void k_rotation_params(V* __restrict__
                                                                     RotArgs r, RotWrites wr,
                                                                     int* __restrict__ status) {
     __shared__ RotScalars sc[kPairsPerBlock];
+    // the write list decoded once per workgroup: (gate << 4 | element, element offset
+    // in a set's region, symbol table or kind); the loop below then reads LDS only
+    extern __shared__ int64_t rot_lds[];  // wr.n entries of each (rot_lds_bytes)
+    int64_t* w_off = rot_lds;
+    uint64_t* w_pack = reinterpret_cast<uint64_t*>(rot_lds + wr.n);
+    int32_t* w_rot_e = reinterpret_cast<int32_t*>(rot_lds + 2 * wr.n);
     const int sets_per_block = kPairsPerBlock / r.n_rot;
     const int first_set = int(blockIdx.x) * sets_per_block;
     const int sets_here = r.n_sets - first_set < sets_per_block ? r.n_sets - first_set : sets_per_block;
     const int tid = threadIdx.x;
+    for (int k = tid; k < wr.n; k += kRotThreads) {
+        const int w = wr.w[k], g = w >> 4, e = w & 15, kind = r.kind[g];
+        w_rot_e[k] = w;
+        w_off[k] = r.offset[g] + e;
+        w_pack[k] = r.dl == 1 ? uint64_t(kind) : kind == QT_ROT_RX ? kPackRx : kind == QT_ROT_RY ? kPackRy : kPackRz;
+    }
     // gate-major within the workgroup, so that the lanes of a wavefront usually share
     // a gate kind (and its branch of rot_scalars)
     const int rot = tid / sets_per_block, la = tid % sets_per_block;
@@ -297,23 +309,24 @@ __global__ __launch_bounds__(kRotThreads) void k_rotation_params(V* __restrict__
         sc[la * r.n_rot + rot] = v;
     }
     __syncthreads();
-    for (int idx = tid; idx < sets_here * wr.n; idx += kRotThreads) {
-        const int la = idx / wr.n;
-        const int w = wr.w[idx - la * wr.n];
-        const int rot = w >> 4, e = w & 15;
-        const int kind = r.kind[rot];
-        const RotScalars& v = sc[la * r.n_rot + rot];
-        double2 val;
-        if (r.dl == 2) {
-            const uint64_t pack = kind == QT_ROT_RX ? kPackRx : kind == QT_ROT_RY ? kPackRy : kPackRz;
-            val = rot_element_select(pack, v, e);
-        } else {
-            val = rot_element(kind, 1, v, e);
+    // entry idx = set la, list entry k (idx = la n + k), stepped without division
+    const int n = wr.n;
+    int wl = tid / n, wk = tid - (tid / n) * n;
+    const int step_l = kRotThreads / n, step_k = kRotThreads - step_l * n;
+    V* const region = pool + r.base + int64_t(first_set) * r.stride;
+    for (; wl < sets_here; wl += step_l, wk += step_k) {
+        if (wk >= n) {
+            wk -= n;
+            ++wl;
+            if (wl >= sets_here) break;
         }
+        const int w = w_rot_e[wk];
+        const RotScalars& v = sc[wl * r.n_rot + (w >> 4)];
+        const double2 val = r.dl == 2 ? rot_element_select(w_pack[wk], v, w & 15) : rot_element(int(w_pack[wk]), 1, v, w & 15);
         V out;
         out.x = decltype(out.x)(val.x);
         out.y = decltype(out.y)(val.y);
-        pool[r.base + int64_t(first_set + la) * r.stride + r.offset[rot] + e] = out;
+        region[int64_t(wl) * r.stride + w_off[wk]] = out;
     }
 }
 
@@ -535,13 +548,15 @@ qt_status qt_batch_rotation_params(qt_batch* b, int dim_log2, int n_rot, const q
         if (s != QT_OK) return s;
         d_ang = b->d_angles;
     }
+    if (wr.n == 0) return QT_OK;  // nothing depends on the angles (not expected for rotations)
     const int sets_per_block = qt::kPairsPerBlock / n_rot;
+    const size_t lds = size_t(wr.n) * 20;  // k_rotation_params' decoded write list
     const unsigned grid = unsigned((n_sets + sets_per_block - 1) / sets_per_block);
     if (b->precision == QT_PLAIN_C128)
-        hipLaunchKernelGGL(qt::k_rotation_params<double2>, dim3(grid), dim3(qt::kRotThreads), 0, ctx->stream,
+        hipLaunchKernelGGL(qt::k_rotation_params<double2>, dim3(grid), dim3(qt::kRotThreads), lds, ctx->stream,
                            static_cast<double2*>(b->d_pool), d_ang, r, wr, b->d_param_status);
     else
-        hipLaunchKernelGGL(qt::k_rotation_params<float2>, dim3(grid), dim3(qt::kRotThreads), 0, ctx->stream,
+        hipLaunchKernelGGL(qt::k_rotation_params<float2>, dim3(grid), dim3(qt::kRotThreads), lds, ctx->stream,
                            static_cast<float2*>(b->d_pool), d_ang, r, wr, b->d_param_status);
     s = check_hip(ctx, hipGetLastError(), "k_rotation_params");
     if (s == QT_OK) {

@@ -540,6 +540,26 @@ int qth_server_create(int device, int slots, int precision, int64_t hbm_budget, 
     });
 }
 
+int qth_server_create_ex(int device, int slots, int precision, int eager_precision, int64_t hbm_budget,
+                         int max_requests_per_launch, qth_server** out) {
+    if (!out || (eager_precision != QT_PLAIN_C128 && eager_precision != QT_C64_4M && eager_precision != QT_C64_3M &&
+                 eager_precision != QT_C64_BF16X3))
+        return QTH_INVALID_FUNCTION_INPUT;
+    *out = nullptr;
+    return Guard([&] {
+        ServeOptions o;
+        o.device = device;
+        o.slots = slots;
+        o.precision = precision;
+        o.eager_precision = eager_precision;
+        o.hbm_budget = hbm_budget;
+        o.max_requests_per_launch = max_requests_per_launch;
+        auto h = std::make_unique<qth_server>();
+        h->s = std::make_unique<Server>(o);
+        *out = h.release();
+    });
+}
+
 int qth_server_destroy(qth_server* s) {
     if (!s) return QTH_INVALID_FUNCTION_INPUT;
     return Guard([&] { delete s; });

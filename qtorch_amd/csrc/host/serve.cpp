@@ -671,8 +671,8 @@ void Server::EagerLoop(int k) {
         res.route = kRouteEager;
         res.planned_bytes = p->st->planned_bytes;
         try {
-            // the large-network path in complex128: fused small subtrees, MFMA GEMMs for
-            // the large steps, FP64 sums of slices
+            // the large-network path (complex128 unless set otherwise): fused small
+            // subtrees, MFMA GEMMs for the large steps, FP64 sums of slices
             if (p->req.kind == kServeQaoaEnergy) {
                 // edge networks on eager_workers threads, one stream each; the per-edge
                 // values are summed in edge order afterwards, so the energy does not
@@ -693,7 +693,7 @@ void Server::EagerLoop(int k) {
                             for (const auto& [leaf, off] : p->st->eager_param_leaves[size_t(e)])
                                 sn.SetLeaf(leaves, leaf,
                                            std::vector<std::complex<double>>(block.begin() + off, block.begin() + off + 16));
-                            zz[size_t(e)] = sn.Run(ctxs[w], QT_PLAIN_C128, nullptr, leaves).real();
+                            zz[size_t(e)] = sn.Run(ctxs[w], opt_.eager_precision, nullptr, leaves).real();
                         }
                     } catch (...) {
                         errs[w] = std::current_exception();
@@ -717,7 +717,7 @@ void Server::EagerLoop(int k) {
                 res.re = f;
             } else {
                 const SlicedNetwork& sn = *p->st->eager_nets[0];
-                const auto v = sn.Run(ctxs[0], QT_PLAIN_C128, nullptr, sn.MakeLeafState(p->req.text, p->req.bits));
+                const auto v = sn.Run(ctxs[0], opt_.eager_precision, nullptr, sn.MakeLeafState(p->req.text, p->req.bits));
                 res.re = v.real();
                 res.im = v.imag();
             }

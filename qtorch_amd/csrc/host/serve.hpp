@@ -23,6 +23,7 @@ struct ServeOptions {
     int device = 0;
     int slots = 2;                      // streams that run batched iterations side by side
     int precision = QT_C64_4M;          // arithmetic form of the batched routes
+    int eager_precision = QT_PLAIN_C128;  // arithmetic form of the large-network path
     int64_t hbm_budget = 16LL << 30;    // planned device bytes of the requests admitted at once
     int max_requests_per_launch = 512;
     int64_t small_scratch = 768;        // elements: programs up to this run in lane-grouped waves (LDS)

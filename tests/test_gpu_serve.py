@@ -155,6 +155,23 @@ def test_large_network_takes_the_large_network_path():
     assert st["peak_planned_bytes"] >= plan["planned_bytes"]
 
 
+def test_large_network_path_in_complex64():
+    """The large-network route in the complex64 4M form (qth_server_create_ex): the
+    20-qubit depth-14 grid amplitude and a large light cone's energy within that
+    form's error of a state-vector simulation (1e-5 relative)."""
+    q = wl.grid_random_circuit(4, 5, 14, seed=1)
+    bits = wl.bitstring(20, 3)
+    srv = qa.Server(slots=2, eager_precision=qa.QT_C64_4M)
+    try:
+        res = srv.wait(srv.submit_amplitude(q, bits))
+    finally:
+        srv.close()
+    assert res["status"] == 0 and res["route"] == "eager"
+    psi, n = sv.simulate(q)
+    ref = psi[_amp_index(bits)]
+    assert abs(res["value"] - ref) <= 1e-5 * max(abs(ref), 2.0 ** (-n / 2))
+
+
 def test_admission_rejects_what_cannot_fit():
     """A request whose planned device bytes exceed the whole budget is refused
     (QTH_OTHER) without touching the device; others still run."""
