@@ -440,8 +440,8 @@ def run_serving(args, qa, np):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--sets", type=int, default=131072, help="angle sets per step")
     ap.add_argument("--form", default="c128", choices=["c64_4m", "c64_3m", "c64_bf16x3", "c128"])
     ap.add_argument("--per-edge", action="store_true",
@@ -546,10 +546,6 @@ def main():
     graph = graph_ms is not None and graph_ms < direct_ms
     if graph_ms is not None and not graph:
         b.uncapture()
-    for _ in range(args.warmup):
-        b.launch()
-    b.flush()
-    ctx.sync()
     ne = b.n_edges
     # an optimiser supplies new angles for every evaluation: each step first rewrites
     # the sets' Rz / Rx blocks on the device from a fresh array of angles already in
@@ -576,6 +572,12 @@ def main():
         ctx.sync()
         lat.append(time.perf_counter() - t0)
     ctx.sync()
+    # warm-up steps right before the timed ones (the device clocks settle under load)
+    for i in range(args.warmup):
+        b.update_angles(device_ptr=ptrs[i % args.steps], n_sets=args.sets)
+        b.launch()
+    b.flush()
+    ctx.sync()
     t0 = time.perf_counter()
     for i in range(args.steps):
         b.update_angles(device_ptr=ptrs[i], n_sets=args.sets)
@@ -585,6 +587,10 @@ def main():
     wall = time.perf_counter() - t0
     e_fresh, _ = b.results()  # the last step's angles
     # the same steps with the angles left as they are (the same sets every step)
+    for _ in range(args.warmup):
+        b.launch()
+    b.flush()
+    ctx.sync()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         b.launch()
@@ -607,8 +613,11 @@ def main():
     fresh_err = float(np.max(np.abs(e_fresh - e_chk) / np.abs(e_chk)))
     b.update_angles(sets)  # back to the sets the accuracy checks below use
     del fresh
-    launch_ms = b.time(args.steps)  # HIP events on the engine's stream around whole launches
-    kernel_ms = b.time_kernel(args.steps)  # the same for the contraction kernel alone
+    # HIP events on the engine's stream around whole launches, then around the
+    # contraction kernel alone, each after launches that bring the clocks up
+    b.time(args.warmup)
+    launch_ms = b.time(max(args.steps, 200))
+    kernel_ms = b.time_kernel(max(args.steps, 200))
     energies, _ = b.results()
     if sums_only:  # one launch that writes every edge's value, for the accuracy checks
         b.sums_only(False)

@@ -1244,8 +1244,10 @@ qt_status qt_batch_launch(qt_batch* b) {
     if (b->n_inst == 0) return QT_OK;
     s = b->graph_exec ? check_hip(ctx, hipGraphLaunch(b->graph_exec, ctx->stream), "hipGraphLaunch")
                                       : qt::launch_batch(b);
-    if (s != QT_OK) return s;
-    return check_hip(ctx, hipEventRecord(b->ev_done, ctx->stream), "event record");
+    // qt_batch_poll records the completion event when it is asked: an event recorded
+    // after every launch costs the device about 5 us between consecutive launches
+    if (s == QT_OK) b->ev_pending = true;
+    return s;
 }
 
 int64_t qt_batch_resident_bytes(const qt_batch* b) {
@@ -1277,11 +1279,12 @@ qt_status qt_batch_set_sums_only(qt_batch* b, int on, int* active) {
 qt_status qt_batch_poll(qt_batch* b, int* done) {
     if (!b || !done) return QT_INVALID_INPUT;
     std::lock_guard<std::recursive_mutex> g(b->ctx->mu);
-    if (b->sums_pending) {  // the last launch is done once its group sums are
+    if (b->sums_pending || b->ev_pending) {  // the last launch is done once its group sums are
         (void)hipSetDevice(b->ctx->device);
         qt_status s = qt::flush_sums(b);
         if (s == QT_OK) s = check_hip(b->ctx, hipEventRecord(b->ev_done, b->ctx->stream), "event record");
         if (s != QT_OK) return s;
+        b->ev_pending = false;
     }
     const hipError_t e = hipEventQuery(b->ev_done);
     if (e == hipErrorNotReady) {
@@ -1471,7 +1474,7 @@ qt_status qt_batch_destroy(qt_batch* b) {
                         (void*)b->d_inst, (void*)b->d_out, (void*)b->d_counter, (void*)b->d_seg, (void*)b->d_members,
                         (void*)b->d_group, b->d_global_scratch, (void*)b->d_chunks, b->d_pool_stage,
                         (void*)b->d_leaf_lds, (void*)b->d_csts, (void*)b->d_tout, (void*)b->d_schunks,
-                        (void*)b->d_out_prev, (void*)b->d_param_status, (void*)b->d_angles})
+                        (void*)b->d_out_prev, (void*)b->d_param_status, (void*)b->d_angles, b->d_rot_desc})
             qt::batch_free(b->ctx, p);
         --b->ctx->live;
     }

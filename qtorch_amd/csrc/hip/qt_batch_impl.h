@@ -123,6 +123,7 @@ struct qt_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_done = nullptr;
+    bool ev_pending = false;  // launches since ev_done was last recorded
     // host copies of the device program arrays (programs can be appended)
     std::vector<qt::ProgHdr> h_hdr;
     std::vector<qt::StepDesc> h_steps;
@@ -184,6 +185,8 @@ struct qt_batch {
     std::vector<char> rot_checked_args;
     bool rot_full_valid = false;      // every element of the rot_full_args layout holds its table value
     std::vector<char> rot_full_args;
+    void* d_rot_desc = nullptr;       // the last launch's gate description (device copy)
+    std::vector<char> rot_desc_host;  // and its host copy
 };
 
 namespace qt {

@@ -67,8 +67,11 @@ __host__ __device__ float angle_text_float(double x, int* ok) {
         return __builtin_nanf("");
     }
     // N = a * 10^(5 - e) rounded to an integer (half to even), e = floor(log10(a)):
-    // 10^5 <= N <= 10^6.  log10's estimate is corrected by exact comparisons.
-    int e = int(floor(log10(a)));
+    // 10^5 <= N <= 10^6.  With a in [2^(b-1), 2^b), floor((b - 1) log10 2) is e or one
+    // less; the exact comparisons below correct it.
+    int b2 = 0;
+    (void)frexp(a, &b2);
+    int e = int(floor(double(b2 - 1) * 0.30102999566398120));
     double n_dig = 0;
     for (int iter = 0; iter < 4; ++iter) {
         const int m = 5 - e;  // in [-10, 22]
@@ -167,6 +170,13 @@ struct RotWrites {
     uint16_t w[kMaxWrites];
 };
 
+// What a launch reads besides the angles, kept in device memory (uploaded when it
+// changes): as kernel arguments its 1.9 KB made every launch start about 5 us late.
+struct RotDesc {
+    RotArgs r;
+    RotWrites wr;
+};
+
 // Values a gate block takes from its angle t (the parsed float, widened): the
 // expressions of the host's gate tables.
 struct RotScalars {
@@ -174,7 +184,8 @@ struct RotScalars {
 };
 
 // Only what the kind's table reads: Rz superoperators cos t and sin t; Rx and Ry
-// cos^2(t/2), sin^2(t/2) and sin t / 2; amplitude gates cos(t/2) and sin(t/2).
+// cos^2(t/2), sin^2(t/2) and sin t / 2 (as sin(t/2) cos(t/2): one sincos); amplitude
+// gates cos(t/2) and sin(t/2).
 __host__ __device__ inline RotScalars rot_scalars(int dl, int kind, double t) {
     RotScalars r{0, 0, 0, 0, 0};
     if (dl == 2 && kind == QT_ROT_RZ) {
@@ -186,8 +197,8 @@ __host__ __device__ inline RotScalars rot_scalars(int dl, int kind, double t) {
     if (dl == 2) {
         r.c2 = ch * ch;
         r.s2 = sh * sh;
-        r.st = sin(t);
-        r.h = r.st / 2.0;
+        r.h = sh * ch;  // sin t / 2 (within an ulp or so of the host's sin(t) / 2.0)
+        r.st = 2.0 * r.h;
     } else {
         r.c2 = ch;
         r.s2 = sh;
@@ -274,8 +285,10 @@ __device__ inline double2 rot_element_select(uint64_t pack, const RotScalars& v,
 template <class V>
 __global__ __launch_bounds__(kRotThreads) void k_rotation_params(V* __restrict__ pool,
                                                                     const double* __restrict__ angles,
-                                                                    RotArgs r, RotWrites wr,
+                                                                    const RotDesc* __restrict__ desc,
                                                                     int* __restrict__ status) {
+    const RotArgs& r = desc->r;
+    const RotWrites& wr = desc->wr;
     __shared__ RotScalars sc[kPairsPerBlock];
     // the write list decoded once per workgroup: (gate << 4 | element, element offset
     // in a set's region, symbol table or kind); the loop below then reads LDS only
@@ -552,12 +565,33 @@ qt_status qt_batch_rotation_params(qt_batch* b, int dim_log2, int n_rot, const q
     const int sets_per_block = qt::kPairsPerBlock / n_rot;
     const size_t lds = size_t(wr.n) * 20;  // k_rotation_params' decoded write list
     const unsigned grid = unsigned((n_sets + sets_per_block - 1) / sets_per_block);
+    // the description in device memory, uploaded when it changes (the first calls of a
+    // layout; a synchronous copy, so the host copy may change afterwards)
+    {
+        qt::RotDesc desc;
+        std::memset(&desc, 0, sizeof desc);
+        desc.r = r;
+        desc.wr = wr;
+        const char* raw = reinterpret_cast<const char*>(&desc);
+        if (!b->d_rot_desc || b->rot_desc_host.size() != sizeof desc ||
+            std::memcmp(b->rot_desc_host.data(), raw, sizeof desc) != 0) {
+            if (!b->d_rot_desc)
+                s = check_hip(ctx, qt::batch_malloc(ctx, &b->d_rot_desc, sizeof desc), "rotation description");
+            if (s == QT_OK) s = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync");
+            if (s == QT_OK)
+                s = check_hip(ctx, hipMemcpy(b->d_rot_desc, &desc, sizeof desc, hipMemcpyHostToDevice),
+                              "rotation description");
+            if (s != QT_OK) return s;
+            b->rot_desc_host.assign(raw, raw + sizeof desc);
+        }
+    }
+    const auto* dd = static_cast<const qt::RotDesc*>(b->d_rot_desc);
     if (b->precision == QT_PLAIN_C128)
         hipLaunchKernelGGL(qt::k_rotation_params<double2>, dim3(grid), dim3(qt::kRotThreads), lds, ctx->stream,
-                           static_cast<double2*>(b->d_pool), d_ang, r, wr, b->d_param_status);
+                           static_cast<double2*>(b->d_pool), d_ang, dd, b->d_param_status);
     else
         hipLaunchKernelGGL(qt::k_rotation_params<float2>, dim3(grid), dim3(qt::kRotThreads), lds, ctx->stream,
-                           static_cast<float2*>(b->d_pool), d_ang, r, wr, b->d_param_status);
+                           static_cast<float2*>(b->d_pool), d_ang, dd, b->d_param_status);
     s = check_hip(ctx, hipGetLastError(), "k_rotation_params");
     if (s == QT_OK) {
         b->params_unchecked = true;
