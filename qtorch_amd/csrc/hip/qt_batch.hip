@@ -515,6 +515,7 @@ qt_status launch_tables(qt_batch* b) {
 }
 
 qt_status launch_batch(qt_batch* b, bool capture = false) {
+    b->in_use = true;  // queued work reads the batch's arrays
     Ctx* c = b->ctx;
     // a failed reconfiguration leaves scratch or chunk arrays that do not fit the shape
     if (!b->shape_ok) return set_error(c, QT_INVALID_INPUT, "batch: executor shape not configured (last change failed)");
@@ -673,7 +674,7 @@ qt_status append_programs(qt_batch* b, int n, qt_program* const* progs, const in
     }
     b->n_programs += n;
     drop_specialised(b);  // generated for the previous program set
-    (void)hipStreamSynchronize(c->stream);  // a launch in flight reads the arrays
+    quiesce(b);  // a launch in flight reads the arrays
     qt_status s = dev_put(c, &b->d_progs, &b->cap_progs, b->h_hdr.data(), b->h_hdr.size());
     if (s == QT_OK) s = dev_put(c, &b->d_steps, &b->cap_steps, b->h_steps.data(), b->h_steps.size());
     if (s == QT_OK) s = dev_put(c, &b->d_tabs, &b->cap_tabs, b->h_tabs.data(), b->h_tabs.size());
@@ -731,21 +732,11 @@ qt_status write_pool(qt_batch* b, int64_t offset, const double* values, int64_t 
     if (b->precision == QT_PLAIN_C128) {
         s = check_hip(c, hipMemcpy(dst, values, size_t(n) * 16, hipMemcpyHostToDevice), "leaf pool copy");
     } else {
-        if (n > b->pool_stage_cap) {
-            batch_free(c, b->d_pool_stage);
-            b->d_pool_stage = nullptr;
-            b->pool_stage_cap = 0;
-            s = check_hip(c, batch_malloc(c, &b->d_pool_stage, size_t(n) * 16), "leaf pool staging");
-            if (s != QT_OK) return s;
-            b->pool_stage_cap = n;
-        }
-        s = check_hip(c, hipMemcpy(b->d_pool_stage, values, size_t(n) * 16, hipMemcpyHostToDevice), "copy");
-        if (s == QT_OK) {
-            hipLaunchKernelGGL(k_to_c64, dim3(unsigned((n + 255) / 256)), dim3(256), 0, c->stream,
-                               static_cast<const double2*>(b->d_pool_stage), reinterpret_cast<float2*>(dst), n);
-            s = check_hip(c, hipGetLastError(), "k_to_c64");
-            if (s == QT_OK) s = check_hip(c, hipStreamSynchronize(c->stream), "sync");
-        }
+        // rounded to complex64 on the host (the same round-to-nearest as a device
+        // conversion) and copied: no kernel, so nothing to wait for afterwards
+        std::vector<float> f(size_t(2 * n));
+        for (int64_t i = 0; i < 2 * n; ++i) f[size_t(i)] = float(values[i]);
+        s = check_hip(c, hipMemcpy(dst, f.data(), size_t(n) * 8, hipMemcpyHostToDevice), "leaf pool copy");
     }
     if (s == QT_OK) b->pool_elems = std::max(b->pool_elems, need);
     return s;
@@ -828,6 +819,7 @@ bool reorder_steps(int dl, int n_leaves, int n_steps, const int32_t* desc, const
 }  // namespace
 
 qt_status launch_group_sums(qt_batch* b) {
+    b->in_use = true;  // queued work reads the batch's arrays
     Ctx* c = b->ctx;
     qt_status s;
     if (b->group_stride_m > 0) {
@@ -843,6 +835,12 @@ qt_status launch_group_sums(qt_batch* b) {
         s = check_hip(c, hipGetLastError(), "k_group_sum");
     }
     return s;
+}
+
+void quiesce(qt_batch* b) {
+    if (!b->in_use) return;
+    (void)hipStreamSynchronize(b->ctx->stream);
+    b->in_use = false;
 }
 
 qt_status after_specialised_drop(qt_batch* b) {
@@ -1071,7 +1069,7 @@ qt_status qt_batch_set_leaves(qt_batch* b, const double* pool, int64_t n_elems) 
     auto* ctx = b->ctx;
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
     (void)hipSetDevice(ctx->device);
-    (void)hipStreamSynchronize(ctx->stream);
+    qt::quiesce(b);
     if (b->max_leaf_ref_end > n_elems)
         return set_error(ctx, QT_INVALID_INPUT, "qt_batch_set_leaves: pool smaller than the leaf references");
     b->pool_elems = 0;
@@ -1086,7 +1084,7 @@ qt_status qt_batch_write_leaves(qt_batch* b, int64_t offset, const double* value
     if (!b || offset < 0 || n_elems < 1 || !values) return QT_INVALID_INPUT;
     std::lock_guard<std::recursive_mutex> g(b->ctx->mu);
     (void)hipSetDevice(b->ctx->device);
-    (void)hipStreamSynchronize(b->ctx->stream);  // a launch in flight may read the pool
+    qt::quiesce(b);  // a launch in flight may read the pool
     return qt::write_pool(b, offset, values, n_elems);
 }
 
@@ -1098,7 +1096,7 @@ qt_status qt_batch_set_instances(qt_batch* b, int n, const int32_t* prog, const 
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
     (void)hipSetDevice(ctx->device);
     // a launch still in flight reads the instance arrays that are replaced below
-    (void)hipStreamSynchronize(ctx->stream);
+    qt::quiesce(b);
     if (b->pool_elems == 0) return set_error(ctx, QT_INVALID_INPUT, "qt_batch_set_instances: set the leaves first");
     // validate everything before any state changes
     std::vector<qt::Instance> inst(static_cast<size_t>(n));
@@ -1191,7 +1189,7 @@ qt_status qt_batch_set_outputs(qt_batch* b, int n, qt_tensor* const* outs) {
                 return set_error(ctx, QT_INVALID_INPUT, "qt_batch_set_outputs: two instances share a tensor");
         ptrs[size_t(i)] = t.data;
     }
-    (void)hipStreamSynchronize(ctx->stream);  // a launch in flight reads the pointer array
+    qt::quiesce(b);  // a launch in flight reads the pointer array
     const size_t cap0 = b->cap_tout;
     qt_status s = qt::dev_put(ctx, &b->d_tout, &b->cap_tout, ptrs.data(), ptrs.size());
     if (s != QT_OK) {
@@ -1235,6 +1233,7 @@ qt_status qt_batch_config_ex(qt_batch* b, int lanes, int wgs_per_cu, int scratch
 }
 
 qt_status qt_batch_launch(qt_batch* b) {
+    b->in_use = true;  // queued work reads the batch's arrays
     if (!b) return QT_INVALID_INPUT;
     auto* ctx = b->ctx;
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
@@ -1346,6 +1345,7 @@ qt_status qt_batch_results(qt_batch* b, double* values, double* group_sums) {
         return set_error(ctx, QT_INVALID_INPUT, "qt_batch_results: group-sums-only launches keep no per-instance values");
     qt_status s = qt::flush_sums(b);
     if (s == QT_OK) s = check_hip(ctx, hipStreamSynchronize(ctx->stream), "batch sync");
+    if (s == QT_OK) b->in_use = false;
     if (s == QT_OK && b->params_unchecked) {
         int bad = 0;
         s = check_hip(ctx, hipMemcpy(&bad, b->d_param_status, sizeof bad, hipMemcpyDeviceToHost), "status");
@@ -1382,6 +1382,7 @@ qt_status qt_batch_time(qt_batch* b, int reps, float* ms_per_launch) {
 }
 
 qt_status qt_batch_time_kernel(qt_batch* b, int reps, float* ms_per_launch) {
+    b->in_use = true;  // queued work reads the batch's arrays
     if (!b || reps < 1 || !ms_per_launch) return QT_INVALID_INPUT;
     auto* ctx = b->ctx;
     std::lock_guard<std::recursive_mutex> g(ctx->mu);

@@ -124,6 +124,7 @@ struct qt_batch {
     hipGraphExec_t graph_exec = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_done = nullptr;
     bool ev_pending = false;  // launches since ev_done was last recorded
+    bool in_use = false;      // work that reads or writes the batch's device arrays may be queued
     // host copies of the device program arrays (programs can be appended)
     std::vector<qt::ProgHdr> h_hdr;
     std::vector<qt::StepDesc> h_steps;
@@ -226,6 +227,9 @@ bool specialised_consts_change(const qt_batch* b, int64_t offset, const double* 
 // the generated kernel's assumptions on parameter blocks hold for the current
 // instances and the pool's host mirror
 bool specialised_params_hold(const qt_batch* b);
+// waits for the batch's queued work before its device arrays are replaced (qt_batch.hip);
+// a batch that has queued nothing since the last wait returns at once
+void quiesce(qt_batch* b);
 // re-reads h_pool_mirror from the device after device-side parameter writes (qt_params.hip)
 qt_status refresh_pool_mirror(qt_batch* b);
 // a dropped specialised executor: rebuilds the captured graph's launch state (qt_batch.hip)

@@ -703,6 +703,7 @@ void check_sums_fit(qt_batch* b) {
 }
 
 qt_status launch_part_sums(qt_batch* b) {
+    b->in_use = true;  // queued work reads the batch's arrays
     Ctx* c = b->ctx;
     const void* pool = b->d_pool;
     int n_groups = b->n_groups;
@@ -719,6 +720,7 @@ qt_status launch_part_sums(qt_batch* b) {
 }
 
 qt_status launch_specialised_sums(qt_batch* b, bool capture) {
+    b->in_use = true;  // queued work reads the batch's arrays
     Ctx* c = b->ctx;
     if (b->sums_pending && !b->pending_parts) {  // a previous launch's per-instance values
         qt_status s = flush_sums(b);
@@ -789,6 +791,7 @@ qt_status reserve_parts(qt_batch* b) {
 }
 
 qt_status launch_specialised(qt_batch* b, bool defer_sums) {
+    b->in_use = true;  // queued work reads the batch's arrays
     Ctx* c = b->ctx;
     if (b->sums_pending && b->pending_parts) {  // partial sums of a sums-only launch
         qt_status s = flush_sums(b);

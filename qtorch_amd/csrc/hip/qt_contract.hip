@@ -927,7 +927,22 @@ PermPlan make_perm_plan(int nbits, const int* src, int tl_max) {
     PermPlan p;
     std::memset(&p, 0, sizeof p);
     p.nbits = nbits;
-    const int tl = nbits < tl_max ? nbits : tl_max;
+    // low bits of the tile: at least tl_max, more while the tile (2^(tl + the output-low
+    // bits that come from outside the input-low ones)) stays within 2^(2 tl_max)
+    // elements and at least 2^10 tiles remain: a large permutation that keeps its low
+    // bits in place would otherwise get tiles of 2^tl_max elements, too few to keep a
+    // workgroup busy, while small ones need many tiles to fill the device
+    auto extra_for = [&](int t) {
+        int n = 0;
+        for (int j = 0; j < t; ++j) n += src[j] >= t ? 1 : 0;
+        return n;
+    };
+    int tl = nbits < tl_max ? nbits : tl_max;
+    while (tl < nbits) {
+        const int tbits = tl + 1 + extra_for(tl + 1);
+        if (tbits > 2 * tl_max || nbits - tbits < 10) break;
+        ++tl;
+    }
     p.tl = tl;
     int inv[64];
     for (int j = 0; j < nbits; ++j) inv[src[j]] = j;

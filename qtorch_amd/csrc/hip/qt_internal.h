@@ -41,9 +41,13 @@ struct Ctx {
     // partial sums of the split reduction kernel
     void* partials = nullptr;
     size_t partials_bytes = 0;
-    // pinned host staging for batched uploads
-    void* staging = nullptr;
-    size_t staging_bytes = 0;
+    // pinned host staging for batched uploads: two buffers used in turn, each reused
+    // once the copy that last read it has finished (its event)
+    void* staging[2] = {nullptr, nullptr};
+    size_t staging_bytes[2] = {0, 0};
+    hipEvent_t staging_ev[2] = {nullptr, nullptr};
+    bool staging_pending[2] = {false, false};
+    int staging_next = 0;
     // tensors and batches alive on this context: it is not destroyed under them
     int64_t live = 0;
     // power-of-two exponents of rescaled complex64 tensors (value = data * 2^exp)

@@ -592,6 +592,7 @@ qt_status qt_batch_rotation_params(qt_batch* b, int dim_log2, int n_rot, const q
     else
         hipLaunchKernelGGL(qt::k_rotation_params<float2>, dim3(grid), dim3(qt::kRotThreads), lds, ctx->stream,
                            static_cast<float2*>(b->d_pool), d_ang, dd, b->d_param_status);
+    b->in_use = true;  // the kernel writes the pool
     s = check_hip(ctx, hipGetLastError(), "k_rotation_params");
     if (s == QT_OK) {
         b->params_unchecked = true;

@@ -154,7 +154,10 @@ qt_status qt_ctx_destroy(qt_ctx* c) {
         if (c->impl.partials) (void)hipFree(c->impl.partials);
         if (c->impl.d_exp) (void)hipFree(c->impl.d_exp);
         if (c->impl.d_absmax) (void)hipFree(c->impl.d_absmax);
-        if (c->impl.staging) (void)hipHostFree(c->impl.staging);
+        for (int k = 0; k < 2; ++k) {
+            if (c->impl.staging[k]) (void)hipHostFree(c->impl.staging[k]);
+            if (c->impl.staging_ev[k]) (void)hipEventDestroy(c->impl.staging_ev[k]);
+        }
         (void)hipStreamDestroy(c->impl.stream);
     }
     delete c;
@@ -246,18 +249,26 @@ qt_status qt_tensor_create_many(qt_ctx* c, int n, const int* ranks, int dim_log2
     }
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
     (void)hipSetDevice(ctx->device);
-    if (ctx->staging_bytes < total) {
-        (void)hipStreamSynchronize(ctx->stream);  // the previous staging copy may still be in flight
-        if (ctx->staging) (void)hipHostFree(ctx->staging);
-        ctx->staging = nullptr;
-        ctx->staging_bytes = 0;
-        qt_status s = check_hip(ctx, hipHostMalloc(&ctx->staging, total), "staging");
-        if (s != QT_OK) return s;
-        ctx->staging_bytes = total;
-    } else {
-        qt_status s = check_hip(ctx, hipStreamSynchronize(ctx->stream), "staging reuse");
+    const int k = ctx->staging_next;
+    ctx->staging_next ^= 1;
+    if (!ctx->staging_ev[k]) {
+        qt_status s = check_hip(ctx, hipEventCreateWithFlags(&ctx->staging_ev[k], hipEventDisableTiming), "event");
         if (s != QT_OK) return s;
     }
+    if (ctx->staging_pending[k]) {  // the copy that last read this buffer
+        qt_status s = check_hip(ctx, hipEventSynchronize(ctx->staging_ev[k]), "staging reuse");
+        if (s != QT_OK) return s;
+        ctx->staging_pending[k] = false;
+    }
+    if (ctx->staging_bytes[k] < total) {
+        if (ctx->staging[k]) (void)hipHostFree(ctx->staging[k]);
+        ctx->staging[k] = nullptr;
+        ctx->staging_bytes[k] = 0;
+        qt_status s = check_hip(ctx, hipHostMalloc(&ctx->staging[k], total), "staging");
+        if (s != QT_OK) return s;
+        ctx->staging_bytes[k] = total;
+    }
+    char* const stage = static_cast<char*>(ctx->staging[k]);
     hipError_t e;
     void* block = ctx->pool.alloc(total, &e);
     if (!block) return check_hip(ctx, e, "qt_tensor_create_many");
@@ -282,12 +293,17 @@ qt_status qt_tensor_create_many(qt_ctx* c, int n, const int* ranks, int dim_log2
         t->impl.data = static_cast<char*>(block) + off[size_t(i)];
         t->impl.block = block;
         t->impl.block_refs = refs;
-        std::memcpy(static_cast<char*>(ctx->staging) + off[size_t(i)], host[i], t->impl.bytes);
+        std::memcpy(stage + off[size_t(i)], host[i], t->impl.bytes);
         out[i] = t;
     }
-    // asynchronous: the staging buffer is reused only after a synchronisation (above)
-    qt_status s = check_hip(ctx, hipMemcpyAsync(block, ctx->staging, total, hipMemcpyHostToDevice, ctx->stream),
+    // asynchronous: the staging buffer is reused once this copy's event has completed
+    qt_status s = check_hip(ctx, hipMemcpyAsync(block, stage, total, hipMemcpyHostToDevice, ctx->stream),
                             "qt_tensor_create_many copy");
+    if (s == QT_OK) {
+        s = check_hip(ctx, hipEventRecord(ctx->staging_ev[k], ctx->stream), "event record");
+        if (s == QT_OK) ctx->staging_pending[k] = true;
+        else (void)hipStreamSynchronize(ctx->stream);  // no event: the copy must end before reuse
+    }
     if (s != QT_OK) {
         for (int i = 0; i < n; ++i) {
             delete out[i];
