@@ -322,54 +322,105 @@ __device__ __forceinline__ V shfl_down_v(V v, int d) {
     return r;
 }
 
-template <class V, int MODE>
-__global__ __launch_bounds__(kThreads) void k_reduce_partial(const V* __restrict__ A,
-                                                             const V* __restrict__ B,
-                                                             V* __restrict__ partials, BitMap m,
-                                                             int64_t nC, int64_t chunk) {
-    __shared__ V wsum[kThreads / 64];
+// Rows of an operand: its elements that share one value of its free bits.  One pass of
+// k_reduce_rows reads GA rows of A and GB rows of B and accumulates the GA x GB outputs
+// they form, so an operand element is read once per pass, not once per output of the
+// other operand.  The summed index s splits into the lane (bits 0-7), the workgroup
+// (the next log2(gridDim.x) bits) and the step (the bits above), so the workgroups that
+// run together read neighbouring 256-element runs (with a contiguous chunk of s per
+// workgroup the kernel measured 2.7x slower on 2^24 summed indices).  Every output is
+// summed in the same order whatever the grouping: each lane over its steps in order,
+// then the lanes of a wave by shuffles, then the waves in order.
+struct RowSet {
+    int64_t oa[16], ob[16];  // free-bit offsets of the pass's rows of A and of B
+    int16_t c[16];           // output index of row pair (i, j) at c[i * GB + j]
+};
+
+template <class V, int MODE, int GA, int GB>
+__global__ __launch_bounds__(kThreads) void k_reduce_rows(const V* __restrict__ A, const V* __restrict__ B,
+                                                          V* __restrict__ partials, BitMap m, int64_t nC,
+                                                          int step_bits, RowSet r) {
+    __shared__ V wsum[GA * GB][kThreads / 64];
+    // offsets of the step bits 0-5 and 6-11: plan bytes read inside the step loop
+    // would be vector loads whose waits also wait for every operand load in flight
+    __shared__ int64_t t_lo[2][64], t_hi[2][64];
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
-    const int64_t nS = int64_t(1) << m.nbits_s;
-    const int64_t s_begin = int64_t(blockIdx.x) * chunk;
-    const int64_t s_end = s_begin + chunk < nS ? s_begin + chunk : nS;
-    const int lo = m.nbits_s < 8 ? m.nbits_s : 8;
-    int64_t la = 0, lb = 0;  // offsets of the low summed bits owned by this lane
-    for (int j = 0; j < lo; ++j)
+    const int first_step_bit = m.nbits_s - step_bits;  // 8 + log2(gridDim.x)
+    int64_t la = 0, lb = 0;
+    for (int j = 0; j < 8; ++j)
         if ((tid >> j) & 1) {
             la += int64_t(1) << m.s_a[j];
             lb += int64_t(1) << m.s_b[j];
         }
-    for (int64_t c = 0; c < nC; ++c) {
-        int64_t oa, ob;
-        c_offsets(m, c, 0, m.nbits_c, oa, ob);
-        Acc<V, MODE> acc;
-        for (int64_t s = s_begin; s < s_end; s += kThreads) {
-            int64_t ha = 0, hb = 0;  // uniform high part of s
-            for (int j = lo; j < m.nbits_s; ++j)
-                if ((s >> j) & 1) {
-                    ha += int64_t(1) << m.s_a[j];
-                    hb += int64_t(1) << m.s_b[j];
-                }
-            if (s + tid < s_end) mac<V, MODE>(acc, A[oa + ha + la], B[ob + hb + lb]);
+    if (tid < 128) {
+        const int x = tid & 63, first = first_step_bit + (tid < 64 ? 0 : 6);
+        int64_t oa = 0, ob = 0;
+        for (int k = 0; k < 6; ++k)
+            if (((x >> k) & 1) && first + k < m.nbits_s) {
+                oa += int64_t(1) << m.s_a[first + k];
+                ob += int64_t(1) << m.s_b[first + k];
+            }
+        if (tid < 64) {
+            t_lo[0][x] = oa;
+            t_lo[1][x] = ob;
+        } else {
+            t_hi[0][x] = oa;
+            t_hi[1][x] = ob;
         }
-        V v = finish<V, MODE>(acc);
+    }
+    int64_t base_a = la, base_b = lb;  // lane and workgroup bits
+    for (int j = 8; j < first_step_bit; ++j)
+        if ((blockIdx.x >> (j - 8)) & 1) {
+            base_a += int64_t(1) << m.s_a[j];
+            base_b += int64_t(1) << m.s_b[j];
+        }
+    __syncthreads();
+    Acc<V, MODE> acc[GA][GB];
+    // U steps per iteration, all loads issued before the first multiply (the steps are
+    // a power of two of at least 16, so U divides them).  Measured on 2^24 summed
+    // indices: 0.19 ms; with the U loads of an iteration on neighbouring runs instead
+    // of U steps apart, 0.60 ms
+    constexpr int U0 = GA + GB >= 16 ? 1 : 16 / (GA + GB);
+    constexpr int U = U0 >= 8 ? 8 : U0 >= 4 ? 4 : U0 >= 2 ? 2 : 1;
+    const int steps = 1 << step_bits;
+    for (int k0 = 0; k0 < steps; k0 += U) {
+        V a[U][GA], b[U][GB];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int k = k0 + u;
+            const int64_t ha = base_a + t_lo[0][k & 63] + t_hi[0][k >> 6];
+            const int64_t hb = base_b + t_lo[1][k & 63] + t_hi[1][k >> 6];
+#pragma unroll
+            for (int i = 0; i < GA; ++i) a[u][i] = A[r.oa[i] + ha];
+#pragma unroll
+            for (int j = 0; j < GB; ++j) b[u][j] = B[r.ob[j] + hb];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int i = 0; i < GA; ++i)
+#pragma unroll
+                for (int j = 0; j < GB; ++j) mac<V, MODE>(acc[i][j], a[u][i], b[u][j]);
+    }
+#pragma unroll
+    for (int q = 0; q < GA * GB; ++q) {
+        V v = finish<V, MODE>(acc[q / GB][q % GB]);
         for (int d = 32; d >= 1; d >>= 1) {
             V o = shfl_down_v(v, d);
             v.x += o.x;
             v.y += o.y;
         }
-        if (lane == 0) wsum[wave] = v;
-        __syncthreads();
-        if (tid == 0) {
-            V t = wsum[0];
-            for (int w = 1; w < kThreads / 64; ++w) {
-                t.x += wsum[w].x;
-                t.y += wsum[w].y;
-            }
-            partials[int64_t(blockIdx.x) * nC + c] = t;
+        if (lane == 0) wsum[q][wave] = v;
+    }
+    __syncthreads();
+    if (tid < GA * GB) {
+        V t = wsum[tid][0];
+        for (int w = 1; w < kThreads / 64; ++w) {
+            t.x += wsum[tid][w].x;
+            t.y += wsum[tid][w].y;
         }
-        __syncthreads();
+        partials[int64_t(blockIdx.x) * nC + r.c[tid]] = t;
     }
 }
 
@@ -404,9 +455,13 @@ __global__ __launch_bounds__(kThreads) void k_reduce_final(const V* __restrict__
 // Tile element t sits at slot t with its low 6 bits xor-ed with its row t / 64: a read
 // that walks the tile 64 elements at a time (an input-low bit moved to an output-high
 // position) hits a different bank with every lane, and the tile stays 2^tbits slots.
+// P = 1 (complex64): each lane moves two elements that are neighbours on the read side
+// (input bit 0) as one 16-byte load, and two that are neighbours on the write side
+// (output bit 0) as one 16-byte store; 8-byte accesses measured well below the rate of
+// 16-byte ones.
 __device__ __forceinline__ int tile_slot(int t) { return t ^ ((t >> 6) & 63); }
 
-template <class V>
+template <class V, int P>
 __global__ __launch_bounds__(kThreads) void k_permute(const V* __restrict__ in, V* __restrict__ out,
                                                       PermPlan p) {
     extern __shared__ unsigned char smem_raw[];
@@ -421,41 +476,73 @@ __global__ __launch_bounds__(kThreads) void k_permute(const V* __restrict__ in, 
     const int tbits = p.tl + p.n_extra_in;
     const int n = 1 << tbits;
     const int tid = threadIdx.x;
-    // element t = tid + 256 q: the offsets split into the thread's part (tile bits
-    // 0-7, computed once) and the step's part (bits 8 and up, uniform)
+    // element e = (tid << P) + (q << (8 + P)): the offsets split into the thread's part
+    // (tile bits P to P + 7, computed once) and the step's part (bits 8 + P and up,
+    // uniform); with P = 1 bit 0 selects the pair's second element
     auto in_bit = [&](int b) -> int64_t { return b < p.tl ? int64_t(1) << b : int64_t(1) << p.extra_in_pos[b - p.tl]; };
     auto out_bit = [&](int b) -> int64_t { return b < p.tl ? int64_t(1) << b : int64_t(1) << p.extra_out_pos[b - p.tl]; };
     auto tile_bit = [&](int b) -> int { return b < p.tl ? p.lowout_tile[b] : p.extra_out_tile[b - p.tl]; };
-    const int lo_bits = tbits < 8 ? tbits : 8;
+    constexpr int hi = 8 + P;
+    const int lo_bits = tbits < hi ? tbits : hi;
     int64_t in_t = base_in, out_t = base_out;
     int slot_t = 0;
-    for (int b = 0; b < lo_bits; ++b)
-        if ((tid >> b) & 1) {
+    for (int b = P; b < lo_bits; ++b)
+        if ((tid >> (b - P)) & 1) {
             in_t += in_bit(b);
             out_t += out_bit(b);
             slot_t |= 1 << tile_bit(b);
         }
-    const int steps = n > kThreads ? n / kThreads : 1;
-    if (tid < n)
+    constexpr int per = kThreads << P;
+    const int steps = n > per ? n / per : 1;
+    // the step bits' offsets, read from the plan before the first data load: a plan
+    // byte read inside the step loops is a vector load whose wait also waits for every
+    // element load in flight
+    constexpr int kStepBits = 4;  // tiles of at most 2^12 elements
+    int64_t step_in[kStepBits], step_out[kStepBits];
+    int step_tile[kStepBits];
+#pragma unroll
+    for (int k = 0; k < kStepBits; ++k) {
+        const bool on = hi + k < tbits;
+        step_in[k] = on ? in_bit(hi + k) : 0;
+        step_out[k] = on ? out_bit(hi + k) : 0;
+        step_tile[k] = on ? 1 << tile_bit(hi + k) : 0;
+    }
+    const int pair_bit = P == 1 ? 1 << tile_bit(0) : 0;
+    if ((tid << P) < n) {
         for (int q = 0; q < steps; ++q) {
             int64_t off = in_t;
-            for (int b = 8; b < tbits; ++b)
-                if ((q >> (b - 8)) & 1) off += in_bit(b);
-            const int t = tid + (q << 8);
-            tile[tile_slot(t)] = in[off];
+#pragma unroll
+            for (int k = 0; k < kStepBits; ++k)
+                if ((q >> k) & 1) off += step_in[k];
+            const int t = (tid << P) + (q << hi);
+            if constexpr (P == 1) {
+                const float4 v = *reinterpret_cast<const float4*>(in + off);
+                tile[tile_slot(t)] = make_float2(v.x, v.y);
+                tile[tile_slot(t + 1)] = make_float2(v.z, v.w);
+            } else {
+                tile[tile_slot(t)] = in[off];
+            }
         }
+    }
     __syncthreads();
-    if (tid < n)
+    if ((tid << P) < n) {
         for (int q = 0; q < steps; ++q) {
             int64_t off = out_t;
             int t = slot_t;
-            for (int b = 8; b < tbits; ++b)
-                if ((q >> (b - 8)) & 1) {
-                    off += out_bit(b);
-                    t |= 1 << tile_bit(b);
+#pragma unroll
+            for (int k = 0; k < kStepBits; ++k)
+                if ((q >> k) & 1) {
+                    off += step_out[k];
+                    t |= step_tile[k];
                 }
-            out[off] = tile[tile_slot(t)];
+            if constexpr (P == 1) {
+                const V a = tile[tile_slot(t)], b2 = tile[tile_slot(t | pair_bit)];
+                *reinterpret_cast<float4*>(out + off) = make_float4(a.x, a.y, b2.x, b2.y);
+            } else {
+                out[off] = tile[tile_slot(t)];
+            }
         }
+    }
 }
 
 // Complex FP64 GEMM on MFMA: C[m + M n] = sum_s A'[s][m] * B'[s][n].
@@ -873,12 +960,49 @@ qt_status run_skinny(Ctx* c, const V* A, const V* B, V* C, const BitMap& m, int 
     }
 }
 
+template <class V, int MODE, int GA, int GB>
+qt_status launch_rows(Ctx* c, dim3 grid, const V* A, const V* B, V* part, const BitMap& m, int64_t nC,
+                      int step_bits, const RowSet& r) {
+    hipLaunchKernelGGL((k_reduce_rows<V, MODE, GA, GB>), grid, dim3(kThreads), 0, c->stream, A, B, part, m, nC,
+                       step_bits, r);
+    return check_hip(c, hipGetLastError(), "k_reduce_rows");
+}
+
+template <class V, int MODE>
+qt_status launch_reduce_rows(Ctx* c, int ga, int gb, dim3 grid, const V* A, const V* B, V* part, const BitMap& m,
+                             int64_t nC, int step_bits, const RowSet& r) {
+    switch (ga * 32 + gb) {
+        case 1 * 32 + 1: return launch_rows<V, MODE, 1, 1>(c, grid, A, B, part, m, nC, step_bits, r);
+        case 1 * 32 + 2: return launch_rows<V, MODE, 1, 2>(c, grid, A, B, part, m, nC, step_bits, r);
+        case 1 * 32 + 4: return launch_rows<V, MODE, 1, 4>(c, grid, A, B, part, m, nC, step_bits, r);
+        case 1 * 32 + 8: return launch_rows<V, MODE, 1, 8>(c, grid, A, B, part, m, nC, step_bits, r);
+        case 1 * 32 + 16: return launch_rows<V, MODE, 1, 16>(c, grid, A, B, part, m, nC, step_bits, r);
+        case 2 * 32 + 1: return launch_rows<V, MODE, 2, 1>(c, grid, A, B, part, m, nC, step_bits, r);
+        case 2 * 32 + 2: return launch_rows<V, MODE, 2, 2>(c, grid, A, B, part, m, nC, step_bits, r);
+        case 2 * 32 + 4: return launch_rows<V, MODE, 2, 4>(c, grid, A, B, part, m, nC, step_bits, r);
+        case 2 * 32 + 8: return launch_rows<V, MODE, 2, 8>(c, grid, A, B, part, m, nC, step_bits, r);
+        case 4 * 32 + 1: return launch_rows<V, MODE, 4, 1>(c, grid, A, B, part, m, nC, step_bits, r);
+        case 4 * 32 + 2: return launch_rows<V, MODE, 4, 2>(c, grid, A, B, part, m, nC, step_bits, r);
+        case 4 * 32 + 4: return launch_rows<V, MODE, 4, 4>(c, grid, A, B, part, m, nC, step_bits, r);
+        case 8 * 32 + 1: return launch_rows<V, MODE, 8, 1>(c, grid, A, B, part, m, nC, step_bits, r);
+        case 8 * 32 + 2: return launch_rows<V, MODE, 8, 2>(c, grid, A, B, part, m, nC, step_bits, r);
+        case 16 * 32 + 1: return launch_rows<V, MODE, 16, 1>(c, grid, A, B, part, m, nC, step_bits, r);
+        default: return set_error(c, QT_UNSUPPORTED, "split reduction: unsupported row grouping");
+    }
+}
+
 template <class V, int MODE>
 qt_status run_reduce(Ctx* c, const V* A, const V* B, V* C, const BitMap& m, const Workspace* ws) {
     const int64_t nC = int64_t(1) << m.nbits_c;
     const int64_t nS = int64_t(1) << m.nbits_s;
+    // each workgroup takes 2^4 to 2^12 steps of 256 summed indices (k_reduce_rows'
+    // offset tables cover 12 step bits), as few workgroups as give each at least 16
+    // steps, at most 2048 unless the steps would exceed 2^12 (nS >= 2^13 is a power of
+    // two, so the steps divide it)
     int64_t chunk = kThreads * 16;
-    while (nS / chunk > 2048) chunk <<= 1;
+    while (nS / chunk > 2048 && chunk < (int64_t(1) << 20)) chunk <<= 1;
+    int step_bits = 0;
+    while ((int64_t(kThreads) << step_bits) < chunk) ++step_bits;
     const int64_t nblocks = (nS + chunk - 1) / chunk;
     const size_t need = size_t(nblocks * nC) * sizeof(V);
     V* part = nullptr;
@@ -897,10 +1021,54 @@ qt_status run_reduce(Ctx* c, const V* A, const V* B, V* C, const BitMap& m, cons
         c->partials_bytes = need;
     }
     if (!part) part = static_cast<V*>(c->partials);
-    hipLaunchKernelGGL((k_reduce_partial<V, MODE>), dim3(unsigned(nblocks)), dim3(kThreads), 0,
-                       c->stream, A, B, part, m, nC, chunk);
-    qt_status s = check_hip(c, hipGetLastError(), "k_reduce_partial");
+    // rows: A's free bits enumerate na rows, B's nb rows; passes of ga x gb row pairs
+    // (at most 16 outputs per pass, so the accumulators stay in registers)
+    int64_t row_a[64], row_b[64];
+    int16_t c_of[64];
+    int na = 0, nb = 0;
+    {
+        int a_bits[6], b_bits[6], ka = 0, kb = 0;
+        for (int i = 0; i < m.nbits_c; ++i) {
+            if (m.c_src[i] < 64) a_bits[ka++] = i;
+            else b_bits[kb++] = i;
+        }
+        na = 1 << ka;
+        nb = 1 << kb;
+        for (int x = 0; x < na; ++x) {
+            row_a[x] = 0;
+            for (int k = 0; k < ka; ++k)
+                if ((x >> k) & 1) row_a[x] += int64_t(1) << m.c_src[a_bits[k]];
+        }
+        for (int y = 0; y < nb; ++y) {
+            row_b[y] = 0;
+            for (int k = 0; k < kb; ++k)
+                if ((y >> k) & 1) row_b[y] += int64_t(1) << (m.c_src[b_bits[k]] - 64);
+        }
+        for (int64_t cc = 0; cc < nC; ++cc) {
+            int x = 0, y = 0;
+            for (int k = 0; k < ka; ++k) x |= int((cc >> a_bits[k]) & 1) << k;
+            for (int k = 0; k < kb; ++k) y |= int((cc >> b_bits[k]) & 1) << k;
+            c_of[x * nb + y] = int16_t(cc);
+        }
+    }
+    int ga = na < 16 ? na : 16, gb = nb < 16 ? nb : 16;
+    while (ga * gb > 16) {
+        if (ga >= gb) ga >>= 1;
+        else gb >>= 1;
+    }
+    qt_status s = QT_OK;
+    for (int x0 = 0; x0 < na && s == QT_OK; x0 += ga)
+        for (int y0 = 0; y0 < nb && s == QT_OK; y0 += gb) {
+            RowSet r;
+            std::memset(&r, 0, sizeof r);
+            for (int i = 0; i < ga; ++i) r.oa[i] = row_a[x0 + i];
+            for (int j = 0; j < gb; ++j) r.ob[j] = row_b[y0 + j];
+            for (int i = 0; i < ga; ++i)
+                for (int j = 0; j < gb; ++j) r.c[i * gb + j] = c_of[(x0 + i) * nb + y0 + j];
+            s = launch_reduce_rows<V, MODE>(c, ga, gb, dim3(unsigned(nblocks)), A, B, part, m, nC, step_bits, r);
+        }
     if (s != QT_OK) return s;
+
     hipLaunchKernelGGL((k_reduce_final<V>), dim3(unsigned(nC)), dim3(kThreads), 0, c->stream, part, C, nC, nblocks);
     return check_hip(c, hipGetLastError(), "k_reduce_final");
 }
@@ -911,7 +1079,17 @@ qt_status run_permute(Ctx* c, const V* in, V* out, const PermPlan& p) {
     const unsigned gx = unsigned(blocks < 65536 ? blocks : 65536);
     const unsigned gy = unsigned((blocks + gx - 1) / gx);
     const size_t lds = (size_t(1) << (p.tl + p.n_extra_in)) * sizeof(V);
-    hipLaunchKernelGGL((k_permute<V>), dim3(gx, gy), dim3(kThreads), lds, c->stream, in, out, p);
+    // element pairs need input and output bit 0 inside the tile (tl >= 1) and 16-byte
+    // aligned buffers
+    const bool pairs = p.tl >= 1 && (reinterpret_cast<uintptr_t>(in) & 15) == 0 &&
+                       (reinterpret_cast<uintptr_t>(out) & 15) == 0;
+    if constexpr (sizeof(V) == 8) {
+        if (pairs) {
+            hipLaunchKernelGGL((k_permute<V, 1>), dim3(gx, gy), dim3(kThreads), lds, c->stream, in, out, p);
+            return check_hip(c, hipGetLastError(), "k_permute");
+        }
+    }
+    hipLaunchKernelGGL((k_permute<V, 0>), dim3(gx, gy), dim3(kThreads), lds, c->stream, in, out, p);
     return check_hip(c, hipGetLastError(), "k_permute");
 }
 
@@ -923,26 +1101,28 @@ bool is_identity(const int* src, int n) {
 
 }  // namespace
 
-PermPlan make_perm_plan(int nbits, const int* src, int tl_max) {
+PermPlan make_perm_plan(int nbits, const int* src, int tile_bits_max) {
     PermPlan p;
     std::memset(&p, 0, sizeof p);
     p.nbits = nbits;
-    // low bits of the tile: at least tl_max, more while the tile (2^(tl + the output-low
-    // bits that come from outside the input-low ones)) stays within 2^(2 tl_max)
-    // elements and at least 2^10 tiles remain: a large permutation that keeps its low
-    // bits in place would otherwise get tiles of 2^tl_max elements, too few to keep a
-    // workgroup busy, while small ones need many tiles to fill the device
-    auto extra_for = [&](int t) {
-        int n = 0;
+    // tile of 2^tbits elements, tbits = tl + (output-low bits that come from outside the
+    // input-low ones): the largest tl whose tile has at most 2^tile_bits_max elements
+    // and leaves at least 2^10 tiles (tbits never falls with tl).  Measured on 2^22- to
+    // 2^28-element operands: the largest tiles (2^12 elements) move the most bytes per
+    // second (2^28 complex64 elements, low and high 14 bits exchanged: 1.14 ms with
+    // 2^12-element tiles, 2.12 ms with 2^10), and tiles under 2^8 leave most of a
+    // workgroup's lanes idle (a permutation that keeps its low 6 bits in place ran at
+    // 0.3 TB/s with tiles of 2^6)
+    auto tbits_for = [&](int t) {
+        int n = t;
         for (int j = 0; j < t; ++j) n += src[j] >= t ? 1 : 0;
         return n;
     };
-    int tl = nbits < tl_max ? nbits : tl_max;
-    while (tl < nbits) {
-        const int tbits = tl + 1 + extra_for(tl + 1);
-        if (tbits > 2 * tl_max || nbits - tbits < 10) break;
-        ++tl;
-    }
+    int cap = nbits - 10 < tile_bits_max ? nbits - 10 : tile_bits_max;
+    if (cap < 6) cap = 6;
+    int tl = 1;
+    while (tl < nbits && tbits_for(tl + 1) <= cap) ++tl;
+    if (tl > nbits) tl = nbits;
     p.tl = tl;
     int inv[64];
     for (int j = 0; j < nbits; ++j) inv[src[j]] = j;
@@ -1284,9 +1464,8 @@ namespace {
 template <class V, int MODE>
 qt_status run_gemm(Ctx* c, const Tensor* A, const Tensor* B, Tensor* C, const BitMap& m, int mbits,
                    const Workspace* ws) {
-    // permute runs of 64 elements (1 KB complex128, 512 B complex64); tiles of up to
-    // 2^12 elements (64 KB of LDS in complex128)
-    constexpr int kPermLowBits = 6;
+    // operand permutes: tiles of up to 2^12 elements (64 KB of LDS in complex128)
+    constexpr int kPermTileBits = 12;
     const int nbits_n = m.nbits_c - mbits;
     const int sb = m.nbits_s;
     int src_a[64], src_b[64];
@@ -1308,7 +1487,7 @@ qt_status run_gemm(Ctx* c, const Tensor* A, const Tensor* B, Tensor* C, const Bi
         hipError_t e = hipSuccess;
         abuf = ws ? static_cast<void*>(ws->base) : c->pool.alloc(A->bytes, &e);
         if (!abuf) return check_hip(c, e, "gemm scratch A");
-        s = permute(c, A->data, abuf, A->dtype, make_perm_plan(mbits + sb, src_a, kPermLowBits));
+        s = permute(c, A->data, abuf, A->dtype, make_perm_plan(mbits + sb, src_a, kPermTileBits));
         ap = static_cast<const V*>(abuf);
     }
     if (s == QT_OK && perm_b) {
@@ -1318,7 +1497,7 @@ qt_status run_gemm(Ctx* c, const Tensor* A, const Tensor* B, Tensor* C, const Bi
             if (!ws) c->pool.release(abuf);
             return check_hip(c, e, "gemm scratch B");
         }
-        s = permute(c, B->data, bbuf, B->dtype, make_perm_plan(nbits_n + sb, src_b, kPermLowBits));
+        s = permute(c, B->data, bbuf, B->dtype, make_perm_plan(nbits_n + sb, src_b, kPermTileBits));
         bp = static_cast<const V*>(bbuf);
     }
     if (s == QT_OK) {

@@ -134,8 +134,9 @@ void release_exponent(Ctx* c, Tensor* t);
 qt_status carry_exponent(Ctx* c, const Tensor* A, const Tensor* B, Tensor* C);
 // acc (complex128 scalar) += t (scalar, exponent applied), in FP64 on the stream.
 qt_status accumulate_scalar(Ctx* c, const Tensor* t, Tensor* acc);
-// tl_max: input-low / output-low bits of a tile (contiguous runs of 2^tl_max elements)
-PermPlan make_perm_plan(int nbits, const int* src, int tl_max = 5);
+// tile_bits_max: a tile holds at most 2^tile_bits_max elements; its input-low and
+// output-low bits (contiguous runs of 2^tl elements) are as many as that allows
+PermPlan make_perm_plan(int nbits, const int* src, int tile_bits_max = 12);
 
 }  // namespace qt
 
