@@ -313,6 +313,7 @@ __global__ __launch_bounds__(kThreads) void k_thin(const V* __restrict__ X, cons
 // summed-index chunk [b*chunk, (b+1)*chunk) for every output and writes
 // partials[b * nC + c]; k_reduce_final adds the partials in block order.
 constexpr int kReduceMaxOut = 64;
+constexpr int64_t kReduceMaxBlocks = int64_t(1) << 15;
 
 template <class V>
 __device__ __forceinline__ V shfl_down_v(V v, int d) {
@@ -995,12 +996,11 @@ template <class V, int MODE>
 qt_status run_reduce(Ctx* c, const V* A, const V* B, V* C, const BitMap& m, const Workspace* ws) {
     const int64_t nC = int64_t(1) << m.nbits_c;
     const int64_t nS = int64_t(1) << m.nbits_s;
-    // each workgroup takes 2^4 to 2^12 steps of 256 summed indices (k_reduce_rows'
-    // offset tables cover 12 step bits), as few workgroups as give each at least 16
-    // steps, at most 2048 unless the steps would exceed 2^12 (nS >= 2^13 is a power of
-    // two, so the steps divide it)
+    // each workgroup takes 16 steps of 256 summed indices, more only beyond 2^15
+    // workgroups, and at most 2^12 steps (k_reduce_rows' offset tables cover 12 step
+    // bits; nS >= 2^13 is a power of two, so the steps divide it)
     int64_t chunk = kThreads * 16;
-    while (nS / chunk > 2048 && chunk < (int64_t(1) << 20)) chunk <<= 1;
+    while (nS / chunk > kReduceMaxBlocks && chunk < (int64_t(1) << 20)) chunk <<= 1;
     int step_bits = 0;
     while ((int64_t(kThreads) << step_bits) < chunk) ++step_bits;
     const int64_t nblocks = (nS + chunk - 1) / chunk;
