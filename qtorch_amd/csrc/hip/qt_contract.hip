@@ -1169,12 +1169,34 @@ namespace {
 // LDS rows are 144 floats apart, so the four k-rows a wave reads at once sit in
 // different bank quarters.
 constexpr int BT = 128, BK = 16, BROW = BT + 16;
+constexpr int64_t kSplitRows = 256;
+
+// C = the sum of S partial products part[z] (z in order), two complex64 per lane
+__global__ __launch_bounds__(kThreads) void k_sum_splits(const float4* __restrict__ part, float4* __restrict__ C,
+                                                         int64_t n4, int S) {
+    const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+    if (i >= n4) return;
+    float4 a = part[i];
+    for (int z = 1; z < S; ++z) {
+        const float4 b = part[int64_t(z) * n4 + i];
+        a.x += b.x;
+        a.y += b.y;
+        a.z += b.z;
+        a.w += b.w;
+    }
+    C[i] = a;
+}
 
 template <int MODE>
 __global__ __launch_bounds__(kThreads) void k_gemm_c64_big(const float2* __restrict__ Ap,
                                                            const float2* __restrict__ Bp, float2* __restrict__ C,
                                                            int64_t M, int64_t N, int64_t K) {
     __shared__ float sa[2][BK * BROW], sb[2][BK * BROW];  // [re|im][k][row]
+    // split K: workgroups with blockIdx.z = z take summed rows [z K, (z + 1) K) and write
+    // their own M x N partial product at C + z M N
+    Ap += int64_t(blockIdx.z) * K * M;
+    Bp += int64_t(blockIdx.z) * K * N;
+    C += int64_t(blockIdx.z) * M * N;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
     const int li = lane & 15, lk = lane >> 4;
@@ -1516,9 +1538,35 @@ qt_status run_gemm(Ctx* c, const Tensor* A, const Tensor* B, Tensor* C, const Bi
                     const int64_t big = (M / BT) * (N / BT);
                     const unsigned bx = unsigned(big < 65536 ? big : 65536);
                     const unsigned by = unsigned((big + bx - 1) / bx);
-                    hipLaunchKernelGGL((k_gemm_c64_big<MODE>), dim3(bx, by), dim3(kThreads), 0, c->stream, ap, bp,
-                                       static_cast<float2*>(C->data), M, N, K);
+                    // fewer output tiles than two per CU: split K (at least kSplitRows
+                    // summed rows per split) into S partial products, then add them in
+                    // order; not with a caller's workspace, which holds no room for them
+                    int S = 1;
+                    if (!ws)
+                        while (big * S < 512 && K / (2 * S) >= kSplitRows && S < 64) S *= 2;
+                    float2* part = nullptr;
+                    if (S > 1) {
+                        hipError_t e = hipSuccess;
+                        part = static_cast<float2*>(c->pool.alloc(size_t(S) * size_t(M * N) * sizeof(float2), &e));
+                        if (!part) {
+                            (void)hipGetLastError();  // no room: one product, unsplit
+                            S = 1;
+                        }
+                    }
+                    float2* out = S > 1 ? part : static_cast<float2*>(C->data);
+                    hipLaunchKernelGGL((k_gemm_c64_big<MODE>), dim3(bx, by, unsigned(S)), dim3(kThreads), 0, c->stream,
+                                       ap, bp, out, M, N, K / S);
                     s = check_hip(c, hipGetLastError(), "k_gemm_c64_big");
+                    if (S > 1) {
+                        const int64_t n4 = M * N / 2;
+                        if (s == QT_OK) {
+                            hipLaunchKernelGGL(k_sum_splits, dim3(unsigned((n4 + kThreads - 1) / kThreads)),
+                                               dim3(kThreads), 0, c->stream, reinterpret_cast<const float4*>(part),
+                                               reinterpret_cast<float4*>(C->data), n4, S);
+                            s = check_hip(c, hipGetLastError(), "k_sum_splits");
+                        }
+                        c->pool.release(part);
+                    }
                     done = true;
                 }
             }
