@@ -560,6 +560,11 @@ __global__ __launch_bounds__(kThreads) void k_gemm_c128(const double2* __restric
                                                         int64_t N, int64_t K) {
     __shared__ double as_re[GK][GM + 1], as_im[GK][GM + 1];
     __shared__ double bs_re[GK][GN + 1], bs_im[GK][GN + 1];
+    // split K: workgroups with blockIdx.z = z take summed rows [z K, (z + 1) K) and write
+    // their own M x N partial product at C + z M N
+    Ap += int64_t(blockIdx.z) * K * M;
+    Bp += int64_t(blockIdx.z) * K * N;
+    C += int64_t(blockIdx.z) * M * N;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
     const int64_t tiles_m = M / GM;
@@ -1171,6 +1176,20 @@ namespace {
 constexpr int BT = 128, BK = 16, BROW = BT + 16;
 constexpr int64_t kSplitRows = 256;
 
+// C = the sum of S partial products part[z] (z in order), one complex128 per lane
+__global__ __launch_bounds__(kThreads) void k_sum_splits_c128(const double2* __restrict__ part,
+                                                              double2* __restrict__ C, int64_t n, int S) {
+    const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+    if (i >= n) return;
+    double2 a = part[i];
+    for (int z = 1; z < S; ++z) {
+        const double2 b = part[int64_t(z) * n + i];
+        a.x += b.x;
+        a.y += b.y;
+    }
+    C[i] = a;
+}
+
 // C = the sum of S partial products part[z] (z in order), two complex64 per lane
 __global__ __launch_bounds__(kThreads) void k_sum_splits(const float4* __restrict__ part, float4* __restrict__ C,
                                                          int64_t n4, int S) {
@@ -1528,9 +1547,32 @@ qt_status run_gemm(Ctx* c, const Tensor* A, const Tensor* B, Tensor* C, const Bi
         const unsigned gx = unsigned(tiles < 65536 ? tiles : 65536);
         const unsigned gy = unsigned((tiles + gx - 1) / gx);
         if constexpr (MODE == MODE_PLAIN) {
-            hipLaunchKernelGGL(k_gemm_c128, dim3(gx, gy), dim3(kThreads), 0, c->stream, ap, bp,
-                               static_cast<double2*>(C->data), M, N, K);
+            // as for complex64 below: fewer than 512 tiles (64 x 64 here) split K
+            int S = 1;
+            if (!ws && K % GK == 0)
+                while (tiles * S < 512 && K / (2 * S) >= kSplitRows && S < 64) S *= 2;
+            double2* part = nullptr;
+            if (S > 1) {
+                hipError_t e = hipSuccess;
+                part = static_cast<double2*>(c->pool.alloc(size_t(S) * size_t(M * N) * sizeof(double2), &e));
+                if (!part) {
+                    (void)hipGetLastError();  // no room: one product, unsplit
+                    S = 1;
+                }
+            }
+            double2* out = S > 1 ? part : static_cast<double2*>(C->data);
+            hipLaunchKernelGGL(k_gemm_c128, dim3(gx, gy, unsigned(S)), dim3(kThreads), 0, c->stream, ap, bp, out, M,
+                               N, K / S);
             s = check_hip(c, hipGetLastError(), "k_gemm_c128");
+            if (S > 1) {
+                const int64_t n = M * N;
+                if (s == QT_OK) {
+                    hipLaunchKernelGGL(k_sum_splits_c128, dim3(unsigned((n + kThreads - 1) / kThreads)), dim3(kThreads),
+                                       0, c->stream, part, static_cast<double2*>(C->data), n, S);
+                    s = check_hip(c, hipGetLastError(), "k_sum_splits_c128");
+                }
+                c->pool.release(part);
+            }
         } else {
             bool done = false;
             if constexpr (MODE == MODE_4M || MODE == MODE_3M) {

@@ -166,7 +166,7 @@ def test_contract_pair_c64_forms(ctx, prec, tol, ra, rb, k, d):
     Shapes with 64+ free elements on each side and 32+ summed run on the MFMA GEMM
     (FP32 MFMA for 4M / 3M, bf16 MFMA for bf16x3), the rest on the vector kernels.
     The last (one 128 x 128 output tile, 2^12 summed) splits the summed range of the
-    4M / 3M GEMM into 4 partial products; its oracle result (2^26 MACs in the
+    4M / 3M GEMM into 16 partial products; its oracle result (2^26 MACs in the
     reference's loop nest) is computed once for the three forms."""
     rng = np.random.default_rng(7 + ra + rb)
     pairs, free = _case(rng, ra, rb, k)
@@ -182,6 +182,24 @@ def test_contract_pair_c64_forms(ctx, prec, tol, ra, rb, k, d):
     ref = _C64_REF[key]
     scale = np.abs(ref).max() * np.sqrt(1 << (d * k))
     assert np.abs(got - ref).max() <= tol * scale
+
+
+def test_gemm_split_k_c128(ctx):
+    """A complex128 GEMM-shaped step with one 128 x 128 output (four 64 x 64 tiles) and
+    2^12 summed indices: the summed range splits into 16 partial products added in
+    order; against the oracle's loop nest (src/Network.h:899-933)."""
+    rng = np.random.default_rng(19)
+    ra = rb = 19
+    pairs, free = _case(rng, ra, rb, 12)
+    a = _rand(rng, 1 << ra)
+    b = _rand(rng, 1 << rb)
+    ta = qa.Tensor(ctx, ra, 1).upload(a)
+    tb = qa.Tensor(ctx, rb, 1).upload(b)
+    tc, _ = qa.contract_pair(ctx, ta, tb, pairs, free)
+    got = tc.download()
+    ref = qo.contract_indices(a, ra, b, rb, free, pairs, 1)
+    assert _regime(free, 12, 1) == "gemm"
+    np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
 
 
 def test_rescaled_chain_does_not_underflow(ctx):
