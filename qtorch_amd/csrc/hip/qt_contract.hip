@@ -684,6 +684,11 @@ __global__ __launch_bounds__(kThreads) void k_gemm_c64(const float2* __restrict_
     constexpr int kRowB = HK + 8;
     constexpr int kPlaneB = GM * kRowB;
     __shared__ __align__(16) unsigned char smem[MODE == MODE_BF16X3 ? 2 * 4 * kPlaneB * 2 : 2 * 2 * kPlaneF * 4];
+    // split K: workgroups with blockIdx.z = z take summed rows [z K, (z + 1) K) and write
+    // their own M x N partial product at C + z M N
+    Ap += int64_t(blockIdx.z) * K * M;
+    Bp += int64_t(blockIdx.z) * K * N;
+    C += int64_t(blockIdx.z) * M * N;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
     const int li = lane & 15, lk = lane >> 4;
@@ -1613,9 +1618,33 @@ qt_status run_gemm(Ctx* c, const Tensor* A, const Tensor* B, Tensor* C, const Bi
                 }
             }
             if (!done) {
-                hipLaunchKernelGGL((k_gemm_c64<MODE>), dim3(gx, gy), dim3(kThreads), 0, c->stream, ap, bp,
-                                   static_cast<float2*>(C->data), M, N, K);
+                // fewer than 512 tiles (64 x 64 here): split K as above
+                int S = 1;
+                if (!ws && K % HK == 0)
+                    while (tiles * S < 512 && K / (2 * S) >= kSplitRows && S < 64) S *= 2;
+                float2* part = nullptr;
+                if (S > 1) {
+                    hipError_t e = hipSuccess;
+                    part = static_cast<float2*>(c->pool.alloc(size_t(S) * size_t(M * N) * sizeof(float2), &e));
+                    if (!part) {
+                        (void)hipGetLastError();  // no room: one product, unsplit
+                        S = 1;
+                    }
+                }
+                float2* out = S > 1 ? part : static_cast<float2*>(C->data);
+                hipLaunchKernelGGL((k_gemm_c64<MODE>), dim3(gx, gy, unsigned(S)), dim3(kThreads), 0, c->stream, ap, bp,
+                                   out, M, N, K / S);
                 s = check_hip(c, hipGetLastError(), "k_gemm_c64");
+                if (S > 1) {
+                    const int64_t n4 = M * N / 2;
+                    if (s == QT_OK) {
+                        hipLaunchKernelGGL(k_sum_splits, dim3(unsigned((n4 + kThreads - 1) / kThreads)), dim3(kThreads),
+                                           0, c->stream, reinterpret_cast<const float4*>(part),
+                                           reinterpret_cast<float4*>(C->data), n4, S);
+                        s = check_hip(c, hipGetLastError(), "k_sum_splits");
+                    }
+                    c->pool.release(part);
+                }
             }
         }
     }

@@ -165,8 +165,8 @@ def test_contract_pair_c64_forms(ctx, prec, tol, ra, rb, k, d):
     """complex64 forms against the FP64 oracle: error relative to the largest |C|.
     Shapes with 64+ free elements on each side and 32+ summed run on the MFMA GEMM
     (FP32 MFMA for 4M / 3M, bf16 MFMA for bf16x3), the rest on the vector kernels.
-    The last (one 128 x 128 output tile, 2^12 summed) splits the summed range of the
-    4M / 3M GEMM into 16 partial products; its oracle result (2^26 MACs in the
+    The last (a 128 x 128 output, 2^12 summed) splits the summed range of the GEMM
+    into 16 partial products in every form; its oracle result (2^26 MACs in the
     reference's loop nest) is computed once for the three forms."""
     rng = np.random.default_rng(7 + ra + rb)
     pairs, free = _case(rng, ra, rb, k)
