@@ -179,7 +179,7 @@ def run_grid(args, qa, np):
     CZ on alternating grid-edge patterns) in the amplitude picture, sliced to at most
     --max-bits index bits per tensor; every form against complex128.  6x8 depth 8 and
     4x6 depth 14 in the line-graph order, every slice; 6x8 depth 14 in the stochastic
-    order sliced to 28 bits (2^27 slices of 2^38 MACs each, beyond reach as a whole):
+    order sliced to 28 bits (2^27 slices of about 2^37 MACs each, beyond reach whole):
     the first --grid-sample slices that are not zero by structure, whose sum each form
     computes; and 6x8 depth 14 whole, in the circuit sweep order with CZ split into
     bonded halves: unsliced (29-bit peak: one 4 GiB complex64 tensor) and sliced to 28
