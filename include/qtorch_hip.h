@@ -129,7 +129,7 @@ qt_status qt_contract_pair(qt_ctx* ctx, const qt_tensor* A, int rank_a, const qt
  * allocate (code under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc).  The caller passes A, B and an allocated C as for qt_contract_pair; the
  * step's scratch (GEMM operand permutes: A's and B's sizes, 256-byte aligned; split
- * reduction partial sums: at most 2 MiB) comes from a fixed 64 MiB device array per
+ * reduction partial sums: at most 32 MiB) comes from a fixed 64 MiB device array per
  * GPU, held from the first qt_ctx_create on that GPU until the process ends.  A step
  * whose scratch exceeds 64 MiB returns QT_INVALID_INPUT before launching anything, as
  * does QT_RESCALE (its exponent table is allocated on first use: use

@@ -219,8 +219,9 @@ SlicedNetwork::SlicedNetwork(const std::string& qasm, const std::string& measure
     for (const auto& w : wires) plan_.peak_bits = std::max(plan_.peak_bits, bits_of(w));
     // device bytes of one slice (complex128): leaves, then intermediates until consumed,
     // plus each step's scratch while it runs (a GEMM-shaped step may permute copies of
-    // both operands); the split reduction's partial sums (at most 2 MiB) and the fused
-    // subtrees' outputs (PlanFusion) are added on top
+    // both operands); the GEMMs' split-K partial products (at most 64 MiB), or the split
+    // reduction's partial sums (at most 32 MiB), and the fused subtrees' outputs
+    // (PlanFusion) are added on top
     int64_t leaves = 0, live = 0, peak = 0;
     for (int i = 0; i < n_leaves_; ++i) leaves += int64_t(16) << bits_of(leaf_wires_[size_t(i)]);
     std::vector<int64_t> bytes(wires.size(), 0);
@@ -238,7 +239,7 @@ SlicedNetwork::SlicedNetwork(const std::string& qasm, const std::string& measure
         for (int op : {st.a, st.b})
             if (op >= n_leaves_) live -= bytes[size_t(op)];
     }
-    plan_.peak_bytes = leaves + peak + (int64_t(2) << 20);
+    plan_.peak_bytes = leaves + peak + (int64_t(64) << 20);
     // sliced axes of every leaf
     cut_.resize(size_t(n_leaves_));
     leaf_rank_.resize(size_t(n_leaves_));

@@ -477,7 +477,8 @@ def test_contract_nodes_from_two_threads():
 def test_sliced_planned_bytes_cover_step_scratch():
     """A slice's planned device bytes cover, at every GEMM-shaped step, the live
     operands and result plus the permuted copies of both operands that the GEMM may
-    make (and the split reduction's partial sums), not only the live tensors."""
+    make (and the GEMMs' split-K partial products or the split reduction's partial
+    sums: 64 MiB at most), not only the live tensors."""
     from qtorch_amd import workloads as wl
     q = wl.grid_random_circuit(5, 6, 10, seed=4)
     bits = wl.bitstring(30, 2)
@@ -490,7 +491,7 @@ def test_sliced_planned_bytes_cover_step_scratch():
         if (ra - k) >= 6 and (rb - k) >= 6 and k >= 3:  # dimension-2 wires: one bit per axis
             need = max(need, 16 * (2 * (2 ** ra) + 2 * (2 ** rb) + 2 ** rc))
     assert plan["gemm_cmacs_per_slice"] > 0 and need > 0
-    assert plan["peak_bytes"] >= need + (2 << 20), (plan["peak_bytes"], need)
+    assert plan["peak_bytes"] >= need + (64 << 20), (plan["peak_bytes"], need)
 
 
 def test_nonzero_slices_report_a_truncated_search():
