@@ -135,7 +135,9 @@ qt_status qt_contract_pair(qt_ctx* ctx, const qt_tensor* A, int rank_a, const qt
  * does QT_RESCALE (its exponent table is allocated on first use: use
  * qt_contract_pair).  Every call on a GPU shares that GPU's array and nothing in it
  * needs to start at zero; calls that overlap on one GPU (from several contexts or
- * threads) race on it. */
+ * threads) race on it.  A GEMM-shaped step with fewer than 512 output tiles and at
+ * least 512 summed indices does not split its summed range here (qt_contract_pair
+ * does), so its result can differ from qt_contract_pair's in the last bits. */
 qt_status qt_contract_pair_noalloc(qt_ctx* ctx, const qt_tensor* A, int rank_a, const qt_tensor* B,
                                    int rank_b, int k, const int32_t* pair_a, const int32_t* pair_b,
                                    int rank_c, const int32_t* free_src, qt_tensor* C, int precision,
